@@ -1,0 +1,299 @@
+"""ctypes bindings for the two CPU checkers -- TEST INFRASTRUCTURE.
+
+``Oracle`` wraps oracle/liboracle.so (this repository's CPU restatement of
+the reference FLIP step) and ``Reference`` wraps oracle/_ref/libflipref.so
+(the reference engine compiled from its own sources plus
+oracle/ref_harness.cpp).  Both expose the same methods so a test can run a
+stage on either and compare.  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libflipref.so")
+
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+_vp = ctypes.c_void_p
+
+
+def build(ref: bool = True) -> None:
+    """Compiles the checkers (make in oracle/)."""
+    targets = ["oracle", "ref"] if ref else ["oracle"]
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"] + targets, check=True)
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class _Base:
+    prefix = ""
+
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = ctypes.CDLL(path)
+        p = self.prefix
+        L = self.lib
+        getattr(L, p + "sim_create").restype = _vp
+        self._sim_create = getattr(L, p + "sim_create")
+        for name in ["sim_destroy", "sim_initialize", "sim_num_particles", "sim_frame_substeps"]:
+            getattr(L, p + name).argtypes = [_vp]
+        getattr(L, p + "sim_update").argtypes = [_vp, _c_double]
+        getattr(L, p + "sim_add_body_force").argtypes = [_vp, _c_double, _c_double, _c_double]
+        getattr(L, p + "sim_set_cfl").argtypes = [_vp, _c_double]
+        getattr(L, p + "sim_set_substeps").argtypes = [_vp, _c_int, _c_int]
+        getattr(L, p + "sim_set_pic_ratio").argtypes = [_vp, _c_double]
+        getattr(L, p + "sim_load_particles").argtypes = [_vp, _vp, _vp, _c_int]
+        getattr(L, p + "sim_get_particles").argtypes = [_vp, _vp, _vp]
+        getattr(L, p + "sim_get_float_grid").argtypes = [_vp, _c_int, _vp]
+        getattr(L, p + "sim_get_bool_grid").argtypes = [_vp, _c_int, _vp]
+        getattr(L, p + "domain_solid").argtypes = [_c_int] * 3 + [_c_double] + [_vp] * 5
+        getattr(L, p + "liquid_sdf").argtypes = [_c_int] * 3 + [_c_double, _vp, _c_int, _c_double, _vp, _vp]
+        getattr(L, p + "p2g").argtypes = [_c_int] * 3 + [_c_double, _vp, _vp, _c_int, _c_double] + [_vp] * 6
+        getattr(L, p + "extrapolate").argtypes = [_c_int] * 3 + [_vp, _vp, _c_int]
+        getattr(L, p + "mic0_apply").argtypes = [_c_int] * 3 + [_c_double, _c_double] + [_vp] * 8
+        getattr(L, p + "advect_particles").argtypes = ([_c_int] * 3 + [_c_double] + [_vp] * 4
+                                                      + [_c_int, _c_double, _c_int, _vp])
+        getattr(L, p + "trilinear_velocity").argtypes = [_c_int] * 3 + [_c_double] + [_vp] * 4 + [_c_int, _vp]
+
+    # -- whole-step driver -------------------------------------------------
+    def sim_create(self, isize, jsize, ksize, dx):
+        return self._sim_create(isize, jsize, ksize, dx)
+
+    def sim(self, isize, jsize, ksize, dx):
+        return _SimHandle(self, isize, jsize, ksize, dx)
+
+    # -- stages ------------------------------------------------------------
+    def domain_solid(self, isize, jsize, ksize, dx):
+        phi = np.zeros((ksize + 1, jsize + 1, isize + 1), np.float32)
+        wu = np.zeros((ksize, jsize, isize + 1), np.float32)
+        wv = np.zeros((ksize, jsize + 1, isize), np.float32)
+        ww = np.zeros((ksize + 1, jsize, isize), np.float32)
+        wc = np.zeros((ksize, jsize, isize), np.float32)
+        getattr(self.lib, self.prefix + "domain_solid")(isize, jsize, ksize, dx, _ptr(phi), _ptr(wu),
+                                                        _ptr(wv), _ptr(ww), _ptr(wc))
+        return phi, (wu, wv, ww, wc)
+
+    def liquid_sdf(self, isize, jsize, ksize, dx, pos, radius, solid_phi=None):
+        pos = np.ascontiguousarray(pos, np.float32)
+        phi = np.zeros((ksize, jsize, isize), np.float32)
+        sp = None if solid_phi is None else np.ascontiguousarray(solid_phi, np.float32)
+        getattr(self.lib, self.prefix + "liquid_sdf")(isize, jsize, ksize, dx, _ptr(pos), pos.shape[0],
+                                                      radius, _ptr(sp), _ptr(phi))
+        return phi
+
+    def p2g(self, isize, jsize, ksize, dx, pos, vel, radius):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        u = np.zeros((ksize, jsize, isize + 1), np.float32)
+        v = np.zeros((ksize, jsize + 1, isize), np.float32)
+        w = np.zeros((ksize + 1, jsize, isize), np.float32)
+        vu = np.zeros(u.shape, np.uint8)
+        vv = np.zeros(v.shape, np.uint8)
+        vw = np.zeros(w.shape, np.uint8)
+        getattr(self.lib, self.prefix + "p2g")(isize, jsize, ksize, dx, _ptr(pos), _ptr(vel), pos.shape[0],
+                                               radius, _ptr(u), _ptr(v), _ptr(w), _ptr(vu), _ptr(vv), _ptr(vw))
+        return (u, v, w), (vu, vv, vw)
+
+    def extrapolate(self, grid, valid, layers):
+        g = np.ascontiguousarray(grid, np.float32).copy()
+        vd = np.ascontiguousarray(valid, np.uint8).copy()
+        nk, nj, ni = g.shape
+        getattr(self.lib, self.prefix + "extrapolate")(ni, nj, nk, _ptr(g), _ptr(vd), layers)
+        return g
+
+    def mic0_apply(self, isize, jsize, ksize, dx, dt, liquid_phi, weights, r):
+        wu, wv, ww, wc = [np.ascontiguousarray(a, np.float32) for a in weights]
+        lp = np.ascontiguousarray(liquid_phi, np.float32)
+        r = np.ascontiguousarray(r, np.float64)
+        inv = np.zeros_like(r)
+        z = np.zeros_like(r)
+        getattr(self.lib, self.prefix + "mic0_apply")(isize, jsize, ksize, dx, dt, _ptr(lp), _ptr(wu),
+                                                      _ptr(wv), _ptr(ww), _ptr(wc), _ptr(r), _ptr(inv), _ptr(z))
+        return inv, z
+
+    def advect_particles(self, isize, jsize, ksize, dx, mac, pos, dt, order):
+        u, v, w = [np.ascontiguousarray(a, np.float32) for a in mac]
+        pos = np.ascontiguousarray(pos, np.float32)
+        out = np.zeros_like(pos)
+        getattr(self.lib, self.prefix + "advect_particles")(isize, jsize, ksize, dx, _ptr(u), _ptr(v), _ptr(w),
+                                                            _ptr(pos), pos.shape[0], dt, order, _ptr(out))
+        return out
+
+    def trilinear_velocity(self, isize, jsize, ksize, dx, mac, pos):
+        u, v, w = [np.ascontiguousarray(a, np.float32) for a in mac]
+        pos = np.ascontiguousarray(pos, np.float32)
+        out = np.zeros_like(pos)
+        getattr(self.lib, self.prefix + "trilinear_velocity")(isize, jsize, ksize, dx, _ptr(u), _ptr(v),
+                                                              _ptr(w), _ptr(pos), pos.shape[0], _ptr(out))
+        return out
+
+
+class _SimHandle:
+    """Owns one FluidSimulation (reference) or Sim (oracle) instance."""
+
+    def __init__(self, lib: _Base, isize, jsize, ksize, dx):
+        self.L = lib
+        self.dims = (isize, jsize, ksize)
+        self.dx = dx
+        self.h = lib.sim_create(isize, jsize, ksize, dx)
+
+    def _f(self, name):
+        return getattr(self.L.lib, self.L.prefix + name)
+
+    def close(self):
+        if self.h:
+            self._f("sim_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_body_force(self, fx, fy, fz):
+        self._f("sim_add_body_force")(self.h, fx, fy, fz)
+
+    def set_cfl(self, c):
+        self._f("sim_set_cfl")(self.h, c)
+
+    def set_substeps(self, mn, mx):
+        self._f("sim_set_substeps")(self.h, mn, mx)
+
+    def set_pic_ratio(self, r):
+        self._f("sim_set_pic_ratio")(self.h, r)
+
+    def load_particles(self, pos, vel):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        self._f("sim_load_particles")(self.h, _ptr(pos), _ptr(vel), pos.shape[0])
+
+    def initialize(self):
+        return self._f("sim_initialize")(self.h)
+
+    def update(self, dt):
+        rc = self._f("sim_update")(self.h, dt)
+        if rc != 0:
+            raise RuntimeError("update failed")
+
+    @property
+    def num_particles(self):
+        return self._f("sim_num_particles")(self.h)
+
+    def particles(self):
+        n = self.num_particles
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        self._f("sim_get_particles")(self.h, _ptr(pos), _ptr(vel))
+        return pos, vel
+
+    def substeps(self):
+        return self._f("sim_frame_substeps")(self.h)
+
+    _FSHAPES = {
+        0: lambda I, J, K: (K, J, I),
+        1: lambda I, J, K: (K + 1, J + 1, I + 1),
+        2: lambda I, J, K: (K, J, I + 1),
+        3: lambda I, J, K: (K, J + 1, I),
+        4: lambda I, J, K: (K + 1, J, I),
+        5: lambda I, J, K: (K, J, I + 1),
+        6: lambda I, J, K: (K, J + 1, I),
+        7: lambda I, J, K: (K + 1, J, I),
+        8: lambda I, J, K: (K, J, I),
+    }
+
+    def float_grid(self, which):
+        shape = self._FSHAPES[which](*self.dims)
+        out = np.zeros(shape, np.float32)
+        self._f("sim_get_float_grid")(self.h, which, _ptr(out))
+        return out
+
+    def valid_grid(self, which):
+        shape = self._FSHAPES[2 + which](*self.dims)
+        out = np.zeros(shape, np.uint8)
+        self._f("sim_get_bool_grid")(self.h, which, _ptr(out))
+        return out
+
+
+class Oracle(_Base):
+    prefix = "orc_"
+
+    def __init__(self, path: str = ORACLE_SO):
+        super().__init__(path)
+        L = self.lib
+        L.orc_sim_create.argtypes = [_c_int] * 3 + [_c_double]
+        L.orc_sim_set_pcg.argtypes = [_vp, _c_int, _c_double, _c_int]
+        L.orc_sim_pressure_info.argtypes = [_vp, _vp, _vp, _vp]
+        L.orc_pressure_solve.argtypes = ([_c_int] * 3 + [_c_double, _c_double, _c_double, _c_int, _c_int]
+                                         + [_vp] * 12 + [_vp] * 5)
+
+    def pressure_solve(self, isize, jsize, ksize, dx, dt, liquid_phi, solid_phi, weights, mac, valid,
+                       tol=1e-9, maxiter=1000, mode=0):
+        return _pressure_solve(self.lib.orc_pressure_solve, True, isize, jsize, ksize, dx, dt, liquid_phi,
+                               solid_phi, weights, mac, valid, tol, maxiter, mode)
+
+
+class Reference(_Base):
+    prefix = "ref_"
+
+    def __init__(self, path: str = REF_SO, threads: int = 1):
+        super().__init__(path)
+        L = self.lib
+        L.ref_sim_create.argtypes = [_c_int] * 3 + [_c_double, _c_int]
+        L.ref_set_thread_count.argtypes = [_c_int]
+        L.ref_pressure_solve.argtypes = ([_c_int] * 3 + [_c_double, _c_double, _c_double, _c_int]
+                                         + [_vp] * 12 + [_vp] * 5)
+        L.ref_sim_pressure_status.argtypes = [_vp, ctypes.c_char_p, _c_int]
+        self.threads = threads
+
+    def sim_create(self, isize, jsize, ksize, dx):
+        return self.lib.ref_sim_create(isize, jsize, ksize, dx, self.threads)
+
+    def set_threads(self, n):
+        self.threads = n
+        self.lib.ref_set_thread_count(n)
+
+    def pressure_solve(self, isize, jsize, ksize, dx, dt, liquid_phi, solid_phi, weights, mac, valid,
+                       tol=1e-9, maxiter=1000, mode=0):
+        return _pressure_solve(self.lib.ref_pressure_solve, False, isize, jsize, ksize, dx, dt, liquid_phi,
+                               solid_phi, weights, mac, valid, tol, maxiter, mode)
+
+
+def _pressure_solve(fn, with_mode, isize, jsize, ksize, dx, dt, liquid_phi, solid_phi, weights, mac, valid,
+                    tol, maxiter, mode):
+    lp = np.ascontiguousarray(liquid_phi, np.float32)
+    sp = np.ascontiguousarray(solid_phi, np.float32)
+    wu, wv, ww, wc = [np.ascontiguousarray(a, np.float32) for a in weights]
+    u, v, w = [np.ascontiguousarray(a, np.float32).copy() for a in mac]
+    vu, vv, vw = [np.ascontiguousarray(a, np.uint8).copy() for a in valid]
+    nmax = isize * jsize * ksize
+    rhs = np.zeros(nmax, np.float64)
+    soln = np.zeros(nmax, np.float64)
+    n = ctypes.c_int(0)
+    it = ctypes.c_int(0)
+    err = ctypes.c_double(0)
+    args = [isize, jsize, ksize, dx, dt, tol, maxiter]
+    if with_mode:
+        args.append(mode)
+    ok = fn(*args, _ptr(lp), _ptr(sp), _ptr(wu), _ptr(wv), _ptr(ww), _ptr(wc), _ptr(u), _ptr(v), _ptr(w),
+            _ptr(vu), _ptr(vv), _ptr(vw), _ptr(rhs), _ptr(soln), ctypes.byref(n), ctypes.byref(it),
+            ctypes.byref(err))
+    m = n.value
+    return dict(ok=int(ok), rhs=rhs[:m], soln=soln[:m], iters=it.value, err=err.value,
+                mac=(u, v, w), valid=(vu, vv, vw))

@@ -1,0 +1,1739 @@
+// CPU oracle for the FLIP step -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// Single-threaded C++17 restatement of the reference FLIP engine step
+// (jklae/FLIPEngine3D, src/engine).  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load this library, as a checker or as the
+// timed CPU baseline.  It must be built with -ffp-contract=off and without
+// -ffast-math: every float/double operation below mirrors the reference's
+// types and evaluation order so results are bit-identical wherever the
+// reference's result does not depend on thread scheduling.
+//
+// All citations are paths relative to the reference repository root.
+//
+// Parity status (pinned by tests/test_oracle_vs_ref.py against the compiled
+// reference in oracle/_ref, and by closed-form checks in
+// tests/test_oracle_math.py):
+//   domain solid SDF + weights ........ bit-exact vs reference
+//   liquid SDF / labels / counts ...... bit-exact vs reference
+//   P2G (velocity + valid) ............ bit-exact vs reference
+//   extrapolation ..................... bit-exact vs reference
+//   pressure RHS / MIC(0) / apply ..... bit-exact vs reference
+//   PCG (ref-exact mode) .............. bit-exact vs reference at 1 thread
+//   PIC/FLIP, RK1-4, collision, removal bit-exact vs reference
+//   whole update(dt) .................. bit-exact vs reference at 1 thread
+
+#include "flip_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Float 3-vector with the reference's vmath::vec3 semantics
+// (src/engine/vmath.h:38-97, vmath.cpp:33-123): every component is float and
+// scalar products take a float scalar; division multiplies by a float
+// reciprocal formed in double.
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    V3() {}
+    V3(float a, float b, float c) : x(a), y(b), z(c) {}
+};
+static inline V3 operator+(V3 a, V3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 operator-(V3 a, V3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 operator*(float s, V3 v) { return V3(v.x * s, v.y * s, v.z * s); }
+static inline V3 operator*(V3 v, float s) { return V3(v.x * s, v.y * s, v.z * s); }
+static inline V3 operator/(V3 v, float s) {
+    float inv = 1.0 / s;
+    return V3(v.x * inv, v.y * inv, v.z * inv);
+}
+static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float length(V3 v) { return (float)std::sqrt((double)dot(v, v)); }
+static inline V3 normalize(V3 v) { return v / length(v); }
+
+struct Idx {
+    int i, j, k;
+};
+
+// grid3d.h:52-57 -- floor of position times 1/dx, in double.
+static inline Idx pos_to_index(double x, double y, double z, double dx) {
+    double inv = 1.0 / dx;
+    return Idx{(int)std::floor(x * inv), (int)std::floor(y * inv), (int)std::floor(z * inv)};
+}
+static inline Idx pos_to_index(V3 p, double dx) { return pos_to_index(p.x, p.y, p.z, dx); }
+// grid3d.h:86-92 -- ((float)i)*dx rounded to float.
+static inline V3 index_to_pos(int i, int j, int k, double dx) {
+    return V3((float)((double)(float)i * dx), (float)((double)(float)j * dx),
+              (float)((double)(float)k * dx));
+}
+// grid3d.h:106-109 -- (float)i*dx + 0.5*dx rounded to float.
+static inline V3 index_to_center(int i, int j, int k, double dx) {
+    double hw = 0.5 * dx;
+    return V3((float)((double)(float)i * dx + hw), (float)((double)(float)j * dx + hw),
+              (float)((double)(float)k * dx + hw));
+}
+
+template <class T>
+struct Grid {
+    int ni = 0, nj = 0, nk = 0;
+    std::vector<T> a;
+    Grid() {}
+    Grid(int i, int j, int k, T v) : ni(i), nj(j), nk(k), a((size_t)i * j * k, v) {}
+    bool in(int i, int j, int k) const {
+        return i >= 0 && j >= 0 && k >= 0 && i < ni && j < nj && k < nk;
+    }
+    size_t id(int i, int j, int k) const { return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * k); }
+    T &operator()(int i, int j, int k) { return a[id(i, j, k)]; }
+    const T &operator()(int i, int j, int k) const { return a[id(i, j, k)]; }
+    void fill(T v) { std::fill(a.begin(), a.end(), v); }
+};
+typedef Grid<float> GridF;
+typedef Grid<uint8_t> GridB;
+
+template <class T>
+static T clampv(T v, T lo, T hi) { return std::max(lo, std::min(v, hi)); }
+
+// ---------------------------------------------------------------------------
+// Axis-aligned box with the reference's AABB semantics (aabb.cpp:32-133,
+// 489-518): float min corner, double extents.
+// ---------------------------------------------------------------------------
+struct Box {
+    V3 pos;
+    double w = 0, h = 0, d = 0;
+    void expand(double v) {  // aabb.cpp:122-128
+        double hh = 0.5 * v;
+        pos = pos - V3((float)hh, (float)hh, (float)hh);
+        w += v; h += v; d += v;
+    }
+    bool inside(V3 p) const {  // aabb.cpp:130-133
+        return p.x >= pos.x && p.y >= pos.y && p.z >= pos.z && p.x < pos.x + w &&
+               p.y < pos.y + h && p.z < pos.z + d;
+    }
+    V3 maxp() const { return pos + V3((float)w, (float)h, (float)d); }  // aabb.cpp:493-495
+    V3 nearest_inside(V3 p) const {                                      // aabb.cpp:501-518
+        const double eps = 1e-6;
+        if (inside(p)) return p;
+        V3 mn = pos, mx = maxp();
+        p.x = (float)std::fmax(p.x, mn.x);
+        p.y = (float)std::fmax(p.y, mn.y);
+        p.z = (float)std::fmax(p.z, mn.z);
+        p.x = (float)std::fmin((double)p.x, (double)mx.x - eps);
+        p.y = (float)std::fmin((double)p.y, (double)mx.y - eps);
+        p.z = (float)std::fmin((double)p.z, (double)mx.z - eps);
+        return p;
+    }
+};
+
+// FluidSimulation::_getBoundaryAABB (fluidsimulation.cpp:2834-2839).
+static Box boundary_box(int isize, int jsize, int ksize, double dx) {
+    Box b;
+    b.pos = V3(0, 0, 0);
+    b.w = isize * dx; b.h = jsize * dx; b.d = ksize * dx;
+    b.expand(-3 * dx - 1e-4);
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// Domain solid SDF (nodal, (I+1)(J+1)(K+1)).  The domain is a box mesh
+// (fluidsimulation.cpp:2805-2845) whose exact-band unsigned distance is
+// computed per 10^3 node block in block-local float coordinates
+// (meshlevelset.cpp:773-1010), signed by inside/outside (meshlevelset.cpp:
+// 1298-1317) and negated (meshlevelset.cpp:702-708, fluidsimulation.cpp:
+// 2918-2930), so the fluid side is positive.
+// ---------------------------------------------------------------------------
+
+// Collision::findClosestPointOnTriangle (collision.cpp:161-270).
+static V3 closest_point_on_triangle(V3 p0, V3 v0, V3 v1, V3 v2) {
+    V3 edge0 = v1 - v0, edge1 = v2 - v0, pv = v0 - p0;
+    double a = dot(edge0, edge0), b = dot(edge0, edge1), c = dot(edge1, edge1);
+    double d = dot(edge0, pv), e = dot(edge1, pv);
+    double det = a * c - b * b;
+    if (std::abs(det) < 1e-9) {
+        float feps = 1e-6f;
+        float len1 = length(v0 - v1), len2 = length(v1 - v2), len3 = length(v2 - v0);
+        if (len1 < feps && len2 < feps && len3 < feps) return v0;
+        V3 dp, dq;
+        if (len1 < feps) { dp = v0; dq = v2; }
+        else if (len2 < feps) { dp = v1; dq = v0; }
+        else { dp = v2; dq = v1; }
+        float num = dot(p0 - dp, dq - dp), den = dot(dq - dp, dq - dp);
+        if (std::abs(den) < feps) return v0;
+        float lam = num / den;
+        if (lam <= 0.0f) return dp;
+        if (lam >= 1.0f) return dq;
+        return dp + lam * (dq - dp);
+    }
+    double s = b * e - c * d, t = b * d - a * e;
+    if (s + t < det) {
+        if (s < 0.0) {
+            if (t < 0.0) {
+                if (d < 0.0) { s = clampv(-d / a, 0.0, 1.0); t = 0.0; }
+                else { s = 0.0; t = clampv(-e / c, 0.0, 1.0); }
+            } else { s = 0.0; t = clampv(-e / c, 0.0, 1.0); }
+        } else if (t < 0.0) { s = clampv(-d / a, 0.0, 1.0); t = 0.0; }
+        else { double inv = 1.0 / det; s *= inv; t *= inv; }
+    } else {
+        if (s < 0.0) {
+            double tmp0 = b + d, tmp1 = c + e;
+            if (tmp1 > tmp0) {
+                double numer = tmp1 - tmp0, denom = a - 2 * b + c;
+                s = clampv(numer / denom, 0.0, 1.0); t = 1 - s;
+            } else { t = clampv(-e / c, 0.0, 1.0); s = 0.0; }
+        } else if (t < 0.0) {
+            if (a + d > b + e) {
+                double numer = c + e - b - d, denom = a - 2 * b + c;
+                s = clampv(numer / denom, 0.0, 1.0); t = 1 - s;
+            } else { s = clampv(-e / c, 0.0, 1.0); t = 0.0; }
+        } else {
+            double numer = c + e - b - d, denom = a - 2 * b + c;
+            s = clampv(numer / denom, 0.0, 1.0); t = 1.0 - s;
+        }
+    }
+    return v0 + (float)s * edge0 + (float)t * edge1;
+}
+
+struct Tri {
+    V3 v[3];
+    Idx gmin, gmax;
+};
+
+// Box mesh (FluidSimulation::_getTriangleMeshFromAABB, fluidsimulation.cpp:
+// 2805-2832) with per-triangle exact-band node ranges
+// (MeshLevelSet::_initializeTriangleData, meshlevelset.cpp:874-915).
+static std::vector<Tri> box_triangles(const Box &bb, int isize, int jsize, int ksize, double dx,
+                                      int band) {
+    V3 p = bb.pos;
+    V3 vert[8] = {
+        V3(p.x, p.y, p.z),
+        V3((float)(p.x + bb.w), p.y, p.z),
+        V3((float)(p.x + bb.w), p.y, (float)(p.z + bb.d)),
+        V3(p.x, p.y, (float)(p.z + bb.d)),
+        V3(p.x, (float)(p.y + bb.h), p.z),
+        V3((float)(p.x + bb.w), (float)(p.y + bb.h), p.z),
+        V3((float)(p.x + bb.w), (float)(p.y + bb.h), (float)(p.z + bb.d)),
+        V3(p.x, (float)(p.y + bb.h), (float)(p.z + bb.d)),
+    };
+    const int tri[12][3] = {{0, 1, 2}, {0, 2, 3}, {4, 7, 6}, {4, 6, 5}, {0, 3, 7}, {0, 7, 4},
+                            {1, 5, 6}, {1, 6, 2}, {0, 4, 5}, {0, 5, 1}, {3, 2, 6}, {3, 6, 7}};
+    std::vector<Tri> out;
+    for (int t = 0; t < 12; t++) {
+        Tri T;
+        for (int c = 0; c < 3; c++) T.v[c] = vert[tri[t][c]];
+        // AABB(Triangle, vertices) (aabb.cpp:84-112)
+        double mnx = T.v[0].x, mny = T.v[0].y, mnz = T.v[0].z;
+        double mxx = mnx, mxy = mny, mxz = mnz;
+        for (int c = 0; c < 3; c++) {
+            mnx = std::fmin((double)T.v[c].x, mnx); mny = std::fmin((double)T.v[c].y, mny);
+            mnz = std::fmin((double)T.v[c].z, mnz); mxx = std::fmax((double)T.v[c].x, mxx);
+            mxy = std::fmax((double)T.v[c].y, mxy); mxz = std::fmax((double)T.v[c].z, mxz);
+        }
+        Box tb;
+        tb.pos = V3((float)mnx, (float)mny, (float)mnz);
+        tb.w = mxx - mnx + 1e-9; tb.h = mxy - mny + 1e-9; tb.d = mxz - mnz + 1e-9;
+        V3 pmax = tb.maxp();
+        Idx gmin = pos_to_index(tb.pos, dx), gmax = pos_to_index(pmax, dx);
+        gmin.i -= band; gmin.j -= band; gmin.k -= band;
+        gmax.i += band + 1; gmax.j += band + 1; gmax.k += band + 1;
+        if (gmax.i < 0 || gmin.i >= isize + 1 || gmax.j < 0 || gmin.j >= jsize + 1 ||
+            gmax.k < 0 || gmin.k >= ksize + 1)
+            continue;
+        gmin.i = std::max(gmin.i, 0); gmin.j = std::max(gmin.j, 0); gmin.k = std::max(gmin.k, 0);
+        gmax.i = std::min(gmax.i, isize); gmax.j = std::min(gmax.j, jsize);
+        gmax.k = std::min(gmax.k, ksize);
+        T.gmin = gmin; T.gmax = gmax;
+        out.push_back(T);
+    }
+    return out;
+}
+
+static void domain_solid_phi(int isize, int jsize, int ksize, double dx, GridF &phi) {
+    const int band = 3;     // fluidsimulation.h:1580 (_solidLevelSetExactBand)
+    const int bw = 10;      // meshlevelset.h:370 (_blockwidth)
+    phi = GridF(isize + 1, jsize + 1, ksize + 1, 0.0f);
+    float upper = (float)((phi.ni + phi.nj + phi.nk) * dx);  // meshlevelset.cpp:760-762
+    Box bb = boundary_box(isize, jsize, ksize, dx);
+    std::vector<Tri> tris = box_triangles(bb, isize, jsize, ksize, dx, band);
+    for (int k = 0; k < phi.nk; k++)
+        for (int j = 0; j < phi.nj; j++)
+            for (int i = 0; i < phi.ni; i++) {
+                // Each node is owned by one 10^3 block; distances are taken in
+                // that block's local frame (meshlevelset.cpp:967-1000).
+                int bi = i / bw, bj = j / bw, bk = k / bw;
+                V3 boff = index_to_pos(bi, bj, bk, bw * dx);
+                V3 gpos = index_to_pos(i - bi * bw, j - bj * bw, k - bk * bw, dx);
+                float best = upper;
+                for (const Tri &t : tris) {
+                    if (i < t.gmin.i || j < t.gmin.j || k < t.gmin.k || i > t.gmax.i ||
+                        j > t.gmax.j || k > t.gmax.k)
+                        continue;
+                    V3 cp = closest_point_on_triangle(gpos, t.v[0] - boff, t.v[1] - boff,
+                                                      t.v[2] - boff);
+                    float dist = length(cp - gpos);
+                    if (dist < best) best = dist;
+                }
+                // Sign: nodes strictly inside the closed box mesh are inside
+                // (MeshUtils::getGridNodesInsideTriangleMesh, meshutils.cpp:
+                // 347-428); the field is then negated (meshlevelset.cpp:702).
+                V3 np = index_to_pos(i, j, k, dx);
+                bool inside = bb.inside(np);
+                float signed_d = inside ? -best : best;
+                phi(i, j, k) = -signed_d;
+            }
+}
+
+// LevelsetUtils::fractionInside(float, float) (levelsetutils.cpp:39-52).
+static float fraction_inside2(float l, float r) {
+    if (l < 0 && r < 0) return 1;
+    if (l < 0 && r >= 0) return l / (l - r);
+    if (l >= 0 && r < 0) return r / (r - l);
+    return 0;
+}
+
+static void cycle4(float *a) {
+    float t = a[0];
+    a[0] = a[1]; a[1] = a[2]; a[2] = a[3]; a[3] = t;
+}
+
+// LevelsetUtils::fractionInside(float x4) (levelsetutils.cpp:62-141).
+static float fraction_inside4(float bl, float br, float tl, float tr) {
+    int n = (bl < 0 ? 1 : 0) + (tl < 0 ? 1 : 0) + (br < 0 ? 1 : 0) + (tr < 0 ? 1 : 0);
+    float L[4] = {bl, br, tr, tl};
+    if (n == 4) return 1;
+    if (n == 3) {
+        while (L[0] < 0) cycle4(L);
+        float s0 = 1 - fraction_inside2(L[0], L[3]);
+        float s1 = 1 - fraction_inside2(L[0], L[1]);
+        return 1.0f - 0.5f * s0 * s1;
+    }
+    if (n == 2) {
+        while (L[0] >= 0 || !(L[1] < 0 || L[2] < 0)) cycle4(L);
+        if (L[1] < 0) {
+            float sl = fraction_inside2(L[0], L[3]);
+            float sr = fraction_inside2(L[1], L[2]);
+            return 0.5f * (sl + sr);
+        }
+        float mid = 0.25f * (L[0] + L[1] + L[2] + L[3]);
+        if (mid < 0) {
+            float area = 0;
+            float s1 = 1 - fraction_inside2(L[0], L[3]);
+            float s3 = 1 - fraction_inside2(L[2], L[3]);
+            area += 0.5f * s1 * s3;
+            float s2 = 1 - fraction_inside2(L[2], L[1]);
+            float s0 = 1 - fraction_inside2(L[0], L[1]);
+            area += 0.5f * s0 * s2;
+            return 1.0f - area;
+        }
+        float area = 0;
+        float s0 = fraction_inside2(L[0], L[1]);
+        float s1 = fraction_inside2(L[0], L[3]);
+        area += 0.5f * s0 * s1;
+        float s2 = fraction_inside2(L[2], L[1]);
+        float s3 = fraction_inside2(L[2], L[3]);
+        area += 0.5f * s2 * s3;
+        return area;
+    }
+    if (n == 1) {
+        while (L[0] >= 0) cycle4(L);
+        float s0 = fraction_inside2(L[0], L[3]);
+        float s1 = fraction_inside2(L[0], L[1]);
+        return 0.5f * s0 * s1;
+    }
+    return 0;
+}
+
+// Tetrahedron volume fractions (levelsetutils.h:62-102, levelsetutils.cpp:
+// 213-259) and the two-decomposition cube average (levelsetutils.cpp:245-259).
+static inline void sort4(float &a, float &b, float &c, float &d) {
+    if (a > b) std::swap(a, b);
+    if (c > d) std::swap(c, d);
+    if (a > c) std::swap(a, c);
+    if (b > d) std::swap(b, d);
+    if (b > c) std::swap(b, c);
+}
+static inline float tet_frac(float p0, float p1, float p2, float p3) {
+    return p0 * p0 * p0 / ((p0 - p1) * (p0 - p2) * (p0 - p3));
+}
+static inline float prism_frac(float p0, float p1, float p2, float p3) {
+    float a = p0 / (p0 - p2), b = p0 / (p0 - p3), c = p1 / (p1 - p3), d = p1 / (p1 - p2);
+    return a * b * (1 - d) + b * (1 - c) * d + c * d;
+}
+static float volume_fraction4(float p0, float p1, float p2, float p3) {
+    sort4(p0, p1, p2, p3);
+    if (p3 <= 0) return 1;
+    if (p2 <= 0) return 1 - tet_frac(p3, p2, p1, p0);
+    if (p1 <= 0) return prism_frac(p0, p1, p2, p3);
+    if (p0 <= 0) return tet_frac(p0, p1, p2, p3);
+    return 0;
+}
+static float volume_fraction8(float p000, float p100, float p010, float p110, float p001,
+                              float p101, float p011, float p111) {
+    return (volume_fraction4(p000, p001, p101, p011) + volume_fraction4(p000, p101, p100, p110) +
+            volume_fraction4(p000, p010, p011, p110) + volume_fraction4(p101, p011, p111, p110) +
+            2 * volume_fraction4(p000, p011, p101, p110) +
+            volume_fraction4(p100, p101, p001, p111) + volume_fraction4(p100, p001, p000, p010) +
+            volume_fraction4(p100, p110, p111, p010) + volume_fraction4(p001, p111, p011, p010) +
+            2 * volume_fraction4(p100, p111, p001, p010)) /
+           12.0f;
+}
+
+struct Weights {
+    GridF U, V, W, C;
+};
+
+// FluidSimulation::_updateWeightGridThread (fluidsimulation.cpp:3688-3730)
+// with MeshLevelSet::getFaceWeightU/V/W and _getCellWeight
+// (meshlevelset.cpp:357-391, 1490-1513).
+static void compute_weights(const GridF &phi, int isize, int jsize, int ksize, Weights &w) {
+    w.U = GridF(isize + 1, jsize, ksize, 0.0f);
+    w.V = GridF(isize, jsize + 1, ksize, 0.0f);
+    w.W = GridF(isize, jsize, ksize + 1, 0.0f);
+    w.C = GridF(isize, jsize, ksize, 0.0f);
+    for (int k = 0; k < ksize; k++)
+        for (int j = 0; j < jsize; j++)
+            for (int i = 0; i < isize + 1; i++) {
+                float f = fraction_inside4(phi(i, j, k), phi(i, j + 1, k), phi(i, j, k + 1),
+                                           phi(i, j + 1, k + 1));
+                w.U(i, j, k) = clampv(1.0f - f, 0.0f, 1.0f);
+            }
+    for (int k = 0; k < ksize; k++)
+        for (int j = 0; j < jsize + 1; j++)
+            for (int i = 0; i < isize; i++) {
+                float f = fraction_inside4(phi(i, j, k), phi(i, j, k + 1), phi(i + 1, j, k),
+                                           phi(i + 1, j, k + 1));
+                w.V(i, j, k) = clampv(1.0f - f, 0.0f, 1.0f);
+            }
+    for (int k = 0; k < ksize + 1; k++)
+        for (int j = 0; j < jsize; j++)
+            for (int i = 0; i < isize; i++) {
+                float f = fraction_inside4(phi(i, j, k), phi(i, j + 1, k), phi(i + 1, j, k),
+                                           phi(i + 1, j + 1, k));
+                w.W(i, j, k) = clampv(1.0f - f, 0.0f, 1.0f);
+            }
+    for (int k = 0; k < ksize; k++)
+        for (int j = 0; j < jsize; j++)
+            for (int i = 0; i < isize; i++) {
+                float p000 = phi(i, j, k), p001 = phi(i, j, k + 1), p010 = phi(i, j + 1, k),
+                      p011 = phi(i, j + 1, k + 1), p100 = phi(i + 1, j, k),
+                      p101 = phi(i + 1, j, k + 1), p110 = phi(i + 1, j + 1, k),
+                      p111 = phi(i + 1, j + 1, k + 1);
+                float cw;
+                if (p000 < 0 && p001 < 0 && p010 < 0 && p011 < 0 && p100 < 0 && p101 < 0 &&
+                    p110 < 0 && p111 < 0)
+                    cw = 1.0f;
+                else if (p000 >= 0 && p001 >= 0 && p010 >= 0 && p011 >= 0 && p100 >= 0 &&
+                         p101 >= 0 && p110 >= 0 && p111 >= 0)
+                    cw = 0.0f;
+                else
+                    cw = volume_fraction8(p000, p100, p010, p110, p001, p101, p011, p111);
+                w.C(i, j, k) = clampv(1.0f - cw, 0.0f, 1.0f);
+            }
+}
+
+// MeshLevelSet::getDistanceAtCellCenter (meshlevelset.cpp:152-162).
+static inline float solid_at_center(const GridF &phi, int i, int j, int k) {
+    return 0.125f * (phi(i, j, k) + phi(i + 1, j, k) + phi(i, j + 1, k) + phi(i + 1, j + 1, k) +
+                     phi(i, j, k + 1) + phi(i + 1, j, k + 1) + phi(i, j + 1, k + 1) +
+                     phi(i + 1, j + 1, k + 1));
+}
+
+// Interpolation::trilinearInterpolate(p[8], x, y, z) (interpolation.cpp:62-71).
+static inline double trilerp(const double *p, double x, double y, double z) {
+    return p[0] * (1 - x) * (1 - y) * (1 - z) + p[1] * x * (1 - y) * (1 - z) +
+           p[2] * (1 - x) * y * (1 - z) + p[3] * (1 - x) * (1 - y) * z +
+           p[4] * x * (1 - y) * z + p[5] * (1 - x) * y * z + p[6] * x * y * (1 - z) +
+           p[7] * x * y * z;
+}
+
+// Interpolation::trilinearInterpolate(vec3, dx, Array3d<float>) (interpolation.cpp:73-112)
+// -- used for the nodal solid SDF (meshlevelset.cpp:261-263).
+static float grid_trilinear(const GridF &g, V3 p, double dx) {
+    Idx c = pos_to_index(p, dx);
+    V3 gp = index_to_pos(c.i, c.j, c.k, dx);
+    double inv = 1.0 / dx;
+    double ix = (double)(p.x - gp.x) * inv, iy = (double)(p.y - gp.y) * inv,
+           iz = (double)(p.z - gp.z) * inv;
+    double pts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                           {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+    for (int q = 0; q < 8; q++) {
+        int a = c.i + off[q][0], b = c.j + off[q][1], d = c.k + off[q][2];
+        if (g.in(a, b, d)) pts[q] = g(a, b, d);
+    }
+    return (float)trilerp(pts, ix, iy, iz);
+}
+
+// Interpolation::bilinearInterpolate / trilinearInterpolateGradient
+// (interpolation.cpp:193-258).
+static inline double bilerp(double v00, double v10, double v01, double v11, double ix, double iy) {
+    double l1 = (1 - ix) * v00 + ix * v10;
+    double l2 = (1 - ix) * v01 + ix * v11;
+    return (1 - iy) * l1 + iy * l2;
+}
+static V3 grid_gradient(const GridF &g, V3 p, double dx) {
+    Idx c = pos_to_index(p, dx);
+    V3 gp = index_to_pos(c.i, c.j, c.k, dx);
+    double inv = 1.0 / dx;
+    double ix = (double)(p.x - gp.x) * inv, iy = (double)(p.y - gp.y) * inv,
+           iz = (double)(p.z - gp.z) * inv;
+    auto at = [&](int a, int b, int d) -> float { return g.in(a, b, d) ? g(a, b, d) : 0.0f; };
+    float v000 = at(c.i, c.j, c.k), v100 = at(c.i + 1, c.j, c.k), v010 = at(c.i, c.j + 1, c.k),
+          v001 = at(c.i, c.j, c.k + 1), v101 = at(c.i + 1, c.j, c.k + 1),
+          v011 = at(c.i, c.j + 1, c.k + 1), v110 = at(c.i + 1, c.j + 1, c.k),
+          v111 = at(c.i + 1, c.j + 1, c.k + 1);
+    float ddx00 = v100 - v000, ddx10 = v110 - v010, ddx01 = v101 - v001, ddx11 = v111 - v011;
+    float dvdx = (float)bilerp(ddx00, ddx10, ddx01, ddx11, iy, iz);
+    float ddy00 = v010 - v000, ddy10 = v110 - v100, ddy01 = v011 - v001, ddy11 = v111 - v101;
+    float dvdy = (float)bilerp(ddy00, ddy10, ddy01, ddy11, ix, iz);
+    float ddz00 = v001 - v000, ddz10 = v101 - v100, ddz01 = v011 - v010, ddz11 = v111 - v110;
+    float dvdz = (float)bilerp(ddz00, ddz10, ddz01, ddz11, ix, iy);
+    return V3(dvdx, dvdy, dvdz);
+}
+
+// ---------------------------------------------------------------------------
+// Block assignment shared by the liquid SDF and P2G: the reference bins each
+// point into every 10^3 block its search box [p-sr, p+sr] overlaps
+// (particlelevelset.cpp:487-536, velocityadvector.cpp:270-320) and then, per
+// block, visits the grid points of the clamped local box
+// (particlelevelset.cpp:582-630, velocityadvector.cpp:390-446).
+// The oracle visits the same (point, block) pairs in ascending point order,
+// which is the order the reference's per-block lists hold.
+// ---------------------------------------------------------------------------
+template <class F>
+static void for_each_block_of_point(V3 p, float sr, float blockdx, int nbi, int nbj, int nbk,
+                                    F &&fn) {
+    Idx b = pos_to_index(p, blockdx);
+    V3 bp = index_to_pos(b.i, b.j, b.k, blockdx);
+    auto in_range = [&](int i, int j, int k) {
+        return i >= 0 && j >= 0 && k >= 0 && i < nbi && j < nbj && k < nbk;
+    };
+    if (p.x - sr > bp.x && p.y - sr > bp.y && p.z - sr > bp.z && p.x + sr < bp.x + blockdx &&
+        p.y + sr < bp.y + blockdx && p.z + sr < bp.z + blockdx) {
+        if (in_range(b.i, b.j, b.k)) fn(b.i, b.j, b.k);
+        return;
+    }
+    Idx gmin = pos_to_index(p.x - sr, p.y - sr, p.z - sr, blockdx);
+    Idx gmax = pos_to_index(p.x + sr, p.y + sr, p.z + sr, blockdx);
+    for (int k = gmin.k; k <= gmax.k; k++)
+        for (int j = gmin.j; j <= gmax.j; j++)
+            for (int i = gmin.i; i <= gmax.i; i++)
+                if (in_range(i, j, k)) fn(i, j, k);
+}
+
+// ParticleLevelSet::calculateSignedDistanceField (particlelevelset.cpp:161-170,
+// 299-630): phi(c) = min(3 dx, min_p |c - p| - r) in block-local float
+// arithmetic.
+static void liquid_sdf(int isize, int jsize, int ksize, double dx, const std::vector<V3> &pts,
+                       double radius, GridF &phi) {
+    const int bw = 10;  // particlelevelset.h:140
+    float maxd = (float)(3.0 * dx);
+    phi = GridF(isize, jsize, ksize, maxd);
+    if (pts.empty()) return;
+    float r = (float)radius;
+    float sr = 2.0f * r;  // particlelevelset.h:142 (_searchRadiusFactor)
+    float blockdx = (float)(bw * dx);
+    int nbi = (isize + bw - 1) / bw, nbj = (jsize + bw - 1) / bw, nbk = (ksize + bw - 1) / bw;
+    for (const V3 &p0 : pts) {
+        for_each_block_of_point(p0, (float)(2.0f * (float)radius), blockdx, nbi, nbj, nbk,
+                                [&](int bi, int bj, int bk) {
+            V3 boff = index_to_pos(bi, bj, bk, bw * dx);
+            V3 p = p0 - boff;
+            V3 pmin(p.x - sr, p.y - sr, p.z - sr), pmax(p.x + sr, p.y + sr, p.z + sr);
+            Idx gmin = pos_to_index(pmin, dx), gmax = pos_to_index(pmax, dx);
+            gmin.i = std::max(gmin.i, 0); gmin.j = std::max(gmin.j, 0); gmin.k = std::max(gmin.k, 0);
+            gmax.i = std::min(gmax.i, bw - 1); gmax.j = std::min(gmax.j, bw - 1);
+            gmax.k = std::min(gmax.k, bw - 1);
+            for (int k = gmin.k; k <= gmax.k; k++)
+                for (int j = gmin.j; j <= gmax.j; j++)
+                    for (int i = gmin.i; i <= gmax.i; i++) {
+                        int gi = bi * bw + i, gj = bj * bw + j, gk = bk * bw + k;
+                        V3 gpos = index_to_center(i, j, k, dx);
+                        float dist = length(gpos - p) - r;
+                        if (!phi.in(gi, gj, gk)) continue;
+                        float &dst = phi(gi, gj, gk);
+                        if (dist < dst) dst = dist;
+                    }
+        });
+    }
+}
+
+// ParticleLevelSet::postProcessSignedDistanceField (particlelevelset.cpp:172-196).
+static void liquid_postprocess(GridF &phi, const GridF &solid, double dx) {
+    float eps = (float)(0.005 * dx);
+    for (int k = 0; k < phi.nk; k++)
+        for (int j = 0; j < phi.nj; j++)
+            for (int i = 0; i < phi.ni; i++) {
+                if ((double)phi(i, j, k) < 0.5 * dx) {
+                    if (solid_at_center(solid, i, j, k) < 0) phi(i, j, k) = (float)(-0.5f * dx);
+                }
+                float val = phi(i, j, k);
+                if (std::abs(val) < eps) phi(i, j, k) = val > 0 ? eps : -eps;
+            }
+}
+
+struct MAC {
+    GridF U, V, W;
+};
+struct Valid {
+    GridB U, V, W;
+};
+
+static void mac_init(MAC &m, int isize, int jsize, int ksize) {
+    m.U = GridF(isize + 1, jsize, ksize, 0.0f);
+    m.V = GridF(isize, jsize + 1, ksize, 0.0f);
+    m.W = GridF(isize, jsize, ksize + 1, 0.0f);
+}
+static void valid_init(Valid &v, int isize, int jsize, int ksize) {
+    v.U = GridB(isize + 1, jsize, ksize, 0);
+    v.V = GridB(isize, jsize + 1, ksize, 0);
+    v.W = GridB(isize, jsize, ksize + 1, 0);
+}
+
+// VelocityAdvector::_advectGrid for one component (velocityadvector.cpp:
+// 64-151, 166-459): Wyvill-kernel weighted average of particle velocity on the
+// faces of one MAC component, accumulated in ascending particle order per face.
+static void p2g_component(int comp, double dx, const std::vector<V3> &pts,
+                          const std::vector<V3> &vel, double radius, GridF &out, GridB &valid) {
+    const int bw = 10;  // velocityadvector.h:163 (_chunkWidth)
+    double chunkdx = dx * bw;
+    V3 offset;  // velocityadvector.cpp:153-164
+    if (comp == 0) offset = V3(0.0f, (float)(0.5 * dx), (float)(0.5 * dx));
+    else if (comp == 1) offset = V3((float)(0.5 * dx), 0.0f, (float)(0.5 * dx));
+    else offset = V3((float)(0.5 * dx), (float)(0.5 * dx), 0.0f);
+    int ni = out.ni, nj = out.nj, nk = out.nk;
+    int nbi = (ni + bw - 1) / bw, nbj = (nj + bw - 1) / bw, nbk = (nk + bw - 1) / bw;
+    float eps = 1e-6f;
+    float sr = (float)(radius + eps);
+    float blockdx = (float)chunkdx;
+    float r = (float)radius, rsq = r * r;
+    float coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
+    float coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
+    float coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
+    GridF sum(ni, nj, nk, 0.0f), wsum(ni, nj, nk, 0.0f);
+    std::vector<uint8_t> touched_block((size_t)nbi * nbj * nbk, 0);
+    for (size_t pi = 0; pi < pts.size(); pi++) {
+        V3 pp = pts[pi] - offset;
+        float pv = comp == 0 ? vel[pi].x : (comp == 1 ? vel[pi].y : vel[pi].z);
+        for_each_block_of_point(pp, sr, blockdx, nbi, nbj, nbk, [&](int bi, int bj, int bk) {
+            touched_block[(size_t)bi + (size_t)nbi * (bj + (size_t)nbj * bk)] = 1;
+            V3 boff = index_to_pos(bi, bj, bk, bw * dx);
+            V3 p = pp - boff;
+            V3 pmin(p.x - sr, p.y - sr, p.z - sr), pmax(p.x + sr, p.y + sr, p.z + sr);
+            Idx gmin = pos_to_index(pmin, dx), gmax = pos_to_index(pmax, dx);
+            gmin.i = std::max(gmin.i, 0); gmin.j = std::max(gmin.j, 0); gmin.k = std::max(gmin.k, 0);
+            gmax.i = std::min(gmax.i, bw - 1); gmax.j = std::min(gmax.j, bw - 1);
+            gmax.k = std::min(gmax.k, bw - 1);
+            for (int k = gmin.k; k <= gmax.k; k++)
+                for (int j = gmin.j; j <= gmax.j; j++)
+                    for (int i = gmin.i; i <= gmax.i; i++) {
+                        V3 gpos = index_to_pos(i, j, k, dx);
+                        V3 v = gpos - p;
+                        float d2 = dot(v, v);
+                        if (d2 < rsq) {
+                            float wt = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
+                            int gi = bi * bw + i, gj = bj * bw + j, gk = bk * bw + k;
+                            if (!sum.in(gi, gj, gk)) continue;
+                            sum(gi, gj, gk) += wt * pv;
+                            wsum(gi, gj, gk) += wt;
+                        }
+                    }
+        });
+    }
+    // Normalise and write back every face of each processed block
+    // (velocityadvector.cpp:116-144, 448-453).
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                if (!touched_block[(size_t)(i / bw) + (size_t)nbi * ((j / bw) + (size_t)nbj * (k / bw))])
+                    continue;
+                float s = sum(i, j, k), w = wsum(i, j, k);
+                if (w > eps) s /= w;
+                out(i, j, k) = s;
+                if (w > eps) valid(i, j, k) = 1;
+            }
+}
+
+static void p2g(int isize, int jsize, int ksize, double dx, const std::vector<V3> &pts,
+                const std::vector<V3> &vel, double radius, MAC &m, Valid &v) {
+    (void)isize; (void)jsize; (void)ksize;
+    p2g_component(0, dx, pts, vel, radius, m.U, v.U);
+    p2g_component(1, dx, pts, vel, radius, m.V, v.V);
+    p2g_component(2, dx, pts, vel, radius, m.W, v.W);
+}
+
+// GridUtils::extrapolateGrid (gridutils.cpp:32-228).  Layer-synchronous
+// breadth-first fill: each new face takes the mean of its finished 6
+// neighbours in the order +i,-i,+j,-j,+k,-k; border faces count as finished.
+static void extrapolate(GridF &g, const GridB &valid, int layers) {
+    const uint8_t UNKNOWN = 0, WAITING = 1, KNOWN = 2, DONE = 3;
+    GridB st(g.ni, g.nj, g.nk, UNKNOWN);
+    for (int k = 0; k < g.nk; k++)
+        for (int j = 0; j < g.nj; j++)
+            for (int i = 0; i < g.ni; i++) {
+                if (i == 0 || j == 0 || k == 0 || i == g.ni - 1 || j == g.nj - 1 || k == g.nk - 1)
+                    st(i, j, k) = DONE;
+                else if (valid(i, j, k)) st(i, j, k) = KNOWN;
+            }
+    const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    std::vector<Idx> cells;
+    for (int layer = 0; layer < layers; layer++) {
+        cells.clear();
+        for (int k = 0; k < g.nk; k++)
+            for (int j = 0; j < g.nj; j++)
+                for (int i = 0; i < g.ni; i++) {
+                    if (st(i, j, k) != KNOWN) continue;
+                    for (int q = 0; q < 6; q++) {
+                        int a = i + nb[q][0], b = j + nb[q][1], c = k + nb[q][2];
+                        if (st(a, b, c) == UNKNOWN) {
+                            st(a, b, c) = WAITING;
+                            cells.push_back(Idx{a, b, c});
+                        }
+                    }
+                    st(i, j, k) = DONE;
+                }
+        for (const Idx &c : cells) {
+            float s = 0.0f;
+            int n = 0;
+            for (int q = 0; q < 6; q++) {
+                int a = c.i + nb[q][0], b = c.j + nb[q][1], d = c.k + nb[q][2];
+                if (st(a, b, d) == DONE) { s += g(a, b, d); n++; }
+            }
+            g(c.i, c.j, c.k) = s / (float)n;
+        }
+        if (layer != layers - 1)
+            for (const Idx &c : cells) st(c.i, c.j, c.k) = KNOWN;
+    }
+}
+
+// MACVelocityField::_interpolateLinearU/V/W + evaluateVelocityAtPositionLinear
+// (macvelocityfield.cpp:519-645): trilinear sampling in double.
+static double interp_component(const GridF &g, int comp, double x, double y, double z,
+                               double dx, int isize, int jsize, int ksize) {
+    if (!(x >= 0 && y >= 0 && z >= 0 && x < dx * isize && y < dx * jsize && z < dx * ksize))
+        return 0.0;
+    if (comp != 0) x -= 0.5 * dx;
+    if (comp != 1) y -= 0.5 * dx;
+    if (comp != 2) z -= 0.5 * dx;
+    double inv = 1.0 / dx;
+    int i = (int)std::floor(x * inv), j = (int)std::floor(y * inv), k = (int)std::floor(z * inv);
+    double gx = (double)i * dx, gy = (double)j * dx, gz = (double)k * dx;
+    double invdx = 1 / dx;
+    double ix = (x - gx) * invdx, iy = (y - gy) * invdx, iz = (z - gz) * invdx;
+    double pts[8];
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                           {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+    for (int q = 0; q < 8; q++) {
+        int a = i + off[q][0], b = j + off[q][1], c = k + off[q][2];
+        pts[q] = g.in(a, b, c) ? (double)g(a, b, c) : 0.0;
+    }
+    return trilerp(pts, ix, iy, iz);
+}
+
+static V3 mac_velocity(const MAC &m, V3 p, double dx, int isize, int jsize, int ksize) {
+    double x = p.x, y = p.y, z = p.z;
+    if (!(x >= 0 && y >= 0 && z >= 0 && x < dx * isize && y < dx * jsize && z < dx * ksize))
+        return V3();
+    double u = interp_component(m.U, 0, x, y, z, dx, isize, jsize, ksize);
+    double v = interp_component(m.V, 1, x, y, z, dx, isize, jsize, ksize);
+    double w = interp_component(m.W, 2, x, y, z, dx, isize, jsize, ksize);
+    return V3((float)u, (float)v, (float)w);
+}
+
+// ---------------------------------------------------------------------------
+// Pressure projection (pressuresolver.cpp:44-1047) and MIC(0)-PCG
+// (pcgsolver/pcgsolver.h:69-302, sparsematrix.h:175-216, blaswrapper.h:35-158).
+// ---------------------------------------------------------------------------
+struct PressureInputs {
+    int isize, jsize, ksize;
+    double dx, dt, tol;
+    int maxiter, mode;
+    const GridF *liquid;
+    const Weights *w;
+    GridF *solidU, *solidV, *solidW;  // solid face velocities (mutable)
+    MAC *mac;
+    Valid *valid;
+};
+
+struct PressureResult {
+    int n = 0, iters = 0, ok = 1;
+    double err = 0.0;
+    bool solved = false;
+    std::vector<double> rhs, soln;
+};
+
+// Row layout: fluid interior cells in k, j, i order (pressuresolver.cpp:
+// 101-110).  For each row the couplings to the +x/+y/+z neighbouring rows
+// (-1 if that neighbour is not a row) are stored; couplings to -x/-y/-z are
+// the neighbours' +x/+y/+z couplings by symmetry.
+struct Rows {
+    std::vector<Idx> cell;
+    Grid<int> key;  // GridIndexKeyMap (gridindexkeymap.cpp:49-69)
+    std::vector<double> diag, offx, offy, offz;
+    std::vector<int> nxp, nyp, nzp, nxm, nym, nzm;
+};
+
+static const double kMaxTheta = 25.0;  // pressuresolver.h (_maxtheta)
+
+// PressureSolver::_conditionSolidVelocityField (pressuresolver.cpp:124-268):
+// zero the solid face velocities around fluid pockets that never touch air.
+static void condition_solid_velocity(const PressureInputs &in) {
+    int I = in.isize, J = in.jsize, K = in.ksize;
+    const GridF &phi = *in.liquid;
+    const Weights &w = *in.w;
+    float eps = 1e-6f;
+    GridB bordersAir(I, J, K, 0);
+    for (int k = 1; k < K - 1; k++)
+        for (int j = 1; j < J - 1; j++)
+            for (int i = 1; i < I - 1; i++) {
+                if ((w.U(i, j, k) >= 0.0f + eps && phi(i - 1, j, k) >= 0.0f) ||
+                    (w.U(i + 1, j, k) >= 0.0f + eps && phi(i + 1, j, k) >= 0.0f) ||
+                    (w.V(i, j, k) >= 0.0f + eps && phi(i, j - 1, k) >= 0.0f) ||
+                    (w.V(i, j + 1, k) >= 0.0f + eps && phi(i, j + 1, k) >= 0.0f) ||
+                    (w.W(i, j, k) >= 0.0f + eps && phi(i, j, k - 1) >= 0.0f) ||
+                    (w.W(i, j, k + 1) >= 0.0f + eps && phi(i, j, k + 1) >= 0.0f))
+                    bordersAir(i, j, k) = 1;
+            }
+    GridB done(I, J, K, 0);
+    std::vector<Idx> queue, group;
+    for (int k = 1; k < K - 1; k++)
+        for (int j = 1; j < J - 1; j++)
+            for (int i = 1; i < I - 1; i++) {
+                if (phi(i, j, k) >= 0.0f) { done(i, j, k) = 1; continue; }
+                if (done(i, j, k)) continue;
+                queue.push_back(Idx{i, j, k});
+                done(i, j, k) = 1;
+                group.clear();
+                while (!queue.empty()) {
+                    Idx g = queue.back();
+                    queue.pop_back();
+                    auto visit = [&](int a, int b, int c, float wf) {
+                        if (!done(a, b, c) && phi(a, b, c) < 0.0f && wf >= 0.0f + eps) {
+                            queue.push_back(Idx{a, b, c});
+                            done(a, b, c) = 1;
+                        }
+                    };
+                    visit(g.i - 1, g.j, g.k, w.U(g.i, g.j, g.k));
+                    visit(g.i + 1, g.j, g.k, w.U(g.i + 1, g.j, g.k));
+                    visit(g.i, g.j - 1, g.k, w.V(g.i, g.j, g.k));
+                    visit(g.i, g.j + 1, g.k, w.V(g.i, g.j + 1, g.k));
+                    visit(g.i, g.j, g.k - 1, w.W(g.i, g.j, g.k));
+                    visit(g.i, g.j, g.k + 1, w.W(g.i, g.j, g.k + 1));
+                    group.push_back(g);
+                }
+                if (group.size() == 1) continue;
+                bool isolated = true;
+                for (const Idx &g : group)
+                    if (bordersAir(g.i, g.j, g.k)) { isolated = false; break; }
+                if (!isolated) continue;
+                for (const Idx &g : group) {
+                    (*in.solidU)(g.i, g.j, g.k) = 0.0f; (*in.solidU)(g.i + 1, g.j, g.k) = 0.0f;
+                    (*in.solidV)(g.i, g.j, g.k) = 0.0f; (*in.solidV)(g.i, g.j + 1, g.k) = 0.0f;
+                    (*in.solidW)(g.i, g.j, g.k) = 0.0f; (*in.solidW)(g.i, g.j, g.k + 1) = 0.0f;
+                }
+            }
+}
+
+static void build_rows(const PressureInputs &in, Rows &R) {
+    int I = in.isize, J = in.jsize, K = in.ksize;
+    const GridF &phi = *in.liquid;
+    R.key = Grid<int>(I, J, K, -1);
+    R.cell.clear();
+    for (int k = 1; k < K - 1; k++)
+        for (int j = 1; j < J - 1; j++)
+            for (int i = 1; i < I - 1; i++)
+                if (phi(i, j, k) < 0) {
+                    R.key(i, j, k) = (int)R.cell.size();
+                    R.cell.push_back(Idx{i, j, k});
+                }
+}
+
+// PressureSolver::_calculateNegativeDivergenceVectorThread (pressuresolver.cpp:580-669).
+static void assemble_rhs(const PressureInputs &in, const Rows &R, std::vector<double> &rhs) {
+    const Weights &w = *in.w;
+    const MAC &m = *in.mac;
+    double factor = 1.0 / in.dx;
+    rhs.assign(R.cell.size(), 0.0);
+    for (size_t r = 0; r < R.cell.size(); r++) {
+        int i = R.cell[r].i, j = R.cell[r].j, k = R.cell[r].k;
+        double volCenter = w.C(i, j, k);
+        double volRight = w.U(i + 1, j, k), volLeft = w.U(i, j, k);
+        double volTop = w.V(i, j + 1, k), volBottom = w.V(i, j, k);
+        double volFront = w.W(i, j, k + 1), volBack = w.W(i, j, k);
+        double div = 0.0;
+        div += -factor * volRight * m.U(i + 1, j, k);
+        div += factor * volLeft * m.U(i, j, k);
+        div += -factor * volTop * m.V(i, j + 1, k);
+        div += factor * volBottom * m.V(i, j, k);
+        div += -factor * volFront * m.W(i, j, k + 1);
+        div += factor * volBack * m.W(i, j, k);
+        div += factor * (volRight - volCenter) * (*in.solidU)(i + 1, j, k);
+        div += -factor * (volLeft - volCenter) * (*in.solidU)(i, j, k);
+        div += factor * (volTop - volCenter) * (*in.solidV)(i, j + 1, k);
+        div += -factor * (volBottom - volCenter) * (*in.solidV)(i, j, k);
+        div += factor * (volFront - volCenter) * (*in.solidW)(i, j, k + 1);
+        div += -factor * (volBack - volCenter) * (*in.solidW)(i, j, k);
+        rhs[r] = div;
+    }
+}
+
+// PressureSolver::_calculateMatrixCoefficientsThread (pressuresolver.cpp:723-808).
+static void assemble_matrix(const PressureInputs &in, Rows &R) {
+    const Weights &w = *in.w;
+    const GridF &phi = *in.liquid;
+    double factor = in.dt / (in.dx * in.dx);
+    double eps = 1e-9;
+    size_t n = R.cell.size();
+    R.diag.assign(n, 0); R.offx.assign(n, 0); R.offy.assign(n, 0); R.offz.assign(n, 0);
+    R.nxp.assign(n, -1); R.nyp.assign(n, -1); R.nzp.assign(n, -1);
+    R.nxm.assign(n, -1); R.nym.assign(n, -1); R.nzm.assign(n, -1);
+    for (size_t r = 0; r < n; r++) {
+        int i = R.cell[r].i, j = R.cell[r].j, k = R.cell[r].k;
+        double vR = w.U(i + 1, j, k), vL = w.U(i, j, k), vT = w.V(i, j + 1, k),
+               vB = w.V(i, j, k), vF = w.W(i, j, k + 1), vK = w.W(i, j, k);
+        double pc = phi(i, j, k);
+        double diag = (vR + vL + vT + vB + vF + vK) * factor;
+        auto side = [&](double vol, int a, int b, int c, int &nbr, double *off) {
+            double pn = phi(a, b, c);
+            if (pn < 0.0) {
+                int key = R.key(a, b, c);
+                if (key != -1) {
+                    nbr = key;
+                    if (off) *off = -vol * factor;
+                }
+            } else {
+                double theta = clampv(pn / (pc + eps), -kMaxTheta, kMaxTheta);
+                diag -= vol * factor * theta;
+            }
+        };
+        side(vR, i + 1, j, k, R.nxp[r], &R.offx[r]);
+        side(vL, i - 1, j, k, R.nxm[r], nullptr);
+        side(vT, i, j + 1, k, R.nyp[r], &R.offy[r]);
+        side(vB, i, j - 1, k, R.nym[r], nullptr);
+        side(vF, i, j, k + 1, R.nzp[r], &R.offz[r]);
+        side(vK, i, j, k - 1, R.nzm[r], nullptr);
+        R.diag[r] = std::max(diag, 0.0);
+    }
+}
+
+// Modified incomplete Cholesky, level zero, tau = 0.97, sigma = 0.25
+// (factorModifiedIncompleteColesky0, pcgsolver.h:69-185).  For the 7-point
+// stencil in k, j, i row order the reference's column-oriented elimination
+// reduces to a per-row recurrence over the lower neighbours taken in
+// increasing row order (-z, -y, -x): for each such neighbour n with a
+// nonzero original diagonal,
+//     L    = A(n, row) * invdiag(n)
+//     d   -= L * L
+//     d   -= (tau * L) * missing(n, row)
+// where missing(n, row) sums, in increasing row order, the scaled couplings
+// of n to its other upper neighbours (which are never coupled to row).  The
+// pivot is then replaced by A(row,row) if it fell below sigma * A(row,row)
+// and inverted as 1/sqrt.  Rows with A(row,row) == 0 keep their accumulated
+// value and their column is left unscaled, as in the reference.
+struct MIC {
+    std::vector<double> invdiag, Lx, Ly, Lz;  // Lx[n] = L(n+x, n) etc. (column n)
+};
+
+static void mic0_factor(const Rows &R, MIC &M) {
+    const double tau = 0.97, sigma = 0.25;
+    size_t n = R.cell.size();
+    M.invdiag.assign(n, 0.0);
+    M.Lx.assign(n, 0.0); M.Ly.assign(n, 0.0); M.Lz.assign(n, 0.0);
+    for (size_t r = 0; r < n; r++) {
+        double d = R.diag[r];
+        // Contributions from lower neighbours in increasing row order.
+        int lower[3] = {R.nzm[r], R.nym[r], R.nxm[r]};
+        for (int q = 0; q < 3; q++) {
+            int nb = lower[q];
+            if (nb < 0 || R.diag[nb] == 0) continue;
+            double Lcx = R.nxp[nb] >= 0 ? M.Lx[nb] : 0.0;
+            double Lcy = R.nyp[nb] >= 0 ? M.Ly[nb] : 0.0;
+            double Lcz = R.nzp[nb] >= 0 ? M.Lz[nb] : 0.0;
+            double L;
+            double missing = 0;
+            if (q == 0) {         // row is nb's +z neighbour: others are +x, +y
+                L = Lcz;
+                if (R.nxp[nb] >= 0) missing += Lcx;
+                if (R.nyp[nb] >= 0) missing += Lcy;
+            } else if (q == 1) {  // row is nb's +y neighbour: others are +x, +z
+                L = Lcy;
+                if (R.nxp[nb] >= 0) missing += Lcx;
+                if (R.nzp[nb] >= 0) missing += Lcz;
+            } else {              // row is nb's +x neighbour: others are +y, +z
+                L = Lcx;
+                if (R.nyp[nb] >= 0) missing += Lcy;
+                if (R.nzp[nb] >= 0) missing += Lcz;
+            }
+            d -= L * L;
+            d -= tau * L * missing;
+        }
+        if (R.diag[r] == 0) {
+            M.invdiag[r] = d;
+            M.Lx[r] = R.offx[r]; M.Ly[r] = R.offy[r]; M.Lz[r] = R.offz[r];
+            continue;
+        }
+        if (d < sigma * R.diag[r]) M.invdiag[r] = 1 / std::sqrt(R.diag[r]);
+        else M.invdiag[r] = 1 / std::sqrt(d);
+        M.Lx[r] = R.offx[r] * M.invdiag[r];
+        M.Ly[r] = R.offy[r] * M.invdiag[r];
+        M.Lz[r] = R.offz[r] * M.invdiag[r];
+    }
+}
+
+// solveLower + solveLowerTransposeInPlace (pcgsolver.h:191-221) written as
+// row gathers with the same operation order.
+static void mic0_apply(const Rows &R, const MIC &M, const std::vector<double> &r,
+                       std::vector<double> &z) {
+    size_t n = R.cell.size();
+    z.assign(n, 0.0);
+    for (size_t c = 0; c < n; c++) {
+        double q = r[c];
+        if (R.nzm[c] >= 0) q -= M.Lz[R.nzm[c]] * z[R.nzm[c]];
+        if (R.nym[c] >= 0) q -= M.Ly[R.nym[c]] * z[R.nym[c]];
+        if (R.nxm[c] >= 0) q -= M.Lx[R.nxm[c]] * z[R.nxm[c]];
+        z[c] = q * M.invdiag[c];
+    }
+    for (size_t cc = n; cc-- > 0;) {
+        double q = z[cc];
+        if (R.nxp[cc] >= 0) q -= M.Lx[cc] * z[R.nxp[cc]];
+        if (R.nyp[cc] >= 0) q -= M.Ly[cc] * z[R.nyp[cc]];
+        if (R.nzp[cc] >= 0) q -= M.Lz[cc] * z[R.nzp[cc]];
+        z[cc] = q * M.invdiag[cc];
+    }
+}
+
+// FixedSparseMatrix multiply (sparsematrix.h:187-216): row entries summed in
+// increasing column order (-z, -y, -x, diagonal, +x, +y, +z).
+static void spmv(const Rows &R, const std::vector<double> &x, std::vector<double> &y) {
+    size_t n = R.cell.size();
+    y.assign(n, 0.0);
+    for (size_t c = 0; c < n; c++) {
+        double s = 0;
+        if (R.nzm[c] >= 0) s += R.offz[R.nzm[c]] * x[R.nzm[c]];
+        if (R.nym[c] >= 0) s += R.offy[R.nym[c]] * x[R.nym[c]];
+        if (R.nxm[c] >= 0) s += R.offx[R.nxm[c]] * x[R.nxm[c]];
+        s += R.diag[c] * x[c];
+        if (R.nxp[c] >= 0) s += R.offx[c] * x[R.nxp[c]];
+        if (R.nyp[c] >= 0) s += R.offy[c] * x[R.nyp[c]];
+        if (R.nzp[c] >= 0) s += R.offz[c] * x[R.nzp[c]];
+        y[c] = s;
+    }
+}
+
+static double dot_seq(const std::vector<double> &a, const std::vector<double> &b) {
+    double s = 0;
+    for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i];
+    return s;
+}
+static double absmax(const std::vector<double> &a) {
+    double m = 0;
+    for (double v : a)
+        if (std::fabs(v) > m) m = std::fabs(v);
+    return m;
+}
+
+// PCGSolver<double>::solve (pcgsolver.h:248-302).  mode 0 rounds alpha/beta
+// to float as BLAS::addScaled's float parameter does (blaswrapper.h:135-158).
+static bool pcg(const Rows &R, const std::vector<double> &b, std::vector<double> &x, double tolf,
+                int maxit, int mode, double &resid, int &iters) {
+    size_t n = R.cell.size();
+    x.assign(n, 0.0);
+    std::vector<double> r = b, z, s, q;
+    resid = absmax(r);
+    if (resid == 0) { iters = 0; return true; }
+    if (tolf < 1e-30) tolf = 1e-30;
+    double tol = tolf * resid;
+    MIC M;
+    mic0_factor(R, M);
+    mic0_apply(R, M, r, z);
+    double rho = dot_seq(z, r);
+    if (rho == 0 || rho != rho) { iters = 0; return false; }
+    s = z;
+    int it;
+    for (it = 0; it < maxit; it++) {
+        spmv(R, s, z);
+        double alpha = rho / dot_seq(s, z);
+        if (mode == ORC_PCG_REF_EXACT) {
+            float af = (float)alpha, naf = (float)(-alpha);
+            for (size_t i = 0; i < n; i++) x[i] += af * s[i];
+            for (size_t i = 0; i < n; i++) r[i] += naf * z[i];
+        } else {
+            for (size_t i = 0; i < n; i++) x[i] += alpha * s[i];
+            for (size_t i = 0; i < n; i++) r[i] -= alpha * z[i];
+        }
+        resid = absmax(r);
+        if (resid <= tol) { iters = it + 1; return true; }
+        mic0_apply(R, M, r, z);
+        double rhoNew = dot_seq(z, r);
+        double beta = rhoNew / rho;
+        if (mode == ORC_PCG_REF_EXACT) {
+            float bf = (float)beta;
+            for (size_t i = 0; i < n; i++) z[i] += bf * s[i];
+        } else {
+            for (size_t i = 0; i < n; i++) z[i] += beta * s[i];
+        }
+        s.swap(z);
+        rho = rhoNew;
+    }
+    iters = it;
+    return false;
+}
+
+// PressureSolver::_applySolutionToVelocityField + _applyPressureToVelocityFieldThread
+// (pressuresolver.cpp:842-1047) without surface tension.
+static void apply_pressure(const PressureInputs &in, const Rows &R, const std::vector<double> &p) {
+    int I = in.isize, J = in.jsize, K = in.ksize;
+    const GridF &phi = *in.liquid;
+    const Weights &w = *in.w;
+    MAC &m = *in.mac;
+    Valid &v = *in.valid;
+    GridF pg(I, J, K, 0.0f);
+    for (size_t r = 0; r < R.cell.size(); r++) pg(R.cell[r].i, R.cell[r].j, R.cell[r].k) = (float)p[r];
+    auto fluid = [&](int i, int j, int k) { return phi(i, j, k) < 0.0; };
+    std::fill(v.U.a.begin(), v.U.a.end(), 0);
+    std::fill(v.V.a.begin(), v.V.a.end(), 0);
+    std::fill(v.W.a.begin(), v.W.a.end(), 0);
+    float factor = (float)(in.dt / in.dx);
+    float eps = 1e-6f;
+    // Ghost-fluid pressure difference across a face between cells a and b.
+    auto diff = [&](int ai, int aj, int ak, int bi, int bj, int bk) -> float {
+        float p1 = 0.0f, p2 = 0.0f;
+        if (fluid(ai, aj, ak) && fluid(bi, bj, bk)) {
+            p1 = pg(ai, aj, ak);
+            p2 = pg(bi, bj, bk);
+        } else {
+            float phi1 = phi(ai, aj, ak), phi2 = phi(bi, bj, bk);
+            double tension = 0.0;
+            if (fluid(ai, aj, ak)) {
+                float tp = phi2 / (phi1 + eps);
+                tp = (float)clampv((double)tp, -kMaxTheta, kMaxTheta);
+                float tt = (phi1 - phi2) / (phi1 + eps);
+                tt = (float)clampv((double)tt, -kMaxTheta, kMaxTheta);
+                p1 = pg(ai, aj, ak);
+                p2 = (float)(tt * tension + tp * p1);
+            } else {
+                float tp = phi1 / (phi2 + eps);
+                tp = (float)clampv((double)tp, -kMaxTheta, kMaxTheta);
+                float tt = (phi2 - phi1) / (phi2 + eps);
+                tt = (float)clampv((double)tt, -kMaxTheta, kMaxTheta);
+                p2 = pg(bi, bj, bk);
+                p1 = (float)(tt * tension + tp * p2);
+            }
+        }
+        return p2 - p1;
+    };
+    // U faces (skip i == 0 and i == I-1, pressuresolver.cpp:918-920).
+    for (int k = 0; k < K; k++)
+        for (int j = 0; j < J; j++)
+            for (int i = 0; i < I + 1; i++) {
+                if (i == 0 || i == I - 1) continue;
+                bool border = (i == I) ? fluid(i - 1, j, k) : (fluid(i, j, k) || fluid(i - 1, j, k));
+                if (w.U(i, j, k) > 0 && border) {
+                    m.U(i, j, k) += (float)(-factor * diff(i - 1, j, k, i, j, k));
+                    v.U(i, j, k) = 1;
+                } else {
+                    m.U(i, j, k) = 0.0f;
+                }
+            }
+    for (int k = 0; k < K; k++)
+        for (int j = 0; j < J + 1; j++)
+            for (int i = 0; i < I; i++) {
+                if (j == 0 || j == J - 1) continue;
+                bool border = (j == J) ? fluid(i, j - 1, k) : (fluid(i, j, k) || fluid(i, j - 1, k));
+                if (w.V(i, j, k) > 0 && border) {
+                    m.V(i, j, k) += (float)(-factor * diff(i, j - 1, k, i, j, k));
+                    v.V(i, j, k) = 1;
+                } else {
+                    m.V(i, j, k) = 0.0f;
+                }
+            }
+    for (int k = 0; k < K + 1; k++)
+        for (int j = 0; j < J; j++)
+            for (int i = 0; i < I; i++) {
+                if (k == 0 || k == K - 1) continue;
+                bool border = (k == K) ? fluid(i, j, k - 1) : (fluid(i, j, k) || fluid(i, j, k - 1));
+                if (w.W(i, j, k) > 0 && border) {
+                    m.W(i, j, k) += (float)(-factor * diff(i, j, k - 1, i, j, k));
+                    v.W(i, j, k) = 1;
+                } else {
+                    m.W(i, j, k) = 0.0f;
+                }
+            }
+}
+
+// PressureSolver::solve (pressuresolver.cpp:44-77) + _solveLinearSystem (:810-840).
+static PressureResult pressure_solve(const PressureInputs &in) {
+    PressureResult res;
+    Rows R;
+    build_rows(in, R);
+    res.n = (int)R.cell.size();
+    condition_solid_velocity(in);
+    assemble_rhs(in, R, res.rhs);
+    double maxAbs = 0.0;
+    for (double v : res.rhs)
+        if (std::fabs(v) > maxAbs) maxAbs = std::fabs(v);
+    res.soln.assign(R.cell.size(), 0.0);
+    if (maxAbs < in.tol) {
+        res.iters = 0; res.err = 0.0; res.ok = 1;
+        return res;
+    }
+    assemble_matrix(in, R);
+    double err;
+    int it;
+    bool success = pcg(R, res.rhs, res.soln, in.tol, in.maxiter, in.mode, err, it);
+    res.iters = it; res.err = err;
+    res.ok = (success || (it == in.maxiter && err < 1.0)) ? 1 : 0;
+    if (res.ok) {
+        apply_pressure(in, R, res.soln);
+        res.solved = true;
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Simulation driver (FluidSimulation, fluidsimulation.cpp)
+// ---------------------------------------------------------------------------
+struct Sim {
+    int I, J, K;
+    double dx;
+    double cfl = 5.0;                 // fluidsimulation.h:1545
+    int minSteps = 1, maxSteps = 6;   // fluidsimulation.h:1541-1542
+    double picRatio = 0.05;           // fluidsimulation.h:1678
+    int pcgMode = ORC_PCG_REF_EXACT;
+    double pcgTol = 1e-9;             // fluidsimulation.h:1657
+    int pcgMaxIter = 1000;            // fluidsimulation.h:1659
+    std::vector<V3> bodyForces;
+    std::vector<V3> pos, vel;
+    bool initialized = false, solidUpToDate = false, weightsUpToDate = false;
+    bool outputInitialized = false;
+    int currentFrame = 0, stepNumber = 0, lastSubsteps = 0;
+    double frameDt = 0, remaining = 0, curStep = 0;
+    double radius = 0;
+    GridF liquid, solid;
+    GridF solidU, solidV, solidW;
+    Weights w;
+    MAC mac, saved;
+    Valid valid;
+    Grid<uint8_t> nearSolid;
+    double nearSolidCellSize = 0;
+    int lastIters = 0, lastOk = 1;
+    double lastErr = 0;
+
+    V3 body_force() const {  // fluidsimulation.cpp:3363-3370
+        V3 bf;
+        for (const V3 &f : bodyForces) bf = bf + f;
+        return bf;
+    }
+};
+
+// FluidSimulation::_updateNearSolidGrid (fluidsimulation.cpp:3094-3126).
+static void update_near_solid(Sim &S) {
+    S.nearSolidCellSize = 3 * S.dx;
+    int gi = (int)std::ceil((double)S.I * S.dx / S.nearSolidCellSize);
+    int gj = (int)std::ceil((double)S.J * S.dx / S.nearSolidCellSize);
+    int gk = (int)std::ceil((double)S.K * S.dx / S.nearSolidCellSize);
+    S.nearSolid = GridB(gi, gj, gk, 0);
+    float maxd = (float)(3 * S.dx);
+    for (int k = 0; k < S.K; k++)
+        for (int j = 0; j < S.J; j++)
+            for (int i = 0; i < S.I; i++)
+                if (std::abs(S.solid(i, j, k)) < maxd) S.nearSolid(i / 3, j / 3, k / 3) = 1;
+    int layers = (int)std::ceil((float)S.cfl / (float)3);
+    const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    for (int l = 0; l < layers; l++) {  // GridUtils::featherGrid6 (gridutils.cpp:230-265)
+        GridB src = S.nearSolid;
+        for (int k = 0; k < gk; k++)
+            for (int j = 0; j < gj; j++)
+                for (int i = 0; i < gi; i++) {
+                    if (!src(i, j, k)) continue;
+                    for (int q = 0; q < 6; q++) {
+                        int a = i + nb[q][0], b = j + nb[q][1], c = k + nb[q][2];
+                        if (S.nearSolid.in(a, b, c)) S.nearSolid(a, b, c) = 1;
+                    }
+                }
+    }
+}
+
+// FluidSimulation::_resolveCollision (fluidsimulation.cpp:4221-4296).
+static V3 resolve_collision(const Sim &S, V3 oldp, V3 newp, const Box &boundary) {
+    Idx g = pos_to_index(newp, S.dx);
+    if (!(g.i >= 0 && g.j >= 0 && g.k >= 0 && g.i < S.I && g.j < S.J && g.k < S.K))
+        newp = boundary.nearest_inside(newp);
+    Idx og = pos_to_index(oldp, S.nearSolidCellSize), ng = pos_to_index(newp, S.nearSolidCellSize);
+    auto near = [&](Idx c) { return S.nearSolid.in(c.i, c.j, c.k) && S.nearSolid(c.i, c.j, c.k); };
+    if (!near(og) && !near(ng)) return newp;
+    float eps = 1e-6f;
+    float stepDistance = 0.5f * (float)S.dx;  // fluidsimulation.h:1691
+    float travel = length(newp - oldp);
+    if (travel < eps) return newp;
+    int numSteps = (int)std::ceil(travel / stepDistance);
+    V3 stepdir = normalize(newp - oldp);
+    V3 last = oldp, cur;
+    bool found = false;
+    float collisionPhi = 0.0f;
+    for (int s = 0; s < numSteps; s++) {
+        if (s == numSteps - 1) cur = newp;
+        else cur = oldp + ((float)(s + 1) * stepDistance) * stepdir;
+        float phi = grid_trilinear(S.solid, cur, S.dx);
+        if (phi < 0.0f || !boundary.inside(cur)) {
+            collisionPhi = phi;
+            found = true;
+            break;
+        }
+        last = cur;
+    }
+    if (!found) return newp;
+    V3 resolved;
+    float maxResolved = (float)(S.cfl * S.dx);
+    V3 grad = grid_gradient(S.solid, cur, S.dx);
+    if (length(grad) > eps) {
+        grad = normalize(grad);
+        resolved = cur - (float)(collisionPhi - 0.1f * S.dx) * grad;
+        float rphi = grid_trilinear(S.solid, resolved, S.dx);
+        float rdist = length(resolved - cur);
+        if (rphi < 0 || rdist > maxResolved) resolved = last;
+    } else {
+        resolved = last;
+    }
+    if (!boundary.inside(resolved)) {
+        V3 orig = resolved;
+        resolved = boundary.nearest_inside(resolved);
+        float rphi = grid_trilinear(S.solid, resolved, S.dx);
+        float rdist = length(resolved - orig);
+        if (rphi < 0.0f || rdist > maxResolved) resolved = last;
+    }
+    return resolved;
+}
+
+// FluidSimulation::_RK3 (fluidsimulation.cpp:4191-4198).
+static V3 rk3(const MAC &m, V3 p0, double dt, double dx, int I, int J, int K) {
+    V3 k1 = mac_velocity(m, p0, dx, I, J, K);
+    V3 k2 = mac_velocity(m, p0 + (float)(0.5 * dt) * k1, dx, I, J, K);
+    V3 k3 = mac_velocity(m, p0 + (float)(0.75 * dt) * k2, dx, I, J, K);
+    return p0 + (float)(dt / 9.0f) * (2.0f * k1 + 3.0f * k2 + 4.0f * k3);
+}
+
+// FluidSimulation::_getMarkerParticleSpeedLimit (fluidsimulation.cpp:4298-4322).
+static float speed_limit(const Sim &S, double dt) {
+    double step = S.cfl * S.dx / dt;
+    std::vector<int> counts(S.maxSteps, 0);
+    for (const V3 &v : S.vel) {
+        double speed = (double)length(v);
+        int idx = (int)std::fmin(std::floor(speed / step), S.maxSteps - 1);
+        counts[idx]++;
+    }
+    double maxpct = 0.0005;  // fluidsimulation.h:1689
+    int maxabs = 35;         // fluidsimulation.h:1690
+    int maxRemoval = (int)std::fmin((int)((double)S.vel.size() * maxpct), maxabs);
+    double maxspeed = S.maxSteps * step;
+    int cur = 0;
+    for (int i = (int)counts.size() - 1; i > 0; i--) {
+        if (cur + counts[i] > maxRemoval) break;
+        cur += counts[i];
+        maxspeed = i * step;
+    }
+    return (float)maxspeed;
+}
+
+// FluidSimulation::_advanceMarkerParticles + _removeMarkerParticles
+// (fluidsimulation.cpp:4200-4391).
+static void advance_particles(Sim &S, double dt) {
+    Box boundary = boundary_box(S.I, S.J, S.K, S.dx);
+    boundary.expand(-0.1f * S.dx);  // fluidsimulation.h:1685 (_solidBufferWidth)
+    std::vector<V3> out(S.pos.size());
+    for (size_t i = 0; i < S.pos.size(); i++) {
+        out[i] = rk3(S.mac, S.pos[i], dt, S.dx, S.I, S.J, S.K);
+        out[i] = resolve_collision(S, S.pos[i], out[i], boundary);
+    }
+    S.pos = out;
+    // Removal (fluidsimulation.cpp:4324-4353) uses the frame step.
+    float maxspeed = speed_limit(S, S.frameDt);
+    double maxsq = maxspeed * maxspeed;
+    Grid<int> count(S.I, S.J, S.K, 0);
+    std::vector<uint8_t> removed(S.pos.size(), 0);
+    for (size_t i = 0; i < S.pos.size(); i++)
+        removed[i] = grid_trilinear(S.solid, S.pos[i], S.dx) < 0.0f;
+    for (size_t i = 0; i < S.pos.size(); i++) {
+        if (removed[i]) continue;
+        Idx g = pos_to_index(S.pos[i], S.dx);
+        if (count(g.i, g.j, g.k) >= 250) { removed[i] = 1; continue; }
+        count(g.i, g.j, g.k) += 1;
+        if (dot(S.vel[i], S.vel[i]) > maxsq) { removed[i] = 1; continue; }
+    }
+    size_t cur = 0;
+    for (size_t i = 0; i < S.pos.size(); i++) {
+        if (!removed[i]) { S.pos[cur] = S.pos[i]; S.vel[cur] = S.vel[i]; cur++; }
+    }
+    S.pos.resize(cur);
+    S.vel.resize(cur);
+}
+
+// FluidSimulation::_constrainVelocityFieldThread (fluidsimulation.cpp:3873-3935)
+// with the domain boundary's friction (0 by default, fluidsimulation.h:1524).
+static void constrain(MAC &m, const Sim &S) {
+    const float f = 0.0f;
+    auto con = [&](GridF &g, const GridF &w, const GridF &sv) {
+        for (size_t q = 0; q < g.a.size(); q++) {
+            if (w.a[q] == 0) g.a[q] = sv.a[q];
+            else if (w.a[q] < 1.0f) g.a[q] = f * sv.a[q] + (1.0f - f) * g.a[q];
+        }
+    };
+    con(m.U, S.w.U, S.solidU);
+    con(m.V, S.w.V, S.solidV);
+    con(m.W, S.w.W, S.solidW);
+}
+
+// FluidSimulation::_stepFluid (fluidsimulation.cpp:5471-5508) for scenes with
+// domain walls only and a constant body force.
+static void step_fluid(Sim &S, double dt) {
+    // _updateObstacleObjects (fluidsimulation.cpp:3196-3210, 3028-3072).
+    if (!S.solidUpToDate && !S.pos.empty()) {
+        domain_solid_phi(S.I, S.J, S.K, S.dx, S.solid);
+        S.solidU = GridF(S.I + 1, S.J, S.K, 0.0f);
+        S.solidV = GridF(S.I, S.J + 1, S.K, 0.0f);
+        S.solidW = GridF(S.I, S.J, S.K + 1, 0.0f);
+        S.solidUpToDate = true;
+        S.weightsUpToDate = false;
+    }
+    update_near_solid(S);
+    // _updateLiquidLevelSet + postProcess (fluidsimulation.cpp:3225-3247).
+    liquid_sdf(S.I, S.J, S.K, S.dx, S.pos, S.radius, S.liquid);
+    liquid_postprocess(S.liquid, S.solid, S.dx);
+    // _advectVelocityField (fluidsimulation.cpp:3253-3277).
+    mac_init(S.mac, S.I, S.J, S.K);
+    valid_init(S.valid, S.I, S.J, S.K);
+    int layers = (int)std::ceil(S.cfl) + 2;
+    if (!S.pos.empty()) {
+        p2g(S.I, S.J, S.K, S.dx, S.pos, S.vel, S.radius, S.mac, S.valid);
+        extrapolate(S.mac.U, S.valid.U, layers);
+        extrapolate(S.mac.V, S.valid.V, layers);
+        extrapolate(S.mac.W, S.valid.W, layers);
+    }
+    S.saved = S.mac;
+    // _applyConstantBodyForces (fluidsimulation.cpp:3450-3489); no inflow faces.
+    V3 bf = S.body_force();
+    float beps = 1e-6f;
+    if (std::fabs(bf.x) > beps) for (float &u : S.mac.U.a) u += (float)(bf.x * dt);
+    if (std::fabs(bf.y) > beps) for (float &v : S.mac.V.a) v += (float)(bf.y * dt);
+    if (std::fabs(bf.z) > beps) for (float &w : S.mac.W.a) w += (float)(bf.z * dt);
+    // _pressureSolve (fluidsimulation.cpp:3732-3769).
+    if (!S.weightsUpToDate) {
+        compute_weights(S.solid, S.I, S.J, S.K, S.w);
+        S.weightsUpToDate = true;
+    }
+    PressureInputs in{S.I, S.J, S.K, S.dx, dt, S.pcgTol, S.pcgMaxIter, S.pcgMode, &S.liquid, &S.w,
+                      &S.solidU, &S.solidV, &S.solidW, &S.mac, &S.valid};
+    PressureResult pr = pressure_solve(in);
+    S.lastIters = pr.iters; S.lastErr = pr.err; S.lastOk = pr.ok;
+    extrapolate(S.mac.U, S.valid.U, layers);
+    extrapolate(S.mac.V, S.valid.V, layers);
+    extrapolate(S.mac.W, S.valid.W, layers);
+    // _constrainVelocityFields (fluidsimulation.cpp:3937-3948).
+    constrain(S.saved, S);
+    constrain(S.mac, S);
+    // _updatePICFLIPMarkerParticleVelocitiesThread (fluidsimulation.cpp:4082-4092).
+    for (size_t i = 0; i < S.pos.size(); i++) {
+        V3 p = S.pos[i];
+        V3 vpic = mac_velocity(S.mac, p, S.dx, S.I, S.J, S.K);
+        V3 vflip = S.vel[i] + vpic - mac_velocity(S.saved, p, S.dx, S.I, S.J, S.K);
+        S.vel[i] = (float)S.picRatio * vpic + (float)(1 - S.picRatio) * vflip;
+    }
+    advance_particles(S, dt);
+}
+
+// FluidSimulation::_calculateNextTimeStep (fluidsimulation.cpp:5593-5613).
+static double next_time_step(const Sim &S, double dt) {
+    double maxu = 0.0;
+    if (S.currentFrame == 0 && S.stepNumber == 0) {
+        maxu = 0.0 + (double)length(S.body_force()) * dt;  // :5524-5551
+    } else {
+        double maxsq = 0.0;
+        for (const V3 &v : S.vel) {
+            double d = dot(v, v);
+            if (d > maxsq) maxsq = d;
+        }
+        maxu = std::sqrt(maxsq);
+    }
+    maxu = std::fmax(0.0, maxu);
+    return S.cfl * S.dx / (maxu + 1e-6);
+}
+
+// FluidSimulation::update (fluidsimulation.cpp:5755-5846).
+static void update(Sim &S, double dt) {
+    if (!S.initialized) throw std::runtime_error("FluidSimulation must be initialized before update");
+    if (dt < 0.0) throw std::domain_error("delta time must be greater than or equal to 0");
+    double epsdt = 1e-6;
+    bool zero = dt < epsdt;
+    dt = std::max(dt, epsdt);
+    S.frameDt = dt; S.remaining = dt; S.stepNumber = 0;
+    bool skipped = zero && S.outputInitialized;
+    double substepTime = S.frameDt / (double)S.minSteps;
+    double eps = 1e-9;
+    do {
+        S.curStep = std::fmin(next_time_step(S, dt), S.remaining);
+        double done = S.frameDt - S.remaining;
+        double limit = (S.stepNumber + 1) * substepTime;
+        if (done + S.curStep > limit) S.curStep = std::fmin(substepTime, S.remaining);
+        if (S.stepNumber == S.maxSteps - 1) S.curStep = S.remaining;
+        S.remaining -= S.curStep;
+        if (!skipped) step_fluid(S, S.curStep);
+        S.stepNumber++;
+    } while (S.remaining > eps);
+    S.lastSubsteps = S.stepNumber;
+    S.outputInitialized = true;
+    S.currentFrame++;
+}
+
+static std::vector<V3> to_v3(const float *p, int n) {
+    std::vector<V3> v((size_t)n);
+    for (int i = 0; i < n; i++) v[i] = V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+    return v;
+}
+
+template <class T>
+static void copy_in(Grid<T> &g, const T *src) { std::memcpy(g.a.data(), src, sizeof(T) * g.a.size()); }
+template <class T>
+static void copy_out(const Grid<T> &g, T *dst) { std::memcpy(dst, g.a.data(), sizeof(T) * g.a.size()); }
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+void orc_domain_solid(int isize, int jsize, int ksize, double dx, float *phi_nodal, float *wU,
+                      float *wV, float *wW, float *wC) {
+    GridF phi;
+    domain_solid_phi(isize, jsize, ksize, dx, phi);
+    copy_out(phi, phi_nodal);
+    Weights w;
+    compute_weights(phi, isize, jsize, ksize, w);
+    if (wU) copy_out(w.U, wU);
+    if (wV) copy_out(w.V, wV);
+    if (wW) copy_out(w.W, wW);
+    if (wC) copy_out(w.C, wC);
+}
+
+void orc_liquid_sdf(int isize, int jsize, int ksize, double dx, const float *pos, int n,
+                    double radius, const float *solid_phi_nodal, float *phi_out) {
+    GridF phi;
+    liquid_sdf(isize, jsize, ksize, dx, to_v3(pos, n), radius, phi);
+    if (solid_phi_nodal) {
+        GridF solid(isize + 1, jsize + 1, ksize + 1, 0.0f);
+        copy_in(solid, solid_phi_nodal);
+        liquid_postprocess(phi, solid, dx);
+    }
+    copy_out(phi, phi_out);
+}
+
+void orc_p2g(int isize, int jsize, int ksize, double dx, const float *pos, const float *vel, int n,
+             double radius, float *u, float *v, float *w, uint8_t *vu, uint8_t *vv, uint8_t *vw) {
+    MAC m;
+    Valid va;
+    mac_init(m, isize, jsize, ksize);
+    valid_init(va, isize, jsize, ksize);
+    p2g(isize, jsize, ksize, dx, to_v3(pos, n), to_v3(vel, n), radius, m, va);
+    copy_out(m.U, u); copy_out(m.V, v); copy_out(m.W, w);
+    copy_out(va.U, vu); copy_out(va.V, vv); copy_out(va.W, vw);
+}
+
+void orc_extrapolate(int ni, int nj, int nk, float *grid, uint8_t *valid, int layers) {
+    GridF g(ni, nj, nk, 0.0f);
+    GridB v(ni, nj, nk, 0);
+    copy_in(g, grid);
+    copy_in(v, valid);
+    extrapolate(g, v, layers);
+    copy_out(g, grid);
+}
+
+int orc_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, double tol,
+                       int maxiter, int mode, const float *liquid_phi,
+                       const float *solid_phi_nodal, const float *wU, const float *wV,
+                       const float *wW, const float *wC, float *u, float *v, float *w,
+                       uint8_t *vu, uint8_t *vv, uint8_t *vw, double *rhs_out, double *soln_out,
+                       int *n_rows, int *iters, double *err) {
+    (void)solid_phi_nodal;
+    GridF liquid(isize, jsize, ksize, 0.0f);
+    copy_in(liquid, liquid_phi);
+    Weights wg;
+    wg.U = GridF(isize + 1, jsize, ksize, 0.0f); copy_in(wg.U, wU);
+    wg.V = GridF(isize, jsize + 1, ksize, 0.0f); copy_in(wg.V, wV);
+    wg.W = GridF(isize, jsize, ksize + 1, 0.0f); copy_in(wg.W, wW);
+    wg.C = GridF(isize, jsize, ksize, 0.0f); copy_in(wg.C, wC);
+    GridF sU(isize + 1, jsize, ksize, 0.0f), sV(isize, jsize + 1, ksize, 0.0f),
+        sW(isize, jsize, ksize + 1, 0.0f);
+    MAC m;
+    mac_init(m, isize, jsize, ksize);
+    copy_in(m.U, u); copy_in(m.V, v); copy_in(m.W, w);
+    Valid va;
+    valid_init(va, isize, jsize, ksize);
+    copy_in(va.U, vu); copy_in(va.V, vv); copy_in(va.W, vw);
+    PressureInputs in{isize, jsize, ksize, dx, dt, tol, maxiter, mode, &liquid, &wg, &sU, &sV, &sW,
+                      &m, &va};
+    PressureResult pr = pressure_solve(in);
+    if (n_rows) *n_rows = pr.n;
+    if (rhs_out) std::copy(pr.rhs.begin(), pr.rhs.end(), rhs_out);
+    if (soln_out) std::copy(pr.soln.begin(), pr.soln.end(), soln_out);
+    if (iters) *iters = pr.iters;
+    if (err) *err = pr.err;
+    copy_out(m.U, u); copy_out(m.V, v); copy_out(m.W, w);
+    copy_out(va.U, vu); copy_out(va.V, vv); copy_out(va.W, vw);
+    return pr.ok;
+}
+
+void orc_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, const float *liquid_phi,
+                    const float *wU, const float *wV, const float *wW, const float *wC,
+                    const double *r_in, double *invdiag_out, double *z_out) {
+    GridF liquid(isize, jsize, ksize, 0.0f);
+    copy_in(liquid, liquid_phi);
+    Weights wg;
+    wg.U = GridF(isize + 1, jsize, ksize, 0.0f); copy_in(wg.U, wU);
+    wg.V = GridF(isize, jsize + 1, ksize, 0.0f); copy_in(wg.V, wV);
+    wg.W = GridF(isize, jsize, ksize + 1, 0.0f); copy_in(wg.W, wW);
+    wg.C = GridF(isize, jsize, ksize, 0.0f); copy_in(wg.C, wC);
+    PressureInputs in{isize, jsize, ksize, dx, dt, 1e-9, 1000, 0, &liquid, &wg, nullptr, nullptr,
+                      nullptr, nullptr, nullptr};
+    Rows R;
+    build_rows(in, R);
+    assemble_matrix(in, R);
+    MIC M;
+    mic0_factor(R, M);
+    std::vector<double> r(r_in, r_in + R.cell.size()), z;
+    mic0_apply(R, M, r, z);
+    std::copy(M.invdiag.begin(), M.invdiag.end(), invdiag_out);
+    std::copy(z.begin(), z.end(), z_out);
+}
+
+void orc_advect_particles(int isize, int jsize, int ksize, double dx, const float *u,
+                          const float *v, const float *w, const float *pos, int n, double dt,
+                          int order, float *out) {
+    MAC m;
+    mac_init(m, isize, jsize, ksize);
+    copy_in(m.U, u); copy_in(m.V, v); copy_in(m.W, w);
+    // ParticleAdvector::_RK1.._RK4 (particleadvector.cpp:412-445).
+    for (int q = 0; q < n; q++) {
+        V3 p0(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2]), p1;
+        auto vel = [&](V3 p) { return mac_velocity(m, p, dx, isize, jsize, ksize); };
+        if (order == 4) {
+            V3 k1 = vel(p0), k2 = vel(p0 + (float)(0.5 * dt) * k1),
+               k3 = vel(p0 + (float)(0.5 * dt) * k2), k4 = vel(p0 + (float)dt * k3);
+            p1 = p0 + (float)(dt / 6.0f) * (k1 + 2.0f * k2 + 2.0f * k3 + k4);
+        } else if (order == 3) {
+            p1 = rk3(m, p0, dt, dx, isize, jsize, ksize);
+        } else if (order == 2) {
+            V3 k1 = vel(p0), k2 = vel(p0 + (float)(0.5 * dt) * k1);
+            p1 = p0 + (float)dt * k2;
+        } else {
+            V3 k1 = vel(p0);
+            p1 = p0 + (float)dt * k1;
+        }
+        out[3 * q] = p1.x; out[3 * q + 1] = p1.y; out[3 * q + 2] = p1.z;
+    }
+}
+
+void orc_trilinear_velocity(int isize, int jsize, int ksize, double dx, const float *u,
+                            const float *v, const float *w, const float *pos, int n, float *out) {
+    MAC m;
+    mac_init(m, isize, jsize, ksize);
+    copy_in(m.U, u); copy_in(m.V, v); copy_in(m.W, w);
+    for (int q = 0; q < n; q++) {
+        V3 r = mac_velocity(m, V3(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2]), dx, isize, jsize, ksize);
+        // ParticleAdvector::_validateOutput (particleadvector.cpp:506-516).
+        if (std::isinf(r.x) || std::isnan(r.x) || std::isinf(r.y) || std::isnan(r.y) ||
+            std::isinf(r.z) || std::isnan(r.z))
+            r = V3();
+        out[3 * q] = r.x; out[3 * q + 1] = r.y; out[3 * q + 2] = r.z;
+    }
+}
+
+void *orc_sim_create(int isize, int jsize, int ksize, double dx) {
+    Sim *S = new Sim();
+    S->I = isize; S->J = jsize; S->K = ksize; S->dx = dx;
+    return S;
+}
+void orc_sim_destroy(void *h) { delete (Sim *)h; }
+void orc_sim_add_body_force(void *h, double fx, double fy, double fz) {
+    ((Sim *)h)->bodyForces.push_back(V3((float)fx, (float)fy, (float)fz));
+}
+void orc_sim_set_cfl(void *h, double cfl) { ((Sim *)h)->cfl = cfl; }
+void orc_sim_set_substeps(void *h, int mn, int mx) { ((Sim *)h)->minSteps = mn; ((Sim *)h)->maxSteps = mx; }
+void orc_sim_set_pic_ratio(void *h, double r) { ((Sim *)h)->picRatio = r; }
+void orc_sim_set_pcg(void *h, int mode, double tol, int maxiter) {
+    Sim *S = (Sim *)h;
+    S->pcgMode = mode; S->pcgTol = tol; S->pcgMaxIter = maxiter;
+}
+// FluidSimulation::loadMarkerParticleData + _loadMarkerParticles
+// (fluidsimulation.cpp:2488-2509, 2773-2785): keep points inside the domain.
+void orc_sim_load_particles(void *h, const float *pos, const float *vel, int n) {
+    Sim *S = (Sim *)h;
+    Box b;
+    b.pos = V3(0, 0, 0);
+    b.w = S->I * S->dx; b.h = S->J * S->dx; b.d = S->K * S->dx;
+    for (int i = 0; i < n; i++) {
+        V3 p(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        if (!b.inside(p)) continue;
+        S->pos.push_back(p);
+        S->vel.push_back(V3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]));
+    }
+}
+int orc_sim_initialize(void *h) {
+    Sim *S = (Sim *)h;
+    // _initializeParticleRadii (fluidsimulation.cpp:2644-2650).
+    S->radius = 0.5 * 1.0 * S->dx * std::sqrt(3.0);
+    S->initialized = true;
+    return 0;
+}
+int orc_sim_update(void *h, double dt) {
+    try {
+        update(*(Sim *)h, dt);
+    } catch (std::exception &) {
+        return -1;
+    }
+    return 0;
+}
+int orc_sim_num_particles(void *h) { return (int)((Sim *)h)->pos.size(); }
+void orc_sim_get_particles(void *h, float *pos, float *vel) {
+    Sim *S = (Sim *)h;
+    for (size_t i = 0; i < S->pos.size(); i++) {
+        pos[3 * i] = S->pos[i].x; pos[3 * i + 1] = S->pos[i].y; pos[3 * i + 2] = S->pos[i].z;
+        vel[3 * i] = S->vel[i].x; vel[3 * i + 1] = S->vel[i].y; vel[3 * i + 2] = S->vel[i].z;
+    }
+}
+void orc_sim_get_float_grid(void *h, int which, float *out) {
+    Sim *S = (Sim *)h;
+    switch (which) {
+        case 0: copy_out(S->liquid, out); break;
+        case 1: copy_out(S->solid, out); break;
+        case 2: copy_out(S->mac.U, out); break;
+        case 3: copy_out(S->mac.V, out); break;
+        case 4: copy_out(S->mac.W, out); break;
+        case 5: copy_out(S->w.U, out); break;
+        case 6: copy_out(S->w.V, out); break;
+        case 7: copy_out(S->w.W, out); break;
+        case 8: copy_out(S->w.C, out); break;
+        default: break;
+    }
+}
+void orc_sim_get_bool_grid(void *h, int which, uint8_t *out) {
+    Sim *S = (Sim *)h;
+    switch (which) {
+        case 0: copy_out(S->valid.U, out); break;
+        case 1: copy_out(S->valid.V, out); break;
+        case 2: copy_out(S->valid.W, out); break;
+        case 3: copy_out(S->nearSolid, out); break;
+        default: break;
+    }
+}
+int orc_sim_frame_substeps(void *h) { return ((Sim *)h)->lastSubsteps; }
+void orc_sim_pressure_info(void *h, int *iters, double *err, int *ok) {
+    Sim *S = (Sim *)h;
+    if (iters) *iters = S->lastIters;
+    if (err) *err = S->lastErr;
+    if (ok) *ok = S->lastOk;
+}
+
+}  // extern "C"
