@@ -1,0 +1,374 @@
+"""flipengine3d_amd -- MI355X-native FLIP fluid step.
+
+The compute path is the HIP library ``libflipmi.so`` (built in-tree by
+``flipengine3d_amd/build.py``) behind the C ABI in ``include/flipmi.h``.  This
+module is a thin ctypes binding used by the tests and the benchmark; it
+mirrors the reference engine's interface names (FluidSimulation,
+PressureSolver, VelocityAdvector, ParticleAdvector) and raises the
+reference's exception types on the same errors.  There is no CPU fallback:
+loading fails loudly when the library is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import scenes  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflipmi.so")
+
+FLIPMI_OK = 0
+FLIPMI_ERR_INVALID_ARG = -1
+FLIPMI_ERR_NOT_INITIALIZED = -2
+FLIPMI_ERR_OOM = -3
+FLIPMI_ERR_NOT_CONVERGED = -4
+FLIPMI_ERR_HIP = -5
+FLIPMI_ERR_BUDGET = -6
+FLIPMI_ERR_TIMEOUT = -7
+
+PCG_FP64 = 0
+PCG_MIXED = 1
+
+STAGES = ["sort", "classify", "p2g", "extrapolate", "assemble", "mic0", "pcg", "apply",
+          "g2p_advect", "remove", "glue", "pcg_iters"]
+
+_c_int = ctypes.c_int
+_c_i64 = ctypes.c_int64
+_c_double = ctypes.c_double
+_vp = ctypes.c_void_p
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("isize", _c_int), ("jsize", _c_int), ("ksize", _c_int), ("dx", _c_double),
+                ("body_force", _c_double * 3), ("cfl", _c_double), ("min_substeps", _c_int),
+                ("max_substeps", _c_int), ("pic_ratio", _c_double), ("pcg_tolerance", _c_double),
+                ("pcg_max_iterations", _c_int), ("pcg_precision", _c_int),
+                ("max_particles", _c_i64)]
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [("substeps", _c_int), ("num_particles", _c_i64), ("pcg_iterations", _c_int),
+                ("pcg_residual", _c_double), ("pressure_ok", _c_int), ("gpu_ms", _c_double)]
+
+
+class StageTimes(ctypes.Structure):
+    _fields_ = [("ms", _c_double * 16), ("calls", _c_i64 * 16)]
+
+
+class PressureArgs(ctypes.Structure):
+    _fields_ = [("isize", _c_int), ("jsize", _c_int), ("ksize", _c_int), ("dx", _c_double),
+                ("dt", _c_double), ("tolerance", _c_double), ("acceptable_tolerance", _c_double),
+                ("max_iterations", _c_int), ("precision", _c_int), ("liquid_phi", _vp),
+                ("weight_u", _vp), ("weight_v", _vp), ("weight_w", _vp), ("weight_c", _vp),
+                ("solid_u", _vp), ("solid_v", _vp), ("solid_w", _vp), ("u", _vp), ("v", _vp),
+                ("w", _vp), ("valid_u", _vp), ("valid_v", _vp), ("valid_w", _vp),
+                ("pressure_rows", _vp)]
+
+
+class PressureReport(ctypes.Structure):
+    _fields_ = [("iterations", _c_int), ("residual", _c_double), ("rows", _c_int),
+                ("success", _c_int), ("refinements", _c_int)]
+
+
+EXPORTED = [
+    "flipmi_ctx_create", "flipmi_ctx_destroy", "flipmi_last_error", "flipmi_scene_desc_default",
+    "flipmi_scene_create", "flipmi_scene_destroy", "flipmi_scene_load_particles",
+    "flipmi_scene_initialize", "flipmi_scene_update", "flipmi_scene_num_particles",
+    "flipmi_scene_get_particles", "flipmi_scene_get_grid", "flipmi_scene_stage_times",
+    "flipmi_scene_set_profiling", "flipmi_domain_solid", "flipmi_liquid_sdf", "flipmi_p2g",
+    "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
+    "flipmi_trilinear_velocity",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads libflipmi.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run flipengine3d_amd/build.py (no CPU fallback exists)")
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    L.flipmi_ctx_create.argtypes = [_c_int, P(_vp)]
+    L.flipmi_ctx_destroy.argtypes = [_vp]
+    L.flipmi_last_error.argtypes = [_vp]
+    L.flipmi_last_error.restype = ctypes.c_char_p
+    L.flipmi_scene_desc_default.argtypes = [P(SceneDesc)]
+    L.flipmi_scene_create.argtypes = [_vp, P(SceneDesc), P(_vp)]
+    L.flipmi_scene_destroy.argtypes = [_vp]
+    L.flipmi_scene_load_particles.argtypes = [_vp, _vp, _vp, _c_i64]
+    L.flipmi_scene_initialize.argtypes = [_vp]
+    L.flipmi_scene_update.argtypes = [_vp, _c_double, P(FrameStats)]
+    L.flipmi_scene_num_particles.argtypes = [_vp, P(_c_i64)]
+    L.flipmi_scene_get_particles.argtypes = [_vp, _vp, _vp]
+    L.flipmi_scene_get_grid.argtypes = [_vp, _c_int, _vp]
+    L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
+    L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
+    L.flipmi_domain_solid.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double] + [_vp] * 5
+    L.flipmi_liquid_sdf.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_double, _vp, _vp]
+    L.flipmi_p2g.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _c_i64, _c_double] + [_vp] * 6
+    L.flipmi_extrapolate.argtypes = [_vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int]
+    L.flipmi_pressure_solve.argtypes = [_vp, P(PressureArgs), P(PressureReport)]
+    L.flipmi_mic0_apply.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _c_double] + [_vp] * 8
+    L.flipmi_advect_particles.argtypes = ([_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _vp, _vp, _c_i64,
+                                          _c_double, _c_int, _vp])
+    L.flipmi_trilinear_velocity.argtypes = ([_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _vp, _vp,
+                                            _c_i64, _vp])
+    _lib = L
+    return L
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class FlipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"flipmi error {code}: {msg}")
+        self.code = code
+
+
+class Context:
+    """One device context (flipmi_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = _vp()
+        rc = self.L.flipmi_ctx_create(device, ctypes.byref(h))
+        if rc != FLIPMI_OK:
+            raise FlipError(rc, "flipmi_ctx_create failed (no usable HIP device?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.flipmi_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self) -> str:
+        m = self.L.flipmi_last_error(self.h)
+        return m.decode() if m else ""
+
+    def check(self, rc: int):
+        if rc == FLIPMI_OK:
+            return
+        msg = self.last_error()
+        if rc == FLIPMI_ERR_NOT_INITIALIZED:
+            raise RuntimeError(msg)
+        if rc == FLIPMI_ERR_INVALID_ARG:
+            raise ValueError(msg)
+        raise FlipError(rc, msg)
+
+    # ---- stage drop-ins --------------------------------------------------
+    def domain_solid(self, isize, jsize, ksize, dx):
+        phi = np.zeros((ksize + 1, jsize + 1, isize + 1), np.float32)
+        wu = np.zeros((ksize, jsize, isize + 1), np.float32)
+        wv = np.zeros((ksize, jsize + 1, isize), np.float32)
+        ww = np.zeros((ksize + 1, jsize, isize), np.float32)
+        wc = np.zeros((ksize, jsize, isize), np.float32)
+        self.check(self.L.flipmi_domain_solid(self.h, isize, jsize, ksize, dx, _ptr(phi), _ptr(wu), _ptr(wv),
+                                              _ptr(ww), _ptr(wc)))
+        return phi, (wu, wv, ww, wc)
+
+    def liquid_sdf(self, isize, jsize, ksize, dx, pos, radius, solid_phi=None):
+        pos = np.ascontiguousarray(pos, np.float32)
+        sp = None if solid_phi is None else np.ascontiguousarray(solid_phi, np.float32)
+        phi = np.zeros((ksize, jsize, isize), np.float32)
+        self.check(self.L.flipmi_liquid_sdf(self.h, isize, jsize, ksize, dx, _ptr(pos), pos.shape[0], radius,
+                                            _ptr(sp), _ptr(phi)))
+        return phi
+
+    def p2g(self, isize, jsize, ksize, dx, pos, vel, radius):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        u = np.zeros((ksize, jsize, isize + 1), np.float32)
+        v = np.zeros((ksize, jsize + 1, isize), np.float32)
+        w = np.zeros((ksize + 1, jsize, isize), np.float32)
+        vu, vv, vw = (np.zeros(a.shape, np.uint8) for a in (u, v, w))
+        self.check(self.L.flipmi_p2g(self.h, isize, jsize, ksize, dx, _ptr(pos), _ptr(vel), pos.shape[0], radius,
+                                     _ptr(u), _ptr(v), _ptr(w), _ptr(vu), _ptr(vv), _ptr(vw)))
+        return (u, v, w), (vu, vv, vw)
+
+    def extrapolate(self, grid, valid, layers):
+        g = np.ascontiguousarray(grid, np.float32).copy()
+        vd = np.ascontiguousarray(valid, np.uint8)
+        nk, nj, ni = g.shape
+        self.check(self.L.flipmi_extrapolate(self.h, ni, nj, nk, _ptr(g), _ptr(vd), layers))
+        return g
+
+    def pressure_solve(self, isize, jsize, ksize, dx, dt, liquid_phi, weights, mac, valid, tol=1e-9,
+                       maxiter=1000, precision=PCG_FP64, acceptable=1.0):
+        lp = np.ascontiguousarray(liquid_phi, np.float32)
+        wu, wv, ww, wc = [np.ascontiguousarray(a, np.float32) for a in weights]
+        u, v, w = [np.ascontiguousarray(a, np.float32).copy() for a in mac]
+        vu, vv, vw = [np.ascontiguousarray(a, np.uint8).copy() for a in valid]
+        rows = np.zeros(isize * jsize * ksize, np.float64)
+        a = PressureArgs(isize, jsize, ksize, dx, dt, tol, acceptable, maxiter, precision, _ptr(lp), _ptr(wu),
+                         _ptr(wv), _ptr(ww), _ptr(wc), None, None, None, _ptr(u), _ptr(v), _ptr(w), _ptr(vu),
+                         _ptr(vv), _ptr(vw), _ptr(rows))
+        rep = PressureReport()
+        rc = self.L.flipmi_pressure_solve(self.h, ctypes.byref(a), ctypes.byref(rep))
+        if rc not in (FLIPMI_OK, FLIPMI_ERR_NOT_CONVERGED):
+            self.check(rc)
+        return dict(ok=int(rc == FLIPMI_OK), iters=rep.iterations, err=rep.residual, rows=rep.rows,
+                    soln=rows[:rep.rows], mac=(u, v, w), valid=(vu, vv, vw), refinements=rep.refinements)
+
+    def mic0_apply(self, isize, jsize, ksize, dx, dt, liquid_phi, weights, r):
+        lp = np.ascontiguousarray(liquid_phi, np.float32)
+        wu, wv, ww, wc = [np.ascontiguousarray(a, np.float32) for a in weights]
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        inv = np.zeros_like(r)
+        self.check(self.L.flipmi_mic0_apply(self.h, isize, jsize, ksize, dx, dt, _ptr(lp), _ptr(wu), _ptr(wv),
+                                            _ptr(ww), _ptr(wc), _ptr(r), _ptr(z), _ptr(inv)))
+        return inv, z
+
+    def advect_particles(self, isize, jsize, ksize, dx, mac, pos, dt, order):
+        u, v, w = [np.ascontiguousarray(a, np.float32) for a in mac]
+        pos = np.ascontiguousarray(pos, np.float32)
+        out = np.zeros_like(pos)
+        self.check(self.L.flipmi_advect_particles(self.h, isize, jsize, ksize, dx, _ptr(u), _ptr(v), _ptr(w),
+                                                  _ptr(pos), pos.shape[0], dt, order, _ptr(out)))
+        return out
+
+    def trilinear_velocity(self, isize, jsize, ksize, dx, mac, pos):
+        u, v, w = [np.ascontiguousarray(a, np.float32) for a in mac]
+        pos = np.ascontiguousarray(pos, np.float32)
+        out = np.zeros_like(pos)
+        self.check(self.L.flipmi_trilinear_velocity(self.h, isize, jsize, ksize, dx, _ptr(u), _ptr(v), _ptr(w),
+                                                    _ptr(pos), pos.shape[0], _ptr(out)))
+        return out
+
+
+class FluidSimulation:
+    """Device-resident scene with the reference's FluidSimulation interface
+    (fluidsimulation.h:127-153): construct, add forces / load particles,
+    initialize(), then update(dt) per frame."""
+
+    _GRID_SHAPES = {
+        0: lambda I, J, K: ((K, J, I), np.float32),
+        1: lambda I, J, K: ((K + 1, J + 1, I + 1), np.float32),
+        2: lambda I, J, K: ((K, J, I + 1), np.float32),
+        3: lambda I, J, K: ((K, J + 1, I), np.float32),
+        4: lambda I, J, K: ((K + 1, J, I), np.float32),
+        5: lambda I, J, K: ((K, J, I + 1), np.float32),
+        6: lambda I, J, K: ((K, J + 1, I), np.float32),
+        7: lambda I, J, K: ((K + 1, J, I), np.float32),
+        8: lambda I, J, K: ((K, J, I), np.float32),
+        9: lambda I, J, K: ((K, J, I + 1), np.uint8),
+        10: lambda I, J, K: ((K, J + 1, I), np.uint8),
+        11: lambda I, J, K: ((K + 1, J, I), np.uint8),
+    }
+
+    def __init__(self, isize: int, jsize: int, ksize: int, dx: float, ctx: Optional[Context] = None,
+                 precision: int = PCG_FP64, max_particles: int = 0):
+        self.ctx = ctx or Context()
+        self.L = self.ctx.L
+        self.dims = (isize, jsize, ksize)
+        self.dx = dx
+        d = SceneDesc()
+        self.L.flipmi_scene_desc_default(ctypes.byref(d))
+        d.isize, d.jsize, d.ksize, d.dx = isize, jsize, ksize, dx
+        d.pcg_precision = precision
+        d.max_particles = max_particles
+        self.desc = d
+        self.h = None
+        self.last_stats = FrameStats()
+
+    def _ensure(self):
+        if self.h is None:
+            h = _vp()
+            self.ctx.check(self.L.flipmi_scene_create(self.ctx.h, ctypes.byref(self.desc), ctypes.byref(h)))
+            self.h = h
+
+    def addBodyForce(self, fx: float, fy: float, fz: float):
+        if self.h is not None:
+            raise RuntimeError("body forces must be set before particles are loaded")
+        self.desc.body_force[0] += fx
+        self.desc.body_force[1] += fy
+        self.desc.body_force[2] += fz
+
+    def setCFLConditionNumber(self, c: float):
+        self.desc.cfl = c
+
+    def setMinTimeStepsPerFrame(self, n: int):
+        self.desc.min_substeps = n
+
+    def setMaxTimeStepsPerFrame(self, n: int):
+        self.desc.max_substeps = n
+
+    def setPICFLIPRatio(self, r: float):
+        self.desc.pic_ratio = r
+
+    def setPressureSolverTolerance(self, tol: float, max_iterations: int = 1000):
+        self.desc.pcg_tolerance = tol
+        self.desc.pcg_max_iterations = max_iterations
+
+    def loadMarkerParticleData(self, positions: np.ndarray, velocities: Optional[np.ndarray] = None):
+        self._ensure()
+        pos = np.ascontiguousarray(positions, np.float32)
+        vel = None if velocities is None else np.ascontiguousarray(velocities, np.float32)
+        self.ctx.check(self.L.flipmi_scene_load_particles(self.h, _ptr(pos), _ptr(vel), pos.shape[0]))
+
+    def initialize(self):
+        self._ensure()
+        self.ctx.check(self.L.flipmi_scene_initialize(self.h))
+
+    def update(self, dt: float) -> FrameStats:
+        if self.h is None:
+            raise RuntimeError("Error: FluidSimulation must be initialized before update.")
+        st = FrameStats()
+        self.ctx.check(self.L.flipmi_scene_update(self.h, dt, ctypes.byref(st)))
+        self.last_stats = st
+        return st
+
+    def set_profiling(self, on: bool = True):
+        self._ensure()
+        self.ctx.check(self.L.flipmi_scene_set_profiling(self.h, 1 if on else 0))
+
+    def stage_times(self, reset: bool = False) -> dict:
+        t = StageTimes()
+        self.ctx.check(self.L.flipmi_scene_stage_times(self.h, ctypes.byref(t), 1 if reset else 0))
+        return {name: (t.ms[q], t.calls[q]) for q, name in enumerate(STAGES)}
+
+    def getNumMarkerParticles(self) -> int:
+        n = _c_i64()
+        self.ctx.check(self.L.flipmi_scene_num_particles(self.h, ctypes.byref(n)))
+        return n.value
+
+    def particles(self) -> Tuple[np.ndarray, np.ndarray]:
+        n = self.getNumMarkerParticles()
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        self.ctx.check(self.L.flipmi_scene_get_particles(self.h, _ptr(pos), _ptr(vel)))
+        return pos, vel
+
+    def grid(self, which: int) -> np.ndarray:
+        shape, dt = self._GRID_SHAPES[which](*self.dims)
+        out = np.zeros(shape, dt)
+        self.ctx.check(self.L.flipmi_scene_get_grid(self.h, which, _ptr(out)))
+        return out
+
+    def close(self):
+        if self.h is not None:
+            self.L.flipmi_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

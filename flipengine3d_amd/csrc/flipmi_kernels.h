@@ -1,0 +1,159 @@
+// Kernel launch interface between the host orchestration (flipmi_scene.cpp)
+// and the HIP kernels (k_*.hip).  Every launcher enqueues on the given
+// stream and returns the hipError_t of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flipmi {
+
+// One triangle of the domain box with its exact-band node range
+// (MeshLevelSet::_initializeTriangleData, meshlevelset.cpp:874-915).
+struct BoxTri {
+    float v[3][3];
+    int gmin[3], gmax[3];
+};
+
+struct BoxMesh {
+    BoxTri tri[12];
+    int ntri;
+    float bpos[3];   // boundary AABB min corner (float)
+    double bext[3];  // boundary AABB extents (double)
+};
+
+// Grid geometry shared by all kernels.
+struct GridGeom {
+    int I, J, K;
+    double dx, invdx;
+};
+
+// ---- setup (k_setup.hip) ----
+hipError_t launch_domain_solid(const GridGeom &g, const BoxMesh &mesh, float *phi_nodal,
+                               hipStream_t s);
+hipError_t launch_weights(const GridGeom &g, const float *phi_nodal, float *wU, float *wV,
+                          float *wW, float *wC, hipStream_t s);
+hipError_t launch_near_solid(const GridGeom &g, const float *phi_nodal, uint8_t *near_a,
+                             uint8_t *near_b, int ci, int cj, int ck, int feather_layers,
+                             uint8_t **result, hipStream_t s);
+
+// ---- sort (k_sort.hip) ----
+struct SortBuffers {
+    int *cell;       // [n] cell id per particle (-1: outside the grid)
+    int *slot;       // [n] arrival slot within the cell
+    int *count;      // [ncell]
+    int *offset;     // [ncell + 1]
+    int *order;      // [n] particle ids grouped by cell, ascending within a cell
+    int *scan_tmp;   // scratch for the scan
+    size_t scan_tmp_elems;
+};
+size_t scan_scratch_elems(int n);
+hipError_t launch_exclusive_scan(const int *in, int *out, int n, int *tmp, hipStream_t s);
+hipError_t launch_sort_by_cell(const GridGeom &g, const float *px, const float *py,
+                               const float *pz, int n, SortBuffers &sb, hipStream_t s);
+// Gathers the first *nsorted (= offset[ncell]) entries of the cell order.
+hipError_t launch_gather_sorted(const int *order, const int *nsorted, int n,
+                                const float *const src[6], float *const dst[6], hipStream_t s);
+
+// ---- classification (k_classify.hip) ----
+hipError_t launch_liquid_sdf(const GridGeom &g, const float *sx, const float *sy, const float *sz,
+                             const int *offset, double radius, const float *solid_phi_nodal,
+                             float *phi, hipStream_t s);
+
+// ---- P2G (k_p2g.hip) ----
+hipError_t launch_p2g(const GridGeom &g, int comp, const float *sx, const float *sy,
+                      const float *sz, const float *svel, const int *offset, double radius,
+                      float *out, uint8_t *valid, hipStream_t s);
+
+// ---- grid passes (k_grid.hip) ----
+hipError_t launch_extrapolate(int ni, int nj, int nk, float *grid, const uint8_t *valid,
+                              uint8_t *status, int layers, hipStream_t s);
+hipError_t launch_save_and_body_force(size_t n, const float *mac, float *saved, float add,
+                                      int do_add, hipStream_t s);
+hipError_t launch_constrain(size_t n, float *mac, const float *weight, const float *solid_vel,
+                            hipStream_t s);
+hipError_t launch_fill_u8(uint8_t *p, size_t n, uint8_t v, hipStream_t s);
+hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t s);
+hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
+hipError_t launch_fill_i32(int *p, size_t n, int v, hipStream_t s);
+
+// ---- pressure (k_pressure.hip) ----
+struct PcgScalars {
+    double rho, alpha, beta, resid, tol, bnorm, sz;
+    float alpha_f, beta_f;
+    int iters, done, converged, failed, abort, maxiter, precision;
+    unsigned int counter[4];
+    unsigned long long absmax_bits;
+};
+
+struct PressureBuffers {
+    double *diag, *offx, *offy, *offz;  // matrix, dense cell layout
+    double *rhs;
+    double *x, *r, *z, *s;              // PCG vectors (dense, FP64)
+    double *invdiag, *Lx, *Ly, *Lz;     // MIC(0) factor
+    double *partials;                   // per-block partial sums
+    int npartials;
+    unsigned int *progress;             // wavefront tile progress counters
+    int ntiles_j, ntiles_k;
+    unsigned int *ticket;
+    PcgScalars *sc;                     // device scalars
+};
+
+hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *liquid,
+                                    const float *wU, const float *wV, const float *wW,
+                                    const float *wC, const float *u, const float *v,
+                                    const float *w, const float *sU, const float *sV,
+                                    const float *sW, PressureBuffers &pb, hipStream_t s);
+hipError_t launch_pressure_reset(PressureBuffers &pb, hipStream_t s);
+hipError_t launch_mic0_factor(const GridGeom &g, PressureBuffers &pb, hipStream_t s);
+hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, PressureBuffers &pb,
+                           hipStream_t s);
+hipError_t launch_pcg_iteration(const GridGeom &g, PressureBuffers &pb, hipStream_t s);
+hipError_t launch_pressure_apply(const GridGeom &g, double dt, const float *liquid,
+                                 const float *wU, const float *wV, const float *wW,
+                                 const double *p, float *u, float *v, float *w, uint8_t *vu,
+                                 uint8_t *vv, uint8_t *vw, hipStream_t s);
+hipError_t launch_precond_apply(const GridGeom &g, PressureBuffers &pb, const double *r,
+                                double *z, hipStream_t s);
+hipError_t launch_spmv(const GridGeom &g, PressureBuffers &pb, const double *x, double *y,
+                       hipStream_t s);
+
+// ---- particles (k_particles.hip) ----
+struct ParticleStepArgs {
+    float *px, *py, *pz, *vx, *vy, *vz;  // in/out, canonical order
+    int n;
+    const float *u, *v, *w;              // projected MAC
+    const float *su, *sv, *sw;           // saved (pre-force) MAC
+    const float *solid;                  // nodal solid phi
+    const uint8_t *near_solid;
+    int nci, ncj, nck;
+    double near_cell;                    // near-solid coarse cell size
+    float pic_ratio_f, flip_ratio_f;
+    double dt;
+    double cfl;
+    float bpos[3];                       // collision boundary (already shrunk)
+    double bext[3];
+    uint8_t *solid_removed;              // [n] out
+    int *speed_hist;                     // [maxsteps] out (accumulated)
+    double speed_step;
+    int maxsteps;
+};
+hipError_t launch_particle_step(const GridGeom &g, const ParticleStepArgs &a, hipStream_t s);
+hipError_t launch_max_speed(const float *vx, const float *vy, const float *vz, int n,
+                            unsigned int *out_bits, hipStream_t s);
+hipError_t launch_cell_rank_cells(const int *offset, int ncell, const int *order,
+                                  const uint8_t *solid_removed, int *rank, hipStream_t s);
+hipError_t launch_keep_flags(const float *vx, const float *vy, const float *vz, int n,
+                             const uint8_t *solid_removed, const int *rank, double maxspeedsq,
+                             int *keep, hipStream_t s);
+hipError_t launch_compact_keep(const int *keep, int n, int *pos, int *scan_tmp,
+                               const float *const src[6], float *const dst[6], int *new_count,
+                               hipStream_t s);
+hipError_t launch_advect_rk(const GridGeom &g, const float *u, const float *v, const float *w,
+                            const float *pin, int n, double dt, int order, float *pout,
+                            hipStream_t s);
+hipError_t launch_trilinear_velocity(const GridGeom &g, const float *u, const float *v,
+                                     const float *w, const float *pin, int n, float *out,
+                                     hipStream_t s);
+
+}  // namespace flipmi

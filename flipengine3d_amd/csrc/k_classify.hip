@@ -1,0 +1,141 @@
+// Marker/cell classification: the liquid signed distance field and the
+// fluid/air labels it defines (phi < 0 is fluid).
+//
+// Reference: ParticleLevelSet::calculateSignedDistanceField and
+// postProcessSignedDistanceField (particlelevelset.cpp:161-196, 299-630).
+// The reference bins particles into 10^3 cell blocks and takes, per block,
+//     phi(c) = min(3 dx, min_p |c_local - p_local| - r)
+// in block-local float coordinates.  Here one workgroup owns one 10^3 block,
+// gathers the particles of the cells around it from the cell-sorted arrays,
+// replays the reference's particle-to-block assignment and local arithmetic,
+// and folds distances into LDS with an order-independent atomic minimum.
+// The result is bit-identical to the reference.
+#include "device_math.h"
+#include "flipmi_kernels.h"
+
+namespace flipmi {
+
+constexpr int kBW = 10;            // particlelevelset.h:140 (_blockwidth)
+constexpr int kHalo = 2;           // search radius 2r = sqrt(3) dx < 2 dx
+constexpr int kSpan = kBW + 2 * kHalo;
+constexpr int kSdfThreads = 256;
+
+// Whether the reference assigns point p to block (bi, bj, bk)
+// (particlelevelset.cpp:497-536; velocityadvector.cpp:280-320).
+__device__ __forceinline__ bool point_in_block(F3 p, float sr, float blockdx, double inv_blockdx,
+                                               int bi, int bj, int bk) {
+    I3 b = pos_to_index(p, inv_blockdx);
+    F3 bp = idx_to_pos(b.i, b.j, b.k, (double)blockdx);
+    if (p.x - sr > bp.x && p.y - sr > bp.y && p.z - sr > bp.z && p.x + sr < bp.x + blockdx &&
+        p.y + sr < bp.y + blockdx && p.z + sr < bp.z + blockdx) {
+        return b.i == bi && b.j == bj && b.k == bk;
+    }
+    I3 lo = pos_to_index((double)(p.x - sr), (double)(p.y - sr), (double)(p.z - sr), inv_blockdx);
+    I3 hi = pos_to_index((double)(p.x + sr), (double)(p.y + sr), (double)(p.z + sr), inv_blockdx);
+    return bi >= lo.i && bi <= hi.i && bj >= lo.j && bj <= hi.j && bk >= lo.k && bk <= hi.k;
+}
+
+__global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
+    GridGeom g, const float *__restrict__ sx, const float *__restrict__ sy,
+    const float *__restrict__ sz, const int *__restrict__ offset, float radius,
+    const float *__restrict__ solid, float *phi_out) {
+    __shared__ uint32_t phi_ord[kBW * kBW * kBW];
+    __shared__ int run_begin[kSpan * kSpan];
+    __shared__ int run_prefix[kSpan * kSpan + 1];
+
+    int nbi = (g.I + kBW - 1) / kBW, nbj = (g.J + kBW - 1) / kBW;
+    int bid = blockIdx.x;
+    int bi = bid % nbi, bj = (bid / nbi) % nbj, bk = bid / (nbi * nbj);
+    float maxd = (float)(3.0 * g.dx);  // particlelevelset.cpp:295-297
+    uint32_t init = f2ord(maxd);
+    for (int c = threadIdx.x; c < kBW * kBW * kBW; c += kSdfThreads) phi_ord[c] = init;
+
+    // Candidate particles: cells [b0 - 2, b0 + 12) per axis, one contiguous
+    // run of the cell-sorted arrays per (j, k) row.
+    int i0 = max(bi * kBW - kHalo, 0), i1 = min(bi * kBW + kBW + kHalo, g.I);
+    int j0 = max(bj * kBW - kHalo, 0), j1 = min(bj * kBW + kBW + kHalo, g.J);
+    int k0 = max(bk * kBW - kHalo, 0), k1 = min(bk * kBW + kBW + kHalo, g.K);
+    int nj = j1 - j0, nk = k1 - k0, nrows = nj * nk;
+    for (int r = threadIdx.x; r < nrows; r += kSdfThreads) {
+        int j = j0 + r % nj, k = k0 + r / nj;
+        size_t c0 = (size_t)i0 + (size_t)g.I * ((size_t)j + (size_t)g.J * k);
+        size_t c1 = c0 + (size_t)(i1 - i0);
+        run_begin[r] = offset[c0];
+        run_prefix[r + 1] = offset[c1] - offset[c0];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        run_prefix[0] = 0;
+        for (int r = 0; r < nrows; r++) run_prefix[r + 1] += run_prefix[r];
+    }
+    __syncthreads();
+    int total = run_prefix[nrows];
+
+    float r = radius;
+    float sr = 2.0f * r;  // particlelevelset.h:142 (_searchRadiusFactor)
+    float blockdx = (float)(kBW * g.dx);
+    double inv_blockdx = 1.0 / (double)blockdx;
+    F3 boff = idx_to_pos(bi, bj, bk, kBW * g.dx);
+    for (int t = threadIdx.x; t < total; t += kSdfThreads) {
+        int lo = 0, hi = nrows - 1;
+        while (lo < hi) {  // last row whose prefix <= t
+            int mid = (lo + hi + 1) >> 1;
+            if (run_prefix[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        int q = run_begin[lo] + (t - run_prefix[lo]);
+        F3 p0 = f3(sx[q], sy[q], sz[q]);
+        if (!point_in_block(p0, sr, blockdx, inv_blockdx, bi, bj, bk)) continue;
+        F3 p = p0 - boff;
+        I3 gmin = pos_to_index((double)(p.x - sr), (double)(p.y - sr), (double)(p.z - sr), g.invdx);
+        I3 gmax = pos_to_index((double)(p.x + sr), (double)(p.y + sr), (double)(p.z + sr), g.invdx);
+        gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
+        gmax.i = min(gmax.i, kBW - 1); gmax.j = min(gmax.j, kBW - 1); gmax.k = min(gmax.k, kBW - 1);
+        for (int k = gmin.k; k <= gmax.k; k++) {
+            float cz = idx_to_center1(k, g.dx);
+            for (int j = gmin.j; j <= gmax.j; j++) {
+                float cy = idx_to_center1(j, g.dx);
+                for (int i = gmin.i; i <= gmax.i; i++) {
+                    F3 c = f3(idx_to_center1(i, g.dx), cy, cz);
+                    float dist = len3(c - p) - r;
+                    atomicMin(&phi_ord[i + kBW * (j + kBW * k)], f2ord(dist));
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // Write back with the post-processing pass (particlelevelset.cpp:172-196).
+    float eps = (float)(0.005 * g.dx);
+    Dims N{g.I + 1, g.J + 1, g.K + 1};
+    for (int c = threadIdx.x; c < kBW * kBW * kBW; c += kSdfThreads) {
+        int li = c % kBW, lj = (c / kBW) % kBW, lk = c / (kBW * kBW);
+        int i = bi * kBW + li, j = bj * kBW + lj, k = bk * kBW + lk;
+        if (i >= g.I || j >= g.J || k >= g.K) continue;
+        float v = ord2f(phi_ord[c]);
+        if (!solid) {  // calculateSignedDistanceField without the post-process
+            phi_out[(size_t)i + (size_t)g.I * ((size_t)j + (size_t)g.J * k)] = v;
+            continue;
+        }
+        if ((double)v < 0.5 * g.dx) {
+            // MeshLevelSet::getDistanceAtCellCenter (meshlevelset.cpp:152-162).
+            float sc = 0.125f * (solid[N.at(i, j, k)] + solid[N.at(i + 1, j, k)] +
+                                 solid[N.at(i, j + 1, k)] + solid[N.at(i + 1, j + 1, k)] +
+                                 solid[N.at(i, j, k + 1)] + solid[N.at(i + 1, j, k + 1)] +
+                                 solid[N.at(i, j + 1, k + 1)] + solid[N.at(i + 1, j + 1, k + 1)]);
+            if (sc < 0) v = (float)(-0.5f * g.dx);
+        }
+        if (fabsf(v) < eps) v = v > 0 ? eps : -eps;
+        phi_out[(size_t)i + (size_t)g.I * ((size_t)j + (size_t)g.J * k)] = v;
+    }
+}
+
+hipError_t launch_liquid_sdf(const GridGeom &g, const float *sx, const float *sy, const float *sz,
+                             const int *offset, double radius, const float *solid_phi_nodal,
+                             float *phi, hipStream_t s) {
+    int nbi = (g.I + kBW - 1) / kBW, nbj = (g.J + kBW - 1) / kBW, nbk = (g.K + kBW - 1) / kBW;
+    k_liquid_sdf<<<nbi * nbj * nbk, kSdfThreads, 0, s>>>(g, sx, sy, sz, offset, (float)radius,
+                                                         solid_phi_nodal, phi);
+    return hipGetLastError();
+}
+
+}  // namespace flipmi

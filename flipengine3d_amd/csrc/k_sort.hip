@@ -1,0 +1,170 @@
+// Counting sort of particles by cell and an exclusive prefix scan.
+//
+// The sorted order groups particle ids by cell (i fastest, as Array3d) and
+// keeps ids ascending inside a cell, so every later gather over a cell's
+// particles visits them in the same order on every run: the step is
+// deterministic although slots are claimed with atomics.
+#include "device_math.h"
+#include "flipmi_kernels.h"
+
+namespace flipmi {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanPerThread = 4;
+constexpr int kScanChunk = kScanThreads * kScanPerThread;
+
+// Inclusive scan of one value per lane across a 64-wide wavefront.
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of kScanPerThread consecutive elements per thread.
+__global__ void k_scan_chunk(const int *__restrict__ in, int *out, int n, int *block_sums) {
+    __shared__ int wave_tot[kScanThreads / 64];
+    size_t base = (size_t)blockIdx.x * kScanChunk + (size_t)threadIdx.x * kScanPerThread;
+    int v[kScanPerThread];
+    int local = 0;
+#pragma unroll
+    for (int q = 0; q < kScanPerThread; q++) {
+        size_t idx = base + q;
+        v[q] = idx < (size_t)n ? in[idx] : 0;
+        local += v[q];
+    }
+    int inc = wave_inclusive_scan(local);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) wave_tot[wave] = inc;
+    __syncthreads();
+    int wave_off = 0;
+    for (int w = 0; w < wave; w++) wave_off += wave_tot[w];
+    int run = wave_off + inc - local;
+#pragma unroll
+    for (int q = 0; q < kScanPerThread; q++) {
+        size_t idx = base + q;
+        if (idx < (size_t)n) out[idx] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == kScanThreads - 1 && block_sums) block_sums[blockIdx.x] = wave_off + inc;
+}
+
+__global__ void k_scan_add(int *out, int n, const int *__restrict__ block_offsets) {
+    size_t idx = (size_t)blockIdx.x * kScanChunk + threadIdx.x;
+    int add = block_offsets[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < kScanPerThread; q++) {
+        size_t e = idx + (size_t)q * kScanThreads;
+        if (e < (size_t)n) out[e] += add;
+    }
+}
+
+size_t scan_scratch_elems(int n) {
+    size_t total = 0;
+    size_t m = (size_t)n;
+    while (m > 1) {
+        m = (m + kScanChunk - 1) / kScanChunk;
+        total += 2 * m + 1;
+    }
+    return total + 8;
+}
+
+// Exclusive scan in[0..n) -> out[0..n); out[n] is not written.
+hipError_t launch_exclusive_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int nb = (n + kScanChunk - 1) / kScanChunk;
+    if (nb == 1) {
+        k_scan_chunk<<<1, kScanThreads, 0, s>>>(in, out, n, nullptr);
+        return hipGetLastError();
+    }
+    int *sums = tmp;
+    int *sums_scanned = tmp + nb;
+    k_scan_chunk<<<nb, kScanThreads, 0, s>>>(in, out, n, sums);
+    hipError_t e = launch_exclusive_scan(sums, sums_scanned, nb, tmp + 2 * (size_t)nb + 1, s);
+    if (e != hipSuccess) return e;
+    k_scan_add<<<nb, kScanThreads, 0, s>>>(out, n, sums_scanned);
+    return hipGetLastError();
+}
+
+__global__ void k_cell_key(GridGeom g, const float *__restrict__ px, const float *__restrict__ py,
+                           const float *__restrict__ pz, int n, int *cell, int *slot, int *count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    I3 c = pos_to_index(f3(px[i], py[i], pz[i]), g.invdx);
+    int id = -1, sl = 0;
+    if (c.i >= 0 && c.j >= 0 && c.k >= 0 && c.i < g.I && c.j < g.J && c.k < g.K) {
+        id = c.i + g.I * (c.j + g.J * c.k);
+        sl = atomicAdd(&count[id], 1);
+    }
+    cell[i] = id;
+    slot[i] = sl;
+}
+
+__global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
+                          const int *__restrict__ offset, int n, int *order) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = cell[i];
+    if (c < 0) return;
+    order[offset[c] + slot[i]] = i;
+}
+
+// Ascending ids inside each cell (cells hold a handful of particles).
+__global__ void k_sort_within_cell(const int *__restrict__ offset, int ncell, int *order) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int b = offset[c], e = offset[c + 1];
+    for (int p = b + 1; p < e; p++) {
+        int v = order[p];
+        int q = p - 1;
+        while (q >= b && order[q] > v) {
+            order[q + 1] = order[q];
+            q--;
+        }
+        order[q + 1] = v;
+    }
+}
+
+__global__ void k_set_last(int *offset, int ncell, const int *__restrict__ count) {
+    offset[ncell] = offset[ncell - 1] + count[ncell - 1];
+}
+
+hipError_t launch_sort_by_cell(const GridGeom &g, const float *px, const float *py,
+                               const float *pz, int n, SortBuffers &sb, hipStream_t s) {
+    int ncell = g.I * g.J * g.K;
+    hipError_t e = launch_fill_i32(sb.count, (size_t)ncell, 0, s);
+    if (e != hipSuccess) return e;
+    if (n > 0) k_cell_key<<<(n + 255) / 256, 256, 0, s>>>(g, px, py, pz, n, sb.cell, sb.slot, sb.count);
+    e = launch_exclusive_scan(sb.count, sb.offset, ncell, sb.scan_tmp, s);
+    if (e != hipSuccess) return e;
+    k_set_last<<<1, 1, 0, s>>>(sb.offset, ncell, sb.count);
+    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, s>>>(sb.cell, sb.slot, sb.offset, n, sb.order);
+    k_sort_within_cell<<<(ncell + 255) / 256, 256, 0, s>>>(sb.offset, ncell, sb.order);
+    return hipGetLastError();
+}
+
+__global__ void k_gather6(const int *__restrict__ order, const int *__restrict__ nsorted,
+                          const float *__restrict__ a0,
+                          const float *__restrict__ a1, const float *__restrict__ a2,
+                          const float *__restrict__ a3, const float *__restrict__ a4,
+                          const float *__restrict__ a5, float *b0, float *b1, float *b2, float *b3,
+                          float *b4, float *b5) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *nsorted) return;  // particles outside the grid are not in the order
+    int o = order[i];
+    b0[i] = a0[o]; b1[i] = a1[o]; b2[i] = a2[o];
+    b3[i] = a3[o]; b4[i] = a4[o]; b5[i] = a5[o];
+}
+
+hipError_t launch_gather_sorted(const int *order, const int *nsorted, int n,
+                                const float *const src[6], float *const dst[6], hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_gather6<<<(n + 255) / 256, 256, 0, s>>>(order, nsorted, src[0], src[1], src[2], src[3], src[4],
+                                              src[5], dst[0], dst[1], dst[2], dst[3], dst[4], dst[5]);
+    return hipGetLastError();
+}
+
+}  // namespace flipmi

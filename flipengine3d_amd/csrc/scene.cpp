@@ -1,0 +1,595 @@
+// Host orchestration of the FLIP step on one device stream.
+//
+// Scene::update mirrors FluidSimulation::update (fluidsimulation.cpp:
+// 5755-5846) and Scene::step_fluid mirrors FluidSimulation::_stepFluid
+// (fluidsimulation.cpp:5471-5508) for scenes with domain walls and a constant
+// body force; every stage runs as HIP kernels on device-resident data.  The
+// host reads back a handful of scalars per substep (maximum speed, |rhs|,
+// PCG progress, the speed histogram and the surviving particle count).
+#include "scene.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace flipmi {
+
+// ---------------------------------------------------------------------------
+// Domain geometry
+// ---------------------------------------------------------------------------
+
+namespace {
+struct HBox {
+    float pos[3];
+    double ext[3];
+};
+
+// AABB(0,0,0, I dx, J dx, K dx).expand(-3 dx - 1e-4) (fluidsimulation.cpp:
+// 2834-2839, aabb.cpp:122-128): float min corner, double extents.
+HBox boundary_box(int isize, int jsize, int ksize, double dx) {
+    HBox b;
+    double v = -3 * dx - 1e-4;
+    double h = 0.5 * v;
+    for (int a = 0; a < 3; a++) b.pos[a] = 0.0f - (float)h;
+    b.ext[0] = isize * dx + v;
+    b.ext[1] = jsize * dx + v;
+    b.ext[2] = ksize * dx + v;
+    return b;
+}
+
+inline int floor_index(double x, double dx) { return (int)std::floor(x * (1.0 / dx)); }
+}  // namespace
+
+BoxMesh make_domain_box(int isize, int jsize, int ksize, double dx) {
+    const int band = 3;  // fluidsimulation.h:1580 (_solidLevelSetExactBand)
+    HBox bb = boundary_box(isize, jsize, ksize, dx);
+    float px = bb.pos[0], py = bb.pos[1], pz = bb.pos[2];
+    float xw = (float)(px + bb.ext[0]), yh = (float)(py + bb.ext[1]), zd = (float)(pz + bb.ext[2]);
+    // Vertices and triangles of FluidSimulation::_getTriangleMeshFromAABB.
+    const float vert[8][3] = {{px, py, pz}, {xw, py, pz}, {xw, py, zd}, {px, py, zd},
+                              {px, yh, pz}, {xw, yh, pz}, {xw, yh, zd}, {px, yh, zd}};
+    const int tri[12][3] = {{0, 1, 2}, {0, 2, 3}, {4, 7, 6}, {4, 6, 5}, {0, 3, 7}, {0, 7, 4},
+                            {1, 5, 6}, {1, 6, 2}, {0, 4, 5}, {0, 5, 1}, {3, 2, 6}, {3, 6, 7}};
+    BoxMesh m{};
+    m.ntri = 0;
+    for (int t = 0; t < 12; t++) {
+        BoxTri T{};
+        double mn[3], mx[3];
+        for (int a = 0; a < 3; a++) {
+            mn[a] = mx[a] = vert[tri[t][0]][a];
+        }
+        for (int c = 0; c < 3; c++) {
+            for (int a = 0; a < 3; a++) {
+                T.v[c][a] = vert[tri[t][c]][a];
+                mn[a] = std::fmin((double)vert[tri[t][c]][a], mn[a]);
+                mx[a] = std::fmax((double)vert[tri[t][c]][a], mx[a]);
+            }
+        }
+        // AABB(Triangle) min corner in float, extents + 1e-9, max point in float.
+        float bmin[3], bmax[3];
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = (float)mn[a];
+            double ext = mx[a] - mn[a] + 1e-9;
+            bmax[a] = bmin[a] + (float)ext;
+        }
+        int gmin[3], gmax[3], lim[3] = {isize, jsize, ksize};
+        for (int a = 0; a < 3; a++) {
+            gmin[a] = floor_index((double)bmin[a], dx) - band;
+            gmax[a] = floor_index((double)bmax[a], dx) + band + 1;
+        }
+        bool outside = false;
+        for (int a = 0; a < 3; a++)
+            if (gmax[a] < 0 || gmin[a] >= lim[a] + 1) outside = true;
+        if (outside) continue;
+        for (int a = 0; a < 3; a++) {
+            T.gmin[a] = std::max(gmin[a], 0);
+            T.gmax[a] = std::min(gmax[a], lim[a]);
+        }
+        m.tri[m.ntri++] = T;
+    }
+    for (int a = 0; a < 3; a++) {
+        m.bpos[a] = bb.pos[a];
+        m.bext[a] = bb.ext[a];
+    }
+    return m;
+}
+
+void collision_box(int isize, int jsize, int ksize, double dx, float pos[3], double ext[3]) {
+    HBox b = boundary_box(isize, jsize, ksize, dx);
+    double v = -0.1f * dx;  // fluidsimulation.h:1685 (_solidBufferWidth)
+    double h = 0.5 * v;
+    for (int a = 0; a < 3; a++) {
+        pos[a] = b.pos[a] - (float)h;
+        ext[a] = b.ext[a] + v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pressure solve driver
+// ---------------------------------------------------------------------------
+
+void PressureStore::alloc(const GridGeom &g, hipStream_t s) {
+    size_t nc = (size_t)g.I * g.J * g.K;
+    int ntj = (g.J + 7) / 8, ntk = (g.K + 7) / 8;
+    int npart = 2 * 1024 + ntj * ntk;
+    for (int q = 0; q < 13; q++)
+        if (arr[q].alloc(nc) != hipSuccess)
+            throw StatusError{FLIPMI_ERR_OOM, "pressure arrays: out of device memory"};
+    if (arr[13].alloc((size_t)npart) != hipSuccess ||
+        progress.alloc((size_t)ntj * ntk) != hipSuccess || ticket.alloc(1) != hipSuccess ||
+        scal.alloc(1) != hipSuccess)
+        throw StatusError{FLIPMI_ERR_OOM, "pressure scratch: out of device memory"};
+    (void)launch_fill_i32((int *)progress.p, progress.n, 0, s);
+    (void)launch_fill_i32((int *)ticket.p, 1, 0, s);
+    (void)launch_fill_u8((uint8_t *)scal.p, sizeof(PcgScalars), 0, s);
+    (void)launch_fill_f64(arr[0].p, nc, 0.0, s);
+    pb.diag = arr[0].p; pb.offx = arr[1].p; pb.offy = arr[2].p; pb.offz = arr[3].p;
+    pb.rhs = arr[4].p; pb.x = arr[5].p; pb.r = arr[6].p; pb.z = arr[7].p; pb.s = arr[8].p;
+    pb.invdiag = arr[9].p; pb.Lx = arr[10].p; pb.Ly = arr[11].p; pb.Lz = arr[12].p;
+    pb.partials = arr[13].p;
+    pb.npartials = npart;
+    pb.progress = progress.p;
+    pb.ntiles_j = ntj;
+    pb.ntiles_k = ntk;
+    pb.ticket = ticket.p;
+    pb.sc = scal.p;
+}
+
+static void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw StatusError{FLIPMI_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScalars *hs,
+                             hipStream_t s, Scene *prof) {
+    PressureOutcome out;
+    hipEvent_t ev = nullptr;
+    if (prof) prof->begin_stage(FLIPMI_STAGE_ASSEMBLE, &ev);
+    hip_check(launch_pressure_reset(pb, s), "pressure reset");
+    hip_check(launch_pressure_assemble(w.g, w.dt, w.liquid, w.wU, w.wV, w.wW, w.wC, w.u, w.v, w.w,
+                                       w.sU, w.sV, w.sW, pb, s),
+              "pressure assemble");
+    if (prof) prof->end_stage(FLIPMI_STAGE_ASSEMBLE, ev);
+    hip_check(hipMemcpyAsync(hs, pb.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), "scalars");
+    hip_check(hipStreamSynchronize(s), "assemble sync");
+    double bnorm;
+    std::memcpy(&bnorm, &hs->absmax_bits, sizeof(double));
+    // Early exit of PressureSolver::solve when max|rhs| < tolerance
+    // (pressuresolver.cpp:52-62): nothing is solved or applied.
+    if (bnorm < w.tol) {
+        out.iterations = 0;
+        out.residual = 0.0;
+        out.success = true;
+        return out;
+    }
+    if (prof) prof->begin_stage(FLIPMI_STAGE_MIC0, &ev);
+    hip_check(launch_mic0_factor(w.g, pb, s), "mic0 factor");
+    hip_check(launch_pcg_init(w.g, w.tol, w.maxiter, pb, s), "pcg init");
+    if (prof) prof->end_stage(FLIPMI_STAGE_MIC0, ev);
+    if (prof) prof->begin_stage(FLIPMI_STAGE_PCG, &ev);
+    int batch = 8;
+    int launched = 0;
+    for (;;) {
+        for (int q = 0; q < batch; q++) hip_check(launch_pcg_iteration(w.g, pb, s), "pcg iteration");
+        launched += batch;
+        hip_check(hipMemcpyAsync(hs, pb.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), "scalars");
+        hip_check(hipStreamSynchronize(s), "pcg sync");
+        if (hs->abort) throw StatusError{FLIPMI_ERR_TIMEOUT, "wavefront sweep wait timed out"};
+        if (hs->done) break;
+        if (launched > w.maxiter + 16)
+            throw StatusError{FLIPMI_ERR_HIP, "pcg did not terminate within its iteration cap"};
+        batch = std::min(batch * 2, 64);
+    }
+    if (prof) prof->end_stage(FLIPMI_STAGE_PCG, ev);
+    if (prof) prof->stage_calls[FLIPMI_STAGE_PCG_ITERS] += hs->iters;
+    out.iterations = hs->failed ? 0 : hs->iters;
+    out.residual = hs->resid;
+    bool converged = hs->converged != 0;
+    // PressureSolver::_solveLinearSystem (pressuresolver.cpp:810-840).
+    out.success = converged || (!hs->failed && out.iterations == w.maxiter && out.residual < w.acceptable);
+    if (out.success) {
+        if (prof) prof->begin_stage(FLIPMI_STAGE_APPLY, &ev);
+        hip_check(launch_pressure_apply(w.g, w.dt, w.liquid, w.wU, w.wV, w.wW, pb.x, w.u, w.v, w.w,
+                                        w.vu, w.vv, w.vw, s),
+                  "pressure apply");
+        if (prof) prof->end_stage(FLIPMI_STAGE_APPLY, ev);
+        out.solved = true;
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Scene
+// ---------------------------------------------------------------------------
+
+Scene::~Scene() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &pe : pending_events) {
+        (void)hipEventDestroy(pe.a);
+        (void)hipEventDestroy(pe.b);
+    }
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    if (host_stage) (void)hipHostFree(host_stage);
+    delete press;
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Scene::check(hipError_t e, const char *what) { hip_check(e, what); }
+
+void Scene::sync() { check(hipStreamSynchronize(stream), "stream sync"); }
+
+hipEvent_t Scene::take_event() {
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    check(hipEventCreate(&e), "event create");
+    return e;
+}
+
+void Scene::begin_stage(int stage, hipEvent_t *a) {
+    (void)stage;
+    if (!profiling) return;
+    *a = take_event();
+    check(hipEventRecord(*a, stream), "event record");
+}
+
+void Scene::end_stage(int stage, hipEvent_t a) {
+    if (!profiling) return;
+    hipEvent_t b = take_event();
+    check(hipEventRecord(b, stream), "event record");
+    pending_events.push_back(PendingEvent{stage, a, b});
+}
+
+void Scene::collect_events() {
+    if (pending_events.empty()) return;
+    sync();
+    for (auto &pe : pending_events) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, pe.a, pe.b), "event elapsed");
+        stage_ms[pe.stage] += ms;
+        stage_calls[pe.stage] += 1;
+        event_pool.push_back(pe.a);
+        event_pool.push_back(pe.b);
+    }
+    pending_events.clear();
+}
+
+void Scene::alloc_all() {
+    int I = desc.isize, J = desc.jsize, K = desc.ksize;
+    size_t nc = (size_t)I * J * K;
+    size_t nn = (size_t)(I + 1) * (J + 1) * (K + 1);
+    size_t nU = (size_t)(I + 1) * J * K, nV = (size_t)I * (J + 1) * K, nW = (size_t)I * J * (K + 1);
+    size_t nf = std::max(nU, std::max(nV, nW));
+    auto need = [&](hipError_t e) {
+        if (e != hipSuccess) throw StatusError{FLIPMI_ERR_OOM, "scene allocation: out of device memory"};
+    };
+    size_t pc = (size_t)std::max(cap, 1);
+    for (int a = 0; a < 6; a++) {
+        need(part[a].alloc(pc));
+        need(part_alt[a].alloc(pc));
+        need(sorted[a].alloc(pc));
+    }
+    need(cell.alloc(pc)); need(slot.alloc(pc)); need(order.alloc(pc));
+    need(keep.alloc(pc)); need(keep_pos.alloc(pc)); need(rank.alloc(pc));
+    need(solid_removed.alloc(pc));
+    need(count.alloc(nc)); need(offset.alloc(nc + 1));
+    size_t st = std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc));
+    need(scan_tmp.alloc(st));
+    need(new_count.alloc(1)); need(speed_hist.alloc(64)); need(max_speed_bits.alloc(1));
+    need(solid.alloc(nn));
+    need(wU.alloc(nU)); need(wV.alloc(nV)); need(wW.alloc(nW)); need(wC.alloc(nc));
+    need(sU.alloc(nU)); need(sV.alloc(nV)); need(sW.alloc(nW));
+    nci = (int)std::ceil((double)I * g.dx / (3 * g.dx));
+    ncj = (int)std::ceil((double)J * g.dx / (3 * g.dx));
+    nck = (int)std::ceil((double)K * g.dx / (3 * g.dx));
+    need(near_a.alloc((size_t)nci * ncj * nck)); need(near_b.alloc((size_t)nci * ncj * nck));
+    need(liquid.alloc(nc));
+    need(u.alloc(nU)); need(v.alloc(nV)); need(w.alloc(nW));
+    need(su.alloc(nU)); need(sv.alloc(nV)); need(sw.alloc(nW));
+    need(vu.alloc(nU)); need(vv.alloc(nV)); need(vw.alloc(nW));
+    need(status.alloc(nf));
+    press = new PressureStore();
+    press->alloc(g, stream);
+    check(hipHostMalloc(&host_stage, 4096, hipHostMallocDefault), "pinned staging");
+    // Fresh MeshLevelSet: phi = distance upper bound everywhere
+    // (meshlevelset.cpp:35-42, 760-762); zero solid face velocities.
+    float upper = (float)((double)((I + 1) + (J + 1) + (K + 1)) * g.dx);
+    check(launch_fill_f32(solid.p, nn, upper, stream), "fill");
+    check(launch_fill_f32(sU.p, nU, 0.0f, stream), "fill");
+    check(launch_fill_f32(sV.p, nV, 0.0f, stream), "fill");
+    check(launch_fill_f32(sW.p, nW, 0.0f, stream), "fill");
+}
+
+void Scene::load(const float *pos, const float *vel, int64_t count_in) {
+    // FluidSimulation::_loadMarkerParticles (fluidsimulation.cpp:2773-2785):
+    // keep particles inside [0, I dx) x [0, J dx) x [0, K dx).
+    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    double ex = desc.isize * desc.dx, ey = desc.jsize * desc.dx, ez = desc.ksize * desc.dx;
+    for (int64_t q = 0; q < count_in; q++) {
+        float x = pos[3 * q], y = pos[3 * q + 1], z = pos[3 * q + 2];
+        if (!(x >= bx && y >= by && z >= bz && (double)x < (double)bx + ex &&
+              (double)y < (double)by + ey && (double)z < (double)bz + ez))
+            continue;
+        pending_pos.push_back(x); pending_pos.push_back(y); pending_pos.push_back(z);
+        for (int a = 0; a < 3; a++) pending_vel.push_back(vel ? vel[3 * q + a] : 0.0f);
+    }
+}
+
+void Scene::initialize() {
+    if (initialized) return;
+    int64_t np = (int64_t)(pending_pos.size() / 3);
+    if (np > INT32_MAX / 2) throw StatusError{FLIPMI_ERR_INVALID_ARG, "too many particles"};
+    cap = (int)std::max<int64_t>(np, desc.max_particles);
+    alloc_all();
+    n = (int)np;
+    if (n > 0) {
+        // Host AoS -> device SoA.
+        std::vector<float> tmp((size_t)n);
+        for (int a = 0; a < 6; a++) {
+            const std::vector<float> &src = a < 3 ? pending_pos : pending_vel;
+            for (int q = 0; q < n; q++) tmp[q] = src[3 * (size_t)q + (a % 3)];
+            check(hipMemcpyAsync(part[a].p, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream),
+                  "upload particles");
+            sync();
+        }
+    }
+    pending_pos.clear(); pending_vel.clear();
+    pending_pos.shrink_to_fit(); pending_vel.shrink_to_fit();
+    // FluidSimulation::_initializeParticleRadii (fluidsimulation.cpp:2644-2650).
+    radius = 0.5 * 1.0 * g.dx * std::sqrt(3.0);
+    initialized = true;
+}
+
+void Scene::ensure_solid() {
+    // FluidSimulation::_updateSolidLevelSet (fluidsimulation.cpp:3028-3072):
+    // computed once, when marker particles exist.
+    if (solid_ready || n == 0) return;
+    BoxMesh mesh = make_domain_box(desc.isize, desc.jsize, desc.ksize, desc.dx);
+    check(launch_domain_solid(g, mesh, solid.p, stream), "domain solid");
+    weights_ready = false;
+    // _updateNearSolidGrid (fluidsimulation.cpp:3094-3126) depends only on
+    // the static solid, so it is built once as well.
+    int layers = (int)std::ceil((float)desc.cfl / (float)3);
+    check(launch_near_solid(g, solid.p, near_a.p, near_b.p, nci, ncj, nck, layers, &near, stream),
+          "near solid");
+    solid_ready = true;
+}
+
+void Scene::sort_particles() {
+    SortBuffers sb{cell.p, slot.p, count.p, offset.p, order.p, scan_tmp.p, scan_tmp.n};
+    check(launch_sort_by_cell(g, part[0].p, part[1].p, part[2].p, n, sb, stream), "sort");
+}
+
+void Scene::extrapolate_all(int layers) {
+    hipEvent_t ev = nullptr;
+    begin_stage(FLIPMI_STAGE_EXTRAPOLATE, &ev);
+    check(launch_extrapolate(g.I + 1, g.J, g.K, u.p, vu.p, status.p, layers, stream), "extrapolate");
+    check(launch_extrapolate(g.I, g.J + 1, g.K, v.p, vv.p, status.p, layers, stream), "extrapolate");
+    check(launch_extrapolate(g.I, g.J, g.K + 1, w.p, vw.p, status.p, layers, stream), "extrapolate");
+    end_stage(FLIPMI_STAGE_EXTRAPOLATE, ev);
+}
+
+void Scene::pressure_solve(double dt) {
+    if (!weights_ready) {
+        check(launch_weights(g, solid.p, wU.p, wV.p, wW.p, wC.p, stream), "weights");
+        weights_ready = true;
+    }
+    PressureWork pw;
+    pw.g = g; pw.dt = dt; pw.tol = desc.pcg_tolerance; pw.acceptable = 1.0;
+    pw.maxiter = desc.pcg_max_iterations; pw.precision = desc.pcg_precision;
+    pw.liquid = liquid.p; pw.wU = wU.p; pw.wV = wV.p; pw.wW = wW.p; pw.wC = wC.p;
+    pw.sU = sU.p; pw.sV = sV.p; pw.sW = sW.p;
+    pw.u = u.p; pw.v = v.p; pw.w = w.p; pw.vu = vu.p; pw.vv = vv.p; pw.vw = vw.p;
+    PressureOutcome o =
+        run_pressure(pw, press->pb, (PcgScalars *)host_stage, stream, profiling ? this : nullptr);
+    last_iters = o.iterations;
+    last_resid = o.residual;
+    last_ok = o.success ? 1 : 0;
+}
+
+void Scene::advance_particles(double dt) {
+    hipEvent_t ev = nullptr;
+    begin_stage(FLIPMI_STAGE_G2P_ADVECT, &ev);
+    ParticleStepArgs a{};
+    a.px = part[0].p; a.py = part[1].p; a.pz = part[2].p;
+    a.vx = part[3].p; a.vy = part[4].p; a.vz = part[5].p;
+    a.n = n;
+    a.u = u.p; a.v = v.p; a.w = w.p;
+    a.su = su.p; a.sv = sv.p; a.sw = sw.p;
+    a.solid = solid.p;
+    a.near_solid = near;
+    a.nci = nci; a.ncj = ncj; a.nck = nck;
+    a.near_cell = 3 * g.dx;
+    a.pic_ratio_f = (float)desc.pic_ratio;
+    a.flip_ratio_f = (float)(1 - desc.pic_ratio);
+    a.dt = dt;
+    a.cfl = desc.cfl;
+    collision_box(desc.isize, desc.jsize, desc.ksize, desc.dx, a.bpos, a.bext);
+    a.solid_removed = solid_removed.p;
+    a.speed_hist = speed_hist.p;
+    // Speed limit uses the frame step (fluidsimulation.cpp:4385).
+    a.speed_step = desc.cfl * g.dx / frame_dt;
+    a.maxsteps = desc.max_substeps;
+    check(launch_fill_i32(speed_hist.p, 64, 0, stream), "fill");
+    check(launch_particle_step(g, a, stream), "particle step");
+    end_stage(FLIPMI_STAGE_G2P_ADVECT, ev);
+
+    // _removeMarkerParticles (fluidsimulation.cpp:4324-4353).
+    begin_stage(FLIPMI_STAGE_REMOVE, &ev);
+    int *hist = (int *)host_stage;
+    check(hipMemcpyAsync(hist, speed_hist.p, sizeof(int) * desc.max_substeps, hipMemcpyDeviceToHost, stream),
+          "hist");
+    sort_particles();
+    check(launch_fill_i32(rank.p, (size_t)std::max(n, 1), -1, stream), "fill");
+    check(launch_cell_rank_cells(offset.p, g.I * g.J * g.K, order.p, solid_removed.p, rank.p, stream),
+          "cell rank");
+    sync();
+    // _getMarkerParticleSpeedLimit (fluidsimulation.cpp:4298-4322).
+    double step = desc.cfl * g.dx / frame_dt;
+    int maxRemoval = (int)std::fmin((int)((double)n * 0.0005), 35);
+    double maxspeed = desc.max_substeps * step;
+    int cur = 0;
+    for (int i = desc.max_substeps - 1; i > 0; i--) {
+        if (cur + hist[i] > maxRemoval) break;
+        cur += hist[i];
+        maxspeed = i * step;
+    }
+    float maxspeed_f = (float)maxspeed;
+    double maxsq = maxspeed_f * maxspeed_f;
+    check(launch_keep_flags(part[3].p, part[4].p, part[5].p, n, solid_removed.p, rank.p, maxsq,
+                            keep.p, stream),
+          "keep flags");
+    const float *src[6];
+    float *dst[6];
+    for (int a2 = 0; a2 < 6; a2++) {
+        src[a2] = part[a2].p;
+        dst[a2] = part_alt[a2].p;
+    }
+    check(launch_compact_keep(keep.p, n, keep_pos.p, scan_tmp.p, src, dst, new_count.p, stream), "compact");
+    int *nc_host = (int *)host_stage + 64;
+    check(hipMemcpyAsync(nc_host, new_count.p, sizeof(int), hipMemcpyDeviceToHost, stream), "count");
+    sync();
+    n = *nc_host;
+    for (int a2 = 0; a2 < 6; a2++) std::swap(part[a2].p, part_alt[a2].p);
+    end_stage(FLIPMI_STAGE_REMOVE, ev);
+}
+
+double Scene::next_time_step(double dt) {
+    // FluidSimulation::_calculateNextTimeStep (fluidsimulation.cpp:5593-5613).
+    double maxu;
+    if (current_frame == 0 && step_number == 0) {
+        float bf[3] = {(float)desc.body_force[0], (float)desc.body_force[1], (float)desc.body_force[2]};
+        float len = std::sqrt(bf[0] * bf[0] + bf[1] * bf[1] + bf[2] * bf[2]);
+        maxu = 0.0 + (double)len * dt;
+    } else {
+        check(launch_max_speed(part[3].p, part[4].p, part[5].p, n, max_speed_bits.p, stream), "max speed");
+        unsigned *bits = (unsigned *)host_stage;
+        check(hipMemcpyAsync(bits, max_speed_bits.p, sizeof(unsigned), hipMemcpyDeviceToHost, stream),
+              "max speed copy");
+        sync();
+        float maxsq;
+        std::memcpy(&maxsq, bits, sizeof(float));
+        maxu = std::sqrt((double)maxsq);
+    }
+    maxu = std::fmax(0.0, maxu);
+    return desc.cfl * g.dx / (maxu + 1e-6);
+}
+
+void Scene::step_fluid(double dt) {
+    ensure_solid();
+    hipEvent_t ev = nullptr;
+    // Liquid SDF + post-process (fluidsimulation.cpp:3225-3247).
+    begin_stage(FLIPMI_STAGE_SORT, &ev);
+    sort_particles();
+    const float *src[6];
+    float *dst[6];
+    for (int a = 0; a < 6; a++) {
+        src[a] = part[a].p;
+        dst[a] = sorted[a].p;
+    }
+    check(launch_gather_sorted(order.p, offset.p + (size_t)g.I * g.J * g.K, n, src, dst, stream),
+          "gather");
+    end_stage(FLIPMI_STAGE_SORT, ev);
+    begin_stage(FLIPMI_STAGE_CLASSIFY, &ev);
+    check(launch_liquid_sdf(g, sorted[0].p, sorted[1].p, sorted[2].p, offset.p, radius, solid.p,
+                            liquid.p, stream),
+          "liquid sdf");
+    end_stage(FLIPMI_STAGE_CLASSIFY, ev);
+    // _advectVelocityField (fluidsimulation.cpp:3253-3277).
+    int layers = (int)std::ceil(desc.cfl) + 2;
+    size_t nU = u.n, nV = v.n, nW = w.n;
+    if (n > 0) {
+        begin_stage(FLIPMI_STAGE_P2G, &ev);
+        check(launch_p2g(g, 0, sorted[0].p, sorted[1].p, sorted[2].p, sorted[3].p, offset.p, radius,
+                         u.p, vu.p, stream), "p2g u");
+        check(launch_p2g(g, 1, sorted[0].p, sorted[1].p, sorted[2].p, sorted[4].p, offset.p, radius,
+                         v.p, vv.p, stream), "p2g v");
+        check(launch_p2g(g, 2, sorted[0].p, sorted[1].p, sorted[2].p, sorted[5].p, offset.p, radius,
+                         w.p, vw.p, stream), "p2g w");
+        end_stage(FLIPMI_STAGE_P2G, ev);
+        extrapolate_all(layers);
+    } else {
+        check(launch_fill_f32(u.p, nU, 0.0f, stream), "fill");
+        check(launch_fill_f32(v.p, nV, 0.0f, stream), "fill");
+        check(launch_fill_f32(w.p, nW, 0.0f, stream), "fill");
+        check(launch_fill_u8(vu.p, nU, 0, stream), "fill");
+        check(launch_fill_u8(vv.p, nV, 0, stream), "fill");
+        check(launch_fill_u8(vw.p, nW, 0, stream), "fill");
+    }
+    // _saveVelocityField + _applyConstantBodyForces (fluidsimulation.cpp:
+    // 3287-3297, 3450-3489).
+    begin_stage(FLIPMI_STAGE_GLUE, &ev);
+    float bf[3] = {(float)desc.body_force[0], (float)desc.body_force[1], (float)desc.body_force[2]};
+    float *macs[3] = {u.p, v.p, w.p};
+    float *saved[3] = {su.p, sv.p, sw.p};
+    size_t sizes[3] = {nU, nV, nW};
+    for (int c = 0; c < 3; c++) {
+        int do_add = std::fabs(bf[c]) > 1e-6f ? 1 : 0;
+        float add = (float)((double)bf[c] * dt);
+        check(launch_save_and_body_force(sizes[c], macs[c], saved[c], add, do_add, stream), "save/force");
+    }
+    end_stage(FLIPMI_STAGE_GLUE, ev);
+    // _pressureSolve (fluidsimulation.cpp:3732-3769).
+    pressure_solve(dt);
+    extrapolate_all(layers);
+    // _constrainVelocityFields (fluidsimulation.cpp:3937-3948).
+    begin_stage(FLIPMI_STAGE_GLUE, &ev);
+    const float *weights[3] = {wU.p, wV.p, wW.p};
+    const float *svel[3] = {sU.p, sV.p, sW.p};
+    for (int c = 0; c < 3; c++) {
+        check(launch_constrain(sizes[c], saved[c], weights[c], svel[c], stream), "constrain");
+        check(launch_constrain(sizes[c], macs[c], weights[c], svel[c], stream), "constrain");
+    }
+    end_stage(FLIPMI_STAGE_GLUE, ev);
+    // _updateMarkerParticleVelocities + _advanceMarkerParticles.
+    if (n > 0) advance_particles(dt);
+}
+
+flipmi_frame_stats Scene::update(double dt) {
+    if (!initialized)
+        throw StatusError{FLIPMI_ERR_NOT_INITIALIZED, "FluidSimulation must be initialized before update"};
+    if (dt < 0.0) throw StatusError{FLIPMI_ERR_INVALID_ARG, "delta time must be greater than or equal to 0"};
+    hipEvent_t f0 = take_event(), f1 = take_event();
+    check(hipEventRecord(f0, stream), "event");
+    double epsdt = 1e-6;
+    bool zero = dt < epsdt;
+    dt = std::max(dt, epsdt);
+    frame_dt = dt;
+    remaining = dt;
+    step_number = 0;
+    bool skipped = zero && output_initialized;
+    double substepTime = frame_dt / (double)desc.min_substeps;
+    const double eps = 1e-9;
+    do {
+        cur_step = std::fmin(next_time_step(dt), remaining);
+        double done = frame_dt - remaining;
+        double limit = (step_number + 1) * substepTime;
+        if (done + cur_step > limit) cur_step = std::fmin(substepTime, remaining);
+        if (step_number == desc.max_substeps - 1) cur_step = remaining;
+        remaining -= cur_step;
+        if (!skipped) step_fluid(cur_step);
+        step_number++;
+    } while (remaining > eps);
+    check(hipEventRecord(f1, stream), "event");
+    sync();
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, f0, f1), "event");
+    event_pool.push_back(f0);
+    event_pool.push_back(f1);
+    collect_events();
+    output_initialized = true;
+    current_frame++;
+    flipmi_frame_stats st{};
+    st.substeps = step_number;
+    st.num_particles = n;
+    st.pcg_iterations = last_iters;
+    st.pcg_residual = last_resid;
+    st.pressure_ok = last_ok;
+    st.gpu_ms = ms;
+    return st;
+}
+
+}  // namespace flipmi

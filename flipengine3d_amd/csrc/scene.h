@@ -1,0 +1,161 @@
+// Host-side scene: device-resident state of one FLIP simulation and the
+// FluidSimulation::update(dt) substep loop that drives the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/flipmi.h"
+#include "flipmi_kernels.h"
+
+namespace flipmi {
+
+struct Ctx {
+    int device = 0;
+    std::string last_error;
+    hipStream_t stream = nullptr;  // stream for stage drop-ins
+};
+
+// Device buffer that frees itself.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() {}
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void **)&p, sizeof(T) * count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return sizeof(T) * n; }
+};
+
+struct StatusError {
+    int code;
+    std::string msg;
+};
+
+// Builds the domain wall box mesh and the per-triangle exact-band node
+// ranges of FluidSimulation::_getBoundaryAABB / _getTriangleMeshFromAABB /
+// MeshLevelSet::_initializeTriangleData (fluidsimulation.cpp:2805-2845,
+// meshlevelset.cpp:874-915).
+BoxMesh make_domain_box(int isize, int jsize, int ksize, double dx);
+
+// Collision boundary: the wall box shrunk by the 0.1 dx solid buffer
+// (fluidsimulation.cpp:4213-4215).
+void collision_box(int isize, int jsize, int ksize, double dx, float pos[3], double ext[3]);
+
+struct Scene {
+    Ctx *ctx = nullptr;
+    flipmi_scene_desc desc{};
+    GridGeom g{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    // particles (canonical order) and work buffers
+    int n = 0, cap = 0;
+    DBuf<float> part[6], part_alt[6], sorted[6];
+    DBuf<int> cell, slot, count, offset, order, scan_tmp;
+    DBuf<int> keep, keep_pos, rank, new_count;
+    DBuf<uint8_t> solid_removed;
+    DBuf<int> speed_hist;
+    DBuf<unsigned> max_speed_bits;
+
+    // grids
+    DBuf<float> solid, wU, wV, wW, wC, sU, sV, sW;
+    DBuf<uint8_t> near_a, near_b;
+    uint8_t *near = nullptr;
+    int nci = 0, ncj = 0, nck = 0;
+    DBuf<float> liquid, u, v, w, su, sv, sw;
+    DBuf<uint8_t> vu, vv, vw, status;
+
+    // pressure
+    struct PressureStore *press = nullptr;
+
+    // pinned host staging
+    void *host_stage = nullptr;
+
+    // host-side state mirroring FluidSimulation's update bookkeeping
+    bool initialized = false, solid_ready = false, weights_ready = false;
+    bool output_initialized = false;
+    int current_frame = 0, step_number = 0;
+    double frame_dt = 0, remaining = 0, cur_step = 0;
+    double radius = 0;
+    std::vector<float> pending_pos, pending_vel;
+
+    // last pressure solve
+    int last_iters = 0, last_ok = 1;
+    double last_resid = 0;
+
+    // profiling
+    bool profiling = false;
+    double stage_ms[16] = {0};
+    int64_t stage_calls[16] = {0};
+    struct PendingEvent {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingEvent> pending_events;
+    std::vector<hipEvent_t> event_pool;
+
+    ~Scene();
+    void check(hipError_t e, const char *what);
+    void alloc_all();
+    void load(const float *pos, const float *vel, int64_t count);
+    void initialize();
+    flipmi_frame_stats update(double dt);
+    void step_fluid(double dt);
+    double next_time_step(double dt);
+    void ensure_solid();
+    void pressure_solve(double dt);
+    void advance_particles(double dt);
+    void sort_particles();
+    void extrapolate_all(int layers);
+    void begin_stage(int stage, hipEvent_t *a);
+    void end_stage(int stage, hipEvent_t a);
+    void collect_events();
+    hipEvent_t take_event();
+    void sync();
+};
+
+// Pressure solve over caller grids already on the device (used by the
+// PressureSolver drop-in and by Scene::pressure_solve).
+struct PressureWork {
+    GridGeom g;
+    double dt, tol, acceptable;
+    int maxiter, precision;
+    const float *liquid, *wU, *wV, *wW, *wC, *sU, *sV, *sW;
+    float *u, *v, *w;
+    uint8_t *vu, *vv, *vw;
+};
+struct PressureOutcome {
+    int iterations = 0;
+    double residual = 0;
+    bool success = true;
+    bool solved = false;  // the pressure update was applied
+    int refinements = 0;
+};
+PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScalars *host_sc,
+                             hipStream_t s, Scene *prof);
+
+// Device storage of the pressure system for one grid size.
+struct PressureStore {
+    DBuf<double> arr[14];
+    DBuf<unsigned> progress, ticket;
+    DBuf<PcgScalars> scal;
+    PressureBuffers pb{};
+    void alloc(const GridGeom &g, hipStream_t s);
+};
+
+}  // namespace flipmi

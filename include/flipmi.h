@@ -1,0 +1,231 @@
+/*
+ * flipmi -- MI355X-native FLIP fluid step behind a plain C ABI.
+ *
+ * Every entry point replaces one call of the reference engine
+ * (jklae/FLIPEngine3D, src/engine) and keeps its argument meaning and error
+ * behaviour; the citation above each declaration names the reference
+ * interface it stands in for.  No C++ or framework types cross the ABI:
+ * plain pointers (host memory unless stated), sizes and POD structs.
+ *
+ * Grids are flat float/byte arrays, i fastest (Array3d layout,
+ * src/engine/array3d.h:425-428):
+ *   cell grid         I x J x K
+ *   MAC U / V / W     (I+1) x J x K,  I x (J+1) x K,  I x J x (K+1)
+ *   nodal solid SDF   (I+1) x (J+1) x (K+1)
+ * Particle data are xyz float triples, byte-compatible with the reference's
+ * vmath::vec3 blobs (fluidsimulation.cpp:2408-2420, 2488-2509).
+ *
+ * All functions return a flipmi_status (0 = success).  On failure
+ * flipmi_last_error() holds a message.  A scene is used by one host thread
+ * at a time; distinct scenes may be driven concurrently.
+ */
+#ifndef FLIPMI_H
+#define FLIPMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLIPMI_ABI_VERSION 1
+
+typedef enum {
+    FLIPMI_OK = 0,
+    FLIPMI_ERR_INVALID_ARG = -1,    /* std::domain_error in the reference */
+    FLIPMI_ERR_NOT_INITIALIZED = -2,/* std::runtime_error "must be initialized" */
+    FLIPMI_ERR_OOM = -3,
+    FLIPMI_ERR_NOT_CONVERGED = -4,  /* PressureSolver::solve returned false */
+    FLIPMI_ERR_HIP = -5,
+    FLIPMI_ERR_BUDGET = -6,         /* serving: admission budget exceeded */
+    FLIPMI_ERR_TIMEOUT = -7         /* a device-side bounded wait ran out */
+} flipmi_status;
+
+typedef enum {
+    FLIPMI_PCG_FP64 = 0,  /* reference arithmetic: FP64 vectors, alpha/beta as float */
+    FLIPMI_PCG_MIXED = 1  /* FP32 PCG, FP64 dots + iterative refinement */
+} flipmi_pcg_precision;
+
+typedef struct flipmi_ctx flipmi_ctx;
+typedef struct flipmi_scene flipmi_scene;
+
+/* Scene description: FluidSimulation(isize, jsize, ksize, dx)
+ * (fluidsimulation.h:127) plus the setters the workloads use:
+ * addBodyForce (fluidsimulation.cpp:1582), setCFLConditionNumber (:1765),
+ * setMinTimeStepsPerFrame / setMaxTimeStepsPerFrame (:1801, :1819),
+ * setPICFLIPRatio (:1891) and the pressure-solver tolerance / iteration cap
+ * (fluidsimulation.h:1657-1659). */
+typedef struct {
+    int isize, jsize, ksize;
+    double dx;
+    double body_force[3];
+    double cfl;                /* default 5 */
+    int min_substeps;          /* default 1 */
+    int max_substeps;          /* default 6 */
+    double pic_ratio;          /* default 0.05 */
+    double pcg_tolerance;      /* default 1e-9 (relative to |b|_inf) */
+    int pcg_max_iterations;    /* default 1000 */
+    int pcg_precision;         /* flipmi_pcg_precision, default FP64 */
+    int64_t max_particles;     /* particle capacity; 0 = size of the loaded set */
+} flipmi_scene_desc;
+
+/* FluidSimulationFrameStats (fluidsimulation.h:67-100) plus device timing. */
+typedef struct {
+    int substeps;
+    int64_t num_particles;
+    int pcg_iterations;        /* last substep */
+    double pcg_residual;       /* last substep, |r|_inf ("Estimated Error") */
+    int pressure_ok;           /* 0 if the last pressure solve failed */
+    double gpu_ms;             /* device time of the frame */
+} flipmi_frame_stats;
+
+/* Per-stage device time accumulated since the last reset (hipEvents). */
+typedef struct {
+    double ms[16];
+    int64_t calls[16];
+} flipmi_stage_times;
+
+enum {
+    FLIPMI_STAGE_SORT = 0,
+    FLIPMI_STAGE_CLASSIFY = 1,
+    FLIPMI_STAGE_P2G = 2,
+    FLIPMI_STAGE_EXTRAPOLATE = 3,
+    FLIPMI_STAGE_ASSEMBLE = 4,
+    FLIPMI_STAGE_MIC0 = 5,
+    FLIPMI_STAGE_PCG = 6,           /* all PCG iterations of a solve */
+    FLIPMI_STAGE_APPLY = 7,
+    FLIPMI_STAGE_G2P_ADVECT = 8,
+    FLIPMI_STAGE_REMOVE = 9,
+    FLIPMI_STAGE_GLUE = 10,         /* save/body force/constrain */
+    FLIPMI_STAGE_PCG_ITERS = 11     /* calls = iterations, ms unused */
+};
+
+/* Grids readable with flipmi_scene_get_grid (sizes as in the header). */
+enum {
+    FLIPMI_GRID_LIQUID_PHI = 0,
+    FLIPMI_GRID_SOLID_PHI = 1,
+    FLIPMI_GRID_U = 2,
+    FLIPMI_GRID_V = 3,
+    FLIPMI_GRID_W = 4,
+    FLIPMI_GRID_WEIGHT_U = 5,
+    FLIPMI_GRID_WEIGHT_V = 6,
+    FLIPMI_GRID_WEIGHT_W = 7,
+    FLIPMI_GRID_WEIGHT_C = 8,
+    FLIPMI_GRID_VALID_U = 9,   /* uint8 */
+    FLIPMI_GRID_VALID_V = 10,  /* uint8 */
+    FLIPMI_GRID_VALID_W = 11   /* uint8 */
+};
+
+/* ---- context ----------------------------------------------------------- */
+int flipmi_ctx_create(int device, flipmi_ctx **out);
+int flipmi_ctx_destroy(flipmi_ctx *ctx);
+const char *flipmi_last_error(const flipmi_ctx *ctx);
+void flipmi_scene_desc_default(flipmi_scene_desc *desc);
+
+/* ---- FluidSimulation (fluidsimulation.h:127-153) ------------------------ */
+/* FluidSimulation::FluidSimulation(int, int, int, double) */
+int flipmi_scene_create(flipmi_ctx *ctx, const flipmi_scene_desc *desc, flipmi_scene **out);
+int flipmi_scene_destroy(flipmi_scene *scene);
+/* FluidSimulation::loadMarkerParticleData (fluidsimulation.cpp:2488-2509);
+ * call before flipmi_scene_initialize.  vel may be NULL (zero velocity). */
+int flipmi_scene_load_particles(flipmi_scene *scene, const float *pos_xyz, const float *vel_xyz,
+                                int64_t n);
+/* FluidSimulation::initialize (fluidsimulation.cpp:82-88) */
+int flipmi_scene_initialize(flipmi_scene *scene);
+/* FluidSimulation::update(double dt) (fluidsimulation.cpp:5755-5846).
+ * Returns FLIPMI_ERR_NOT_INITIALIZED before initialize and
+ * FLIPMI_ERR_INVALID_ARG for dt < 0, as the reference throws.  A failed
+ * pressure solve leaves the velocity unprojected (as the reference does) and
+ * is reported in stats->pressure_ok, not as an error. stats may be NULL. */
+int flipmi_scene_update(flipmi_scene *scene, double dt, flipmi_frame_stats *stats);
+/* FluidSimulation::getNumMarkerParticles */
+int flipmi_scene_num_particles(flipmi_scene *scene, int64_t *n);
+/* FluidSimulation::getMarkerParticlePositionData / VelocityData
+ * (fluidsimulation.cpp:2408-2420); either pointer may be NULL. */
+int flipmi_scene_get_particles(flipmi_scene *scene, float *pos_xyz, float *vel_xyz);
+/* Copies an internal grid (FLIPMI_GRID_*) to host memory, e.g.
+ * FluidSimulation::getVelocityField (fluidsimulation.cpp:2242). */
+int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out);
+int flipmi_scene_stage_times(flipmi_scene *scene, flipmi_stage_times *out, int reset);
+/* Enables hipEvent timing of each stage (adds one event pair per stage). */
+int flipmi_scene_set_profiling(flipmi_scene *scene, int enable);
+
+/* ---- stage drop-ins on host buffers ------------------------------------ */
+/* Domain-wall solid SDF (nodal) and the weight grid: FluidSimulation::
+ * _addStaticObjectsToSDF + _updateWeightGrid (fluidsimulation.cpp:2918-2930,
+ * 3646-3730).  Weight pointers may be NULL. */
+int flipmi_domain_solid(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx,
+                        float *phi_nodal, float *wU, float *wV, float *wW, float *wC);
+
+/* ParticleLevelSet::calculateSignedDistanceField + postProcessSignedDistanceField
+ * (particlelevelset.cpp:161-196).  solid_phi_nodal may be NULL (no post-process). */
+int flipmi_liquid_sdf(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx,
+                      const float *pos_xyz, int64_t n, double radius,
+                      const float *solid_phi_nodal, float *phi_out);
+
+/* VelocityAdvector::advect(VelocityAdvectorParameters) (velocityadvector.h:75)
+ * onto a cleared MAC grid: velocities and valid flags of U, V and W. */
+int flipmi_p2g(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx, const float *pos_xyz,
+               const float *vel_xyz, int64_t n, double radius, float *u, float *v, float *w,
+               uint8_t *valid_u, uint8_t *valid_v, uint8_t *valid_w);
+
+/* MACVelocityField::extrapolateVelocityField / GridUtils::extrapolateGrid
+ * (gridutils.cpp:32-228) on one component grid, in place. */
+int flipmi_extrapolate(flipmi_ctx *ctx, int ni, int nj, int nk, float *grid,
+                       const uint8_t *valid, int layers);
+
+/* PressureSolverParameters (pressuresolver.h:63-79). */
+typedef struct {
+    int isize, jsize, ksize;
+    double dx, dt;
+    double tolerance;          /* relative to |rhs|_inf */
+    double acceptable_tolerance; /* accept a capped solve below this |r|_inf (1.0) */
+    int max_iterations;
+    int precision;             /* flipmi_pcg_precision */
+    const float *liquid_phi;   /* I x J x K */
+    const float *weight_u, *weight_v, *weight_w, *weight_c;
+    const float *solid_u, *solid_v, *solid_w;  /* solid face velocities, NULL = 0 */
+    float *u, *v, *w;          /* MAC velocity, updated in place */
+    uint8_t *valid_u, *valid_v, *valid_w;      /* updated in place */
+    double *pressure_rows;     /* optional out: solution in reference row order */
+} flipmi_pressure_args;
+
+typedef struct {
+    int iterations;            /* "Pressure Solver Iterations" */
+    double residual;           /* "Estimated Error" (|r|_inf) */
+    int rows;                  /* fluid interior cells */
+    int success;               /* the bool PressureSolver::solve returns */
+    int refinements;           /* mixed precision: outer refinement passes */
+} flipmi_pressure_report;
+
+/* bool PressureSolver::solve(PressureSolverParameters) + getSolverStatus()
+ * (pressuresolver.h:91-92).  Returns FLIPMI_OK when the solve succeeded (or
+ * was accepted), FLIPMI_ERR_NOT_CONVERGED otherwise (velocity unchanged). */
+int flipmi_pressure_solve(flipmi_ctx *ctx, const flipmi_pressure_args *args,
+                          flipmi_pressure_report *report);
+
+/* One application of the MIC(0) preconditioner z = M^-1 r on the system
+ * assembled from (liquid_phi, weights), vectors in reference row order
+ * (pcgsolver.h:69-221).  invdiag_out may be NULL. */
+int flipmi_mic0_apply(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx, double dt,
+                      const float *liquid_phi, const float *wU, const float *wV,
+                      const float *wW, const float *wC, const double *r_rows, double *z_rows,
+                      double *invdiag_rows);
+
+/* ParticleAdvector::advectParticlesRK4/RK3/RK2/RK1 (particleadvector.h:53-71):
+ * rk_order in {1, 2, 3, 4}. */
+int flipmi_advect_particles(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx,
+                            const float *u, const float *v, const float *w,
+                            const float *pos_in, int64_t n, double dt, int rk_order,
+                            float *pos_out);
+
+/* ParticleAdvector::trilinearInterpolate (particleadvector.h:73-79). */
+int flipmi_trilinear_velocity(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx,
+                              const float *u, const float *v, const float *w,
+                              const float *pos, int64_t n, float *vel_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLIPMI_H */
