@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""FLIP step throughput on one MI355X: particle-substeps/s.
+
+Workload: the 256x128x128 double dam break (dx = 1/128, a 2 x 1 x 1 m box
+with solid walls; two cuboids x in [0,0.5] and [1.5,2.0], y in [0,0.6],
+full z, 8 jittered particles per cell, seed 3, gravity -9.81).  One step is
+one FluidSimulation::update(1/240) with min = max substeps = 1, i.e. exactly
+one substep of the whole FLIP pipeline (sort, classify, P2G, extrapolate,
+assemble, MIC(0), PCG, apply, G2P + RK3 advection, collision, removal) over
+every particle.  Particles and grids are resident in HBM before the timed
+region; nothing is copied to the host inside it except the per-substep
+scalars the step itself needs (speed maximum, histogram, PCG status).
+
+Prints ONE JSON line.  Besides the contract keys it carries
+  roofline:     the dominant kernel's algorithmic HBM bytes per launch over
+                its average launch time (hipEvents on the scene's stream)
+                against the MI355X peak of 8 TB/s;
+  cpu_baseline: the reference CPU engine (oracle/_ref, when built) or the
+                CPU oracle timed on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per matrix row (fluid cell) of each PCG kernel: every
+# logically required vector / coefficient read or written once.  Vectors and
+# coefficients are 8 bytes on the FP64 path and 4 bytes on the mixed path.
+#   spmv_dot:   s, diag, 3 upper couplings (the lower three are the
+#               neighbours' upper ones), z               -> 6 words
+#   axpy_norm:  x r/w, s, r r/w, z                       -> 6 words
+#   forward:    r, invdiag, 3 lower factor entries, z    -> 6 words
+#   backward:   z, invdiag, 3 upper factor entries, z    -> 6 words
+#   update_s:   z, s r/w                                 -> 3 words
+PCG_WORDS = {"k_spmv_dot": 6, "k_axpy_norm": 6, "k_forward": 6, "k_backward": 6, "k_update_s": 3}
+
+
+def parse_args(argv=None):
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--size", type=int, nargs=3, default=[256, 128, 128], metavar=("I", "J", "K"))
+    p.add_argument("--dt", type=float, default=1.0 / 240.0)
+    p.add_argument("--seed", type=int, default=3)
+    p.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    p.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
+    p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpus available)")
+    p.add_argument("--cpu-seconds", type=float, default=20.0,
+                   help="stop the CPU sample after the first substep that ends past this many seconds")
+    p.add_argument("--verbose", action="store_true")
+    return p.parse_args(argv)
+
+
+def log(msg, args):
+    if args.verbose:
+        print(msg, file=sys.stderr, flush=True)
+
+
+def make_sim(fa, scene, args):
+    I, J, K = args.size
+    sim = fa.FluidSimulation(I, J, K, scene.dx)
+    sim.addBodyForce(0.0, -9.81, 0.0)
+    sim.setMinTimeStepsPerFrame(1)
+    sim.setMaxTimeStepsPerFrame(1)
+    sim.setPressureSolverPrecision(fa.PCG_MIXED if args.precision == "mixed" else fa.PCG_FP64)
+    sim.loadMarkerParticleData(scene.positions, scene.velocities)
+    sim.initialize()
+    return sim
+
+
+def cpu_threads(args):
+    if args.cpu_threads > 0:
+        return args.cpu_threads
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(scene, args):
+    """Times the reference engine (or, where it was not built, the CPU
+    oracle) on the same scene for whole substeps until --cpu-seconds pass."""
+    from oracle import bindings
+    I, J, K = args.size
+    if bindings.ref_available():
+        threads = cpu_threads(args)
+        lib = bindings.Reference(threads=threads)
+        kind = "reference"
+    else:
+        threads = 1
+        lib = bindings.Oracle()
+        kind = "port"
+    h = lib.sim(I, J, K, scene.dx)
+    h.add_body_force(0.0, -9.81, 0.0)
+    h.set_substeps(1, 1)
+    h.load_particles(scene.positions, scene.velocities)
+    h.initialize()
+    done = 0
+    work = 0
+    elapsed = 0.0
+    while True:
+        n = h.num_particles
+        t0 = time.perf_counter()
+        h.update(args.dt)
+        elapsed += time.perf_counter() - t0
+        work += n
+        done += 1
+        log(f"cpu substep {done}: {n} particles, {elapsed:.1f} s total", args)
+        if elapsed >= args.cpu_seconds or done >= 8:
+            break
+    h.close()
+    return {
+        "value": work / elapsed,
+        "unit": "particle-substeps/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": (f"{done} substep(s) of dt={args.dt:.6g} s of the same {I}x{J}x{K} double dam break "
+                   f"({scene.num_particles} particles), FP64 PCG, {threads} thread(s), {elapsed:.1f} s"),
+    }
+
+
+def main(argv=None):
+    args = parse_args(argv)
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the FLIP step runs one scene per device and this build has no "
+              "multi-GPU path (a single scene would shard as x-slabs with halo exchange; serving shards "
+              "whole scenes). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import flipengine3d_amd as fa
+    from flipengine3d_amd import scenes
+
+    I, J, K = args.size
+    dx = 1.0 / 128.0 if (I, J, K) == (256, 128, 128) else 1.0 / max(J, K)
+    t = time.perf_counter()
+    scene = scenes.double_dam_break(I, J, K, dx, seed=args.seed)
+    log(f"scene: {scene.num_particles} particles ({time.perf_counter() - t:.1f} s)", args)
+
+    sim = make_sim(fa, scene, args)
+    sim.set_profiling(2)
+    n = sim.getNumMarkerParticles()
+    for w in range(args.warmup):
+        st = sim.update(args.dt)
+        log(f"warmup {w}: {st.pcg_iterations} PCG iterations, {st.gpu_ms:.1f} ms", args)
+        n = st.num_particles
+    sim.stage_times(reset=True)
+
+    lat = []
+    work = 0
+    rows_iters = 0
+    iters = 0
+    ok = True
+    sim.ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        st = sim.update(args.dt)
+        lat.append(time.perf_counter() - ts)
+        work += n * st.substeps
+        n = st.num_particles
+        rows_iters += st.pressure_rows * st.pcg_iterations
+        iters += st.pcg_iterations
+        ok = ok and st.pressure_ok == 1
+    sim.ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    times = sim.stage_times()
+
+    # Dominant kernel: the PCG kernel with the largest total time.
+    kern = max(PCG_WORDS, key=lambda k: times[k][0])
+    kms, kcalls = times[kern]
+    word = 4 if args.precision == "mixed" else 8
+    rows_per_launch = rows_iters / max(iters, 1)
+    bytes_per_launch = PCG_WORDS[kern] * word * rows_per_launch
+    launch_s = kms * 1e-3 / max(kcalls, 1)
+    achieved = bytes_per_launch / launch_s / 1e9 if launch_s > 0 else 0.0
+    stages = {k: round(v[0] / args.steps, 3) for k, v in times.items() if k != "pcg_iters"}
+
+    lat_ms = sorted(x * 1e3 for x in lat)
+    p99 = lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))]
+    result = {
+        "metric": "particle-substeps/s @256x128x128 + PCG HBM GB/s; p50/p99 step latency",
+        "value": work / elapsed,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "p50_ms": statistics.median(lat_ms),
+        "p99_ms": p99,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 particles/grids, f64 PCG" if args.precision == "fp64" else "f32 PCG, f64 dots/refinement",
+        "data": "synthetic",
+        "config": {
+            "workload": f"double dam break {I}x{J}x{K}, dx=1/{round(1 / dx)}, seed {args.seed}, "
+                        f"one substep of dt={args.dt:.6g} s per step",
+            "particles": scene.num_particles,
+            "pcg_precision": args.precision,
+            "pcg_iterations_per_step": iters / args.steps,
+            "pressure_rows": round(rows_per_launch),
+            "pressure_ok": ok,
+        },
+        "roofline": {
+            "kernel": f"k_wavefront<{'forward' if kern == 'k_forward' else 'backward'}>"
+            if kern in ("k_forward", "k_backward") else kern,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": launch_s * 1e6,
+            "launches": kcalls,
+        },
+        "stage_ms_per_step": stages,
+    }
+    sim.close()
+    if args.cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(scene, args)
+        result["config"]["gpu_over_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

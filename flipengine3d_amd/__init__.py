@@ -35,7 +35,9 @@ PCG_FP64 = 0
 PCG_MIXED = 1
 
 STAGES = ["sort", "classify", "p2g", "extrapolate", "assemble", "mic0", "pcg", "apply",
-          "g2p_advect", "remove", "glue", "pcg_iters"]
+          "g2p_advect", "remove", "glue", "pcg_iters", "k_spmv_dot", "k_axpy_norm", "k_forward",
+          "k_backward", "k_update_s"]
+NUM_STAGES = 24
 
 _c_int = ctypes.c_int
 _c_i64 = ctypes.c_int64
@@ -53,11 +55,12 @@ class SceneDesc(ctypes.Structure):
 
 class FrameStats(ctypes.Structure):
     _fields_ = [("substeps", _c_int), ("num_particles", _c_i64), ("pcg_iterations", _c_int),
-                ("pcg_residual", _c_double), ("pressure_ok", _c_int), ("gpu_ms", _c_double)]
+                ("pcg_residual", _c_double), ("pressure_ok", _c_int), ("gpu_ms", _c_double),
+                ("pressure_rows", _c_i64)]
 
 
 class StageTimes(ctypes.Structure):
-    _fields_ = [("ms", _c_double * 16), ("calls", _c_i64 * 16)]
+    _fields_ = [("ms", _c_double * 24), ("calls", _c_i64 * 24)]
 
 
 class PressureArgs(ctypes.Structure):
@@ -76,7 +79,7 @@ class PressureReport(ctypes.Structure):
 
 
 EXPORTED = [
-    "flipmi_ctx_create", "flipmi_ctx_destroy", "flipmi_last_error", "flipmi_scene_desc_default",
+    "flipmi_ctx_create", "flipmi_ctx_destroy", "flipmi_ctx_synchronize", "flipmi_last_error", "flipmi_scene_desc_default",
     "flipmi_scene_create", "flipmi_scene_destroy", "flipmi_scene_load_particles",
     "flipmi_scene_initialize", "flipmi_scene_update", "flipmi_scene_num_particles",
     "flipmi_scene_get_particles", "flipmi_scene_get_grid", "flipmi_scene_stage_times",
@@ -99,6 +102,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     P = ctypes.POINTER
     L.flipmi_ctx_create.argtypes = [_c_int, P(_vp)]
     L.flipmi_ctx_destroy.argtypes = [_vp]
+    L.flipmi_ctx_synchronize.argtypes = [_vp]
     L.flipmi_last_error.argtypes = [_vp]
     L.flipmi_last_error.restype = ctypes.c_char_p
     L.flipmi_scene_desc_default.argtypes = [P(SceneDesc)]
@@ -157,6 +161,9 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def synchronize(self):
+        self.check(self.L.flipmi_ctx_synchronize(self.h))
 
     def last_error(self) -> str:
         m = self.L.flipmi_last_error(self.h)
@@ -317,6 +324,12 @@ class FluidSimulation:
         self.desc.pcg_tolerance = tol
         self.desc.pcg_max_iterations = max_iterations
 
+    def setPressureSolverPrecision(self, precision: int):
+        """PCG_FP64 (the reference's arithmetic) or PCG_MIXED."""
+        if self.h is not None:
+            raise RuntimeError("the solver precision must be set before particles are loaded")
+        self.desc.pcg_precision = precision
+
     def loadMarkerParticleData(self, positions: np.ndarray, velocities: Optional[np.ndarray] = None):
         self._ensure()
         pos = np.ascontiguousarray(positions, np.float32)
@@ -335,9 +348,10 @@ class FluidSimulation:
         self.last_stats = st
         return st
 
-    def set_profiling(self, on: bool = True):
+    def set_profiling(self, level=True):
         self._ensure()
-        self.ctx.check(self.L.flipmi_scene_set_profiling(self.h, 1 if on else 0))
+        lvl = int(level) if not isinstance(level, bool) else (1 if level else 0)
+        self.ctx.check(self.L.flipmi_scene_set_profiling(self.h, lvl))
 
     def stage_times(self, reset: bool = False) -> dict:
         t = StageTimes()

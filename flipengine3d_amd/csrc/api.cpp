@@ -122,6 +122,14 @@ int flipmi_ctx_destroy(flipmi_ctx *ctx) {
     return FLIPMI_OK;
 }
 
+int flipmi_ctx_synchronize(flipmi_ctx *ctx) {
+    if (!ctx) return FLIPMI_ERR_INVALID_ARG;
+    return guarded(&ctx->c, [&] {
+        hipck(hipSetDevice(ctx->c.device), "set device");
+        hipck(hipDeviceSynchronize(), "device synchronize");
+    });
+}
+
 const char *flipmi_last_error(const flipmi_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : ""; }
 
 int flipmi_scene_create(flipmi_ctx *ctx, const flipmi_scene_desc *desc, flipmi_scene **out) {
@@ -134,6 +142,8 @@ int flipmi_scene_create(flipmi_ctx *ctx, const flipmi_scene_desc *desc, flipmi_s
         if (!(desc->cfl > 0)) throw StatusError{FLIPMI_ERR_INVALID_ARG, "CFL number must be positive"};
         if (desc->pcg_max_iterations < 1)
             throw StatusError{FLIPMI_ERR_INVALID_ARG, "pcg_max_iterations must be positive"};
+        if (desc->pcg_precision != FLIPMI_PCG_FP64)
+            throw StatusError{FLIPMI_ERR_INVALID_ARG, "only FLIPMI_PCG_FP64 is available"};
         std::unique_ptr<flipmi_scene> sc(new flipmi_scene());
         Scene &s = sc->s;
         s.ctx = &ctx->c;
@@ -231,7 +241,7 @@ int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out) {
 int flipmi_scene_stage_times(flipmi_scene *scene, flipmi_stage_times *out, int reset) {
     if (!scene || !out) return FLIPMI_ERR_INVALID_ARG;
     Scene &s = scene->s;
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < FLIPMI_NUM_STAGES; q++) {
         out->ms[q] = s.stage_ms[q];
         out->calls[q] = s.stage_calls[q];
         if (reset) {
@@ -244,7 +254,7 @@ int flipmi_scene_stage_times(flipmi_scene *scene, flipmi_stage_times *out, int r
 
 int flipmi_scene_set_profiling(flipmi_scene *scene, int enable) {
     if (!scene) return FLIPMI_ERR_INVALID_ARG;
-    scene->s.profiling = enable != 0;
+    scene->s.profiling = enable < 0 ? 0 : enable;
     return FLIPMI_OK;
 }
 
@@ -466,7 +476,7 @@ int flipmi_mic0_apply(flipmi_ctx *ctx, int I, int J, int K, double dx, double dt
         hipck(launch_pressure_assemble(g, dt, liq.p, wu.p, wv.p, ww.p, wc.p, zero_u.p, zero_v.p, zero_w.p,
                                        zero_u.p, zero_v.p, zero_w.p, ps.pb, s),
               "assemble");
-        hipck(launch_mic0_factor(g, ps.pb, s), "factor");
+        hipck(launch_mic0_factor(g, liq.p, ps.pb, s), "factor");
         // Scatter r from reference row order into the dense layout.
         std::vector<float> hl(nc);
         download(hl.data(), liq, nc, s);
@@ -482,6 +492,7 @@ int flipmi_mic0_apply(flipmi_ctx *ctx, int I, int J, int K, double dx, double dt
         DBuf<double> dr, dz;
         upload(dr, rd.data(), nc, s);
         alloc_dev(dz, nc);
+        hipck(launch_fill_f64(dz.p, nc, 0.0, s), "fill");
         hipck(launch_precond_apply(g, ps.pb, dr.p, dz.p, s), "precond");
         std::vector<double> zd(nc), inv(nc);
         download(zd.data(), dz, nc, s);

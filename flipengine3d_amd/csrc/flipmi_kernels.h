@@ -83,6 +83,7 @@ struct PcgScalars {
     float alpha_f, beta_f;
     int iters, done, converged, failed, abort, maxiter, precision;
     unsigned int counter[4];
+    unsigned int rows;                  // rows of the system (fluid interior cells)
     unsigned long long absmax_bits;
 };
 
@@ -96,6 +97,7 @@ struct PressureBuffers {
     unsigned int *progress;             // wavefront tile progress counters
     int ntiles_j, ntiles_k;
     unsigned int *ticket;
+    int *tile_irange;                   // per (j,k) tile: min/max i of its rows (-1: none)
     PcgScalars *sc;                     // device scalars
 };
 
@@ -105,10 +107,15 @@ hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *l
                                     const float *w, const float *sU, const float *sV,
                                     const float *sW, PressureBuffers &pb, hipStream_t s);
 hipError_t launch_pressure_reset(PressureBuffers &pb, hipStream_t s);
-hipError_t launch_mic0_factor(const GridGeom &g, PressureBuffers &pb, hipStream_t s);
+// Zeroes the factor, finds each column tile's row extent and factorises.
+hipError_t launch_mic0_factor(const GridGeom &g, const float *liquid, PressureBuffers &pb,
+                              hipStream_t s);
 hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, PressureBuffers &pb,
                            hipStream_t s);
 hipError_t launch_pcg_iteration(const GridGeom &g, PressureBuffers &pb, hipStream_t s);
+// The five kernels of one PCG iteration, in order.
+enum { kPcgSpmv = 0, kPcgAxpy = 1, kPcgForward = 2, kPcgBackward = 3, kPcgUpdate = 4, kPcgPieces = 5 };
+hipError_t launch_pcg_piece(int piece, const GridGeom &g, PressureBuffers &pb, hipStream_t s);
 hipError_t launch_pressure_apply(const GridGeom &g, double dt, const float *liquid,
                                  const float *wU, const float *wV, const float *wW,
                                  const double *p, float *u, float *v, float *w, uint8_t *vu,

@@ -182,9 +182,22 @@ __device__ F3 resolve_collision(const ParticleStepArgs &a, const GridGeom &g, co
     return resolved;
 }
 
+__device__ __forceinline__ void particle_step_one(const GridGeom &g, const ParticleStepArgs &a, int i,
+                                                  int *hist);
+
 __global__ __launch_bounds__(256) void k_particle_step(GridGeom g, ParticleStepArgs a) {
+    __shared__ int hist[64];
+    for (int q = threadIdx.x; q < a.maxsteps; q += blockDim.x) hist[q] = 0;
+    __syncthreads();
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    if (i < a.n) particle_step_one(g, a, i, hist);
+    __syncthreads();
+    for (int q = threadIdx.x; q < a.maxsteps; q += blockDim.x)
+        if (hist[q]) atomicAdd(&a.speed_hist[q], hist[q]);
+}
+
+__device__ __forceinline__ void particle_step_one(const GridGeom &g, const ParticleStepArgs &a, int i,
+                                                  int *hist) {
     MacView mac{a.u, a.v, a.w}, saved{a.su, a.sv, a.sw};
     F3 p = f3(a.px[i], a.py[i], a.pz[i]);
     F3 vel = f3(a.vx[i], a.vy[i], a.vz[i]);
@@ -202,7 +215,7 @@ __global__ __launch_bounds__(256) void k_particle_step(GridGeom g, ParticleStepA
     a.solid_removed[i] = solid_trilinear(a.solid, g, p1) < 0.0f ? 1 : 0;
     double speed = (double)len3(vnew);
     int idx = (int)fmin(floor(speed / a.speed_step), (double)(a.maxsteps - 1));
-    atomicAdd(&a.speed_hist[idx], 1);
+    atomicAdd(&hist[idx], 1);
 }
 
 hipError_t launch_particle_step(const GridGeom &g, const ParticleStepArgs &a, hipStream_t s) {

@@ -99,6 +99,7 @@ __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liqu
         // max|rhs| as an order-independent integer maximum over IEEE bits.
         unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(rhs));
         atomicMax(&pb.sc->absmax_bits, bits);
+        atomicAdd(&pb.sc->rows, 1u);
     }
     pb.rhs[c] = rhs;
     pb.diag[c] = diag;
@@ -125,23 +126,52 @@ hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *l
 enum { kFactor = 0, kForward = 1, kBackward = 2 };
 enum { kDotNone = 0, kDotInit = 1, kDotIter = 2 };
 
-__device__ __forceinline__ unsigned load_acquire(const unsigned *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Cross-workgroup exchange without cache maintenance.  Values that another
+// workgroup reads within the same kernel are stored and loaded as relaxed
+// device-scope atomics, which are coherent across XCDs by themselves; a
+// producer orders its data stores before its progress flag by draining its
+// outstanding memory operations (s_waitcnt), and a consumer issues its data
+// loads only after the flag load has returned.  No acquire/release fence is
+// needed, so the sweeps never write back or invalidate the L2 caches.
+__device__ __forceinline__ double ld_dev(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void store_release(unsigned *p, unsigned v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_dev(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_dev(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(unsigned *p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_dev(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until every memory operation this wave issued has completed.
+__device__ __forceinline__ void drain_memory() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
-// Waits until tile `up` has completed `need` steps.  Returns false if the
+// Waits until the upstream tiles (index < 0: none) have completed `need`
+// steps; both flags are polled in one round trip.  Returns false if the
 // bounded spin ran out or another wave aborted.
-__device__ bool wait_progress(PressureBuffers &pb, int up, unsigned need) {
+__device__ bool wait_progress2(PressureBuffers &pb, int up0, int up1, unsigned need) {
     unsigned spins = 0;
-    while (load_acquire(&pb.progress[up]) < need) {
-        if (load_acquire((const unsigned *)&pb.sc->abort) != 0) return false;
+    for (;;) {
+        unsigned p0 = up0 >= 0 ? ld_dev(&pb.progress[up0]) : need;
+        unsigned p1 = up1 >= 0 ? ld_dev(&pb.progress[up1]) : need;
+        if (p0 >= need && p1 >= need) break;
+        if (ld_dev(&pb.sc->abort) != 0) return false;
         if (++spins > kSpinLimit) {
-            __hip_atomic_store(&pb.sc->abort, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            st_dev(&pb.sc->abort, 1);
             return false;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -155,15 +185,61 @@ __device__ __forceinline__ double wave_sum_fixed(double v) {
     return v;
 }
 
-// MODE kFactor : computes invdiag, Lx, Ly, Lz from diag/offx/offy/offz.
+// Extent of each 8 x 8 column tile's rows along i, so sweeps can skip the
+// leading and trailing steps where a tile has nothing to do.
+__global__ __launch_bounds__(64) void k_tile_irange(GridGeom g, const float *__restrict__ liquid,
+                                                    PressureBuffers pb) {
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    const int jt = tile % pb.ntiles_j, kt = tile / pb.ntiles_j;
+    const int j = jt * kTile + (lane & 7), k = kt * kTile + (lane >> 3);
+    int lo = 1 << 30, hi = -1;
+    if (j >= 1 && k >= 1 && j <= g.J - 2 && k <= g.K - 2) {
+        const float *row = liquid + (size_t)g.I * ((size_t)j + (size_t)g.J * k);
+        for (int i = 1; i <= g.I - 2; i++) {
+            if (row[i] < 0) {
+                lo = min(lo, i);
+                hi = max(hi, i);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off, 64));
+        hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    if (lane == 0) {
+        pb.tile_irange[2 * tile] = lo;
+        pb.tile_irange[2 * tile + 1] = hi;
+    }
+}
+
+// MODE kFactor : computes invdiag, Lx, Ly, Lz from diag/offx/offy/offz
+//                (the factor arrays must be zero on entry).
 // MODE kForward: out = L^-1 in        (solveLower, pcgsolver.h:191-204)
 // MODE kBackward: out = L^-T out (in place), optional dot(out, rvec)
 //                 (solveLowerTransposeInPlace, pcgsolver.h:207-221).
+// Cells without rows are never written: `out` and the factor hold zeros there.
+//
+// The march is software-pipelined: the per-cell operands of the next chunk
+// of kChunk steps are loaded while the current chunk computes, and the
+// values a tile reads from its upstream neighbours are fetched once per
+// chunk, after the neighbours have published enough progress.
+constexpr int kChunk = 8;  // 5-6 loads per step: keeps a chunk's prefetch under the 63 outstanding loads vmcnt tracks
+
+template <int MODE>
+struct SweepVals {
+    static constexpr int NV = MODE == kFactor ? 4 : (MODE == kForward ? 5 : 6);
+    static constexpr int NE = MODE == kFactor ? 3 : 1;
+};
+
 template <int MODE>
 __global__ __launch_bounds__(64) void k_wavefront(GridGeom g, PressureBuffers pb,
                                                   const double *__restrict__ in, double *out,
                                                   const double *__restrict__ rvec, int dotmode) {
-    if (load_acquire((const unsigned *)&pb.sc->done) != 0) return;
+    constexpr int C = kChunk;
+    constexpr int NV = SweepVals<MODE>::NV;
+    constexpr int NE = SweepVals<MODE>::NE;
+    if (ld_dev(&pb.sc->done) != 0) return;
     const int lane = threadIdx.x;
     const int ntj = pb.ntiles_j, ntk = pb.ntiles_k, ntiles = ntj * ntk;
     int ticket = 0;
@@ -180,126 +256,193 @@ __global__ __launch_bounds__(64) void k_wavefront(GridGeom g, PressureBuffers pb
     const int tile = jt + ntj * kt;
     const int a = lane & 7, b = lane >> 3;
     const int j = jt * kTile + a, k = kt * kTile + b;
-    const bool lane_ok = j < g.J && k < g.K;
+    const bool col_ok = j >= 1 && k >= 1 && j <= g.J - 2 && k <= g.K - 2;
     const int up_j = MODE == kBackward ? (jt + 1 < ntj ? tile + 1 : -1) : (jt > 0 ? tile - 1 : -1);
     const int up_k = MODE == kBackward ? (kt + 1 < ntk ? tile + ntj : -1) : (kt > 0 ? tile - ntj : -1);
     const int skew = MODE == kBackward ? (7 - a) + (7 - b) : a + b;
-    const int src_j = MODE == kBackward ? lane + 1 : lane - 1;   // neighbour lane along j
-    const int src_k = MODE == kBackward ? lane + 8 : lane - 8;   // neighbour lane along k
-    const bool edge_j = MODE == kBackward ? (a == 7) : (a == 0);
+    const int src_j = (MODE == kBackward ? lane + 1 : lane - 1) & 63;  // neighbour lane along j
+    const int src_k = (MODE == kBackward ? lane + 8 : lane - 8) & 63;  // neighbour lane along k
+    const bool edge_j = MODE == kBackward ? (a == 7) : (a == 0);       // reads the upstream tile
     const bool edge_k = MODE == kBackward ? (b == 7) : (b == 0);
-    const size_t sj = (size_t)g.I, sk = (size_t)g.I * g.J;
-    const int dir = MODE == kBackward ? 1 : -1;  // offset to the dependency cells
+    // Lanes whose results a downstream tile reads publish them coherently.
+    const bool pub = MODE == kBackward ? (a == 0 || b == 0) : (a == 7 || b == 7);
+    const long sj = (long)g.I, sk = (long)g.I * g.J;
+    const long base = sj * j + sk * k;
+    const long dy = MODE == kBackward ? sj : -sj, dz = MODE == kBackward ? sk : -sk;
     const unsigned total = (unsigned)(g.I + 14);
-    const int chunk = 8;
+    const int imin = pb.tile_irange[2 * tile], imax = pb.tile_irange[2 * tile + 1];
 
-    // Per-lane carried state: the value(s) computed at the previous step.
-    double cur = 0.0, curLx = 0.0, curLy = 0.0, curLz = 0.0;
     double dot_acc = 0.0;
     bool ok = true;
+    if (imax >= 0) {
+        // Steps before tb and from te on touch no row of this tile.
+        const unsigned tb = MODE == kBackward ? (unsigned)(g.I - 1 - imax) : (unsigned)imin;
+        unsigned te = (MODE == kBackward ? (unsigned)(g.I - 1 - imin) : (unsigned)imax) + 15;
+        if (te > total) te = total;
+        if (lane == 0) st_dev(&pb.progress[tile], tb);
 
-    for (unsigned t0 = 0; t0 < total; t0 += chunk) {
-        unsigned need = t0 + chunk + 7;
-        if (need > total) need = total;
-        if (up_j >= 0) ok = ok && wait_progress(pb, up_j, need);
-        if (up_k >= 0) ok = ok && wait_progress(pb, up_k, need);
-        if (!ok) break;
-        unsigned t1 = t0 + chunk < total ? t0 + chunk : total;
-        for (unsigned t = t0; t < t1; t++) {
-            // Values produced one step earlier by the neighbouring lanes.
-            double nj_cur = shfl_d(cur, src_j & 63), nk_cur = shfl_d(cur, src_k & 63);
-            double nj_Lx = 0, nj_Ly = 0, nj_Lz = 0, nk_Lx = 0, nk_Ly = 0, nk_Lz = 0;
-            if (MODE == kFactor) {
-                nj_Lx = shfl_d(curLx, src_j & 63); nj_Ly = shfl_d(curLy, src_j & 63);
-                nj_Lz = shfl_d(curLz, src_j & 63); nk_Lx = shfl_d(curLx, src_k & 63);
-                nk_Ly = shfl_d(curLy, src_k & 63); nk_Lz = shfl_d(curLz, src_k & 63);
-            }
+        // Cell index of this lane at step t, or -1 if it is not interior.
+        auto cell_at = [&](unsigned t) -> long {
             int step = (int)t - skew;
             int i = MODE == kBackward ? g.I - 1 - step : step;
-            if (!lane_ok || step < 0 || step >= g.I) {
-                cur = 0.0; curLx = curLy = curLz = 0.0;
-                continue;
-            }
-            size_t c = (size_t)i + sj * (size_t)j + sk * (size_t)k;
-            bool interior = i >= 1 && j >= 1 && k >= 1 && i <= g.I - 2 && j <= g.J - 2 && k <= g.K - 2;
-            if (!interior) {
-                cur = 0.0; curLx = curLy = curLz = 0.0;
-                if (MODE != kFactor) out[c] = 0.0;
-                else { pb.invdiag[c] = 0.0; pb.Lx[c] = 0.0; pb.Ly[c] = 0.0; pb.Lz[c] = 0.0; }
-                continue;
-            }
-            size_t cy = MODE == kBackward ? c + sj : c - sj;
-            size_t cz = MODE == kBackward ? c + sk : c - sk;
-            if (MODE == kFactor) {
-                // Lower neighbours' factor columns: x from this lane's
-                // previous step, y/z from neighbouring lanes or tiles.
-                double yLx, yLy, yLz, zLx, zLy, zLz;
-                if (edge_j) { yLx = pb.Lx[cy]; yLy = pb.Ly[cy]; yLz = pb.Lz[cy]; }
-                else { yLx = nj_Lx; yLy = nj_Ly; yLz = nj_Lz; }
-                if (edge_k) { zLx = pb.Lx[cz]; zLy = pb.Ly[cz]; zLz = pb.Lz[cz]; }
-                else { zLx = nk_Lx; zLy = nk_Ly; zLz = nk_Lz; }
-                const double tau = 0.97, sigma = 0.25;  // pcgsolver.h:72-73
-                double A = pb.diag[c];
-                double d = A;
-                double L, missing;
-                L = zLz; missing = 0; missing += zLx; missing += zLy;   // -z neighbour
-                d -= L * L; d -= tau * L * missing;
-                L = yLy; missing = 0; missing += yLx; missing += yLz;   // -y neighbour
-                d -= L * L; d -= tau * L * missing;
-                L = curLx; missing = 0; missing += curLy; missing += curLz;  // -x
-                d -= L * L; d -= tau * L * missing;
-                double inv;
-                double fx = pb.offx[c], fy = pb.offy[c], fz = pb.offz[c];
-                if (A == 0) {
-                    inv = d;
-                } else {
-                    inv = 1 / sqrt(d < sigma * A ? A : d);
-                    fx = fx * inv; fy = fy * inv; fz = fz * inv;
+            if (!col_ok || step < 0 || step >= g.I || i < 1 || i > g.I - 2) return -1;
+            return base + i;
+        };
+        auto prefetch = [&](unsigned t0, double (&v)[NV][C]) {
+#pragma unroll
+            for (int u = 0; u < C; u++) {
+                long c = cell_at(t0 + u);
+                if (c < 0) {
+#pragma unroll
+                    for (int q = 0; q < NV; q++) v[q][u] = 0.0;
+                    continue;
                 }
-                pb.invdiag[c] = inv;
-                pb.Lx[c] = fx; pb.Ly[c] = fy; pb.Lz[c] = fz;
-                curLx = fx; curLy = fy; curLz = fz;
-            } else if (MODE == kForward) {
-                double qy = edge_j ? out[cy] : nj_cur;
-                double qz = edge_k ? out[cz] : nk_cur;
-                double q = in[c];
-                q -= pb.Lz[cz] * qz;
-                q -= pb.Ly[cy] * qy;
-                q -= pb.Lx[c - 1] * cur;
-                q *= pb.invdiag[c];
-                out[c] = q;
-                cur = q;
-            } else {
-                double yy = edge_j ? out[cy] : nj_cur;
-                double yz = edge_k ? out[cz] : nk_cur;
-                double q = out[c];
-                q -= pb.Lx[c] * cur;
-                q -= pb.Ly[c] * yy;
-                q -= pb.Lz[c] * yz;
-                q *= pb.invdiag[c];
-                out[c] = q;
-                cur = q;
-                if (dotmode != kDotNone) dot_acc += q * rvec[c];
+                if (MODE == kFactor) {
+                    v[0][u] = pb.diag[c]; v[1][u] = pb.offx[c];
+                    v[2][u] = pb.offy[c]; v[3][u] = pb.offz[c];
+                } else if (MODE == kForward) {
+                    v[0][u] = in[c]; v[1][u] = pb.Lz[c + dz]; v[2][u] = pb.Ly[c + dy];
+                    v[3][u] = pb.Lx[c - 1]; v[4][u] = pb.invdiag[c];
+                } else {
+                    v[0][u] = out[c]; v[1][u] = pb.Lx[c]; v[2][u] = pb.Ly[c];
+                    v[3][u] = pb.Lz[c]; v[4][u] = pb.invdiag[c];
+                    v[5][u] = dotmode != kDotNone ? rvec[c] : 0.0;
+                }
             }
+        };
+
+        double cur = 0.0, curLx = 0.0, curLy = 0.0, curLz = 0.0;
+        double nxt[NV][C];
+        prefetch(tb, nxt);
+        for (unsigned t0 = tb; t0 < te; t0 += C) {
+            const unsigned t1 = t0 + C < te ? t0 + C : te;
+            unsigned need = t1 + 7;
+            if (need > total) need = total;
+            double v[NV][C];
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+#pragma unroll
+                for (int u = 0; u < C; u++) v[q][u] = nxt[q][u];
+            if (up_j >= 0 || up_k >= 0) ok = wait_progress2(pb, up_j, up_k, need);
+            if (!ok) break;
+            // Upstream tiles' values for this chunk (one batch of loads).
+            double ey[NE][C], ez[NE][C];
+#pragma unroll
+            for (int u = 0; u < C; u++) {
+                long c = cell_at(t0 + u);
+                bool live = c >= 0 && t0 + u < t1;
+#pragma unroll
+                for (int q = 0; q < NE; q++) { ey[q][u] = 0.0; ez[q][u] = 0.0; }
+                if (live && edge_j) {
+                    if (MODE == kFactor) {
+                        ey[0][u] = ld_dev(&pb.Lx[c + dy]); ey[1][u] = ld_dev(&pb.Ly[c + dy]);
+                        ey[2][u] = ld_dev(&pb.Lz[c + dy]);
+                    } else {
+                        ey[0][u] = ld_dev(&out[c + dy]);
+                    }
+                }
+                if (live && edge_k) {
+                    if (MODE == kFactor) {
+                        ez[0][u] = ld_dev(&pb.Lx[c + dz]); ez[1][u] = ld_dev(&pb.Ly[c + dz]);
+                        ez[2][u] = ld_dev(&pb.Lz[c + dz]);
+                    } else {
+                        ez[0][u] = ld_dev(&out[c + dz]);
+                    }
+                }
+            }
+            // Next chunk's operands stream in while this chunk computes; loads
+            // complete in issue order, so the edge values above are not held
+            // up by them.
+            if (t0 + C < te) prefetch(t0 + C, nxt);
+#pragma unroll
+            for (int u = 0; u < C; u++) {
+                const unsigned t = t0 + u;
+                // Values produced one step earlier by the neighbouring lanes.
+                double nj_cur = shfl_d(cur, src_j), nk_cur = shfl_d(cur, src_k);
+                double nj_Lx = 0, nj_Ly = 0, nj_Lz = 0, nk_Lx = 0, nk_Ly = 0, nk_Lz = 0;
+                if (MODE == kFactor) {
+                    nj_Lx = shfl_d(curLx, src_j); nj_Ly = shfl_d(curLy, src_j);
+                    nj_Lz = shfl_d(curLz, src_j); nk_Lx = shfl_d(curLx, src_k);
+                    nk_Ly = shfl_d(curLy, src_k); nk_Lz = shfl_d(curLz, src_k);
+                }
+                if (t >= t1) continue;  // uniform across the wave
+                long c = cell_at(t);
+                if (c < 0) {
+                    cur = 0.0; curLx = curLy = curLz = 0.0;
+                    continue;
+                }
+                if (MODE == kFactor) {
+                    double yLx = edge_j ? ey[0][u] : nj_Lx, yLy = edge_j ? ey[1][u] : nj_Ly;
+                    double yLz = edge_j ? ey[2][u] : nj_Lz;
+                    double zLx = edge_k ? ez[0][u] : nk_Lx, zLy = edge_k ? ez[1][u] : nk_Ly;
+                    double zLz = edge_k ? ez[2][u] : nk_Lz;
+                    const double tau = 0.97, sigma = 0.25;  // pcgsolver.h:72-73
+                    double A = v[0][u];
+                    double d = A;
+                    double L, missing;
+                    L = zLz; missing = 0; missing += zLx; missing += zLy;   // -z neighbour
+                    d -= L * L; d -= tau * L * missing;
+                    L = yLy; missing = 0; missing += yLx; missing += yLz;   // -y neighbour
+                    d -= L * L; d -= tau * L * missing;
+                    L = curLx; missing = 0; missing += curLy; missing += curLz;  // -x neighbour
+                    d -= L * L; d -= tau * L * missing;
+                    double inv;
+                    double fx = v[1][u], fy = v[2][u], fz = v[3][u];
+                    if (A == 0) {
+                        inv = d;
+                    } else {
+                        inv = 1 / sqrt(d < sigma * A ? A : d);
+                        fx = fx * inv; fy = fy * inv; fz = fz * inv;
+                    }
+                    if (pub) {
+                        st_dev(&pb.invdiag[c], inv); st_dev(&pb.Lx[c], fx);
+                        st_dev(&pb.Ly[c], fy); st_dev(&pb.Lz[c], fz);
+                    } else {
+                        pb.invdiag[c] = inv; pb.Lx[c] = fx; pb.Ly[c] = fy; pb.Lz[c] = fz;
+                    }
+                    curLx = fx; curLy = fy; curLz = fz;
+                } else if (MODE == kForward) {
+                    double qy = edge_j ? ey[0][u] : nj_cur;
+                    double qz = edge_k ? ez[0][u] : nk_cur;
+                    double q = v[0][u];
+                    q -= v[1][u] * qz;
+                    q -= v[2][u] * qy;
+                    q -= v[3][u] * cur;
+                    q *= v[4][u];
+                    if (pub) st_dev(&out[c], q); else out[c] = q;
+                    cur = q;
+                } else {
+                    double yy = edge_j ? ey[0][u] : nj_cur;
+                    double yz = edge_k ? ez[0][u] : nk_cur;
+                    double q = v[0][u];
+                    q -= v[1][u] * cur;
+                    q -= v[2][u] * yy;
+                    q -= v[3][u] * yz;
+                    q *= v[4][u];
+                    if (pub) st_dev(&out[c], q); else out[c] = q;
+                    cur = q;
+                    dot_acc += q * v[5][u];
+                }
+            }
+            // Publish: this chunk's stores have completed before the flag moves.
+            drain_memory();
+            if (lane == 0) st_dev(&pb.progress[tile], t1);
         }
-        // Publish: every lane's stores of this chunk become visible first.
-        __threadfence();
-        if (lane == 0) store_release(&pb.progress[tile], t1);
     }
-    if (!ok && lane == 0) store_release(&pb.progress[tile], total);
+    drain_memory();
+    if (lane == 0) st_dev(&pb.progress[tile], total);
 
     if (MODE == kBackward && dotmode != kDotNone) {
         double part = wave_sum_fixed(dot_acc);
-        if (lane == 0) pb.partials[tile] = part;
+        if (lane == 0) st_dev(&pb.partials[tile], part);
     }
     // The last wave to finish reduces the dot product and resets the
     // progress counters and the ticket for the next launch.
-    __threadfence();
+    drain_memory();
     unsigned prev = 0;
     if (lane == 0) prev = atomicAdd(&pb.sc->counter[0], 1u);
     prev = __shfl(prev, 0, 64);
     if (prev != (unsigned)ntiles - 1) return;
-    __threadfence();
     for (int q = lane; q < ntiles; q += 64) pb.progress[q] = 0;
     if (lane == 0) {
         *pb.ticket = 0;
@@ -308,7 +451,7 @@ __global__ __launch_bounds__(64) void k_wavefront(GridGeom g, PressureBuffers pb
     if (MODE == kBackward && dotmode != kDotNone) {
         // Fixed-order sum of the per-tile partials.
         double acc = 0.0;
-        for (int q = lane; q < ntiles; q += 64) acc += pb.partials[q];
+        for (int q = lane; q < ntiles; q += 64) acc += ld_dev(&pb.partials[q]);
         double rho = wave_sum_fixed(acc);
         if (lane == 0) {
             PcgScalars *sc = pb.sc;
@@ -323,7 +466,6 @@ __global__ __launch_bounds__(64) void k_wavefront(GridGeom g, PressureBuffers pb
             }
         }
     }
-    __threadfence();
 }
 
 static hipError_t launch_wave(int mode, const GridGeom &g, PressureBuffers &pb, const double *in,
@@ -335,7 +477,15 @@ static hipError_t launch_wave(int mode, const GridGeom &g, PressureBuffers &pb, 
     return hipGetLastError();
 }
 
-hipError_t launch_mic0_factor(const GridGeom &g, PressureBuffers &pb, hipStream_t s) {
+hipError_t launch_mic0_factor(const GridGeom &g, const float *liquid, PressureBuffers &pb,
+                              hipStream_t s) {
+    size_t nc = (size_t)g.I * g.J * g.K;
+    hipError_t e;
+    if ((e = launch_fill_f64(pb.invdiag, nc, 0.0, s)) != hipSuccess) return e;
+    if ((e = launch_fill_f64(pb.Lx, nc, 0.0, s)) != hipSuccess) return e;
+    if ((e = launch_fill_f64(pb.Ly, nc, 0.0, s)) != hipSuccess) return e;
+    if ((e = launch_fill_f64(pb.Lz, nc, 0.0, s)) != hipSuccess) return e;
+    k_tile_irange<<<pb.ntiles_j * pb.ntiles_k, 64, 0, s>>>(g, liquid, pb);
     return launch_wave(kFactor, g, pb, nullptr, nullptr, nullptr, kDotNone, s);
 }
 
@@ -367,14 +517,15 @@ __device__ double block_reduce(double v, double *sh) {
     return r;
 }
 
-// Returns true in the one block that finished last; that block sees every
-// other block's partial.
+// Returns true in the one block that finished last.  Each block stores its
+// partial with a coherent store and drains it before taking a ticket, so the
+// last block reads every partial with coherent loads.
 __device__ bool last_block(unsigned *counter) {
     __shared__ bool is_last;
-    __threadfence();
+    drain_memory();
+    __syncthreads();
     if (threadIdx.x == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
     __syncthreads();
-    if (is_last) __threadfence();
     return is_last;
 }
 
@@ -414,10 +565,10 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBu
         acc += pb.s[c] * zc;
     }
     double bs = block_reduce<0>(acc, sh);
-    if (threadIdx.x == 0) pb.partials[blockIdx.x] = bs;
+    if (threadIdx.x == 0) st_dev(&pb.partials[blockIdx.x], bs);
     if (!last_block(&pb.sc->counter[1])) return;
     double tot = 0.0;
-    for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) tot += pb.partials[q];
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) tot += ld_dev(&pb.partials[q]);
     tot = block_reduce<0>(tot, sh);
     if (threadIdx.x == 0) {
         PcgScalars *sc = pb.sc;
@@ -447,11 +598,11 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm(GridGeom g, PressureB
         mx = a > mx ? a : mx;
     }
     double bm = block_reduce<1>(mx, sh);
-    if (threadIdx.x == 0) pb.partials[kRedBlocks + blockIdx.x] = bm;
+    if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
     if (!last_block(&pb.sc->counter[2])) return;
     double m = 0.0;
     for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) {
-        double v = pb.partials[kRedBlocks + q];
+        double v = ld_dev(&pb.partials[kRedBlocks + q]);
         m = v > m ? v : m;
     }
     m = block_reduce<1>(m, sh);
@@ -512,6 +663,7 @@ __global__ void k_pcg_init_scalars(PressureBuffers pb, double tolf, int maxiter)
 __global__ void k_reset_scalars(PressureBuffers pb) {
     PcgScalars *sc = pb.sc;
     sc->absmax_bits = 0ull;
+    sc->rows = 0u;
     sc->done = 0;
     sc->abort = 0;
     sc->failed = 0;
@@ -537,15 +689,24 @@ hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, Pressure
     return hipGetLastError();
 }
 
-hipError_t launch_pcg_iteration(const GridGeom &g, PressureBuffers &pb, hipStream_t s) {
-    k_spmv_dot<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb);
-    k_axpy_norm<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb);
-    hipError_t e = launch_wave(kForward, g, pb, pb.r, pb.z, nullptr, kDotNone, s);
-    if (e != hipSuccess) return e;
-    e = launch_wave(kBackward, g, pb, nullptr, pb.z, pb.r, kDotIter, s);
-    if (e != hipSuccess) return e;
-    k_update_s<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb, 0);
+hipError_t launch_pcg_piece(int piece, const GridGeom &g, PressureBuffers &pb, hipStream_t s) {
+    switch (piece) {
+        case kPcgSpmv: k_spmv_dot<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
+        case kPcgAxpy: k_axpy_norm<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
+        case kPcgForward: return launch_wave(kForward, g, pb, pb.r, pb.z, nullptr, kDotNone, s);
+        case kPcgBackward: return launch_wave(kBackward, g, pb, nullptr, pb.z, pb.r, kDotIter, s);
+        case kPcgUpdate: k_update_s<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb, 0); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
+}
+
+hipError_t launch_pcg_iteration(const GridGeom &g, PressureBuffers &pb, hipStream_t s) {
+    for (int piece = 0; piece < kPcgPieces; piece++) {
+        hipError_t e = launch_pcg_piece(piece, g, pb, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 __global__ void k_spmv_plain(GridGeom g, PressureBuffers pb, const double *__restrict__ x,

@@ -117,7 +117,7 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s) {
             throw StatusError{FLIPMI_ERR_OOM, "pressure arrays: out of device memory"};
     if (arr[13].alloc((size_t)npart) != hipSuccess ||
         progress.alloc((size_t)ntj * ntk) != hipSuccess || ticket.alloc(1) != hipSuccess ||
-        scal.alloc(1) != hipSuccess)
+        scal.alloc(1) != hipSuccess || irange.alloc((size_t)2 * ntj * ntk) != hipSuccess)
         throw StatusError{FLIPMI_ERR_OOM, "pressure scratch: out of device memory"};
     (void)launch_fill_i32((int *)progress.p, progress.n, 0, s);
     (void)launch_fill_i32((int *)ticket.p, 1, 0, s);
@@ -132,6 +132,7 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s) {
     pb.ntiles_j = ntj;
     pb.ntiles_k = ntk;
     pb.ticket = ticket.p;
+    pb.tile_irange = irange.p;
     pb.sc = scal.p;
 }
 
@@ -154,6 +155,7 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
     hip_check(hipStreamSynchronize(s), "assemble sync");
     double bnorm;
     std::memcpy(&bnorm, &hs->absmax_bits, sizeof(double));
+    out.rows = hs->rows;
     // Early exit of PressureSolver::solve when max|rhs| < tolerance
     // (pressuresolver.cpp:52-62): nothing is solved or applied.
     if (bnorm < w.tol) {
@@ -163,14 +165,26 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
         return out;
     }
     if (prof) prof->begin_stage(FLIPMI_STAGE_MIC0, &ev);
-    hip_check(launch_mic0_factor(w.g, pb, s), "mic0 factor");
+    hip_check(launch_mic0_factor(w.g, w.liquid, pb, s), "mic0 factor");
     hip_check(launch_pcg_init(w.g, w.tol, w.maxiter, pb, s), "pcg init");
     if (prof) prof->end_stage(FLIPMI_STAGE_MIC0, ev);
     if (prof) prof->begin_stage(FLIPMI_STAGE_PCG, &ev);
     int batch = 8;
     int launched = 0;
     for (;;) {
-        for (int q = 0; q < batch; q++) hip_check(launch_pcg_iteration(w.g, pb, s), "pcg iteration");
+        for (int q = 0; q < batch; q++) {
+            if (prof && prof->profiling >= 2) {
+                // Time each kernel of the iteration on the launch stream.
+                for (int piece = 0; piece < kPcgPieces; piece++) {
+                    hipEvent_t e0 = nullptr;
+                    prof->begin_stage(FLIPMI_STAGE_K_SPMV + piece, &e0);
+                    hip_check(launch_pcg_piece(piece, w.g, pb, s), "pcg kernel");
+                    prof->end_stage(FLIPMI_STAGE_K_SPMV + piece, e0);
+                }
+            } else {
+                hip_check(launch_pcg_iteration(w.g, pb, s), "pcg iteration");
+            }
+        }
         launched += batch;
         hip_check(hipMemcpyAsync(hs, pb.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), "scalars");
         hip_check(hipStreamSynchronize(s), "pcg sync");
@@ -388,6 +402,7 @@ void Scene::pressure_solve(double dt) {
     last_iters = o.iterations;
     last_resid = o.residual;
     last_ok = o.success ? 1 : 0;
+    last_rows = o.rows;
 }
 
 void Scene::advance_particles(double dt) {
@@ -589,6 +604,7 @@ flipmi_frame_stats Scene::update(double dt) {
     st.pcg_residual = last_resid;
     st.pressure_ok = last_ok;
     st.gpu_ms = ms;
+    st.pressure_rows = last_rows;
     return st;
 }
 

@@ -97,11 +97,12 @@ struct Scene {
     // last pressure solve
     int last_iters = 0, last_ok = 1;
     double last_resid = 0;
+    int64_t last_rows = 0;
 
     // profiling
-    bool profiling = false;
-    double stage_ms[16] = {0};
-    int64_t stage_calls[16] = {0};
+    int profiling = 0;  // 0 off, 1 stage events, 2 also each PCG kernel
+    double stage_ms[FLIPMI_NUM_STAGES] = {0};
+    int64_t stage_calls[FLIPMI_NUM_STAGES] = {0};
     struct PendingEvent {
         int stage;
         hipEvent_t a, b;
@@ -145,6 +146,7 @@ struct PressureOutcome {
     bool success = true;
     bool solved = false;  // the pressure update was applied
     int refinements = 0;
+    int64_t rows = 0;
 };
 PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScalars *host_sc,
                              hipStream_t s, Scene *prof);
@@ -153,6 +155,7 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
 struct PressureStore {
     DBuf<double> arr[14];
     DBuf<unsigned> progress, ticket;
+    DBuf<int> irange;
     DBuf<PcgScalars> scal;
     PressureBuffers pb{};
     void alloc(const GridGeom &g, hipStream_t s);

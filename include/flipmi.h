@@ -77,12 +77,14 @@ typedef struct {
     double pcg_residual;       /* last substep, |r|_inf ("Estimated Error") */
     int pressure_ok;           /* 0 if the last pressure solve failed */
     double gpu_ms;             /* device time of the frame */
+    int64_t pressure_rows;     /* last substep: rows of the Poisson system */
 } flipmi_frame_stats;
 
 /* Per-stage device time accumulated since the last reset (hipEvents). */
+#define FLIPMI_NUM_STAGES 24
 typedef struct {
-    double ms[16];
-    int64_t calls[16];
+    double ms[FLIPMI_NUM_STAGES];
+    int64_t calls[FLIPMI_NUM_STAGES];
 } flipmi_stage_times;
 
 enum {
@@ -97,7 +99,12 @@ enum {
     FLIPMI_STAGE_G2P_ADVECT = 8,
     FLIPMI_STAGE_REMOVE = 9,
     FLIPMI_STAGE_GLUE = 10,         /* save/body force/constrain */
-    FLIPMI_STAGE_PCG_ITERS = 11     /* calls = iterations, ms unused */
+    FLIPMI_STAGE_PCG_ITERS = 11,    /* calls = iterations, ms unused */
+    FLIPMI_STAGE_K_SPMV = 12,       /* single kernels of a PCG iteration */
+    FLIPMI_STAGE_K_AXPY = 13,
+    FLIPMI_STAGE_K_FORWARD = 14,
+    FLIPMI_STAGE_K_BACKWARD = 15,
+    FLIPMI_STAGE_K_UPDATE = 16
 };
 
 /* Grids readable with flipmi_scene_get_grid (sizes as in the header). */
@@ -147,8 +154,11 @@ int flipmi_scene_get_particles(flipmi_scene *scene, float *pos_xyz, float *vel_x
  * FluidSimulation::getVelocityField (fluidsimulation.cpp:2242). */
 int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out);
 int flipmi_scene_stage_times(flipmi_scene *scene, flipmi_stage_times *out, int reset);
-/* Enables hipEvent timing of each stage (adds one event pair per stage). */
-int flipmi_scene_set_profiling(flipmi_scene *scene, int enable);
+/* hipEvent timing on the scene's stream: level 0 off, 1 one event pair per
+ * stage, 2 additionally one pair around every PCG kernel launch. */
+int flipmi_scene_set_profiling(flipmi_scene *scene, int level);
+/* Blocks until all work queued by the context's scenes has finished. */
+int flipmi_ctx_synchronize(flipmi_ctx *ctx);
 
 /* ---- stage drop-ins on host buffers ------------------------------------ */
 /* Domain-wall solid SDF (nodal) and the weight grid: FluidSimulation::

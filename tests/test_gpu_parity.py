@@ -94,7 +94,7 @@ def _projection_inputs(oracle, I, J, K, seed=4):
     return s, dx, solid, weights, liquid, mac, valid
 
 
-@pytest.mark.parametrize("dims", SIZES)
+@pytest.mark.parametrize("dims", SIZES + [(48, 72, 64), (33, 90, 17)])
 def test_mic0_preconditioner_bitexact(gpu, oracle, dims):
     I, J, K = dims
     s, dx, solid, weights, liquid, mac, valid = _projection_inputs(oracle, I, J, K)
