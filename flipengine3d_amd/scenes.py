@@ -125,7 +125,7 @@ def seed_cells(isize: int, jsize: int, ksize: int, dx: float,
 
 
 def dam_break(isize: int, jsize: int, ksize: int, dx: float, seed: int,
-              box: Sequence[float] = (0.0, 0.0, 0.0, 0.4, 0.5, 1.0)) -> Scene:
+              box: Sequence[float] = (0.0, 0.0, 0.0, 0.4, 0.5, 1.0), jitter_factor: float = 1.0) -> Scene:
     """Single fluid cuboid [x0,x1]x[y0,y1]x[z0,z1] given as fractions of the
     domain extent (config: dam break, 64^3 at dx=1/64, cuboid
     [0,0.4]x[0,0.5]x[0,1])."""
@@ -136,11 +136,11 @@ def dam_break(isize: int, jsize: int, ksize: int, dx: float, seed: int,
         return ((x >= x0 * W) & (x < x1 * W) & (y >= y0 * H) & (y < y1 * H)
                 & (z >= z0 * D) & (z < z1 * D))
 
-    return seed_cells(isize, jsize, ksize, dx, inside, seed, name="dam_break")
+    return seed_cells(isize, jsize, ksize, dx, inside, seed, jitter_factor, name="dam_break")
 
 
 def double_dam_break(isize: int, jsize: int, ksize: int, dx: float, seed: int,
-                     width: float = 0.25, height: float = 0.6) -> Scene:
+                     width: float = 0.25, height: float = 0.6, jitter_factor: float = 1.0) -> Scene:
     """Two cuboids at opposite x ends, each ``width`` of the x extent wide and
     ``height`` of the y extent tall, full z (config: 256x128x128 mixed
     precision)."""
@@ -149,11 +149,12 @@ def double_dam_break(isize: int, jsize: int, ksize: int, dx: float, seed: int,
     def inside(x, y, z):
         return (((x < width * W) | (x >= (1.0 - width) * W)) & (y < height * H))
 
-    return seed_cells(isize, jsize, ksize, dx, inside, seed, name="double_dam_break")
+    return seed_cells(isize, jsize, ksize, dx, inside, seed, jitter_factor, name="double_dam_break")
 
 
 def pool_drop(isize: int, jsize: int, ksize: int, dx: float, seed: int,
-              depth: float = 0.3, centre=(0.5, 0.7, 0.5), radius: float = 0.15) -> Scene:
+              depth: float = 0.3, centre=(0.5, 0.7, 0.5), radius: float = 0.15,
+              jitter_factor: float = 1.0) -> Scene:
     """Pool filling the bottom ``depth`` of the domain plus a sphere (config:
     128^3 drop into pool)."""
     W, H, D = isize * dx, jsize * dx, ksize * dx
@@ -163,4 +164,4 @@ def pool_drop(isize: int, jsize: int, ksize: int, dx: float, seed: int,
     def inside(x, y, z):
         return (y < depth * H) | ((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 < r * r)
 
-    return seed_cells(isize, jsize, ksize, dx, inside, seed, name="pool_drop")
+    return seed_cells(isize, jsize, ksize, dx, inside, seed, jitter_factor, name="pool_drop")

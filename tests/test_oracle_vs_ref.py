@@ -1,0 +1,143 @@
+"""The CPU oracle against the reference engine compiled from its own sources.
+
+oracle/_ref/libflipref.so is the reference's src/engine built by
+oracle/Makefile with a small C harness (oracle/ref_harness.cpp).  Each stage
+of the FLIP step is run on both with identical inputs and compared bit for
+bit; the whole update(dt) is compared over several frames with the reference
+on one thread (its PCG dot products and removal are then in a fixed order).
+Skipped where the reference build is absent (it is built only where the
+reference sources are present).
+"""
+
+import numpy as np
+import pytest
+
+from tests.helpers import bits_equal, dam, ndiff, radius, random_velocities
+
+SIZES = [(16, 16, 16), (23, 17, 20)]
+
+
+@pytest.mark.parametrize("dims", SIZES + [(12, 31, 9)])
+def test_domain_solid_and_weights(oracle, reference, dims):
+    I, J, K = dims
+    dx = 1.0 / max(dims)
+    po, wo = oracle.domain_solid(I, J, K, dx)
+    pr, wr = reference.domain_solid(I, J, K, dx)
+    assert bits_equal(po, pr), f"solid phi ndiff={ndiff(po, pr)}"
+    for a, b in zip(wo, wr):
+        assert bits_equal(a, b)
+
+
+@pytest.mark.parametrize("dims", SIZES)
+def test_liquid_sdf(oracle, reference, dims):
+    I, J, K = dims
+    s = dam(I, J, K, seed=3)
+    solid, _ = oracle.domain_solid(I, J, K, s.dx)
+    for sp in (None, solid):
+        a = oracle.liquid_sdf(I, J, K, s.dx, s.positions, radius(s.dx), sp)
+        b = reference.liquid_sdf(I, J, K, s.dx, s.positions, radius(s.dx), sp)
+        assert bits_equal(a, b), f"ndiff={ndiff(a, b)}"
+
+
+@pytest.mark.parametrize("dims", SIZES)
+def test_p2g(oracle, reference, dims):
+    I, J, K = dims
+    s = dam(I, J, K, seed=5)
+    vel = random_velocities(s.num_particles, seed=1)
+    (mo, vo) = oracle.p2g(I, J, K, s.dx, s.positions, vel, radius(s.dx))
+    (mr, vr) = reference.p2g(I, J, K, s.dx, s.positions, vel, radius(s.dx))
+    for a, b in zip(mo + vo, mr + vr):
+        assert bits_equal(a, b), f"ndiff={ndiff(a, b)}"
+
+
+@pytest.mark.parametrize("layers", [1, 2, 7])
+def test_extrapolation(oracle, reference, layers):
+    I, J, K = 18, 14, 16
+    s = dam(I, J, K, seed=2)
+    vel = random_velocities(s.num_particles, seed=2)
+    (mo, vo) = oracle.p2g(I, J, K, s.dx, s.positions, vel, radius(s.dx))
+    for g, v in zip(mo, vo):
+        assert bits_equal(oracle.extrapolate(g, v, layers), reference.extrapolate(g, v, layers))
+
+
+def _projection_inputs(oracle, I, J, K, seed=4):
+    s = dam(I, J, K, seed=seed)
+    solid, weights = oracle.domain_solid(I, J, K, s.dx)
+    liquid = oracle.liquid_sdf(I, J, K, s.dx, s.positions, radius(s.dx), solid)
+    vel = random_velocities(s.num_particles, seed=seed, scale=0.3)
+    mac, valid = oracle.p2g(I, J, K, s.dx, s.positions, vel, radius(s.dx))
+    mac = [oracle.extrapolate(g, v, 7) for g, v in zip(mac, valid)]
+    mac[1] = (mac[1] + np.float32(-9.81 / 60)).astype(np.float32)
+    return s, solid, weights, liquid, mac, valid
+
+
+@pytest.mark.parametrize("dims", SIZES)
+def test_mic0(oracle, reference, dims):
+    I, J, K = dims
+    s, solid, weights, liquid, mac, valid = _projection_inputs(oracle, I, J, K)
+    n = int(np.count_nonzero(liquid[1:-1, 1:-1, 1:-1] < 0))
+    r = np.random.default_rng(3).standard_normal(n)
+    io, zo = oracle.mic0_apply(I, J, K, s.dx, 1 / 60, liquid, weights, r)
+    ir, zr = reference.mic0_apply(I, J, K, s.dx, 1 / 60, liquid, weights, r)
+    assert bits_equal(io, ir)
+    assert bits_equal(zo, zr)
+
+
+@pytest.mark.parametrize("dims", SIZES)
+def test_pressure_solve_reference_exact(oracle, reference, dims):
+    I, J, K = dims
+    s, solid, weights, liquid, mac, valid = _projection_inputs(oracle, I, J, K)
+    reference.set_threads(1)
+    a = oracle.pressure_solve(I, J, K, s.dx, 1 / 60, liquid, solid, weights, mac, valid, mode=0)
+    b = reference.pressure_solve(I, J, K, s.dx, 1 / 60, liquid, solid, weights, mac, valid)
+    assert a["ok"] == b["ok"] == 1
+    assert a["iters"] == b["iters"]
+    assert bits_equal(a["rhs"], b["rhs"])
+    assert bits_equal(a["soln"], b["soln"])
+    for x, y in zip(a["mac"] + a["valid"], b["mac"] + b["valid"]):
+        assert bits_equal(x, y)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_particle_advector(oracle, reference, order):
+    I, J, K = 20, 18, 22
+    s, solid, weights, liquid, mac, valid = _projection_inputs(oracle, I, J, K)
+    rng = np.random.default_rng(order)
+    pos = (rng.random((3000, 3)) * np.array([I, J, K]) * s.dx * 1.1 - 0.05 * s.dx).astype(np.float32)
+    assert bits_equal(oracle.advect_particles(I, J, K, s.dx, mac, pos, 1 / 30, order),
+                      reference.advect_particles(I, J, K, s.dx, mac, pos, 1 / 30, order))
+    assert bits_equal(oracle.trilinear_velocity(I, J, K, s.dx, mac, pos),
+                      reference.trilinear_velocity(I, J, K, s.dx, mac, pos))
+
+
+@pytest.mark.parametrize("case", [
+    # (dims, cuboid, frames, frame dt, substep bounds)
+    ((20, 20, 20), (0.0, 0.0, 0.0, 0.4, 0.5, 1.0), 8, 1 / 30, (1, 6)),
+    ((24, 16, 12), (0.0, 0.0, 0.0, 0.5, 0.7, 1.0), 5, 1 / 60, (2, 4)),
+])
+def test_whole_update(oracle, reference, case):
+    """FluidSimulation::update over several frames: particle count, positions,
+    velocities and substep counts agree bit for bit (reference on 1 thread)."""
+    (I, J, K), box, frames, dt, (mn, mx) = case
+    s = dam(I, J, K, seed=1, box=box)
+    reference.set_threads(1)
+    sims = []
+    for lib in (oracle, reference):
+        h = lib.sim(I, J, K, s.dx)
+        h.add_body_force(0.0, -9.81, 0.0)
+        h.set_substeps(mn, mx)
+        h.load_particles(s.positions, s.velocities)
+        h.initialize()
+        sims.append(h)
+    for f in range(frames):
+        for h in sims:
+            h.update(dt)
+        a, b = sims
+        assert a.substeps() == b.substeps(), f"frame {f}"
+        assert a.num_particles == b.num_particles, f"frame {f}"
+        pa, va = a.particles()
+        pb, vb = b.particles()
+        assert bits_equal(pa, pb), f"frame {f}: positions ndiff={ndiff(pa, pb)}"
+        assert bits_equal(va, vb), f"frame {f}: velocities ndiff={ndiff(va, vb)}"
+    for which in (0, 2, 3, 4):
+        assert bits_equal(sims[0].float_grid(which), sims[1].float_grid(which))
