@@ -426,9 +426,10 @@ int flipmi_pressure_solve(flipmi_ctx *ctx, const flipmi_pressure_args *a, flipmi
         std::vector<float> hl(nc);
         int rows = 0;
         std::vector<double> x;
+        const SlotMap &sm = ps.pb.sm;
         if (a->pressure_rows) {
-            x.resize(nc);
-            download(x.data(), ps.arr[5], nc, s);
+            x.resize(sm.count());
+            download(x.data(), ps.arr[5], sm.count(), s);
         }
         download(hl.data(), liq, nc, s);
         hipck(hipStreamSynchronize(s), "sync");
@@ -438,7 +439,7 @@ int flipmi_pressure_solve(flipmi_ctx *ctx, const flipmi_pressure_args *a, flipmi
                 for (int i = 1; i < I - 1; i++) {
                     size_t c = (size_t)i + (size_t)I * ((size_t)j + (size_t)J * k);
                     if (hl[c] < 0) {
-                        if (a->pressure_rows) a->pressure_rows[rows] = o.solved ? x[c] : 0.0;
+                        if (a->pressure_rows) a->pressure_rows[rows] = o.solved ? x[sm.slot(i, j, k)] : 0.0;
                         rows++;
                     }
                 }
@@ -477,26 +478,32 @@ int flipmi_mic0_apply(flipmi_ctx *ctx, int I, int J, int K, double dx, double dt
                                        zero_u.p, zero_v.p, zero_w.p, ps.pb, s),
               "assemble");
         hipck(launch_mic0_factor(g, liq.p, ps.pb, s), "factor");
-        // Scatter r from reference row order into the dense layout.
+        // Scatter r from reference row order into slot order.
         std::vector<float> hl(nc);
         download(hl.data(), liq, nc, s);
         hipck(hipStreamSynchronize(s), "sync");
-        std::vector<double> rd(nc, 0.0);
+        const SlotMap &sm = ps.pb.sm;
+        size_t ns = sm.count();
+        std::vector<double> rd(ns, 0.0);
         std::vector<size_t> rows;
         for (int k = 1; k < K - 1; k++)
             for (int j = 1; j < J - 1; j++)
                 for (int i = 1; i < I - 1; i++) {
                     size_t c = (size_t)i + (size_t)I * ((size_t)j + (size_t)J * k);
-                    if (hl[c] < 0) { rd[c] = r_rows[rows.size()]; rows.push_back(c); }
+                    if (hl[c] < 0) {
+                        size_t p = sm.slot(i, j, k);
+                        rd[p] = r_rows[rows.size()];
+                        rows.push_back(p);
+                    }
                 }
         DBuf<double> dr, dz;
-        upload(dr, rd.data(), nc, s);
-        alloc_dev(dz, nc);
-        hipck(launch_fill_f64(dz.p, nc, 0.0, s), "fill");
+        upload(dr, rd.data(), ns, s);
+        alloc_dev(dz, ns);
+        hipck(launch_fill_f64(dz.p, ns, 0.0, s), "fill");
         hipck(launch_precond_apply(g, ps.pb, dr.p, dz.p, s), "precond");
-        std::vector<double> zd(nc), inv(nc);
-        download(zd.data(), dz, nc, s);
-        download(inv.data(), ps.arr[9], nc, s);
+        std::vector<double> zd(ns), inv(ns);
+        download(zd.data(), dz, ns, s);
+        download(inv.data(), ps.arr[9], ns, s);
         hipck(hipStreamSynchronize(s), "sync");
         PcgScalars hs;
         hipck(hipMemcpy(&hs, ps.pb.sc, sizeof(hs), hipMemcpyDeviceToHost), "scalars");

@@ -87,14 +87,43 @@ struct PcgScalars {
     unsigned long long absmax_bits;
 };
 
+// Sweep-ordered ("slot") storage of the pressure system.  Cells are grouped
+// into 8 x 8 column tiles over (j, k).  Tile (jt, kt) stores, for every step
+// t = i + a + b (a = j % 8, b = k % 8, 0 <= t < S = I + 14), one row of 64
+// slots with lane a + 8 b.  A wave that owns a tile and marches over t finds
+// the 64 cells of a step in one contiguous row, and each cell's -x, -y and -z
+// neighbours of the same tile one row earlier in lanes l, l - 1 and l - 8.
+// Slots that are not grid cells (the skew corners, padding of J and K up to
+// a multiple of 8) hold zeros in every array.
+struct SlotMap {
+    int I, J, K, ntj, ntk, S;
+    __host__ __device__ size_t count() const { return (size_t)ntj * ntk * S * 64; }
+    __host__ __device__ size_t slot(int i, int j, int k) const {
+        int a = j & 7, b = k & 7;
+        size_t tile = (size_t)(j >> 3) + (size_t)ntj * (size_t)(k >> 3);
+        return ((tile * (size_t)S + (size_t)(i + a + b)) << 6) + (size_t)(a + 8 * b);
+    }
+};
+inline SlotMap make_slot_map(int I, int J, int K) {
+    SlotMap m;
+    m.I = I; m.J = J; m.K = K;
+    m.ntj = (J + 7) / 8; m.ntk = (K + 7) / 8;
+    m.S = I + 14;
+    return m;
+}
+
 struct PressureBuffers {
-    double *diag, *offx, *offy, *offz;  // matrix, dense cell layout
+    SlotMap sm;
+    double *diag, *offx, *offy, *offz;  // matrix: diagonal and couplings to the +x/+y/+z rows
     double *rhs;
-    double *x, *r, *z, *s;              // PCG vectors (dense, FP64)
-    double *invdiag, *Lx, *Ly, *Lz;     // MIC(0) factor
+    double *x, *r, *z, *s;              // PCG vectors (FP64)
+    double *invdiag, *Lx, *Ly, *Lz;     // MIC(0) factor: column entries (rows +x/+y/+z)
+    double *Lxm, *Lym, *Lzm;            // the same entries read from the row: columns -x/-y/-z
     double *partials;                   // per-block partial sums
     int npartials;
-    unsigned int *progress;             // wavefront tile progress counters
+    unsigned long long *edge_j, *edge_k;  // tagged values a tile hands to its j / k neighbour
+    unsigned int epoch;                 // tag of the current sweep launch (host counter)
+    unsigned int *progress;             // factorisation: tile progress counters
     int ntiles_j, ntiles_k;
     unsigned int *ticket;
     int *tile_irange;                   // per (j,k) tile: min/max i of its rows (-1: none)
@@ -116,7 +145,8 @@ hipError_t launch_pcg_iteration(const GridGeom &g, PressureBuffers &pb, hipStrea
 // The five kernels of one PCG iteration, in order.
 enum { kPcgSpmv = 0, kPcgAxpy = 1, kPcgForward = 2, kPcgBackward = 3, kPcgUpdate = 4, kPcgPieces = 5 };
 hipError_t launch_pcg_piece(int piece, const GridGeom &g, PressureBuffers &pb, hipStream_t s);
-hipError_t launch_pressure_apply(const GridGeom &g, double dt, const float *liquid,
+// p is in slot order (pb.sm).
+hipError_t launch_pressure_apply(const GridGeom &g, const SlotMap &sm, double dt, const float *liquid,
                                  const float *wU, const float *wV, const float *wW,
                                  const double *p, float *u, float *v, float *w, uint8_t *vu,
                                  uint8_t *vv, uint8_t *vw, hipStream_t s);
@@ -124,6 +154,8 @@ hipError_t launch_precond_apply(const GridGeom &g, PressureBuffers &pb, const do
                                 double *z, hipStream_t s);
 hipError_t launch_spmv(const GridGeom &g, PressureBuffers &pb, const double *x, double *y,
                        hipStream_t s);
+// Number of tagged records in each of pb.edge_j / pb.edge_k.
+size_t edge_record_count(const SlotMap &sm);
 
 // ---- particles (k_particles.hip) ----
 struct ParticleStepArgs {

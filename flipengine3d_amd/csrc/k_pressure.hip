@@ -5,18 +5,21 @@
 // PCGSolver / factorModifiedIncompleteColesky0 / solveLower /
 // solveLowerTransposeInPlace (pcgsolver/pcgsolver.h:69-302).
 //
-// Layout: every vector and matrix array is a dense cell grid (i fastest);
-// a cell is a row of the system when it is interior and its liquid phi is
-// negative.  Non-row entries hold zeros, so stencils need no row lookup.
+// Layout: every vector and matrix array is stored in slot order (SlotMap in
+// flipmi_kernels.h): 8 x 8 (j, k) column tiles, and within a tile one row of
+// 64 slots per wavefront step t = i + (j % 8) + (k % 8).  A cell is a row of
+// the system when it is interior and its liquid phi is negative; every other
+// slot holds zeros, so stencils need no row lookup.
 //
 // The MIC(0) factorisation and both triangular solves run as a pipelined
-// wavefront: one 64-lane wave owns an 8 x 8 block of (j, k) columns and
-// marches along i with lane (a, b) skewed by a + b steps, so every cell's
-// -x, -y and -z dependencies were produced one step earlier by itself or a
-// neighbouring lane (exchanged with lane shuffles).  Across blocks a wave
-// waits on its upstream neighbours' published progress.  Each cell performs
-// exactly the reference's operations in the reference's order, so the
-// preconditioner is the reference's bit for bit.
+// wavefront: one 64-lane wave owns a tile and marches over its steps, so each
+// step reads one contiguous row of every array, and every cell's -x, -y and
+// -z dependencies were produced one step earlier by itself or a neighbouring
+// lane (exchanged with lane shuffles).  Values a tile needs from its upstream
+// neighbour tiles arrive as tagged records (value + launch epoch), so a
+// consumer polls the data itself and a producer never waits for its stores.
+// Each cell performs exactly the reference's operations in the reference's
+// order, so the preconditioner is the reference's bit for bit.
 #include "device_math.h"
 #include "flipmi_kernels.h"
 
@@ -27,6 +30,27 @@ constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
 constexpr double kMaxTheta = 25.0;  // pressuresolver.h:163
 constexpr unsigned kSpinLimit = 1u << 20;  // ~1 s of polling: a legitimate wait is microseconds
+constexpr int kRecWords = 2;       // 64-bit words per tagged FP64 record
+
+size_t edge_record_count(const SlotMap &sm) {
+    return (size_t)sm.ntj * sm.ntk * sm.S * kTile * kRecWords;
+}
+
+// Cell of a slot (may lie outside the grid: skew corners and padding).
+__device__ __forceinline__ void decode_slot(const SlotMap &sm, size_t p, int &i, int &j, int &k) {
+    int lane = (int)(p & 63);
+    size_t row = p >> 6;
+    int t = (int)(row % (size_t)sm.S);
+    size_t tile = row / (size_t)sm.S;
+    int a = lane & 7, b = lane >> 3;
+    j = (int)(tile % (size_t)sm.ntj) * kTile + a;
+    k = (int)(tile / (size_t)sm.ntj) * kTile + b;
+    i = t - a - b;
+}
+
+__device__ __forceinline__ bool interior(const GridGeom &g, int i, int j, int k) {
+    return i >= 1 && j >= 1 && k >= 1 && i <= g.I - 2 && j <= g.J - 2 && k <= g.K - 2;
+}
 
 // ---------------------------------------------------------------------------
 // Assembly
@@ -34,13 +58,14 @@ constexpr unsigned kSpinLimit = 1u << 20;  // ~1 s of polling: a legitimate wait
 
 __device__ __forceinline__ bool is_row_cell(const GridGeom &g, const float *liquid, int i, int j,
                                             int k) {
-    if (i < 1 || j < 1 || k < 1 || i > g.I - 2 || j > g.J - 2 || k > g.K - 2) return false;
+    if (!interior(g, i, j, k)) return false;
     return liquid[(size_t)i + (size_t)g.I * ((size_t)j + (size_t)g.J * k)] < 0;
 }
 
 // Right-hand side (pressuresolver.cpp:580-669) and matrix coefficients
 // (pressuresolver.cpp:723-808) of one row; couplings to the +x/+y/+z rows
-// are stored on this row, couplings to -x/-y/-z on the neighbour.
+// are stored on this row, couplings to -x/-y/-z on the neighbour.  One
+// thread per slot, so every slot (rows, other cells, padding) is written.
 __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liquid,
                            const float *__restrict__ wU, const float *__restrict__ wV,
                            const float *__restrict__ wW, const float *__restrict__ wC,
@@ -48,12 +73,13 @@ __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liqu
                            const float *__restrict__ w, const float *__restrict__ sU,
                            const float *__restrict__ sV, const float *__restrict__ sW,
                            PressureBuffers pb) {
-    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t nc = (size_t)g.I * g.J * g.K;
-    if (c >= nc) return;
-    int i = (int)(c % g.I), j = (int)((c / g.I) % g.J), k = (int)(c / ((size_t)g.I * g.J));
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pb.sm.count()) return;
+    int i, j, k;
+    decode_slot(pb.sm, p, i, j, k);
     double rhs = 0, diag = 0, ox = 0, oy = 0, oz = 0;
     if (is_row_cell(g, liquid, i, j, k)) {
+        size_t c = (size_t)i + (size_t)g.I * ((size_t)j + (size_t)g.J * k);
         Dims DU{g.I + 1, g.J, g.K}, DV{g.I, g.J + 1, g.K}, DW{g.I, g.J, g.K + 1};
         double volCenter = wC[c];
         double volRight = wU[DU.at(i + 1, j, k)], volLeft = wU[DU.at(i, j, k)];
@@ -101,11 +127,11 @@ __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liqu
         atomicMax(&pb.sc->absmax_bits, bits);
         atomicAdd(&pb.sc->rows, 1u);
     }
-    pb.rhs[c] = rhs;
-    pb.diag[c] = diag;
-    pb.offx[c] = ox;
-    pb.offy[c] = oy;
-    pb.offz[c] = oz;
+    pb.rhs[p] = rhs;
+    pb.diag[p] = diag;
+    pb.offx[p] = ox;
+    pb.offy[p] = oy;
+    pb.offz[p] = oz;
 }
 
 hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *liquid,
@@ -113,26 +139,25 @@ hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *l
                                     const float *wC, const float *u, const float *v,
                                     const float *w, const float *sU, const float *sV,
                                     const float *sW, PressureBuffers &pb, hipStream_t s) {
-    size_t nc = (size_t)g.I * g.J * g.K;
-    k_assemble<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(g, dt, liquid, wU, wV, wW, wC, u, v,
-                                                            w, sU, sV, sW, pb);
+    size_t n = pb.sm.count();
+    k_assemble<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, dt, liquid, wU, wV, wW, wC, u, v, w,
+                                                           sU, sV, sW, pb);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Wavefront sweeps (factorisation, forward and backward substitution)
+// Cross-workgroup exchange
 // ---------------------------------------------------------------------------
 
 enum { kFactor = 0, kForward = 1, kBackward = 2 };
 enum { kDotNone = 0, kDotInit = 1, kDotIter = 2 };
 
-// Cross-workgroup exchange without cache maintenance.  Values that another
-// workgroup reads within the same kernel are stored and loaded as relaxed
-// device-scope atomics, which are coherent across XCDs by themselves; a
-// producer orders its data stores before its progress flag by draining its
-// outstanding memory operations (s_waitcnt), and a consumer issues its data
-// loads only after the flag load has returned.  No acquire/release fence is
-// needed, so the sweeps never write back or invalidate the L2 caches.
+// Values another workgroup reads within the same kernel are stored and
+// loaded as relaxed device-scope atomics, which are coherent across XCDs by
+// themselves, so no acquire/release fence (and no L2 writeback/invalidate)
+// is ever issued.  Ordering between data and a flag is obtained either by
+// draining the wave's memory operations (s_waitcnt) before the flag store,
+// or -- in the solves -- by tagging every value with the launch epoch.
 __device__ __forceinline__ double ld_dev(const double *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -151,6 +176,12 @@ __device__ __forceinline__ int ld_dev(const int *p) {
 __device__ __forceinline__ void st_dev(int *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ unsigned long long ld_dev(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // Waits until every memory operation this wave issued has completed.
 __device__ __forceinline__ void drain_memory() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -159,6 +190,23 @@ __device__ __forceinline__ void drain_memory() {
 }
 
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
+
+// A tagged FP64 record: two 64-bit words, each holding 32 bits of the value
+// and the 32-bit epoch of the launch that wrote it.  Each word is written
+// and read as one single-copy-atomic 64-bit access, so a reader that sees
+// the current epoch in both words has the whole value.
+__device__ __forceinline__ void put_record(unsigned long long *rec, double v, unsigned tag) {
+    unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+    unsigned long long t = (unsigned long long)tag << 32;
+    st_dev(rec, t | (bits & 0xffffffffull));
+    st_dev(rec + 1, t | (bits >> 32));
+}
+__device__ __forceinline__ bool record_ok(unsigned long long lo, unsigned long long hi, unsigned tag) {
+    return (unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag;
+}
+__device__ __forceinline__ double record_value(unsigned long long lo, unsigned long long hi) {
+    return __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+}
 
 // Waits until the upstream tiles (index < 0: none) have completed `need`
 // steps; both flags are polled in one round trip.  Returns false if the
@@ -185,12 +233,89 @@ __device__ __forceinline__ double wave_sum_fixed(double v) {
     return v;
 }
 
+// Tile order of the dispatch: an atomic ticket hands tiles out in
+// dependency order, so every tile's producers were dispatched before it.
+__device__ __forceinline__ int take_tile(PressureBuffers &pb, bool reverse, int &jt, int &kt) {
+    const int ntj = pb.sm.ntj, ntk = pb.sm.ntk;
+    int ticket = 0;
+    if (threadIdx.x == 0) ticket = (int)atomicAdd(pb.ticket, 1u);
+    ticket = __shfl(ticket, 0, 64);
+    if (reverse) {
+        jt = ntj - 1 - ticket % ntj;
+        kt = ntk - 1 - ticket / ntj;
+    } else {
+        jt = ticket % ntj;
+        kt = ticket / ntj;
+    }
+    return jt + ntj * kt;
+}
+
+// Processing range [ub, ue) of a tile in its march order u (forward: u = t;
+// backward: u = S - 1 - t): the steps that hold at least one of its rows.
+__device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const GridGeom &g, int tile,
+                                            bool reverse, int &ub, int &ue) {
+    int imin = pb.tile_irange[2 * tile], imax = pb.tile_irange[2 * tile + 1];
+    if (imax < 0) {
+        ub = ue = 0;
+        return;
+    }
+    if (reverse) {
+        ub = g.I - 1 - imax;
+        ue = g.I + 14 - imin;
+    } else {
+        ub = imin;
+        ue = imax + 15;
+    }
+    if (ue > pb.sm.S) ue = pb.sm.S;
+}
+
+// The last wave of a sweep launch resets the shared counters for the next
+// launch and, for the backward sweep, reduces the dot product.
+__device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, int tile,
+                             bool reset_progress) {
+    const int lane = threadIdx.x;
+    const int ntiles = pb.sm.ntj * pb.sm.ntk;
+    if (dotmode != kDotNone) {
+        double part = wave_sum_fixed(dot_acc);
+        if (lane == 0) st_dev(&pb.partials[tile], part);
+    }
+    drain_memory();
+    unsigned prev = 0;
+    if (lane == 0) prev = atomicAdd(&pb.sc->counter[0], 1u);
+    prev = __shfl(prev, 0, 64);
+    if (prev != (unsigned)ntiles - 1) return;
+    if (reset_progress)
+        for (int q = lane; q < ntiles; q += 64) pb.progress[q] = 0;
+    if (lane == 0) {
+        *pb.ticket = 0;
+        pb.sc->counter[0] = 0;
+    }
+    if (dotmode != kDotNone) {
+        // Fixed-order sum of the per-tile partials.
+        double acc = 0.0;
+        for (int q = lane; q < ntiles; q += 64) acc += ld_dev(&pb.partials[q]);
+        double rho = wave_sum_fixed(acc);
+        if (lane == 0) {
+            PcgScalars *sc = pb.sc;
+            if (dotmode == kDotInit) {
+                sc->rho = rho;
+                if (rho == 0 || rho != rho) { sc->failed = 1; sc->done = 1; }
+            } else {
+                double beta = rho / sc->rho;
+                sc->beta = beta;
+                sc->beta_f = (float)beta;
+                sc->rho = rho;
+            }
+        }
+    }
+}
+
 // Extent of each 8 x 8 column tile's rows along i, so sweeps can skip the
 // leading and trailing steps where a tile has nothing to do.
 __global__ __launch_bounds__(64) void k_tile_irange(GridGeom g, const float *__restrict__ liquid,
                                                     PressureBuffers pb) {
     const int tile = blockIdx.x, lane = threadIdx.x;
-    const int jt = tile % pb.ntiles_j, kt = tile / pb.ntiles_j;
+    const int jt = tile % pb.sm.ntj, kt = tile / pb.sm.ntj;
     const int j = jt * kTile + (lane & 7), k = kt * kTile + (lane >> 3);
     int lo = 1 << 30, hi = -1;
     if (j >= 1 && k >= 1 && j <= g.J - 2 && k <= g.K - 2) {
@@ -213,287 +338,372 @@ __global__ __launch_bounds__(64) void k_tile_irange(GridGeom g, const float *__r
     }
 }
 
-// MODE kFactor : computes invdiag, Lx, Ly, Lz from diag/offx/offy/offz
-//                (the factor arrays must be zero on entry).
-// MODE kForward: out = L^-1 in        (solveLower, pcgsolver.h:191-204)
-// MODE kBackward: out = L^-T out (in place), optional dot(out, rvec)
-//                 (solveLowerTransposeInPlace, pcgsolver.h:207-221).
-// Cells without rows are never written: `out` and the factor hold zeros there.
-//
-// The march is software-pipelined: the per-cell operands of the next chunk
-// of kChunk steps are loaded while the current chunk computes, and the
-// values a tile reads from its upstream neighbours are fetched once per
-// chunk, after the neighbours have published enough progress.
-constexpr int kChunk = 8;  // 5-6 loads per step: keeps a chunk's prefetch under the 63 outstanding loads vmcnt tracks
+// ---------------------------------------------------------------------------
+// MIC(0) factorisation (factorModifiedIncompleteColesky0, pcgsolver.h:69-185)
+// ---------------------------------------------------------------------------
+// Computes invdiag and the column entries Lx, Ly, Lz (the factor arrays must
+// be zero on entry).  Runs once per solve; tiles exchange three values per
+// edge cell through the slot arrays, ordered by per-chunk progress flags.
+constexpr int kFactorChunk = 8;
 
-template <int MODE>
-struct SweepVals {
-    static constexpr int NV = MODE == kFactor ? 4 : (MODE == kForward ? 5 : 6);
-    static constexpr int NE = MODE == kFactor ? 3 : 1;
-};
-
-template <int MODE>
-__global__ __launch_bounds__(64) void k_wavefront(GridGeom g, PressureBuffers pb,
-                                                  const double *__restrict__ in, double *out,
-                                                  const double *__restrict__ rvec, int dotmode) {
-    constexpr int C = kChunk;
-    constexpr int NV = SweepVals<MODE>::NV;
-    constexpr int NE = SweepVals<MODE>::NE;
-    if (ld_dev(&pb.sc->done) != 0) return;
+__global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
+    constexpr int C = kFactorChunk;
     const int lane = threadIdx.x;
-    const int ntj = pb.ntiles_j, ntk = pb.ntiles_k, ntiles = ntj * ntk;
-    int ticket = 0;
-    if (lane == 0) ticket = (int)atomicAdd(pb.ticket, 1u);
-    ticket = __shfl(ticket, 0, 64);
+    const SlotMap sm = pb.sm;
+    const int ntj = sm.ntj;
     int jt, kt;
-    if (MODE == kBackward) {
-        jt = ntj - 1 - ticket % ntj;
-        kt = ntk - 1 - ticket / ntj;
-    } else {
-        jt = ticket % ntj;
-        kt = ticket / ntj;
-    }
-    const int tile = jt + ntj * kt;
+    const int tile = take_tile(pb, false, jt, kt);
     const int a = lane & 7, b = lane >> 3;
     const int j = jt * kTile + a, k = kt * kTile + b;
     const bool col_ok = j >= 1 && k >= 1 && j <= g.J - 2 && k <= g.K - 2;
-    const int up_j = MODE == kBackward ? (jt + 1 < ntj ? tile + 1 : -1) : (jt > 0 ? tile - 1 : -1);
-    const int up_k = MODE == kBackward ? (kt + 1 < ntk ? tile + ntj : -1) : (kt > 0 ? tile - ntj : -1);
-    const int skew = MODE == kBackward ? (7 - a) + (7 - b) : a + b;
-    const int src_j = (MODE == kBackward ? lane + 1 : lane - 1) & 63;  // neighbour lane along j
-    const int src_k = (MODE == kBackward ? lane + 8 : lane - 8) & 63;  // neighbour lane along k
-    const bool edge_j = MODE == kBackward ? (a == 7) : (a == 0);       // reads the upstream tile
-    const bool edge_k = MODE == kBackward ? (b == 7) : (b == 0);
-    // Lanes whose results a downstream tile reads publish them coherently.
-    const bool pub = MODE == kBackward ? (a == 0 || b == 0) : (a == 7 || b == 7);
-    const long sj = (long)g.I, sk = (long)g.I * g.J;
-    const long base = sj * j + sk * k;
-    const long dy = MODE == kBackward ? sj : -sj, dz = MODE == kBackward ? sk : -sk;
-    const unsigned total = (unsigned)(g.I + 14);
-    const int imin = pb.tile_irange[2 * tile], imax = pb.tile_irange[2 * tile + 1];
+    const int up_j = jt > 0 ? tile - 1 : -1;
+    const int up_k = kt > 0 ? tile - ntj : -1;
+    const int src_j = (lane - 1) & 63, src_k = (lane - 8) & 63;
+    const bool edge_j = a == 0, edge_k = b == 0;
+    const bool pub = a == 7 || b == 7;
+    const unsigned total = (unsigned)sm.S;
+    const size_t row0 = (size_t)tile * sm.S;
+    int tb, te;
+    tile_urange(pb, g, tile, false, tb, te);
 
-    double dot_acc = 0.0;
     bool ok = true;
-    if (imax >= 0) {
-        // Steps before tb and from te on touch no row of this tile.
-        const unsigned tb = MODE == kBackward ? (unsigned)(g.I - 1 - imax) : (unsigned)imin;
-        unsigned te = (MODE == kBackward ? (unsigned)(g.I - 1 - imin) : (unsigned)imax) + 15;
-        if (te > total) te = total;
-        if (lane == 0) st_dev(&pb.progress[tile], tb);
-
-        // Cell index of this lane at step t, or -1 if it is not interior.
-        auto cell_at = [&](unsigned t) -> long {
-            int step = (int)t - skew;
-            int i = MODE == kBackward ? g.I - 1 - step : step;
-            if (!col_ok || step < 0 || step >= g.I || i < 1 || i > g.I - 2) return -1;
-            return base + i;
+    if (te > tb) {
+        if (lane == 0) st_dev(&pb.progress[tile], (unsigned)tb);
+        auto slot_at = [&](int t) -> size_t { return ((row0 + (size_t)t) << 6) + (size_t)lane; };
+        auto live_at = [&](int t) -> bool {
+            int i = t - a - b;
+            return col_ok && i >= 1 && i <= g.I - 2;
         };
-        auto prefetch = [&](unsigned t0, double (&v)[NV][C]) {
+        double curLx = 0.0, curLy = 0.0, curLz = 0.0;
+        for (int t0 = tb; t0 < te; t0 += C) {
+            const int t1 = t0 + C < te ? t0 + C : te;
+            unsigned need = (unsigned)t1 + 7;
+            if (need > total) need = total;
+            double v[4][C];
 #pragma unroll
             for (int u = 0; u < C; u++) {
-                long c = cell_at(t0 + u);
-                if (c < 0) {
-#pragma unroll
-                    for (int q = 0; q < NV; q++) v[q][u] = 0.0;
-                    continue;
-                }
-                if (MODE == kFactor) {
-                    v[0][u] = pb.diag[c]; v[1][u] = pb.offx[c];
-                    v[2][u] = pb.offy[c]; v[3][u] = pb.offz[c];
-                } else if (MODE == kForward) {
-                    v[0][u] = in[c]; v[1][u] = pb.Lz[c + dz]; v[2][u] = pb.Ly[c + dy];
-                    v[3][u] = pb.Lx[c - 1]; v[4][u] = pb.invdiag[c];
-                } else {
-                    v[0][u] = out[c]; v[1][u] = pb.Lx[c]; v[2][u] = pb.Ly[c];
-                    v[3][u] = pb.Lz[c]; v[4][u] = pb.invdiag[c];
-                    v[5][u] = dotmode != kDotNone ? rvec[c] : 0.0;
-                }
+                bool live = t0 + u < t1 && live_at(t0 + u);
+                size_t p = slot_at(t0 + u);
+                v[0][u] = live ? pb.diag[p] : 0.0;
+                v[1][u] = live ? pb.offx[p] : 0.0;
+                v[2][u] = live ? pb.offy[p] : 0.0;
+                v[3][u] = live ? pb.offz[p] : 0.0;
             }
-        };
-
-        double cur = 0.0, curLx = 0.0, curLy = 0.0, curLz = 0.0;
-        double nxt[NV][C];
-        prefetch(tb, nxt);
-        for (unsigned t0 = tb; t0 < te; t0 += C) {
-            const unsigned t1 = t0 + C < te ? t0 + C : te;
-            unsigned need = t1 + 7;
-            if (need > total) need = total;
-            double v[NV][C];
-#pragma unroll
-            for (int q = 0; q < NV; q++)
-#pragma unroll
-                for (int u = 0; u < C; u++) v[q][u] = nxt[q][u];
             if (up_j >= 0 || up_k >= 0) ok = wait_progress2(pb, up_j, up_k, need);
             if (!ok) break;
-            // Upstream tiles' values for this chunk (one batch of loads).
-            double ey[NE][C], ez[NE][C];
+            double ey[3][C], ez[3][C];
 #pragma unroll
             for (int u = 0; u < C; u++) {
-                long c = cell_at(t0 + u);
-                bool live = c >= 0 && t0 + u < t1;
+                int t = t0 + u;
+                bool live = t < t1 && live_at(t);
+                int i = t - a - b;
 #pragma unroll
-                for (int q = 0; q < NE; q++) { ey[q][u] = 0.0; ez[q][u] = 0.0; }
+                for (int q = 0; q < 3; q++) { ey[q][u] = 0.0; ez[q][u] = 0.0; }
                 if (live && edge_j) {
-                    if (MODE == kFactor) {
-                        ey[0][u] = ld_dev(&pb.Lx[c + dy]); ey[1][u] = ld_dev(&pb.Ly[c + dy]);
-                        ey[2][u] = ld_dev(&pb.Lz[c + dy]);
-                    } else {
-                        ey[0][u] = ld_dev(&out[c + dy]);
-                    }
+                    size_t pn = sm.slot(i, j - 1, k);
+                    ey[0][u] = ld_dev(&pb.Lx[pn]); ey[1][u] = ld_dev(&pb.Ly[pn]);
+                    ey[2][u] = ld_dev(&pb.Lz[pn]);
                 }
                 if (live && edge_k) {
-                    if (MODE == kFactor) {
-                        ez[0][u] = ld_dev(&pb.Lx[c + dz]); ez[1][u] = ld_dev(&pb.Ly[c + dz]);
-                        ez[2][u] = ld_dev(&pb.Lz[c + dz]);
-                    } else {
-                        ez[0][u] = ld_dev(&out[c + dz]);
-                    }
+                    size_t pn = sm.slot(i, j, k - 1);
+                    ez[0][u] = ld_dev(&pb.Lx[pn]); ez[1][u] = ld_dev(&pb.Ly[pn]);
+                    ez[2][u] = ld_dev(&pb.Lz[pn]);
                 }
             }
-            // Next chunk's operands stream in while this chunk computes; loads
-            // complete in issue order, so the edge values above are not held
-            // up by them.
-            if (t0 + C < te) prefetch(t0 + C, nxt);
 #pragma unroll
             for (int u = 0; u < C; u++) {
-                const unsigned t = t0 + u;
-                // Values produced one step earlier by the neighbouring lanes.
-                double nj_cur = shfl_d(cur, src_j), nk_cur = shfl_d(cur, src_k);
-                double nj_Lx = 0, nj_Ly = 0, nj_Lz = 0, nk_Lx = 0, nk_Ly = 0, nk_Lz = 0;
-                if (MODE == kFactor) {
-                    nj_Lx = shfl_d(curLx, src_j); nj_Ly = shfl_d(curLy, src_j);
-                    nj_Lz = shfl_d(curLz, src_j); nk_Lx = shfl_d(curLx, src_k);
-                    nk_Ly = shfl_d(curLy, src_k); nk_Lz = shfl_d(curLz, src_k);
-                }
+                const int t = t0 + u;
+                double nj_Lx = shfl_d(curLx, src_j), nj_Ly = shfl_d(curLy, src_j);
+                double nj_Lz = shfl_d(curLz, src_j), nk_Lx = shfl_d(curLx, src_k);
+                double nk_Ly = shfl_d(curLy, src_k), nk_Lz = shfl_d(curLz, src_k);
                 if (t >= t1) continue;  // uniform across the wave
-                long c = cell_at(t);
-                if (c < 0) {
-                    cur = 0.0; curLx = curLy = curLz = 0.0;
+                if (!live_at(t)) {
+                    curLx = curLy = curLz = 0.0;
                     continue;
                 }
-                if (MODE == kFactor) {
-                    double yLx = edge_j ? ey[0][u] : nj_Lx, yLy = edge_j ? ey[1][u] : nj_Ly;
-                    double yLz = edge_j ? ey[2][u] : nj_Lz;
-                    double zLx = edge_k ? ez[0][u] : nk_Lx, zLy = edge_k ? ez[1][u] : nk_Ly;
-                    double zLz = edge_k ? ez[2][u] : nk_Lz;
-                    const double tau = 0.97, sigma = 0.25;  // pcgsolver.h:72-73
-                    double A = v[0][u];
-                    double d = A;
-                    double L, missing;
-                    L = zLz; missing = 0; missing += zLx; missing += zLy;   // -z neighbour
-                    d -= L * L; d -= tau * L * missing;
-                    L = yLy; missing = 0; missing += yLx; missing += yLz;   // -y neighbour
-                    d -= L * L; d -= tau * L * missing;
-                    L = curLx; missing = 0; missing += curLy; missing += curLz;  // -x neighbour
-                    d -= L * L; d -= tau * L * missing;
-                    double inv;
-                    double fx = v[1][u], fy = v[2][u], fz = v[3][u];
-                    if (A == 0) {
-                        inv = d;
-                    } else {
-                        inv = 1 / sqrt(d < sigma * A ? A : d);
-                        fx = fx * inv; fy = fy * inv; fz = fz * inv;
-                    }
-                    if (pub) {
-                        st_dev(&pb.invdiag[c], inv); st_dev(&pb.Lx[c], fx);
-                        st_dev(&pb.Ly[c], fy); st_dev(&pb.Lz[c], fz);
-                    } else {
-                        pb.invdiag[c] = inv; pb.Lx[c] = fx; pb.Ly[c] = fy; pb.Lz[c] = fz;
-                    }
-                    curLx = fx; curLy = fy; curLz = fz;
-                } else if (MODE == kForward) {
-                    double qy = edge_j ? ey[0][u] : nj_cur;
-                    double qz = edge_k ? ez[0][u] : nk_cur;
-                    double q = v[0][u];
-                    q -= v[1][u] * qz;
-                    q -= v[2][u] * qy;
-                    q -= v[3][u] * cur;
-                    q *= v[4][u];
-                    if (pub) st_dev(&out[c], q); else out[c] = q;
-                    cur = q;
+                double yLx = edge_j ? ey[0][u] : nj_Lx, yLy = edge_j ? ey[1][u] : nj_Ly;
+                double yLz = edge_j ? ey[2][u] : nj_Lz;
+                double zLx = edge_k ? ez[0][u] : nk_Lx, zLy = edge_k ? ez[1][u] : nk_Ly;
+                double zLz = edge_k ? ez[2][u] : nk_Lz;
+                const double tau = 0.97, sigma = 0.25;  // pcgsolver.h:72-73
+                double A = v[0][u];
+                double d = A;
+                double L, missing;
+                // Columns in increasing index order: -z, -y, -x neighbour.
+                L = zLz; missing = 0; missing += zLx; missing += zLy;
+                d -= L * L; d -= tau * L * missing;
+                L = yLy; missing = 0; missing += yLx; missing += yLz;
+                d -= L * L; d -= tau * L * missing;
+                L = curLx; missing = 0; missing += curLy; missing += curLz;
+                d -= L * L; d -= tau * L * missing;
+                double inv;
+                double fx = v[1][u], fy = v[2][u], fz = v[3][u];
+                if (A == 0) {
+                    inv = d;
                 } else {
-                    double yy = edge_j ? ey[0][u] : nj_cur;
-                    double yz = edge_k ? ez[0][u] : nk_cur;
-                    double q = v[0][u];
-                    q -= v[1][u] * cur;
-                    q -= v[2][u] * yy;
-                    q -= v[3][u] * yz;
-                    q *= v[4][u];
-                    if (pub) st_dev(&out[c], q); else out[c] = q;
-                    cur = q;
-                    dot_acc += q * v[5][u];
+                    inv = 1 / sqrt(d < sigma * A ? A : d);
+                    fx = fx * inv; fy = fy * inv; fz = fz * inv;
                 }
+                size_t p = slot_at(t);
+                if (pub) {
+                    st_dev(&pb.invdiag[p], inv); st_dev(&pb.Lx[p], fx);
+                    st_dev(&pb.Ly[p], fy); st_dev(&pb.Lz[p], fz);
+                } else {
+                    pb.invdiag[p] = inv; pb.Lx[p] = fx; pb.Ly[p] = fy; pb.Lz[p] = fz;
+                }
+                curLx = fx; curLy = fy; curLz = fz;
             }
             // Publish: this chunk's stores have completed before the flag moves.
             drain_memory();
-            if (lane == 0) st_dev(&pb.progress[tile], t1);
+            if (lane == 0) st_dev(&pb.progress[tile], (unsigned)t1);
         }
     }
     drain_memory();
     if (lane == 0) st_dev(&pb.progress[tile], total);
+    finish_sweep(pb, kDotNone, 0.0, tile, true);
+}
 
-    if (MODE == kBackward && dotmode != kDotNone) {
-        double part = wave_sum_fixed(dot_acc);
-        if (lane == 0) st_dev(&pb.partials[tile], part);
-    }
-    // The last wave to finish reduces the dot product and resets the
-    // progress counters and the ticket for the next launch.
-    drain_memory();
-    unsigned prev = 0;
-    if (lane == 0) prev = atomicAdd(&pb.sc->counter[0], 1u);
-    prev = __shfl(prev, 0, 64);
-    if (prev != (unsigned)ntiles - 1) return;
-    for (int q = lane; q < ntiles; q += 64) pb.progress[q] = 0;
-    if (lane == 0) {
-        *pb.ticket = 0;
-        pb.sc->counter[0] = 0;
-    }
-    if (MODE == kBackward && dotmode != kDotNone) {
-        // Fixed-order sum of the per-tile partials.
-        double acc = 0.0;
-        for (int q = lane; q < ntiles; q += 64) acc += ld_dev(&pb.partials[q]);
-        double rho = wave_sum_fixed(acc);
-        if (lane == 0) {
-            PcgScalars *sc = pb.sc;
-            if (dotmode == kDotInit) {
-                sc->rho = rho;
-                if (rho == 0 || rho != rho) { sc->failed = 1; sc->done = 1; }
-            } else {
-                double beta = rho / sc->rho;
-                sc->beta = beta;
-                sc->beta_f = (float)beta;
-                sc->rho = rho;
-            }
+// The factor's entries seen from each row: Lxm(c) = Lx(c - x) etc., so the
+// forward solve reads all its operands from the row's own slot.
+__global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
+    size_t n = pb.sm.count();
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode_slot(pb.sm, p, i, j, k);
+        double lx = 0.0, ly = 0.0, lz = 0.0;
+        if (interior(g, i, j, k)) {
+            lx = pb.Lx[p - 64];
+            ly = pb.Ly[pb.sm.slot(i, j - 1, k)];
+            lz = pb.Lz[pb.sm.slot(i, j, k - 1)];
         }
+        pb.Lxm[p] = lx;
+        pb.Lym[p] = ly;
+        pb.Lzm[p] = lz;
     }
 }
 
-static hipError_t launch_wave(int mode, const GridGeom &g, PressureBuffers &pb, const double *in,
-                              double *out, const double *rvec, int dotmode, hipStream_t s) {
-    int ntiles = pb.ntiles_j * pb.ntiles_k;
-    if (mode == kFactor) k_wavefront<kFactor><<<ntiles, 64, 0, s>>>(g, pb, in, out, rvec, dotmode);
-    else if (mode == kForward) k_wavefront<kForward><<<ntiles, 64, 0, s>>>(g, pb, in, out, rvec, dotmode);
-    else k_wavefront<kBackward><<<ntiles, 64, 0, s>>>(g, pb, in, out, rvec, dotmode);
+// ---------------------------------------------------------------------------
+// Triangular solves
+// ---------------------------------------------------------------------------
+// DIR kForward : out = L^-1 in              (solveLower, pcgsolver.h:191-204)
+// DIR kBackward: out = L^-T out, in place, optional dot(out, dotvec)
+//                                           (solveLowerTransposeInPlace,
+//                                            pcgsolver.h:207-221)
+//
+// One wave per tile marches over u (forward: step t = u; backward: t =
+// S - 1 - u) in chunks of kChunk steps.  The operands of the next chunk and
+// the tagged records the next chunk needs from the upstream tiles are loaded
+// while the current chunk computes.  A cell at march step u needs its
+// upstream neighbour's value from that tile's march step u + 7, published
+// there as record (tile, u, lane-within-edge).  Lane L of the consumer loads
+// the record for step u0 + 7 + L / 8, edge lane L % 8, and the values are
+// then handed to the edge lanes with shuffles.
+constexpr int kChunk = 8;  // 8 steps x 8 edge lanes = one record per lane
+
+template <int DIR>
+__global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb,
+                                              const double *__restrict__ in, double *out,
+                                              const double *__restrict__ dotvec, int dotmode) {
+    constexpr int C = kChunk;
+    constexpr bool FWD = DIR == kForward;
+    constexpr int NV = FWD ? 5 : 6;
+    if (ld_dev(&pb.sc->done) != 0) return;
+    const int lane = threadIdx.x;
+    const SlotMap sm = pb.sm;
+    const int S = sm.S, ntj = sm.ntj, ntk = sm.ntk;
+    int jt, kt;
+    const int tile = take_tile(pb, !FWD, jt, kt);
+    const int a = lane & 7, b = lane >> 3;
+    const int j = jt * kTile + a, k = kt * kTile + b;
+    const bool col_ok = j >= 1 && k >= 1 && j <= g.J - 2 && k <= g.K - 2;
+    const int up_j = FWD ? (jt > 0 ? tile - 1 : -1) : (jt + 1 < ntj ? tile + 1 : -1);
+    const int up_k = FWD ? (kt > 0 ? tile - ntj : -1) : (kt + 1 < ntk ? tile + ntj : -1);
+    const int src_j = (FWD ? lane - 1 : lane + 1) & 63;  // neighbour lane along j
+    const int src_k = (FWD ? lane - 8 : lane + 8) & 63;  // neighbour lane along k
+    const bool edge_j = FWD ? a == 0 : a == 7;            // reads the upstream j tile
+    const bool edge_k = FWD ? b == 0 : b == 7;
+    const bool pub_j = FWD ? a == 7 : a == 0;             // writes for the downstream j tile
+    const bool pub_k = FWD ? b == 7 : b == 0;
+    const unsigned epoch = pb.epoch;
+    const size_t row0 = (size_t)tile * S;
+    const bool dot = !FWD && dotmode != kDotNone;
+
+    int tb, te, ubj = 0, uej = 0, ubk = 0, uek = 0;
+    tile_urange(pb, g, tile, !FWD, tb, te);
+    if (up_j >= 0) tile_urange(pb, g, up_j, !FWD, ubj, uej);
+    if (up_k >= 0) tile_urange(pb, g, up_k, !FWD, ubk, uek);
+    const int rs = lane >> 3, re = lane & 7;  // record this lane loads: step offset, edge lane
+
+    double dot_acc = 0.0;
+    if (te > tb) {
+        auto slot_at = [&](int u) -> size_t {
+            int t = FWD ? u : S - 1 - u;
+            return ((row0 + (size_t)t) << 6) + (size_t)lane;
+        };
+        auto live_at = [&](int u) -> bool {
+            int t = FWD ? u : S - 1 - u;
+            int i = t - a - b;
+            return col_ok && i >= 1 && i <= g.I - 2;
+        };
+        auto prefetch = [&](int u0, double (&v)[NV][C]) {
+#pragma unroll
+            for (int s = 0; s < C; s++) {
+                const int u = u0 + s;
+                if (u >= te) {
+#pragma unroll
+                    for (int q = 0; q < NV; q++) v[q][s] = 0.0;
+                    continue;
+                }
+                size_t p = slot_at(u);
+                if (FWD) {
+                    v[0][s] = in[p]; v[1][s] = pb.Lzm[p]; v[2][s] = pb.Lym[p];
+                    v[3][s] = pb.Lxm[p]; v[4][s] = pb.invdiag[p];
+                } else {
+                    v[0][s] = out[p]; v[1][s] = pb.Lx[p]; v[2][s] = pb.Ly[p];
+                    v[3][s] = pb.Lz[p]; v[4][s] = pb.invdiag[p];
+                    v[5][s] = dot ? dotvec[p] : 0.0;
+                }
+            }
+        };
+        // Whether this lane's record of chunk u0 is needed: the consumer step
+        // exists and the producer tile processes the producer step.
+        auto need_rec = [&](int u0, int ub, int ue) -> bool {
+            int uu = u0 + 7 + rs;
+            return u0 + rs < te && uu >= ub && uu < ue;
+        };
+        auto rec_ptr = [&](unsigned long long *base, int up, int u0) -> unsigned long long * {
+            return base + ((((size_t)up * S + (size_t)(u0 + 7 + rs)) * kTile + re) * kRecWords);
+        };
+        auto load_recs = [&](int u0, unsigned long long (&rj)[2], unsigned long long (&rk)[2]) {
+            rj[0] = rj[1] = rk[0] = rk[1] = 0ull;
+            if (up_j >= 0 && need_rec(u0, ubj, uej)) {
+                const unsigned long long *q = rec_ptr(pb.edge_j, up_j, u0);
+                rj[0] = ld_dev(q); rj[1] = ld_dev(q + 1);
+            }
+            if (up_k >= 0 && need_rec(u0, ubk, uek)) {
+                const unsigned long long *q = rec_ptr(pb.edge_k, up_k, u0);
+                rk[0] = ld_dev(q); rk[1] = ld_dev(q + 1);
+            }
+        };
+
+        double cur = 0.0;
+        double nxt[NV][C];
+        unsigned long long nrj[2], nrk[2];
+        prefetch(tb, nxt);
+        load_recs(tb, nrj, nrk);
+        bool aborted = false;
+        for (int u0 = tb; u0 < te; u0 += C) {
+            double v[NV][C];
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+#pragma unroll
+                for (int s = 0; s < C; s++) v[q][s] = nxt[q][s];
+            unsigned long long rj[2] = {nrj[0], nrj[1]}, rk[2] = {nrk[0], nrk[1]};
+            if (u0 + C < te) {
+                prefetch(u0 + C, nxt);
+                load_recs(u0 + C, nrj, nrk);
+            }
+            // Records of this chunk: poll until every needed one carries this
+            // launch's epoch.
+            const bool nj = up_j >= 0 && need_rec(u0, ubj, uej);
+            const bool nk = up_k >= 0 && need_rec(u0, ubk, uek);
+            unsigned spins = 0;
+            for (;;) {
+                bool bad = (nj && !record_ok(rj[0], rj[1], epoch)) ||
+                           (nk && !record_ok(rk[0], rk[1], epoch));
+                if (__ballot(bad) == 0ull) break;
+                if (ld_dev(&pb.sc->abort) != 0 || ++spins > kSpinLimit) {
+                    if (lane == 0) st_dev(&pb.sc->abort, 1);
+                    aborted = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                if (nj && !record_ok(rj[0], rj[1], epoch)) {
+                    const unsigned long long *q = rec_ptr(pb.edge_j, up_j, u0);
+                    rj[0] = ld_dev(q); rj[1] = ld_dev(q + 1);
+                }
+                if (nk && !record_ok(rk[0], rk[1], epoch)) {
+                    const unsigned long long *q = rec_ptr(pb.edge_k, up_k, u0);
+                    rk[0] = ld_dev(q); rk[1] = ld_dev(q + 1);
+                }
+            }
+            if (aborted) break;
+            const double vj = nj ? record_value(rj[0], rj[1]) : 0.0;
+            const double vk = nk ? record_value(rk[0], rk[1]) : 0.0;
+            double evj[C], evk[C];
+#pragma unroll
+            for (int s = 0; s < C; s++) {
+                evj[s] = shfl_d(vj, s * kTile + b);
+                evk[s] = shfl_d(vk, s * kTile + a);
+            }
+#pragma unroll
+            for (int s = 0; s < C; s++) {
+                const int u = u0 + s;
+                // Values produced one step earlier by the neighbouring lanes.
+                double nj_cur = shfl_d(cur, src_j), nk_cur = shfl_d(cur, src_k);
+                if (u >= te) continue;  // uniform across the wave
+                double qy = edge_j ? evj[s] : nj_cur;
+                double qz = edge_k ? evk[s] : nk_cur;
+                double q = v[0][s];
+                if (FWD) {
+                    q -= v[1][s] * qz;
+                    q -= v[2][s] * qy;
+                    q -= v[3][s] * cur;
+                    q *= v[4][s];
+                } else {
+                    q -= v[1][s] * cur;
+                    q -= v[2][s] * qy;
+                    q -= v[3][s] * qz;
+                    q *= v[4][s];
+                    dot_acc += q * v[5][s];
+                }
+                if (live_at(u)) out[slot_at(u)] = q;
+                if (pub_j) put_record(pb.edge_j + (((size_t)tile * S + u) * kTile + b) * kRecWords, q, epoch);
+                if (pub_k) put_record(pb.edge_k + (((size_t)tile * S + u) * kTile + a) * kRecWords, q, epoch);
+                cur = q;
+            }
+        }
+    }
+    finish_sweep(pb, dot ? dotmode : kDotNone, dot_acc, tile, false);
+}
+
+static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, const double *in,
+                               double *out, const double *dotvec, int dotmode, hipStream_t s) {
+    // A fresh epoch per launch; records of older launches never match it.
+    if (++pb.epoch >= 0xfffffff0u) {
+        size_t n = edge_record_count(pb.sm);
+        hipError_t e = launch_fill_u8((uint8_t *)pb.edge_j, n * 8, 0, s);
+        if (e == hipSuccess) e = launch_fill_u8((uint8_t *)pb.edge_k, n * 8, 0, s);
+        if (e != hipSuccess) return e;
+        pb.epoch = 1;
+    }
+    int ntiles = pb.sm.ntj * pb.sm.ntk;
+    if (dir == kForward) k_sweep<kForward><<<ntiles, 64, 0, s>>>(g, pb, in, out, dotvec, dotmode);
+    else k_sweep<kBackward><<<ntiles, 64, 0, s>>>(g, pb, in, out, dotvec, dotmode);
     return hipGetLastError();
 }
 
 hipError_t launch_mic0_factor(const GridGeom &g, const float *liquid, PressureBuffers &pb,
                               hipStream_t s) {
-    size_t nc = (size_t)g.I * g.J * g.K;
+    size_t n = pb.sm.count();
     hipError_t e;
-    if ((e = launch_fill_f64(pb.invdiag, nc, 0.0, s)) != hipSuccess) return e;
-    if ((e = launch_fill_f64(pb.Lx, nc, 0.0, s)) != hipSuccess) return e;
-    if ((e = launch_fill_f64(pb.Ly, nc, 0.0, s)) != hipSuccess) return e;
-    if ((e = launch_fill_f64(pb.Lz, nc, 0.0, s)) != hipSuccess) return e;
-    k_tile_irange<<<pb.ntiles_j * pb.ntiles_k, 64, 0, s>>>(g, liquid, pb);
-    return launch_wave(kFactor, g, pb, nullptr, nullptr, nullptr, kDotNone, s);
+    if ((e = launch_fill_f64(pb.invdiag, n, 0.0, s)) != hipSuccess) return e;
+    if ((e = launch_fill_f64(pb.Lx, n, 0.0, s)) != hipSuccess) return e;
+    if ((e = launch_fill_f64(pb.Ly, n, 0.0, s)) != hipSuccess) return e;
+    if ((e = launch_fill_f64(pb.Lz, n, 0.0, s)) != hipSuccess) return e;
+    int ntiles = pb.sm.ntj * pb.sm.ntk;
+    k_tile_irange<<<ntiles, 64, 0, s>>>(g, liquid, pb);
+    k_factor<<<ntiles, 64, 0, s>>>(g, pb);
+    k_lower_couplings<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb);
+    return hipGetLastError();
 }
 
 hipError_t launch_precond_apply(const GridGeom &g, PressureBuffers &pb, const double *r,
                                 double *z, hipStream_t s) {
-    hipError_t e = launch_wave(kForward, g, pb, r, z, nullptr, kDotNone, s);
+    hipError_t e = launch_sweep(kForward, g, pb, r, z, nullptr, kDotNone, s);
     if (e != hipSuccess) return e;
-    return launch_wave(kBackward, g, pb, nullptr, z, nullptr, kDotNone, s);
+    return launch_sweep(kBackward, g, pb, nullptr, z, nullptr, kDotNone, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -529,23 +739,24 @@ __device__ bool last_block(unsigned *counter) {
     return is_last;
 }
 
-__device__ __forceinline__ bool interior_cell(const GridGeom &g, size_t c, int &i, int &j, int &k) {
-    i = (int)(c % g.I); j = (int)((c / g.I) % g.J); k = (int)(c / ((size_t)g.I * g.J));
-    return i >= 1 && j >= 1 && k >= 1 && i <= g.I - 2 && j <= g.J - 2 && k <= g.K - 2;
-}
-
-// y = A x (sparsematrix.h:187-216: row entries in increasing column order).
-__device__ __forceinline__ double spmv_row(const GridGeom &g, const PressureBuffers &pb,
-                                           const double *__restrict__ x, size_t c) {
-    size_t sj = (size_t)g.I, sk = (size_t)g.I * g.J;
+// y = A x at slot p of cell (i, j, k) (sparsematrix.h:187-216: row entries
+// in increasing column order).
+__device__ __forceinline__ double spmv_slot(const PressureBuffers &pb, const double *__restrict__ x,
+                                            size_t p, int i, int j, int k) {
+    const SlotMap &sm = pb.sm;
+    const int a = j & 7, b = k & 7;
+    const size_t pzm = b > 0 ? p - 72 : sm.slot(i, j, k - 1);
+    const size_t pym = a > 0 ? p - 65 : sm.slot(i, j - 1, k);
+    const size_t pyp = a < 7 ? p + 65 : sm.slot(i, j + 1, k);
+    const size_t pzp = b < 7 ? p + 72 : sm.slot(i, j, k + 1);
     double s = 0;
-    s += pb.offz[c - sk] * x[c - sk];
-    s += pb.offy[c - sj] * x[c - sj];
-    s += pb.offx[c - 1] * x[c - 1];
-    s += pb.diag[c] * x[c];
-    s += pb.offx[c] * x[c + 1];
-    s += pb.offy[c] * x[c + sj];
-    s += pb.offz[c] * x[c + sk];
+    s += pb.offz[pzm] * x[pzm];
+    s += pb.offy[pym] * x[pym];
+    s += pb.offx[p - 64] * x[p - 64];
+    s += pb.diag[p] * x[p];
+    s += pb.offx[p] * x[p + 64];
+    s += pb.offy[p] * x[pyp];
+    s += pb.offz[p] * x[pzp];
     return s;
 }
 
@@ -554,15 +765,16 @@ __device__ __forceinline__ double spmv_row(const GridGeom &g, const PressureBuff
 __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBuffers pb) {
     __shared__ double sh[kRedThreads / 64];
     if (pb.sc->done) return;
-    size_t nc = (size_t)g.I * g.J * g.K;
+    size_t n = pb.sm.count();
     double acc = 0.0;
-    for (size_t c = (size_t)blockIdx.x * kRedThreads + threadIdx.x; c < nc;
-         c += (size_t)gridDim.x * kRedThreads) {
+    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * kRedThreads) {
         int i, j, k;
+        decode_slot(pb.sm, p, i, j, k);
         double zc = 0.0;
-        if (interior_cell(g, c, i, j, k)) zc = spmv_row(g, pb, pb.s, c);
-        pb.z[c] = zc;
-        acc += pb.s[c] * zc;
+        if (interior(g, i, j, k)) zc = spmv_slot(pb, pb.s, p, i, j, k);
+        pb.z[p] = zc;
+        acc += pb.s[p] * zc;
     }
     double bs = block_reduce<0>(acc, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[blockIdx.x], bs);
@@ -585,15 +797,15 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBu
 __global__ __launch_bounds__(kRedThreads) void k_axpy_norm(GridGeom g, PressureBuffers pb) {
     __shared__ double sh[kRedThreads / 64];
     if (pb.sc->done) return;
-    size_t nc = (size_t)g.I * g.J * g.K;
+    size_t n = pb.sm.count();
     float af = pb.sc->alpha_f;
     float naf = (float)(-pb.sc->alpha);
     double mx = 0.0;
-    for (size_t c = (size_t)blockIdx.x * kRedThreads + threadIdx.x; c < nc;
-         c += (size_t)gridDim.x * kRedThreads) {
-        pb.x[c] += af * pb.s[c];
-        double r = pb.r[c] + naf * pb.z[c];
-        pb.r[c] = r;
+    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * kRedThreads) {
+        pb.x[p] += af * pb.s[p];
+        double r = pb.r[p] + naf * pb.z[p];
+        pb.r[p] = r;
         double a = fabs(r);
         mx = a > mx ? a : mx;
     }
@@ -623,24 +835,24 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm(GridGeom g, PressureB
 // s = z + beta s (pcgsolver.h:295-296), or s = z on the first application.
 __global__ void k_update_s(GridGeom g, PressureBuffers pb, int first) {
     if (pb.sc->done) return;
-    size_t nc = (size_t)g.I * g.J * g.K;
+    size_t n = pb.sm.count();
     float bf = first ? 0.0f : pb.sc->beta_f;
-    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (size_t)gridDim.x * blockDim.x) {
-        if (first) pb.s[c] = pb.z[c];
-        else pb.s[c] = pb.z[c] + bf * pb.s[c];
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        if (first) pb.s[p] = pb.z[p];
+        else pb.s[p] = pb.z[p] + bf * pb.s[p];
     }
 }
 
 // x = 0, r = b; scalars for a new solve.
 __global__ void k_pcg_init_vectors(GridGeom g, PressureBuffers pb) {
-    size_t nc = (size_t)g.I * g.J * g.K;
-    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (size_t)gridDim.x * blockDim.x) {
-        pb.x[c] = 0.0;
-        pb.r[c] = pb.rhs[c];
-        pb.z[c] = 0.0;
-        pb.s[c] = 0.0;
+    size_t n = pb.sm.count();
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        pb.x[p] = 0.0;
+        pb.r[p] = pb.rhs[p];
+        pb.z[p] = 0.0;
+        pb.s[p] = 0.0;
     }
 }
 
@@ -655,7 +867,6 @@ __global__ void k_pcg_init_scalars(PressureBuffers pb, double tolf, int maxiter)
     sc->done = 0;
     sc->converged = 0;
     sc->failed = 0;
-    sc->abort = 0;
     sc->maxiter = maxiter;
     sc->rho = 0;
 }
@@ -681,9 +892,9 @@ hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, Pressure
     k_pcg_init_vectors<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb);
     k_pcg_init_scalars<<<1, 1, 0, s>>>(pb, tolf, maxiter);
     // z = M^-1 r, rho = z . r, s = z (pcgsolver.h:270-279).
-    hipError_t e = launch_wave(kForward, g, pb, pb.r, pb.z, nullptr, kDotNone, s);
+    hipError_t e = launch_sweep(kForward, g, pb, pb.r, pb.z, nullptr, kDotNone, s);
     if (e != hipSuccess) return e;
-    e = launch_wave(kBackward, g, pb, nullptr, pb.z, pb.r, kDotInit, s);
+    e = launch_sweep(kBackward, g, pb, nullptr, pb.z, pb.r, kDotInit, s);
     if (e != hipSuccess) return e;
     k_update_s<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb, 1);
     return hipGetLastError();
@@ -693,8 +904,8 @@ hipError_t launch_pcg_piece(int piece, const GridGeom &g, PressureBuffers &pb, h
     switch (piece) {
         case kPcgSpmv: k_spmv_dot<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
         case kPcgAxpy: k_axpy_norm<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
-        case kPcgForward: return launch_wave(kForward, g, pb, pb.r, pb.z, nullptr, kDotNone, s);
-        case kPcgBackward: return launch_wave(kBackward, g, pb, nullptr, pb.z, pb.r, kDotIter, s);
+        case kPcgForward: return launch_sweep(kForward, g, pb, pb.r, pb.z, nullptr, kDotNone, s);
+        case kPcgBackward: return launch_sweep(kBackward, g, pb, nullptr, pb.z, pb.r, kDotIter, s);
         case kPcgUpdate: k_update_s<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb, 0); break;
         default: return hipErrorInvalidValue;
     }
@@ -711,11 +922,12 @@ hipError_t launch_pcg_iteration(const GridGeom &g, PressureBuffers &pb, hipStrea
 
 __global__ void k_spmv_plain(GridGeom g, PressureBuffers pb, const double *__restrict__ x,
                              double *y) {
-    size_t nc = (size_t)g.I * g.J * g.K;
-    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (size_t)gridDim.x * blockDim.x) {
+    size_t n = pb.sm.count();
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
         int i, j, k;
-        y[c] = interior_cell(g, c, i, j, k) ? spmv_row(g, pb, x, c) : 0.0;
+        decode_slot(pb.sm, p, i, j, k);
+        y[p] = interior(g, i, j, k) ? spmv_slot(pb, x, p, i, j, k) : 0.0;
     }
 }
 
@@ -729,7 +941,7 @@ hipError_t launch_spmv(const GridGeom &g, PressureBuffers &pb, const double *x, 
 // Pressure gradient update (pressuresolver.cpp:842-1047, no surface tension)
 // ---------------------------------------------------------------------------
 template <int COMP>
-__global__ void k_apply(GridGeom g, double dt, const float *__restrict__ liquid,
+__global__ void k_apply(GridGeom g, SlotMap sm, double dt, const float *__restrict__ liquid,
                         const float *__restrict__ weight, const double *__restrict__ p,
                         float *vel, uint8_t *valid) {
     const int ni = g.I + (COMP == 0), nj = g.J + (COMP == 1), nk = g.K + (COMP == 2);
@@ -760,8 +972,8 @@ __global__ void k_apply(GridGeom g, double dt, const float *__restrict__ liquid,
         float factor = (float)(dt / g.dx);
         float p1, p2;
         if (fa && fb) {
-            p1 = (float)p[cell(ai, aj, ak)];
-            p2 = (float)p[cell(i, j, k)];
+            p1 = (float)p[sm.slot(ai, aj, ak)];
+            p2 = (float)p[sm.slot(i, j, k)];
         } else {
             float phi1 = liquid[cell(ai, aj, ak)], phi2 = liquid[cell(i, j, k)];
             const double tension = 0.0;
@@ -772,7 +984,7 @@ __global__ void k_apply(GridGeom g, double dt, const float *__restrict__ liquid,
                 float tt = (phi1 - phi2) / (phi1 + eps);
                 double ttd = (double)tt;
                 tt = (float)(ttd < -kMaxTheta ? -kMaxTheta : (ttd > kMaxTheta ? kMaxTheta : ttd));
-                p1 = (float)p[cell(ai, aj, ak)];
+                p1 = (float)p[sm.slot(ai, aj, ak)];
                 p2 = (float)(tt * tension + tp * p1);
             } else {
                 float tp = phi1 / (phi2 + eps);
@@ -781,7 +993,7 @@ __global__ void k_apply(GridGeom g, double dt, const float *__restrict__ liquid,
                 float tt = (phi2 - phi1) / (phi2 + eps);
                 double ttd = (double)tt;
                 tt = (float)(ttd < -kMaxTheta ? -kMaxTheta : (ttd > kMaxTheta ? kMaxTheta : ttd));
-                p2 = (float)p[cell(i, j, k)];
+                p2 = (float)p[sm.slot(i, j, k)];
                 p1 = (float)(tt * tension + tp * p2);
             }
         }
@@ -793,15 +1005,15 @@ __global__ void k_apply(GridGeom g, double dt, const float *__restrict__ liquid,
     valid[f] = vflag;
 }
 
-hipError_t launch_pressure_apply(const GridGeom &g, double dt, const float *liquid,
+hipError_t launch_pressure_apply(const GridGeom &g, const SlotMap &sm, double dt, const float *liquid,
                                  const float *wU, const float *wV, const float *wW,
                                  const double *p, float *u, float *v, float *w, uint8_t *vu,
                                  uint8_t *vv, uint8_t *vw, hipStream_t s) {
     size_t nU = (size_t)(g.I + 1) * g.J * g.K, nV = (size_t)g.I * (g.J + 1) * g.K;
     size_t nW = (size_t)g.I * g.J * (g.K + 1);
-    k_apply<0><<<(unsigned)((nU + 255) / 256), 256, 0, s>>>(g, dt, liquid, wU, p, u, vu);
-    k_apply<1><<<(unsigned)((nV + 255) / 256), 256, 0, s>>>(g, dt, liquid, wV, p, v, vv);
-    k_apply<2><<<(unsigned)((nW + 255) / 256), 256, 0, s>>>(g, dt, liquid, wW, p, w, vw);
+    k_apply<0><<<(unsigned)((nU + 255) / 256), 256, 0, s>>>(g, sm, dt, liquid, wU, p, u, vu);
+    k_apply<1><<<(unsigned)((nV + 255) / 256), 256, 0, s>>>(g, sm, dt, liquid, wV, p, v, vv);
+    k_apply<2><<<(unsigned)((nW + 255) / 256), 256, 0, s>>>(g, sm, dt, liquid, wW, p, w, vw);
     return hipGetLastError();
 }
 

@@ -109,25 +109,36 @@ void collision_box(int isize, int jsize, int ksize, double dx, float pos[3], dou
 // ---------------------------------------------------------------------------
 
 void PressureStore::alloc(const GridGeom &g, hipStream_t s) {
-    size_t nc = (size_t)g.I * g.J * g.K;
-    int ntj = (g.J + 7) / 8, ntk = (g.K + 7) / 8;
+    SlotMap sm = make_slot_map(g.I, g.J, g.K);
+    size_t ns = sm.count();
+    int ntj = sm.ntj, ntk = sm.ntk;
     int npart = 2 * 1024 + ntj * ntk;
-    for (int q = 0; q < 13; q++)
-        if (arr[q].alloc(nc) != hipSuccess)
+    for (int q = 0; q < kSlotArrays; q++)
+        if (arr[q].alloc(ns) != hipSuccess)
             throw StatusError{FLIPMI_ERR_OOM, "pressure arrays: out of device memory"};
-    if (arr[13].alloc((size_t)npart) != hipSuccess ||
-        progress.alloc((size_t)ntj * ntk) != hipSuccess || ticket.alloc(1) != hipSuccess ||
-        scal.alloc(1) != hipSuccess || irange.alloc((size_t)2 * ntj * ntk) != hipSuccess)
+    size_t nrec = edge_record_count(sm);
+    if (arr[kSlotArrays].alloc((size_t)npart) != hipSuccess || edge_j.alloc(nrec) != hipSuccess ||
+        edge_k.alloc(nrec) != hipSuccess || progress.alloc((size_t)ntj * ntk) != hipSuccess ||
+        ticket.alloc(1) != hipSuccess || scal.alloc(1) != hipSuccess ||
+        irange.alloc((size_t)2 * ntj * ntk) != hipSuccess)
         throw StatusError{FLIPMI_ERR_OOM, "pressure scratch: out of device memory"};
     (void)launch_fill_i32((int *)progress.p, progress.n, 0, s);
     (void)launch_fill_i32((int *)ticket.p, 1, 0, s);
     (void)launch_fill_u8((uint8_t *)scal.p, sizeof(PcgScalars), 0, s);
-    (void)launch_fill_f64(arr[0].p, nc, 0.0, s);
+    // Slots outside the grid are never written by the kernels: start at zero.
+    for (int q = 0; q < kSlotArrays; q++) (void)launch_fill_f64(arr[q].p, ns, 0.0, s);
+    (void)launch_fill_u8((uint8_t *)edge_j.p, nrec * sizeof(unsigned long long), 0, s);
+    (void)launch_fill_u8((uint8_t *)edge_k.p, nrec * sizeof(unsigned long long), 0, s);
+    pb.sm = sm;
     pb.diag = arr[0].p; pb.offx = arr[1].p; pb.offy = arr[2].p; pb.offz = arr[3].p;
     pb.rhs = arr[4].p; pb.x = arr[5].p; pb.r = arr[6].p; pb.z = arr[7].p; pb.s = arr[8].p;
     pb.invdiag = arr[9].p; pb.Lx = arr[10].p; pb.Ly = arr[11].p; pb.Lz = arr[12].p;
-    pb.partials = arr[13].p;
+    pb.Lxm = arr[13].p; pb.Lym = arr[14].p; pb.Lzm = arr[15].p;
+    pb.partials = arr[kSlotArrays].p;
     pb.npartials = npart;
+    pb.edge_j = edge_j.p;
+    pb.edge_k = edge_k.p;
+    pb.epoch = 0;
     pb.progress = progress.p;
     pb.ntiles_j = ntj;
     pb.ntiles_k = ntk;
@@ -203,7 +214,7 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
     out.success = converged || (!hs->failed && out.iterations == w.maxiter && out.residual < w.acceptable);
     if (out.success) {
         if (prof) prof->begin_stage(FLIPMI_STAGE_APPLY, &ev);
-        hip_check(launch_pressure_apply(w.g, w.dt, w.liquid, w.wU, w.wV, w.wW, pb.x, w.u, w.v, w.w,
+        hip_check(launch_pressure_apply(w.g, pb.sm, w.dt, w.liquid, w.wU, w.wV, w.wW, pb.x, w.u, w.v, w.w,
                                         w.vu, w.vv, w.vw, s),
                   "pressure apply");
         if (prof) prof->end_stage(FLIPMI_STAGE_APPLY, ev);

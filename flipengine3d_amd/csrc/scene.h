@@ -152,8 +152,13 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
                              hipStream_t s, Scene *prof);
 
 // Device storage of the pressure system for one grid size.
+// Device storage of one pressure system in slot order: arr[0..15] are the
+// slot arrays (diag, offx, offy, offz, rhs, x, r, z, s, invdiag, Lx, Ly, Lz,
+// Lxm, Lym, Lzm), arr[16] the reduction partials.
 struct PressureStore {
-    DBuf<double> arr[14];
+    static constexpr int kSlotArrays = 16;
+    DBuf<double> arr[kSlotArrays + 1];
+    DBuf<unsigned long long> edge_j, edge_k;
     DBuf<unsigned> progress, ticket;
     DBuf<int> irange;
     DBuf<PcgScalars> scal;
