@@ -217,7 +217,7 @@ def main(argv=None):
             "pressure_ok": ok,
         },
         "roofline": {
-            "kernel": f"k_sweep<{"forward" if kern == "k_forward" else "backward"}>"
+            "kernel": f"k_sweep<{'forward' if kern == 'k_forward' else 'backward'}>"
             if kern in ("k_forward", "k_backward") else kern,
             "bound": "hbm",
             "achieved": achieved,
