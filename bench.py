@@ -186,6 +186,9 @@ def main(argv=None):
     word = 4 if args.precision == "mixed" else 8
     rows_per_launch = rows_iters / max(iters, 1)
     bytes_per_launch = PCG_WORDS[kern] * word * rows_per_launch
+    # Iterations are enqueued in batches, so a few launches after convergence
+    # return at once; the average is taken over the launches that iterated.
+    kcalls = min(kcalls, iters)
     launch_s = kms * 1e-3 / max(kcalls, 1)
     achieved = bytes_per_launch / launch_s / 1e9 if launch_s > 0 else 0.0
     stages = {k: round(v[0] / args.steps, 3) for k, v in times.items() if k != "pcg_iters"}

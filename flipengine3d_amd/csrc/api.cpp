@@ -142,8 +142,8 @@ int flipmi_scene_create(flipmi_ctx *ctx, const flipmi_scene_desc *desc, flipmi_s
         if (!(desc->cfl > 0)) throw StatusError{FLIPMI_ERR_INVALID_ARG, "CFL number must be positive"};
         if (desc->pcg_max_iterations < 1)
             throw StatusError{FLIPMI_ERR_INVALID_ARG, "pcg_max_iterations must be positive"};
-        if (desc->pcg_precision != FLIPMI_PCG_FP64)
-            throw StatusError{FLIPMI_ERR_INVALID_ARG, "only FLIPMI_PCG_FP64 is available"};
+        if (desc->pcg_precision != FLIPMI_PCG_FP64 && desc->pcg_precision != FLIPMI_PCG_MIXED)
+            throw StatusError{FLIPMI_ERR_INVALID_ARG, "unknown pcg precision"};
         std::unique_ptr<flipmi_scene> sc(new flipmi_scene());
         Scene &s = sc->s;
         s.ctx = &ctx->c;
@@ -402,8 +402,10 @@ int flipmi_pressure_solve(flipmi_ctx *ctx, const flipmi_pressure_args *a, flipmi
         else hipck(launch_fill_f32(sw.p, nW, 0.0f, s), "fill");
         upload(u, a->u, nU, s); upload(v, a->v, nV, s); upload(w, a->w, nW, s);
         upload(vu, a->valid_u, nU, s); upload(vv, a->valid_v, nV, s); upload(vw, a->valid_w, nW, s);
+        if (a->precision != FLIPMI_PCG_FP64 && a->precision != FLIPMI_PCG_MIXED)
+            throw StatusError{FLIPMI_ERR_INVALID_ARG, "unknown pcg precision"};
         PressureStore ps;
-        ps.alloc(g, s);
+        ps.alloc(g, s, a->precision == FLIPMI_PCG_MIXED);
         PressureWork pw;
         pw.g = g; pw.dt = a->dt; pw.tol = a->tolerance; pw.acceptable = a->acceptable_tolerance;
         pw.maxiter = a->max_iterations; pw.precision = a->precision;
@@ -472,7 +474,7 @@ int flipmi_mic0_apply(flipmi_ctx *ctx, int I, int J, int K, double dx, double dt
         hipck(launch_fill_f32(zero_v.p, nV, 0.0f, s), "fill");
         hipck(launch_fill_f32(zero_w.p, nW, 0.0f, s), "fill");
         PressureStore ps;
-        ps.alloc(g, s);
+        ps.alloc(g, s, false);
         hipck(launch_pressure_reset(ps.pb, s), "reset");
         hipck(launch_pressure_assemble(g, dt, liq.p, wu.p, wv.p, ww.p, wc.p, zero_u.p, zero_v.p, zero_w.p,
                                        zero_u.p, zero_v.p, zero_w.p, ps.pb, s),

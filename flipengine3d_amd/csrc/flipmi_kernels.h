@@ -85,6 +85,11 @@ struct PcgScalars {
     unsigned int counter[4];
     unsigned int rows;                  // rows of the system (fluid interior cells)
     unsigned long long absmax_bits;
+    // mixed precision: true-residual replacement ("reliable update")
+    double rmax;                        // |r| at the last true residual
+    double delta;                       // replace when |r32| < delta * rmax
+    int refresh;                        // a true residual is due
+    int refinements;                    // true residuals computed
 };
 
 // Sweep-ordered ("slot") storage of the pressure system.  Cells are grouped
@@ -119,6 +124,11 @@ struct PressureBuffers {
     double *x, *r, *z, *s;              // PCG vectors (FP64)
     double *invdiag, *Lx, *Ly, *Lz;     // MIC(0) factor: column entries (rows +x/+y/+z)
     double *Lxm, *Lym, *Lzm;            // the same entries read from the row: columns -x/-y/-z
+    // Mixed precision (null unless allocated): FP32 copies of the matrix and
+    // factor, and the FP32 PCG vectors; x stays FP64.
+    float *diag32, *offx32, *offy32, *offz32;
+    float *invdiag32, *Lx32, *Ly32, *Lz32, *Lxm32, *Lym32, *Lzm32;
+    float *r32, *z32, *s32;
     double *partials;                   // per-block partial sums
     int npartials;
     unsigned long long *edge_j, *edge_k;  // tagged values a tile hands to its j / k neighbour
@@ -139,12 +149,21 @@ hipError_t launch_pressure_reset(PressureBuffers &pb, hipStream_t s);
 // Zeroes the factor, finds each column tile's row extent and factorises.
 hipError_t launch_mic0_factor(const GridGeom &g, const float *liquid, PressureBuffers &pb,
                               hipStream_t s);
-hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, PressureBuffers &pb,
-                           hipStream_t s);
-hipError_t launch_pcg_iteration(const GridGeom &g, PressureBuffers &pb, hipStream_t s);
-// The five kernels of one PCG iteration, in order.
-enum { kPcgSpmv = 0, kPcgAxpy = 1, kPcgForward = 2, kPcgBackward = 3, kPcgUpdate = 4, kPcgPieces = 5 };
-hipError_t launch_pcg_piece(int piece, const GridGeom &g, PressureBuffers &pb, hipStream_t s);
+// mixed: FP32 inner iteration with FP64 dots, FP64 x and true-residual
+// replacement (requires the FP32 arrays of pb).
+hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, bool mixed,
+                           PressureBuffers &pb, hipStream_t s);
+hipError_t launch_pcg_iteration(const GridGeom &g, bool mixed, PressureBuffers &pb, hipStream_t s);
+// The kernels of one PCG iteration, in order; kPcgResidual runs only on the
+// mixed path (it exits at once unless a true residual is due).
+enum {
+    kPcgSpmv = 0, kPcgAxpy = 1, kPcgForward = 2, kPcgBackward = 3, kPcgUpdate = 4,
+    kPcgResidual = 5, kPcgPieces = 6
+};
+bool pcg_piece_used(int piece, bool mixed);
+int pcg_piece_at(int q);  // piece executed q-th within an iteration
+hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBuffers &pb,
+                            hipStream_t s);
 // p is in slot order (pb.sm).
 hipError_t launch_pressure_apply(const GridGeom &g, const SlotMap &sm, double dt, const float *liquid,
                                  const float *wU, const float *wV, const float *wW,

@@ -191,22 +191,42 @@ __device__ __forceinline__ void drain_memory() {
 
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
-// A tagged FP64 record: two 64-bit words, each holding 32 bits of the value
-// and the 32-bit epoch of the launch that wrote it.  Each word is written
-// and read as one single-copy-atomic 64-bit access, so a reader that sees
-// the current epoch in both words has the whole value.
-__device__ __forceinline__ void put_record(unsigned long long *rec, double v, unsigned tag) {
-    unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-    unsigned long long t = (unsigned long long)tag << 32;
-    st_dev(rec, t | (bits & 0xffffffffull));
-    st_dev(rec + 1, t | (bits >> 32));
-}
-__device__ __forceinline__ bool record_ok(unsigned long long lo, unsigned long long hi, unsigned tag) {
-    return (unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag;
-}
-__device__ __forceinline__ double record_value(unsigned long long lo, unsigned long long hi) {
-    return __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
-}
+// Tagged records.  A record carries a value and the 32-bit epoch of the
+// launch that wrote it, in 64-bit words written and read as single-copy-
+// atomic accesses, so a reader that sees the current epoch in every word has
+// the whole value.  FP32: one word (epoch | bits); FP64: two words holding
+// 32 bits of the value each.
+template <class T>
+struct Record;
+template <>
+struct Record<double> {
+    static constexpr int W = 2;
+    __device__ static void put(unsigned long long *rec, double v, unsigned tag) {
+        unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+        unsigned long long t = (unsigned long long)tag << 32;
+        st_dev(rec, t | (bits & 0xffffffffull));
+        st_dev(rec + 1, t | (bits >> 32));
+    }
+    __device__ static bool ok(const unsigned long long (&w)[2], unsigned tag) {
+        return (unsigned)(w[0] >> 32) == tag && (unsigned)(w[1] >> 32) == tag;
+    }
+    __device__ static double value(const unsigned long long (&w)[2]) {
+        return __longlong_as_double((long long)((w[1] << 32) | (w[0] & 0xffffffffull)));
+    }
+};
+template <>
+struct Record<float> {
+    static constexpr int W = 1;
+    __device__ static void put(unsigned long long *rec, float v, unsigned tag) {
+        st_dev(rec, ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v));
+    }
+    __device__ static bool ok(const unsigned long long (&w)[2], unsigned tag) {
+        return (unsigned)(w[0] >> 32) == tag;
+    }
+    __device__ static float value(const unsigned long long (&w)[2]) {
+        return __uint_as_float((unsigned)(w[0] & 0xffffffffull));
+    }
+};
 
 // Waits until the upstream tiles (index < 0: none) have completed `need`
 // steps; both flags are polled in one round trip.  Returns false if the
@@ -490,6 +510,8 @@ __global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
 // DIR kBackward: out = L^-T out, in place, optional dot(out, dotvec)
 //                                           (solveLowerTransposeInPlace,
 //                                            pcgsolver.h:207-221)
+// T = double is the reference's arithmetic; T = float is the mixed-precision
+// inner solve (FP32 values, FP64 dot accumulation).
 //
 // One wave per tile marches over u (forward: step t = u; backward: t =
 // S - 1 - u) in chunks of kChunk steps.  The operands of the next chunk and
@@ -501,13 +523,25 @@ __global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
 // then handed to the edge lanes with shuffles.
 constexpr int kChunk = 8;  // 8 steps x 8 edge lanes = one record per lane
 
-template <int DIR>
-__global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb,
-                                              const double *__restrict__ in, double *out,
-                                              const double *__restrict__ dotvec, int dotmode) {
+template <class T>
+struct SweepOps {
+    const T *in;       // forward: r; backward: unused (in place on out)
+    T *out;
+    const T *dotvec;   // backward: vector of the fused dot product, or null
+    const T *inv;      // invdiag
+    const T *c1, *c2, *c3;  // forward: Lzm, Lym, Lxm; backward: Lx, Ly, Lz
+};
+
+template <class T>
+__device__ __forceinline__ T shfl_t(T v, int src) { return __shfl(v, src, 64); }
+
+template <class T, int DIR>
+__global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, SweepOps<T> op,
+                                              int dotmode) {
     constexpr int C = kChunk;
     constexpr bool FWD = DIR == kForward;
     constexpr int NV = FWD ? 5 : 6;
+    constexpr int RW = Record<T>::W;
     if (ld_dev(&pb.sc->done) != 0) return;
     const int lane = threadIdx.x;
     const SlotMap sm = pb.sm;
@@ -546,24 +580,22 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb,
             int i = t - a - b;
             return col_ok && i >= 1 && i <= g.I - 2;
         };
-        auto prefetch = [&](int u0, double (&v)[NV][C]) {
+        auto prefetch = [&](int u0, T (&v)[NV][C]) {
 #pragma unroll
             for (int s = 0; s < C; s++) {
                 const int u = u0 + s;
                 if (u >= te) {
 #pragma unroll
-                    for (int q = 0; q < NV; q++) v[q][s] = 0.0;
+                    for (int q = 0; q < NV; q++) v[q][s] = T(0);
                     continue;
                 }
                 size_t p = slot_at(u);
-                if (FWD) {
-                    v[0][s] = in[p]; v[1][s] = pb.Lzm[p]; v[2][s] = pb.Lym[p];
-                    v[3][s] = pb.Lxm[p]; v[4][s] = pb.invdiag[p];
-                } else {
-                    v[0][s] = out[p]; v[1][s] = pb.Lx[p]; v[2][s] = pb.Ly[p];
-                    v[3][s] = pb.Lz[p]; v[4][s] = pb.invdiag[p];
-                    v[5][s] = dot ? dotvec[p] : 0.0;
-                }
+                v[0][s] = FWD ? op.in[p] : op.out[p];
+                v[1][s] = op.c1[p];
+                v[2][s] = op.c2[p];
+                v[3][s] = op.c3[p];
+                v[4][s] = op.inv[p];
+                if (!FWD) v[NV - 1][s] = dot ? op.dotvec[p] : T(0);
             }
         };
         // Whether this lane's record of chunk u0 is needed: the consumer step
@@ -572,29 +604,26 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb,
             int uu = u0 + 7 + rs;
             return u0 + rs < te && uu >= ub && uu < ue;
         };
-        auto rec_ptr = [&](unsigned long long *base, int up, int u0) -> unsigned long long * {
-            return base + ((((size_t)up * S + (size_t)(u0 + 7 + rs)) * kTile + re) * kRecWords);
+        auto rec_ptr = [&](const unsigned long long *base, int up, int u0) -> const unsigned long long * {
+            return base + ((((size_t)up * S + (size_t)(u0 + 7 + rs)) * kTile + re) * RW);
+        };
+        auto fetch = [&](const unsigned long long *q, unsigned long long (&w)[2]) {
+            w[0] = ld_dev(q);
+            w[1] = RW == 2 ? ld_dev(q + 1) : 0ull;
         };
         auto load_recs = [&](int u0, unsigned long long (&rj)[2], unsigned long long (&rk)[2]) {
             rj[0] = rj[1] = rk[0] = rk[1] = 0ull;
-            if (up_j >= 0 && need_rec(u0, ubj, uej)) {
-                const unsigned long long *q = rec_ptr(pb.edge_j, up_j, u0);
-                rj[0] = ld_dev(q); rj[1] = ld_dev(q + 1);
-            }
-            if (up_k >= 0 && need_rec(u0, ubk, uek)) {
-                const unsigned long long *q = rec_ptr(pb.edge_k, up_k, u0);
-                rk[0] = ld_dev(q); rk[1] = ld_dev(q + 1);
-            }
+            if (up_j >= 0 && need_rec(u0, ubj, uej)) fetch(rec_ptr(pb.edge_j, up_j, u0), rj);
+            if (up_k >= 0 && need_rec(u0, ubk, uek)) fetch(rec_ptr(pb.edge_k, up_k, u0), rk);
         };
 
-        double cur = 0.0;
-        double nxt[NV][C];
+        T cur = T(0);
+        T nxt[NV][C];
         unsigned long long nrj[2], nrk[2];
         prefetch(tb, nxt);
         load_recs(tb, nrj, nrk);
-        bool aborted = false;
         for (int u0 = tb; u0 < te; u0 += C) {
-            double v[NV][C];
+            T v[NV][C];
 #pragma unroll
             for (int q = 0; q < NV; q++)
 #pragma unroll
@@ -609,43 +638,38 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb,
             const bool nj = up_j >= 0 && need_rec(u0, ubj, uej);
             const bool nk = up_k >= 0 && need_rec(u0, ubk, uek);
             unsigned spins = 0;
+            bool aborted = false;
             for (;;) {
-                bool bad = (nj && !record_ok(rj[0], rj[1], epoch)) ||
-                           (nk && !record_ok(rk[0], rk[1], epoch));
-                if (__ballot(bad) == 0ull) break;
+                bool bad_j = nj && !Record<T>::ok(rj, epoch);
+                bool bad_k = nk && !Record<T>::ok(rk, epoch);
+                if (__ballot(bad_j || bad_k) == 0ull) break;
                 if (ld_dev(&pb.sc->abort) != 0 || ++spins > kSpinLimit) {
                     if (lane == 0) st_dev(&pb.sc->abort, 1);
                     aborted = true;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
-                if (nj && !record_ok(rj[0], rj[1], epoch)) {
-                    const unsigned long long *q = rec_ptr(pb.edge_j, up_j, u0);
-                    rj[0] = ld_dev(q); rj[1] = ld_dev(q + 1);
-                }
-                if (nk && !record_ok(rk[0], rk[1], epoch)) {
-                    const unsigned long long *q = rec_ptr(pb.edge_k, up_k, u0);
-                    rk[0] = ld_dev(q); rk[1] = ld_dev(q + 1);
-                }
+                if (bad_j) fetch(rec_ptr(pb.edge_j, up_j, u0), rj);
+                if (bad_k) fetch(rec_ptr(pb.edge_k, up_k, u0), rk);
             }
             if (aborted) break;
-            const double vj = nj ? record_value(rj[0], rj[1]) : 0.0;
-            const double vk = nk ? record_value(rk[0], rk[1]) : 0.0;
-            double evj[C], evk[C];
+            const T vj = nj ? Record<T>::value(rj) : T(0);
+            const T vk = nk ? Record<T>::value(rk) : T(0);
+            T evj[C], evk[C];
 #pragma unroll
             for (int s = 0; s < C; s++) {
-                evj[s] = shfl_d(vj, s * kTile + b);
-                evk[s] = shfl_d(vk, s * kTile + a);
+                evj[s] = shfl_t(vj, s * kTile + b);
+                evk[s] = shfl_t(vk, s * kTile + a);
             }
 #pragma unroll
             for (int s = 0; s < C; s++) {
                 const int u = u0 + s;
                 // Values produced one step earlier by the neighbouring lanes.
-                double nj_cur = shfl_d(cur, src_j), nk_cur = shfl_d(cur, src_k);
+                T nj_cur = shfl_t(cur, src_j), nk_cur = shfl_t(cur, src_k);
                 if (u >= te) continue;  // uniform across the wave
-                double qy = edge_j ? evj[s] : nj_cur;
-                double qz = edge_k ? evk[s] : nk_cur;
-                double q = v[0][s];
+                T qy = edge_j ? evj[s] : nj_cur;
+                T qz = edge_k ? evk[s] : nk_cur;
+                T q = v[0][s];
                 if (FWD) {
                     q -= v[1][s] * qz;
                     q -= v[2][s] * qy;
@@ -656,11 +680,11 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb,
                     q -= v[2][s] * qy;
                     q -= v[3][s] * qz;
                     q *= v[4][s];
-                    dot_acc += q * v[5][s];
+                    dot_acc += (double)q * (double)v[NV - 1][s];
                 }
-                if (live_at(u)) out[slot_at(u)] = q;
-                if (pub_j) put_record(pb.edge_j + (((size_t)tile * S + u) * kTile + b) * kRecWords, q, epoch);
-                if (pub_k) put_record(pb.edge_k + (((size_t)tile * S + u) * kTile + a) * kRecWords, q, epoch);
+                if (live_at(u)) op.out[slot_at(u)] = q;
+                if (pub_j) Record<T>::put(pb.edge_j + (((size_t)tile * S + u) * kTile + b) * RW, q, epoch);
+                if (pub_k) Record<T>::put(pb.edge_k + (((size_t)tile * S + u) * kTile + a) * RW, q, epoch);
                 cur = q;
             }
         }
@@ -668,8 +692,9 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb,
     finish_sweep(pb, dot ? dotmode : kDotNone, dot_acc, tile, false);
 }
 
-static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, const double *in,
-                               double *out, const double *dotvec, int dotmode, hipStream_t s) {
+template <class T>
+static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, SweepOps<T> op,
+                               int dotmode, hipStream_t s) {
     // A fresh epoch per launch; records of older launches never match it.
     if (++pb.epoch >= 0xfffffff0u) {
         size_t n = edge_record_count(pb.sm);
@@ -679,9 +704,22 @@ static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, 
         pb.epoch = 1;
     }
     int ntiles = pb.sm.ntj * pb.sm.ntk;
-    if (dir == kForward) k_sweep<kForward><<<ntiles, 64, 0, s>>>(g, pb, in, out, dotvec, dotmode);
-    else k_sweep<kBackward><<<ntiles, 64, 0, s>>>(g, pb, in, out, dotvec, dotmode);
+    if (dir == kForward) k_sweep<T, kForward><<<ntiles, 64, 0, s>>>(g, pb, op, dotmode);
+    else k_sweep<T, kBackward><<<ntiles, 64, 0, s>>>(g, pb, op, dotmode);
     return hipGetLastError();
+}
+
+static SweepOps<double> fwd_ops64(PressureBuffers &pb, const double *r, double *z) {
+    return SweepOps<double>{r, z, nullptr, pb.invdiag, pb.Lzm, pb.Lym, pb.Lxm};
+}
+static SweepOps<double> bwd_ops64(PressureBuffers &pb, double *z, const double *dotvec) {
+    return SweepOps<double>{nullptr, z, dotvec, pb.invdiag, pb.Lx, pb.Ly, pb.Lz};
+}
+static SweepOps<float> fwd_ops32(PressureBuffers &pb) {
+    return SweepOps<float>{pb.r32, pb.z32, nullptr, pb.invdiag32, pb.Lzm32, pb.Lym32, pb.Lxm32};
+}
+static SweepOps<float> bwd_ops32(PressureBuffers &pb) {
+    return SweepOps<float>{nullptr, pb.z32, pb.r32, pb.invdiag32, pb.Lx32, pb.Ly32, pb.Lz32};
 }
 
 hipError_t launch_mic0_factor(const GridGeom &g, const float *liquid, PressureBuffers &pb,
@@ -701,9 +739,9 @@ hipError_t launch_mic0_factor(const GridGeom &g, const float *liquid, PressureBu
 
 hipError_t launch_precond_apply(const GridGeom &g, PressureBuffers &pb, const double *r,
                                 double *z, hipStream_t s) {
-    hipError_t e = launch_sweep(kForward, g, pb, r, z, nullptr, kDotNone, s);
+    hipError_t e = launch_sweep(kForward, g, pb, fwd_ops64(pb, r, z), kDotNone, s);
     if (e != hipSuccess) return e;
-    return launch_sweep(kBackward, g, pb, nullptr, z, nullptr, kDotNone, s);
+    return launch_sweep(kBackward, g, pb, bwd_ops64(pb, z, nullptr), kDotNone, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -887,38 +925,259 @@ hipError_t launch_pressure_reset(PressureBuffers &pb, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, PressureBuffers &pb,
-                           hipStream_t s) {
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 inner iteration, FP64 dots, FP64 x, and a true
+// residual r = b - A x in FP64 that replaces the FP32 recursive residual
+// whenever the latter has fallen by `delta` since the last replacement or
+// reached the tolerance ("reliable update").  Convergence is declared only
+// on the FP64 true residual, |b - A x|_inf <= tol |b|_inf.
+// ---------------------------------------------------------------------------
+
+// FP32 copies of the matrix and of the (FP64-computed) factor.
+__global__ void k_to_f32(PressureBuffers pb) {
+    size_t n = pb.sm.count();
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        pb.diag32[p] = (float)pb.diag[p];
+        pb.offx32[p] = (float)pb.offx[p];
+        pb.offy32[p] = (float)pb.offy[p];
+        pb.offz32[p] = (float)pb.offz[p];
+        pb.invdiag32[p] = (float)pb.invdiag[p];
+        pb.Lx32[p] = (float)pb.Lx[p];
+        pb.Ly32[p] = (float)pb.Ly[p];
+        pb.Lz32[p] = (float)pb.Lz[p];
+        pb.Lxm32[p] = (float)pb.Lxm[p];
+        pb.Lym32[p] = (float)pb.Lym[p];
+        pb.Lzm32[p] = (float)pb.Lzm[p];
+    }
+}
+
+__global__ void k_pcg_init_vectors32(PressureBuffers pb) {
+    size_t n = pb.sm.count();
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        pb.x[p] = 0.0;
+        pb.r32[p] = (float)pb.rhs[p];
+        pb.z32[p] = 0.0f;
+        pb.s32[p] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ float spmv_slot32(const PressureBuffers &pb, const float *__restrict__ x,
+                                             size_t p, int i, int j, int k) {
+    const SlotMap &sm = pb.sm;
+    const int a = j & 7, b = k & 7;
+    const size_t pzm = b > 0 ? p - 72 : sm.slot(i, j, k - 1);
+    const size_t pym = a > 0 ? p - 65 : sm.slot(i, j - 1, k);
+    const size_t pyp = a < 7 ? p + 65 : sm.slot(i, j + 1, k);
+    const size_t pzp = b < 7 ? p + 72 : sm.slot(i, j, k + 1);
+    float s = 0.0f;
+    s += pb.offz32[pzm] * x[pzm];
+    s += pb.offy32[pym] * x[pym];
+    s += pb.offx32[p - 64] * x[p - 64];
+    s += pb.diag32[p] * x[p];
+    s += pb.offx32[p] * x[p + 64];
+    s += pb.offy32[p] * x[pyp];
+    s += pb.offz32[p] * x[pzp];
+    return s;
+}
+
+// z32 = A32 s32 and alpha = rho / (s . z) with an FP64 dot.
+__global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, PressureBuffers pb) {
+    __shared__ double sh[kRedThreads / 64];
+    if (pb.sc->done) return;
+    size_t n = pb.sm.count();
+    double acc = 0.0;
+    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * kRedThreads) {
+        int i, j, k;
+        decode_slot(pb.sm, p, i, j, k);
+        float zc = 0.0f;
+        if (interior(g, i, j, k)) zc = spmv_slot32(pb, pb.s32, p, i, j, k);
+        pb.z32[p] = zc;
+        acc += (double)pb.s32[p] * (double)zc;
+    }
+    double bs = block_reduce<0>(acc, sh);
+    if (threadIdx.x == 0) st_dev(&pb.partials[blockIdx.x], bs);
+    if (!last_block(&pb.sc->counter[1])) return;
+    double tot = 0.0;
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) tot += ld_dev(&pb.partials[q]);
+    tot = block_reduce<0>(tot, sh);
+    if (threadIdx.x == 0) {
+        PcgScalars *sc = pb.sc;
+        sc->sz = tot;
+        double alpha = sc->rho / tot;
+        sc->alpha = alpha;
+        sc->alpha_f = (float)alpha;
+        sc->counter[1] = 0;
+    }
+}
+
+// x += alpha s32 (FP64), r32 -= alpha z32 (FP32), |r32|_inf; schedules a
+// true residual when it is due.
+__global__ __launch_bounds__(kRedThreads) void k_axpy_norm32(GridGeom g, PressureBuffers pb) {
+    __shared__ double sh[kRedThreads / 64];
+    if (pb.sc->done) return;
+    size_t n = pb.sm.count();
+    double alpha = pb.sc->alpha;
+    float naf = -pb.sc->alpha_f;
+    double mx = 0.0;
+    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * kRedThreads) {
+        float sp = pb.s32[p];
+        pb.x[p] += alpha * (double)sp;
+        float r = pb.r32[p] + naf * pb.z32[p];
+        pb.r32[p] = r;
+        double a = fabs((double)r);
+        mx = a > mx ? a : mx;
+    }
+    double bm = block_reduce<1>(mx, sh);
+    if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
+    if (!last_block(&pb.sc->counter[2])) return;
+    double m = 0.0;
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) {
+        double v = ld_dev(&pb.partials[kRedBlocks + q]);
+        m = v > m ? v : m;
+    }
+    m = block_reduce<1>(m, sh);
+    if (threadIdx.x == 0) {
+        PcgScalars *sc = pb.sc;
+        sc->iters += 1;
+        if (m <= sc->tol || m < sc->delta * sc->rmax || sc->iters >= sc->maxiter) sc->refresh = 1;
+        else sc->resid = m;
+        sc->counter[2] = 0;
+    }
+}
+
+// r = b - A x in FP64 (A in FP64), r32 = (float) r, |r|_inf; decides
+// convergence.  Exits at once unless a true residual is due.
+__global__ __launch_bounds__(kRedThreads) void k_true_residual(GridGeom g, PressureBuffers pb) {
+    __shared__ double sh[kRedThreads / 64];
+    if (pb.sc->done || !pb.sc->refresh) return;
+    size_t n = pb.sm.count();
+    double mx = 0.0;
+    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * kRedThreads) {
+        int i, j, k;
+        decode_slot(pb.sm, p, i, j, k);
+        double r = 0.0;
+        if (interior(g, i, j, k)) r = pb.rhs[p] - spmv_slot(pb, pb.x, p, i, j, k);
+        pb.r32[p] = (float)r;
+        double a = fabs(r);
+        mx = a > mx ? a : mx;
+    }
+    double bm = block_reduce<1>(mx, sh);
+    if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
+    if (!last_block(&pb.sc->counter[3])) return;
+    double m = 0.0;
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) {
+        double v = ld_dev(&pb.partials[kRedBlocks + q]);
+        m = v > m ? v : m;
+    }
+    m = block_reduce<1>(m, sh);
+    if (threadIdx.x == 0) {
+        PcgScalars *sc = pb.sc;
+        sc->resid = m;
+        sc->refinements += 1;
+        sc->refresh = 0;
+        sc->rmax = m;
+        if (m <= sc->tol) {
+            sc->converged = 1;
+            sc->done = 1;
+        } else if (sc->iters >= sc->maxiter) {
+            sc->done = 1;
+        }
+        sc->counter[3] = 0;
+    }
+}
+
+__global__ void k_update_s32(PressureBuffers pb, int first) {
+    if (pb.sc->done) return;
+    size_t n = pb.sm.count();
+    float bf = first ? 0.0f : pb.sc->beta_f;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        if (first) pb.s32[p] = pb.z32[p];
+        else pb.s32[p] = pb.z32[p] + bf * pb.s32[p];
+    }
+}
+
+__global__ void k_mixed_init_scalars(PressureBuffers pb) {
+    PcgScalars *sc = pb.sc;
+    sc->rmax = sc->bnorm;
+    sc->delta = 1e-2;
+    sc->refresh = 0;
+    sc->refinements = 0;
+}
+
+hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, bool mixed,
+                           PressureBuffers &pb, hipStream_t s) {
+    if (mixed) {
+        k_to_f32<<<kRedBlocks, kRedThreads, 0, s>>>(pb);
+        k_pcg_init_vectors32<<<kRedBlocks, kRedThreads, 0, s>>>(pb);
+        k_pcg_init_scalars<<<1, 1, 0, s>>>(pb, tolf, maxiter);
+        k_mixed_init_scalars<<<1, 1, 0, s>>>(pb);
+        hipError_t e = launch_sweep(kForward, g, pb, fwd_ops32(pb), kDotNone, s);
+        if (e != hipSuccess) return e;
+        e = launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotInit, s);
+        if (e != hipSuccess) return e;
+        k_update_s32<<<kRedBlocks, kRedThreads, 0, s>>>(pb, 1);
+        return hipGetLastError();
+    }
     k_pcg_init_vectors<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb);
     k_pcg_init_scalars<<<1, 1, 0, s>>>(pb, tolf, maxiter);
     // z = M^-1 r, rho = z . r, s = z (pcgsolver.h:270-279).
-    hipError_t e = launch_sweep(kForward, g, pb, pb.r, pb.z, nullptr, kDotNone, s);
+    hipError_t e = launch_sweep(kForward, g, pb, fwd_ops64(pb, pb.r, pb.z), kDotNone, s);
     if (e != hipSuccess) return e;
-    e = launch_sweep(kBackward, g, pb, nullptr, pb.z, pb.r, kDotInit, s);
+    e = launch_sweep(kBackward, g, pb, bwd_ops64(pb, pb.z, pb.r), kDotInit, s);
     if (e != hipSuccess) return e;
     k_update_s<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb, 1);
     return hipGetLastError();
 }
 
-hipError_t launch_pcg_piece(int piece, const GridGeom &g, PressureBuffers &pb, hipStream_t s) {
+bool pcg_piece_used(int piece, bool mixed) { return piece != kPcgResidual || mixed; }
+
+hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBuffers &pb,
+                            hipStream_t s) {
+    if (mixed) {
+        switch (piece) {
+            case kPcgSpmv: k_spmv_dot32<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
+            case kPcgAxpy: k_axpy_norm32<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
+            case kPcgResidual: k_true_residual<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
+            case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kDotNone, s);
+            case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotIter, s);
+            case kPcgUpdate: k_update_s32<<<kRedBlocks, kRedThreads, 0, s>>>(pb, 0); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (piece) {
         case kPcgSpmv: k_spmv_dot<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
         case kPcgAxpy: k_axpy_norm<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
-        case kPcgForward: return launch_sweep(kForward, g, pb, pb.r, pb.z, nullptr, kDotNone, s);
-        case kPcgBackward: return launch_sweep(kBackward, g, pb, nullptr, pb.z, pb.r, kDotIter, s);
+        case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops64(pb, pb.r, pb.z), kDotNone, s);
+        case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops64(pb, pb.z, pb.r), kDotIter, s);
         case kPcgUpdate: k_update_s<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb, 0); break;
+        case kPcgResidual: return hipSuccess;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_pcg_iteration(const GridGeom &g, PressureBuffers &pb, hipStream_t s) {
-    for (int piece = 0; piece < kPcgPieces; piece++) {
-        hipError_t e = launch_pcg_piece(piece, g, pb, s);
+// Execution order of the pieces within one iteration.
+static const int kPieceOrder[kPcgPieces] = {kPcgSpmv, kPcgAxpy, kPcgResidual,
+                                            kPcgForward, kPcgBackward, kPcgUpdate};
+
+hipError_t launch_pcg_iteration(const GridGeom &g, bool mixed, PressureBuffers &pb, hipStream_t s) {
+    for (int q = 0; q < kPcgPieces; q++) {
+        int piece = kPieceOrder[q];
+        if (!pcg_piece_used(piece, mixed)) continue;
+        hipError_t e = launch_pcg_piece(piece, g, mixed, pb, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
+
+int pcg_piece_at(int q) { return kPieceOrder[q]; }
 
 __global__ void k_spmv_plain(GridGeom g, PressureBuffers pb, const double *__restrict__ x,
                              double *y) {

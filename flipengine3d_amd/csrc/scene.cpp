@@ -108,7 +108,7 @@ void collision_box(int isize, int jsize, int ksize, double dx, float pos[3], dou
 // Pressure solve driver
 // ---------------------------------------------------------------------------
 
-void PressureStore::alloc(const GridGeom &g, hipStream_t s) {
+void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     SlotMap sm = make_slot_map(g.I, g.J, g.K);
     size_t ns = sm.count();
     int ntj = sm.ntj, ntk = sm.ntk;
@@ -135,6 +135,17 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s) {
     pb.invdiag = arr[9].p; pb.Lx = arr[10].p; pb.Ly = arr[11].p; pb.Lz = arr[12].p;
     pb.Lxm = arr[13].p; pb.Lym = arr[14].p; pb.Lzm = arr[15].p;
     pb.partials = arr[kSlotArrays].p;
+    if (mixed) {
+        for (int q = 0; q < kSlotArrays32; q++) {
+            if (arr32[q].alloc(ns) != hipSuccess)
+                throw StatusError{FLIPMI_ERR_OOM, "mixed-precision arrays: out of device memory"};
+            (void)launch_fill_f32(arr32[q].p, ns, 0.0f, s);
+        }
+        float **dst[kSlotArrays32] = {&pb.diag32, &pb.offx32, &pb.offy32, &pb.offz32, &pb.invdiag32,
+                                      &pb.Lx32, &pb.Ly32, &pb.Lz32, &pb.Lxm32, &pb.Lym32, &pb.Lzm32,
+                                      &pb.r32, &pb.z32, &pb.s32};
+        for (int q = 0; q < kSlotArrays32; q++) *dst[q] = arr32[q].p;
+    }
     pb.npartials = npart;
     pb.edge_j = edge_j.p;
     pb.edge_k = edge_k.p;
@@ -176,8 +187,10 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
         return out;
     }
     if (prof) prof->begin_stage(FLIPMI_STAGE_MIC0, &ev);
+    const bool mixed = w.precision == FLIPMI_PCG_MIXED;
+    if (mixed && !pb.r32) throw StatusError{FLIPMI_ERR_INVALID_ARG, "mixed precision arrays not allocated"};
     hip_check(launch_mic0_factor(w.g, w.liquid, pb, s), "mic0 factor");
-    hip_check(launch_pcg_init(w.g, w.tol, w.maxiter, pb, s), "pcg init");
+    hip_check(launch_pcg_init(w.g, w.tol, w.maxiter, mixed, pb, s), "pcg init");
     if (prof) prof->end_stage(FLIPMI_STAGE_MIC0, ev);
     if (prof) prof->begin_stage(FLIPMI_STAGE_PCG, &ev);
     int batch = 8;
@@ -186,14 +199,17 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
         for (int q = 0; q < batch; q++) {
             if (prof && prof->profiling >= 2) {
                 // Time each kernel of the iteration on the launch stream.
-                for (int piece = 0; piece < kPcgPieces; piece++) {
+                for (int q = 0; q < kPcgPieces; q++) {
+                    int piece = pcg_piece_at(q);
+                    if (!pcg_piece_used(piece, mixed)) continue;
+                    int stage = piece == kPcgResidual ? FLIPMI_STAGE_K_RESIDUAL : FLIPMI_STAGE_K_SPMV + piece;
                     hipEvent_t e0 = nullptr;
-                    prof->begin_stage(FLIPMI_STAGE_K_SPMV + piece, &e0);
-                    hip_check(launch_pcg_piece(piece, w.g, pb, s), "pcg kernel");
-                    prof->end_stage(FLIPMI_STAGE_K_SPMV + piece, e0);
+                    prof->begin_stage(stage, &e0);
+                    hip_check(launch_pcg_piece(piece, w.g, mixed, pb, s), "pcg kernel");
+                    prof->end_stage(stage, e0);
                 }
             } else {
-                hip_check(launch_pcg_iteration(w.g, pb, s), "pcg iteration");
+                hip_check(launch_pcg_iteration(w.g, mixed, pb, s), "pcg iteration");
             }
         }
         launched += batch;
@@ -209,6 +225,7 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
     if (prof) prof->stage_calls[FLIPMI_STAGE_PCG_ITERS] += hs->iters;
     out.iterations = hs->failed ? 0 : hs->iters;
     out.residual = hs->resid;
+    out.refinements = mixed ? hs->refinements : 0;
     bool converged = hs->converged != 0;
     // PressureSolver::_solveLinearSystem (pressuresolver.cpp:810-840).
     out.success = converged || (!hs->failed && out.iterations == w.maxiter && out.residual < w.acceptable);
@@ -317,7 +334,7 @@ void Scene::alloc_all() {
     need(vu.alloc(nU)); need(vv.alloc(nV)); need(vw.alloc(nW));
     need(status.alloc(nf));
     press = new PressureStore();
-    press->alloc(g, stream);
+    press->alloc(g, stream, desc.pcg_precision == FLIPMI_PCG_MIXED);
     check(hipHostMalloc(&host_stage, 4096, hipHostMallocDefault), "pinned staging");
     // Fresh MeshLevelSet: phi = distance upper bound everywhere
     // (meshlevelset.cpp:35-42, 760-762); zero solid face velocities.

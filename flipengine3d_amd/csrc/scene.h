@@ -157,13 +157,15 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
 // Lxm, Lym, Lzm), arr[16] the reduction partials.
 struct PressureStore {
     static constexpr int kSlotArrays = 16;
+    static constexpr int kSlotArrays32 = 14;  // FP32 matrix, factor and vectors (mixed precision)
     DBuf<double> arr[kSlotArrays + 1];
+    DBuf<float> arr32[kSlotArrays32];
     DBuf<unsigned long long> edge_j, edge_k;
     DBuf<unsigned> progress, ticket;
     DBuf<int> irange;
     DBuf<PcgScalars> scal;
     PressureBuffers pb{};
-    void alloc(const GridGeom &g, hipStream_t s);
+    void alloc(const GridGeom &g, hipStream_t s, bool mixed);
 };
 
 }  // namespace flipmi

@@ -104,7 +104,8 @@ enum {
     FLIPMI_STAGE_K_AXPY = 13,
     FLIPMI_STAGE_K_FORWARD = 14,
     FLIPMI_STAGE_K_BACKWARD = 15,
-    FLIPMI_STAGE_K_UPDATE = 16
+    FLIPMI_STAGE_K_UPDATE = 16,
+    FLIPMI_STAGE_K_RESIDUAL = 17    /* mixed precision: FP64 true residual */
 };
 
 /* Grids readable with flipmi_scene_get_grid (sizes as in the header). */
