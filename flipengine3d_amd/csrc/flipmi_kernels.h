@@ -93,27 +93,31 @@ struct PcgScalars {
 };
 
 // Sweep-ordered ("slot") storage of the pressure system.  Cells are grouped
-// into 8 x 8 column tiles over (j, k).  Tile (jt, kt) stores, for every step
-// t = i + a + b (a = j % 8, b = k % 8, 0 <= t < S = I + 14), one row of 64
-// slots with lane a + 8 b.  A wave that owns a tile and marches over t finds
-// the 64 cells of a step in one contiguous row, and each cell's -x, -y and -z
-// neighbours of the same tile one row earlier in lanes l, l - 1 and l - 8.
-// Slots that are not grid cells (the skew corners, padding of J and K up to
-// a multiple of 8) hold zeros in every array.
+// into 8 x 8 column tiles over (j, k); a cell of tile (jt, kt) has lane
+// a + 8 b (a = j % 8, b = k % 8) and wavefront step t = i + a + b,
+// 0 <= t < S (S >= I + 14, a multiple of 8).  Steps are stored in groups of
+// four: the slot of (tile, t, lane) is
+//     ((tile * S/4 + t/4) * 64 + lane) * 4 + t % 4,
+// so a wave that owns a tile loads four steps of all its 64 cells with one
+// contiguous 16-byte-per-lane access (two for FP64), and each cell's -x, -y
+// and -z neighbours of the same tile sit one step earlier in lanes l, l - 1
+// and l - 8.  Slots that are not grid cells (skew corners, step padding,
+// padding of J and K up to a multiple of 8) hold zeros in every array.
 struct SlotMap {
     int I, J, K, ntj, ntk, S;
     __host__ __device__ size_t count() const { return (size_t)ntj * ntk * S * 64; }
     __host__ __device__ size_t slot(int i, int j, int k) const {
-        int a = j & 7, b = k & 7;
+        int a = j & 7, b = k & 7, t = i + a + b;
         size_t tile = (size_t)(j >> 3) + (size_t)ntj * (size_t)(k >> 3);
-        return ((tile * (size_t)S + (size_t)(i + a + b)) << 6) + (size_t)(a + 8 * b);
+        size_t grp = tile * (size_t)(S >> 2) + (size_t)(t >> 2);
+        return ((grp * 64 + (size_t)(a + 8 * b)) << 2) + (size_t)(t & 3);
     }
 };
 inline SlotMap make_slot_map(int I, int J, int K) {
     SlotMap m;
     m.I = I; m.J = J; m.K = K;
     m.ntj = (J + 7) / 8; m.ntk = (K + 7) / 8;
-    m.S = I + 14;
+    m.S = (I + 14 + 7) & ~7;
     return m;
 }
 

@@ -38,10 +38,11 @@ size_t edge_record_count(const SlotMap &sm) {
 
 // Cell of a slot (may lie outside the grid: skew corners and padding).
 __device__ __forceinline__ void decode_slot(const SlotMap &sm, size_t p, int &i, int &j, int &k) {
-    int lane = (int)(p & 63);
-    size_t row = p >> 6;
-    int t = (int)(row % (size_t)sm.S);
-    size_t tile = row / (size_t)sm.S;
+    int lane = (int)((p >> 2) & 63);
+    size_t grp = p >> 8;
+    const size_t S4 = (size_t)(sm.S >> 2);
+    int t = (int)(grp % S4) * 4 + (int)(p & 3);
+    size_t tile = grp / S4;
     int a = lane & 7, b = lane >> 3;
     j = (int)(tile % (size_t)sm.ntj) * kTile + a;
     k = (int)(tile / (size_t)sm.ntj) * kTile + b;
@@ -208,7 +209,7 @@ struct Record<double> {
         st_dev(rec + 1, t | (bits >> 32));
     }
     __device__ static bool ok(const unsigned long long (&w)[2], unsigned tag) {
-        return (unsigned)(w[0] >> 32) == tag && (unsigned)(w[1] >> 32) == tag;
+        return (int)((unsigned)(w[0] >> 32) == tag) & (int)((unsigned)(w[1] >> 32) == tag);
     }
     __device__ static double value(const unsigned long long (&w)[2]) {
         return __longlong_as_double((long long)((w[1] << 32) | (w[0] & 0xffffffffull)));
@@ -259,7 +260,9 @@ __device__ __forceinline__ int take_tile(PressureBuffers &pb, bool reverse, int 
     const int ntj = pb.sm.ntj, ntk = pb.sm.ntk;
     int ticket = 0;
     if (threadIdx.x == 0) ticket = (int)atomicAdd(pb.ticket, 1u);
-    ticket = __shfl(ticket, 0, 64);
+    // Wave-uniform from here on (kept in scalar registers, so every loop and
+    // branch that depends on the tile is uniform).
+    ticket = __builtin_amdgcn_readfirstlane(__shfl(ticket, 0, 64));
     if (reverse) {
         jt = ntj - 1 - ticket % ntj;
         kt = ntk - 1 - ticket / ntj;
@@ -274,7 +277,8 @@ __device__ __forceinline__ int take_tile(PressureBuffers &pb, bool reverse, int 
 // backward: u = S - 1 - t): the steps that hold at least one of its rows.
 __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const GridGeom &g, int tile,
                                             bool reverse, int &ub, int &ue) {
-    int imin = pb.tile_irange[2 * tile], imax = pb.tile_irange[2 * tile + 1];
+    int imin = __builtin_amdgcn_readfirstlane(pb.tile_irange[2 * tile]);
+    int imax = __builtin_amdgcn_readfirstlane(pb.tile_irange[2 * tile + 1]);
     if (imax < 0) {
         ub = ue = 0;
         return;
@@ -287,6 +291,10 @@ __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const Gri
         ue = imax + 15;
     }
     if (ue > pb.sm.S) ue = pb.sm.S;
+    // Whole chunks of 8 steps (S is a multiple of 8): the extra steps hold no
+    // rows and compute zeros.
+    ub &= ~7;
+    ue = (ue + 7) & ~7;
 }
 
 // The last wave of a sweep launch resets the shared counters for the next
@@ -382,14 +390,16 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
     const bool edge_j = a == 0, edge_k = b == 0;
     const bool pub = a == 7 || b == 7;
     const unsigned total = (unsigned)sm.S;
-    const size_t row0 = (size_t)tile * sm.S;
+    const size_t grp0 = (size_t)tile * (sm.S >> 2);
     int tb, te;
     tile_urange(pb, g, tile, false, tb, te);
 
     bool ok = true;
     if (te > tb) {
         if (lane == 0) st_dev(&pb.progress[tile], (unsigned)tb);
-        auto slot_at = [&](int t) -> size_t { return ((row0 + (size_t)t) << 6) + (size_t)lane; };
+        auto slot_at = [&](int t) -> size_t {
+            return (((grp0 + (size_t)(t >> 2)) * 64 + (size_t)lane) << 2) + (size_t)(t & 3);
+        };
         auto live_at = [&](int t) -> bool {
             int i = t - a - b;
             return col_ok && i >= 1 && i <= g.I - 2;
@@ -493,7 +503,7 @@ __global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
         decode_slot(pb.sm, p, i, j, k);
         double lx = 0.0, ly = 0.0, lz = 0.0;
         if (interior(g, i, j, k)) {
-            lx = pb.Lx[p - 64];
+            lx = pb.Lx[pb.sm.slot(i - 1, j, k)];
             ly = pb.Ly[pb.sm.slot(i, j - 1, k)];
             lz = pb.Lz[pb.sm.slot(i, j, k - 1)];
         }
@@ -514,13 +524,14 @@ __global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
 // inner solve (FP32 values, FP64 dot accumulation).
 //
 // One wave per tile marches over u (forward: step t = u; backward: t =
-// S - 1 - u) in chunks of kChunk steps.  The operands of the next chunk and
-// the tagged records the next chunk needs from the upstream tiles are loaded
-// while the current chunk computes.  A cell at march step u needs its
-// upstream neighbour's value from that tile's march step u + 7, published
-// there as record (tile, u, lane-within-edge).  Lane L of the consumer loads
-// the record for step u0 + 7 + L / 8, edge lane L % 8, and the values are
-// then handed to the edge lanes with shuffles.
+// S - 1 - u) in chunks of kChunk = 8 steps (two groups of four steps of the
+// slot layout).  The operands of D chunks are in flight at once in a ring of
+// register buffers, so the march is not held up by memory latency.  A cell
+// at march step u needs its upstream neighbour's value from that tile's march
+// step u + 7, published there as record (tile, u, lane-within-edge).  Lane L
+// of the consumer loads the record for step u0 + 7 + L / 8, edge lane L % 8,
+// one chunk ahead; each step proceeds as soon as its own records carry the
+// launch epoch, and the values reach the edge lanes by shuffles.
 constexpr int kChunk = 8;  // 8 steps x 8 edge lanes = one record per lane
 
 template <class T>
@@ -535,7 +546,25 @@ struct SweepOps {
 template <class T>
 __device__ __forceinline__ T shfl_t(T v, int src) { return __shfl(v, src, 64); }
 
-template <class T, int DIR>
+// The four steps of one slot group of this lane (16 bytes per access).
+__device__ __forceinline__ void load_group(const float *A, size_t base, float (&e)[4]) {
+    float4 v = *reinterpret_cast<const float4 *>(A + base);
+    e[0] = v.x; e[1] = v.y; e[2] = v.z; e[3] = v.w;
+}
+__device__ __forceinline__ void load_group(const double *A, size_t base, double (&e)[4]) {
+    double2 v0 = *reinterpret_cast<const double2 *>(A + base);
+    double2 v1 = *reinterpret_cast<const double2 *>(A + base + 2);
+    e[0] = v0.x; e[1] = v0.y; e[2] = v1.x; e[3] = v1.y;
+}
+__device__ __forceinline__ void store_group(float *A, size_t base, const float (&e)[4]) {
+    *reinterpret_cast<float4 *>(A + base) = make_float4(e[0], e[1], e[2], e[3]);
+}
+__device__ __forceinline__ void store_group(double *A, size_t base, const double (&e)[4]) {
+    *reinterpret_cast<double2 *>(A + base) = make_double2(e[0], e[1]);
+    *reinterpret_cast<double2 *>(A + base + 2) = make_double2(e[2], e[3]);
+}
+
+template <class T, int DIR, int D>
 __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, SweepOps<T> op,
                                               int dotmode) {
     constexpr int C = kChunk;
@@ -549,8 +578,6 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
     int jt, kt;
     const int tile = take_tile(pb, !FWD, jt, kt);
     const int a = lane & 7, b = lane >> 3;
-    const int j = jt * kTile + a, k = kt * kTile + b;
-    const bool col_ok = j >= 1 && k >= 1 && j <= g.J - 2 && k <= g.K - 2;
     const int up_j = FWD ? (jt > 0 ? tile - 1 : -1) : (jt + 1 < ntj ? tile + 1 : -1);
     const int up_k = FWD ? (kt > 0 ? tile - ntj : -1) : (kt + 1 < ntk ? tile + ntj : -1);
     const int src_j = (FWD ? lane - 1 : lane + 1) & 63;  // neighbour lane along j
@@ -560,137 +587,191 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
     const bool pub_j = FWD ? a == 7 : a == 0;             // writes for the downstream j tile
     const bool pub_k = FWD ? b == 7 : b == 0;
     const unsigned epoch = pb.epoch;
-    const size_t row0 = (size_t)tile * S;
+    const size_t grp0 = (size_t)tile * (size_t)(S >> 2);
     const bool dot = !FWD && dotmode != kDotNone;
+    // Without a dot product the backward sweep still streams a vector (its
+    // own output) so the hot loop has no data-dependent branches.
+    const T *dotsrc = dot ? op.dotvec : op.out;
 
     int tb, te, ubj = 0, uej = 0, ubk = 0, uek = 0;
     tile_urange(pb, g, tile, !FWD, tb, te);
     if (up_j >= 0) tile_urange(pb, g, up_j, !FWD, ubj, uej);
     if (up_k >= 0) tile_urange(pb, g, up_k, !FWD, ubk, uek);
     const int rs = lane >> 3, re = lane & 7;  // record this lane loads: step offset, edge lane
+    // Tiles without an upstream neighbour read their own (never needed) records.
+    const int rj_tile = up_j >= 0 ? up_j : tile, rk_tile = up_k >= 0 ? up_k : tile;
 
     double dot_acc = 0.0;
     if (te > tb) {
-        auto slot_at = [&](int u) -> size_t {
-            int t = FWD ? u : S - 1 - u;
-            return ((row0 + (size_t)t) << 6) + (size_t)lane;
-        };
-        auto live_at = [&](int u) -> bool {
-            int t = FWD ? u : S - 1 - u;
-            int i = t - a - b;
-            return col_ok && i >= 1 && i <= g.I - 2;
-        };
-        auto prefetch = [&](int u0, T (&v)[NV][C]) {
+        auto gbase = [&](int gi) -> size_t { return ((grp0 + (size_t)gi) * 64 + (size_t)lane) << 2; };
+        // Lower slot group of the chunk at march step u0 (u0 % 8 == 0).
+        auto chunk_group = [&](int u0) -> int { return FWD ? (u0 >> 2) : ((S - 8 - u0) >> 2); };
+        // Step s of a chunk is element m of group h: forward s = 4h + m,
+        // backward s = 7 - (4h + m).
+        auto load_chunk = [&](int u0, T (&v)[NV][C]) {
+            const int gl = chunk_group(u0);
 #pragma unroll
-            for (int s = 0; s < C; s++) {
-                const int u = u0 + s;
-                if (u >= te) {
+            for (int h = 0; h < 2; h++) {
+                const size_t base = gbase(gl + h);
+                T e[NV][4];
+                load_group(FWD ? op.in : op.out, base, e[0]);
+                load_group(op.c1, base, e[1]);
+                load_group(op.c2, base, e[2]);
+                load_group(op.c3, base, e[3]);
+                load_group(op.inv, base, e[4]);
+                if (!FWD) load_group(dotsrc, base, e[NV - 1]);
 #pragma unroll
-                    for (int q = 0; q < NV; q++) v[q][s] = T(0);
-                    continue;
+                for (int m = 0; m < 4; m++) {
+                    const int s = FWD ? 4 * h + m : 7 - (4 * h + m);
+#pragma unroll
+                    for (int q = 0; q < NV; q++) v[q][s] = e[q][m];
                 }
-                size_t p = slot_at(u);
-                v[0][s] = FWD ? op.in[p] : op.out[p];
-                v[1][s] = op.c1[p];
-                v[2][s] = op.c2[p];
-                v[3][s] = op.c3[p];
-                v[4][s] = op.inv[p];
-                if (!FWD) v[NV - 1][s] = dot ? op.dotvec[p] : T(0);
             }
         };
-        // Whether this lane's record of chunk u0 is needed: the consumer step
-        // exists and the producer tile processes the producer step.
-        auto need_rec = [&](int u0, int ub, int ue) -> bool {
-            int uu = u0 + 7 + rs;
-            return u0 + rs < te && uu >= ub && uu < ue;
+        auto store_chunk = [&](int u0, const T (&qv)[C]) {
+            const int gl = chunk_group(u0);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                T e[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) e[m] = qv[FWD ? 4 * h + m : 7 - (4 * h + m)];
+                store_group(op.out, gbase(gl + h), e);
+            }
         };
-        auto rec_ptr = [&](const unsigned long long *base, int up, int u0) -> const unsigned long long * {
-            return base + ((((size_t)up * S + (size_t)(u0 + 7 + rs)) * kTile + re) * RW);
+        // Whether this lane's record of chunk u0 is needed: the producer tile
+        // processes the producer step.
+        auto need_rec = [&](int up, int u0, int ub, int ue) -> bool {
+            int uu = u0 + 7 + rs;
+            return (int)(up >= 0) & (int)(uu >= ub) & (int)(uu < ue);
+        };
+        auto rec_ptr = [&](const unsigned long long *base, int rt, int u0) -> const unsigned long long * {
+            int uu = u0 + 7 + rs;
+            uu = uu < S ? uu : S - 1;
+            return base + ((((size_t)rt * S + (size_t)uu) * kTile + re) * RW);
         };
         auto fetch = [&](const unsigned long long *q, unsigned long long (&w)[2]) {
             w[0] = ld_dev(q);
             w[1] = RW == 2 ? ld_dev(q + 1) : 0ull;
         };
         auto load_recs = [&](int u0, unsigned long long (&rj)[2], unsigned long long (&rk)[2]) {
-            rj[0] = rj[1] = rk[0] = rk[1] = 0ull;
-            if (up_j >= 0 && need_rec(u0, ubj, uej)) fetch(rec_ptr(pb.edge_j, up_j, u0), rj);
-            if (up_k >= 0 && need_rec(u0, ubk, uek)) fetch(rec_ptr(pb.edge_k, up_k, u0), rk);
+            fetch(rec_ptr(pb.edge_j, rj_tile, u0), rj);
+            fetch(rec_ptr(pb.edge_k, rk_tile, u0), rk);
         };
 
+        // Issue order in the prologue matches the loop's steady state (records
+        // of a chunk before the operands issued after them), so the compiler's
+        // outstanding-load accounting at the loop head stays tight.
+        unsigned long long recj[D][2], reck[D][2];
+        load_recs(tb, recj[0], reck[0]);
+        T buf[D][NV][C];
+        const int last = te - C;  // last chunk; loads past the end repeat it
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            int uf = tb + d * C;
+            load_chunk(uf < last ? uf : last, buf[d]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         T cur = T(0);
-        T nxt[NV][C];
-        unsigned long long nrj[2], nrk[2];
-        prefetch(tb, nxt);
-        load_recs(tb, nrj, nrk);
-        for (int u0 = tb; u0 < te; u0 += C) {
-            T v[NV][C];
+        bool aborted = false;
+        for (int u0 = tb; u0 < te; u0 += D * C) {
 #pragma unroll
-            for (int q = 0; q < NV; q++)
-#pragma unroll
-                for (int s = 0; s < C; s++) v[q][s] = nxt[q][s];
-            unsigned long long rj[2] = {nrj[0], nrj[1]}, rk[2] = {nrk[0], nrk[1]};
-            if (u0 + C < te) {
-                prefetch(u0 + C, nxt);
-                load_recs(u0 + C, nrj, nrk);
-            }
-            // Records of this chunk: poll until every needed one carries this
-            // launch's epoch.
-            const bool nj = up_j >= 0 && need_rec(u0, ubj, uej);
-            const bool nk = up_k >= 0 && need_rec(u0, ubk, uek);
-            unsigned spins = 0;
-            bool aborted = false;
-            for (;;) {
-                bool bad_j = nj && !Record<T>::ok(rj, epoch);
-                bool bad_k = nk && !Record<T>::ok(rk, epoch);
-                if (__ballot(bad_j || bad_k) == 0ull) break;
-                if (ld_dev(&pb.sc->abort) != 0 || ++spins > kSpinLimit) {
-                    if (lane == 0) st_dev(&pb.sc->abort, 1);
-                    aborted = true;
-                    break;
+            for (int d = 0; d < D; d++) {
+                const int uc = u0 + d * C;
+                if (uc >= te) break;  // uniform across the wave
+                unsigned long long(&rj)[2] = recj[d];
+                unsigned long long(&rk)[2] = reck[d];
+                const int un = uc + C < te ? uc + C : uc;
+                load_recs(un, recj[(d + 1) % D], reck[(d + 1) % D]);
+                // Keep the prefetches where they are issued: the scheduler would
+                // otherwise sink them towards their use to save registers.
+                __builtin_amdgcn_sched_barrier(0);
+                const bool nj = need_rec(up_j, uc, ubj, uej);
+                const bool nk = need_rec(up_k, uc, ubk, uek);
+                const int bad = ((int)nj & (int)!Record<T>::ok(rj, epoch)) |
+                                ((int)nk & (int)!Record<T>::ok(rk, epoch));
+                // Slow path (the upstream tile is behind): poll this chunk's
+                // records into separate registers until they carry this
+                // launch's epoch, then drain.  The hot path's registers are
+                // not written here, so the compiler's outstanding-load
+                // accounting for them stays exact.
+                unsigned long long tj[2] = {0ull, 0ull}, tk[2] = {0ull, 0ull};
+                const bool slow = __ballot(bad) != 0ull;
+                if (slow) {
+                    unsigned spins = 0;
+                    for (;;) {
+                        fetch(rec_ptr(pb.edge_j, rj_tile, uc), tj);
+                        fetch(rec_ptr(pb.edge_k, rk_tile, uc), tk);
+                        drain_memory();
+                        const int still = ((int)nj & (int)!Record<T>::ok(tj, epoch)) |
+                                          ((int)nk & (int)!Record<T>::ok(tk, epoch));
+                        if (__ballot(still) == 0ull) break;
+                        if (ld_dev(&pb.sc->abort) != 0 || ++spins > kSpinLimit) {
+                            if (lane == 0) st_dev(&pb.sc->abort, 1);
+                            aborted = true;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    drain_memory();
+                    if (aborted) break;
                 }
-                __builtin_amdgcn_s_sleep(1);
-                if (bad_j) fetch(rec_ptr(pb.edge_j, up_j, u0), rj);
-                if (bad_k) fetch(rec_ptr(pb.edge_k, up_k, u0), rk);
+                const T vj = nj ? (slow ? Record<T>::value(tj) : Record<T>::value(rj)) : T(0);
+                const T vk = nk ? (slow ? Record<T>::value(tk) : Record<T>::value(rk)) : T(0);
+                T evj[C], evk[C];
+#pragma unroll
+                for (int s = 0; s < C; s++) {
+                    evj[s] = shfl_t(vj, s * kTile + b);
+                    evk[s] = shfl_t(vk, s * kTile + a);
+                }
+                const T(&v)[NV][C] = buf[d];
+                T qv[C];
+#pragma unroll
+                for (int s = 0; s < C; s++) {
+                    // Values produced one step earlier by the neighbouring lanes.
+                    const T nj_cur = shfl_t(cur, src_j), nk_cur = shfl_t(cur, src_k);
+                    const T qy = edge_j ? evj[s] : nj_cur;
+                    const T qz = edge_k ? evk[s] : nk_cur;
+                    T q = v[0][s];
+                    if (FWD) {
+                        q -= v[1][s] * qz;
+                        q -= v[2][s] * qy;
+                        q -= v[3][s] * cur;
+                        q *= v[4][s];
+                    } else {
+                        q -= v[1][s] * cur;
+                        q -= v[2][s] * qy;
+                        q -= v[3][s] * qz;
+                        q *= v[4][s];
+                        dot_acc += (double)q * (double)v[NV - 1][s];
+                    }
+                    qv[s] = q;
+                    const int u = uc + s;
+                    if (pub_j) Record<T>::put(pb.edge_j + (((size_t)tile * S + u) * kTile + b) * RW, q, epoch);
+                    if (pub_k) Record<T>::put(pb.edge_k + (((size_t)tile * S + u) * kTile + a) * RW, q, epoch);
+                    cur = q;
+                }
+                store_chunk(uc, qv);
+                const int uf = uc + D * C;
+                load_chunk(uf < last ? uf : last, buf[d]);
+                __builtin_amdgcn_sched_barrier(0);
             }
             if (aborted) break;
-            const T vj = nj ? Record<T>::value(rj) : T(0);
-            const T vk = nk ? Record<T>::value(rk) : T(0);
-            T evj[C], evk[C];
-#pragma unroll
-            for (int s = 0; s < C; s++) {
-                evj[s] = shfl_t(vj, s * kTile + b);
-                evk[s] = shfl_t(vk, s * kTile + a);
-            }
-#pragma unroll
-            for (int s = 0; s < C; s++) {
-                const int u = u0 + s;
-                // Values produced one step earlier by the neighbouring lanes.
-                T nj_cur = shfl_t(cur, src_j), nk_cur = shfl_t(cur, src_k);
-                if (u >= te) continue;  // uniform across the wave
-                T qy = edge_j ? evj[s] : nj_cur;
-                T qz = edge_k ? evk[s] : nk_cur;
-                T q = v[0][s];
-                if (FWD) {
-                    q -= v[1][s] * qz;
-                    q -= v[2][s] * qy;
-                    q -= v[3][s] * cur;
-                    q *= v[4][s];
-                } else {
-                    q -= v[1][s] * cur;
-                    q -= v[2][s] * qy;
-                    q -= v[3][s] * qz;
-                    q *= v[4][s];
-                    dot_acc += (double)q * (double)v[NV - 1][s];
-                }
-                if (live_at(u)) op.out[slot_at(u)] = q;
-                if (pub_j) Record<T>::put(pb.edge_j + (((size_t)tile * S + u) * kTile + b) * RW, q, epoch);
-                if (pub_k) Record<T>::put(pb.edge_k + (((size_t)tile * S + u) * kTile + a) * RW, q, epoch);
-                cur = q;
-            }
         }
     }
     finish_sweep(pb, dot ? dotmode : kDotNone, dot_acc, tile, false);
 }
+
+// Register ring depth per value type: FP32 operands are half the size.
+template <class T>
+struct SweepDepth;
+template <>
+struct SweepDepth<double> {
+    static constexpr int D = 2;
+};
+template <>
+struct SweepDepth<float> {
+    static constexpr int D = 4;
+};
 
 template <class T>
 static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, SweepOps<T> op,
@@ -704,8 +785,9 @@ static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, 
         pb.epoch = 1;
     }
     int ntiles = pb.sm.ntj * pb.sm.ntk;
-    if (dir == kForward) k_sweep<T, kForward><<<ntiles, 64, 0, s>>>(g, pb, op, dotmode);
-    else k_sweep<T, kBackward><<<ntiles, 64, 0, s>>>(g, pb, op, dotmode);
+    constexpr int D = SweepDepth<T>::D;
+    if (dir == kForward) k_sweep<T, kForward, D><<<ntiles, 64, 0, s>>>(g, pb, op, dotmode);
+    else k_sweep<T, kBackward, D><<<ntiles, 64, 0, s>>>(g, pb, op, dotmode);
     return hipGetLastError();
 }
 
@@ -782,17 +864,14 @@ __device__ bool last_block(unsigned *counter) {
 __device__ __forceinline__ double spmv_slot(const PressureBuffers &pb, const double *__restrict__ x,
                                             size_t p, int i, int j, int k) {
     const SlotMap &sm = pb.sm;
-    const int a = j & 7, b = k & 7;
-    const size_t pzm = b > 0 ? p - 72 : sm.slot(i, j, k - 1);
-    const size_t pym = a > 0 ? p - 65 : sm.slot(i, j - 1, k);
-    const size_t pyp = a < 7 ? p + 65 : sm.slot(i, j + 1, k);
-    const size_t pzp = b < 7 ? p + 72 : sm.slot(i, j, k + 1);
+    const size_t pzm = sm.slot(i, j, k - 1), pym = sm.slot(i, j - 1, k), pxm = sm.slot(i - 1, j, k);
+    const size_t pxp = sm.slot(i + 1, j, k), pyp = sm.slot(i, j + 1, k), pzp = sm.slot(i, j, k + 1);
     double s = 0;
     s += pb.offz[pzm] * x[pzm];
     s += pb.offy[pym] * x[pym];
-    s += pb.offx[p - 64] * x[p - 64];
+    s += pb.offx[pxm] * x[pxm];
     s += pb.diag[p] * x[p];
-    s += pb.offx[p] * x[p + 64];
+    s += pb.offx[p] * x[pxp];
     s += pb.offy[p] * x[pyp];
     s += pb.offz[p] * x[pzp];
     return s;
@@ -966,17 +1045,14 @@ __global__ void k_pcg_init_vectors32(PressureBuffers pb) {
 __device__ __forceinline__ float spmv_slot32(const PressureBuffers &pb, const float *__restrict__ x,
                                              size_t p, int i, int j, int k) {
     const SlotMap &sm = pb.sm;
-    const int a = j & 7, b = k & 7;
-    const size_t pzm = b > 0 ? p - 72 : sm.slot(i, j, k - 1);
-    const size_t pym = a > 0 ? p - 65 : sm.slot(i, j - 1, k);
-    const size_t pyp = a < 7 ? p + 65 : sm.slot(i, j + 1, k);
-    const size_t pzp = b < 7 ? p + 72 : sm.slot(i, j, k + 1);
+    const size_t pzm = sm.slot(i, j, k - 1), pym = sm.slot(i, j - 1, k), pxm = sm.slot(i - 1, j, k);
+    const size_t pxp = sm.slot(i + 1, j, k), pyp = sm.slot(i, j + 1, k), pzp = sm.slot(i, j, k + 1);
     float s = 0.0f;
     s += pb.offz32[pzm] * x[pzm];
     s += pb.offy32[pym] * x[pym];
-    s += pb.offx32[p - 64] * x[p - 64];
+    s += pb.offx32[pxm] * x[pxm];
     s += pb.diag32[p] * x[p];
-    s += pb.offx32[p] * x[p + 64];
+    s += pb.offx32[p] * x[pxp];
     s += pb.offy32[p] * x[pyp];
     s += pb.offz32[p] * x[pzp];
     return s;
