@@ -95,7 +95,7 @@ struct PcgScalars {
 // Sweep-ordered ("slot") storage of the pressure system.  Cells are grouped
 // into 8 x 8 column tiles over (j, k); a cell of tile (jt, kt) has lane
 // a + 8 b (a = j % 8, b = k % 8) and wavefront step t = i + a + b,
-// 0 <= t < S (S >= I + 14, a multiple of 8).  Steps are stored in groups of
+// 0 <= t < S (S >= I + 14, a multiple of 32).  Steps are stored in groups of
 // four: the slot of (tile, t, lane) is
 //     ((tile * S/4 + t/4) * 64 + lane) * 4 + t % 4,
 // so a wave that owns a tile loads four steps of all its 64 cells with one
@@ -117,7 +117,7 @@ inline SlotMap make_slot_map(int I, int J, int K) {
     SlotMap m;
     m.I = I; m.J = J; m.K = K;
     m.ntj = (J + 7) / 8; m.ntk = (K + 7) / 8;
-    m.S = (I + 14 + 7) & ~7;
+    m.S = (I + 14 + 31) & ~31;
     return m;
 }
 
@@ -135,7 +135,7 @@ struct PressureBuffers {
     float *r32, *z32, *s32;
     double *partials;                   // per-block partial sums
     int npartials;
-    unsigned long long *edge_j, *edge_k;  // tagged values a tile hands to its j / k neighbour
+    unsigned long long *edge;           // tagged values a tile hands to its j / k neighbours
     unsigned int epoch;                 // tag of the current sweep launch (host counter)
     unsigned int *progress;             // factorisation: tile progress counters
     int ntiles_j, ntiles_k;
@@ -177,7 +177,7 @@ hipError_t launch_precond_apply(const GridGeom &g, PressureBuffers &pb, const do
                                 double *z, hipStream_t s);
 hipError_t launch_spmv(const GridGeom &g, PressureBuffers &pb, const double *x, double *y,
                        hipStream_t s);
-// Number of tagged records in each of pb.edge_j / pb.edge_k.
+// Number of 64-bit words of pb.edge.
 size_t edge_record_count(const SlotMap &sm);
 
 // ---- particles (k_particles.hip) ----

@@ -32,8 +32,13 @@ constexpr double kMaxTheta = 25.0;  // pressuresolver.h:163
 constexpr unsigned kSpinLimit = 1u << 20;  // ~1 s of polling: a legitimate wait is microseconds
 constexpr int kRecWords = 2;       // 64-bit words per tagged FP64 record
 
+// Records per launch: for every tile and march step, 16 slots -- the values
+// of the tile's 8 cells on its j edge and of the 7 further cells on its k
+// edge (the corner cell's one value serves both neighbours).
+constexpr int kRecSlots = 16;
+
 size_t edge_record_count(const SlotMap &sm) {
-    return (size_t)sm.ntj * sm.ntk * sm.S * kTile * kRecWords;
+    return (size_t)sm.ntj * sm.ntk * sm.S * kRecSlots * kRecWords;
 }
 
 // Cell of a slot (may lie outside the grid: skew corners and padding).
@@ -291,10 +296,11 @@ __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const Gri
         ue = imax + 15;
     }
     if (ue > pb.sm.S) ue = pb.sm.S;
-    // Whole chunks of 8 steps (S is a multiple of 8): the extra steps hold no
-    // rows and compute zeros.
-    ub &= ~7;
-    ue = (ue + 7) & ~7;
+    // Whole groups of 32 steps (S is a multiple of 32), so a sweep's register
+    // ring turns over an exact number of times; the extra steps hold no rows
+    // and compute zeros.
+    ub &= ~31;
+    ue = (ue + 31) & ~31;
 }
 
 // The last wave of a sweep launch resets the shared counters for the next
@@ -584,8 +590,11 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
     const int src_k = (FWD ? lane - 8 : lane + 8) & 63;  // neighbour lane along k
     const bool edge_j = FWD ? a == 0 : a == 7;            // reads the upstream j tile
     const bool edge_k = FWD ? b == 0 : b == 7;
-    const bool pub_j = FWD ? a == 7 : a == 0;             // writes for the downstream j tile
-    const bool pub_k = FWD ? b == 7 : b == 0;
+    // Edge cells (read by the downstream neighbours) publish one record per
+    // step: slot b for the j edge, slot 8 + a for the rest of the k edge.
+    const int aj = FWD ? 7 : 0, bk = FWD ? 7 : 0;
+    const bool pub = a == aj || b == bk;
+    const int pub_slot = a == aj ? b : 8 + a;
     const unsigned epoch = pb.epoch;
     const size_t grp0 = (size_t)tile * (size_t)(S >> 2);
     const bool dot = !FWD && dotmode != kDotNone;
@@ -644,18 +653,22 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
             int uu = u0 + 7 + rs;
             return (int)(up >= 0) & (int)(uu >= ub) & (int)(uu < ue);
         };
-        auto rec_ptr = [&](const unsigned long long *base, int rt, int u0) -> const unsigned long long * {
+        // Record of lane L for the chunk at u0: step u0 + 7 + L / 8 of the
+        // upstream tile, edge lane L % 8.
+        const int rslot_j = re;
+        const int rslot_k = re == aj ? bk : 8 + re;
+        auto rec_ptr = [&](int rt, int slot, int u0) -> const unsigned long long * {
             int uu = u0 + 7 + rs;
             uu = uu < S ? uu : S - 1;
-            return base + ((((size_t)rt * S + (size_t)uu) * kTile + re) * RW);
+            return pb.edge + ((((size_t)rt * S + (size_t)uu) * kRecSlots + slot) * RW);
         };
         auto fetch = [&](const unsigned long long *q, unsigned long long (&w)[2]) {
             w[0] = ld_dev(q);
             w[1] = RW == 2 ? ld_dev(q + 1) : 0ull;
         };
         auto load_recs = [&](int u0, unsigned long long (&rj)[2], unsigned long long (&rk)[2]) {
-            fetch(rec_ptr(pb.edge_j, rj_tile, u0), rj);
-            fetch(rec_ptr(pb.edge_k, rk_tile, u0), rk);
+            fetch(rec_ptr(rj_tile, rslot_j, u0), rj);
+            fetch(rec_ptr(rk_tile, rslot_k, u0), rk);
         };
 
         // Issue order in the prologue matches the loop's steady state (records
@@ -672,12 +685,12 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
             __builtin_amdgcn_sched_barrier(0);
         }
         T cur = T(0);
-        bool aborted = false;
         for (int u0 = tb; u0 < te; u0 += D * C) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
+                // Ranges are whole multiples of D chunks: no exit inside the
+                // ring (an exit here costs the compiler its load accounting).
                 const int uc = u0 + d * C;
-                if (uc >= te) break;  // uniform across the wave
                 unsigned long long(&rj)[2] = recj[d];
                 unsigned long long(&rk)[2] = reck[d];
                 const int un = uc + C < te ? uc + C : uc;
@@ -699,21 +712,22 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
                 if (slow) {
                     unsigned spins = 0;
                     for (;;) {
-                        fetch(rec_ptr(pb.edge_j, rj_tile, uc), tj);
-                        fetch(rec_ptr(pb.edge_k, rk_tile, uc), tk);
+                        fetch(rec_ptr(rj_tile, rslot_j, uc), tj);
+                        fetch(rec_ptr(rk_tile, rslot_k, uc), tk);
                         drain_memory();
                         const int still = ((int)nj & (int)!Record<T>::ok(tj, epoch)) |
                                           ((int)nk & (int)!Record<T>::ok(tk, epoch));
                         if (__ballot(still) == 0ull) break;
                         if (ld_dev(&pb.sc->abort) != 0 || ++spins > kSpinLimit) {
+                            // Give up: the launch is reported as timed out and
+                            // its results discarded; the wave runs on without
+                            // waiting so every wave of the grid exits.
                             if (lane == 0) st_dev(&pb.sc->abort, 1);
-                            aborted = true;
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
                     }
                     drain_memory();
-                    if (aborted) break;
                 }
                 const T vj = nj ? (slow ? Record<T>::value(tj) : Record<T>::value(rj)) : T(0);
                 const T vk = nk ? (slow ? Record<T>::value(tk) : Record<T>::value(rk)) : T(0);
@@ -724,6 +738,8 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
                     evk[s] = shfl_t(vk, s * kTile + a);
                 }
                 const T(&v)[NV][C] = buf[d];
+                unsigned long long *const pubp =
+                    pb.edge + ((((size_t)tile * S + (size_t)uc) * kRecSlots + pub_slot) * RW);
                 T qv[C];
 #pragma unroll
                 for (int s = 0; s < C; s++) {
@@ -745,9 +761,7 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
                         dot_acc += (double)q * (double)v[NV - 1][s];
                     }
                     qv[s] = q;
-                    const int u = uc + s;
-                    if (pub_j) Record<T>::put(pb.edge_j + (((size_t)tile * S + u) * kTile + b) * RW, q, epoch);
-                    if (pub_k) Record<T>::put(pb.edge_k + (((size_t)tile * S + u) * kTile + a) * RW, q, epoch);
+                    if (pub) Record<T>::put(pubp + s * kRecSlots * RW, q, epoch);
                     cur = q;
                 }
                 store_chunk(uc, qv);
@@ -755,7 +769,6 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
                 load_chunk(uf < last ? uf : last, buf[d]);
                 __builtin_amdgcn_sched_barrier(0);
             }
-            if (aborted) break;
         }
     }
     finish_sweep(pb, dot ? dotmode : kDotNone, dot_acc, tile, false);
@@ -779,8 +792,7 @@ static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, 
     // A fresh epoch per launch; records of older launches never match it.
     if (++pb.epoch >= 0xfffffff0u) {
         size_t n = edge_record_count(pb.sm);
-        hipError_t e = launch_fill_u8((uint8_t *)pb.edge_j, n * 8, 0, s);
-        if (e == hipSuccess) e = launch_fill_u8((uint8_t *)pb.edge_k, n * 8, 0, s);
+        hipError_t e = launch_fill_u8((uint8_t *)pb.edge, n * 8, 0, s);
         if (e != hipSuccess) return e;
         pb.epoch = 1;
     }

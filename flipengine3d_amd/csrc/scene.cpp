@@ -117,8 +117,8 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
         if (arr[q].alloc(ns) != hipSuccess)
             throw StatusError{FLIPMI_ERR_OOM, "pressure arrays: out of device memory"};
     size_t nrec = edge_record_count(sm);
-    if (arr[kSlotArrays].alloc((size_t)npart) != hipSuccess || edge_j.alloc(nrec) != hipSuccess ||
-        edge_k.alloc(nrec) != hipSuccess || progress.alloc((size_t)ntj * ntk) != hipSuccess ||
+    if (arr[kSlotArrays].alloc((size_t)npart) != hipSuccess || edge.alloc(nrec) != hipSuccess ||
+        progress.alloc((size_t)ntj * ntk) != hipSuccess ||
         ticket.alloc(1) != hipSuccess || scal.alloc(1) != hipSuccess ||
         irange.alloc((size_t)2 * ntj * ntk) != hipSuccess)
         throw StatusError{FLIPMI_ERR_OOM, "pressure scratch: out of device memory"};
@@ -127,8 +127,7 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     (void)launch_fill_u8((uint8_t *)scal.p, sizeof(PcgScalars), 0, s);
     // Slots outside the grid are never written by the kernels: start at zero.
     for (int q = 0; q < kSlotArrays; q++) (void)launch_fill_f64(arr[q].p, ns, 0.0, s);
-    (void)launch_fill_u8((uint8_t *)edge_j.p, nrec * sizeof(unsigned long long), 0, s);
-    (void)launch_fill_u8((uint8_t *)edge_k.p, nrec * sizeof(unsigned long long), 0, s);
+    (void)launch_fill_u8((uint8_t *)edge.p, nrec * sizeof(unsigned long long), 0, s);
     pb.sm = sm;
     pb.diag = arr[0].p; pb.offx = arr[1].p; pb.offy = arr[2].p; pb.offz = arr[3].p;
     pb.rhs = arr[4].p; pb.x = arr[5].p; pb.r = arr[6].p; pb.z = arr[7].p; pb.s = arr[8].p;
@@ -147,8 +146,7 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
         for (int q = 0; q < kSlotArrays32; q++) *dst[q] = arr32[q].p;
     }
     pb.npartials = npart;
-    pb.edge_j = edge_j.p;
-    pb.edge_k = edge_k.p;
+    pb.edge = edge.p;
     pb.epoch = 0;
     pb.progress = progress.p;
     pb.ntiles_j = ntj;

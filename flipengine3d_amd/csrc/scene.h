@@ -160,7 +160,7 @@ struct PressureStore {
     static constexpr int kSlotArrays32 = 14;  // FP32 matrix, factor and vectors (mixed precision)
     DBuf<double> arr[kSlotArrays + 1];
     DBuf<float> arr32[kSlotArrays32];
-    DBuf<unsigned long long> edge_j, edge_k;
+    DBuf<unsigned long long> edge;
     DBuf<unsigned> progress, ticket;
     DBuf<int> irange;
     DBuf<PcgScalars> scal;
