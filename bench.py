@@ -54,7 +54,13 @@ def parse_args(argv=None):
     p.add_argument("--size", type=int, nargs=3, default=[256, 128, 128], metavar=("I", "J", "K"))
     p.add_argument("--dt", type=float, default=1.0 / 240.0)
     p.add_argument("--seed", type=int, default=3)
-    p.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    p.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
+                   help="PCG arithmetic of the measured path (the 256x128x128 workload is the mixed-precision one)")
+    p.add_argument("--fp64-path", dest="fp64_path", action="store_true", default=True,
+                   help="also time the reference-arithmetic FP64 PCG path (reported under config.fp64_path)")
+    p.add_argument("--no-fp64-path", dest="fp64_path", action="store_false")
+    p.add_argument("--profiling", type=int, default=2, choices=[0, 1, 2],
+                   help="hipEvent timing: 2 = every PCG kernel launch (needed for the roofline line)")
     p.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpus available)")
@@ -133,6 +139,94 @@ def cpu_baseline(scene, args):
     }
 
 
+
+def run_gpu(fa, scene, args, precision, steps, warmup, profiling):
+    """Warmup + timed substeps of one scene; returns throughput, latency
+    percentiles, the PCG bandwidth and the dominant kernel's roofline."""
+    a = argparse.Namespace(**vars(args))
+    a.precision = precision
+    sim = make_sim(fa, scene, a)
+    sim.set_profiling(profiling)
+    n = sim.getNumMarkerParticles()
+    for w in range(warmup):
+        st = sim.update(args.dt)
+        log(f"{precision} warmup {w}: {st.pcg_iterations} PCG iterations, {st.gpu_ms:.1f} ms", args)
+        n = st.num_particles
+    kern = None
+    if profiling >= 2:
+        # The warmup timed every PCG kernel; the timed region times only the
+        # dominant one (each timed launch costs an event pair).
+        wt = sim.stage_times()
+        kern = max(PCG_WORDS, key=lambda k: wt[k][0])
+        sim.set_kernel_timing([kern])
+    sim.stage_times(reset=True)
+
+    lat = []
+    work = 0
+    rows_iters = 0
+    iters = 0
+    refinements = 0
+    ok = True
+    sim.ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        st = sim.update(args.dt)
+        lat.append(time.perf_counter() - ts)
+        work += n * st.substeps
+        n = st.num_particles
+        rows_iters += st.pressure_rows * st.pcg_iterations
+        iters += st.pcg_iterations
+        refinements += getattr(st, "pcg_refinements", 0)
+        ok = ok and st.pressure_ok == 1
+    sim.ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    times = sim.stage_times()
+    sim.close()
+
+    word = 4 if precision == "mixed" else 8
+    rows_per_launch = rows_iters / max(iters, 1)
+    # PCG bandwidth: algorithmic bytes of every iteration (27 words per row,
+    # see PCG_WORDS) plus 40 bytes per row for each FP64 true residual, over
+    # the PCG stage time.
+    pcg_bytes = rows_iters * sum(PCG_WORDS.values()) * word + refinements * rows_per_launch * 40
+    pcg_ms = times["pcg"][0]
+    roof = {"kernel": None, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": None, "traffic": None}
+    if kern is not None:
+        kms, kcalls = times[kern]
+        bytes_per_launch = PCG_WORDS[kern] * word * rows_per_launch
+        # Iterations are enqueued in batches, so a few launches after
+        # convergence return at once; the average is over the launches that
+        # iterated.
+        kcalls = min(kcalls, iters)
+        launch_s = kms * 1e-3 / max(kcalls, 1)
+        achieved = bytes_per_launch / launch_s / 1e9 if launch_s > 0 else 0.0
+        roof.update({
+            "kernel": f"k_sweep<{'forward' if kern == 'k_forward' else 'backward'}>"
+            if kern in ("k_forward", "k_backward") else kern,
+            "achieved": achieved,
+            "frac": achieved / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": launch_s * 1e6,
+            "launches": kcalls,
+        })
+    lat_ms = sorted(x * 1e3 for x in lat)
+    p99 = lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))]
+    return {
+        "value": work / elapsed,
+        "ms_per_step": elapsed * 1e3 / steps,
+        "p50_ms": statistics.median(lat_ms),
+        "p99_ms": p99,
+        "pcg_hbm_gbs": pcg_bytes / (pcg_ms * 1e-3) / 1e9 if pcg_ms > 0 else None,
+        "iters_per_step": iters / steps,
+        "refinements_per_step": refinements / steps,
+        "rows": round(rows_per_launch),
+        "ok": ok,
+        "roofline": roof,
+        "stages": {k: round(v[0] / steps, 3) for k, v in times.items() if k != "pcg_iters"},
+    }
+
 def main(argv=None):
     args = parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -151,90 +245,44 @@ def main(argv=None):
     scene = scenes.double_dam_break(I, J, K, dx, seed=args.seed)
     log(f"scene: {scene.num_particles} particles ({time.perf_counter() - t:.1f} s)", args)
 
-    sim = make_sim(fa, scene, args)
-    sim.set_profiling(2)
-    n = sim.getNumMarkerParticles()
-    for w in range(args.warmup):
-        st = sim.update(args.dt)
-        log(f"warmup {w}: {st.pcg_iterations} PCG iterations, {st.gpu_ms:.1f} ms", args)
-        n = st.num_particles
-    sim.stage_times(reset=True)
-
-    lat = []
-    work = 0
-    rows_iters = 0
-    iters = 0
-    ok = True
-    sim.ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        st = sim.update(args.dt)
-        lat.append(time.perf_counter() - ts)
-        work += n * st.substeps
-        n = st.num_particles
-        rows_iters += st.pressure_rows * st.pcg_iterations
-        iters += st.pcg_iterations
-        ok = ok and st.pressure_ok == 1
-    sim.ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    times = sim.stage_times()
-
-    # Dominant kernel: the PCG kernel with the largest total time.
-    kern = max(PCG_WORDS, key=lambda k: times[k][0])
-    kms, kcalls = times[kern]
-    word = 4 if args.precision == "mixed" else 8
-    rows_per_launch = rows_iters / max(iters, 1)
-    bytes_per_launch = PCG_WORDS[kern] * word * rows_per_launch
-    # Iterations are enqueued in batches, so a few launches after convergence
-    # return at once; the average is taken over the launches that iterated.
-    kcalls = min(kcalls, iters)
-    launch_s = kms * 1e-3 / max(kcalls, 1)
-    achieved = bytes_per_launch / launch_s / 1e9 if launch_s > 0 else 0.0
-    stages = {k: round(v[0] / args.steps, 3) for k, v in times.items() if k != "pcg_iters"}
-
-    lat_ms = sorted(x * 1e3 for x in lat)
-    p99 = lat_ms[min(len(lat_ms) - 1, int(round(0.99 * (len(lat_ms) - 1))))]
+    primary = run_gpu(fa, scene, args, args.precision, args.steps, args.warmup, args.profiling)
     result = {
         "metric": "particle-substeps/s @256x128x128 + PCG HBM GB/s; p50/p99 step latency",
-        "value": work / elapsed,
+        "value": primary["value"],
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
-        "p50_ms": statistics.median(lat_ms),
-        "p99_ms": p99,
+        "ms_per_step": primary["ms_per_step"],
+        "p50_ms": primary["p50_ms"],
+        "p99_ms": primary["p99_ms"],
+        "pcg_hbm_gbs": primary["pcg_hbm_gbs"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 particles/grids, f64 PCG" if args.precision == "fp64" else "f32 PCG, f64 dots/refinement",
+        "dtype": "f32 particles/grids, f64 PCG" if args.precision == "fp64" else
+                 "f32 particles/grids and PCG, f64 dots/x/true residual",
         "data": "synthetic",
         "config": {
             "workload": f"double dam break {I}x{J}x{K}, dx=1/{round(1 / dx)}, seed {args.seed}, "
                         f"one substep of dt={args.dt:.6g} s per step",
             "particles": scene.num_particles,
             "pcg_precision": args.precision,
-            "pcg_iterations_per_step": iters / args.steps,
-            "pressure_rows": round(rows_per_launch),
-            "pressure_ok": ok,
+            "pcg_iterations_per_step": primary["iters_per_step"],
+            "pcg_refinements_per_step": primary["refinements_per_step"],
+            "pressure_rows": primary["rows"],
+            "pressure_ok": primary["ok"],
+            "profiling_level": args.profiling,
         },
-        "roofline": {
-            "kernel": f"k_sweep<{'forward' if kern == 'k_forward' else 'backward'}>"
-            if kern in ("k_forward", "k_backward") else kern,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_us": launch_s * 1e6,
-            "launches": kcalls,
-        },
-        "stage_ms_per_step": stages,
+        "roofline": primary["roofline"],
+        "stage_ms_per_step": primary["stages"],
     }
-    sim.close()
+    if args.fp64_path and args.precision != "fp64":
+        # The reference-arithmetic FP64 PCG path on the same scene, for comparison.
+        alt = run_gpu(fa, scene, args, "fp64", max(3, args.steps // 4), 1, args.profiling)
+        result["config"]["fp64_path"] = {k: alt[k] for k in ("value", "ms_per_step", "p50_ms", "iters_per_step",
+                                                              "pcg_hbm_gbs")}
+        result["config"]["fp64_path"]["roofline_frac"] = alt["roofline"]["frac"]
     if args.cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, args)
         result["config"]["gpu_over_cpu"] = result["value"] / result["cpu_baseline"]["value"]

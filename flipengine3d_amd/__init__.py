@@ -56,7 +56,7 @@ class SceneDesc(ctypes.Structure):
 class FrameStats(ctypes.Structure):
     _fields_ = [("substeps", _c_int), ("num_particles", _c_i64), ("pcg_iterations", _c_int),
                 ("pcg_residual", _c_double), ("pressure_ok", _c_int), ("gpu_ms", _c_double),
-                ("pressure_rows", _c_i64)]
+                ("pressure_rows", _c_i64), ("pcg_refinements", _c_int)]
 
 
 class StageTimes(ctypes.Structure):
@@ -83,7 +83,7 @@ EXPORTED = [
     "flipmi_scene_create", "flipmi_scene_destroy", "flipmi_scene_load_particles",
     "flipmi_scene_initialize", "flipmi_scene_update", "flipmi_scene_num_particles",
     "flipmi_scene_get_particles", "flipmi_scene_get_grid", "flipmi_scene_stage_times",
-    "flipmi_scene_set_profiling", "flipmi_domain_solid", "flipmi_liquid_sdf", "flipmi_p2g",
+    "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf", "flipmi_p2g",
     "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
     "flipmi_trilinear_velocity",
 ]
@@ -116,6 +116,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_get_grid.argtypes = [_vp, _c_int, _vp]
     L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
     L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
+    L.flipmi_scene_set_kernel_timing.argtypes = [_vp, ctypes.c_uint]
     L.flipmi_domain_solid.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double] + [_vp] * 5
     L.flipmi_liquid_sdf.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_double, _vp, _vp]
     L.flipmi_p2g.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _c_i64, _c_double] + [_vp] * 6
@@ -352,6 +353,17 @@ class FluidSimulation:
         self._ensure()
         lvl = int(level) if not isinstance(level, bool) else (1 if level else 0)
         self.ctx.check(self.L.flipmi_scene_set_profiling(self.h, lvl))
+
+    def set_kernel_timing(self, kernels=None):
+        """PCG kernels timed at profiling level 2 (names from STAGES, e.g.
+        "k_backward"); None = all."""
+        self._ensure()
+        mask = 0xFFFFFFFF
+        if kernels is not None:
+            mask = 0
+            for k in kernels:
+                mask |= 1 << (STAGES.index(k) - STAGES.index("k_spmv_dot"))
+        self.ctx.check(self.L.flipmi_scene_set_kernel_timing(self.h, mask))
 
     def stage_times(self, reset: bool = False) -> dict:
         t = StageTimes()

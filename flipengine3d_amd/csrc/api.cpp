@@ -258,6 +258,12 @@ int flipmi_scene_set_profiling(flipmi_scene *scene, int enable) {
     return FLIPMI_OK;
 }
 
+int flipmi_scene_set_kernel_timing(flipmi_scene *scene, unsigned mask) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    scene->s.kernel_mask = mask;
+    return FLIPMI_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Stage drop-ins
 // ---------------------------------------------------------------------------

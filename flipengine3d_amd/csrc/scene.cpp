@@ -191,20 +191,28 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
     hip_check(launch_pcg_init(w.g, w.tol, w.maxiter, mixed, pb, s), "pcg init");
     if (prof) prof->end_stage(FLIPMI_STAGE_MIC0, ev);
     if (prof) prof->begin_stage(FLIPMI_STAGE_PCG, &ev);
+    // Iterations are enqueued in batches between host checks of the device
+    // scalars.  After the first batch, the batch size is the number of
+    // iterations the observed convergence rate still needs (plus a margin),
+    // so few launches are wasted past convergence.
     int batch = 8;
     int launched = 0;
+    int prev_iters = 0;
+    double prev_resid = bnorm;
+    const double target = w.tol * bnorm;
     for (;;) {
         for (int q = 0; q < batch; q++) {
-            if (prof && prof->profiling >= 2) {
-                // Time each kernel of the iteration on the launch stream.
+            if (prof && prof->profiling >= 2 && prof->kernel_mask != 0) {
+                // Time the selected kernels of the iteration on the launch stream.
                 for (int q = 0; q < kPcgPieces; q++) {
                     int piece = pcg_piece_at(q);
                     if (!pcg_piece_used(piece, mixed)) continue;
                     int stage = piece == kPcgResidual ? FLIPMI_STAGE_K_RESIDUAL : FLIPMI_STAGE_K_SPMV + piece;
+                    const bool timed = (prof->kernel_mask >> piece) & 1u;
                     hipEvent_t e0 = nullptr;
-                    prof->begin_stage(stage, &e0);
+                    if (timed) prof->begin_stage(stage, &e0);
                     hip_check(launch_pcg_piece(piece, w.g, mixed, pb, s), "pcg kernel");
-                    prof->end_stage(stage, e0);
+                    if (timed) prof->end_stage(stage, e0);
                 }
             } else {
                 hip_check(launch_pcg_iteration(w.g, mixed, pb, s), "pcg iteration");
@@ -217,7 +225,16 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
         if (hs->done) break;
         if (launched > w.maxiter + 16)
             throw StatusError{FLIPMI_ERR_HIP, "pcg did not terminate within its iteration cap"};
-        batch = std::min(batch * 2, 64);
+        int next = std::min(batch * 2, 32);
+        const int done_iters = hs->iters - prev_iters;
+        if (done_iters > 0 && hs->resid > 0 && hs->resid < prev_resid && target > 0) {
+            const double rate = std::log(hs->resid / prev_resid) / done_iters;  // < 0
+            const double need = std::log(target / hs->resid) / rate;
+            if (need > 0 && need < 1e6) next = (int)std::ceil(need) + 2;
+        }
+        prev_iters = hs->iters;
+        prev_resid = hs->resid;
+        batch = std::max(2, std::min(next, 64));
     }
     if (prof) prof->end_stage(FLIPMI_STAGE_PCG, ev);
     if (prof) prof->stage_calls[FLIPMI_STAGE_PCG_ITERS] += hs->iters;
@@ -429,6 +446,7 @@ void Scene::pressure_solve(double dt) {
     last_resid = o.residual;
     last_ok = o.success ? 1 : 0;
     last_rows = o.rows;
+    last_refinements = o.refinements;
 }
 
 void Scene::advance_particles(double dt) {
@@ -631,6 +649,7 @@ flipmi_frame_stats Scene::update(double dt) {
     st.pressure_ok = last_ok;
     st.gpu_ms = ms;
     st.pressure_rows = last_rows;
+    st.pcg_refinements = last_refinements;
     return st;
 }
 

@@ -98,9 +98,11 @@ struct Scene {
     int last_iters = 0, last_ok = 1;
     double last_resid = 0;
     int64_t last_rows = 0;
+    int last_refinements = 0;
 
     // profiling
-    int profiling = 0;  // 0 off, 1 stage events, 2 also each PCG kernel
+    int profiling = 0;  // 0 off, 1 stage events, 2 also each selected PCG kernel
+    unsigned kernel_mask = ~0u;  // PCG kernels timed at level 2 (bit = kPcg* piece)
     double stage_ms[FLIPMI_NUM_STAGES] = {0};
     int64_t stage_calls[FLIPMI_NUM_STAGES] = {0};
     struct PendingEvent {

@@ -78,6 +78,7 @@ typedef struct {
     int pressure_ok;           /* 0 if the last pressure solve failed */
     double gpu_ms;             /* device time of the frame */
     int64_t pressure_rows;     /* last substep: rows of the Poisson system */
+    int pcg_refinements;       /* last substep, mixed precision: FP64 true residuals */
 } flipmi_frame_stats;
 
 /* Per-stage device time accumulated since the last reset (hipEvents). */
@@ -158,6 +159,10 @@ int flipmi_scene_stage_times(flipmi_scene *scene, flipmi_stage_times *out, int r
 /* hipEvent timing on the scene's stream: level 0 off, 1 one event pair per
  * stage, 2 additionally one pair around every PCG kernel launch. */
 int flipmi_scene_set_profiling(flipmi_scene *scene, int level);
+/* PCG kernels timed at profiling level 2, one bit per FLIPMI_STAGE_K_*
+ * stage minus FLIPMI_STAGE_K_SPMV (bit 5: the true residual); default all.
+ * Each timed launch adds an event pair, which delays the launches around it. */
+int flipmi_scene_set_kernel_timing(flipmi_scene *scene, unsigned mask);
 /* Blocks until all work queued by the context's scenes has finished. */
 int flipmi_ctx_synchronize(flipmi_ctx *ctx);
 
