@@ -485,6 +485,7 @@ int flipmi_mic0_apply(flipmi_ctx *ctx, int I, int J, int K, double dx, double dt
         hipck(launch_pressure_assemble(g, dt, liq.p, wu.p, wv.p, ww.p, wc.p, zero_u.p, zero_v.p, zero_w.p,
                                        zero_u.p, zero_v.p, zero_w.p, ps.pb, s),
               "assemble");
+        hipck(launch_group_mask(ps.pb, s), "group mask");
         hipck(launch_mic0_factor(g, liq.p, ps.pb, s), "factor");
         // Scatter r from reference row order into slot order.
         std::vector<float> hl(nc);
@@ -504,13 +505,12 @@ int flipmi_mic0_apply(flipmi_ctx *ctx, int I, int J, int K, double dx, double dt
                         rows.push_back(p);
                     }
                 }
-        DBuf<double> dr, dz;
-        upload(dr, rd.data(), ns, s);
-        alloc_dev(dz, ns);
-        hipck(launch_fill_f64(dz.p, ns, 0.0, s), "fill");
-        hipck(launch_precond_apply(g, ps.pb, dr.p, dz.p, s), "precond");
+        // r and z live in the store's own slot arrays (with their zero and
+        // sink groups); z starts at zero from the allocation.
+        hipck(hipMemcpyAsync(ps.pb.r, rd.data(), ns * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+        hipck(launch_precond_apply(g, ps.pb, ps.pb.r, ps.pb.z, s), "precond");
         std::vector<double> zd(ns), inv(ns);
-        download(zd.data(), dz, ns, s);
+        hipck(hipMemcpyAsync(zd.data(), ps.pb.z, ns * sizeof(double), hipMemcpyDeviceToHost, s), "download");
         download(inv.data(), ps.arr[9], ns, s);
         hipck(hipStreamSynchronize(s), "sync");
         PcgScalars hs;

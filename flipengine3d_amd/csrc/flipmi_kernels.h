@@ -106,6 +106,12 @@ struct PcgScalars {
 struct SlotMap {
     int I, J, K, ntj, ntk, S;
     __host__ __device__ size_t count() const { return (size_t)ntj * ntk * S * 64; }
+    // Elements allocated per slot array: count() plus two trailing groups,
+    // a zero group (read in place of a group without rows, never written)
+    // and a sink group (written in place of such a group, never read).
+    __host__ __device__ size_t alloc_count() const { return count() + 512; }
+    __host__ __device__ size_t zero_group() const { return count(); }
+    __host__ __device__ size_t sink_group() const { return count() + 256; }
     __host__ __device__ size_t slot(int i, int j, int k) const {
         int a = j & 7, b = k & 7, t = i + a + b;
         size_t tile = (size_t)(j >> 3) + (size_t)ntj * (size_t)(k >> 3);
@@ -136,6 +142,10 @@ struct PressureBuffers {
     double *partials;                   // per-block partial sums
     int npartials;
     unsigned long long *edge;           // tagged values a tile hands to its j / k neighbours
+    // Groups (4 steps x 64 lanes) holding at least one row: bit g % 64 of
+    // word g / 64 of the tile's mask_words words (null: treat all as active).
+    unsigned long long *group_mask;
+    int mask_words;
     unsigned int epoch;                 // tag of the current sweep launch (host counter)
     unsigned int *progress;             // factorisation: tile progress counters
     int ntiles_j, ntiles_k;
@@ -150,6 +160,8 @@ hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *l
                                     const float *w, const float *sU, const float *sV,
                                     const float *sW, PressureBuffers &pb, hipStream_t s);
 hipError_t launch_pressure_reset(PressureBuffers &pb, hipStream_t s);
+// Marks the slot groups that hold a row (after assembly).
+hipError_t launch_group_mask(PressureBuffers &pb, hipStream_t s);
 // Zeroes the factor, finds each column tile's row extent and factorises.
 hipError_t launch_mic0_factor(const GridGeom &g, const float *liquid, PressureBuffers &pb,
                               hipStream_t s);

@@ -58,6 +58,15 @@ __device__ __forceinline__ bool interior(const GridGeom &g, int i, int j, int k)
     return i >= 1 && j >= 1 && k >= 1 && i <= g.I - 2 && j <= g.J - 2 && k <= g.K - 2;
 }
 
+// Whether slot group grp (256 consecutive slots) holds a row.
+__device__ __forceinline__ bool group_active(const PressureBuffers &pb, size_t grp) {
+    if (!pb.group_mask) return true;
+    const size_t S4 = (size_t)(pb.sm.S >> 2);
+    const size_t tile = grp / S4;
+    const int gi = (int)(grp % S4);
+    return (pb.group_mask[tile * pb.mask_words + (gi >> 6)] >> (gi & 63)) & 1ull;
+}
+
 // ---------------------------------------------------------------------------
 // Assembly
 // ---------------------------------------------------------------------------
@@ -148,6 +157,32 @@ hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *l
     size_t n = pb.sm.count();
     k_assemble<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, dt, liquid, wU, wV, wW, wC, u, v, w,
                                                            sU, sV, sW, pb);
+    return hipGetLastError();
+}
+
+// One 256-thread block per slot group: a group is active when any of its
+// slots has a nonzero matrix entry or right-hand side.
+__global__ __launch_bounds__(256) void k_group_mask(PressureBuffers pb) {
+    const size_t ngroups = pb.sm.count() >> 8;
+    const size_t S4 = (size_t)(pb.sm.S >> 2);
+    for (size_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const size_t p = (grp << 8) + threadIdx.x;
+        const int nz = (pb.diag[p] != 0.0) | (pb.offx[p] != 0.0) | (pb.offy[p] != 0.0) |
+                       (pb.offz[p] != 0.0) | (pb.rhs[p] != 0.0);
+        if (__syncthreads_or(nz) && threadIdx.x == 0) {
+            const size_t tile = grp / S4;
+            const int gi = (int)(grp % S4);
+            atomicOr(&pb.group_mask[tile * pb.mask_words + (gi >> 6)], 1ull << (gi & 63));
+        }
+    }
+}
+
+hipError_t launch_group_mask(PressureBuffers &pb, hipStream_t s) {
+    if (!pb.group_mask) return hipSuccess;
+    const size_t words = (size_t)pb.sm.ntj * pb.sm.ntk * pb.mask_words;
+    hipError_t e = launch_fill_u8((uint8_t *)pb.group_mask, words * sizeof(unsigned long long), 0, s);
+    if (e != hipSuccess) return e;
+    k_group_mask<<<kRedBlocks * 4, 256, 0, s>>>(pb);
     return hipGetLastError();
 }
 
@@ -610,6 +645,36 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
     // Tiles without an upstream neighbour read their own (never needed) records.
     const int rj_tile = up_j >= 0 ? up_j : tile, rk_tile = up_k >= 0 ? up_k : tile;
 
+    // Active-group bits of this tile (up to 256 groups; longer tiles treat
+    // every group as active).  Four scalar words, selected with uniform
+    // conditionals (an array here would be indexed through scratch memory).
+    unsigned long long gm0 = ~0ull, gm1 = ~0ull, gm2 = ~0ull, gm3 = ~0ull;
+    const bool use_mask = pb.group_mask && pb.mask_words <= 4;
+    if (use_mask) {
+        const unsigned long long *mw = pb.group_mask + (size_t)tile * pb.mask_words;
+        auto rd = [&](int w) -> unsigned long long {
+            if (w >= pb.mask_words) return ~0ull;
+            const unsigned long long v = mw[w];
+            // readfirstlane yields an int: widen through unsigned (no sign extension).
+            const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffull));
+            const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+            return (unsigned long long)lo | ((unsigned long long)hi << 32);
+        };
+        gm0 = rd(0); gm1 = rd(1); gm2 = rd(2); gm3 = rd(3);
+    }
+    // Without a usable mask all four words stay all-ones, so the bit test
+    // alone decides (no second condition for the optimiser to thread).
+    auto active = [&](int gi) -> bool {
+        const int w = gi >> 6;
+        const unsigned long long lo = (w & 1) ? gm1 : gm0, hi = (w & 1) ? gm3 : gm2;
+        const unsigned long long word = (w & 2) ? hi : lo;
+        return (word >> (gi & 63)) & 1ull;
+    };
+    // Offsets of this lane's slots in the zero and sink groups.  Only the
+    // offset depends on the mask, never the array pointer.
+    const size_t zbase = sm.zero_group() + ((size_t)lane << 2);
+    const size_t sbase = sm.sink_group() + ((size_t)lane << 2);
+
     double dot_acc = 0.0;
     if (te > tb) {
         auto gbase = [&](int gi) -> size_t { return ((grp0 + (size_t)gi) * 64 + (size_t)lane) << 2; };
@@ -621,7 +686,9 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
             const int gl = chunk_group(u0);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const size_t base = gbase(gl + h);
+                // A group without rows reads the arrays' zero group instead
+                // (same instructions; the 2 KB it spans stay in cache).
+                const size_t base = active(gl + h) ? gbase(gl + h) : zbase;
                 T e[NV][4];
                 load_group(FWD ? op.in : op.out, base, e[0]);
                 load_group(op.c1, base, e[1]);
@@ -644,7 +711,9 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
                 T e[4];
 #pragma unroll
                 for (int m = 0; m < 4; m++) e[m] = qv[FWD ? 4 * h + m : 7 - (4 * h + m)];
-                store_group(op.out, gbase(gl + h), e);
+                // The output of a group without rows stays zero: its values
+                // (all zero) go to the sink group instead.
+                store_group(op.out, active(gl + h) ? gbase(gl + h) : sbase, e);
             }
         };
         // Whether this lane's record of chunk u0 is needed: the producer tile
@@ -896,8 +965,9 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBu
     if (pb.sc->done) return;
     size_t n = pb.sm.count();
     double acc = 0.0;
-    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * kRedThreads) {
+    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+        if (!group_active(pb, grp)) continue;  // uniform per block
+        const size_t p = (grp << 8) + threadIdx.x;
         int i, j, k;
         decode_slot(pb.sm, p, i, j, k);
         double zc = 0.0;
@@ -930,8 +1000,9 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm(GridGeom g, PressureB
     float af = pb.sc->alpha_f;
     float naf = (float)(-pb.sc->alpha);
     double mx = 0.0;
-    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * kRedThreads) {
+    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+        if (!group_active(pb, grp)) continue;  // uniform per block
+        const size_t p = (grp << 8) + threadIdx.x;
         pb.x[p] += af * pb.s[p];
         double r = pb.r[p] + naf * pb.z[p];
         pb.r[p] = r;
@@ -966,8 +1037,9 @@ __global__ void k_update_s(GridGeom g, PressureBuffers pb, int first) {
     if (pb.sc->done) return;
     size_t n = pb.sm.count();
     float bf = first ? 0.0f : pb.sc->beta_f;
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * blockDim.x) {
+    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+        if (!group_active(pb, grp)) continue;  // uniform per block
+        const size_t p = (grp << 8) + threadIdx.x;
         if (first) pb.s[p] = pb.z[p];
         else pb.s[p] = pb.z[p] + bf * pb.s[p];
     }
@@ -1076,8 +1148,9 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, Pressure
     if (pb.sc->done) return;
     size_t n = pb.sm.count();
     double acc = 0.0;
-    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * kRedThreads) {
+    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+        if (!group_active(pb, grp)) continue;  // uniform per block
+        const size_t p = (grp << 8) + threadIdx.x;
         int i, j, k;
         decode_slot(pb.sm, p, i, j, k);
         float zc = 0.0f;
@@ -1110,8 +1183,9 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm32(GridGeom g, Pressur
     double alpha = pb.sc->alpha;
     float naf = -pb.sc->alpha_f;
     double mx = 0.0;
-    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * kRedThreads) {
+    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+        if (!group_active(pb, grp)) continue;  // uniform per block
+        const size_t p = (grp << 8) + threadIdx.x;
         float sp = pb.s32[p];
         pb.x[p] += alpha * (double)sp;
         float r = pb.r32[p] + naf * pb.z32[p];
@@ -1144,8 +1218,9 @@ __global__ __launch_bounds__(kRedThreads) void k_true_residual(GridGeom g, Press
     if (pb.sc->done || !pb.sc->refresh) return;
     size_t n = pb.sm.count();
     double mx = 0.0;
-    for (size_t p = (size_t)blockIdx.x * kRedThreads + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * kRedThreads) {
+    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+        if (!group_active(pb, grp)) continue;  // uniform per block
+        const size_t p = (grp << 8) + threadIdx.x;
         int i, j, k;
         decode_slot(pb.sm, p, i, j, k);
         double r = 0.0;
@@ -1183,8 +1258,9 @@ __global__ void k_update_s32(PressureBuffers pb, int first) {
     if (pb.sc->done) return;
     size_t n = pb.sm.count();
     float bf = first ? 0.0f : pb.sc->beta_f;
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * blockDim.x) {
+    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+        if (!group_active(pb, grp)) continue;  // uniform per block
+        const size_t p = (grp << 8) + threadIdx.x;
         if (first) pb.s32[p] = pb.z32[p];
         else pb.s32[p] = pb.z32[p] + bf * pb.s32[p];
     }

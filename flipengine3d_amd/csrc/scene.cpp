@@ -110,7 +110,8 @@ void collision_box(int isize, int jsize, int ksize, double dx, float pos[3], dou
 
 void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     SlotMap sm = make_slot_map(g.I, g.J, g.K);
-    size_t ns = sm.count();
+    // Slot arrays carry the zero and sink groups after the last tile.
+    size_t ns = sm.alloc_count();
     int ntj = sm.ntj, ntk = sm.ntk;
     int npart = 2 * 1024 + ntj * ntk;
     for (int q = 0; q < kSlotArrays; q++)
@@ -147,6 +148,13 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     }
     pb.npartials = npart;
     pb.edge = edge.p;
+    const int groups_per_tile = sm.S / 4;
+    pb.mask_words = (groups_per_tile + 63) / 64;
+    if (mask.alloc((size_t)ntj * ntk * pb.mask_words) != hipSuccess)
+        throw StatusError{FLIPMI_ERR_OOM, "pressure scratch: out of device memory"};
+    // Every group counts as active until the first assembly marks them.
+    (void)launch_fill_u8((uint8_t *)mask.p, mask.n * sizeof(unsigned long long), 0xff, s);
+    pb.group_mask = mask.p;
     pb.epoch = 0;
     pb.progress = progress.p;
     pb.ntiles_j = ntj;
@@ -187,6 +195,7 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
     if (prof) prof->begin_stage(FLIPMI_STAGE_MIC0, &ev);
     const bool mixed = w.precision == FLIPMI_PCG_MIXED;
     if (mixed && !pb.r32) throw StatusError{FLIPMI_ERR_INVALID_ARG, "mixed precision arrays not allocated"};
+    hip_check(launch_group_mask(pb, s), "group mask");
     hip_check(launch_mic0_factor(w.g, w.liquid, pb, s), "mic0 factor");
     hip_check(launch_pcg_init(w.g, w.tol, w.maxiter, mixed, pb, s), "pcg init");
     if (prof) prof->end_stage(FLIPMI_STAGE_MIC0, ev);

@@ -162,7 +162,7 @@ struct PressureStore {
     static constexpr int kSlotArrays32 = 14;  // FP32 matrix, factor and vectors (mixed precision)
     DBuf<double> arr[kSlotArrays + 1];
     DBuf<float> arr32[kSlotArrays32];
-    DBuf<unsigned long long> edge;
+    DBuf<unsigned long long> edge, mask;
     DBuf<unsigned> progress, ticket;
     DBuf<int> irange;
     DBuf<PcgScalars> scal;
