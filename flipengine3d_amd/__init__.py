@@ -83,7 +83,8 @@ EXPORTED = [
     "flipmi_scene_create", "flipmi_scene_destroy", "flipmi_scene_load_particles",
     "flipmi_scene_initialize", "flipmi_scene_update", "flipmi_scene_num_particles",
     "flipmi_scene_get_particles", "flipmi_scene_get_grid", "flipmi_scene_stage_times",
-    "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf", "flipmi_p2g",
+    "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
+    "flipmi_liquid_sdf_postprocess", "flipmi_p2g",
     "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
     "flipmi_trilinear_velocity",
 ]
@@ -119,6 +120,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_set_kernel_timing.argtypes = [_vp, ctypes.c_uint]
     L.flipmi_domain_solid.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double] + [_vp] * 5
     L.flipmi_liquid_sdf.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_double, _vp, _vp]
+    L.flipmi_liquid_sdf_postprocess.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp]
     L.flipmi_p2g.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _c_i64, _c_double] + [_vp] * 6
     L.flipmi_extrapolate.argtypes = [_vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int]
     L.flipmi_pressure_solve.argtypes = [_vp, P(PressureArgs), P(PressureReport)]
@@ -198,6 +200,12 @@ class Context:
         self.check(self.L.flipmi_liquid_sdf(self.h, isize, jsize, ksize, dx, _ptr(pos), pos.shape[0], radius,
                                             _ptr(sp), _ptr(phi)))
         return phi
+
+    def liquid_sdf_postprocess(self, isize, jsize, ksize, dx, phi, solid_phi):
+        out = np.ascontiguousarray(phi, np.float32).copy()
+        sp = np.ascontiguousarray(solid_phi, np.float32)
+        self.check(self.L.flipmi_liquid_sdf_postprocess(self.h, isize, jsize, ksize, dx, _ptr(sp), _ptr(out)))
+        return out
 
     def p2g(self, isize, jsize, ksize, dx, pos, vel, radius):
         pos = np.ascontiguousarray(pos, np.float32)

@@ -47,6 +47,24 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_cpp_check(verbose: bool = False) -> str:
+    """Compiles tests/cpp/wrapper_check (the C++ engine-class wrappers of
+    include/flipmi_engine.hpp driven as the reference's callers drive them)
+    with g++ as C++14, the reference engine's language level."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "wrapper_check.cpp")
+    out = os.path.join(root, "tests", "cpp", "wrapper_check")
+    cmd = ["g++", "-std=c++14", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(root, "include"), src,
+           "-o", out + ".tmp", "-L", HERE, "-lflipmi", "-Wl,-rpath,$ORIGIN/../../flipengine3d_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -60,6 +78,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cpp_check(verbose)
     return LIB
 
 

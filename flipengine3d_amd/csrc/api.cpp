@@ -337,6 +337,22 @@ int flipmi_liquid_sdf(flipmi_ctx *ctx, int I, int J, int K, double dx, const flo
     });
 }
 
+int flipmi_liquid_sdf_postprocess(flipmi_ctx *ctx, int I, int J, int K, double dx,
+                                  const float *solid_phi_nodal, float *phi_inout) {
+    if (!ctx || !solid_phi_nodal || !phi_inout) return FLIPMI_ERR_INVALID_ARG;
+    return guarded(&ctx->c, [&] {
+        GridGeom g = make_geom(I, J, K, dx);
+        hipStream_t s = ctx->c.stream;
+        size_t nn = (size_t)(I + 1) * (J + 1) * (K + 1);
+        DBuf<float> solid, phi;
+        upload(solid, solid_phi_nodal, nn, s);
+        upload(phi, phi_inout, (size_t)I * J * K, s);
+        hipck(launch_liquid_postprocess(g, solid.p, phi.p, s), "liquid post-process");
+        download(phi_inout, phi, phi.n, s);
+        hipck(hipStreamSynchronize(s), "sync");
+    });
+}
+
 int flipmi_p2g(flipmi_ctx *ctx, int I, int J, int K, double dx, const float *pos, const float *vel,
                int64_t n, double radius, float *u, float *v, float *w, uint8_t *vu, uint8_t *vv,
                uint8_t *vw) {

@@ -59,6 +59,9 @@ hipError_t launch_gather_sorted(const int *order, const int *nsorted, int n,
 hipError_t launch_liquid_sdf(const GridGeom &g, const float *sx, const float *sy, const float *sz,
                              const int *offset, double radius, const float *solid_phi_nodal,
                              float *phi, hipStream_t s);
+// ParticleLevelSet::postProcessSignedDistanceField on an existing field.
+hipError_t launch_liquid_postprocess(const GridGeom &g, const float *solid_phi_nodal, float *phi,
+                                     hipStream_t s);
 
 // ---- P2G (k_p2g.hip) ----
 hipError_t launch_p2g(const GridGeom &g, int comp, const float *sx, const float *sy,

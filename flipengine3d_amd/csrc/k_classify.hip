@@ -35,6 +35,25 @@ __device__ __forceinline__ bool point_in_block(F3 p, float sr, float blockdx, do
     return bi >= lo.i && bi <= hi.i && bj >= lo.j && bj <= hi.j && bk >= lo.k && bk <= hi.k;
 }
 
+// ParticleLevelSet::postProcessSignedDistanceField (particlelevelset.cpp:
+// 172-196) for one cell: cells within half a cell of the surface whose centre
+// lies inside the solid become -dx/2 (MeshLevelSet::getDistanceAtCellCenter,
+// meshlevelset.cpp:152-162), and |phi| < 0.005 dx is pushed out to 0.005 dx.
+__device__ __forceinline__ float postprocess_phi(const GridGeom &g, const float *__restrict__ solid,
+                                                 int i, int j, int k, float v) {
+    Dims N{g.I + 1, g.J + 1, g.K + 1};
+    float eps = (float)(0.005 * g.dx);
+    if ((double)v < 0.5 * g.dx) {
+        float sc = 0.125f * (solid[N.at(i, j, k)] + solid[N.at(i + 1, j, k)] +
+                             solid[N.at(i, j + 1, k)] + solid[N.at(i + 1, j + 1, k)] +
+                             solid[N.at(i, j, k + 1)] + solid[N.at(i + 1, j, k + 1)] +
+                             solid[N.at(i, j + 1, k + 1)] + solid[N.at(i + 1, j + 1, k + 1)]);
+        if (sc < 0) v = (float)(-0.5f * g.dx);
+    }
+    if (fabsf(v) < eps) v = v > 0 ? eps : -eps;
+    return v;
+}
+
 __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
     GridGeom g, const float *__restrict__ sx, const float *__restrict__ sy,
     const float *__restrict__ sz, const int *__restrict__ offset, float radius,
@@ -104,29 +123,34 @@ __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
     }
     __syncthreads();
 
-    // Write back with the post-processing pass (particlelevelset.cpp:172-196).
-    float eps = (float)(0.005 * g.dx);
-    Dims N{g.I + 1, g.J + 1, g.K + 1};
+    // Write back, fused with the post-processing pass when a solid is given
+    // (calculateSignedDistanceField alone otherwise).
     for (int c = threadIdx.x; c < kBW * kBW * kBW; c += kSdfThreads) {
         int li = c % kBW, lj = (c / kBW) % kBW, lk = c / (kBW * kBW);
         int i = bi * kBW + li, j = bj * kBW + lj, k = bk * kBW + lk;
         if (i >= g.I || j >= g.J || k >= g.K) continue;
         float v = ord2f(phi_ord[c]);
-        if (!solid) {  // calculateSignedDistanceField without the post-process
-            phi_out[(size_t)i + (size_t)g.I * ((size_t)j + (size_t)g.J * k)] = v;
-            continue;
-        }
-        if ((double)v < 0.5 * g.dx) {
-            // MeshLevelSet::getDistanceAtCellCenter (meshlevelset.cpp:152-162).
-            float sc = 0.125f * (solid[N.at(i, j, k)] + solid[N.at(i + 1, j, k)] +
-                                 solid[N.at(i, j + 1, k)] + solid[N.at(i + 1, j + 1, k)] +
-                                 solid[N.at(i, j, k + 1)] + solid[N.at(i + 1, j, k + 1)] +
-                                 solid[N.at(i, j + 1, k + 1)] + solid[N.at(i + 1, j + 1, k + 1)]);
-            if (sc < 0) v = (float)(-0.5f * g.dx);
-        }
-        if (fabsf(v) < eps) v = v > 0 ? eps : -eps;
+        if (solid) v = postprocess_phi(g, solid, i, j, k, v);
         phi_out[(size_t)i + (size_t)g.I * ((size_t)j + (size_t)g.J * k)] = v;
     }
+}
+
+// The post-processing pass on its own, over an existing field (one thread
+// per cell).
+__global__ void k_liquid_postprocess(GridGeom g, const float *__restrict__ solid, float *phi) {
+    size_t n = (size_t)g.I * g.J * g.K;
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int i = (int)(c % (size_t)g.I), j = (int)((c / (size_t)g.I) % (size_t)g.J), k = (int)(c / ((size_t)g.I * g.J));
+    phi[c] = postprocess_phi(g, solid, i, j, k, phi[c]);
+}
+
+hipError_t launch_liquid_postprocess(const GridGeom &g, const float *solid_phi_nodal, float *phi,
+                                     hipStream_t s) {
+    size_t n = (size_t)g.I * g.J * g.K;
+    if (n == 0) return hipSuccess;
+    k_liquid_postprocess<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, solid_phi_nodal, phi);
+    return hipGetLastError();
 }
 
 hipError_t launch_liquid_sdf(const GridGeom &g, const float *sx, const float *sy, const float *sz,

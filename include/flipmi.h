@@ -179,6 +179,12 @@ int flipmi_liquid_sdf(flipmi_ctx *ctx, int isize, int jsize, int ksize, double d
                       const float *pos_xyz, int64_t n, double radius,
                       const float *solid_phi_nodal, float *phi_out);
 
+/* ParticleLevelSet::postProcessSignedDistanceField(MeshLevelSet&)
+ * (particlelevelset.cpp:172-196) on a field from flipmi_liquid_sdf computed
+ * without a solid, in place. */
+int flipmi_liquid_sdf_postprocess(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx,
+                                  const float *solid_phi_nodal, float *phi_inout);
+
 /* VelocityAdvector::advect(VelocityAdvectorParameters) (velocityadvector.h:75)
  * onto a cleared MAC grid: velocities and valid flags of U, V and W. */
 int flipmi_p2g(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx, const float *pos_xyz,

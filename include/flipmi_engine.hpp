@@ -1,0 +1,839 @@
+/*
+ * flipmi_engine.hpp -- C++ classes with the reference engine's signatures
+ * (jklae/FLIPEngine3D, src/engine), forwarding to the flipmi C ABI.
+ *
+ * A caller of the reference's FluidSimulation, PressureSolver,
+ * VelocityAdvector or ParticleAdvector switches by including this header and
+ * using namespace flipmi instead of the engine headers: class names, method
+ * signatures, parameter structs and data layouts are the reference's.  All
+ * computation runs on the HIP device; there is no CPU path.
+ *
+ * Errors: PressureSolver::solve returns false with a status text as the
+ * reference does; FluidSimulation throws std::runtime_error /
+ * std::domain_error where the reference throws; any device or argument error
+ * from the ABI becomes std::runtime_error carrying flipmi_last_error().
+ *
+ * Header-only; link with -lflipmi.
+ */
+#ifndef FLIPMI_ENGINE_HPP
+#define FLIPMI_ENGINE_HPP
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "flipmi.h"
+
+namespace flipmi {
+
+// ---------------------------------------------------------------------------
+// Device context shared by all objects of the process (the reference sizes its
+// worker threads process-wide, threadutils.cpp:43-52; here the shared resource
+// is the device).
+// ---------------------------------------------------------------------------
+
+namespace detail {
+inline int &device_index() {
+    static int d = 0;
+    return d;
+}
+inline std::mutex &device_mutex() {
+    static std::mutex m;
+    return m;
+}
+inline flipmi_ctx *&device_slot() {
+    static flipmi_ctx *c = nullptr;
+    return c;
+}
+inline flipmi_ctx *ctx() {
+    std::lock_guard<std::mutex> g(device_mutex());
+    flipmi_ctx *&c = device_slot();
+    if (!c) {
+        flipmi_ctx *made = nullptr;
+        int rc = flipmi_ctx_create(device_index(), &made);
+        if (rc != FLIPMI_OK)
+            throw std::runtime_error("flipmi: no usable HIP device (status " + std::to_string(rc) + ")");
+        c = made;
+    }
+    return c;
+}
+inline void check(int rc, const char *what) {
+    if (rc != FLIPMI_OK) throw std::runtime_error(std::string(what) + ": " + flipmi_last_error(ctx()));
+}
+}  // namespace detail
+
+// Selects the HIP device used by objects created afterwards (before the first
+// call that touches the device).
+inline void setDevice(int device) {
+    std::lock_guard<std::mutex> g(detail::device_mutex());
+    if (detail::device_slot()) throw std::runtime_error("flipmi: the device is already in use");
+    detail::device_index() = device;
+}
+
+// ---------------------------------------------------------------------------
+// Value types (vmath.h, array3d.h, gridindexkeymap / grid3d.h)
+// ---------------------------------------------------------------------------
+
+namespace vmath {
+// 3 x float, 12 bytes (vmath.h:76-97).
+struct vec3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    vec3() {}
+    vec3(float a, float b, float c) : x(a), y(b), z(c) {}
+    vec3(double a, double b, double c) : x((float)a), y((float)b), z((float)c) {}
+    float operator[](int q) const { return q == 0 ? x : (q == 1 ? y : z); }
+    float &operator[](int q) { return q == 0 ? x : (q == 1 ? y : z); }
+};
+}  // namespace vmath
+static_assert(sizeof(vmath::vec3) == 12, "vec3 must be 3 packed floats");
+
+struct GridIndex {
+    int i = 0, j = 0, k = 0;
+    GridIndex() {}
+    GridIndex(int ii, int jj, int kk) : i(ii), j(jj), k(kk) {}
+};
+
+// Dense 3-D array, i fastest (array3d.h:63-450); bool elements are one byte.
+template <class T>
+class Array3d {
+public:
+    int width = 0, height = 0, depth = 0;
+
+    Array3d() {}
+    Array3d(int i, int j, int k) : width(i), height(j), depth(k), _data(new T[count(i, j, k)]()) {}
+    Array3d(int i, int j, int k, T v) : Array3d(i, j, k) { fill(v); }
+    Array3d(const Array3d &o) : Array3d(o.width, o.height, o.depth) {
+        std::copy(o._data.get(), o._data.get() + o.getNumElements(), _data.get());
+    }
+    Array3d &operator=(const Array3d &o) {
+        if (this != &o) {
+            Array3d t(o);
+            swap(t);
+        }
+        return *this;
+    }
+    Array3d(Array3d &&) = default;
+    Array3d &operator=(Array3d &&) = default;
+
+    T get(int i, int j, int k) const { return _data[index(i, j, k)]; }
+    T get(GridIndex g) const { return get(g.i, g.j, g.k); }
+    T operator()(int i, int j, int k) const { return get(i, j, k); }
+    T operator()(GridIndex g) const { return get(g); }
+    void set(int i, int j, int k, T v) { _data[index(i, j, k)] = v; }
+    void set(GridIndex g, T v) { set(g.i, g.j, g.k, v); }
+    void add(int i, int j, int k, T v) { _data[index(i, j, k)] += v; }
+    void fill(T v) { std::fill(_data.get(), _data.get() + getNumElements(), v); }
+    bool isIndexInRange(int i, int j, int k) const {
+        return i >= 0 && j >= 0 && k >= 0 && i < width && j < height && k < depth;
+    }
+    int getNumElements() const { return width * height * depth; }
+    T *getRawArray() { return _data.get(); }
+    const T *getRawArray() const { return _data.get(); }
+
+private:
+    static size_t count(int i, int j, int k) { return (size_t)i * (size_t)j * (size_t)k; }
+    size_t index(int i, int j, int k) const {
+        return (size_t)i + (size_t)width * ((size_t)j + (size_t)height * (size_t)k);
+    }
+    void swap(Array3d &o) {
+        std::swap(width, o.width);
+        std::swap(height, o.height);
+        std::swap(depth, o.depth);
+        std::swap(_data, o._data);
+    }
+    std::unique_ptr<T[]> _data;
+};
+static_assert(sizeof(bool) == 1, "bool grids are passed as bytes");
+
+// MarkerParticle (markerparticle.h:30-44): position and velocity, 24 bytes.
+struct MarkerParticle {
+    vmath::vec3 position;
+    vmath::vec3 velocity;
+    MarkerParticle() {}
+    MarkerParticle(vmath::vec3 p) : position(p) {}
+    MarkerParticle(vmath::vec3 p, vmath::vec3 v) : position(p), velocity(v) {}
+    MarkerParticle(double x, double y, double z) : position(x, y, z) {}
+};
+static_assert(sizeof(MarkerParticle) == 24, "MarkerParticle must be 6 packed floats");
+
+// FragmentedVector<T> (fragmentedvector.h:32-328): the element interface the
+// callers use, stored contiguously.
+template <class T>
+class FragmentedVector {
+public:
+    void push_back(const T &v) { _v.push_back(v); }
+    void pop_back() { _v.pop_back(); }
+    size_t size() const { return _v.size(); }
+    bool empty() const { return _v.empty(); }
+    T &operator[](size_t q) { return _v[q]; }
+    const T &operator[](size_t q) const { return _v[q]; }
+    T &at(size_t q) { return _v.at(q); }
+    T &front() { return _v.front(); }
+    T &back() { return _v.back(); }
+    void clear() { _v.clear(); }
+    void reserve(size_t n) { _v.reserve(n); }
+    void shrink_to_fit() { _v.shrink_to_fit(); }
+    void insert(const std::vector<T> &vs) { _v.insert(_v.end(), vs.begin(), vs.end()); }
+    T *data() { return _v.data(); }
+    const T *data() const { return _v.data(); }
+
+private:
+    std::vector<T> _v;
+};
+
+// ---------------------------------------------------------------------------
+// Grids (macvelocityfield.h:36-164, pressuresolver.h:48-60)
+// ---------------------------------------------------------------------------
+
+struct ValidVelocityComponentGrid {
+    Array3d<bool> validU, validV, validW;
+    ValidVelocityComponentGrid() {}
+    ValidVelocityComponentGrid(int i, int j, int k)
+        : validU(i + 1, j, k, false), validV(i, j + 1, k, false), validW(i, j, k + 1, false) {}
+    void reset() {
+        validU.fill(false);
+        validV.fill(false);
+        validW.fill(false);
+    }
+};
+
+struct WeightGrid {
+    Array3d<float> center, U, V, W;
+    WeightGrid() {}
+    WeightGrid(int i, int j, int k)
+        : center(i, j, k, 0.0f), U(i + 1, j, k, 0.0f), V(i, j + 1, k, 0.0f), W(i, j, k + 1, 0.0f) {}
+};
+
+class MACVelocityField {
+public:
+    MACVelocityField() {}
+    MACVelocityField(int isize, int jsize, int ksize, double dx)
+        : _isize(isize), _jsize(jsize), _ksize(ksize), _dx(dx), _u(isize + 1, jsize, ksize, 0.0f),
+          _v(isize, jsize + 1, ksize, 0.0f), _w(isize, jsize, ksize + 1, 0.0f) {}
+
+    void getGridDimensions(int *i, int *j, int *k) {
+        *i = _isize;
+        *j = _jsize;
+        *k = _ksize;
+    }
+    double getGridCellSize() { return _dx; }
+
+    float U(int i, int j, int k) { return _u(i, j, k); }
+    float V(int i, int j, int k) { return _v(i, j, k); }
+    float W(int i, int j, int k) { return _w(i, j, k); }
+    float U(GridIndex g) { return _u(g); }
+    float V(GridIndex g) { return _v(g); }
+    float W(GridIndex g) { return _w(g); }
+    void set(MACVelocityField &o) { *this = o; }
+    void setU(int i, int j, int k, double val) { _u.set(i, j, k, (float)val); }
+    void setV(int i, int j, int k, double val) { _v.set(i, j, k, (float)val); }
+    void setW(int i, int j, int k, double val) { _w.set(i, j, k, (float)val); }
+    void setU(GridIndex g, double val) { _u.set(g, (float)val); }
+    void setV(GridIndex g, double val) { _v.set(g, (float)val); }
+    void setW(GridIndex g, double val) { _w.set(g, (float)val); }
+    void setU(Array3d<float> &g) { _u = g; }
+    void setV(Array3d<float> &g) { _v = g; }
+    void setW(Array3d<float> &g) { _w = g; }
+    // addU(double) stores through float (macvelocityfield.cpp:247-293).
+    void addU(int i, int j, int k, double val) { _u.set(i, j, k, (float)(_u(i, j, k) + val)); }
+    void addV(int i, int j, int k, double val) { _v.set(i, j, k, (float)(_v(i, j, k) + val)); }
+    void addW(int i, int j, int k, double val) { _w.set(i, j, k, (float)(_w(i, j, k) + val)); }
+    Array3d<float> *getArray3dU() { return &_u; }
+    Array3d<float> *getArray3dV() { return &_v; }
+    Array3d<float> *getArray3dW() { return &_w; }
+    float *getRawArrayU() { return _u.getRawArray(); }
+    float *getRawArrayV() { return _v.getRawArray(); }
+    float *getRawArrayW() { return _w.getRawArray(); }
+    void clear() {
+        clearU();
+        clearV();
+        clearW();
+    }
+    void clearU() { _u.fill(0.0f); }
+    void clearV() { _v.fill(0.0f); }
+    void clearW() { _w.fill(0.0f); }
+    bool isIndexInRangeU(int i, int j, int k) { return _u.isIndexInRange(i, j, k); }
+    bool isIndexInRangeV(int i, int j, int k) { return _v.isIndexInRange(i, j, k); }
+    bool isIndexInRangeW(int i, int j, int k) { return _w.isIndexInRange(i, j, k); }
+
+private:
+    int _isize = 0, _jsize = 0, _ksize = 0;
+    double _dx = 0.0;
+    Array3d<float> _u, _v, _w;
+};
+
+// Solid signed distance, nodal (I+1)(J+1)(K+1), with the solid face
+// velocities (meshlevelset.h:94-137).
+class MeshLevelSet {
+public:
+    MeshLevelSet() {}
+    MeshLevelSet(int isize, int jsize, int ksize, double dx)
+        : _isize(isize), _jsize(jsize), _ksize(ksize), _dx(dx), _phi(isize + 1, jsize + 1, ksize + 1, 0.0f),
+          _vu(isize + 1, jsize, ksize, 0.0f), _vv(isize, jsize + 1, ksize, 0.0f), _vw(isize, jsize, ksize + 1, 0.0f) {}
+
+    float operator()(int i, int j, int k) { return _phi(i, j, k); }
+    float operator()(GridIndex g) { return _phi(g); }
+    float get(int i, int j, int k) { return _phi(i, j, k); }
+    float get(GridIndex g) { return _phi(g); }
+    void set(int i, int j, int k, float d) { _phi.set(i, j, k, d); }
+    void set(GridIndex g, float d) { _phi.set(g, d); }
+    // meshlevelset.cpp:152-162
+    float getDistanceAtCellCenter(int i, int j, int k) {
+        return 0.125f * (_phi(i, j, k) + _phi(i + 1, j, k) + _phi(i, j + 1, k) + _phi(i + 1, j + 1, k) +
+                         _phi(i, j, k + 1) + _phi(i + 1, j, k + 1) + _phi(i, j + 1, k + 1) +
+                         _phi(i + 1, j + 1, k + 1));
+    }
+    float getFaceVelocityU(int i, int j, int k) { return _vu(i, j, k); }
+    float getFaceVelocityV(int i, int j, int k) { return _vv(i, j, k); }
+    float getFaceVelocityW(int i, int j, int k) { return _vw(i, j, k); }
+    void setFaceVelocityU(int i, int j, int k, float v) { _vu.set(i, j, k, v); }
+    void setFaceVelocityV(int i, int j, int k, float v) { _vv.set(i, j, k, v); }
+    void setFaceVelocityW(int i, int j, int k, float v) { _vw.set(i, j, k, v); }
+
+    // The domain's wall box as FluidSimulation builds it at initialisation
+    // (_addStaticObjectsToSDF and _updateWeightGrid, fluidsimulation.cpp:
+    // 2918-2930, 3646-3730), computed on the device; weights may be null.
+    void calculateDomainBoundary(WeightGrid *weights = nullptr) {
+        detail::check(flipmi_domain_solid(detail::ctx(), _isize, _jsize, _ksize, _dx, _phi.getRawArray(),
+                                          weights ? weights->U.getRawArray() : nullptr,
+                                          weights ? weights->V.getRawArray() : nullptr,
+                                          weights ? weights->W.getRawArray() : nullptr,
+                                          weights ? weights->center.getRawArray() : nullptr),
+                      "MeshLevelSet::calculateDomainBoundary");
+    }
+
+    Array3d<float> *getPhiArray3d() { return &_phi; }
+    float *getRawFaceVelocityU() { return _vu.getRawArray(); }
+    float *getRawFaceVelocityV() { return _vv.getRawArray(); }
+    float *getRawFaceVelocityW() { return _vw.getRawArray(); }
+    void getGridDimensions(int *i, int *j, int *k) {
+        *i = _isize;
+        *j = _jsize;
+        *k = _ksize;
+    }
+    double getCellSize() { return _dx; }
+
+private:
+    int _isize = 0, _jsize = 0, _ksize = 0;
+    double _dx = 0.0;
+    Array3d<float> _phi, _vu, _vv, _vw;
+};
+
+// Liquid signed distance, cell-centred (particlelevelset.h:47-69).
+class ParticleLevelSet {
+public:
+    ParticleLevelSet() {}
+    ParticleLevelSet(int i, int j, int k, double dx) : _isize(i), _jsize(j), _ksize(k), _dx(dx), _phi(i, j, k, 0.0f) {}
+
+    float operator()(int i, int j, int k) { return _phi(i, j, k); }
+    float operator()(GridIndex g) { return _phi(g); }
+    float get(int i, int j, int k) { return _phi(i, j, k); }
+    float get(GridIndex g) { return _phi(g); }
+
+    // particlelevelset.cpp:161-170
+    void calculateSignedDistanceField(FragmentedVector<MarkerParticle> &particles, double radius) {
+        std::vector<float> pos(3 * particles.size());
+        for (size_t q = 0; q < particles.size(); q++) {
+            pos[3 * q] = particles[q].position.x;
+            pos[3 * q + 1] = particles[q].position.y;
+            pos[3 * q + 2] = particles[q].position.z;
+        }
+        detail::check(flipmi_liquid_sdf(detail::ctx(), _isize, _jsize, _ksize, _dx, pos.data(),
+                                        (int64_t)particles.size(), radius, nullptr, _phi.getRawArray()),
+                      "ParticleLevelSet::calculateSignedDistanceField");
+    }
+    // particlelevelset.cpp:172-196
+    void postProcessSignedDistanceField(MeshLevelSet &solidPhi) {
+        detail::check(flipmi_liquid_sdf_postprocess(detail::ctx(), _isize, _jsize, _ksize, _dx,
+                                                    solidPhi.getPhiArray3d()->getRawArray(), _phi.getRawArray()),
+                      "ParticleLevelSet::postProcessSignedDistanceField");
+    }
+
+    Array3d<float> *getPhiArray3d() { return &_phi; }
+
+private:
+    int _isize = 0, _jsize = 0, _ksize = 0;
+    double _dx = 0.0;
+    Array3d<float> _phi;
+};
+
+// ---------------------------------------------------------------------------
+// PressureSolver (pressuresolver.h:63-92)
+// ---------------------------------------------------------------------------
+
+struct PressureSolverParameters {
+    double cellwidth;
+    double deltaTime;
+    double tolerance;
+    double acceptableTolerance;
+    int maxIterations;
+
+    MACVelocityField *velocityField;
+    ValidVelocityComponentGrid *validVelocities;
+    ParticleLevelSet *liquidSDF;
+    MeshLevelSet *solidSDF;
+    WeightGrid *weightGrid;
+
+    bool isSurfaceTensionEnabled = false;
+    double surfaceTensionConstant;
+    Array3d<float> *curvatureGrid;
+
+    // Not in the reference: FLIPMI_PCG_FP64 keeps the reference's arithmetic.
+    int precision = FLIPMI_PCG_FP64;
+};
+
+class PressureSolver {
+public:
+    PressureSolver() {}
+
+    // pressuresolver.cpp:44-77: projects velocityField in place and marks
+    // the projected faces valid; returns false (velocity untouched) when the
+    // solve fails.  Surface tension is not available on this path.
+    bool solve(PressureSolverParameters params) {
+        if (params.isSurfaceTensionEnabled) {
+            _status = "***Pressure Solver FAILED\nsurface tension is not supported";
+            return false;
+        }
+        int I, J, K;
+        params.velocityField->getGridDimensions(&I, &J, &K);
+        flipmi_pressure_args a;
+        std::memset(&a, 0, sizeof(a));
+        a.isize = I;
+        a.jsize = J;
+        a.ksize = K;
+        a.dx = params.cellwidth;
+        a.dt = params.deltaTime;
+        a.tolerance = params.tolerance;
+        a.acceptable_tolerance = params.acceptableTolerance;
+        a.max_iterations = params.maxIterations;
+        a.precision = params.precision;
+        a.liquid_phi = params.liquidSDF->getPhiArray3d()->getRawArray();
+        a.weight_u = params.weightGrid->U.getRawArray();
+        a.weight_v = params.weightGrid->V.getRawArray();
+        a.weight_w = params.weightGrid->W.getRawArray();
+        a.weight_c = params.weightGrid->center.getRawArray();
+        if (params.solidSDF) {
+            a.solid_u = params.solidSDF->getRawFaceVelocityU();
+            a.solid_v = params.solidSDF->getRawFaceVelocityV();
+            a.solid_w = params.solidSDF->getRawFaceVelocityW();
+        }
+        a.u = params.velocityField->getRawArrayU();
+        a.v = params.velocityField->getRawArrayV();
+        a.w = params.velocityField->getRawArrayW();
+        a.valid_u = reinterpret_cast<uint8_t *>(params.validVelocities->validU.getRawArray());
+        a.valid_v = reinterpret_cast<uint8_t *>(params.validVelocities->validV.getRawArray());
+        a.valid_w = reinterpret_cast<uint8_t *>(params.validVelocities->validW.getRawArray());
+        flipmi_pressure_report rep;
+        std::memset(&rep, 0, sizeof(rep));
+        int rc = flipmi_pressure_solve(detail::ctx(), &a, &rep);
+        if (rc != FLIPMI_OK && rc != FLIPMI_ERR_NOT_CONVERGED) detail::check(rc, "PressureSolver::solve");
+        // Status text of PressureSolver::_solveLinearSystem (pressuresolver.cpp:810-840).
+        std::ostringstream ss;
+        if (rc != FLIPMI_OK) ss << "***Pressure Solver FAILED\n";
+        ss << "Pressure Solver Iterations: " << rep.iterations << "\nEstimated Error: " << rep.residual;
+        if (rc == FLIPMI_OK && rep.iterations == 0) ss.str("Pressure Solver Iterations: 0\nEstimated Error: 0.0");
+        _status = ss.str();
+        return rc == FLIPMI_OK;
+    }
+    std::string getSolverStatus() { return _status; }
+
+private:
+    std::string _status;
+};
+
+// ---------------------------------------------------------------------------
+// VelocityAdvector (velocityadvector.h:62-75)
+// ---------------------------------------------------------------------------
+
+struct VelocityAdvectorParameters {
+    FragmentedVector<MarkerParticle> *particles;
+    MACVelocityField *vfield;
+    ValidVelocityComponentGrid *validVelocities;
+    double particleRadius = 1.0;
+};
+
+class VelocityAdvector {
+public:
+    VelocityAdvector() {}
+
+    // Splats particle velocities onto the faces of vfield and sets
+    // validVelocities.  Every face is written: faces no particle reaches get
+    // 0 and invalid, which is what the reference leaves on the cleared grids
+    // its caller passes (fluidsimulation.cpp:3259-3260).
+    void advect(VelocityAdvectorParameters params) {
+        int I, J, K;
+        params.vfield->getGridDimensions(&I, &J, &K);
+        FragmentedVector<MarkerParticle> &ps = *params.particles;
+        size_t n = ps.size();
+        std::vector<float> pos(3 * n), vel(3 * n);
+        for (size_t q = 0; q < n; q++) {
+            for (int c = 0; c < 3; c++) {
+                pos[3 * q + c] = ps[q].position[c];
+                vel[3 * q + c] = ps[q].velocity[c];
+            }
+        }
+        detail::check(
+            flipmi_p2g(detail::ctx(), I, J, K, params.vfield->getGridCellSize(), pos.data(), vel.data(), (int64_t)n,
+                       params.particleRadius, params.vfield->getRawArrayU(), params.vfield->getRawArrayV(),
+                       params.vfield->getRawArrayW(),
+                       reinterpret_cast<uint8_t *>(params.validVelocities->validU.getRawArray()),
+                       reinterpret_cast<uint8_t *>(params.validVelocities->validV.getRawArray()),
+                       reinterpret_cast<uint8_t *>(params.validVelocities->validW.getRawArray())),
+            "VelocityAdvector::advect");
+    }
+};
+
+// ---------------------------------------------------------------------------
+// ParticleAdvector (particleadvector.h:37-79)
+// ---------------------------------------------------------------------------
+
+class ParticleAdvector {
+public:
+    ParticleAdvector() {}
+
+    // The reference probes OpenCL here; this checks for a HIP device.
+    bool initialize() {
+        try {
+            detail::ctx();
+            _initialized = true;
+        } catch (const std::exception &e) {
+            _error = e.what();
+            _initialized = false;
+        }
+        return _initialized;
+    }
+    bool isInitialized() { return _initialized; }
+    std::string getInitializationErrorMessage() { return _error; }
+    std::string getDeviceInfo() { return "HIP device " + std::to_string(detail::device_index()); }
+    std::string getKernelInfo() { return "flipmi advect_particles"; }
+    // The OpenCL switches of the reference: the device path is the only one.
+    void disableOpenCL() {}
+    void enableOpenCL() {}
+    bool isOpenCLEnabled() { return false; }
+    int getKernelWorkLoadSize() { return _workload; }
+    void setKernelWorkLoadSize(int n) { _workload = n; }
+
+    void advectParticlesRK4(std::vector<vmath::vec3> &particles, MACVelocityField *vfield, double dt,
+                            std::vector<vmath::vec3> &output) {
+        _advect(particles, vfield, dt, 4, output);
+    }
+    void advectParticlesRK3(std::vector<vmath::vec3> &particles, MACVelocityField *vfield, double dt,
+                            std::vector<vmath::vec3> &output) {
+        _advect(particles, vfield, dt, 3, output);
+    }
+    void advectParticlesRK2(std::vector<vmath::vec3> &particles, MACVelocityField *vfield, double dt,
+                            std::vector<vmath::vec3> &output) {
+        _advect(particles, vfield, dt, 2, output);
+    }
+    void advectParticlesRK1(std::vector<vmath::vec3> &particles, MACVelocityField *vfield, double dt,
+                            std::vector<vmath::vec3> &output) {
+        _advect(particles, vfield, dt, 1, output);
+    }
+    void trilinearInterpolate(std::vector<vmath::vec3> &particles, MACVelocityField *vfield,
+                              std::vector<vmath::vec3> &output) {
+        int I, J, K;
+        vfield->getGridDimensions(&I, &J, &K);
+        output.resize(particles.size());
+        if (particles.empty()) return;
+        detail::check(flipmi_trilinear_velocity(detail::ctx(), I, J, K, vfield->getGridCellSize(),
+                                                vfield->getRawArrayU(), vfield->getRawArrayV(),
+                                                vfield->getRawArrayW(), &particles[0].x, (int64_t)particles.size(),
+                                                &output[0].x),
+                      "ParticleAdvector::trilinearInterpolate");
+    }
+    // Overwrites particles with the sampled velocities.
+    void trilinearInterpolate(std::vector<vmath::vec3> &particles, MACVelocityField *vfield) {
+        std::vector<vmath::vec3> out;
+        trilinearInterpolate(particles, vfield, out);
+        particles.swap(out);
+    }
+
+private:
+    void _advect(std::vector<vmath::vec3> &particles, MACVelocityField *vfield, double dt, int order,
+                 std::vector<vmath::vec3> &output) {
+        int I, J, K;
+        vfield->getGridDimensions(&I, &J, &K);
+        output.resize(particles.size());
+        if (particles.empty()) return;
+        detail::check(flipmi_advect_particles(detail::ctx(), I, J, K, vfield->getGridCellSize(),
+                                              vfield->getRawArrayU(), vfield->getRawArrayV(),
+                                              vfield->getRawArrayW(), &particles[0].x, (int64_t)particles.size(),
+                                              dt, order, &output[0].x),
+                      "ParticleAdvector::advectParticles");
+    }
+    bool _initialized = false;
+    std::string _error;
+    int _workload = 0;
+};
+
+// ---------------------------------------------------------------------------
+// FluidSimulation (fluidsimulation.h:60-153, 304-1126)
+// ---------------------------------------------------------------------------
+
+struct FluidSimulationMeshStats {
+    int enabled = 0;
+    int vertices = -1;
+    int triangles = -1;
+    unsigned int bytes = 0;
+};
+
+struct FluidSimulationTimingStats {
+    double total = 0.0;
+    double mesh = 0.0;
+    double advection = 0.0;
+    double particles = 0.0;
+    double pressure = 0.0;
+    double diffuse = 0.0;
+    double viscosity = 0.0;
+    double objects = 0.0;
+};
+
+struct FluidSimulationFrameStats {
+    int frame = 0;
+    int substeps = 0;
+    double deltaTime = 0.0;
+    int fluidParticles = 0;
+    int diffuseParticles = 0;
+    FluidSimulationMeshStats surface, preview, surfaceblur, foam, bubble, spray, dust, foamblur, bubbleblur,
+        sprayblur, dustblur, particles, obstacle, forcefield;
+    FluidSimulationTimingStats timing;
+};
+
+struct FluidSimulationMarkerParticleData {
+    int size = 0;
+    char *positions;
+    char *velocities;
+};
+
+class FluidSimulation {
+public:
+    FluidSimulation(int isize, int jsize, int ksize, double dx) : _isize(isize), _jsize(jsize), _ksize(ksize), _dx(dx) {
+        flipmi_scene_desc_default(&_desc);
+        _desc.isize = isize;
+        _desc.jsize = jsize;
+        _desc.ksize = ksize;
+        _desc.dx = dx;
+    }
+    FluidSimulation(const FluidSimulation &) = delete;
+    FluidSimulation &operator=(const FluidSimulation &) = delete;
+    virtual ~FluidSimulation() {
+        if (_scene) flipmi_scene_destroy(_scene);
+    }
+
+    // fluidsimulation.cpp:82-88: builds the device scene from the settings
+    // and the particles loaded so far.
+    virtual void initialize() {
+        if (_initialized) return;
+        flipmi_ctx *c = detail::ctx();
+        _desc.max_particles = _maxParticles;
+        detail::check(flipmi_scene_create(c, &_desc, &_scene), "FluidSimulation::initialize");
+        int64_t n = (int64_t)(_loadPos.size() / 3);
+        if (n > 0)
+            detail::check(flipmi_scene_load_particles(_scene, _loadPos.data(), _loadVel.data(), n),
+                          "FluidSimulation::loadMarkerParticleData");
+        _loadPos.clear();
+        _loadVel.clear();
+        detail::check(flipmi_scene_initialize(_scene), "FluidSimulation::initialize");
+        _initialized = true;
+    }
+    bool isInitialized() { return _initialized; }
+
+    // fluidsimulation.cpp:5755-5846
+    void update(double dt) {
+        if (!_initialized) throw std::runtime_error("Error: FluidSimulation must be initialized before update.");
+        if (dt < 0.0) throw std::domain_error("Error: delta time must be greater than or equal to 0.");
+        auto t0 = std::chrono::steady_clock::now();
+        flipmi_frame_stats st;
+        detail::check(flipmi_scene_update(_scene, dt, &st), "FluidSimulation::update");
+        auto t1 = std::chrono::steady_clock::now();
+        _stats = FluidSimulationFrameStats();
+        _stats.frame = _currentFrame;
+        _stats.substeps = st.substeps;
+        _stats.deltaTime = dt;
+        _stats.fluidParticles = (int)st.num_particles;
+        _stats.timing.total = std::chrono::duration<double>(t1 - t0).count();
+        _lastPcgIterations = st.pcg_iterations;
+        _lastPcgResidual = st.pcg_residual;
+        _lastPressureOk = st.pressure_ok != 0;
+        _velocityStale = true;
+        _currentFrame++;
+    }
+
+    int getCurrentFrame() { return _currentFrame; }
+    double getCellSize() { return _dx; }
+    void getGridDimensions(int *i, int *j, int *k) {
+        *i = _isize;
+        *j = _jsize;
+        *k = _ksize;
+    }
+    void getSimulationDimensions(double *width, double *height, double *depth) {
+        *width = _isize * _dx;
+        *height = _jsize * _dx;
+        *depth = _ksize * _dx;
+    }
+    double getSimulationWidth() { return _isize * _dx; }
+    double getSimulationHeight() { return _jsize * _dx; }
+    double getSimulationDepth() { return _ksize * _dx; }
+
+    // Settings: fixed when initialize() builds the device scene.
+    void setMaxThreadCount(int n) {
+        if (n < 1) throw std::domain_error("Error: thread count must be greater than or equal to 1.");
+        _maxThreads = n;  // host threads play no part in the device step
+    }
+    int getMaxThreadCount() { return _maxThreads; }
+    void addBodyForce(double fx, double fy, double fz) {
+        _settable("addBodyForce");
+        _desc.body_force[0] += fx;
+        _desc.body_force[1] += fy;
+        _desc.body_force[2] += fz;
+    }
+    void addBodyForce(vmath::vec3 f) { addBodyForce(f.x, f.y, f.z); }
+    void setBoundaryFriction(double f) {
+        if (f < 0.0 || f > 1.0) throw std::domain_error("Error: boundary friction must be in range [0.0, 1.0].");
+        if (f != 0.0) throw std::domain_error("flipmi: only frictionless boundaries (0.0) are available");
+    }
+    double getBoundaryFriction() { return 0.0; }
+    void setCFLConditionNumber(int n) {
+        if (n < 1) throw std::domain_error("Error: CFL must be greater than or equal to 1.");
+        _settable("setCFLConditionNumber");
+        _desc.cfl = n;
+    }
+    int getCFLConditionNumber() { return (int)_desc.cfl; }
+    void setMinTimeStepsPerFrame(int n) {
+        if (n < 1) throw std::domain_error("Error: min step count must be greater than or equal to 1.");
+        _settable("setMinTimeStepsPerFrame");
+        _desc.min_substeps = n;
+    }
+    void setMaxTimeStepsPerFrame(int n) {
+        if (n < 1) throw std::domain_error("Error: max step count must be greater than or equal to 1.");
+        _settable("setMaxTimeStepsPerFrame");
+        _desc.max_substeps = n;
+    }
+    void setPICFLIPRatio(double r) {
+        if (r < 0.0 || r > 1.0) throw std::domain_error("Error: PICFLIP ratio must be in range [0.0, 1.0].");
+        _settable("setPICFLIPRatio");
+        _desc.pic_ratio = r;
+    }
+    double getPICFLIPRatio() { return _desc.pic_ratio; }
+    // No surface mesh is produced and nothing is printed on this path.
+    void disableSurfaceReconstruction() {}
+    bool isSurfaceReconstructionEnabled() { return false; }
+    void disableConsoleOutput() {}
+    void enableConsoleOutput() {}
+
+    // Not in the reference: PCG arithmetic (FLIPMI_PCG_FP64 = the reference's).
+    void setPressureSolverPrecision(int precision) {
+        _settable("setPressureSolverPrecision");
+        _desc.pcg_precision = precision;
+    }
+    // Not in the reference: particle capacity of the device scene (0 = size
+    // of the loaded set).
+    void setMaxMarkerParticles(int64_t n) {
+        _settable("setMaxMarkerParticles");
+        _maxParticles = n;
+    }
+
+    // fluidsimulation.cpp:2488-2509: particles queue up until initialize().
+    void loadMarkerParticleData(FluidSimulationMarkerParticleData data) {
+        if (data.size == 0) return;
+        if (_initialized) throw std::runtime_error("flipmi: marker particles must be loaded before initialize()");
+        const float *p = reinterpret_cast<const float *>(data.positions);
+        const float *v = reinterpret_cast<const float *>(data.velocities);
+        _loadPos.insert(_loadPos.end(), p, p + 3 * (size_t)data.size);
+        if (v) _loadVel.insert(_loadVel.end(), v, v + 3 * (size_t)data.size);
+        else _loadVel.insert(_loadVel.end(), 3 * (size_t)data.size, 0.0f);
+    }
+
+    unsigned int getNumMarkerParticles() {
+        if (!_initialized) return (unsigned int)(_loadPos.size() / 3);
+        int64_t n = 0;
+        detail::check(flipmi_scene_num_particles(_scene, &n), "FluidSimulation::getNumMarkerParticles");
+        return (unsigned int)n;
+    }
+    std::vector<MarkerParticle> getMarkerParticles() {
+        std::vector<vmath::vec3> p = getMarkerParticlePositions(), v = getMarkerParticleVelocities();
+        std::vector<MarkerParticle> out(p.size());
+        for (size_t q = 0; q < p.size(); q++) out[q] = MarkerParticle(p[q], v[q]);
+        return out;
+    }
+    std::vector<vmath::vec3> getMarkerParticlePositions() {
+        std::vector<vmath::vec3> out(getNumMarkerParticles());
+        if (!out.empty()) getMarkerParticlePositionData(reinterpret_cast<char *>(out.data()));
+        return out;
+    }
+    std::vector<vmath::vec3> getMarkerParticleVelocities() {
+        std::vector<vmath::vec3> out(getNumMarkerParticles());
+        if (!out.empty()) getMarkerParticleVelocityData(reinterpret_cast<char *>(out.data()));
+        return out;
+    }
+    // fluidsimulation.cpp:2408-2420: vec3 blobs, 12 bytes per particle.
+    void getMarkerParticlePositionData(char *data) {
+        _requireScene();
+        detail::check(flipmi_scene_get_particles(_scene, reinterpret_cast<float *>(data), nullptr),
+                      "FluidSimulation::getMarkerParticlePositionData");
+    }
+    void getMarkerParticleVelocityData(char *data) {
+        _requireScene();
+        detail::check(flipmi_scene_get_particles(_scene, nullptr, reinterpret_cast<float *>(data)),
+                      "FluidSimulation::getMarkerParticleVelocityData");
+    }
+    unsigned int getMarkerParticlePositionDataSize() { return getNumMarkerParticles() * 3 * sizeof(float); }
+    unsigned int getMarkerParticleVelocityDataSize() { return getNumMarkerParticles() * 3 * sizeof(float); }
+
+    // fluidsimulation.cpp:2242: a host copy of the device MAC grid, refreshed
+    // on each call after an update.
+    MACVelocityField *getVelocityField() {
+        _requireScene();
+        if (!_mac || _velocityStale) {
+            if (!_mac) _mac.reset(new MACVelocityField(_isize, _jsize, _ksize, _dx));
+            detail::check(flipmi_scene_get_grid(_scene, FLIPMI_GRID_U, _mac->getRawArrayU()), "getVelocityField");
+            detail::check(flipmi_scene_get_grid(_scene, FLIPMI_GRID_V, _mac->getRawArrayV()), "getVelocityField");
+            detail::check(flipmi_scene_get_grid(_scene, FLIPMI_GRID_W, _mac->getRawArrayW()), "getVelocityField");
+            _velocityStale = false;
+        }
+        return _mac.get();
+    }
+
+    FluidSimulationFrameStats getFrameStatsData() { return _stats; }
+
+    // Not in the reference: the last substep's pressure solve.
+    int getLastPressureSolverIterations() { return _lastPcgIterations; }
+    double getLastPressureSolverError() { return _lastPcgResidual; }
+    bool getLastPressureSolveSucceeded() { return _lastPressureOk; }
+
+private:
+    void _settable(const char *what) {
+        if (_initialized) throw std::runtime_error(std::string("flipmi: ") + what + " must be called before initialize()");
+    }
+    void _requireScene() {
+        if (!_initialized) throw std::runtime_error("Error: FluidSimulation must be initialized first.");
+    }
+
+    int _isize, _jsize, _ksize;
+    double _dx;
+    flipmi_scene_desc _desc;
+    flipmi_scene *_scene = nullptr;
+    bool _initialized = false;
+    int _currentFrame = 0;
+    int _maxThreads = 1;
+    int64_t _maxParticles = 0;
+    std::vector<float> _loadPos, _loadVel;
+    FluidSimulationFrameStats _stats;
+    std::unique_ptr<MACVelocityField> _mac;
+    bool _velocityStale = true;
+    int _lastPcgIterations = 0;
+    double _lastPcgResidual = 0.0;
+    bool _lastPressureOk = true;
+};
+
+}  // namespace flipmi
+
+#endif  // FLIPMI_ENGINE_HPP

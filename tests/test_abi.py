@@ -81,3 +81,20 @@ def test_missing_library_raises(tmp_path):
             fa.load_library(str(tmp_path / "libflipmi.so"))
     finally:
         fa._lib = saved
+
+
+def test_cpp_engine_classes_compile_as_cpp14(tmp_path):
+    # include/flipmi_engine.hpp (the reference's class signatures over the
+    # C ABI) builds as C++14, the reference engine's language level, and
+    # calls only entry points the C header declares.
+    import subprocess
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "flipmi_engine.hpp"\n'
+                   "int main() { flipmi::PressureSolver p; flipmi::VelocityAdvector v; flipmi::ParticleAdvector a;\n"
+                   "  flipmi::FluidSimulation s(8, 8, 8, 0.125); return p.getSolverStatus().size(); }\n")
+    r = subprocess.run(["g++", "-std=c++14", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    text = open(os.path.join(ROOT, "include", "flipmi_engine.hpp")).read()
+    used = set(re.findall(r"\b(flipmi_[a-z_0-9]+)\s*\(", text))
+    assert used and used <= set(declared()), used - set(declared())

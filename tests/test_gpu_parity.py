@@ -39,6 +39,11 @@ def test_liquid_sdf_labels_bitexact(gpu, oracle, dims):
         po = oracle.liquid_sdf(I, J, K, dx, s.positions, radius(dx), sp)
         assert bits_equal(pg, po), f"ndiff={ndiff(pg, po)}"
         assert np.array_equal(pg < 0, po < 0)
+    # calculateSignedDistanceField, then postProcessSignedDistanceField as a
+    # separate call: the same bits as the fused pass.
+    raw = gpu.liquid_sdf(I, J, K, dx, s.positions, radius(dx))
+    post = gpu.liquid_sdf_postprocess(I, J, K, dx, raw, solid)
+    assert bits_equal(post, oracle.liquid_sdf(I, J, K, dx, s.positions, radius(dx), solid))
 
 
 def test_liquid_sdf_scattered_points(gpu, oracle):
