@@ -226,21 +226,27 @@ class Context:
         return g
 
     def pressure_solve(self, isize, jsize, ksize, dx, dt, liquid_phi, weights, mac, valid, tol=1e-9,
-                       maxiter=1000, precision=PCG_FP64, acceptable=1.0):
+                       maxiter=1000, precision=PCG_FP64, acceptable=1.0, solid_vel=None):
+        """solid_vel: optional (U, V, W) solid face velocities; returned in
+        'solid_vel' after the pocket conditioning."""
         lp = np.ascontiguousarray(liquid_phi, np.float32)
+        sv = (None, None, None) if solid_vel is None else [np.ascontiguousarray(a, np.float32).copy()
+                                                           for a in solid_vel]
         wu, wv, ww, wc = [np.ascontiguousarray(a, np.float32) for a in weights]
         u, v, w = [np.ascontiguousarray(a, np.float32).copy() for a in mac]
         vu, vv, vw = [np.ascontiguousarray(a, np.uint8).copy() for a in valid]
         rows = np.zeros(isize * jsize * ksize, np.float64)
         a = PressureArgs(isize, jsize, ksize, dx, dt, tol, acceptable, maxiter, precision, _ptr(lp), _ptr(wu),
-                         _ptr(wv), _ptr(ww), _ptr(wc), None, None, None, _ptr(u), _ptr(v), _ptr(w), _ptr(vu),
+                         _ptr(wv), _ptr(ww), _ptr(wc), _ptr(sv[0]), _ptr(sv[1]), _ptr(sv[2]), _ptr(u), _ptr(v),
+                         _ptr(w), _ptr(vu),
                          _ptr(vv), _ptr(vw), _ptr(rows))
         rep = PressureReport()
         rc = self.L.flipmi_pressure_solve(self.h, ctypes.byref(a), ctypes.byref(rep))
         if rc not in (FLIPMI_OK, FLIPMI_ERR_NOT_CONVERGED):
             self.check(rc)
         return dict(ok=int(rc == FLIPMI_OK), iters=rep.iterations, err=rep.residual, rows=rep.rows,
-                    soln=rows[:rep.rows], mac=(u, v, w), valid=(vu, vv, vw), refinements=rep.refinements)
+                    soln=rows[:rep.rows], mac=(u, v, w), valid=(vu, vv, vw), refinements=rep.refinements,
+                    solid_vel=None if solid_vel is None else tuple(sv))
 
     def mic0_apply(self, isize, jsize, ksize, dx, dt, liquid_phi, weights, r):
         lp = np.ascontiguousarray(liquid_phi, np.float32)

@@ -424,6 +424,18 @@ int flipmi_pressure_solve(flipmi_ctx *ctx, const flipmi_pressure_args *a, flipmi
         else hipck(launch_fill_f32(sw.p, nW, 0.0f, s), "fill");
         upload(u, a->u, nU, s); upload(v, a->v, nV, s); upload(w, a->w, nW, s);
         upload(vu, a->valid_u, nU, s); upload(vv, a->valid_v, nV, s); upload(vw, a->valid_w, nW, s);
+        // _conditionSolidVelocityField (pressuresolver.cpp:124-268) precedes the
+        // assembly; with all solid velocities zero it changes nothing.
+        const bool solid_given = a->solid_u || a->solid_v || a->solid_w;
+        if (solid_given) {
+            DBuf<int> parent;
+            DBuf<unsigned> flags;
+            alloc_dev(parent, nc);
+            alloc_dev(flags, nc);
+            hipck(launch_condition_solid(g, liq.p, wu.p, wv.p, ww.p, su.p, sv.p, sw.p, parent.p, flags.p, s),
+                  "solid velocity conditioning");
+            hipck(hipStreamSynchronize(s), "sync");
+        }
         if (a->precision != FLIPMI_PCG_FP64 && a->precision != FLIPMI_PCG_MIXED)
             throw StatusError{FLIPMI_ERR_INVALID_ARG, "unknown pcg precision"};
         PressureStore ps;
@@ -447,6 +459,9 @@ int flipmi_pressure_solve(flipmi_ctx *ctx, const flipmi_pressure_args *a, flipmi
         ok = o.success;
         download(a->u, u, nU, s); download(a->v, v, nV, s); download(a->w, w, nW, s);
         download(a->valid_u, vu, nU, s); download(a->valid_v, vv, nV, s); download(a->valid_w, vw, nW, s);
+        if (a->solid_u) download(a->solid_u, su, nU, s);
+        if (a->solid_v) download(a->solid_v, sv, nV, s);
+        if (a->solid_w) download(a->solid_w, sw, nW, s);
         std::vector<float> hl(nc);
         int rows = 0;
         std::vector<double> x;

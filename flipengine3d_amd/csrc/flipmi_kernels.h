@@ -163,6 +163,12 @@ hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *l
                                     const float *w, const float *sU, const float *sV,
                                     const float *sW, PressureBuffers &pb, hipStream_t s);
 hipError_t launch_pressure_reset(PressureBuffers &pb, hipStream_t s);
+// PressureSolver::_conditionSolidVelocityField: zeroes the solid face
+// velocities (sU, sV, sW, in place) around isolated fluid pockets.  Scratch:
+// parent and flags, one element per cell each.
+hipError_t launch_condition_solid(const GridGeom &g, const float *liquid, const float *wU, const float *wV,
+                                  const float *wW, float *sU, float *sV, float *sW, int *parent,
+                                  unsigned *flags, hipStream_t s);
 // Marks the slot groups that hold a row (after assembly).
 hipError_t launch_group_mask(PressureBuffers &pb, hipStream_t s);
 // Zeroes the factor, finds each column tile's row extent and factorises.

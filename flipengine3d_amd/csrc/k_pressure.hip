@@ -20,6 +20,8 @@
 // consumer polls the data itself and a producer never waits for its stores.
 // Each cell performs exactly the reference's operations in the reference's
 // order, so the preconditioner is the reference's bit for bit.
+#include <algorithm>
+
 #include "device_math.h"
 #include "flipmi_kernels.h"
 
@@ -1081,6 +1083,130 @@ __global__ void k_reset_scalars(PressureBuffers pb) {
     sc->failed = 0;
     sc->converged = 0;
     sc->iters = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Isolated fluid pockets (PressureSolver::_conditionSolidVelocityField,
+// pressuresolver.cpp:124-268).  A region of fluid cells joined through faces
+// of weight >= 1e-6 that has more than one cell, contains an interior cell
+// and borders air through no open face gets zero solid velocities on all six
+// faces of each of its cells, which makes its Neumann-only block consistent.
+// The reference grows each region by a serial flood fill; here regions are
+// the connected components of a lock-free union-find (roots are linked under
+// the smaller index with a compare-and-swap; finds halve their paths), then
+// one pass collects per-region flags at the roots and one applies them.
+// ---------------------------------------------------------------------------
+
+constexpr unsigned kPocketInterior = 1u, kPocketMulti = 2u, kPocketAir = 4u;
+constexpr float kOpenFace = 1e-6f;  // "0.0f + eps", pressuresolver.cpp:149
+
+__device__ int uf_find(int *parent, int x) {
+    for (;;) {
+        const int p = ld_dev(parent + x);
+        if (p == x) return x;
+        const int gp = ld_dev(parent + p);
+        // Parents only ever point to smaller indices, so x (not a root) keeps
+        // an ancestor; x is never the target of a link (links need a root).
+        if (gp != p) st_dev(parent + x, gp);
+        x = gp;
+    }
+}
+
+__device__ void uf_unite(int *parent, int a, int b) {
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a < b) { const int t = a; a = b; b = t; }
+        if (atomicCAS(parent + a, a, b) == a) return;
+    }
+}
+
+__global__ void k_pocket_init(GridGeom g, const float *__restrict__ liquid, int *parent, unsigned *flags) {
+    const size_t n = (size_t)g.I * g.J * g.K;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        parent[c] = liquid[c] < 0.0f ? (int)c : -1;
+        flags[c] = 0u;
+    }
+}
+
+__global__ void k_pocket_union(GridGeom g, const float *__restrict__ liquid, const float *__restrict__ wU,
+                               const float *__restrict__ wV, const float *__restrict__ wW, int *parent) {
+    const size_t n = (size_t)g.I * g.J * g.K;
+    Dims DU{g.I + 1, g.J, g.K}, DV{g.I, g.J + 1, g.K}, DW{g.I, g.J, g.K + 1};
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        if (!(liquid[c] < 0.0f)) continue;
+        const int i = (int)(c % (size_t)g.I), j = (int)((c / (size_t)g.I) % (size_t)g.J);
+        const int k = (int)(c / ((size_t)g.I * g.J));
+        // Each face once, from the cell on its low side.
+        if (i + 1 < g.I && liquid[c + 1] < 0.0f && wU[DU.at(i + 1, j, k)] >= kOpenFace)
+            uf_unite(parent, (int)c, (int)(c + 1));
+        if (j + 1 < g.J && liquid[c + g.I] < 0.0f && wV[DV.at(i, j + 1, k)] >= kOpenFace)
+            uf_unite(parent, (int)c, (int)(c + g.I));
+        const size_t sk = (size_t)g.I * g.J;
+        if (k + 1 < g.K && liquid[c + sk] < 0.0f && wW[DW.at(i, j, k + 1)] >= kOpenFace)
+            uf_unite(parent, (int)c, (int)(c + sk));
+    }
+}
+
+__global__ void k_pocket_flags(GridGeom g, const float *__restrict__ liquid, const float *__restrict__ wU,
+                               const float *__restrict__ wV, const float *__restrict__ wW, int *parent,
+                               unsigned *flags) {
+    const size_t n = (size_t)g.I * g.J * g.K;
+    Dims DU{g.I + 1, g.J, g.K}, DV{g.I, g.J + 1, g.K}, DW{g.I, g.J, g.K + 1};
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        if (ld_dev(parent + c) < 0) continue;
+        const int r = uf_find(parent, (int)c);
+        const int i = (int)(c % (size_t)g.I), j = (int)((c / (size_t)g.I) % (size_t)g.J);
+        const int k = (int)(c / ((size_t)g.I * g.J));
+        unsigned f = (r != (int)c) ? kPocketMulti : 0u;
+        if (interior(g, i, j, k)) {
+            f |= kPocketInterior;
+            // _computeBordersAirGridThread (pressuresolver.cpp:246-268).
+            const size_t sk = (size_t)g.I * g.J;
+            const bool air = (wU[DU.at(i, j, k)] >= kOpenFace && liquid[c - 1] >= 0.0f) ||
+                             (wU[DU.at(i + 1, j, k)] >= kOpenFace && liquid[c + 1] >= 0.0f) ||
+                             (wV[DV.at(i, j, k)] >= kOpenFace && liquid[c - g.I] >= 0.0f) ||
+                             (wV[DV.at(i, j + 1, k)] >= kOpenFace && liquid[c + g.I] >= 0.0f) ||
+                             (wW[DW.at(i, j, k)] >= kOpenFace && liquid[c - sk] >= 0.0f) ||
+                             (wW[DW.at(i, j, k + 1)] >= kOpenFace && liquid[c + sk] >= 0.0f);
+            if (air) f |= kPocketAir;
+        }
+        if (f) atomicOr(flags + r, f);
+    }
+}
+
+__global__ void k_pocket_apply(GridGeom g, int *parent, const unsigned *__restrict__ flags, float *sU, float *sV,
+                               float *sW) {
+    const size_t n = (size_t)g.I * g.J * g.K;
+    Dims DU{g.I + 1, g.J, g.K}, DV{g.I, g.J + 1, g.K}, DW{g.I, g.J, g.K + 1};
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        if (ld_dev(parent + c) < 0) continue;
+        const unsigned f = flags[uf_find(parent, (int)c)];
+        if (!(f & kPocketInterior) || !(f & kPocketMulti) || (f & kPocketAir)) continue;
+        const int i = (int)(c % (size_t)g.I), j = (int)((c / (size_t)g.I) % (size_t)g.J);
+        const int k = (int)(c / ((size_t)g.I * g.J));
+        sU[DU.at(i, j, k)] = 0.0f;
+        sU[DU.at(i + 1, j, k)] = 0.0f;
+        sV[DV.at(i, j, k)] = 0.0f;
+        sV[DV.at(i, j + 1, k)] = 0.0f;
+        sW[DW.at(i, j, k)] = 0.0f;
+        sW[DW.at(i, j, k + 1)] = 0.0f;
+    }
+}
+
+hipError_t launch_condition_solid(const GridGeom &g, const float *liquid, const float *wU, const float *wV,
+                                  const float *wW, float *sU, float *sV, float *sW, int *parent,
+                                  unsigned *flags, hipStream_t s) {
+    const size_t n = (size_t)g.I * g.J * g.K;
+    if (n == 0) return hipSuccess;
+    if (n > (size_t)0x7fffffff) return hipErrorInvalidValue;  // cell indices are int
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    k_pocket_init<<<blocks, 256, 0, s>>>(g, liquid, parent, flags);
+    k_pocket_union<<<blocks, 256, 0, s>>>(g, liquid, wU, wV, wW, parent);
+    k_pocket_flags<<<blocks, 256, 0, s>>>(g, liquid, wU, wV, wW, parent, flags);
+    k_pocket_apply<<<blocks, 256, 0, s>>>(g, parent, flags, sU, sV, sW);
+    return hipGetLastError();
 }
 
 hipError_t launch_pressure_reset(PressureBuffers &pb, hipStream_t s) {

@@ -206,7 +206,10 @@ typedef struct {
     int precision;             /* flipmi_pcg_precision */
     const float *liquid_phi;   /* I x J x K */
     const float *weight_u, *weight_v, *weight_w, *weight_c;
-    const float *solid_u, *solid_v, *solid_w;  /* solid face velocities, NULL = 0 */
+    /* Solid face velocities (MeshLevelSet::getFaceVelocityU/V/W), NULL = 0.
+     * In/out: zeroed around isolated fluid pockets, as the reference's
+     * _conditionSolidVelocityField does (pressuresolver.cpp:124-268). */
+    float *solid_u, *solid_v, *solid_w;
     float *u, *v, *w;          /* MAC velocity, updated in place */
     uint8_t *valid_u, *valid_v, *valid_w;      /* updated in place */
     double *pressure_rows;     /* optional out: solution in reference row order */

@@ -241,12 +241,12 @@ class Oracle(_Base):
         L.orc_sim_set_pcg.argtypes = [_vp, _c_int, _c_double, _c_int]
         L.orc_sim_pressure_info.argtypes = [_vp, _vp, _vp, _vp]
         L.orc_pressure_solve.argtypes = ([_c_int] * 3 + [_c_double, _c_double, _c_double, _c_int, _c_int]
-                                         + [_vp] * 12 + [_vp] * 5)
+                                         + [_vp] * 15 + [_vp] * 5)
 
     def pressure_solve(self, isize, jsize, ksize, dx, dt, liquid_phi, solid_phi, weights, mac, valid,
-                       tol=1e-9, maxiter=1000, mode=0):
+                       tol=1e-9, maxiter=1000, mode=0, solid_vel=None):
         return _pressure_solve(self.lib.orc_pressure_solve, True, isize, jsize, ksize, dx, dt, liquid_phi,
-                               solid_phi, weights, mac, valid, tol, maxiter, mode)
+                               solid_phi, weights, mac, valid, tol, maxiter, mode, solid_vel)
 
 
 class Reference(_Base):
@@ -258,7 +258,7 @@ class Reference(_Base):
         L.ref_sim_create.argtypes = [_c_int] * 3 + [_c_double, _c_int]
         L.ref_set_thread_count.argtypes = [_c_int]
         L.ref_pressure_solve.argtypes = ([_c_int] * 3 + [_c_double, _c_double, _c_double, _c_int]
-                                         + [_vp] * 12 + [_vp] * 5)
+                                         + [_vp] * 15 + [_vp] * 5)
         L.ref_sim_pressure_status.argtypes = [_vp, ctypes.c_char_p, _c_int]
         self.threads = threads
 
@@ -270,18 +270,21 @@ class Reference(_Base):
         self.lib.ref_set_thread_count(n)
 
     def pressure_solve(self, isize, jsize, ksize, dx, dt, liquid_phi, solid_phi, weights, mac, valid,
-                       tol=1e-9, maxiter=1000, mode=0):
+                       tol=1e-9, maxiter=1000, mode=0, solid_vel=None):
         return _pressure_solve(self.lib.ref_pressure_solve, False, isize, jsize, ksize, dx, dt, liquid_phi,
-                               solid_phi, weights, mac, valid, tol, maxiter, mode)
+                               solid_phi, weights, mac, valid, tol, maxiter, mode, solid_vel)
 
 
 def _pressure_solve(fn, with_mode, isize, jsize, ksize, dx, dt, liquid_phi, solid_phi, weights, mac, valid,
-                    tol, maxiter, mode):
+                    tol, maxiter, mode, solid_vel=None):
+    """solid_vel: optional (U, V, W) solid face velocities; the returned
+    'solid_vel' holds them after the solver's pocket conditioning."""
     lp = np.ascontiguousarray(liquid_phi, np.float32)
     sp = np.ascontiguousarray(solid_phi, np.float32)
     wu, wv, ww, wc = [np.ascontiguousarray(a, np.float32) for a in weights]
     u, v, w = [np.ascontiguousarray(a, np.float32).copy() for a in mac]
     vu, vv, vw = [np.ascontiguousarray(a, np.uint8).copy() for a in valid]
+    sv = (None, None, None) if solid_vel is None else [np.ascontiguousarray(a, np.float32).copy() for a in solid_vel]
     nmax = isize * jsize * ksize
     rhs = np.zeros(nmax, np.float64)
     soln = np.zeros(nmax, np.float64)
@@ -292,8 +295,8 @@ def _pressure_solve(fn, with_mode, isize, jsize, ksize, dx, dt, liquid_phi, soli
     if with_mode:
         args.append(mode)
     ok = fn(*args, _ptr(lp), _ptr(sp), _ptr(wu), _ptr(wv), _ptr(ww), _ptr(wc), _ptr(u), _ptr(v), _ptr(w),
-            _ptr(vu), _ptr(vv), _ptr(vw), _ptr(rhs), _ptr(soln), ctypes.byref(n), ctypes.byref(it),
-            ctypes.byref(err))
+            _ptr(vu), _ptr(vv), _ptr(vw), _ptr(sv[0]), _ptr(sv[1]), _ptr(sv[2]), _ptr(rhs), _ptr(soln),
+            ctypes.byref(n), ctypes.byref(it), ctypes.byref(err))
     m = n.value
     return dict(ok=int(ok), rhs=rhs[:m], soln=soln[:m], iters=it.value, err=err.value,
-                mac=(u, v, w), valid=(vu, vv, vw))
+                mac=(u, v, w), valid=(vu, vv, vw), solid_vel=None if solid_vel is None else tuple(sv))

@@ -811,7 +811,10 @@ static void condition_solid_velocity(const PressureInputs &in) {
                 while (!queue.empty()) {
                     Idx g = queue.back();
                     queue.pop_back();
+                    // A region can reach border cells; their out-of-grid
+                    // neighbours are not cells and are never visited.
                     auto visit = [&](int a, int b, int c, float wf) {
+                        if (a < 0 || b < 0 || c < 0 || a >= I || b >= J || c >= K) return;
                         if (!done(a, b, c) && phi(a, b, c) < 0.0f && wf >= 0.0f + eps) {
                             queue.push_back(Idx{a, b, c});
                             done(a, b, c) = 1;
@@ -1554,8 +1557,8 @@ int orc_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, do
                        int maxiter, int mode, const float *liquid_phi,
                        const float *solid_phi_nodal, const float *wU, const float *wV,
                        const float *wW, const float *wC, float *u, float *v, float *w,
-                       uint8_t *vu, uint8_t *vv, uint8_t *vw, double *rhs_out, double *soln_out,
-                       int *n_rows, int *iters, double *err) {
+                       uint8_t *vu, uint8_t *vv, uint8_t *vw, float *su, float *sv, float *sw,
+                       double *rhs_out, double *soln_out, int *n_rows, int *iters, double *err) {
     (void)solid_phi_nodal;
     GridF liquid(isize, jsize, ksize, 0.0f);
     copy_in(liquid, liquid_phi);
@@ -1564,8 +1567,12 @@ int orc_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, do
     wg.V = GridF(isize, jsize + 1, ksize, 0.0f); copy_in(wg.V, wV);
     wg.W = GridF(isize, jsize, ksize + 1, 0.0f); copy_in(wg.W, wW);
     wg.C = GridF(isize, jsize, ksize, 0.0f); copy_in(wg.C, wC);
+    // Solid face velocities, in/out (null = zero).
     GridF sU(isize + 1, jsize, ksize, 0.0f), sV(isize, jsize + 1, ksize, 0.0f),
         sW(isize, jsize, ksize + 1, 0.0f);
+    if (su) copy_in(sU, su);
+    if (sv) copy_in(sV, sv);
+    if (sw) copy_in(sW, sw);
     MAC m;
     mac_init(m, isize, jsize, ksize);
     copy_in(m.U, u); copy_in(m.V, v); copy_in(m.W, w);
@@ -1582,6 +1589,9 @@ int orc_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, do
     if (err) *err = pr.err;
     copy_out(m.U, u); copy_out(m.V, v); copy_out(m.W, w);
     copy_out(va.U, vu); copy_out(va.V, vv); copy_out(va.W, vw);
+    if (su) copy_out(sU, su);
+    if (sv) copy_out(sV, sv);
+    if (sw) copy_out(sW, sw);
     return pr.ok;
 }
 

@@ -40,8 +40,8 @@ int orc_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, do
                        int maxiter, int mode, const float *liquid_phi,
                        const float *solid_phi_nodal, const float *wU, const float *wV,
                        const float *wW, const float *wC, float *u, float *v, float *w,
-                       uint8_t *vu, uint8_t *vv, uint8_t *vw, double *rhs_out, double *soln_out,
-                       int *n_rows, int *iters, double *err);
+                       uint8_t *vu, uint8_t *vv, uint8_t *vw, float *su, float *sv, float *sw,
+                       double *rhs_out, double *soln_out, int *n_rows, int *iters, double *err);
 
 void orc_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, const float *liquid_phi,
                     const float *wU, const float *wV, const float *wW, const float *wC,

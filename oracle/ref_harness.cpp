@@ -326,6 +326,7 @@ int ref_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, do
                        int maxiter, const float *liquid_phi, const float *solid_phi_nodal,
                        const float *wU, const float *wV, const float *wW, const float *wC,
                        float *u, float *v, float *w, uint8_t *vu, uint8_t *vv, uint8_t *vw,
+                       float *su, float *sv, float *sw,
                        double *rhs_out, double *soln_out, int *n_rows, int *iters,
                        double *err) {
     MACVelocityField mac(isize, jsize, ksize, dx);
@@ -340,6 +341,11 @@ int ref_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, do
     fill_array(liquid._phi, liquid_phi);
     MeshLevelSet solid(isize, jsize, ksize, dx);
     fill_array(solid._phi, solid_phi_nodal);
+    // Solid face velocities (in/out: _conditionSolidVelocityField zeroes them
+    // around isolated pockets); null = the zero field of a fresh level set.
+    if (su) fill_array(solid._velocityData.field._u, (const float *)su);
+    if (sv) fill_array(solid._velocityData.field._v, (const float *)sv);
+    if (sw) fill_array(solid._velocityData.field._w, (const float *)sw);
     WeightGrid wg(isize, jsize, ksize);
     fill_array(wg.U, wU);
     fill_array(wg.V, wV);
@@ -402,6 +408,9 @@ int ref_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, do
         }
         if (ok) ps._applySolutionToVelocityField(soln);
     }
+    if (su) copy_array(solid._velocityData.field._u, su);
+    if (sv) copy_array(solid._velocityData.field._v, sv);
+    if (sw) copy_array(solid._velocityData.field._w, sw);
     copy_array(mac._u, u);
     copy_array(mac._v, v);
     copy_array(mac._w, w);

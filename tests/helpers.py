@@ -27,3 +27,43 @@ def bits_equal(a, b):
 
 def ndiff(a, b):
     return int(np.count_nonzero(np.asarray(a) != np.asarray(b)))
+
+
+def pocket_inputs(oracle, I, J, K, seed=6):
+    """Projection inputs with nonzero solid face velocities and fluid pockets.
+
+    Besides a dam break (which borders air), the liquid field holds
+      * a 3 x 3 x 3 cavity in the air region whose boundary faces all have
+        weight 0: a multi-cell region that borders no air, so the pressure
+        solver zeroes the solid velocities around it
+        (pressuresolver.cpp:124-268);
+      * a single fluid cell with closed faces and zero cell weight: a region of
+        one cell, left alone by that rule (its row is empty).
+    Returns (dx, solid_phi, weights, liquid, mac, valid, solid_vel, cavity, single).
+    """
+    s = dam(I, J, K, seed=seed)
+    dx = s.dx
+    solid, weights = oracle.domain_solid(I, J, K, dx)
+    wu, wv, ww, wc = [w.copy() for w in weights]
+    liquid = oracle.liquid_sdf(I, J, K, dx, s.positions, radius(dx), solid)
+    i0, j0, k0 = int(0.65 * I), int(0.7 * J), K // 2 - 1
+    cavity = (slice(k0, k0 + 3), slice(j0, j0 + 3), slice(i0, i0 + 3))
+    liquid[cavity] = np.float32(-0.5 * dx)
+    wu[k0:k0 + 3, j0:j0 + 3, i0] = 0.0
+    wu[k0:k0 + 3, j0:j0 + 3, i0 + 3] = 0.0
+    wv[k0:k0 + 3, j0, i0:i0 + 3] = 0.0
+    wv[k0:k0 + 3, j0 + 3, i0:i0 + 3] = 0.0
+    ww[k0, j0:j0 + 3, i0:i0 + 3] = 0.0
+    ww[k0 + 3, j0:j0 + 3, i0:i0 + 3] = 0.0
+    si, sj, sk = int(0.85 * I), int(0.3 * J), K // 3
+    liquid[sk, sj, si] = np.float32(-0.5 * dx)
+    wc[sk, sj, si] = 0.0
+    wu[sk, sj, si] = wu[sk, sj, si + 1] = 0.0
+    wv[sk, sj, si] = wv[sk, sj + 1, si] = 0.0
+    ww[sk, sj, si] = ww[sk + 1, sj, si] = 0.0
+    vel = random_velocities(s.num_particles, seed=seed, scale=0.3)
+    mac, valid = oracle.p2g(I, J, K, dx, s.positions, vel, radius(dx))
+    mac = [oracle.extrapolate(g, v, 7) for g, v in zip(mac, valid)]
+    rng = np.random.default_rng(seed)
+    solid_vel = tuple((rng.standard_normal(g.shape) * 0.2).astype(np.float32) for g in mac)
+    return dx, solid, (wu, wv, ww, wc), liquid, mac, valid, solid_vel, cavity, (sk, sj, si)

@@ -151,3 +151,27 @@ def test_trilinear_velocity_bitexact(gpu, oracle):
     pos = (rng.random((5000, 3)) * np.array([I, J, K]) * dx).astype(np.float32)
     assert bits_equal(gpu.trilinear_velocity(I, J, K, dx, mac, pos),
                       oracle.trilinear_velocity(I, J, K, dx, mac, pos))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_pressure_pockets_and_solid_velocities(gpu, oracle, precision):
+    # Nonzero solid face velocities and isolated pockets: the device pocket
+    # conditioning (union-find) zeroes the same faces as the reference's
+    # flood fill, bit for bit, and the projection matches the oracle.
+    from tests.helpers import pocket_inputs
+    I, J, K = 20, 18, 17
+    dx, solid, weights, liquid, mac, valid, svel, cavity, single = pocket_inputs(oracle, I, J, K)
+    dt = 1 / 60
+    pg = gpu.pressure_solve(I, J, K, dx, dt, liquid, weights, mac, valid, precision=precision, solid_vel=svel)
+    po = oracle.pressure_solve(I, J, K, dx, dt, liquid, solid, weights, mac, valid, mode=precision,
+                               solid_vel=svel)
+    assert pg["ok"] == po["ok"] == 1
+    for a, b in zip(pg["solid_vel"], po["solid_vel"]):
+        assert bits_equal(a, b)
+    p_o = po["soln"]
+    assert np.abs(pg["soln"] - p_o).max() <= 1e-6 * np.abs(p_o).max()
+    for a, b in zip(pg["valid"], po["valid"]):
+        assert bits_equal(a, b)
+    vmax = max(float(np.abs(g).max()) for g in po["mac"])
+    for a, b in zip(pg["mac"], po["mac"]):
+        assert np.abs(a.astype(np.float64) - b).max() <= 1e-5 * vmax

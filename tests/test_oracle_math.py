@@ -478,3 +478,25 @@ def test_per_cell_cap_keeps_the_first_250(oracle):
     p, _ = h.particles()
     assert p.shape[0] == 250
     assert np.array_equal(p, pos[:250])
+
+
+def test_pocket_conditioning_zeroes_only_closed_multi_cell_regions(oracle):
+    # pressuresolver.cpp:124-268: solid face velocities are zeroed on the six
+    # faces of every cell of a fluid region that has more than one cell and
+    # borders no air through an open face.  The cavity of pocket_inputs is
+    # such a region; the one-cell pocket and the dam (which borders air) are
+    # not, so every other face keeps its velocity.
+    from tests.helpers import pocket_inputs
+    I, J, K = 20, 18, 17
+    dx, solid, weights, liquid, mac, valid, svel, cavity, single = pocket_inputs(oracle, I, J, K)
+    out = oracle.pressure_solve(I, J, K, dx, 1 / 60, liquid, solid, weights, mac, valid, solid_vel=svel)
+    assert out["ok"] == 1
+    ks, js, is_ = cavity
+    want = [a.copy() for a in svel]
+    want[0][ks, js, is_.start:is_.stop + 1] = 0.0
+    want[1][ks, js.start:js.stop + 1, is_] = 0.0
+    want[2][ks.start:ks.stop + 1, js, is_] = 0.0
+    for got, w in zip(out["solid_vel"], want):
+        assert np.array_equal(got, w)
+    # The conditioned system is consistent: the solve converged to 1e-9 |b|.
+    assert out["err"] <= 1e-9 * np.abs(out["rhs"]).max()

@@ -141,3 +141,21 @@ def test_whole_update(oracle, reference, case):
         assert bits_equal(va, vb), f"frame {f}: velocities ndiff={ndiff(va, vb)}"
     for which in (0, 2, 3, 4):
         assert bits_equal(sims[0].float_grid(which), sims[1].float_grid(which))
+
+
+def test_pressure_solve_pockets_reference_exact(oracle, reference):
+    # Nonzero solid face velocities with an isolated multi-cell pocket and a
+    # one-cell pocket: the solver's pocket conditioning, the solve and the
+    # conditioned solid velocities agree bit for bit.
+    from tests.helpers import pocket_inputs
+    I, J, K = 20, 18, 17
+    dx, solid, weights, liquid, mac, valid, svel, cavity, single = pocket_inputs(oracle, I, J, K)
+    reference.set_threads(1)
+    a = oracle.pressure_solve(I, J, K, dx, 1 / 60, liquid, solid, weights, mac, valid, mode=0, solid_vel=svel)
+    b = reference.pressure_solve(I, J, K, dx, 1 / 60, liquid, solid, weights, mac, valid, solid_vel=svel)
+    assert a["ok"] == b["ok"] == 1
+    assert a["iters"] == b["iters"]
+    assert bits_equal(a["rhs"], b["rhs"])
+    assert bits_equal(a["soln"], b["soln"])
+    for x, y in zip(a["solid_vel"] + a["mac"] + a["valid"], b["solid_vel"] + b["mac"] + b["valid"]):
+        assert bits_equal(x, y)
