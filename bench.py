@@ -45,6 +45,16 @@ HBM_PEAK_GBS = 8000.0
 #   update_s:   z, s r/w                                 -> 3 words
 PCG_WORDS = {"k_spmv_dot": 6, "k_axpy_norm": 6, "k_forward": 6, "k_backward": 6, "k_update_s": 3}
 
+# HBM bytes moved per launch, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes scaled by a plain 256 MiB copy (x2.0 and x1.0), on the default
+# workload; the commands and counter readings are in
+# profiles/traffic_pcg_kernels_double_dam_256x128x128_mixed.md.  Keyed by
+# (precision, grid); other workloads report null.
+MEASURED_TRAFFIC = {
+    ("mixed", (256, 128, 128)): {"k_backward": 68.43e6, "k_forward": 60.61e6, "k_spmv_dot": 92.87e6,
+                                 "k_axpy_norm": 53.39e6, "k_update_s": 19.99e6},
+}
+
 
 def parse_args(argv=None):
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -66,6 +76,8 @@ def parse_args(argv=None):
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpus available)")
     p.add_argument("--cpu-seconds", type=float, default=20.0,
                    help="stop the CPU sample after the first substep that ends past this many seconds")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only copy 256 MiB device to device 3 times (known bytes, to scale HBM counters)")
     p.add_argument("--verbose", action="store_true")
     return p.parse_args(argv)
 
@@ -97,9 +109,35 @@ def cpu_threads(args):
     return max(1, min(16, n))
 
 
+class _StdoutToStderr:
+    """Sends file descriptor 1 to stderr while the CPU engines run: the
+    reference engine prints a banner and a log on stdout, and bench.py's
+    stdout must hold only the JSON line."""
+
+    def __enter__(self):
+        import ctypes
+        sys.stdout.flush()
+        self._libc = ctypes.CDLL(None)
+        self._libc.fflush(None)
+        self._saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        self._libc.fflush(None)
+        os.dup2(self._saved, 1)
+        os.close(self._saved)
+        return False
+
+
 def cpu_baseline(scene, args):
     """Times the reference engine (or, where it was not built, the CPU
     oracle) on the same scene for whole substeps until --cpu-seconds pass."""
+    with _StdoutToStderr():
+        return _cpu_baseline(scene, args)
+
+
+def _cpu_baseline(scene, args):
     from oracle import bindings
     I, J, K = args.size
     if bindings.ref_available():
@@ -202,11 +240,13 @@ def run_gpu(fa, scene, args, precision, steps, warmup, profiling):
         kcalls = min(kcalls, iters)
         launch_s = kms * 1e-3 / max(kcalls, 1)
         achieved = bytes_per_launch / launch_s / 1e9 if launch_s > 0 else 0.0
+        traffic = MEASURED_TRAFFIC.get((precision, tuple(args.size)), {}).get(kern)
         roof.update({
             "kernel": f"k_sweep<{'forward' if kern == 'k_forward' else 'backward'}>"
             if kern in ("k_forward", "k_backward") else kern,
             "achieved": achieved,
             "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": launch_s * 1e6,
             "launches": kcalls,
@@ -227,6 +267,22 @@ def run_gpu(fa, scene, args, precision, steps, warmup, profiling):
         "stages": {k: round(v[0] / steps, 3) for k, v in times.items() if k != "pcg_iters"},
     }
 
+def copy_calibration():
+    """A plain device copy of known size: reads and writes 256 MiB per
+    launch.  Its FETCH_SIZE / WRITE_SIZE readings give the scale from those
+    counters to bytes."""
+    import torch
+    n = 256 << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}), flush=True)
+    return 0
+
+
 def main(argv=None):
     args = parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -235,6 +291,8 @@ def main(argv=None):
               "multi-GPU path (a single scene would shard as x-slabs with halo exchange; serving shards "
               "whole scenes). Run with --gpus 1.", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration()
 
     import flipengine3d_amd as fa
     from flipengine3d_amd import scenes

@@ -25,3 +25,17 @@ def test_pcg_byte_model_matches_the_kernels():
     args = bench.parse_args([])
     assert args.gpus == 1 and args.size == [256, 128, 128]
     assert json.dumps({"x": None})  # the line is plain JSON
+
+
+def test_cpu_baseline_leaves_stdout_to_the_json_line():
+    # The reference engine prints a banner and a log on stdout; bench.py's
+    # stdout must carry only its one JSON line.
+    code = ("import sys, json; sys.path.insert(0, %r); import bench; from flipengine3d_amd import scenes; "
+            "a = bench.parse_args(['--size', '24', '16', '16', '--cpu-seconds', '0.01', '--cpu-threads', '1']); "
+            "s = scenes.double_dam_break(24, 16, 16, 1 / 16, seed=3); r = bench.cpu_baseline(s, a); "
+            "print(json.dumps(r))") % ROOT
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert len(lines) == 1, r.stdout
+    assert json.loads(lines[0])["value"] > 0
