@@ -76,6 +76,15 @@ def parse_args(argv=None):
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpus available)")
     p.add_argument("--cpu-seconds", type=float, default=20.0,
                    help="stop the CPU sample after the first substep that ends past this many seconds")
+    p.add_argument("--serving", action="store_true",
+                   help="the serving workload instead: Poisson arrivals of independent mixed-size scenes")
+    p.add_argument("--requests", type=int, default=256, help="serving: number of requests")
+    p.add_argument("--rate", type=float, default=20.0, help="serving: mean arrivals per second")
+    p.add_argument("--request-steps", type=int, default=30, help="serving: update calls per request")
+    p.add_argument("--max-active", type=int, default=16, help="serving: scenes stepped at once")
+    p.add_argument("--budget-gib", type=float, default=200.0, help="serving: device memory budget")
+    p.add_argument("--plain-requests", type=int, default=24,
+                   help="serving: requests also run one at a time on the plain FP64 path")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 256 MiB device to device 3 times (known bytes, to scale HBM counters)")
     p.add_argument("--verbose", action="store_true")
@@ -283,6 +292,135 @@ def copy_calibration():
     return 0
 
 
+SERVING_GRIDS = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 64, 64)]
+
+
+def serving_requests(args):
+    """The serving workload: request r draws its grid uniformly from
+    SERVING_GRIDS (dx = 1 / largest dimension), a dam-break cuboid whose x
+    extent and height are U(0.2, 0.6) fractions of the domain (full z), 8
+    jittered particles per cell seeded with 4 * 2^32 + r, and arrives after
+    exponential gaps of mean 1 / rate (seed 4)."""
+    import numpy as np
+    from flipengine3d_amd import scenes
+    rng = np.random.default_rng(4)
+    t = 0.0
+    out = []
+    for r in range(args.requests):
+        t += rng.exponential(1.0 / args.rate)
+        I, J, K = SERVING_GRIDS[int(rng.integers(len(SERVING_GRIDS)))]
+        fx, fy = rng.uniform(0.2, 0.6, size=2)
+        dx = 1.0 / max(I, J, K)
+        sc = scenes.dam_break(I, J, K, dx, seed=4 * 2**32 + r, box=(0.0, 0.0, 0.0, float(fx), float(fy), 1.0))
+        out.append((t, (I, J, K), dx, sc))
+    return out
+
+
+def run_serving(args):
+    """Serving: requests arrive on their schedule and are admitted by the
+    server (continuous batching, memory budget); value = particle-substeps of
+    all requests over the makespan.  The plain path (one scene at a time,
+    FP64 PCG, fresh FluidSimulation each) runs a prefix of the same requests
+    back to back for comparison."""
+    import threading
+
+    import flipengine3d_amd as fa
+    t = time.perf_counter()
+    reqs = serving_requests(args)
+    log(f"serving: {len(reqs)} requests built in {time.perf_counter() - t:.1f} s", args)
+    ctx = fa.Context(0)
+    server = fa.Server(ctx, max_active=args.max_active, memory_budget=int(args.budget_gib * (1 << 30)))
+    ids = [None] * len(reqs)
+
+    def submitter(t0):
+        for q, (ta, dims, dx, sc) in enumerate(reqs):
+            delay = t0 + ta - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            desc = fa.scene_desc(*dims, dx, precision=fa.PCG_MIXED, substeps=(1, 1))
+            ids[q] = server.submit(desc, sc.positions, sc.velocities, args.dt, args.request_steps)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter, args=(t0,))
+    th.start()
+    reps = []
+    for q in range(len(reqs)):
+        while ids[q] is None:
+            time.sleep(0.001)
+        rep = server.wait(ids[q])
+        server.discard(ids[q])
+        reps.append(rep)
+        if (q + 1) % 32 == 0:
+            log(f"serving: {q + 1} requests done, {time.perf_counter() - t0:.1f} s", args)
+    th.join()
+    makespan = time.perf_counter() - t0
+    st = server.stats()
+    server.close()
+    work = sum(r["particle_substeps"] for r in reps)
+    failed = sum(1 for r in reps if r["status"] != 0)
+
+    # Plain path: one scene at a time, FP64 PCG, fresh scene per request.
+    plain_work = 0
+    plain_time = 0.0
+    plain_steps = []
+    for ta, dims, dx, sc in reqs[:args.plain_requests]:
+        sim = fa.FluidSimulation(*dims, dx, ctx=ctx, precision=fa.PCG_FP64)
+        sim.addBodyForce(0.0, -9.81, 0.0)
+        sim.setMinTimeStepsPerFrame(1)
+        sim.setMaxTimeStepsPerFrame(1)
+        ts = time.perf_counter()
+        sim.loadMarkerParticleData(sc.positions, sc.velocities)
+        sim.initialize()
+        for _ in range(args.request_steps):
+            n = sim.getNumMarkerParticles()
+            t1 = time.perf_counter()
+            stp = sim.update(args.dt)
+            plain_steps.append((time.perf_counter() - t1) * 1e3)
+            plain_work += n * stp.substeps
+        plain_time += time.perf_counter() - ts
+        sim.close()
+    plain_value = plain_work / plain_time if plain_time > 0 else None
+    value = work / makespan
+    plain_steps.sort()
+    nps = [sc.num_particles for _, _, _, sc in reqs]
+    return {
+        "metric": "serving particle-substeps/s (mixed-size scenes, Poisson arrivals); p50/p99 step latency",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "p50_ms": st["step_ms_p50"],
+        "p99_ms": st["step_ms_p99"],
+        "dtype": "f32 particles/grids and PCG, f64 dots/x/true residual",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{len(reqs)} dam-break requests, grids {SERVING_GRIDS}, fill U(0.2,0.6) in x and height, "
+                         f"{args.request_steps} substeps of dt={args.dt:.6g} s each, Poisson arrivals at "
+                         f"{args.rate:g}/s, seed 4"),
+            "particles_mean": sum(nps) / len(nps),
+            "max_active": args.max_active,
+            "budget_gib": args.budget_gib,
+            "makespan_s": makespan,
+            "offered_particle_substeps_per_s": sum(nps) * args.request_steps / reqs[-1][0],
+            "failed_requests": failed,
+            "pressure_failures": sum(r["pressure_failures"] for r in reps),
+            "scenes_created": st["scenes_created"],
+            "scenes_reused": st["scenes_reused"],
+            "scenes_evicted": st["scenes_evicted"],
+            "peak_resident_gib": st["peak_resident_bytes"] / (1 << 30),
+            "queue_ms_p50": statistics.median(r["queue_ms"] for r in reps),
+        },
+        "plain_fp64_path": {
+            "value": plain_value,
+            "requests": min(args.plain_requests, len(reqs)),
+            "p50_ms": plain_steps[len(plain_steps) // 2] if plain_steps else None,
+            "p99_ms": plain_steps[min(len(plain_steps) - 1, int(round(0.99 * (len(plain_steps) - 1))))]
+            if plain_steps else None,
+        },
+        "vs_plain": value / plain_value if plain_value else None,
+    }
+
+
 def main(argv=None):
     args = parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -293,6 +431,9 @@ def main(argv=None):
         return 2
     if args.copy_calibration:
         return copy_calibration()
+    if args.serving:
+        print(json.dumps(run_serving(args)), flush=True)
+        return 0
 
     import flipengine3d_amd as fa
     from flipengine3d_amd import scenes

@@ -73,6 +73,30 @@ class PressureArgs(ctypes.Structure):
                 ("pressure_rows", _vp)]
 
 
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("max_active", _c_int), ("memory_budget", _c_i64), ("particle_capacity_per_cell", _c_double)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("scene", SceneDesc), ("pos_xyz", _vp), ("vel_xyz", _vp), ("n", _c_i64), ("dt", _c_double),
+                ("steps", _c_int)]
+
+
+class RequestReport(ctypes.Structure):
+    _fields_ = [("status", _c_int), ("steps_done", _c_int), ("num_particles", _c_i64),
+                ("particle_substeps", _c_i64), ("pressure_failures", _c_int), ("reused_scene", _c_int),
+                ("queue_ms", _c_double), ("run_ms", _c_double), ("step_ms_p50", _c_double),
+                ("step_ms_p99", _c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", _c_i64), ("completed", _c_i64), ("failed", _c_i64), ("queued", _c_i64),
+                ("active", _c_i64), ("particle_substeps", _c_i64), ("steps", _c_i64),
+                ("step_ms_p50", _c_double), ("step_ms_p99", _c_double), ("resident_bytes", _c_i64),
+                ("peak_resident_bytes", _c_i64), ("scenes_created", _c_i64), ("scenes_reused", _c_i64),
+                ("scenes_evicted", _c_i64)]
+
+
 class PressureReport(ctypes.Structure):
     _fields_ = [("iterations", _c_int), ("residual", _c_double), ("rows", _c_int),
                 ("success", _c_int), ("refinements", _c_int)]
@@ -86,7 +110,8 @@ EXPORTED = [
     "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
     "flipmi_liquid_sdf_postprocess", "flipmi_p2g",
     "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
-    "flipmi_trilinear_velocity",
+    "flipmi_trilinear_velocity", "flipmi_server_desc_default", "flipmi_server_create", "flipmi_server_submit",
+    "flipmi_server_wait", "flipmi_server_take", "flipmi_server_get_stats", "flipmi_server_destroy",
 ]
 
 _lib = None
@@ -129,6 +154,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                           _c_double, _c_int, _vp])
     L.flipmi_trilinear_velocity.argtypes = ([_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _vp, _vp,
                                             _c_i64, _vp])
+    L.flipmi_server_desc_default.argtypes = [P(ServerDesc)]
+    L.flipmi_server_desc_default.restype = None
+    L.flipmi_server_create.argtypes = [_vp, P(ServerDesc), P(_vp)]
+    L.flipmi_server_submit.argtypes = [_vp, P(Request), P(_c_i64)]
+    L.flipmi_server_wait.argtypes = [_vp, _c_i64, _c_double, P(RequestReport)]
+    L.flipmi_server_take.argtypes = [_vp, _c_i64, _vp, _vp]
+    L.flipmi_server_get_stats.argtypes = [_vp, P(ServerStats)]
+    L.flipmi_server_destroy.argtypes = [_vp]
     _lib = L
     return L
 
@@ -405,6 +438,90 @@ class FluidSimulation:
     def close(self):
         if self.h is not None:
             self.L.flipmi_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scene_desc(isize: int, jsize: int, ksize: int, dx: float, body_force=(0.0, -9.81, 0.0),
+               precision: int = PCG_FP64, substeps=(1, 6), cfl: float = 5.0, pic_ratio: float = 0.05,
+               tolerance: float = 1e-9, max_iterations: int = 1000) -> SceneDesc:
+    """flipmi_scene_desc with the reference's defaults (FluidSimulation's
+    constructor and the setters the workloads use)."""
+    d = SceneDesc()
+    load_library().flipmi_scene_desc_default(ctypes.byref(d))
+    d.isize, d.jsize, d.ksize, d.dx = isize, jsize, ksize, dx
+    d.body_force[0], d.body_force[1], d.body_force[2] = body_force
+    d.pcg_precision = precision
+    d.min_substeps, d.max_substeps = substeps
+    d.cfl, d.pic_ratio = cfl, pic_ratio
+    d.pcg_tolerance, d.pcg_max_iterations = tolerance, max_iterations
+    return d
+
+
+class Server:
+    """The serving loop (flipmi_server): independent scenes admitted by a
+    memory budget and stepped concurrently, one HIP stream per slot."""
+
+    def __init__(self, ctx: Optional[Context] = None, max_active: int = 8, memory_budget: int = 200 << 30,
+                 particle_capacity_per_cell: float = 3.0):
+        self.ctx = ctx or Context()
+        self.L = self.ctx.L
+        d = ServerDesc()
+        self.L.flipmi_server_desc_default(ctypes.byref(d))
+        d.max_active, d.memory_budget, d.particle_capacity_per_cell = max_active, memory_budget, particle_capacity_per_cell
+        h = _vp()
+        self.ctx.check(self.L.flipmi_server_create(self.ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, desc: SceneDesc, positions: np.ndarray, velocities: Optional[np.ndarray], dt: float,
+               steps: int) -> int:
+        pos = np.ascontiguousarray(positions, np.float32)
+        vel = None if velocities is None else np.ascontiguousarray(velocities, np.float32)
+        r = Request(desc, _ptr(pos), _ptr(vel), pos.shape[0], dt, steps)
+        rid = _c_i64()
+        self.ctx.check(self.L.flipmi_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, timeout: float = -1.0) -> dict:
+        rep = RequestReport()
+        rc = self.L.flipmi_server_wait(self.h, rid, timeout, ctypes.byref(rep))
+        if rc == FLIPMI_ERR_TIMEOUT:
+            raise TimeoutError(f"request {rid} not done after {timeout} s")
+        out = {name: getattr(rep, name) for name, _ in RequestReport._fields_}
+        if rc != FLIPMI_OK:
+            out["error"] = self.L.flipmi_last_error(self.ctx.h).decode()
+        return out
+
+    def take(self, rid: int, n: int) -> Tuple[np.ndarray, np.ndarray]:
+        """The final particles of a finished request (n = its report's
+        num_particles); releases the request."""
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        self.L.flipmi_server_take(self.h, rid, _ptr(pos) if n else None, _ptr(vel) if n else None)
+        return pos, vel
+
+    def discard(self, rid: int):
+        """Releases a finished request without copying its particles."""
+        self.L.flipmi_server_take(self.h, rid, None, None)
+
+    def result(self, rid: int, timeout: float = -1.0):
+        rep = self.wait(rid, timeout)
+        pos, vel = self.take(rid, rep["num_particles"])
+        return rep, pos, vel
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self.ctx.check(self.L.flipmi_server_get_stats(self.h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h is not None:
+            self.L.flipmi_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

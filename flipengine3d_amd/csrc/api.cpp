@@ -15,9 +15,6 @@
 
 using namespace flipmi;
 
-struct flipmi_ctx {
-    Ctx c;
-};
 struct flipmi_scene {
     Scene s;
 };
@@ -45,15 +42,6 @@ int guarded(Ctx *ctx, F &&fn) {
 
 void hipck(hipError_t e, const char *what) {
     if (e != hipSuccess) throw StatusError{FLIPMI_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
-}
-
-GridGeom make_geom(int I, int J, int K, double dx) {
-    if (I < 1 || J < 1 || K < 1 || !(dx > 0.0))
-        throw StatusError{FLIPMI_ERR_INVALID_ARG, "dimensions and cell size must be greater than 0"};
-    if ((double)I * J * K > 1.5e9) throw StatusError{FLIPMI_ERR_INVALID_ARG, "grid too large"};
-    GridGeom g;
-    g.I = I; g.J = J; g.K = K; g.dx = dx; g.invdx = 1.0 / dx;
-    return g;
 }
 
 template <class T>
@@ -136,14 +124,8 @@ int flipmi_scene_create(flipmi_ctx *ctx, const flipmi_scene_desc *desc, flipmi_s
     if (!ctx || !desc || !out) return FLIPMI_ERR_INVALID_ARG;
     *out = nullptr;
     return guarded(&ctx->c, [&] {
+        validate_scene_desc(*desc);
         GridGeom g = make_geom(desc->isize, desc->jsize, desc->ksize, desc->dx);
-        if (desc->min_substeps < 1 || desc->max_substeps < desc->min_substeps || desc->max_substeps > 64)
-            throw StatusError{FLIPMI_ERR_INVALID_ARG, "substep bounds must satisfy 1 <= min <= max <= 64"};
-        if (!(desc->cfl > 0)) throw StatusError{FLIPMI_ERR_INVALID_ARG, "CFL number must be positive"};
-        if (desc->pcg_max_iterations < 1)
-            throw StatusError{FLIPMI_ERR_INVALID_ARG, "pcg_max_iterations must be positive"};
-        if (desc->pcg_precision != FLIPMI_PCG_FP64 && desc->pcg_precision != FLIPMI_PCG_MIXED)
-            throw StatusError{FLIPMI_ERR_INVALID_ARG, "unknown pcg precision"};
         std::unique_ptr<flipmi_scene> sc(new flipmi_scene());
         Scene &s = sc->s;
         s.ctx = &ctx->c;

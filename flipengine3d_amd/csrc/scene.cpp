@@ -108,6 +108,13 @@ void collision_box(int isize, int jsize, int ksize, double dx, float pos[3], dou
 // Pressure solve driver
 // ---------------------------------------------------------------------------
 
+size_t PressureStore::device_bytes() const {
+    size_t b = edge.bytes() + mask.bytes() + progress.bytes() + ticket.bytes() + irange.bytes() + scal.bytes();
+    for (int q = 0; q <= kSlotArrays; q++) b += arr[q].bytes();
+    for (int q = 0; q < kSlotArrays32; q++) b += arr32[q].bytes();
+    return b;
+}
+
 void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     SlotMap sm = make_slot_map(g.I, g.J, g.K);
     // Slot arrays carry the zero and sink groups after the last tile.
@@ -384,13 +391,8 @@ void Scene::load(const float *pos, const float *vel, int64_t count_in) {
     }
 }
 
-void Scene::initialize() {
-    if (initialized) return;
-    int64_t np = (int64_t)(pending_pos.size() / 3);
-    if (np > INT32_MAX / 2) throw StatusError{FLIPMI_ERR_INVALID_ARG, "too many particles"};
-    cap = (int)std::max<int64_t>(np, desc.max_particles);
-    alloc_all();
-    n = (int)np;
+void Scene::upload_pending() {
+    n = (int)(pending_pos.size() / 3);
     if (n > 0) {
         // Host AoS -> device SoA.
         std::vector<float> tmp((size_t)n);
@@ -404,9 +406,107 @@ void Scene::initialize() {
     }
     pending_pos.clear(); pending_vel.clear();
     pending_pos.shrink_to_fit(); pending_vel.shrink_to_fit();
+}
+
+void Scene::initialize() {
+    if (initialized) return;
+    int64_t np = (int64_t)(pending_pos.size() / 3);
+    if (np > INT32_MAX / 2) throw StatusError{FLIPMI_ERR_INVALID_ARG, "too many particles"};
+    cap = (int)std::max<int64_t>(np, desc.max_particles);
+    alloc_all();
+    upload_pending();
     // FluidSimulation::_initializeParticleRadii (fluidsimulation.cpp:2644-2650).
     radius = 0.5 * 1.0 * g.dx * std::sqrt(3.0);
     initialized = true;
+}
+
+bool Scene::rearm(const float *pos, const float *vel, int64_t count_in) {
+    if (!initialized || !pending_events.empty()) return false;
+    pending_pos.clear();
+    pending_vel.clear();
+    load(pos, vel, count_in);
+    if ((int64_t)(pending_pos.size() / 3) > cap) {
+        pending_pos.clear();
+        pending_vel.clear();
+        return false;
+    }
+    upload_pending();
+    // The state FluidSimulation has right after initialize(); the static
+    // solid, weights and near-solid grid depend only on the grid and stay.
+    current_frame = 0;
+    step_number = 0;
+    frame_dt = remaining = cur_step = 0.0;
+    output_initialized = false;
+    last_iters = 0;
+    last_ok = 1;
+    last_resid = 0.0;
+    last_rows = 0;
+    last_refinements = 0;
+    for (int q = 0; q < FLIPMI_NUM_STAGES; q++) {
+        stage_ms[q] = 0.0;
+        stage_calls[q] = 0;
+    }
+    return true;
+}
+
+size_t Scene::device_bytes() const {
+    size_t b = 0;
+    for (int a = 0; a < 6; a++) b += part[a].bytes() + part_alt[a].bytes() + sorted[a].bytes();
+    b += cell.bytes() + slot.bytes() + count.bytes() + offset.bytes() + order.bytes() + scan_tmp.bytes();
+    b += keep.bytes() + keep_pos.bytes() + rank.bytes() + new_count.bytes() + solid_removed.bytes();
+    b += speed_hist.bytes() + max_speed_bits.bytes();
+    b += solid.bytes() + wU.bytes() + wV.bytes() + wW.bytes() + wC.bytes() + sU.bytes() + sV.bytes() + sW.bytes();
+    b += near_a.bytes() + near_b.bytes() + liquid.bytes() + u.bytes() + v.bytes() + w.bytes();
+    b += su.bytes() + sv.bytes() + sw.bytes() + vu.bytes() + vv.bytes() + vw.bytes() + status.bytes();
+    if (press) b += press->device_bytes();
+    return b;
+}
+
+size_t estimate_scene_bytes(const flipmi_scene_desc &d, int64_t cap) {
+    // The allocations of Scene::alloc_all and PressureStore::alloc.
+    const size_t I = (size_t)d.isize, J = (size_t)d.jsize, K = (size_t)d.ksize;
+    const size_t nc = I * J * K, nn = (I + 1) * (J + 1) * (K + 1);
+    const size_t nU = (I + 1) * J * K, nV = I * (J + 1) * K, nW = I * J * (K + 1);
+    const size_t nf = std::max(nU, std::max(nV, nW));
+    const size_t pc = (size_t)std::max<int64_t>(cap, 1);
+    size_t b = pc * (18 * sizeof(float) + 6 * sizeof(int) + 1);
+    b += sizeof(int) * (nc + nc + 1);
+    b += sizeof(int) * std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc));
+    b += sizeof(int) * (1 + 64) + sizeof(unsigned);
+    b += sizeof(float) * nn;
+    b += sizeof(float) * 2 * (nU + nV + nW) + sizeof(float) * nc;  // weights (faces, centre), solid face velocities
+    const double dx = d.dx;
+    const size_t nci = (size_t)std::ceil((double)I * dx / (3 * dx)), ncj = (size_t)std::ceil((double)J * dx / (3 * dx)),
+                 nck = (size_t)std::ceil((double)K * dx / (3 * dx));
+    b += 2 * nci * ncj * nck;
+    b += sizeof(float) * nc + sizeof(float) * 2 * (nU + nV + nW) + (nU + nV + nW) + nf;
+    SlotMap sm = make_slot_map(d.isize, d.jsize, d.ksize);
+    const size_t ns = sm.alloc_count(), ntiles = (size_t)sm.ntj * sm.ntk;
+    b += sizeof(double) * (PressureStore::kSlotArrays * ns + 2 * 1024 + ntiles);
+    if (d.pcg_precision == FLIPMI_PCG_MIXED) b += sizeof(float) * PressureStore::kSlotArrays32 * ns;
+    b += sizeof(unsigned long long) * edge_record_count(sm);
+    b += sizeof(unsigned long long) * ntiles * (size_t)((sm.S / 4 + 63) / 64);
+    b += sizeof(unsigned) * (ntiles + 1) + sizeof(int) * 2 * ntiles + sizeof(PcgScalars);
+    return b;
+}
+
+GridGeom make_geom(int I, int J, int K, double dx) {
+    if (I < 1 || J < 1 || K < 1 || !(dx > 0.0))
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "dimensions and cell size must be greater than 0"};
+    if ((double)I * J * K > 1.5e9) throw StatusError{FLIPMI_ERR_INVALID_ARG, "grid too large"};
+    GridGeom g;
+    g.I = I; g.J = J; g.K = K; g.dx = dx; g.invdx = 1.0 / dx;
+    return g;
+}
+
+void validate_scene_desc(const flipmi_scene_desc &d) {
+    (void)make_geom(d.isize, d.jsize, d.ksize, d.dx);
+    if (d.min_substeps < 1 || d.max_substeps < d.min_substeps || d.max_substeps > 64)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "substep bounds must satisfy 1 <= min <= max <= 64"};
+    if (!(d.cfl > 0)) throw StatusError{FLIPMI_ERR_INVALID_ARG, "CFL number must be positive"};
+    if (d.pcg_max_iterations < 1) throw StatusError{FLIPMI_ERR_INVALID_ARG, "pcg_max_iterations must be positive"};
+    if (d.pcg_precision != FLIPMI_PCG_FP64 && d.pcg_precision != FLIPMI_PCG_MIXED)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "unknown pcg precision"};
 }
 
 void Scene::ensure_solid() {

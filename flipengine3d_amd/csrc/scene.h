@@ -117,6 +117,12 @@ struct Scene {
     void alloc_all();
     void load(const float *pos, const float *vel, int64_t count);
     void initialize();
+    void upload_pending();
+    // Reuses an initialized scene for a new particle set on the same grid and
+    // settings (the state right after initialize()); false if the set does not
+    // fit the particle capacity.
+    bool rearm(const float *pos, const float *vel, int64_t count);
+    size_t device_bytes() const;
     flipmi_frame_stats update(double dt);
     void step_fluid(double dt);
     double next_time_step(double dt);
@@ -168,6 +174,19 @@ struct PressureStore {
     DBuf<PcgScalars> scal;
     PressureBuffers pb{};
     void alloc(const GridGeom &g, hipStream_t s, bool mixed);
+    size_t device_bytes() const;
 };
 
+// Device bytes a scene with these settings and particle capacity allocates.
+size_t estimate_scene_bytes(const flipmi_scene_desc &d, int64_t cap);
+
+// Argument checks of flipmi_scene_create; throws StatusError.
+void validate_scene_desc(const flipmi_scene_desc &d);
+GridGeom make_geom(int I, int J, int K, double dx);
+
 }  // namespace flipmi
+
+// The opaque handle of the C ABI.
+struct flipmi_ctx {
+    flipmi::Ctx c;
+};

@@ -249,6 +249,69 @@ int flipmi_trilinear_velocity(flipmi_ctx *ctx, int isize, int jsize, int ksize, 
                               const float *u, const float *v, const float *w,
                               const float *pos, int64_t n, float *vel_out);
 
+/* ---- serving ------------------------------------------------------------
+ * Not in the reference: a device-resident request loop over independent
+ * scenes.  Requests are admitted in arrival order whenever one of max_active
+ * worker slots (a host thread and a HIP stream each) is free and the scene's
+ * device memory fits memory_budget, so scenes of mixed sizes run
+ * concurrently and new arrivals join while others are on the device.
+ * Finished scenes stay resident and are re-armed for the next request of the
+ * same grid and settings; idle ones are evicted (least recently used) when a
+ * new kind needs the memory.  A request's result does not depend on what else
+ * runs beside it. */
+typedef struct flipmi_server flipmi_server;
+
+typedef struct {
+    int max_active;                    /* scenes stepped at once, 1..64 (default 8) */
+    int64_t memory_budget;             /* device bytes for resident scenes (default 200 GiB) */
+    double particle_capacity_per_cell; /* particle capacity of a new scene per cell, so later
+                                          requests of the same kind fit (default 3) */
+} flipmi_server_desc;
+
+typedef struct {
+    flipmi_scene_desc scene;           /* grid and settings, as for flipmi_scene_create */
+    const float *pos_xyz;              /* n particles, copied at submit */
+    const float *vel_xyz;              /* may be NULL (zero velocity) */
+    int64_t n;
+    double dt;                         /* FluidSimulation::update(dt) per step */
+    int steps;                         /* number of update calls */
+} flipmi_request;
+
+typedef struct {
+    int status;                        /* FLIPMI_OK or the error that ended the request */
+    int steps_done;
+    int64_t num_particles;             /* at the end */
+    int64_t particle_substeps;         /* sum over substeps of the particles stepped */
+    int pressure_failures;             /* substeps whose pressure solve failed */
+    int reused_scene;                  /* 1 if a resident scene was re-armed */
+    double queue_ms, run_ms;           /* submit -> start, start -> done (host clock) */
+    double step_ms_p50, step_ms_p99;   /* latency of its update calls */
+} flipmi_request_report;
+
+typedef struct {
+    int64_t submitted, completed, failed, queued, active;
+    int64_t particle_substeps;         /* completed requests */
+    int64_t steps;                     /* update calls of completed requests */
+    double step_ms_p50, step_ms_p99;   /* over those update calls */
+    int64_t resident_bytes, peak_resident_bytes;
+    int64_t scenes_created, scenes_reused, scenes_evicted;
+} flipmi_server_stats;
+
+void flipmi_server_desc_default(flipmi_server_desc *desc);
+int flipmi_server_create(flipmi_ctx *ctx, const flipmi_server_desc *desc, flipmi_server **out);
+/* Queues a request; *request_id identifies it in wait/take. */
+int flipmi_server_submit(flipmi_server *server, const flipmi_request *request, int64_t *request_id);
+/* Blocks until the request is done (timeout_s < 0: no limit; else
+ * FLIPMI_ERR_TIMEOUT when it is not done in time); returns its status. */
+int flipmi_server_wait(flipmi_server *server, int64_t request_id, double timeout_s,
+                       flipmi_request_report *report);
+/* Copies the final particles of a finished request (report.num_particles
+ * triples each; either pointer may be NULL) and releases it. */
+int flipmi_server_take(flipmi_server *server, int64_t request_id, float *pos_xyz, float *vel_xyz);
+int flipmi_server_get_stats(flipmi_server *server, flipmi_server_stats *stats);
+/* Runs the queued and running requests to the end, then frees everything. */
+int flipmi_server_destroy(flipmi_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,7 +50,9 @@ def test_struct_layouts_match_header():
     import flipengine3d_amd as fa
     code = ('#include <stdio.h>\n#include "flipmi.h"\nint main(void){printf("%zu %zu %zu %zu\\n",'
             'sizeof(flipmi_scene_desc),sizeof(flipmi_frame_stats),sizeof(flipmi_stage_times),'
-            'sizeof(flipmi_pressure_report));printf("%zu\\n",sizeof(flipmi_pressure_args));return 0;}\n')
+            'sizeof(flipmi_pressure_report));printf("%zu\\n",sizeof(flipmi_pressure_args));'
+            'printf("%zu %zu %zu %zu\\n",sizeof(flipmi_server_desc),sizeof(flipmi_request),'
+            'sizeof(flipmi_request_report),sizeof(flipmi_server_stats));return 0;}\n')
     import subprocess
     import tempfile
     with tempfile.TemporaryDirectory() as d:
@@ -62,7 +64,8 @@ def test_struct_layouts_match_header():
     sizes = [int(x) for x in out]
     assert sizes == [ctypes.sizeof(fa.SceneDesc), ctypes.sizeof(fa.FrameStats),
                      ctypes.sizeof(fa.StageTimes), ctypes.sizeof(fa.PressureReport),
-                     ctypes.sizeof(fa.PressureArgs)]
+                     ctypes.sizeof(fa.PressureArgs), ctypes.sizeof(fa.ServerDesc), ctypes.sizeof(fa.Request),
+                     ctypes.sizeof(fa.RequestReport), ctypes.sizeof(fa.ServerStats)]
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
