@@ -83,6 +83,8 @@ def parse_args(argv=None):
     p.add_argument("--request-steps", type=int, default=30, help="serving: update calls per request")
     p.add_argument("--max-active", type=int, default=16, help="serving: scenes stepped at once")
     p.add_argument("--budget-gib", type=float, default=200.0, help="serving: device memory budget")
+    p.add_argument("--hw-queues", type=int, default=16,
+                   help="serving: hardware queues per process (GPU_MAX_HW_QUEUES, at most 32; 0 = as set)")
     p.add_argument("--plain-requests", type=int, default=24,
                    help="serving: requests also run one at a time on the plain FP64 path")
     p.add_argument("--copy-calibration", action="store_true",
@@ -324,6 +326,9 @@ def run_serving(args):
     back to back for comparison."""
     import threading
 
+    if args.hw_queues:
+        # Streams beyond the hardware queues share one; read at device init.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, args.hw_queues)))
     import flipengine3d_amd as fa
     t = time.perf_counter()
     reqs = serving_requests(args)
@@ -399,6 +404,7 @@ def run_serving(args):
                          f"{args.rate:g}/s, seed 4"),
             "particles_mean": sum(nps) / len(nps),
             "max_active": args.max_active,
+            "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
             "budget_gib": args.budget_gib,
             "makespan_s": makespan,
             "offered_particle_substeps_per_s": sum(nps) * args.request_steps / reqs[-1][0],
