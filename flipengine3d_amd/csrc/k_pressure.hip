@@ -589,6 +589,79 @@ struct SweepOps {
 template <class T>
 __device__ __forceinline__ T shfl_t(T v, int src) { return __shfl(v, src, 64); }
 
+// Neighbour values within a column tile without the LDS crossbar.  Lane
+// L = a + 8 b of a wave holds column (a, b); a 16-lane DPP row holds b and
+// b + 1.  The j neighbour (L - 1 forward, L + 1 backward) is a row shift by
+// one.  The k neighbour (L - 8 / L + 8) is a rotation by 8 within the row
+// of a register whose other half holds the adjacent row: gfx950's
+// permlane16/32 swaps build the rows shifted by one (forward: rows
+// [., r0, r1, r2]; backward: [r1, r2, r3, .]), and a per-lane select picks,
+// by source half, the lane's own value or the shifted row.  Lanes on the
+// tile's upstream edge take their value from the edge records instead, so
+// the values these produce there are never used.
+// One substitution step, q = (b - c1 x1 - c2 x2 - c3 x3) * inv.
+// ref_step keeps the reference's neighbour order (solveLower /
+// solveLowerTransposeInPlace, pcgsolver.h:191-221) and rounds each product
+// and difference as it does (the FP64 path).  fused_step is for the FP32
+// iteration of the mixed path, which need not match bits: fused terms, taken
+// in the order the values arrive (own previous step, j neighbour, then the k
+// neighbour, whose exchange is longest), for a shorter dependency chain.
+__device__ __forceinline__ double ref_step(double b, double c1, double x1, double c2, double x2, double c3,
+                                           double x3, double inv) {
+    double q = b;
+    q -= c1 * x1;
+    q -= c2 * x2;
+    q -= c3 * x3;
+    return q * inv;
+}
+__device__ __forceinline__ float fused_step(float b, float c_own, float own, float c_j, float xj, float c_k,
+                                            float xk, float inv) {
+    float q = __builtin_fmaf(-c_own, own, b);
+    q = __builtin_fmaf(-c_j, xj, q);
+    q = __builtin_fmaf(-c_k, xk, q);
+    return q * inv;
+}
+template <bool FWD>
+__device__ __forceinline__ unsigned lane_j(unsigned v) {
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, FWD ? 0x111 : 0x101, 0xf, 0xf, true);
+}
+template <bool FWD>
+__device__ __forceinline__ unsigned lane_k(unsigned v, bool low_half, bool row_sel) {
+    auto a = __builtin_amdgcn_permlane16_swap(v, v, false, false);  // [r0 r0 r2 r2], [r1 r1 r3 r3]
+    unsigned shifted, t;
+    if (FWD) {
+        auto b = __builtin_amdgcn_permlane32_swap(a[1], a[1], false, false);  // [r1 r1 r1 r1]
+        shifted = row_sel ? b[0] : a[0];  // row 2 takes r1
+        t = low_half ? v : shifted;
+    } else {
+        auto c = __builtin_amdgcn_permlane32_swap(a[0], a[0], false, false);  // [r2 r2 r2 r2]
+        shifted = row_sel ? c[1] : a[1];  // row 1 takes r2
+        t = low_half ? shifted : v;
+    }
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)t, 0x128, 0xf, 0xf, true);  // row_ror:8
+}
+template <bool FWD>
+__device__ __forceinline__ float lane_j(float v) {
+    return __builtin_bit_cast(float, lane_j<FWD>(__builtin_bit_cast(unsigned, v)));
+}
+template <bool FWD>
+__device__ __forceinline__ float lane_k(float v, bool low_half, bool row_sel) {
+    return __builtin_bit_cast(float, lane_k<FWD>(__builtin_bit_cast(unsigned, v), low_half, row_sel));
+}
+template <bool FWD>
+__device__ __forceinline__ double lane_j(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = lane_j<FWD>((unsigned)u), hi = lane_j<FWD>((unsigned)(u >> 32));
+    return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+template <bool FWD>
+__device__ __forceinline__ double lane_k(double v, bool low_half, bool row_sel) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = lane_k<FWD>((unsigned)u, low_half, row_sel);
+    const unsigned hi = lane_k<FWD>((unsigned)(u >> 32), low_half, row_sel);
+    return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+
 // The four steps of one slot group of this lane (16 bytes per access).
 __device__ __forceinline__ void load_group(const float *A, size_t base, float (&e)[4]) {
     float4 v = *reinterpret_cast<const float4 *>(A + base);
@@ -625,6 +698,10 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
     const int up_k = FWD ? (kt > 0 ? tile - ntj : -1) : (kt + 1 < ntk ? tile + ntj : -1);
     const int src_j = (FWD ? lane - 1 : lane + 1) & 63;  // neighbour lane along j
     const int src_k = (FWD ? lane - 8 : lane + 8) & 63;  // neighbour lane along k
+    // Lane position in its 16-lane DPP row and the row that takes the
+    // permlane32 result (lane_k).
+    const bool low_half = (lane & 15) < 8;
+    const bool row_sel = (lane >> 4) == (FWD ? 2 : 1);
     const bool edge_j = FWD ? a == 0 : a == 7;            // reads the upstream j tile
     const bool edge_k = FWD ? b == 0 : b == 7;
     // Edge cells (read by the downstream neighbours) publish one record per
@@ -684,13 +761,15 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
         auto chunk_group = [&](int u0) -> int { return FWD ? (u0 >> 2) : ((S - 8 - u0) >> 2); };
         // Step s of a chunk is element m of group h: forward s = 4h + m,
         // backward s = 7 - (4h + m).
-        auto load_chunk = [&](int u0, T (&v)[NV][C]) {
+        // act[h] keeps each group's mask bit for the store of the same chunk.
+        auto load_chunk = [&](int u0, T (&v)[NV][C], bool (&act)[2]) {
             const int gl = chunk_group(u0);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 // A group without rows reads the arrays' zero group instead
                 // (same instructions; the 2 KB it spans stay in cache).
-                const size_t base = active(gl + h) ? gbase(gl + h) : zbase;
+                act[h] = active(gl + h);
+                const size_t base = act[h] ? gbase(gl + h) : zbase;
                 T e[NV][4];
                 load_group(FWD ? op.in : op.out, base, e[0]);
                 load_group(op.c1, base, e[1]);
@@ -706,7 +785,7 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
                 }
             }
         };
-        auto store_chunk = [&](int u0, const T (&qv)[C]) {
+        auto store_chunk = [&](int u0, const T (&qv)[C], const bool (&act)[2]) {
             const int gl = chunk_group(u0);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -715,7 +794,7 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
                 for (int m = 0; m < 4; m++) e[m] = qv[FWD ? 4 * h + m : 7 - (4 * h + m)];
                 // The output of a group without rows stays zero: its values
                 // (all zero) go to the sink group instead.
-                store_group(op.out, active(gl + h) ? gbase(gl + h) : sbase, e);
+                store_group(op.out, act[h] ? gbase(gl + h) : sbase, e);
             }
         };
         // Whether this lane's record of chunk u0 is needed: the producer tile
@@ -748,11 +827,12 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
         unsigned long long recj[D][2], reck[D][2];
         load_recs(tb, recj[0], reck[0]);
         T buf[D][NV][C];
+        bool actr[D][2];
         const int last = te - C;  // last chunk; loads past the end repeat it
 #pragma unroll
         for (int d = 0; d < D; d++) {
             int uf = tb + d * C;
-            load_chunk(uf < last ? uf : last, buf[d]);
+            load_chunk(uf < last ? uf : last, buf[d], actr[d]);
             __builtin_amdgcn_sched_barrier(0);
         }
         T cur = T(0);
@@ -815,29 +895,36 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
 #pragma unroll
                 for (int s = 0; s < C; s++) {
                     // Values produced one step earlier by the neighbouring lanes.
-                    const T nj_cur = shfl_t(cur, src_j), nk_cur = shfl_t(cur, src_k);
+                    // FP32: DPP / permlane exchange; FP64: the LDS crossbar
+                    // (two dwords per value make the permlane route longer).
+                    T nj_cur, nk_cur;
+                    if constexpr (sizeof(T) == 4) {
+                        nj_cur = lane_j<FWD>(cur);
+                        nk_cur = lane_k<FWD>(cur, low_half, row_sel);
+                    } else {
+                        nj_cur = shfl_t(cur, src_j);
+                        nk_cur = shfl_t(cur, src_k);
+                    }
                     const T qy = edge_j ? evj[s] : nj_cur;
                     const T qz = edge_k ? evk[s] : nk_cur;
-                    T q = v[0][s];
-                    if (FWD) {
-                        q -= v[1][s] * qz;
-                        q -= v[2][s] * qy;
-                        q -= v[3][s] * cur;
-                        q *= v[4][s];
+                    // c1..c3 multiply the -z, -y, -x neighbours (forward) or
+                    // the +x, +y, +z ones (backward).
+                    T q;
+                    if constexpr (sizeof(T) == 8) {
+                        q = FWD ? ref_step(v[0][s], v[1][s], qz, v[2][s], qy, v[3][s], cur, v[4][s])
+                                : ref_step(v[0][s], v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
                     } else {
-                        q -= v[1][s] * cur;
-                        q -= v[2][s] * qy;
-                        q -= v[3][s] * qz;
-                        q *= v[4][s];
-                        dot_acc += (double)q * (double)v[NV - 1][s];
+                        q = FWD ? fused_step(v[0][s], v[3][s], cur, v[2][s], qy, v[1][s], qz, v[4][s])
+                                : fused_step(v[0][s], v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
                     }
+                    if (!FWD) dot_acc = __builtin_fma((double)q, (double)v[NV - 1][s], dot_acc);
                     qv[s] = q;
                     if (pub) Record<T>::put(pubp + s * kRecSlots * RW, q, epoch);
                     cur = q;
                 }
-                store_chunk(uc, qv);
+                store_chunk(uc, qv, actr[d]);
                 const int uf = uc + D * C;
-                load_chunk(uf < last ? uf : last, buf[d]);
+                load_chunk(uf < last ? uf : last, buf[d], actr[d]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
