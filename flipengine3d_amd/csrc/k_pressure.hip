@@ -325,9 +325,10 @@ __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const Gri
         ub = ue = 0;
         return;
     }
+    // Cell (i, j, k) sits at step t = i + a + b with a + b in [0, 14].
     if (reverse) {
-        ub = g.I - 1 - imax;
-        ue = g.I + 14 - imin;
+        ub = pb.sm.S - 15 - imax;
+        ue = pb.sm.S - imin;
     } else {
         ub = imin;
         ue = imax + 15;

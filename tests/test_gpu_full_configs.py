@@ -1,0 +1,90 @@
+"""The workloads at their full sizes.
+
+* 128^3 drop into a pool (5.2 M particles, FP64 PCG): the first substep's
+  liquid field, hence the fluid/air labels, is bit-exact against the oracle
+  on the same particles; over 200 substeps of dt = 1/120 s every pressure
+  solve succeeds, no particle leaves the collision box, nothing becomes
+  NaN, and the particle count only falls by removals.
+* 256x128x128 double dam break (9.4 M particles): one substep with the
+  mixed-precision PCG against the FP64 path on the same scene.  Stated
+  tolerance: MAC velocities within 1e-5 of the largest face speed, particle
+  velocities within 1e-5 m/s times max(1, |v|max), positions within
+  1e-4 dx (a few float ulps at the far end of the domain: the two solves
+  round differently).
+"""
+
+import numpy as np
+import pytest
+
+import flipengine3d_amd as fa
+from flipengine3d_amd import scenes
+from tests.helpers import bits_equal, ndiff, radius
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(gpu, s, precision, substeps=(1, 1)):
+    sim = fa.FluidSimulation(s.isize, s.jsize, s.ksize, s.dx, ctx=gpu, precision=precision)
+    sim.addBodyForce(0.0, -9.81, 0.0)
+    sim.setMinTimeStepsPerFrame(substeps[0])
+    sim.setMaxTimeStepsPerFrame(substeps[1])
+    sim.loadMarkerParticleData(s.positions, s.velocities)
+    sim.initialize()
+    return sim
+
+
+@pytest.fixture(scope="module")
+def pool_drop():
+    return scenes.pool_drop(128, 128, 128, 1.0 / 128, seed=2)
+
+
+def test_pool_drop_128_first_substep_labels_bitexact(gpu, oracle, pool_drop):
+    s = pool_drop
+    sim = _sim(gpu, s, fa.PCG_FP64)
+    st = sim.update(1 / 120)
+    assert st.pressure_ok == 1
+    phi = sim.grid(0)  # the liquid field of the substep's classification
+    sim.close()
+    solid, _ = oracle.domain_solid(s.isize, s.jsize, s.ksize, s.dx)
+    want = oracle.liquid_sdf(s.isize, s.jsize, s.ksize, s.dx, s.positions, radius(s.dx), solid)
+    assert bits_equal(phi, want), f"ndiff={ndiff(phi, want)}"
+    assert int(np.count_nonzero(phi < 0)) == int(np.count_nonzero(want < 0))
+
+
+def test_pool_drop_128_200_substeps(gpu, pool_drop):
+    s = pool_drop
+    sim = _sim(gpu, s, fa.PCG_FP64)
+    n0 = sim.getNumMarkerParticles()
+    n_prev = n0
+    for step in range(200):
+        st = sim.update(1 / 120)
+        assert st.substeps == 1
+        assert st.pressure_ok == 1, f"pressure solve failed at substep {step}"
+        assert st.num_particles <= n_prev
+        n_prev = st.num_particles
+    pos, vel = sim.particles()
+    sim.close()
+    assert pos.shape[0] == n_prev and n_prev >= 0.99 * n0
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+    # The collision box: the wall box shrunk by the 0.1 dx solid buffer.
+    lo = 1.0 * s.dx
+    hi = np.array([s.isize, s.jsize, s.ksize]) * s.dx - lo
+    assert (pos >= lo - 1e-6).all() and (pos <= hi + 1e-6).all()
+
+
+def test_double_dam_256_mixed_substep_against_fp64(gpu):
+    s = scenes.double_dam_break(256, 128, 128, 1.0 / 128, seed=3)
+    out = {}
+    for prec in (fa.PCG_FP64, fa.PCG_MIXED):
+        sim = _sim(gpu, s, prec)
+        st = sim.update(1 / 240)
+        assert st.pressure_ok == 1
+        out[prec] = (sim.grid(2), sim.grid(3), sim.grid(4)) + sim.particles()
+        sim.close()
+    a, b = out[fa.PCG_FP64], out[fa.PCG_MIXED]
+    vmax = max(float(np.abs(g).max()) for g in a[:3])
+    for ga, gb in zip(a[:3], b[:3]):
+        assert np.abs(ga.astype(np.float64) - gb).max() <= 1e-5 * vmax
+    assert a[3].shape == b[3].shape
+    assert np.abs(a[3] - b[3]).max() <= 1e-4 * s.dx
+    assert np.abs(a[4] - b[4]).max() <= 1e-5 * max(1.0, float(np.abs(a[4]).max()))

@@ -175,3 +175,25 @@ def test_pressure_pockets_and_solid_velocities(gpu, oracle, precision):
     vmax = max(float(np.abs(g).max()) for g in po["mac"])
     for a, b in zip(pg["mac"], po["mac"]):
         assert np.abs(a.astype(np.float64) - b).max() <= 1e-5 * vmax
+
+
+@pytest.mark.parametrize("dims,box", [((19, 16, 16), (0.6, 0.0, 0.0, 0.95, 0.5, 1.0)),
+                                      ((50, 12, 20), (0.55, 0.0, 0.0, 1.0, 0.7, 1.0)),
+                                      ((128, 24, 16), (0.45, 0.3, 0.0, 0.6, 0.8, 1.0))])
+def test_mic0_rows_far_along_i(gpu, oracle, dims, box):
+    # Rows only at large i, on grids whose step count S (I + 14 rounded up to
+    # 32) exceeds I + 14 by up to 31: the backward sweep's step range must be
+    # mirrored through S, or it misses the rows and leaves z = y there.
+    from flipengine3d_amd import scenes
+    I, J, K = dims
+    dx = 1.0 / max(dims)
+    s = scenes.dam_break(I, J, K, dx, seed=9, box=box)
+    solid, weights = oracle.domain_solid(I, J, K, dx)
+    liquid = oracle.liquid_sdf(I, J, K, dx, s.positions, radius(dx), solid)
+    n = int(np.count_nonzero(liquid[1:-1, 1:-1, 1:-1] < 0))
+    assert n > 0
+    r = np.random.default_rng(5).standard_normal(n)
+    ig, zg = gpu.mic0_apply(I, J, K, dx, 1 / 60, liquid, weights, r)
+    io, zo = oracle.mic0_apply(I, J, K, dx, 1 / 60, liquid, weights, r)
+    assert bits_equal(ig, io)
+    assert bits_equal(zg, zo), f"ndiff={ndiff(zg, zo)}"
