@@ -87,6 +87,8 @@ def parse_args(argv=None):
                    help="serving: hardware queues per process (GPU_MAX_HW_QUEUES, at most 32; 0 = as set)")
     p.add_argument("--plain-requests", type=int, default=24,
                    help="serving: requests also run one at a time on the plain FP64 path")
+    p.add_argument("--workload", choices=["dam_break_64", "pool_drop_128", "large_pool_512"], default=None,
+                   help="one of the other workloads instead of the headline (see run_workload)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 256 MiB device to device 3 times (known bytes, to scale HBM counters)")
     p.add_argument("--verbose", action="store_true")
@@ -141,14 +143,14 @@ class _StdoutToStderr:
         return False
 
 
-def cpu_baseline(scene, args):
+def cpu_baseline(scene, args, substeps=(1, 1)):
     """Times the reference engine (or, where it was not built, the CPU
-    oracle) on the same scene for whole substeps until --cpu-seconds pass."""
+    oracle) on the same scene for whole updates until --cpu-seconds pass."""
     with _StdoutToStderr():
-        return _cpu_baseline(scene, args)
+        return _cpu_baseline(scene, args, substeps)
 
 
-def _cpu_baseline(scene, args):
+def _cpu_baseline(scene, args, substeps=(1, 1)):
     from oracle import bindings
     I, J, K = args.size
     if bindings.ref_available():
@@ -161,7 +163,7 @@ def _cpu_baseline(scene, args):
         kind = "port"
     h = lib.sim(I, J, K, scene.dx)
     h.add_body_force(0.0, -9.81, 0.0)
-    h.set_substeps(1, 1)
+    h.set_substeps(*substeps)
     h.load_particles(scene.positions, scene.velocities)
     h.initialize()
     done = 0
@@ -172,9 +174,9 @@ def _cpu_baseline(scene, args):
         t0 = time.perf_counter()
         h.update(args.dt)
         elapsed += time.perf_counter() - t0
-        work += n
+        work += n * h.substeps()
         done += 1
-        log(f"cpu substep {done}: {n} particles, {elapsed:.1f} s total", args)
+        log(f"cpu update {done}: {n} particles, {elapsed:.1f} s total", args)
         if elapsed >= args.cpu_seconds or done >= 8:
             break
     h.close()
@@ -183,7 +185,7 @@ def _cpu_baseline(scene, args):
         "unit": "particle-substeps/s",
         "cores": threads,
         "kind": kind,
-        "sample": (f"{done} substep(s) of dt={args.dt:.6g} s of the same {I}x{J}x{K} double dam break "
+        "sample": (f"{done} update(s) of dt={args.dt:.6g} s of the same {I}x{J}x{K} {scene.name.replace('_', ' ')} "
                    f"({scene.num_particles} particles), FP64 PCG, {threads} thread(s), {elapsed:.1f} s"),
     }
 
@@ -292,6 +294,85 @@ def copy_calibration():
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}), flush=True)
     return 0
+
+
+# The other workloads: (grid, dx, scene builder, PCG precision, tolerance,
+# substep bounds, dt per update, updates).
+WORKLOADS = {
+    # 64^3 dam break: cuboid [0,0.4]x[0,0.5]x[0,1], 30 fps frames whose substeps
+    # follow the CFL condition (1..6), 60 frames, FP64 PCG, seed 1.
+    "dam_break_64": ((64, 64, 64), 1 / 64, lambda sc: sc.dam_break(64, 64, 64, 1 / 64, seed=1), "fp64", 1e-9,
+                     (1, 6), 1 / 30, 60),
+    # 128^3 drop into a pool: bottom 30 % plus a sphere of radius 0.15 at
+    # (0.5, 0.7, 0.5), 200 substeps of 1/120 s, FP64 PCG, seed 2.
+    "pool_drop_128": ((128, 128, 128), 1 / 128, lambda sc: sc.pool_drop(128, 128, 128, 1 / 128, seed=2), "fp64",
+                      1e-9, (1, 1), 1 / 120, 200),
+    # 512x256x256 pool, 25 % deep, mixed PCG at 1e-6 relative, 10 substeps of
+    # 1/480 s, seed 5 (the inflow jet of the full workload is not built).
+    "large_pool_512": ((512, 256, 256), 1 / 256,
+                       lambda sc: sc.pool_drop(512, 256, 256, 1 / 256, seed=5, depth=0.25, radius=0.0), "mixed",
+                       1e-6, (1, 1), 1 / 480, 10),
+}
+
+
+def run_workload(args):
+    """One of WORKLOADS on the GPU (particle-substeps/s, step latency, PCG
+    iterations) with the reference engine timed on a bounded sample of it."""
+    import flipengine3d_amd as fa
+    from flipengine3d_amd import scenes
+    (I, J, K), dx, build, precision, tol, substeps, dt, updates = WORKLOADS[args.workload]
+    t = time.perf_counter()
+    scene = build(scenes)
+    log(f"{args.workload}: {scene.num_particles} particles ({time.perf_counter() - t:.1f} s)", args)
+    sim = fa.FluidSimulation(I, J, K, dx, precision=fa.PCG_MIXED if precision == "mixed" else fa.PCG_FP64)
+    sim.addBodyForce(0.0, -9.81, 0.0)
+    sim.setMinTimeStepsPerFrame(substeps[0])
+    sim.setMaxTimeStepsPerFrame(substeps[1])
+    sim.setPressureSolverTolerance(tol)
+    sim.loadMarkerParticleData(scene.positions, scene.velocities)
+    sim.initialize()
+    lat, work, nsub, iters, failures = [], 0, 0, 0, 0
+    n = sim.getNumMarkerParticles()
+    sim.ctx.synchronize()
+    t0 = time.perf_counter()
+    for q in range(updates):
+        ts = time.perf_counter()
+        st = sim.update(dt)
+        lat.append((time.perf_counter() - ts) * 1e3 / max(st.substeps, 1))
+        work += n * st.substeps
+        nsub += st.substeps
+        iters += st.pcg_iterations
+        failures += 0 if st.pressure_ok else 1
+        n = st.num_particles
+        if (q + 1) % 20 == 0:
+            log(f"{args.workload}: {q + 1} updates, {time.perf_counter() - t0:.1f} s", args)
+    elapsed = time.perf_counter() - t0
+    sim.close()
+    lat.sort()
+    out = {
+        "metric": f"particle-substeps/s ({args.workload})",
+        "value": work / elapsed,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "ms_per_substep": elapsed * 1e3 / nsub,
+        "p50_ms": lat[len(lat) // 2],
+        "p99_ms": lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))],
+        "dtype": "f32 particles/grids, f64 PCG" if precision == "fp64" else
+                 "f32 particles/grids and PCG, f64 dots/x/true residual",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "grid": [I, J, K], "dx": dx, "particles": scene.num_particles,
+                   "precision": precision, "tolerance": tol, "updates": updates, "substeps": nsub, "dt": dt,
+                   "pcg_iterations_last_substep_mean": iters / updates, "failed_pressure_solves": failures},
+    }
+    if args.cpu_baseline and scene.num_particles <= 20_000_000:
+        a = argparse.Namespace(**vars(args))
+        a.size, a.dt = [I, J, K], dt
+        out["cpu_baseline"] = cpu_baseline(scene, a, substeps=substeps)
+        out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    else:
+        out["cpu_baseline"] = None
+    return out
 
 
 SERVING_GRIDS = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 64, 64)]
@@ -439,6 +520,9 @@ def main(argv=None):
         return copy_calibration()
     if args.serving:
         print(json.dumps(run_serving(args)), flush=True)
+        return 0
+    if args.workload:
+        print(json.dumps(run_workload(args)), flush=True)
         return 0
 
     import flipengine3d_amd as fa
