@@ -187,8 +187,13 @@ enum {
 };
 bool pcg_piece_used(int piece, bool mixed);
 int pcg_piece_at(int q);  // piece executed q-th within an iteration
+// With kt.start set, the kernel's dispatch records its start and end in the
+// two events.
+struct KernelTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
 hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBuffers &pb,
-                            hipStream_t s);
+                            hipStream_t s, const KernelTiming &kt);
 // p is in slot order (pb.sm).
 hipError_t launch_pressure_apply(const GridGeom &g, const SlotMap &sm, double dt, const float *liquid,
                                  const float *wU, const float *wV, const float *wW,

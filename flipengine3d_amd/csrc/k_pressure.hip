@@ -22,6 +22,8 @@
 // order, so the preconditioner is the reference's bit for bit.
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "device_math.h"
 #include "flipmi_kernels.h"
 
@@ -315,6 +317,21 @@ __device__ __forceinline__ int take_tile(PressureBuffers &pb, bool reverse, int 
     return jt + ntj * kt;
 }
 
+// The tile of a ticket (the order of take_tile), for workgroups whose
+// ticket was taken by one thread and shared through LDS.
+__device__ __forceinline__ int tile_of_ticket(const PressureBuffers &pb, bool reverse, int ticket, int &jt,
+                                              int &kt) {
+    const int ntj = pb.sm.ntj, ntk = pb.sm.ntk;
+    if (reverse) {
+        jt = ntj - 1 - ticket % ntj;
+        kt = ntk - 1 - ticket / ntj;
+    } else {
+        jt = ticket % ntj;
+        kt = ticket / ntj;
+    }
+    return jt + ntj * kt;
+}
+
 // Processing range [ub, ue) of a tile in its march order u (forward: u = t;
 // backward: u = S - 1 - t): the steps that hold at least one of its rows.
 __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const GridGeom &g, int tile,
@@ -567,16 +584,54 @@ __global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
 // T = double is the reference's arithmetic; T = float is the mixed-precision
 // inner solve (FP32 values, FP64 dot accumulation).
 //
-// One wave per tile marches over u (forward: step t = u; backward: t =
-// S - 1 - u) in chunks of kChunk = 8 steps (two groups of four steps of the
-// slot layout).  The operands of D chunks are in flight at once in a ring of
-// register buffers, so the march is not held up by memory latency.  A cell
-// at march step u needs its upstream neighbour's value from that tile's march
-// step u + 7, published there as record (tile, u, lane-within-edge).  Lane L
-// of the consumer loads the record for step u0 + 7 + L / 8, edge lane L % 8,
-// one chunk ahead; each step proceeds as soon as its own records carry the
-// launch epoch, and the values reach the edge lanes by shuffles.
-constexpr int kChunk = 8;  // 8 steps x 8 edge lanes = one record per lane
+// One workgroup of two waves per tile.  The compute wave marches over u
+// (forward: step t = u; backward: t = S - 1 - u) in chunks of kChunk = 8
+// steps (two groups of four steps of the slot layout).  The operands of D
+// chunks are in flight at once in a ring of register buffers, so the march
+// is not held up by memory latency.  A cell at march step u needs its
+// upstream neighbour's value from that tile's march step u + 7, published
+// there as record (tile, u, lane-within-edge).  The poller wave fetches the
+// records of each chunk (lane L: step u0 + 7 + L / 8, edge lane L % 8) until
+// they carry the launch epoch and hands the values over in LDS, laid out so
+// that each edge lane of the compute wave reads its eight values with two
+// 16-byte LDS loads.  The compute wave thus never waits on global memory
+// for its neighbours: a chunk that is not there yet costs an LDS poll, not
+// a drain of its own operand loads and record stores.
+constexpr int kChunk = 8;  // 8 steps x 8 edge lanes = one record per poller lane
+constexpr int kMailChunks = 4;  // chunks the poller may deliver ahead
+
+// LDS mailbox of a sweep workgroup.
+template <class T>
+struct SweepMailbox {
+    alignas(16) T val[kMailChunks][2][kTile][kChunk];  // [ring][j / k edge][edge lane][step]
+    int ready;     // chunks delivered by the poller
+    int consumed;  // chunks read by the compute wave
+    int tile;      // the workgroup's ticket
+    int skip;      // the solve has finished: nothing to do
+};
+
+__device__ __forceinline__ int lds_acquire(int *p) {
+    return __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+__device__ __forceinline__ void lds_release(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Eight consecutive mailbox values of one edge lane.
+__device__ __forceinline__ void read_mail(const float *p, float (&e)[kChunk]) {
+    const float4 v0 = *reinterpret_cast<const float4 *>(p);
+    const float4 v1 = *reinterpret_cast<const float4 *>(p + 4);
+    e[0] = v0.x; e[1] = v0.y; e[2] = v0.z; e[3] = v0.w;
+    e[4] = v1.x; e[5] = v1.y; e[6] = v1.z; e[7] = v1.w;
+}
+__device__ __forceinline__ void read_mail(const double *p, double (&e)[kChunk]) {
+#pragma unroll
+    for (int q = 0; q < kChunk; q += 2) {
+        const double2 v = *reinterpret_cast<const double2 *>(p + q);
+        e[q] = v.x; e[q + 1] = v.y;
+    }
+}
 
 template <class T>
 struct SweepOps {
@@ -681,22 +736,105 @@ __device__ __forceinline__ void store_group(double *A, size_t base, const double
     *reinterpret_cast<double2 *>(A + base + 2) = make_double2(e[2], e[3]);
 }
 
+// The poller wave of a sweep workgroup: for each chunk of the tile's march,
+// polls the upstream records until every needed one carries this launch's
+// epoch, then delivers the values to the mailbox.  Records outside an
+// upstream tile's march range are never written and read as zero.  On a
+// timeout it raises the abort flag and releases the compute wave.
+template <class T, bool FWD>
+__device__ void sweep_poller(PressureBuffers &pb, SweepMailbox<T> &mb, int tile, int up_j, int up_k, int tb,
+                             int nchunks, int ubj, int uej, int ubk, int uek, int lane) {
+    constexpr int C = kChunk, RW = Record<T>::W;
+    const int S = pb.sm.S;
+    const unsigned epoch = pb.epoch;
+    const int rs = lane >> 3, re = lane & 7;  // step within the chunk, edge lane
+    const int aj = FWD ? 7 : 0, bk = FWD ? 7 : 0;
+    const int rslot_j = re, rslot_k = re == aj ? bk : 8 + re;
+    // Tiles without an upstream neighbour read their own (never needed) records.
+    const int rj_tile = up_j >= 0 ? up_j : tile, rk_tile = up_k >= 0 ? up_k : tile;
+    auto rec_ptr = [&](int rt, int slot, int uu) -> const unsigned long long * {
+        uu = uu < S ? uu : S - 1;
+        return pb.edge + ((((size_t)rt * S + (size_t)uu) * kRecSlots + slot) * RW);
+    };
+    auto fetch = [&](const unsigned long long *q, unsigned long long (&w)[2]) {
+        w[0] = ld_dev(q);
+        w[1] = RW == 2 ? ld_dev(q + 1) : 0ull;
+    };
+    auto give_up = [&]() {
+        if (lane == 0) {
+            st_dev(&pb.sc->abort, 1);
+            lds_release(&mb.ready, 1 << 29);
+        }
+    };
+    for (int ci = 0; ci < nchunks; ci++) {
+        const int slot = ci % kMailChunks;
+        // Chunk ci overwrites chunk ci - kMailChunks of the ring.
+        unsigned spins = 0;
+        while (ci - lds_acquire(&mb.consumed) >= kMailChunks) {
+            if (++spins > kSpinLimit) return give_up();
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const int uu = tb + ci * C + 7 + rs;
+        const bool nj = (up_j >= 0) & (uu >= ubj) & (uu < uej);
+        const bool nk = (up_k >= 0) & (uu >= ubk) & (uu < uek);
+        T vj = T(0), vk = T(0);
+        if (__ballot(nj | nk) != 0ull) {
+            unsigned long long rj[2], rk[2];
+            spins = 0;
+            for (;;) {
+                // The records and the abort flag in one round trip.
+                fetch(rec_ptr(rj_tile, rslot_j, uu), rj);
+                fetch(rec_ptr(rk_tile, rslot_k, uu), rk);
+                const int aborted = ld_dev(&pb.sc->abort);
+                drain_memory();
+                const int bad = ((int)nj & (int)!Record<T>::ok(rj, epoch)) |
+                                ((int)nk & (int)!Record<T>::ok(rk, epoch));
+                if (__ballot(bad) == 0ull) break;
+                if (aborted != 0 || ++spins > kSpinLimit) return give_up();
+                __builtin_amdgcn_s_sleep(1);
+            }
+            vj = nj ? Record<T>::value(rj) : T(0);
+            vk = nk ? Record<T>::value(rk) : T(0);
+        }
+        mb.val[slot][0][re][rs] = vj;
+        mb.val[slot][1][re][rs] = vk;
+        // The release orders the value stores of every lane before the count.
+        if (lane == 0) lds_release(&mb.ready, ci + 1);
+    }
+}
+
 template <class T, int DIR, int D>
-__global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, SweepOps<T> op,
-                                              int dotmode) {
+__global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, SweepOps<T> op,
+                                               int dotmode) {
     constexpr int C = kChunk;
     constexpr bool FWD = DIR == kForward;
     constexpr int NV = FWD ? 5 : 6;
     constexpr int RW = Record<T>::W;
-    if (ld_dev(&pb.sc->done) != 0) return;
-    const int lane = threadIdx.x;
+    __shared__ SweepMailbox<T> mb;
+    if (threadIdx.x == 0) {
+        const int skip = ld_dev(&pb.sc->done) != 0;
+        mb.skip = skip;
+        mb.tile = skip ? 0 : (int)atomicAdd(pb.ticket, 1u);
+        mb.ready = 0;
+        mb.consumed = 0;
+    }
+    __syncthreads();
+    if (mb.skip) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const SlotMap sm = pb.sm;
     const int S = sm.S, ntj = sm.ntj, ntk = sm.ntk;
     int jt, kt;
-    const int tile = take_tile(pb, !FWD, jt, kt);
+    const int tile = tile_of_ticket(pb, !FWD, __builtin_amdgcn_readfirstlane(mb.tile), jt, kt);
     const int a = lane & 7, b = lane >> 3;
     const int up_j = FWD ? (jt > 0 ? tile - 1 : -1) : (jt + 1 < ntj ? tile + 1 : -1);
     const int up_k = FWD ? (kt > 0 ? tile - ntj : -1) : (kt + 1 < ntk ? tile + ntj : -1);
+    int tb, te, ubj = 0, uej = 0, ubk = 0, uek = 0;
+    tile_urange(pb, g, tile, !FWD, tb, te);
+    if (up_j >= 0) tile_urange(pb, g, up_j, !FWD, ubj, uej);
+    if (up_k >= 0) tile_urange(pb, g, up_k, !FWD, ubk, uek);
+    const int nchunks = te > tb ? (te - tb) / C : 0;
+    if (wave == 1) return sweep_poller<T, FWD>(pb, mb, tile, up_j, up_k, tb, nchunks, ubj, uej, ubk, uek, lane);
+
     const int src_j = (FWD ? lane - 1 : lane + 1) & 63;  // neighbour lane along j
     const int src_k = (FWD ? lane - 8 : lane + 8) & 63;  // neighbour lane along k
     // Lane position in its 16-lane DPP row and the row that takes the
@@ -717,13 +855,6 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
     // own output) so the hot loop has no data-dependent branches.
     const T *dotsrc = dot ? op.dotvec : op.out;
 
-    int tb, te, ubj = 0, uej = 0, ubk = 0, uek = 0;
-    tile_urange(pb, g, tile, !FWD, tb, te);
-    if (up_j >= 0) tile_urange(pb, g, up_j, !FWD, ubj, uej);
-    if (up_k >= 0) tile_urange(pb, g, up_k, !FWD, ubk, uek);
-    const int rs = lane >> 3, re = lane & 7;  // record this lane loads: step offset, edge lane
-    // Tiles without an upstream neighbour read their own (never needed) records.
-    const int rj_tile = up_j >= 0 ? up_j : tile, rk_tile = up_k >= 0 ? up_k : tile;
 
     // Active-group bits of this tile (up to 256 groups; longer tiles treat
     // every group as active).  Four scalar words, selected with uniform
@@ -798,35 +929,7 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
                 store_group(op.out, act[h] ? gbase(gl + h) : sbase, e);
             }
         };
-        // Whether this lane's record of chunk u0 is needed: the producer tile
-        // processes the producer step.
-        auto need_rec = [&](int up, int u0, int ub, int ue) -> bool {
-            int uu = u0 + 7 + rs;
-            return (int)(up >= 0) & (int)(uu >= ub) & (int)(uu < ue);
-        };
-        // Record of lane L for the chunk at u0: step u0 + 7 + L / 8 of the
-        // upstream tile, edge lane L % 8.
-        const int rslot_j = re;
-        const int rslot_k = re == aj ? bk : 8 + re;
-        auto rec_ptr = [&](int rt, int slot, int u0) -> const unsigned long long * {
-            int uu = u0 + 7 + rs;
-            uu = uu < S ? uu : S - 1;
-            return pb.edge + ((((size_t)rt * S + (size_t)uu) * kRecSlots + slot) * RW);
-        };
-        auto fetch = [&](const unsigned long long *q, unsigned long long (&w)[2]) {
-            w[0] = ld_dev(q);
-            w[1] = RW == 2 ? ld_dev(q + 1) : 0ull;
-        };
-        auto load_recs = [&](int u0, unsigned long long (&rj)[2], unsigned long long (&rk)[2]) {
-            fetch(rec_ptr(rj_tile, rslot_j, u0), rj);
-            fetch(rec_ptr(rk_tile, rslot_k, u0), rk);
-        };
-
-        // Issue order in the prologue matches the loop's steady state (records
-        // of a chunk before the operands issued after them), so the compiler's
-        // outstanding-load accounting at the loop head stays tight.
-        unsigned long long recj[D][2], reck[D][2];
-        load_recs(tb, recj[0], reck[0]);
+        // The operands of the first D chunks.
         T buf[D][NV][C];
         bool actr[D][2];
         const int last = te - C;  // last chunk; loads past the end repeat it
@@ -837,58 +940,33 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
             __builtin_amdgcn_sched_barrier(0);
         }
         T cur = T(0);
+        int rdy = 0;  // chunks known to be in the mailbox
         for (int u0 = tb; u0 < te; u0 += D * C) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
                 // Ranges are whole multiples of D chunks: no exit inside the
                 // ring (an exit here costs the compiler its load accounting).
                 const int uc = u0 + d * C;
-                unsigned long long(&rj)[2] = recj[d];
-                unsigned long long(&rk)[2] = reck[d];
-                const int un = uc + C < te ? uc + C : uc;
-                load_recs(un, recj[(d + 1) % D], reck[(d + 1) % D]);
-                // Keep the prefetches where they are issued: the scheduler would
-                // otherwise sink them towards their use to save registers.
-                __builtin_amdgcn_sched_barrier(0);
-                const bool nj = need_rec(up_j, uc, ubj, uej);
-                const bool nk = need_rec(up_k, uc, ubk, uek);
-                const int bad = ((int)nj & (int)!Record<T>::ok(rj, epoch)) |
-                                ((int)nk & (int)!Record<T>::ok(rk, epoch));
-                // Slow path (the upstream tile is behind): poll this chunk's
-                // records into separate registers until they carry this
-                // launch's epoch, then drain.  The hot path's registers are
-                // not written here, so the compiler's outstanding-load
-                // accounting for them stays exact.
-                unsigned long long tj[2] = {0ull, 0ull}, tk[2] = {0ull, 0ull};
-                const bool slow = __ballot(bad) != 0ull;
-                if (slow) {
+                const int ci = (uc - tb) / C;
+                // The chunk's upstream values: an LDS poll while the poller
+                // has not delivered them.
+                if (rdy <= ci) {
                     unsigned spins = 0;
-                    for (;;) {
-                        fetch(rec_ptr(rj_tile, rslot_j, uc), tj);
-                        fetch(rec_ptr(rk_tile, rslot_k, uc), tk);
-                        drain_memory();
-                        const int still = ((int)nj & (int)!Record<T>::ok(tj, epoch)) |
-                                          ((int)nk & (int)!Record<T>::ok(tk, epoch));
-                        if (__ballot(still) == 0ull) break;
-                        if (ld_dev(&pb.sc->abort) != 0 || ++spins > kSpinLimit) {
-                            // Give up: the launch is reported as timed out and
-                            // its results discarded; the wave runs on without
-                            // waiting so every wave of the grid exits.
+                    while ((rdy = lds_acquire(&mb.ready)) <= ci) {
+                        if (++spins > kSpinLimit) {
                             if (lane == 0) st_dev(&pb.sc->abort, 1);
+                            rdy = 1 << 29;
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
                     }
-                    drain_memory();
                 }
-                const T vj = nj ? (slow ? Record<T>::value(tj) : Record<T>::value(rj)) : T(0);
-                const T vk = nk ? (slow ? Record<T>::value(tk) : Record<T>::value(rk)) : T(0);
                 T evj[C], evk[C];
-#pragma unroll
-                for (int s = 0; s < C; s++) {
-                    evj[s] = shfl_t(vj, s * kTile + b);
-                    evk[s] = shfl_t(vk, s * kTile + a);
-                }
+                read_mail(mb.val[ci % kMailChunks][0][b], evj);
+                read_mail(mb.val[ci % kMailChunks][1][a], evk);
+                // The release waits for the reads above, so the poller may
+                // then reuse the ring slot.
+                if (lane == 0) lds_release(&mb.consumed, ci + 1);
                 const T(&v)[NV][C] = buf[d];
                 unsigned long long *const pubp =
                     pb.edge + ((((size_t)tile * S + (size_t)uc) * kRecSlots + pub_slot) * RW);
@@ -933,6 +1011,17 @@ __global__ __launch_bounds__(64) void k_sweep(GridGeom g, PressureBuffers pb, Sw
     finish_sweep(pb, dot ? dotmode : kDotNone, dot_acc, tile, false);
 }
 
+// Launches a kernel on s.  With a start event the dispatch itself records
+// the kernel's start and end (hipExtLaunchKernel), so timing a kernel adds
+// no marker packets to the queue between kernels.
+template <typename... KArgs, typename... Args>
+static hipError_t launch_on(void (*kern)(KArgs...), dim3 grid, dim3 block, hipStream_t s,
+                            const KernelTiming &kt, Args... args) {
+    if (kt.start) hipExtLaunchKernelGGL(kern, grid, block, 0, s, kt.start, kt.stop, 0, args...);
+    else kern<<<grid, block, 0, s>>>(args...);
+    return hipGetLastError();
+}
+
 // Register ring depth per value type: FP32 operands are half the size.
 template <class T>
 struct SweepDepth;
@@ -947,7 +1036,7 @@ struct SweepDepth<float> {
 
 template <class T>
 static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, SweepOps<T> op,
-                               int dotmode, hipStream_t s) {
+                               int dotmode, hipStream_t s, const KernelTiming &kt = KernelTiming{}) {
     // A fresh epoch per launch; records of older launches never match it.
     if (++pb.epoch >= 0xfffffff0u) {
         size_t n = edge_record_count(pb.sm);
@@ -957,9 +1046,8 @@ static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, 
     }
     int ntiles = pb.sm.ntj * pb.sm.ntk;
     constexpr int D = SweepDepth<T>::D;
-    if (dir == kForward) k_sweep<T, kForward, D><<<ntiles, 64, 0, s>>>(g, pb, op, dotmode);
-    else k_sweep<T, kBackward, D><<<ntiles, 64, 0, s>>>(g, pb, op, dotmode);
-    return hipGetLastError();
+    if (dir == kForward) return launch_on(k_sweep<T, kForward, D>, ntiles, 128, s, kt, g, pb, op, dotmode);
+    return launch_on(k_sweep<T, kBackward, D>, ntiles, 128, s, kt, g, pb, op, dotmode);
 }
 
 static SweepOps<double> fwd_ops64(PressureBuffers &pb, const double *r, double *z) {
@@ -1516,29 +1604,28 @@ hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, bool mix
 bool pcg_piece_used(int piece, bool mixed) { return piece != kPcgResidual || mixed; }
 
 hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBuffers &pb,
-                            hipStream_t s) {
+                            hipStream_t s, const KernelTiming &kt) {
+    const dim3 rg(kRedBlocks), rb(kRedThreads);
     if (mixed) {
         switch (piece) {
-            case kPcgSpmv: k_spmv_dot32<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
-            case kPcgAxpy: k_axpy_norm32<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
-            case kPcgResidual: k_true_residual<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
-            case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kDotNone, s);
-            case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotIter, s);
-            case kPcgUpdate: k_update_s32<<<kRedBlocks, kRedThreads, 0, s>>>(pb, 0); break;
+            case kPcgSpmv: return launch_on(k_spmv_dot32, rg, rb, s, kt, g, pb);
+            case kPcgAxpy: return launch_on(k_axpy_norm32, rg, rb, s, kt, g, pb);
+            case kPcgResidual: return launch_on(k_true_residual, rg, rb, s, kt, g, pb);
+            case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kDotNone, s, kt);
+            case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotIter, s, kt);
+            case kPcgUpdate: return launch_on(k_update_s32, rg, rb, s, kt, pb, 0);
             default: return hipErrorInvalidValue;
         }
-        return hipGetLastError();
     }
     switch (piece) {
-        case kPcgSpmv: k_spmv_dot<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
-        case kPcgAxpy: k_axpy_norm<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb); break;
-        case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops64(pb, pb.r, pb.z), kDotNone, s);
-        case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops64(pb, pb.z, pb.r), kDotIter, s);
-        case kPcgUpdate: k_update_s<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb, 0); break;
+        case kPcgSpmv: return launch_on(k_spmv_dot, rg, rb, s, kt, g, pb);
+        case kPcgAxpy: return launch_on(k_axpy_norm, rg, rb, s, kt, g, pb);
+        case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops64(pb, pb.r, pb.z), kDotNone, s, kt);
+        case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops64(pb, pb.z, pb.r), kDotIter, s, kt);
+        case kPcgUpdate: return launch_on(k_update_s, rg, rb, s, kt, g, pb, 0);
         case kPcgResidual: return hipSuccess;
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 // Execution order of the pieces within one iteration.
@@ -1549,7 +1636,7 @@ hipError_t launch_pcg_iteration(const GridGeom &g, bool mixed, PressureBuffers &
     for (int q = 0; q < kPcgPieces; q++) {
         int piece = kPieceOrder[q];
         if (!pcg_piece_used(piece, mixed)) continue;
-        hipError_t e = launch_pcg_piece(piece, g, mixed, pb, s);
+        hipError_t e = launch_pcg_piece(piece, g, mixed, pb, s, KernelTiming{});
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

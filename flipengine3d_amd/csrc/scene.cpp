@@ -224,11 +224,13 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
                     int piece = pcg_piece_at(q);
                     if (!pcg_piece_used(piece, mixed)) continue;
                     int stage = piece == kPcgResidual ? FLIPMI_STAGE_K_RESIDUAL : FLIPMI_STAGE_K_SPMV + piece;
-                    const bool timed = (prof->kernel_mask >> piece) & 1u;
-                    hipEvent_t e0 = nullptr;
-                    if (timed) prof->begin_stage(stage, &e0);
-                    hip_check(launch_pcg_piece(piece, w.g, mixed, pb, s), "pcg kernel");
-                    if (timed) prof->end_stage(stage, e0);
+                    KernelTiming kt;
+                    if ((prof->kernel_mask >> piece) & 1u) {
+                        kt.start = prof->take_event();
+                        kt.stop = prof->take_event();
+                    }
+                    hip_check(launch_pcg_piece(piece, w.g, mixed, pb, s, kt), "pcg kernel");
+                    if (kt.start) prof->add_timed(stage, kt.start, kt.stop);
                 }
             } else {
                 hip_check(launch_pcg_iteration(w.g, mixed, pb, s), "pcg iteration");
@@ -313,6 +315,10 @@ void Scene::end_stage(int stage, hipEvent_t a) {
     if (!profiling) return;
     hipEvent_t b = take_event();
     check(hipEventRecord(b, stream), "event record");
+    pending_events.push_back(PendingEvent{stage, a, b});
+}
+
+void Scene::add_timed(int stage, hipEvent_t a, hipEvent_t b) {
     pending_events.push_back(PendingEvent{stage, a, b});
 }
 

@@ -133,6 +133,8 @@ struct Scene {
     void extrapolate_all(int layers);
     void begin_stage(int stage, hipEvent_t *a);
     void end_stage(int stage, hipEvent_t a);
+    // Books a pair of events already recorded (by a timed dispatch).
+    void add_timed(int stage, hipEvent_t a, hipEvent_t b);
     void collect_events();
     hipEvent_t take_event();
     void sync();
