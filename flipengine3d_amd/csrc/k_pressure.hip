@@ -63,6 +63,24 @@ __device__ __forceinline__ bool interior(const GridGeom &g, int i, int j, int k)
 }
 
 // Whether slot group grp (256 consecutive slots) holds a row.
+// Slot groups of this block in the per-group kernels.  Blocks are
+// dispatched to the eight XCDs round-robin (block b on XCD b % 8); the
+// blocks of one XCD cover one contiguous eighth of the groups, so the
+// neighbour groups a stencil reads (the adjacent steps of a tile, the
+// adjacent tiles) sit in the same L2 instead of being fetched by every
+// XCD.  Any other placement only costs locality.  gridDim.x is a multiple
+// of kXcds.
+constexpr unsigned kXcds = 8;
+struct GroupRange {
+    size_t begin, end, step;
+};
+__device__ __forceinline__ GroupRange xcd_groups(size_t ngroups) {
+    const unsigned x = blockIdx.x % kXcds, l = blockIdx.x / kXcds;
+    const size_t part = (ngroups + kXcds - 1) / kXcds;
+    const size_t b = (size_t)x * part, e = b + part < ngroups ? b + part : ngroups;
+    return GroupRange{b + l, e, gridDim.x / kXcds};
+}
+
 __device__ __forceinline__ bool group_active(const PressureBuffers &pb, size_t grp) {
     if (!pb.group_mask) return true;
     const size_t S4 = (size_t)(pb.sm.S >> 2);
@@ -236,6 +254,19 @@ __device__ __forceinline__ void drain_memory() {
 
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
+// Buffer views (raw buffer resources) over [base, base + bytes).  A load
+// through an offset beyond the view returns zero and a store through one is
+// dropped, so an offset of kOutside stands for "no such slot" at no cost.
+constexpr unsigned kOutside = 0x80000000u;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_view(const void *base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+
+// sched_barrier mask: ALU (0x1-0x8) and LDS (0x80) instructions may be
+// scheduled across the barrier, vector memory instructions may not.
+constexpr int kAluAndLdsCross = 0x8F;
+
 // Tagged records.  A record carries a value and the 32-bit epoch of the
 // launch that wrote it, in 64-bit words written and read as single-copy-
 // atomic accesses, so a reader that sees the current epoch in every word has
@@ -246,30 +277,32 @@ struct Record;
 template <>
 struct Record<double> {
     static constexpr int W = 2;
-    __device__ static void put(unsigned long long *rec, double v, unsigned tag) {
-        unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-        unsigned long long t = (unsigned long long)tag << 32;
-        st_dev(rec, t | (bits & 0xffffffffull));
-        st_dev(rec + 1, t | (bits >> 32));
-    }
     __device__ static bool ok(const unsigned long long (&w)[2], unsigned tag) {
         return (int)((unsigned)(w[0] >> 32) == tag) & (int)((unsigned)(w[1] >> 32) == tag);
     }
     __device__ static double value(const unsigned long long (&w)[2]) {
         return __longlong_as_double((long long)((w[1] << 32) | (w[0] & 0xffffffffull)));
     }
+    // Device-scope atomic stores: published where they are issued (the
+    // compiler does not sink them towards the end of the chunk).
+    __device__ static void put(unsigned long long *rec, double v, unsigned tag) {
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+        const unsigned long long t = (unsigned long long)tag << 32;
+        st_dev(rec, t | (bits & 0xffffffffull));
+        st_dev(rec + 1, t | (bits >> 32));
+    }
 };
 template <>
 struct Record<float> {
     static constexpr int W = 1;
-    __device__ static void put(unsigned long long *rec, float v, unsigned tag) {
-        st_dev(rec, ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v));
-    }
     __device__ static bool ok(const unsigned long long (&w)[2], unsigned tag) {
         return (unsigned)(w[0] >> 32) == tag;
     }
     __device__ static float value(const unsigned long long (&w)[2]) {
         return __uint_as_float((unsigned)(w[0] & 0xffffffffull));
+    }
+    __device__ static void put(unsigned long long *rec, float v, unsigned tag) {
+        st_dev(rec, ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v));
     }
 };
 
@@ -594,7 +627,7 @@ __global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
 // records of each chunk (lane L: step u0 + 7 + L / 8, edge lane L % 8) until
 // they carry the launch epoch and hands the values over in LDS, laid out so
 // that each edge lane of the compute wave reads its eight values with two
-// 16-byte LDS loads.  The compute wave thus never waits on global memory
+// 16-byte LDS loads (FP32).  The compute wave thus never waits on global memory
 // for its neighbours: a chunk that is not there yet costs an LDS poll, not
 // a drain of its own operand loads and record stores.
 constexpr int kChunk = 8;  // 8 steps x 8 edge lanes = one record per poller lane
@@ -618,19 +651,15 @@ __device__ __forceinline__ void lds_release(int *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Eight consecutive mailbox values of one edge lane.
-__device__ __forceinline__ void read_mail(const float *p, float (&e)[kChunk]) {
-    const float4 v0 = *reinterpret_cast<const float4 *>(p);
-    const float4 v1 = *reinterpret_cast<const float4 *>(p + 4);
-    e[0] = v0.x; e[1] = v0.y; e[2] = v0.z; e[3] = v0.w;
-    e[4] = v1.x; e[5] = v1.y; e[6] = v1.z; e[7] = v1.w;
+// Four consecutive mailbox values (steps h * 4 .. h * 4 + 3) of one edge lane.
+__device__ __forceinline__ void read_mail(const float *p, int h, float (&e)[kChunk]) {
+    const float4 v = *reinterpret_cast<const float4 *>(p + 4 * h);
+    e[4 * h] = v.x; e[4 * h + 1] = v.y; e[4 * h + 2] = v.z; e[4 * h + 3] = v.w;
 }
-__device__ __forceinline__ void read_mail(const double *p, double (&e)[kChunk]) {
-#pragma unroll
-    for (int q = 0; q < kChunk; q += 2) {
-        const double2 v = *reinterpret_cast<const double2 *>(p + q);
-        e[q] = v.x; e[q + 1] = v.y;
-    }
+__device__ __forceinline__ void read_mail(const double *p, int h, double (&e)[kChunk]) {
+    const double2 v0 = *reinterpret_cast<const double2 *>(p + 4 * h);
+    const double2 v1 = *reinterpret_cast<const double2 *>(p + 4 * h + 2);
+    e[4 * h] = v0.x; e[4 * h + 1] = v0.y; e[4 * h + 2] = v1.x; e[4 * h + 3] = v1.y;
 }
 
 template <class T>
@@ -718,22 +747,33 @@ __device__ __forceinline__ double lane_k(double v, bool low_half, bool row_sel) 
     return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
 }
 
-// The four steps of one slot group of this lane (16 bytes per access).
-__device__ __forceinline__ void load_group(const float *A, size_t base, float (&e)[4]) {
-    float4 v = *reinterpret_cast<const float4 *>(A + base);
-    e[0] = v.x; e[1] = v.y; e[2] = v.z; e[3] = v.w;
+// The four steps of one slot group of this lane (16 bytes per access for
+// FP32) through a buffer view.
+__device__ __forceinline__ void load_group(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, float (&e)[4]) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0);
+    e[0] = __uint_as_float(v.x); e[1] = __uint_as_float(v.y);
+    e[2] = __uint_as_float(v.z); e[3] = __uint_as_float(v.w);
 }
-__device__ __forceinline__ void load_group(const double *A, size_t base, double (&e)[4]) {
-    double2 v0 = *reinterpret_cast<const double2 *>(A + base);
-    double2 v1 = *reinterpret_cast<const double2 *>(A + base + 2);
-    e[0] = v0.x; e[1] = v0.y; e[2] = v1.x; e[3] = v1.y;
+__device__ __forceinline__ void load_group(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, double (&e)[4]) {
+    const u32x4 v0 = __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0);
+    const u32x4 v1 = __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16u, so, 0);
+    e[0] = __hiloint2double((int)v0.y, (int)v0.x); e[1] = __hiloint2double((int)v0.w, (int)v0.z);
+    e[2] = __hiloint2double((int)v1.y, (int)v1.x); e[3] = __hiloint2double((int)v1.w, (int)v1.z);
 }
-__device__ __forceinline__ void store_group(float *A, size_t base, const float (&e)[4]) {
-    *reinterpret_cast<float4 *>(A + base) = make_float4(e[0], e[1], e[2], e[3]);
+__device__ __forceinline__ void store_group(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, const float (&e)[4]) {
+    u32x4 v;
+    v.x = __float_as_uint(e[0]); v.y = __float_as_uint(e[1]);
+    v.z = __float_as_uint(e[2]); v.w = __float_as_uint(e[3]);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, vo, so, 0);
 }
-__device__ __forceinline__ void store_group(double *A, size_t base, const double (&e)[4]) {
-    *reinterpret_cast<double2 *>(A + base) = make_double2(e[0], e[1]);
-    *reinterpret_cast<double2 *>(A + base + 2) = make_double2(e[2], e[3]);
+__device__ __forceinline__ void store_group(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, const double (&e)[4]) {
+    u32x4 v0, v1;
+    v0.x = (unsigned)__double2loint(e[0]); v0.y = (unsigned)__double2hiint(e[0]);
+    v0.z = (unsigned)__double2loint(e[1]); v0.w = (unsigned)__double2hiint(e[1]);
+    v1.x = (unsigned)__double2loint(e[2]); v1.y = (unsigned)__double2hiint(e[2]);
+    v1.z = (unsigned)__double2loint(e[3]); v1.w = (unsigned)__double2hiint(e[3]);
+    __builtin_amdgcn_raw_buffer_store_b128(v0, r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(v1, r, vo + 16u, so, 0);
 }
 
 // The poller wave of a sweep workgroup: for each chunk of the tile's march,
@@ -779,13 +819,15 @@ __device__ void sweep_poller(PressureBuffers &pb, SweepMailbox<T> &mb, int tile,
         const bool nk = (up_k >= 0) & (uu >= ubk) & (uu < uek);
         T vj = T(0), vk = T(0);
         if (__ballot(nj | nk) != 0ull) {
+            // The records of the chunk in one round trip per poll.  Every
+            // 64th poll also reads the abort flag (one address that every
+            // poller would otherwise hammer).
             unsigned long long rj[2], rk[2];
             spins = 0;
             for (;;) {
-                // The records and the abort flag in one round trip.
                 fetch(rec_ptr(rj_tile, rslot_j, uu), rj);
                 fetch(rec_ptr(rk_tile, rslot_k, uu), rk);
-                const int aborted = ld_dev(&pb.sc->abort);
+                const int aborted = (spins & 63u) == 63u ? ld_dev(&pb.sc->abort) : 0;
                 drain_memory();
                 const int bad = ((int)nj & (int)!Record<T>::ok(rj, epoch)) |
                                 ((int)nk & (int)!Record<T>::ok(rk, epoch));
@@ -881,14 +923,29 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
         const unsigned long long word = (w & 2) ? hi : lo;
         return (word >> (gi & 63)) & 1ull;
     };
-    // Offsets of this lane's slots in the zero and sink groups.  Only the
-    // offset depends on the mask, never the array pointer.
-    const size_t zbase = sm.zero_group() + ((size_t)lane << 2);
-    const size_t sbase = sm.sink_group() + ((size_t)lane << 2);
+    // Buffer views of the tile's slots in each array (group gi of the tile
+    // at byte gi * 256 * sizeof(T), this lane's four slots at lane * 4 *
+    // sizeof(T) within it).  A group without rows is read and written
+    // through the offset kOutside: it reads as zeros and its stores (zeros)
+    // are dropped.  The group offset is scalar, so addressing a group costs
+    // one select for all arrays.
+    const unsigned tile_bytes = (unsigned)S * 64u * (unsigned)sizeof(T);
+    const size_t tile0 = grp0 * 256;
+    const auto r_b = buffer_view((FWD ? op.in : op.out) + tile0, tile_bytes);
+    const auto r_c1 = buffer_view(op.c1 + tile0, tile_bytes);
+    const auto r_c2 = buffer_view(op.c2 + tile0, tile_bytes);
+    const auto r_c3 = buffer_view(op.c3 + tile0, tile_bytes);
+    const auto r_inv = buffer_view(op.inv + tile0, tile_bytes);
+    const auto r_dot = buffer_view(dotsrc + tile0, tile_bytes);
+    const auto r_out = buffer_view(op.out + tile0, tile_bytes);
+    const unsigned vlane = (unsigned)lane * 4u * (unsigned)sizeof(T);
+    // Every lane stores a record each step (no branch around the store):
+    // lanes that publish nothing write slot 8 + aj, which the corner lane
+    // leaves unused and no consumer reads.
+    const int rec_slot = pub ? pub_slot : 8 + aj;
 
     double dot_acc = 0.0;
     if (te > tb) {
-        auto gbase = [&](int gi) -> size_t { return ((grp0 + (size_t)gi) * 64 + (size_t)lane) << 2; };
         // Lower slot group of the chunk at march step u0 (u0 % 8 == 0).
         auto chunk_group = [&](int u0) -> int { return FWD ? (u0 >> 2) : ((S - 8 - u0) >> 2); };
         // Step s of a chunk is element m of group h: forward s = 4h + m,
@@ -898,17 +955,16 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
             const int gl = chunk_group(u0);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                // A group without rows reads the arrays' zero group instead
-                // (same instructions; the 2 KB it spans stay in cache).
                 act[h] = active(gl + h);
-                const size_t base = act[h] ? gbase(gl + h) : zbase;
+                const unsigned vo = act[h] ? vlane : kOutside;
+                const unsigned so = (unsigned)(gl + h) * 256u * (unsigned)sizeof(T);
                 T e[NV][4];
-                load_group(FWD ? op.in : op.out, base, e[0]);
-                load_group(op.c1, base, e[1]);
-                load_group(op.c2, base, e[2]);
-                load_group(op.c3, base, e[3]);
-                load_group(op.inv, base, e[4]);
-                if (!FWD) load_group(dotsrc, base, e[NV - 1]);
+                load_group(r_b, vo, so, e[0]);
+                load_group(r_c1, vo, so, e[1]);
+                load_group(r_c2, vo, so, e[2]);
+                load_group(r_c3, vo, so, e[3]);
+                load_group(r_inv, vo, so, e[4]);
+                if (!FWD) load_group(r_dot, vo, so, e[NV - 1]);
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
                     const int s = FWD ? 4 * h + m : 7 - (4 * h + m);
@@ -925,8 +981,8 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
 #pragma unroll
                 for (int m = 0; m < 4; m++) e[m] = qv[FWD ? 4 * h + m : 7 - (4 * h + m)];
                 // The output of a group without rows stays zero: its values
-                // (all zero) go to the sink group instead.
-                store_group(op.out, act[h] ? gbase(gl + h) : sbase, e);
+                // (all zero) are dropped.
+                store_group(r_out, act[h] ? vlane : kOutside, (unsigned)(gl + h) * 256u * (unsigned)sizeof(T), e);
             }
         };
         // The operands of the first D chunks.
@@ -941,6 +997,19 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
         }
         T cur = T(0);
         int rdy = 0;  // chunks known to be in the mailbox
+        // An LDS poll while the poller has not delivered `need` chunks.
+        auto wait_mail = [&](int need) {
+            if (rdy >= need) return;
+            unsigned spins = 0;
+            while ((rdy = lds_acquire(&mb.ready)) < need) {
+                if (++spins > kSpinLimit) {
+                    if (lane == 0) st_dev(&pb.sc->abort, 1);
+                    rdy = 1 << 29;
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        };
         for (int u0 = tb; u0 < te; u0 += D * C) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
@@ -948,28 +1017,22 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                 // ring (an exit here costs the compiler its load accounting).
                 const int uc = u0 + d * C;
                 const int ci = (uc - tb) / C;
+                const T *const mj = mb.val[ci % kMailChunks][0][b];
+                const T *const mk = mb.val[ci % kMailChunks][1][a];
                 // The chunk's upstream values: an LDS poll while the poller
                 // has not delivered them.
-                if (rdy <= ci) {
-                    unsigned spins = 0;
-                    while ((rdy = lds_acquire(&mb.ready)) <= ci) {
-                        if (++spins > kSpinLimit) {
-                            if (lane == 0) st_dev(&pb.sc->abort, 1);
-                            rdy = 1 << 29;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                }
+                wait_mail(ci + 1);
                 T evj[C], evk[C];
-                read_mail(mb.val[ci % kMailChunks][0][b], evj);
-                read_mail(mb.val[ci % kMailChunks][1][a], evk);
+                read_mail(mj, 0, evj);
+                read_mail(mk, 0, evk);
+                read_mail(mj, 1, evj);
+                read_mail(mk, 1, evk);
                 // The release waits for the reads above, so the poller may
                 // then reuse the ring slot.
                 if (lane == 0) lds_release(&mb.consumed, ci + 1);
                 const T(&v)[NV][C] = buf[d];
-                unsigned long long *const pubp =
-                    pb.edge + ((((size_t)tile * S + (size_t)uc) * kRecSlots + pub_slot) * RW);
+                unsigned long long *const recp =
+                    pb.edge + ((((size_t)tile * S + (size_t)uc) * kRecSlots + rec_slot) * RW);
                 T qv[C];
 #pragma unroll
                 for (int s = 0; s < C; s++) {
@@ -998,7 +1061,12 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                     }
                     if (!FWD) dot_acc = __builtin_fma((double)q, (double)v[NV - 1][s], dot_acc);
                     qv[s] = q;
-                    if (pub) Record<T>::put(pubp + s * kRecSlots * RW, q, epoch);
+                    Record<T>::put(recp + (size_t)s * kRecSlots * RW, q, epoch);
+                    // Keep each step's record where it is computed: no
+                    // memory operation is scheduled across this point (ALU
+                    // and LDS instructions may be).
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    __builtin_amdgcn_sched_barrier(0);
                     cur = q;
                 }
                 store_chunk(uc, qv, actr[d]);
@@ -1143,7 +1211,8 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBu
     if (pb.sc->done) return;
     size_t n = pb.sm.count();
     double acc = 0.0;
-    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+    const GroupRange gr = xcd_groups(n >> 8);
+    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
         if (!group_active(pb, grp)) continue;  // uniform per block
         const size_t p = (grp << 8) + threadIdx.x;
         int i, j, k;
@@ -1178,7 +1247,8 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm(GridGeom g, PressureB
     float af = pb.sc->alpha_f;
     float naf = (float)(-pb.sc->alpha);
     double mx = 0.0;
-    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+    const GroupRange gr = xcd_groups(n >> 8);
+    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
         if (!group_active(pb, grp)) continue;  // uniform per block
         const size_t p = (grp << 8) + threadIdx.x;
         pb.x[p] += af * pb.s[p];
@@ -1215,7 +1285,8 @@ __global__ void k_update_s(GridGeom g, PressureBuffers pb, int first) {
     if (pb.sc->done) return;
     size_t n = pb.sm.count();
     float bf = first ? 0.0f : pb.sc->beta_f;
-    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+    const GroupRange gr = xcd_groups(n >> 8);
+    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
         if (!group_active(pb, grp)) continue;  // uniform per block
         const size_t p = (grp << 8) + threadIdx.x;
         if (first) pb.s[p] = pb.z[p];
@@ -1450,7 +1521,8 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, Pressure
     if (pb.sc->done) return;
     size_t n = pb.sm.count();
     double acc = 0.0;
-    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+    const GroupRange gr = xcd_groups(n >> 8);
+    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
         if (!group_active(pb, grp)) continue;  // uniform per block
         const size_t p = (grp << 8) + threadIdx.x;
         int i, j, k;
@@ -1485,7 +1557,8 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm32(GridGeom g, Pressur
     double alpha = pb.sc->alpha;
     float naf = -pb.sc->alpha_f;
     double mx = 0.0;
-    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+    const GroupRange gr = xcd_groups(n >> 8);
+    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
         if (!group_active(pb, grp)) continue;  // uniform per block
         const size_t p = (grp << 8) + threadIdx.x;
         float sp = pb.s32[p];
@@ -1520,7 +1593,8 @@ __global__ __launch_bounds__(kRedThreads) void k_true_residual(GridGeom g, Press
     if (pb.sc->done || !pb.sc->refresh) return;
     size_t n = pb.sm.count();
     double mx = 0.0;
-    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+    const GroupRange gr = xcd_groups(n >> 8);
+    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
         if (!group_active(pb, grp)) continue;  // uniform per block
         const size_t p = (grp << 8) + threadIdx.x;
         int i, j, k;
@@ -1560,7 +1634,8 @@ __global__ void k_update_s32(PressureBuffers pb, int first) {
     if (pb.sc->done) return;
     size_t n = pb.sm.count();
     float bf = first ? 0.0f : pb.sc->beta_f;
-    for (size_t grp = blockIdx.x; grp < (n >> 8); grp += gridDim.x) {
+    const GroupRange gr = xcd_groups(n >> 8);
+    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
         if (!group_active(pb, grp)) continue;  // uniform per block
         const size_t p = (grp << 8) + threadIdx.x;
         if (first) pb.s32[p] = pb.z32[p];
