@@ -106,7 +106,8 @@ EXPORTED = [
     "flipmi_ctx_create", "flipmi_ctx_destroy", "flipmi_ctx_synchronize", "flipmi_last_error", "flipmi_scene_desc_default",
     "flipmi_scene_create", "flipmi_scene_destroy", "flipmi_scene_load_particles",
     "flipmi_scene_initialize", "flipmi_scene_update", "flipmi_scene_num_particles",
-    "flipmi_scene_get_particles", "flipmi_scene_get_grid", "flipmi_scene_stage_times",
+    "flipmi_scene_get_particles", "flipmi_scene_get_particles_range", "flipmi_scene_set_domain",
+    "flipmi_scene_get_grid", "flipmi_scene_stage_times",
     "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
     "flipmi_liquid_sdf_postprocess", "flipmi_p2g",
     "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
@@ -139,6 +140,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_update.argtypes = [_vp, _c_double, P(FrameStats)]
     L.flipmi_scene_num_particles.argtypes = [_vp, P(_c_i64)]
     L.flipmi_scene_get_particles.argtypes = [_vp, _vp, _vp]
+    L.flipmi_scene_get_particles_range.argtypes = [_vp, _c_i64, _c_i64, _vp, _vp]
+    L.flipmi_scene_set_domain.argtypes = [_vp, _c_double, _c_double, _c_double, _c_double]
     L.flipmi_scene_get_grid.argtypes = [_vp, _c_int, _vp]
     L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
     L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
@@ -348,6 +351,7 @@ class FluidSimulation:
             h = _vp()
             self.ctx.check(self.L.flipmi_scene_create(self.ctx.h, ctypes.byref(self.desc), ctypes.byref(h)))
             self.h = h
+            self._push_domain()
 
     def addBodyForce(self, fx: float, fy: float, fz: float):
         if self.h is not None:
@@ -423,11 +427,46 @@ class FluidSimulation:
         return n.value
 
     def particles(self) -> Tuple[np.ndarray, np.ndarray]:
+        """getMarkerParticlePositionData / VelocityData: positions in domain
+        coordinates."""
         n = self.getNumMarkerParticles()
         pos = np.zeros((n, 3), np.float32)
         vel = np.zeros((n, 3), np.float32)
         self.ctx.check(self.L.flipmi_scene_get_particles(self.h, _ptr(pos), _ptr(vel)))
         return pos, vel
+
+    def particles_range(self, start: int, end: int) -> Tuple[np.ndarray, np.ndarray]:
+        """getMarkerParticle{Position,Velocity}DataRange (fluidsimulation.cpp:
+        2314-2338); an invalid range raises, as the reference throws
+        domain_error."""
+        m = max(end - start, 0)
+        pos = np.zeros((m, 3), np.float32)
+        vel = np.zeros((m, 3), np.float32)
+        self.ctx.check(self.L.flipmi_scene_get_particles_range(self.h, start, end, _ptr(pos) if m else None,
+                                                               _ptr(vel) if m else None))
+        return pos, vel
+
+    def setDomainOffset(self, x: float, y: float, z: float):
+        """fluidsimulation.cpp:307-318 (stored in float, like vmath::vec3)."""
+        self._domain_offset = tuple(float(np.float32(v)) for v in (x, y, z))
+        self._push_domain()
+
+    def getDomainOffset(self):
+        return getattr(self, "_domain_offset", (0.0, 0.0, 0.0))
+
+    def setDomainScale(self, scale: float):
+        """fluidsimulation.cpp:324-330."""
+        self._domain_scale = float(scale)
+        self._push_domain()
+
+    def getDomainScale(self) -> float:
+        return getattr(self, "_domain_scale", 1.0)
+
+    def _push_domain(self):
+        if self.h is None:
+            return  # pushed when the scene is created
+        o = self.getDomainOffset()
+        self.ctx.check(self.L.flipmi_scene_set_domain(self.h, o[0], o[1], o[2], self.getDomainScale()))
 
     def grid(self, which: int) -> np.ndarray:
         shape, dt = self._GRID_SHAPES[which](*self.dims)

@@ -172,8 +172,42 @@ int flipmi_scene_update(flipmi_scene *scene, double dt, flipmi_frame_stats *stat
 
 int flipmi_scene_num_particles(flipmi_scene *scene, int64_t *n) {
     if (!scene || !n) return FLIPMI_ERR_INVALID_ARG;
-    *n = scene->s.initialized ? scene->s.n : (int64_t)(scene->s.pending_pos.size() / 3);
+    // Loaded particles join the scene at initialize (the reference counts
+    // none before).
+    *n = scene->s.initialized ? scene->s.n : 0;
     return FLIPMI_OK;
+}
+
+int flipmi_scene_set_domain(flipmi_scene *scene, double ox, double oy, double oz, double scale) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    s.domain_offset[0] = (float)ox;
+    s.domain_offset[1] = (float)oy;
+    s.domain_offset[2] = (float)oz;
+    s.domain_scale = scale;
+    return FLIPMI_OK;
+}
+
+int flipmi_scene_get_particles_range(flipmi_scene *scene, int64_t start, int64_t end, float *pos, float *vel) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    if (!s.initialized) return fail(s.ctx, FLIPMI_ERR_NOT_INITIALIZED, "scene not initialized");
+    if (start < 0 || end > (int64_t)s.n || start > end)
+        return fail(s.ctx, FLIPMI_ERR_INVALID_ARG, "invalid particle range");
+    const int64_t m = end - start;
+    if (m == 0) return FLIPMI_OK;
+    return guarded(s.ctx, [&] {
+        std::vector<float> t((size_t)m);
+        for (int a = 0; a < 6; a++) {
+            float *dst = a < 3 ? pos : vel;
+            if (!dst) continue;
+            hipck(hipMemcpyAsync(t.data(), s.part[a].p + start, sizeof(float) * m, hipMemcpyDeviceToHost, s.stream),
+                  "download");
+            hipck(hipStreamSynchronize(s.stream), "sync");
+            for (int64_t q = 0; q < m; q++) dst[3 * (size_t)q + (a % 3)] = t[q];
+        }
+        if (pos) s.to_domain(pos, m);
+    });
 }
 
 int flipmi_scene_get_particles(flipmi_scene *scene, float *pos, float *vel) {
@@ -190,6 +224,7 @@ int flipmi_scene_get_particles(flipmi_scene *scene, float *pos, float *vel) {
             hipck(hipStreamSynchronize(s.stream), "sync");
             for (int q = 0; q < s.n; q++) dst[3 * (size_t)q + (a % 3)] = t[q];
         }
+        if (pos) s.to_domain(pos, s.n);
     });
 }
 

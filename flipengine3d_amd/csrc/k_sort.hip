@@ -89,18 +89,44 @@ hipError_t launch_exclusive_scan(const int *in, int *out, int n, int *tmp, hipSt
     return hipGetLastError();
 }
 
-__global__ void k_cell_key(GridGeom g, const float *__restrict__ px, const float *__restrict__ py,
-                           const float *__restrict__ pz, int n, int *cell, int *slot, int *count) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    I3 c = pos_to_index(f3(px[i], py[i], pz[i]), g.invdx);
-    int id = -1, sl = 0;
-    if (c.i >= 0 && c.j >= 0 && c.k >= 0 && c.i < g.I && c.j < g.J && c.k < g.K) {
-        id = c.i + g.I * (c.j + g.J * c.k);
-        sl = atomicAdd(&count[id], 1);
+// Cell of each particle and its arrival slot in that cell.  Particles that
+// are neighbours in memory mostly share cells, so the lanes of a wave that
+// hold the same cell take their slots with one atomic (the leader adds the
+// group's size, each lane its rank in the group): a cell's count is then
+// bumped once per wave instead of once per particle.  Slots only need to be
+// distinct; the within-cell order is fixed afterwards by particle id.
+__global__ __launch_bounds__(256) void k_cell_key(GridGeom g, const float *__restrict__ px,
+                                                  const float *__restrict__ py, const float *__restrict__ pz,
+                                                  int n, int *cell, int *slot, int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int id = -1;
+    if (i < n) {
+        I3 c = pos_to_index(f3(px[i], py[i], pz[i]), g.invdx);
+        if (c.i >= 0 && c.j >= 0 && c.k >= 0 && c.i < g.I && c.j < g.J && c.k < g.K)
+            id = c.i + g.I * (c.j + g.J * c.k);
     }
-    cell[i] = id;
-    slot[i] = sl;
+    int sl = 0;
+    bool pending = id >= 0;
+    for (;;) {
+        const unsigned long long m = __ballot(pending);
+        if (m == 0ull) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const int key = __shfl(id, leader, 64);
+        const bool mine = pending && id == key;
+        const unsigned long long peers = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&count[key], __popcll(peers));
+        base = __shfl(base, leader, 64);
+        if (mine) {
+            sl = base + __popcll(peers & ((1ull << lane) - 1ull));
+            pending = false;
+        }
+    }
+    if (i < n) {
+        cell[i] = id;
+        slot[i] = sl;
+    }
 }
 
 __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ slot,

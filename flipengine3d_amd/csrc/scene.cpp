@@ -382,19 +382,45 @@ void Scene::alloc_all() {
     check(launch_fill_f32(sW.p, nW, 0.0f, stream), "fill");
 }
 
+// FluidSimulation::loadMarkerParticleData (fluidsimulation.cpp:2488-2509):
+// the particles queue up as given; initialize() admits them.
 void Scene::load(const float *pos, const float *vel, int64_t count_in) {
+    pending_pos.insert(pending_pos.end(), pos, pos + 3 * (size_t)count_in);
+    if (vel) pending_vel.insert(pending_vel.end(), vel, vel + 3 * (size_t)count_in);
+    else pending_vel.insert(pending_vel.end(), 3 * (size_t)count_in, 0.0f);
+}
+
+void Scene::admit_pending() {
     // FluidSimulation::_loadMarkerParticles (fluidsimulation.cpp:2773-2785):
-    // keep particles inside [0, I dx) x [0, J dx) x [0, K dx).
-    float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    double ex = desc.isize * desc.dx, ey = desc.jsize * desc.dx, ez = desc.ksize * desc.dx;
-    for (int64_t q = 0; q < count_in; q++) {
-        float x = pos[3 * q], y = pos[3 * q + 1], z = pos[3 * q + 2];
+    // p = (p - offset) / scale in vmath's float arithmetic (the division is
+    // a multiplication by (float)(1.0 / (float)scale), vmath.cpp:100-103),
+    // then keep the points inside [0, I dx) x [0, J dx) x [0, K dx)
+    // (AABB::isPointInside, aabb.cpp:130-133).
+    const float inv = (float)(1.0 / (double)(float)domain_scale);
+    const float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    const double ex = desc.isize * desc.dx, ey = desc.jsize * desc.dx, ez = desc.ksize * desc.dx;
+    const size_t m = pending_pos.size() / 3;
+    size_t kept = 0;
+    for (size_t q = 0; q < m; q++) {
+        const float x = (pending_pos[3 * q] - domain_offset[0]) * inv;
+        const float y = (pending_pos[3 * q + 1] - domain_offset[1]) * inv;
+        const float z = (pending_pos[3 * q + 2] - domain_offset[2]) * inv;
         if (!(x >= bx && y >= by && z >= bz && (double)x < (double)bx + ex &&
               (double)y < (double)by + ey && (double)z < (double)bz + ez))
             continue;
-        pending_pos.push_back(x); pending_pos.push_back(y); pending_pos.push_back(z);
-        for (int a = 0; a < 3; a++) pending_vel.push_back(vel ? vel[3 * q + a] : 0.0f);
+        pending_pos[3 * kept] = x; pending_pos[3 * kept + 1] = y; pending_pos[3 * kept + 2] = z;
+        for (int a = 0; a < 3; a++) pending_vel[3 * kept + a] = pending_vel[3 * q + a];
+        kept++;
     }
+    pending_pos.resize(3 * kept);
+    pending_vel.resize(3 * kept);
+}
+
+// getMarkerParticlePositionData (fluidsimulation.cpp:2408-2413):
+// position * scale + offset, in float.
+void Scene::to_domain(float *pos_xyz, int64_t count) const {
+    const float sc = (float)domain_scale;
+    for (int64_t q = 0; q < 3 * count; q++) pos_xyz[q] = pos_xyz[q] * sc + domain_offset[q % 3];
 }
 
 void Scene::upload_pending() {
@@ -416,6 +442,7 @@ void Scene::upload_pending() {
 
 void Scene::initialize() {
     if (initialized) return;
+    admit_pending();
     int64_t np = (int64_t)(pending_pos.size() / 3);
     if (np > INT32_MAX / 2) throw StatusError{FLIPMI_ERR_INVALID_ARG, "too many particles"};
     cap = (int)std::max<int64_t>(np, desc.max_particles);
@@ -431,6 +458,7 @@ bool Scene::rearm(const float *pos, const float *vel, int64_t count_in) {
     pending_pos.clear();
     pending_vel.clear();
     load(pos, vel, count_in);
+    admit_pending();
     if ((int64_t)(pending_pos.size() / 3) > cap) {
         pending_pos.clear();
         pending_vel.clear();

@@ -92,7 +92,11 @@ struct Scene {
     int current_frame = 0, step_number = 0;
     double frame_dt = 0, remaining = 0, cur_step = 0;
     double radius = 0;
-    std::vector<float> pending_pos, pending_vel;
+    std::vector<float> pending_pos, pending_vel;  // loaded, not yet in the scene
+    // setDomainOffset / setDomainScale (fluidsimulation.h:1612-1613):
+    // particle positions cross the boundary in domain coordinates.
+    float domain_offset[3] = {0.0f, 0.0f, 0.0f};
+    double domain_scale = 1.0;
 
     // last pressure solve
     int last_iters = 0, last_ok = 1;
@@ -117,7 +121,12 @@ struct Scene {
     void alloc_all();
     void load(const float *pos, const float *vel, int64_t count);
     void initialize();
+    // Domain to grid coordinates and the grid-box filter of the pending
+    // particles (FluidSimulation::_loadMarkerParticles).
+    void admit_pending();
     void upload_pending();
+    // Grid to domain coordinates of n positions (xyz), in place.
+    void to_domain(float *pos_xyz, int64_t n) const;
     // Reuses an initialized scene for a new particle set on the same grid and
     // settings (the state right after initialize()); false if the set does not
     // fit the particle capacity.

@@ -136,7 +136,9 @@ void flipmi_scene_desc_default(flipmi_scene_desc *desc);
 int flipmi_scene_create(flipmi_ctx *ctx, const flipmi_scene_desc *desc, flipmi_scene **out);
 int flipmi_scene_destroy(flipmi_scene *scene);
 /* FluidSimulation::loadMarkerParticleData (fluidsimulation.cpp:2488-2509);
- * call before flipmi_scene_initialize.  vel may be NULL (zero velocity). */
+ * call before flipmi_scene_initialize, which admits the particles
+ * (_loadMarkerParticles, fluidsimulation.cpp:2773-2785: domain to grid
+ * coordinates, then the grid-box filter).  vel may be NULL (zero velocity). */
 int flipmi_scene_load_particles(flipmi_scene *scene, const float *pos_xyz, const float *vel_xyz,
                                 int64_t n);
 /* FluidSimulation::initialize (fluidsimulation.cpp:82-88) */
@@ -147,11 +149,26 @@ int flipmi_scene_initialize(flipmi_scene *scene);
  * pressure solve leaves the velocity unprojected (as the reference does) and
  * is reported in stats->pressure_ok, not as an error. stats may be NULL. */
 int flipmi_scene_update(flipmi_scene *scene, double dt, flipmi_frame_stats *stats);
-/* FluidSimulation::getNumMarkerParticles */
+/* FluidSimulation::getNumMarkerParticles (0 before initialize) */
 int flipmi_scene_num_particles(flipmi_scene *scene, int64_t *n);
 /* FluidSimulation::getMarkerParticlePositionData / VelocityData
- * (fluidsimulation.cpp:2408-2420); either pointer may be NULL. */
+ * (fluidsimulation.cpp:2408-2420): vec3 float blobs, 12 bytes per particle,
+ * positions in domain coordinates (position * scale + offset); either
+ * pointer may be NULL. */
 int flipmi_scene_get_particles(flipmi_scene *scene, float *pos_xyz, float *vel_xyz);
+/* FluidSimulation::getMarkerParticlePositionDataRange / VelocityDataRange
+ * (fluidsimulation.cpp:2314-2338): particles [start, end).  An invalid
+ * range (start < 0, end > count, start > end) is FLIPMI_ERR_INVALID_ARG, as
+ * the reference throws domain_error.  Either pointer may be NULL. */
+int flipmi_scene_get_particles_range(flipmi_scene *scene, int64_t start, int64_t end, float *pos_xyz,
+                                     float *vel_xyz);
+/* FluidSimulation::setDomainOffset(double, double, double) and
+ * setDomainScale(double) (fluidsimulation.cpp:307-330): particles are
+ * loaded and read back in domain coordinates, p_domain = p * scale +
+ * offset.  Takes effect for particles admitted by a later initialize and
+ * for every read-back. */
+int flipmi_scene_set_domain(flipmi_scene *scene, double offset_x, double offset_y, double offset_z,
+                            double scale);
 /* Copies an internal grid (FLIPMI_GRID_*) to host memory, e.g.
  * FluidSimulation::getVelocityField (fluidsimulation.cpp:2242). */
 int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out);

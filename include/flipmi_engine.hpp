@@ -635,6 +635,8 @@ public:
         flipmi_ctx *c = detail::ctx();
         _desc.max_particles = _maxParticles;
         detail::check(flipmi_scene_create(c, &_desc, &_scene), "FluidSimulation::initialize");
+        detail::check(flipmi_scene_set_domain(_scene, _domainOffset.x, _domainOffset.y, _domainOffset.z, _domainScale),
+                      "FluidSimulation::initialize");
         int64_t n = (int64_t)(_loadPos.size() / 3);
         if (n > 0)
             detail::check(flipmi_scene_load_particles(_scene, _loadPos.data(), _loadVel.data(), n),
@@ -665,6 +667,22 @@ public:
         _lastPressureOk = st.pressure_ok != 0;
         _velocityStale = true;
         _currentFrame++;
+    }
+
+    // fluidsimulation.cpp:303-330: particles are loaded and read back in
+    // domain coordinates (p * scale + offset).
+    vmath::vec3 getDomainOffset() { return _domainOffset; }
+    void setDomainOffset(double x, double y, double z) { setDomainOffset(vmath::vec3(x, y, z)); }
+    void setDomainOffset(vmath::vec3 offset) {
+        _domainOffset = offset;
+        if (_scene) detail::check(flipmi_scene_set_domain(_scene, offset.x, offset.y, offset.z, _domainScale),
+                                  "FluidSimulation::setDomainOffset");
+    }
+    double getDomainScale() { return _domainScale; }
+    void setDomainScale(double scale) {
+        _domainScale = scale;
+        if (_scene) detail::check(flipmi_scene_set_domain(_scene, _domainOffset.x, _domainOffset.y, _domainOffset.z, scale),
+                                  "FluidSimulation::setDomainScale");
     }
 
     int getCurrentFrame() { return _currentFrame; }
@@ -752,8 +770,9 @@ public:
         else _loadVel.insert(_loadVel.end(), 3 * (size_t)data.size, 0.0f);
     }
 
+    // Loaded particles join at initialize(); none are counted before.
     unsigned int getNumMarkerParticles() {
-        if (!_initialized) return (unsigned int)(_loadPos.size() / 3);
+        if (!_initialized) return 0;
         int64_t n = 0;
         detail::check(flipmi_scene_num_particles(_scene, &n), "FluidSimulation::getNumMarkerParticles");
         return (unsigned int)n;
@@ -784,6 +803,13 @@ public:
         _requireScene();
         detail::check(flipmi_scene_get_particles(_scene, nullptr, reinterpret_cast<float *>(data)),
                       "FluidSimulation::getMarkerParticleVelocityData");
+    }
+    // fluidsimulation.cpp:2314-2338: particles [start_idx, end_idx).
+    void getMarkerParticlePositionDataRange(int start_idx, int end_idx, char *data) {
+        _range(start_idx, end_idx, reinterpret_cast<float *>(data), nullptr);
+    }
+    void getMarkerParticleVelocityDataRange(int start_idx, int end_idx, char *data) {
+        _range(start_idx, end_idx, nullptr, reinterpret_cast<float *>(data));
     }
     unsigned int getMarkerParticlePositionDataSize() { return getNumMarkerParticles() * 3 * sizeof(float); }
     unsigned int getMarkerParticleVelocityDataSize() { return getNumMarkerParticles() * 3 * sizeof(float); }
@@ -816,6 +842,13 @@ private:
     void _requireScene() {
         if (!_initialized) throw std::runtime_error("Error: FluidSimulation must be initialized first.");
     }
+    void _range(int start, int end, float *pos, float *vel) {
+        if (start < 0 || end > (int)getNumMarkerParticles() || start > end)
+            throw std::domain_error("Error: invalid range.");
+        if (start == end) return;
+        detail::check(flipmi_scene_get_particles_range(_scene, start, end, pos, vel),
+                      "FluidSimulation::getMarkerParticleDataRange");
+    }
 
     int _isize, _jsize, _ksize;
     double _dx;
@@ -825,6 +858,8 @@ private:
     int _currentFrame = 0;
     int _maxThreads = 1;
     int64_t _maxParticles = 0;
+    vmath::vec3 _domainOffset;
+    double _domainScale = 1.0;
     std::vector<float> _loadPos, _loadVel;
     FluidSimulationFrameStats _stats;
     std::unique_ptr<MACVelocityField> _mac;

@@ -60,6 +60,8 @@ class _Base:
         getattr(L, p + "sim_set_pic_ratio").argtypes = [_vp, _c_double]
         getattr(L, p + "sim_load_particles").argtypes = [_vp, _vp, _vp, _c_int]
         getattr(L, p + "sim_get_particles").argtypes = [_vp, _vp, _vp]
+        getattr(L, p + "sim_set_domain").argtypes = [_vp, _c_double, _c_double, _c_double, _c_double]
+        getattr(L, p + "sim_get_particle_range").argtypes = [_vp, _c_int, _c_int, _vp, _vp]
         getattr(L, p + "sim_get_float_grid").argtypes = [_vp, _c_int, _vp]
         getattr(L, p + "sim_get_bool_grid").argtypes = [_vp, _c_int, _vp]
         getattr(L, p + "domain_solid").argtypes = [_c_int] * 3 + [_c_double] + [_vp] * 5
@@ -202,6 +204,19 @@ class _SimHandle:
         vel = np.zeros((n, 3), np.float32)
         self._f("sim_get_particles")(self.h, _ptr(pos), _ptr(vel))
         return pos, vel
+
+    def set_domain(self, offset, scale):
+        """setDomainOffset + setDomainScale (fluidsimulation.cpp:307-330)."""
+        self._f("sim_set_domain")(self.h, float(offset[0]), float(offset[1]), float(offset[2]), float(scale))
+
+    def particle_range(self, start, end):
+        """getMarkerParticle{Position,Velocity}DataRange: positions in domain
+        coordinates; None for a range the reference rejects."""
+        n = max(end - start, 0)
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        rc = self._f("sim_get_particle_range")(self.h, start, end, _ptr(pos) if n else None, _ptr(vel) if n else None)
+        return None if rc != 0 else (pos, vel)
 
     def substeps(self):
         return self._f("sim_frame_substeps")(self.h)

@@ -1211,6 +1211,9 @@ struct Sim {
     int pcgMaxIter = 1000;            // fluidsimulation.h:1659
     std::vector<V3> bodyForces;
     std::vector<V3> pos, vel;
+    std::vector<V3> loadPos, loadVel;  // loadMarkerParticleData queue (applied by initialize)
+    V3 domainOffset;                   // fluidsimulation.h:1612-1613
+    double domainScale = 1.0;
     bool initialized = false, solidUpToDate = false, weightsUpToDate = false;
     bool outputInitialized = false;
     int currentFrame = 0, stepNumber = 0, lastSubsteps = 0;
@@ -1676,22 +1679,53 @@ void orc_sim_set_pcg(void *h, int mode, double tol, int maxiter) {
     Sim *S = (Sim *)h;
     S->pcgMode = mode; S->pcgTol = tol; S->pcgMaxIter = maxiter;
 }
-// FluidSimulation::loadMarkerParticleData + _loadMarkerParticles
-// (fluidsimulation.cpp:2488-2509, 2773-2785): keep points inside the domain.
+// FluidSimulation::loadMarkerParticleData (fluidsimulation.cpp:2488-2509):
+// the particles queue up; initialize() loads them.
 void orc_sim_load_particles(void *h, const float *pos, const float *vel, int n) {
     Sim *S = (Sim *)h;
-    Box b;
-    b.pos = V3(0, 0, 0);
-    b.w = S->I * S->dx; b.h = S->J * S->dx; b.d = S->K * S->dx;
     for (int i = 0; i < n; i++) {
-        V3 p(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-        if (!b.inside(p)) continue;
-        S->pos.push_back(p);
-        S->vel.push_back(V3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]));
+        S->loadPos.push_back(V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+        S->loadVel.push_back(V3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]));
     }
+}
+// setDomainOffset(double, double, double) / setDomainScale(double)
+// (fluidsimulation.cpp:307-330).
+void orc_sim_set_domain(void *h, double ox, double oy, double oz, double scale) {
+    Sim *S = (Sim *)h;
+    S->domainOffset = V3((float)ox, (float)oy, (float)oz);
+    S->domainScale = scale;
+}
+// getMarkerParticle{Position,Velocity}DataRange (fluidsimulation.cpp:
+// 2314-2338): positions leave in domain coordinates, p * scale + offset;
+// an invalid range is a domain_error (-1 here).
+int orc_sim_get_particle_range(void *h, int start, int end, float *pos, float *vel) {
+    Sim *S = (Sim *)h;
+    if (start < 0 || end > (int)S->pos.size() || start > end) return -1;
+    for (int i = start; i < end; i++) {
+        V3 p = S->pos[i] * (float)S->domainScale + S->domainOffset;
+        float *o = pos + 3 * (size_t)(i - start), *u = vel + 3 * (size_t)(i - start);
+        o[0] = p.x; o[1] = p.y; o[2] = p.z;
+        u[0] = S->vel[i].x; u[1] = S->vel[i].y; u[2] = S->vel[i].z;
+    }
+    return 0;
 }
 int orc_sim_initialize(void *h) {
     Sim *S = (Sim *)h;
+    if (S->initialized) return 0;
+    // _loadParticles / _loadMarkerParticles (fluidsimulation.cpp:2669-2673,
+    // 2773-2785): domain coordinates back to grid coordinates, then keep
+    // the points inside the grid box.
+    Box b;
+    b.pos = V3(0, 0, 0);
+    b.w = S->I * S->dx; b.h = S->J * S->dx; b.d = S->K * S->dx;
+    for (size_t i = 0; i < S->loadPos.size(); i++) {
+        V3 p = (S->loadPos[i] - S->domainOffset) / (float)S->domainScale;
+        if (!b.inside(p)) continue;
+        S->pos.push_back(p);
+        S->vel.push_back(S->loadVel[i]);
+    }
+    S->loadPos.clear();
+    S->loadVel.clear();
     // _initializeParticleRadii (fluidsimulation.cpp:2644-2650).
     S->radius = 0.5 * 1.0 * S->dx * std::sqrt(3.0);
     S->initialized = true;

@@ -149,6 +149,24 @@ void ref_sim_load_particles(void *hp, const float *pos, const float *vel, int n)
     ((RefSim *)hp)->sim->loadMarkerParticleData(d);
 }
 
+void ref_sim_set_domain(void *hp, double ox, double oy, double oz, double scale) {
+    FluidSimulation *s = ((RefSim *)hp)->sim;
+    s->setDomainOffset(ox, oy, oz);
+    s->setDomainScale(scale);
+}
+
+int ref_sim_get_particle_range(void *hp, int start, int end, float *pos, float *vel) {
+    RefSim *h = (RefSim *)hp;
+    try {
+        h->sim->getMarkerParticlePositionDataRange(start, end, (char *)pos);
+        h->sim->getMarkerParticleVelocityDataRange(start, end, (char *)vel);
+    } catch (std::exception &e) {
+        h->last_error = e.what();
+        return -1;
+    }
+    return 0;
+}
+
 int ref_sim_initialize(void *hp) {
     RefSim *h = (RefSim *)hp;
     try {

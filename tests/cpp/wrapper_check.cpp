@@ -232,6 +232,23 @@ int run_sim() {
     }
     write_raw("positions_out.f32", pblob.data(), pblob.size());
     write_raw("velocities_out.f32", vblob.data(), vblob.size());
+    // fluidsimulation.cpp:2314-2338: a sub-range, and the domain_error of an
+    // invalid one.
+    if (n >= 8) {
+        std::vector<char> pr(12 * (size_t)(n - 5)), vr(12 * (size_t)(n - 5));
+        sim.getMarkerParticlePositionDataRange(2, (int)n - 3, pr.data());
+        sim.getMarkerParticleVelocityDataRange(2, (int)n - 3, vr.data());
+        write_raw("positions_range.f32", pr.data(), pr.size());
+        write_raw("velocities_range.f32", vr.data(), vr.size());
+    }
+    bool range_threw = false;
+    try {
+        std::vector<char> tmp(24);
+        sim.getMarkerParticlePositionDataRange(3, 1, tmp.data());
+    } catch (const std::domain_error &) {
+        range_threw = true;
+    }
+    if (!range_threw) throw std::runtime_error("an invalid range did not throw std::domain_error");
     MACVelocityField *mac = sim.getVelocityField();
     save(*mac->getArray3dU(), "u_out.f32");
     save(*mac->getArray3dV(), "v_out.f32");

@@ -163,3 +163,8 @@ def test_fluid_simulation(tmp_path, oracle):
     assert pg.shape == po.shape
     assert np.abs(pg - po).max() <= 1e-3 * dx
     assert np.abs(vg - vo).max() <= 1e-3 * max(1.0, float(np.abs(vo).max()))
+    # getMarkerParticle{Position,Velocity}DataRange(2, n - 3): the same bytes.
+    n = pg.shape[0]
+    pr = np.fromfile(tmp_path / "positions_range.f32", np.float32).reshape(-1, 3)
+    vr = np.fromfile(tmp_path / "velocities_range.f32", np.float32).reshape(-1, 3)
+    assert bits_equal(pr, pg[2:n - 3]) and bits_equal(vr, vg[2:n - 3])

@@ -500,3 +500,33 @@ def test_pocket_conditioning_zeroes_only_closed_multi_cell_regions(oracle):
         assert np.array_equal(got, w)
     # The conditioned system is consistent: the solve converged to 1e-9 |b|.
     assert out["err"] <= 1e-9 * np.abs(out["rhs"]).max()
+
+
+def test_domain_transform_closed_form(oracle):
+    """With scale 2 and an integer offset every mapping is exact: particles
+    loaded at 2 p + o are admitted at p, read back at 2 p + o, and the grid
+    box [0, I dx) keeps p = 0 and drops p = I dx (AABB::isPointInside)."""
+    I, J, K, dx = 8, 8, 8, 0.125
+    p = np.array([[0.0, 0.0, 0.0], [0.5, 0.25, 0.75], [0.9375, 0.5, 0.5], [1.0, 0.5, 0.5], [0.5, 0.5, -0.125]],
+                 np.float32)
+    off = np.array([1.0, -2.0, 3.0], np.float32)
+    pd = (p * np.float32(2.0) + off).astype(np.float32)
+    h = oracle.sim(I, J, K, dx)
+    h.set_domain(off, 2.0)
+    h.load_particles(pd, np.zeros_like(pd))
+    h.initialize()
+    assert h.num_particles == 3
+    inner, _ = h.particles()
+    assert np.array_equal(inner, p[:3])
+    back, _ = h.particle_range(0, 3)
+    assert np.array_equal(back, pd[:3])
+    # The reverse map is vmath's: (p - o) * (float)(1.0 / (float)s).
+    s = np.float32(0.37)
+    inv = np.float32(1.0 / np.float64(s))
+    q = np.array([[0.31, 0.17, 0.05]], np.float32)
+    h2 = oracle.sim(I, J, K, dx)
+    h2.set_domain((0.0, 0.0, 0.0), float(s))
+    h2.load_particles(q, np.zeros_like(q))
+    h2.initialize()
+    got, _ = h2.particles()
+    assert np.array_equal(got, (q * inv).astype(np.float32))

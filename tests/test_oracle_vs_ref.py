@@ -159,3 +159,51 @@ def test_pressure_solve_pockets_reference_exact(oracle, reference):
     assert bits_equal(a["soln"], b["soln"])
     for x, y in zip(a["solid_vel"] + a["mac"] + a["valid"], b["solid_vel"] + b["mac"] + b["valid"]):
         assert bits_equal(x, y)
+
+
+def _domain_particles(s, offset, scale, seed=9):
+    """The scene's particles in domain coordinates (p * scale + offset in
+    float32, as getMarkerParticlePositionData writes them), plus points that
+    fall outside the grid box once mapped back."""
+    off = np.asarray(offset, np.float32)
+    pd = (s.positions * np.float32(scale) + off).astype(np.float32)
+    box = np.array([s.isize, s.jsize, s.ksize], np.float32) * np.float32(s.dx)
+    outside = np.stack([off - np.float32(0.01), off + box * np.float32(scale) + np.float32(0.01),
+                        off + np.float32(0.5 * scale) * box * np.array([1, 1, 3], np.float32)]).astype(np.float32)
+    pos = np.concatenate([pd, outside]).astype(np.float32)
+    vel = random_velocities(pos.shape[0], seed=seed, scale=0.3)
+    return pos, vel
+
+
+@pytest.mark.parametrize("offset,scale", [((0.3, -0.2, 0.15), 2.5), ((0.0, 0.0, 0.0), 1.0), ((-1.0, 4.0, 0.5), 0.37)])
+def test_particle_io_domain_coordinates(oracle, reference, offset, scale):
+    """setDomainOffset / setDomainScale, loadMarkerParticleData admitted at
+    initialize (domain to grid coordinates, then the grid-box filter) and the
+    ranged vec3 read-back, bit for bit after one frame; invalid ranges are
+    rejected by both."""
+    I, J, K = 18, 14, 12
+    s = dam(I, J, K, seed=4)
+    pos, vel = _domain_particles(s, offset, scale)
+    reference.set_threads(1)
+    sims = []
+    for lib in (oracle, reference):
+        h = lib.sim(I, J, K, s.dx)
+        h.add_body_force(0.0, -9.81, 0.0)
+        h.set_domain(offset, scale)
+        h.load_particles(pos, vel)
+        assert h.num_particles == 0  # queued until initialize
+        h.initialize()
+        sims.append(h)
+    a, b = sims
+    assert a.num_particles == b.num_particles
+    assert a.num_particles <= s.num_particles  # the outside points were dropped
+    for h in sims:
+        h.update(1 / 60)
+    n = a.num_particles
+    assert n == b.num_particles
+    for rng in ((0, n), (3, 17), (n - 5, n), (7, 7)):
+        ra, rb = a.particle_range(*rng), b.particle_range(*rng)
+        assert ra is not None and rb is not None
+        assert bits_equal(ra[0], rb[0]) and bits_equal(ra[1], rb[1]), f"range {rng}"
+    for bad in ((-1, 3), (0, n + 1), (5, 4)):
+        assert a.particle_range(*bad) is None and b.particle_range(*bad) is None
