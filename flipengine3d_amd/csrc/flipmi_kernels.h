@@ -149,6 +149,10 @@ struct PressureBuffers {
     // word g / 64 of the tile's mask_words words (null: treat all as active).
     unsigned long long *group_mask;
     int mask_words;
+    // The active groups in ascending order (rebuilt with the mask for every
+    // solve) and their number; the per-group PCG kernels walk this list.
+    int *active_list;
+    int *active_count;
     unsigned int epoch;                 // tag of the current sweep launch (host counter)
     unsigned int *progress;             // factorisation: tile progress counters
     int ntiles_j, ntiles_k;

@@ -62,7 +62,21 @@ __device__ __forceinline__ bool interior(const GridGeom &g, int i, int j, int k)
     return i >= 1 && j >= 1 && k >= 1 && i <= g.I - 2 && j <= g.J - 2 && k <= g.K - 2;
 }
 
-// Whether slot group grp (256 consecutive slots) holds a row.
+// Cell of slot p for a stencil.  A slot off the interior (skew corner,
+// padding, wall layer) is moved to cell (1, 1, 1), so that every neighbour
+// slot it reads is in range; the caller discards its result.
+__device__ __forceinline__ bool stencil_cell(const GridGeom &g, const SlotMap &sm, size_t p, int &i, int &j,
+                                             int &k) {
+    decode_slot(sm, p, i, j, k);
+    const bool in = interior(g, i, j, k);
+    if (!in) {
+        i = 1;
+        j = 1;
+        k = 1;
+    }
+    return in;
+}
+
 // Slot groups of this block in the per-group kernels.  Blocks are
 // dispatched to the eight XCDs round-robin (block b on XCD b % 8); the
 // blocks of one XCD cover one contiguous eighth of the groups, so the
@@ -81,12 +95,39 @@ __device__ __forceinline__ GroupRange xcd_groups(size_t ngroups) {
     return GroupRange{b + l, e, gridDim.x / kXcds};
 }
 
+// Whether slot group grp (256 consecutive slots) holds a row.
 __device__ __forceinline__ bool group_active(const PressureBuffers &pb, size_t grp) {
     if (!pb.group_mask) return true;
     const size_t S4 = (size_t)(pb.sm.S >> 2);
     const size_t tile = grp / S4;
     const int gi = (int)(grp % S4);
     return (pb.group_mask[tile * pb.mask_words + (gi >> 6)] >> (gi & 63)) & 1ull;
+}
+
+// Entries of the active group list for this block: as in xcd_groups, the
+// blocks of one XCD take one contiguous eighth of the list.
+struct ListRange {
+    int begin, end, step;
+};
+__device__ __forceinline__ ListRange xcd_list(int n) {
+    const int x = (int)(blockIdx.x % kXcds), l = (int)(blockIdx.x / kXcds);
+    const int part = (n + (int)kXcds - 1) / (int)kXcds;
+    const int b = x * part, e = b + part < n ? b + part : n;
+    return ListRange{b + l, e, (int)(gridDim.x / kXcds)};
+}
+// This thread's slots in the U list entries of one round (entries e0,
+// e0 + step, ...).  An entry past the block's end repeats the round's first
+// entry: its values are loaded like the others and never stored, so every
+// load of the round is issued before any result is used.
+template <int U>
+__device__ __forceinline__ void list_slots(const PressureBuffers &pb, const ListRange &lr, int e0,
+                                           size_t (&p)[U], bool (&ok)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int e = e0 + u * lr.step;
+        ok[u] = e < lr.end;
+        p[u] = ((size_t)pb.active_list[ok[u] ? e : e0] << 8) + threadIdx.x;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -199,12 +240,64 @@ __global__ __launch_bounds__(256) void k_group_mask(PressureBuffers pb) {
     }
 }
 
+// The active group list: the groups whose mask bit is set, ascending.  One
+// block: per tile the number of active groups (popcounts of its mask
+// words), an exclusive scan over the tiles, then each tile writes its group
+// ids from its offset.  Groups of the last mask word past the tile's length
+// are never set (the mask is cleared before it is built).
+__global__ __launch_bounds__(1024) void k_active_list(PressureBuffers pb) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int ntiles = pb.sm.ntj * pb.sm.ntk, S4 = pb.sm.S >> 2, W = pb.mask_words;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int t0 = 0; t0 < ntiles; t0 += 1024) {
+        const int t = t0 + (int)threadIdx.x;
+        int c = 0;
+        if (t < ntiles)
+            for (int w = 0; w < W; w++) c += __popcll(pb.group_mask[(size_t)t * W + w]);
+        int incl = c;  // inclusive scan within the wave
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int wbase = 0, total = 0;
+        for (int q = 0; q < 16; q++) {
+            const int v = wsum[q];
+            wbase += q < wave ? v : 0;
+            total += v;
+        }
+        int pos = carry + wbase + incl - c;
+        if (t < ntiles) {
+            for (int w = 0; w < W; w++) {
+                unsigned long long m = pb.group_mask[(size_t)t * W + w];
+                while (m) {
+                    const int bit = __ffsll((long long)m) - 1;
+                    m &= m - 1ull;
+                    pb.active_list[pos++] = t * S4 + w * 64 + bit;
+                }
+            }
+        }
+        __syncthreads();  // every read of wsum and carry is done
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *pb.active_count = carry;
+}
+
 hipError_t launch_group_mask(PressureBuffers &pb, hipStream_t s) {
-    if (!pb.group_mask) return hipSuccess;
+    // The PCG kernels walk the list built here: without a mask they would
+    // see no rows.
+    if (!pb.group_mask || !pb.active_list) return hipErrorInvalidValue;
     const size_t words = (size_t)pb.sm.ntj * pb.sm.ntk * pb.mask_words;
     hipError_t e = launch_fill_u8((uint8_t *)pb.group_mask, words * sizeof(unsigned long long), 0, s);
     if (e != hipSuccess) return e;
     k_group_mask<<<kRedBlocks * 4, 256, 0, s>>>(pb);
+    k_active_list<<<1, 1024, 0, s>>>(pb);
     return hipGetLastError();
 }
 
@@ -1209,18 +1302,28 @@ __device__ __forceinline__ double spmv_slot(const PressureBuffers &pb, const dou
 __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBuffers pb) {
     __shared__ double sh[kRedThreads / 64];
     if (pb.sc->done) return;
-    size_t n = pb.sm.count();
     double acc = 0.0;
-    const GroupRange gr = xcd_groups(n >> 8);
-    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
-        if (!group_active(pb, grp)) continue;  // uniform per block
-        const size_t p = (grp << 8) + threadIdx.x;
-        int i, j, k;
-        decode_slot(pb.sm, p, i, j, k);
-        double zc = 0.0;
-        if (interior(g, i, j, k)) zc = spmv_slot(pb, pb.s, p, i, j, k);
-        pb.z[p] = zc;
-        acc += pb.s[p] * zc;
+    const ListRange lr = xcd_list(*pb.active_count);
+    constexpr int U = 2;
+    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
+        size_t p[U];
+        bool ok[U], in[U];
+        double zc[U], sp[U];
+        list_slots<U>(pb, lr, e0, p, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int i, j, k;
+            in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
+            zc[u] = spmv_slot(pb, pb.s, p[u], i, j, k);
+            sp[u] = pb.s[p[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;  // uniform
+            const double z = in[u] ? zc[u] : 0.0;
+            pb.z[p[u]] = z;
+            acc += sp[u] * z;
+        }
     }
     double bs = block_reduce<0>(acc, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[blockIdx.x], bs);
@@ -1243,19 +1346,30 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBu
 __global__ __launch_bounds__(kRedThreads) void k_axpy_norm(GridGeom g, PressureBuffers pb) {
     __shared__ double sh[kRedThreads / 64];
     if (pb.sc->done) return;
-    size_t n = pb.sm.count();
     float af = pb.sc->alpha_f;
     float naf = (float)(-pb.sc->alpha);
     double mx = 0.0;
-    const GroupRange gr = xcd_groups(n >> 8);
-    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
-        if (!group_active(pb, grp)) continue;  // uniform per block
-        const size_t p = (grp << 8) + threadIdx.x;
-        pb.x[p] += af * pb.s[p];
-        double r = pb.r[p] + naf * pb.z[p];
-        pb.r[p] = r;
-        double a = fabs(r);
-        mx = a > mx ? a : mx;
+    const ListRange lr = xcd_list(*pb.active_count);
+    constexpr int U = 4;
+    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
+        size_t p[U];
+        bool ok[U];
+        double xs[U], ss[U], rs[U], zs[U];
+        list_slots<U>(pb, lr, e0, p, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            xs[u] = pb.x[p[u]]; ss[u] = pb.s[p[u]];
+            rs[u] = pb.r[p[u]]; zs[u] = pb.z[p[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;  // uniform
+            pb.x[p[u]] = xs[u] + af * ss[u];
+            const double r = rs[u] + naf * zs[u];
+            pb.r[p[u]] = r;
+            const double a = fabs(r);
+            mx = a > mx ? a : mx;
+        }
     }
     double bm = block_reduce<1>(mx, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
@@ -1281,16 +1395,26 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm(GridGeom g, PressureB
 }
 
 // s = z + beta s (pcgsolver.h:295-296), or s = z on the first application.
-__global__ void k_update_s(GridGeom g, PressureBuffers pb, int first) {
+__global__ __launch_bounds__(kRedThreads) void k_update_s(GridGeom g, PressureBuffers pb, int first) {
     if (pb.sc->done) return;
-    size_t n = pb.sm.count();
     float bf = first ? 0.0f : pb.sc->beta_f;
-    const GroupRange gr = xcd_groups(n >> 8);
-    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
-        if (!group_active(pb, grp)) continue;  // uniform per block
-        const size_t p = (grp << 8) + threadIdx.x;
-        if (first) pb.s[p] = pb.z[p];
-        else pb.s[p] = pb.z[p] + bf * pb.s[p];
+    const ListRange lr = xcd_list(*pb.active_count);
+    constexpr int U = 4;
+    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
+        size_t p[U];
+        bool ok[U];
+        double zs[U], ss[U];
+        list_slots<U>(pb, lr, e0, p, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            zs[u] = pb.z[p[u]];
+            ss[u] = pb.s[p[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;  // uniform
+            pb.s[p[u]] = first ? zs[u] : zs[u] + bf * ss[u];
+        }
     }
 }
 
@@ -1519,18 +1643,28 @@ __device__ __forceinline__ float spmv_slot32(const PressureBuffers &pb, const fl
 __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, PressureBuffers pb) {
     __shared__ double sh[kRedThreads / 64];
     if (pb.sc->done) return;
-    size_t n = pb.sm.count();
     double acc = 0.0;
-    const GroupRange gr = xcd_groups(n >> 8);
-    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
-        if (!group_active(pb, grp)) continue;  // uniform per block
-        const size_t p = (grp << 8) + threadIdx.x;
-        int i, j, k;
-        decode_slot(pb.sm, p, i, j, k);
-        float zc = 0.0f;
-        if (interior(g, i, j, k)) zc = spmv_slot32(pb, pb.s32, p, i, j, k);
-        pb.z32[p] = zc;
-        acc += (double)pb.s32[p] * (double)zc;
+    const ListRange lr = xcd_list(*pb.active_count);
+    constexpr int U = 2;
+    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
+        size_t p[U];
+        bool ok[U], in[U];
+        float zc[U], sp[U];
+        list_slots<U>(pb, lr, e0, p, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int i, j, k;
+            in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
+            zc[u] = spmv_slot32(pb, pb.s32, p[u], i, j, k);
+            sp[u] = pb.s32[p[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;  // uniform
+            const float z = in[u] ? zc[u] : 0.0f;
+            pb.z32[p[u]] = z;
+            acc += (double)sp[u] * (double)z;
+        }
     }
     double bs = block_reduce<0>(acc, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[blockIdx.x], bs);
@@ -1553,20 +1687,31 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, Pressure
 __global__ __launch_bounds__(kRedThreads) void k_axpy_norm32(GridGeom g, PressureBuffers pb) {
     __shared__ double sh[kRedThreads / 64];
     if (pb.sc->done) return;
-    size_t n = pb.sm.count();
     double alpha = pb.sc->alpha;
     float naf = -pb.sc->alpha_f;
     double mx = 0.0;
-    const GroupRange gr = xcd_groups(n >> 8);
-    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
-        if (!group_active(pb, grp)) continue;  // uniform per block
-        const size_t p = (grp << 8) + threadIdx.x;
-        float sp = pb.s32[p];
-        pb.x[p] += alpha * (double)sp;
-        float r = pb.r32[p] + naf * pb.z32[p];
-        pb.r32[p] = r;
-        double a = fabs((double)r);
-        mx = a > mx ? a : mx;
+    const ListRange lr = xcd_list(*pb.active_count);
+    constexpr int U = 4;
+    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
+        size_t p[U];
+        bool ok[U];
+        double xs[U];
+        float ss[U], rs[U], zs[U];
+        list_slots<U>(pb, lr, e0, p, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            xs[u] = pb.x[p[u]]; ss[u] = pb.s32[p[u]];
+            rs[u] = pb.r32[p[u]]; zs[u] = pb.z32[p[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;  // uniform
+            pb.x[p[u]] = xs[u] + alpha * (double)ss[u];
+            const float r = rs[u] + naf * zs[u];
+            pb.r32[p[u]] = r;
+            const double a = fabs((double)r);
+            mx = a > mx ? a : mx;
+        }
     }
     double bm = block_reduce<1>(mx, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
@@ -1591,19 +1736,29 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm32(GridGeom g, Pressur
 __global__ __launch_bounds__(kRedThreads) void k_true_residual(GridGeom g, PressureBuffers pb) {
     __shared__ double sh[kRedThreads / 64];
     if (pb.sc->done || !pb.sc->refresh) return;
-    size_t n = pb.sm.count();
     double mx = 0.0;
-    const GroupRange gr = xcd_groups(n >> 8);
-    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
-        if (!group_active(pb, grp)) continue;  // uniform per block
-        const size_t p = (grp << 8) + threadIdx.x;
-        int i, j, k;
-        decode_slot(pb.sm, p, i, j, k);
-        double r = 0.0;
-        if (interior(g, i, j, k)) r = pb.rhs[p] - spmv_slot(pb, pb.x, p, i, j, k);
-        pb.r32[p] = (float)r;
-        double a = fabs(r);
-        mx = a > mx ? a : mx;
+    const ListRange lr = xcd_list(*pb.active_count);
+    constexpr int U = 2;
+    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
+        size_t p[U];
+        bool ok[U], in[U];
+        double ax[U], bs[U];
+        list_slots<U>(pb, lr, e0, p, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int i, j, k;
+            in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
+            ax[u] = spmv_slot(pb, pb.x, p[u], i, j, k);
+            bs[u] = pb.rhs[p[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;  // uniform
+            const double r = in[u] ? bs[u] - ax[u] : 0.0;
+            pb.r32[p[u]] = (float)r;
+            const double a = fabs(r);
+            mx = a > mx ? a : mx;
+        }
     }
     double bm = block_reduce<1>(mx, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
@@ -1630,16 +1785,26 @@ __global__ __launch_bounds__(kRedThreads) void k_true_residual(GridGeom g, Press
     }
 }
 
-__global__ void k_update_s32(PressureBuffers pb, int first) {
+__global__ __launch_bounds__(kRedThreads) void k_update_s32(PressureBuffers pb, int first) {
     if (pb.sc->done) return;
-    size_t n = pb.sm.count();
     float bf = first ? 0.0f : pb.sc->beta_f;
-    const GroupRange gr = xcd_groups(n >> 8);
-    for (size_t grp = gr.begin; grp < gr.end; grp += gr.step) {
-        if (!group_active(pb, grp)) continue;  // uniform per block
-        const size_t p = (grp << 8) + threadIdx.x;
-        if (first) pb.s32[p] = pb.z32[p];
-        else pb.s32[p] = pb.z32[p] + bf * pb.s32[p];
+    const ListRange lr = xcd_list(*pb.active_count);
+    constexpr int U = 4;
+    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
+        size_t p[U];
+        bool ok[U];
+        float zs[U], ss[U];
+        list_slots<U>(pb, lr, e0, p, ok);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            zs[u] = pb.z32[p[u]];
+            ss[u] = pb.s32[p[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;  // uniform
+            pb.s32[p[u]] = first ? zs[u] : zs[u] + bf * ss[u];
+        }
     }
 }
 

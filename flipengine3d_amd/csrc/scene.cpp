@@ -109,7 +109,8 @@ void collision_box(int isize, int jsize, int ksize, double dx, float pos[3], dou
 // ---------------------------------------------------------------------------
 
 size_t PressureStore::device_bytes() const {
-    size_t b = edge.bytes() + mask.bytes() + progress.bytes() + ticket.bytes() + irange.bytes() + scal.bytes();
+    size_t b = edge.bytes() + mask.bytes() + progress.bytes() + ticket.bytes() + irange.bytes() + scal.bytes() +
+               alist.bytes();
     for (int q = 0; q <= kSlotArrays; q++) b += arr[q].bytes();
     for (int q = 0; q < kSlotArrays32; q++) b += arr32[q].bytes();
     return b;
@@ -162,6 +163,12 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     // Every group counts as active until the first assembly marks them.
     (void)launch_fill_u8((uint8_t *)mask.p, mask.n * sizeof(unsigned long long), 0xff, s);
     pb.group_mask = mask.p;
+    const size_t ngroups = sm.count() >> 8;
+    if (alist.alloc(ngroups + 1) != hipSuccess)
+        throw StatusError{FLIPMI_ERR_OOM, "pressure scratch: out of device memory"};
+    (void)launch_fill_i32(alist.p, ngroups + 1, 0, s);
+    pb.active_list = alist.p;
+    pb.active_count = alist.p + ngroups;
     pb.epoch = 0;
     pb.progress = progress.p;
     pb.ntiles_j = ntj;
@@ -521,6 +528,7 @@ size_t estimate_scene_bytes(const flipmi_scene_desc &d, int64_t cap) {
     b += sizeof(unsigned long long) * edge_record_count(sm);
     b += sizeof(unsigned long long) * ntiles * (size_t)((sm.S / 4 + 63) / 64);
     b += sizeof(unsigned) * (ntiles + 1) + sizeof(int) * 2 * ntiles + sizeof(PcgScalars);
+    b += sizeof(int) * ((sm.count() >> 8) + 1);
     return b;
 }
 

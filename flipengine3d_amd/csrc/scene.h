@@ -182,6 +182,7 @@ struct PressureStore {
     DBuf<unsigned long long> edge, mask;
     DBuf<unsigned> progress, ticket;
     DBuf<int> irange;
+    DBuf<int> alist;  // active group list (groups) and its length (last element)
     DBuf<PcgScalars> scal;
     PressureBuffers pb{};
     void alloc(const GridGeom &g, hipStream_t s, bool mixed);
