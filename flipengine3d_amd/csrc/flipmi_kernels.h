@@ -137,9 +137,8 @@ struct PressureBuffers {
     double *x, *r, *z, *s;              // PCG vectors (FP64)
     double *invdiag, *Lx, *Ly, *Lz;     // MIC(0) factor: column entries (rows +x/+y/+z)
     double *Lxm, *Lym, *Lzm;            // the same entries read from the row: columns -x/-y/-z
-    // Mixed precision (null unless allocated): FP32 copies of the matrix and
-    // factor, and the FP32 PCG vectors; x stays FP64.
-    float *diag32, *offx32, *offy32, *offz32;
+    // Mixed precision (null unless allocated): FP32 copies of the factor and
+    // the FP32 PCG vectors; x and the matrix stay FP64.
     float *invdiag32, *Lx32, *Ly32, *Lz32, *Lxm32, *Lym32, *Lzm32;
     float *r32, *z32, *s32;
     double *partials;                   // per-block partial sums
