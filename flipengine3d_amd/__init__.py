@@ -36,7 +36,7 @@ PCG_MIXED = 1
 
 STAGES = ["sort", "classify", "p2g", "extrapolate", "assemble", "mic0", "pcg", "apply",
           "g2p_advect", "remove", "glue", "pcg_iters", "k_spmv_dot", "k_axpy_norm", "k_forward",
-          "k_backward", "k_update_s", "k_true_residual"]
+          "k_backward", "k_update_s", "k_true_residual", "inflow"]
 NUM_STAGES = 24
 
 _c_int = ctypes.c_int
@@ -107,6 +107,8 @@ EXPORTED = [
     "flipmi_scene_create", "flipmi_scene_destroy", "flipmi_scene_load_particles",
     "flipmi_scene_initialize", "flipmi_scene_update", "flipmi_scene_num_particles",
     "flipmi_scene_get_particles", "flipmi_scene_get_particles_range", "flipmi_scene_set_domain",
+    "flipmi_scene_add_inflow_source", "flipmi_scene_set_inflow_source", "flipmi_scene_set_marker_jitter",
+    "flipmi_scene_get_inflow_source",
     "flipmi_scene_get_grid", "flipmi_scene_stage_times",
     "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
     "flipmi_liquid_sdf_postprocess", "flipmi_p2g",
@@ -142,6 +144,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_get_particles.argtypes = [_vp, _vp, _vp]
     L.flipmi_scene_get_particles_range.argtypes = [_vp, _c_i64, _c_i64, _vp, _vp]
     L.flipmi_scene_set_domain.argtypes = [_vp, _c_double, _c_double, _c_double, _c_double]
+    L.flipmi_scene_add_inflow_source.argtypes = [_vp, _vp, _c_int, _vp, _c_int, _vp, P(_c_int)]
+    L.flipmi_scene_set_inflow_source.argtypes = [_vp, _c_int, _c_int, _c_int, _c_int]
+    L.flipmi_scene_set_marker_jitter.argtypes = [_vp, _c_double, _c_int, ctypes.c_uint64]
+    L.flipmi_scene_get_inflow_source.argtypes = [_vp, _c_int, _vp, _vp, _vp]
     L.flipmi_scene_get_grid.argtypes = [_vp, _c_int, _vp]
     L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
     L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
@@ -445,6 +451,50 @@ class FluidSimulation:
         self.ctx.check(self.L.flipmi_scene_get_particles_range(self.h, start, end, _ptr(pos) if m else None,
                                                                _ptr(vel) if m else None))
         return pos, vel
+
+    # ---- mesh inflow sources ------------------------------------------------
+    def addMeshFluidSource(self, vertices: np.ndarray, triangles: np.ndarray, velocity) -> int:
+        """MeshFluidSource with a static mesh (updateMeshStatic), setInflow(),
+        setVelocity(velocity), added with addMeshFluidSource
+        (meshfluidsource.cpp:32-44, 74-76, 114-116; fluidsimulation.cpp:
+        1953-1965).  vertices: (n, 3) float32 in grid coordinates; triangles:
+        (m, 3) int32.  Returns the source's index."""
+        self._ensure()
+        v = np.ascontiguousarray(vertices, np.float32)
+        t = np.ascontiguousarray(triangles, np.int32)
+        vel = (ctypes.c_double * 3)(*[float(c) for c in velocity])
+        sid = _c_int(-1)
+        self.ctx.check(self.L.flipmi_scene_add_inflow_source(self.h, _ptr(v), v.shape[0], _ptr(t), t.shape[0],
+                                                             vel, ctypes.byref(sid)))
+        return sid.value
+
+    def setMeshFluidSource(self, source_id: int, enabled: bool = True, constrain_velocity: bool = True,
+                           substep_emissions: int = 1):
+        """enable()/disable(), enable/disableConstrainedFluidVelocity() and
+        setSubstepEmissions() of one source (meshfluidsource.cpp:51-72, 146-156)."""
+        self._ensure()
+        self.ctx.check(self.L.flipmi_scene_set_inflow_source(self.h, source_id, int(enabled),
+                                                             int(constrain_velocity), int(substep_emissions)))
+
+    def meshFluidSource(self, source_id: int):
+        """(sub-grid offset, nodal SDF (nk, nj, ni), cells (K, J, I)) of a
+        source (MeshFluidSource::getMeshLevelSet / getCells)."""
+        if self.h is None:
+            raise RuntimeError("Error: FluidSimulation must be initialized first.")
+        d = np.zeros(6, np.int32)
+        self.ctx.check(self.L.flipmi_scene_get_inflow_source(self.h, source_id, _ptr(d), None, None))
+        sdf = np.zeros((d[5], d[4], d[3]), np.float32)
+        cells = np.zeros((self.desc.ksize, self.desc.jsize, self.desc.isize), np.uint8)
+        self.ctx.check(self.L.flipmi_scene_get_inflow_source(self.h, source_id, _ptr(d), _ptr(sdf), _ptr(cells)))
+        return tuple(int(v) for v in d[:3]), sdf, cells
+
+    def setMarkerParticleJitter(self, factor: float = 1.0, jitter_surface: bool = False, seed: int = 0):
+        """setMarkerParticleJitterFactor + enable/disableJitterSurfaceMarkerParticles
+        (fluidsimulation.cpp:185-210) for inflow particles; `seed` keys the
+        jitter draws."""
+        self._ensure()
+        self.ctx.check(self.L.flipmi_scene_set_marker_jitter(self.h, float(factor), int(jitter_surface),
+                                                             int(seed)))
 
     def setDomainOffset(self, x: float, y: float, z: float):
         """fluidsimulation.cpp:307-318 (stored in float, like vmath::vec3)."""

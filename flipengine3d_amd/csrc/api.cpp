@@ -188,6 +188,58 @@ int flipmi_scene_set_domain(flipmi_scene *scene, double ox, double oy, double oz
     return FLIPMI_OK;
 }
 
+int flipmi_scene_add_inflow_source(flipmi_scene *scene, const float *verts, int nv, const int *tris, int nt,
+                                   const double velocity[3], int *source_id) {
+    if (!scene || !velocity) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    return guarded(s.ctx, [&] {
+        int id = s.add_inflow(verts, nv, tris, nt, velocity);
+        if (source_id) *source_id = id;
+    });
+}
+
+int flipmi_scene_get_inflow_source(flipmi_scene *scene, int id, int dims[6], float *sdf, uint8_t *cells) {
+    if (!scene || !dims) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    if (!s.initialized) return fail(s.ctx, FLIPMI_ERR_NOT_INITIALIZED, "scene not initialized");
+    if (id < 0 || id >= (int)s.sources.size()) return fail(s.ctx, FLIPMI_ERR_INVALID_ARG, "unknown inflow source");
+    return guarded(s.ctx, [&] {
+        s.ensure_inflow();
+        const InflowSource &src = *s.sources[(size_t)id];
+        dims[0] = src.go[0]; dims[1] = src.go[1]; dims[2] = src.go[2];
+        dims[3] = src.ni; dims[4] = src.nj; dims[5] = src.nk;
+        if (sdf) s.check(hipMemcpyAsync(sdf, src.sdf.p, src.sdf.bytes(), hipMemcpyDeviceToHost, s.stream), "copy");
+        if (cells) s.check(hipMemcpyAsync(cells, src.cells.p, src.cells.bytes(), hipMemcpyDeviceToHost, s.stream), "copy");
+        s.sync();
+    });
+}
+
+int flipmi_scene_set_inflow_source(flipmi_scene *scene, int id, int enabled, int constrain, int emissions) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    if (id < 0 || id >= (int)s.sources.size()) return fail(s.ctx, FLIPMI_ERR_INVALID_ARG, "unknown inflow source");
+    InflowSource &src = *s.sources[(size_t)id];
+    src.enabled = enabled != 0;
+    src.constrain = constrain != 0;
+    src.emissions = emissions < 0 ? 0 : emissions;  // meshfluidsource.cpp:63-68
+    return FLIPMI_OK;
+}
+
+int flipmi_scene_set_marker_jitter(flipmi_scene *scene, double factor, int jitter_surface, uint64_t seed) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    // The reference accepts any factor >= 0 (fluidsimulation.cpp:185-196);
+    // factors above 1 can move a particle out of its subcell, which the
+    // parallel emission does not order the way the reference's serial loop
+    // does, so they are rejected.
+    if (!(factor >= 0.0 && factor <= 1.0))
+        return fail(s.ctx, FLIPMI_ERR_INVALID_ARG, "marker particle jitter factor must be in [0, 1]");
+    s.jitter_factor = factor;
+    s.jitter_surface = jitter_surface != 0;
+    s.jitter_seed = seed;
+    return FLIPMI_OK;
+}
+
 int flipmi_scene_get_particles_range(flipmi_scene *scene, int64_t start, int64_t end, float *pos, float *vel) {
     if (!scene) return FLIPMI_ERR_INVALID_ARG;
     Scene &s = scene->s;

@@ -70,7 +70,7 @@ struct Dims {
     __device__ __forceinline__ size_t at(int i, int j, int k) const {
         return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
     }
-    __device__ __forceinline__ size_t count() const { return (size_t)ni * nj * nk; }
+    __host__ __device__ __forceinline__ size_t count() const { return (size_t)ni * nj * nk; }
 };
 
 // Interpolation::trilinearInterpolate(p[8], x, y, z) (interpolation.cpp:62-71).
@@ -79,6 +79,88 @@ __device__ __forceinline__ double trilerp8(const double *p, double x, double y, 
            p[2] * (1 - x) * y * (1 - z) + p[3] * (1 - x) * (1 - y) * z +
            p[4] * x * (1 - y) * z + p[5] * (1 - x) * y * z + p[6] * x * y * (1 - z) +
            p[7] * x * y * z;
+}
+
+// Collision::findClosestPointOnTriangle (collision.cpp:161-270), in the
+// reference's float/double mix.
+__device__ inline F3 closest_point_on_triangle(F3 p0, F3 v0, F3 v1, F3 v2) {
+    F3 edge0 = v1 - v0, edge1 = v2 - v0, pv = v0 - p0;
+    double a = dot3(edge0, edge0), b = dot3(edge0, edge1), c = dot3(edge1, edge1);
+    double d = dot3(edge0, pv), e = dot3(edge1, pv);
+    double det = a * c - b * b;
+    if (fabs(det) < 1e-9) {
+        float feps = 1e-6f;
+        float len1 = len3(v0 - v1), len2 = len3(v1 - v2), len3v = len3(v2 - v0);
+        if (len1 < feps && len2 < feps && len3v < feps) return v0;
+        F3 dp, dq;
+        if (len1 < feps) { dp = v0; dq = v2; }
+        else if (len2 < feps) { dp = v1; dq = v0; }
+        else { dp = v2; dq = v1; }
+        float num = dot3(p0 - dp, dq - dp), den = dot3(dq - dp, dq - dp);
+        if (fabsf(den) < feps) return v0;
+        float lam = num / den;
+        if (lam <= 0.0f) return dp;
+        if (lam >= 1.0f) return dq;
+        return dp + lam * (dq - dp);
+    }
+    double s = b * e - c * d, t = b * d - a * e;
+    auto clamp01 = [](double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); };
+    if (s + t < det) {
+        if (s < 0.0) {
+            if (t < 0.0) {
+                if (d < 0.0) { s = clamp01(-d / a); t = 0.0; }
+                else { s = 0.0; t = clamp01(-e / c); }
+            } else { s = 0.0; t = clamp01(-e / c); }
+        } else if (t < 0.0) { s = clamp01(-d / a); t = 0.0; }
+        else { double inv = 1.0 / det; s *= inv; t *= inv; }
+    } else {
+        if (s < 0.0) {
+            double tmp0 = b + d, tmp1 = c + e;
+            if (tmp1 > tmp0) { s = clamp01((tmp1 - tmp0) / (a - 2 * b + c)); t = 1 - s; }
+            else { t = clamp01(-e / c); s = 0.0; }
+        } else if (t < 0.0) {
+            if (a + d > b + e) { s = clamp01((c + e - b - d) / (a - 2 * b + c)); t = 1 - s; }
+            else { s = clamp01(-e / c); t = 0.0; }
+        } else {
+            s = clamp01((c + e - b - d) / (a - 2 * b + c)); t = 1.0 - s;
+        }
+    }
+    return v0 + (float)s * edge0 + (float)t * edge1;
+}
+
+// Interpolation::trilinearInterpolate(vec3, dx, Array3d<float>)
+// (interpolation.cpp:73-112) on a nodal grid of dims N: nodes outside the
+// grid read as 0.
+__device__ inline double nodal_trilinear(const float *__restrict__ phi, Dims N, double dx,
+                                         double invdx, F3 p) {
+    I3 c = pos_to_index(p, invdx);
+    F3 gp = idx_to_pos(c.i, c.j, c.k, dx);
+    double ix = (double)(p.x - gp.x) * invdx, iy = (double)(p.y - gp.y) * invdx,
+           iz = (double)(p.z - gp.z) * invdx;
+    double pts[8];
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                           {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        int a = c.i + off[q][0], b = c.j + off[q][1], d = c.k + off[q][2];
+        pts[q] = N.in(a, b, d) ? (double)phi[N.at(a, b, d)] : 0.0;
+    }
+    return (double)(float)trilerp8(pts, ix, iy, iz);
+}
+
+// SplitMix64 and a keyed uniform draw in [0, 1): the generator behind the
+// inflow jitter.  The reference draws rand() (fluidsimulation.h:1501-1503)
+// from several threads at once; here draw (seed, stream, item) is a pure
+// function of what it jitters, so results do not depend on scheduling.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ double draw_unit(uint64_t seed, uint64_t stream, uint64_t item) {
+    uint64_t h = splitmix64(splitmix64(seed ^ (stream * 0xD1B54A32D192ED03ull)) + item);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
 }
 
 }  // namespace flipmi

@@ -22,6 +22,14 @@ struct BoxMesh {
     double bext[3];  // boundary AABB extents (double)
 };
 
+// A mesh triangle for the ray-column inside test: vertices in the test's
+// local frame and the range of ray columns its bounding box covers.
+struct RayTri {
+    float v[3][3];
+    int c0[2], c1[2];
+};
+constexpr int kMaxRayCrossings = 64;  // crossings one ray column may collect
+
 // Grid geometry shared by all kernels.
 struct GridGeom {
     int I, J, K;
@@ -210,6 +218,59 @@ hipError_t launch_spmv(const GridGeom &g, PressureBuffers &pb, const double *x, 
 size_t edge_record_count(const SlotMap &sm);
 
 // ---- particles (k_particles.hip) ----
+// ---- mesh inflow sources (k_source.hip) ----
+// Device view of one inflow source.  sdf is nodal on the source's sub-grid
+// (ni x nj x nk nodes) whose origin is `off` in grid coordinates; cells is
+// the I*J*K byte grid of source cells, cmin/cmax their index box, list the
+// ncells source cells in raster order.
+struct InflowView {
+    const float *sdf;
+    int ni, nj, nk;
+    float off[3];
+    const uint8_t *cells;
+    int cmin[3], cmax[3];
+    const int *list;
+    int ncells;
+    float vel[3];
+    int emit;       // enabled: emits this substep
+    int constrain;  // enabled with constrained fluid velocity
+};
+constexpr int kMaxInflow = 8;
+struct InflowViews {
+    int n;
+    InflowView v[kMaxInflow];
+};
+struct EmitParams {
+    double jitter;       // 0.25 * (jitter factor - 1e-3) * dx
+    int jitter_surface;  // jitter every new particle, not only the deep ones
+    uint64_t seed, stream;
+};
+struct EmitOut {
+    float *px, *py, *pz, *vx, *vy, *vz;
+};
+hipError_t launch_mesh_exact_band(const BoxTri *tris, int ntri, int ni, int nj, int nk, double dx,
+                                  float upper, float *phi, hipStream_t s);
+hipError_t launch_ray_inside(const RayTri *tris, int ntri, int ci, int cj, int ck, double dx,
+                             const float jitter[3], const int go[3], int ni, int nj, int nk,
+                             int mode, uint8_t *nodes, float *phi, int *overflow, hipStream_t s);
+// Cells around inside nodes: byte grid, scan flags/positions and count.
+hipError_t launch_source_cell_scan(const uint8_t *nodes, int I, int J, int K, uint8_t *cells,
+                                  int *flags, int *pos, int *scan_tmp, int *count, hipStream_t s);
+hipError_t launch_source_cell_list(const int *flags, const int *pos, size_t n, int *list,
+                                   hipStream_t s);
+hipError_t launch_inflow_mask(const GridGeom &g, const float *px, const float *py, const float *pz,
+                              int n, const InflowViews &views, uint32_t *mask, hipStream_t s);
+hipError_t launch_inflow_emit(const GridGeom &g, const InflowView &v, const float *solid,
+                              uint32_t *mask, const EmitParams &ep, float *cand, int *flags,
+                              int *pos, int *scan_tmp, int *nbase, const EmitOut &out,
+                              hipStream_t s);
+// Body force on one MAC component that skips the valid faces an inflow
+// source constrains (fluidsimulation.cpp:3372-3438, 3450-3489).
+hipError_t launch_save_and_body_force_inflow(const GridGeom &g, int comp, const float *mac,
+                                             float *saved, float add, int do_add,
+                                             const uint8_t *valid, const InflowViews &views,
+                                             hipStream_t s);
+
 struct ParticleStepArgs {
     float *px, *py, *pz, *vx, *vy, *vz;  // in/out, canonical order
     int n;
@@ -228,6 +289,7 @@ struct ParticleStepArgs {
     int *speed_hist;                     // [maxsteps] out (accumulated)
     double speed_step;
     int maxsteps;
+    InflowViews inflow;                  // sources whose particles take their velocity
 };
 hipError_t launch_particle_step(const GridGeom &g, const ParticleStepArgs &a, hipStream_t s);
 hipError_t launch_max_speed(const float *vx, const float *vy, const float *vz, int n,

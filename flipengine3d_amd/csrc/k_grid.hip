@@ -2,6 +2,7 @@
 // grid with the body force, and the solid velocity constraint.
 #include "device_math.h"
 #include "flipmi_kernels.h"
+#include "inflow_device.h"
 
 namespace flipmi {
 
@@ -98,6 +99,39 @@ hipError_t launch_save_and_body_force(size_t n, const float *mac, float *saved, 
                                       int do_add, hipStream_t s) {
     k_save_force<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, const_cast<float *>(mac), saved,
                                                              add, do_add);
+    return hipGetLastError();
+}
+
+// The same with the inflow constraint: valid faces held by an inflow source
+// take no body force (fluidsimulation.cpp:3372-3438, 3450-3489).  Face
+// positions are Grid3d::FaceIndexToPositionU/V/W (grid3d.h:111-133).
+__global__ void k_save_force_inflow(GridGeom g, int comp, size_t n, float *mac, float *saved,
+                                    float add, int do_add, const uint8_t *__restrict__ valid,
+                                    InflowViews views) {
+    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    float v = mac[f];
+    saved[f] = v;
+    if (!do_add) return;
+    if (valid[f]) {
+        const int ni = g.I + (comp == 0), nj = g.J + (comp == 1);
+        int i = (int)(f % ni), j = (int)((f / ni) % nj), k = (int)(f / ((size_t)ni * nj));
+        F3 p = f3(comp == 0 ? (float)((double)(float)i * g.dx) : (float)(((double)(float)i + 0.5) * g.dx),
+                  comp == 1 ? (float)((double)(float)j * g.dx) : (float)(((double)(float)j + 0.5) * g.dx),
+                  comp == 2 ? (float)((double)(float)k * g.dx) : (float)(((double)(float)k + 0.5) * g.dx));
+        for (int s = 0; s < views.n; s++)
+            if (views.v[s].constrain && inflow_holds_face(views.v[s], g, p)) return;
+    }
+    mac[f] = v + add;
+}
+
+hipError_t launch_save_and_body_force_inflow(const GridGeom &g, int comp, const float *mac,
+                                             float *saved, float add, int do_add,
+                                             const uint8_t *valid, const InflowViews &views,
+                                             hipStream_t s) {
+    size_t n = (size_t)(g.I + (comp == 0)) * (g.J + (comp == 1)) * (g.K + (comp == 2));
+    k_save_force_inflow<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, comp, n, const_cast<float *>(mac),
+                                                                    saved, add, do_add, valid, views);
     return hipGetLastError();
 }
 

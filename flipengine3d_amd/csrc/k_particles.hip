@@ -12,6 +12,7 @@
 // reference bit for bit given the same grids.
 #include "device_math.h"
 #include "flipmi_kernels.h"
+#include "inflow_device.h"
 
 namespace flipmi {
 
@@ -205,6 +206,11 @@ __device__ __forceinline__ void particle_step_one(const GridGeom &g, const Parti
     F3 vpic = mac_velocity(mac, g, p);
     F3 vflip = (vel + vpic) - mac_velocity(saved, g, p);
     F3 vnew = a.pic_ratio_f * vpic + a.flip_ratio_f * vflip;
+    // Inflow sources hold their particles' velocity (fluidsimulation.cpp:4172-4185).
+    for (int s = 0; s < a.inflow.n; s++) {
+        const InflowView &src = a.inflow.v[s];
+        if (src.constrain && inflow_holds_particle(src, g, p)) vnew = f3(src.vel[0], src.vel[1], src.vel[2]);
+    }
     // Advection + collision (fluidsimulation.cpp:4200-4219).
     Box bnd{f3(a.bpos[0], a.bpos[1], a.bpos[2]), a.bext[0], a.bext[1], a.bext[2]};
     F3 p1 = rk3(mac, g, p, a.dt);

@@ -6,53 +6,6 @@
 
 namespace flipmi {
 
-// Collision::findClosestPointOnTriangle (collision.cpp:161-270), in the
-// reference's float/double mix.
-__device__ F3 closest_point_on_triangle(F3 p0, F3 v0, F3 v1, F3 v2) {
-    F3 edge0 = v1 - v0, edge1 = v2 - v0, pv = v0 - p0;
-    double a = dot3(edge0, edge0), b = dot3(edge0, edge1), c = dot3(edge1, edge1);
-    double d = dot3(edge0, pv), e = dot3(edge1, pv);
-    double det = a * c - b * b;
-    if (fabs(det) < 1e-9) {
-        float feps = 1e-6f;
-        float len1 = len3(v0 - v1), len2 = len3(v1 - v2), len3v = len3(v2 - v0);
-        if (len1 < feps && len2 < feps && len3v < feps) return v0;
-        F3 dp, dq;
-        if (len1 < feps) { dp = v0; dq = v2; }
-        else if (len2 < feps) { dp = v1; dq = v0; }
-        else { dp = v2; dq = v1; }
-        float num = dot3(p0 - dp, dq - dp), den = dot3(dq - dp, dq - dp);
-        if (fabsf(den) < feps) return v0;
-        float lam = num / den;
-        if (lam <= 0.0f) return dp;
-        if (lam >= 1.0f) return dq;
-        return dp + lam * (dq - dp);
-    }
-    double s = b * e - c * d, t = b * d - a * e;
-    auto clamp01 = [](double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); };
-    if (s + t < det) {
-        if (s < 0.0) {
-            if (t < 0.0) {
-                if (d < 0.0) { s = clamp01(-d / a); t = 0.0; }
-                else { s = 0.0; t = clamp01(-e / c); }
-            } else { s = 0.0; t = clamp01(-e / c); }
-        } else if (t < 0.0) { s = clamp01(-d / a); t = 0.0; }
-        else { double inv = 1.0 / det; s *= inv; t *= inv; }
-    } else {
-        if (s < 0.0) {
-            double tmp0 = b + d, tmp1 = c + e;
-            if (tmp1 > tmp0) { s = clamp01((tmp1 - tmp0) / (a - 2 * b + c)); t = 1 - s; }
-            else { t = clamp01(-e / c); s = 0.0; }
-        } else if (t < 0.0) {
-            if (a + d > b + e) { s = clamp01((c + e - b - d) / (a - 2 * b + c)); t = 1 - s; }
-            else { s = clamp01(-e / c); t = 0.0; }
-        } else {
-            s = clamp01((c + e - b - d) / (a - 2 * b + c)); t = 1.0 - s;
-        }
-    }
-    return v0 + (float)s * edge0 + (float)t * edge1;
-}
-
 // Nodal solid SDF of the domain walls.  Each node takes the minimum distance
 // to the box triangles whose exact band covers it, evaluated in the frame of
 // the 10^3 node block that owns it (meshlevelset.cpp:967-1000), is negated

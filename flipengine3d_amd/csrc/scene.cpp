@@ -499,6 +499,9 @@ size_t Scene::device_bytes() const {
     b += near_a.bytes() + near_b.bytes() + liquid.bytes() + u.bytes() + v.bytes() + w.bytes();
     b += su.bytes() + sv.bytes() + sw.bytes() + vu.bytes() + vv.bytes() + vw.bytes() + status.bytes();
     if (press) b += press->device_bytes();
+    for (auto &src : sources) b += src->sdf.bytes() + src->cells.bytes() + src->list.bytes();
+    b += inflow_mask.bytes() + emit_cand.bytes() + emit_flags.bytes() + emit_pos.bytes() + emit_scan.bytes() +
+         emit_nbase.bytes();
     return b;
 }
 
@@ -552,8 +555,8 @@ void validate_scene_desc(const flipmi_scene_desc &d) {
 
 void Scene::ensure_solid() {
     // FluidSimulation::_updateSolidLevelSet (fluidsimulation.cpp:3028-3072):
-    // computed once, when marker particles exist.
-    if (solid_ready || n == 0) return;
+    // computed once, when marker particles or mesh fluid sources exist.
+    if (solid_ready || (n == 0 && sources.empty())) return;
     BoxMesh mesh = make_domain_box(desc.isize, desc.jsize, desc.ksize, desc.dx);
     check(launch_domain_solid(g, mesh, solid.p, stream), "domain solid");
     weights_ready = false;
@@ -622,6 +625,7 @@ void Scene::advance_particles(double dt) {
     // Speed limit uses the frame step (fluidsimulation.cpp:4385).
     a.speed_step = desc.cfl * g.dx / frame_dt;
     a.maxsteps = desc.max_substeps;
+    a.inflow = inflow_views(false);
     check(launch_fill_i32(speed_hist.p, 64, 0, stream), "fill");
     check(launch_particle_step(g, a, stream), "particle step");
     end_stage(FLIPMI_STAGE_G2P_ADVECT, ev);
@@ -672,7 +676,16 @@ double Scene::next_time_step(double dt) {
     if (current_frame == 0 && step_number == 0) {
         float bf[3] = {(float)desc.body_force[0], (float)desc.body_force[1], (float)desc.body_force[2]};
         float len = std::sqrt(bf[0] * bf[0] + bf[1] * bf[1] + bf[2] * bf[2]);
-        maxu = 0.0 + (double)len * dt;
+        // _predictMaximumMarkerParticleSpeed (fluidsimulation.cpp:5529-5551):
+        // the fastest enabled inflow source plus one step of body force.
+        maxu = 0.0;
+        for (auto &src : sources) {
+            if (!src->enabled) continue;
+            const float *u3 = src->vel;
+            float sp = std::sqrt(u3[0] * u3[0] + u3[1] * u3[1] + u3[2] * u3[2]);
+            maxu = std::fmax(std::fmax((double)sp, 0.0), maxu);
+        }
+        maxu += (double)len * dt;
     } else {
         check(launch_max_speed(part[3].p, part[4].p, part[5].p, n, max_speed_bits.p, stream), "max speed");
         unsigned *bits = (unsigned *)host_stage;
@@ -687,8 +700,405 @@ double Scene::next_time_step(double dt) {
     return desc.cfl * g.dx / (maxu + 1e-6);
 }
 
+// ---------------------------------------------------------------------------
+// Mesh inflow sources
+// ---------------------------------------------------------------------------
+namespace {
+
+// AABB(std::vector<vec3>) (aabb.cpp:56-80): float corner, double extents
+// widened by 1e-9.
+struct HostBox {
+    float pos[3];
+    double ext[3];
+};
+HostBox points_box(const float *v, size_t nv) {
+    double mn[3] = {v[0], v[1], v[2]}, mx[3] = {v[0], v[1], v[2]};
+    for (size_t q = 0; q < nv; q++)
+        for (int a = 0; a < 3; a++) {
+            mn[a] = std::fmin((double)v[3 * q + a], mn[a]);
+            mx[a] = std::fmax((double)v[3 * q + a], mx[a]);
+        }
+    HostBox b;
+    for (int a = 0; a < 3; a++) {
+        b.pos[a] = (float)mn[a];
+        b.ext[a] = mx[a] - mn[a] + 1e-9;
+    }
+    return b;
+}
+// Grid3d::positionToGridIndex and GridIndexToPosition (grid3d.h:49-83).
+int floor_index(float x, double dx) { return (int)std::floor((double)x * (1.0 / dx)); }
+float index_position(int i, double dx) { return (float)((double)(float)i * dx); }
+// AABB::isPointInside (aabb.cpp:130-133) for the box [0, ext).
+bool inside_origin_box(const float *p, const double ext[3]) {
+    for (int a = 0; a < 3; a++)
+        if (!(p[a] >= 0.0f && (double)p[a] < 0.0 + ext[a])) return false;
+    return true;
+}
+
+uint64_t splitmix64_host(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+double draw_unit_host(uint64_t seed, uint64_t stream, uint64_t item) {
+    uint64_t h = splitmix64_host(splitmix64_host(seed ^ (stream * 0xD1B54A32D192ED03ull)) + item);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+// The ray jitter of one inside test (MeshUtils::_getCollisionGridZ,
+// meshutils.cpp:107-110: _randomDouble(jit, -jit) per axis, jit = 0.001 dx),
+// drawn from the keyed generator: `which` 0 keys the source SDF's signs, 1
+// the source cells.
+void ray_jitter(uint64_t seed, int source, int which, double dx, float out[3]) {
+    const double jit = 0.001 * dx;
+    const uint64_t stream = ((uint64_t)(source + 1) << 8) | (uint64_t)which;
+    for (int a = 0; a < 3; a++) out[a] = (float)(jit + draw_unit_host(seed, stream, a) * (-jit - jit));
+}
+
+// MeshLevelSet::_initializeTriangleData for one triangle (meshlevelset.cpp:
+// 878-915): the node range of its AABB widened by the band, clamped to the
+// (isize+1)(jsize+1)(ksize+1) node grid; false when it misses the grid.
+bool triangle_band(const float v[3][3], int isize, int jsize, int ksize, double dx, int band,
+                   BoxTri &T) {
+    HostBox tb = points_box(&v[0][0], 3);
+    const int size[3] = {isize, jsize, ksize};
+    for (int a = 0; a < 3; a++) {
+        float mx = tb.pos[a] + (float)tb.ext[a];  // AABB::getMaxPoint (aabb.cpp:493-495)
+        T.gmin[a] = floor_index(tb.pos[a], dx) - band;
+        T.gmax[a] = floor_index(mx, dx) + band + 1;
+    }
+    for (int a = 0; a < 3; a++)
+        if (T.gmax[a] < 0 || T.gmin[a] >= size[a] + 1) return false;
+    for (int a = 0; a < 3; a++) {
+        T.gmin[a] = std::max(T.gmin[a], 0);
+        T.gmax[a] = std::min(T.gmax[a], size[a]);
+    }
+    std::memcpy(T.v, v, sizeof(T.v));
+    return true;
+}
+
+// The ray-column inputs of MeshUtils::getGridNodesInsideTriangleMesh for a
+// mesh inside the node grid's box (meshutils.cpp:347-372): the mesh shifted
+// by half a cell and moved to the grid offset `go` of its bounding box, the
+// local cell grid (ci, cj, ck) and each triangle's ray-column range.
+struct RayInput {
+    std::vector<RayTri> tris;
+    int ci = 0, cj = 0, ck = 0;
+    int go[3] = {0, 0, 0};
+};
+RayInput ray_input(const std::vector<float> &verts, const std::vector<int> &tris, double dx) {
+    RayInput r;
+    const size_t nv = verts.size() / 3;
+    std::vector<float> m(verts);
+    const float hw = (float)(0.5 * dx);
+    for (float &x : m) x = x + hw;  // TriangleMesh::translate (trianglemesh.cpp:732-736)
+    HostBox bb = points_box(m.data(), nv);
+    float off[3];
+    for (int a = 0; a < 3; a++) {
+        r.go[a] = floor_index(bb.pos[a], dx);
+        off[a] = index_position(r.go[a], dx);
+    }
+    for (size_t q = 0; q < nv; q++)
+        for (int a = 0; a < 3; a++) m[3 * q + a] = m[3 * q + a] + -off[a];
+    int gmax[3];
+    for (int a = 0; a < 3; a++) {
+        float lo = bb.pos[a] - off[a];
+        gmax[a] = floor_index(lo + (float)bb.ext[a], dx);  // Grid3d::getGridIndexBounds
+    }
+    r.ci = gmax[0] + 1; r.cj = gmax[1] + 1; r.ck = gmax[2] + 1;
+    const size_t nt = tris.size() / 3;
+    r.tris.resize(nt);
+    for (size_t t = 0; t < nt; t++) {
+        RayTri &T = r.tris[t];
+        double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+        for (int c = 0; c < 3; c++) {
+            const float *p = &m[3 * (size_t)tris[3 * t + c]];
+            for (int a = 0; a < 3; a++) T.v[c][a] = p[a];
+            for (int a = 0; a < 2; a++) {
+                lo[a] = std::min(lo[a], (double)p[a]);
+                hi[a] = std::max(hi[a], (double)p[a]);
+            }
+        }
+        // A column's ray runs inside its cell, so only the columns the
+        // triangle's box covers (one column of slack for rounding) can hit it.
+        for (int a = 0; a < 2; a++) {
+            T.c0[a] = (int)std::floor(lo[a] / dx) - 1;
+            T.c1[a] = (int)std::floor(hi[a] / dx) + 1;
+        }
+    }
+    return r;
+}
+
+}  // namespace
+
+int Scene::add_inflow(const float *verts, int nv, const int *tris, int nt, const double vel[3]) {
+    if (!verts || !tris || nv < 3 || nt < 1)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "inflow mesh needs at least 3 vertices and 1 triangle"};
+    if (sources.size() >= (size_t)kMaxInflow)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "too many inflow sources"};
+    for (int q = 0; q < 3 * nt; q++)
+        if (tris[q] < 0 || tris[q] >= nv)
+            throw StatusError{FLIPMI_ERR_INVALID_ARG, "inflow mesh triangle index out of range"};
+    const double dx = g.dx;
+    const double domain[3] = {g.I * dx, g.J * dx, g.K * dx};
+    for (int q = 0; q < nv; q++) {
+        if (!std::isfinite(verts[3 * q]) || !std::isfinite(verts[3 * q + 1]) || !std::isfinite(verts[3 * q + 2]))
+            throw StatusError{FLIPMI_ERR_INVALID_ARG, "inflow mesh vertex is not finite"};
+        // MeshUtils::getGridNodesInsideTriangleMesh splits meshes that leave
+        // the node grid's box (meshutils.cpp:392-428); only contained meshes
+        // are supported here.
+        if (!inside_origin_box(verts + 3 * q, domain))
+            throw StatusError{FLIPMI_ERR_INVALID_ARG, "inflow mesh must lie inside the domain grid"};
+    }
+    auto src = std::make_unique<InflowSource>();
+    src->verts.assign(verts, verts + 3 * (size_t)nv);
+    src->tris.assign(tris, tris + 3 * (size_t)nt);
+    for (int a = 0; a < 3; a++) src->vel[a] = (float)vel[a];
+    // Sub-grid of the source SDF: MeshFluidSource::_getGridBoundsFromTriangleMesh
+    // (meshfluidsource.cpp:318-333) with grid pad 4 (meshfluidsource.h:133).
+    HostBox bb = points_box(verts, (size_t)nv);
+    const double pad = 4.0 * dx, h = 0.5 * pad;  // AABB::expand (aabb.cpp:122-128)
+    const int top[3] = {g.I - 1, g.J - 1, g.K - 1};
+    int gmin[3], gmax[3];
+    for (int a = 0; a < 3; a++) {
+        float lo = bb.pos[a] - (float)h;
+        double ext = bb.ext[a] + pad;
+        gmin[a] = (int)std::fmax(floor_index(lo, dx), 0);
+        gmax[a] = (int)std::fmin(floor_index(lo + (float)ext, dx), top[a]);
+        src->go[a] = gmin[a];
+        src->off[a] = index_position(gmin[a], dx);
+    }
+    src->ni = std::max(gmax[0] - gmin[0] + 1, 1) + 1;
+    src->nj = std::max(gmax[1] - gmin[1] + 1, 1) + 1;
+    src->nk = std::max(gmax[2] - gmin[2] + 1, 1) + 1;
+    const double sub[3] = {(src->ni - 1) * dx, (src->nj - 1) * dx, (src->nk - 1) * dx};
+    for (int q = 0; q < nv; q++) {
+        float p[3];
+        for (int a = 0; a < 3; a++) p[a] = verts[3 * q + a] + -src->off[a];
+        if (!inside_origin_box(p, sub))
+            throw StatusError{FLIPMI_ERR_INVALID_ARG,
+                              "inflow mesh must lie at least 4 cells inside the domain grid"};
+    }
+    sources.push_back(std::move(src));
+    return (int)sources.size() - 1;
+}
+
+void Scene::build_inflow(InflowSource &src, int id) {
+    const double dx = g.dx;
+    const int I = g.I, J = g.J, K = g.K;
+    const size_t nt = src.tris.size() / 3;
+    auto need = [&](hipError_t e) {
+        if (e != hipSuccess) throw StatusError{FLIPMI_ERR_OOM, "inflow source allocation: out of device memory"};
+    };
+    DBuf<int> overflow;
+    need(overflow.alloc(1));
+    check(launch_fill_i32(overflow.p, 1, 0, stream), "fill");
+    // Source SDF: MeshLevelSet::fastCalculateSignedDistanceField with exact
+    // band 2 (meshfluidsource.h:116) on the mesh moved to the sub-grid.
+    std::vector<float> local(src.verts.size());
+    for (size_t q = 0; q < local.size(); q++) local[q] = src.verts[q] + -src.off[q % 3];
+    std::vector<BoxTri> band;
+    for (size_t t = 0; t < nt; t++) {
+        float v[3][3];
+        for (int c = 0; c < 3; c++)
+            for (int a = 0; a < 3; a++) v[c][a] = local[3 * (size_t)src.tris[3 * t + c] + a];
+        BoxTri T;
+        if (triangle_band(v, src.ni - 1, src.nj - 1, src.nk - 1, dx, 2, T)) band.push_back(T);
+    }
+    const size_t nn = (size_t)src.ni * src.nj * src.nk;
+    need(src.sdf.alloc(nn));
+    const float upper = (float)((double)(src.ni + src.nj + src.nk) * dx);  // meshlevelset.cpp:769-771
+    DBuf<BoxTri> dband;
+    if (band.empty()) {
+        check(launch_fill_f32(src.sdf.p, nn, upper, stream), "fill");
+    } else {
+        need(dband.alloc(band.size()));
+        check(hipMemcpyAsync(dband.p, band.data(), dband.bytes(), hipMemcpyHostToDevice, stream), "upload");
+        check(launch_mesh_exact_band(dband.p, (int)band.size(), src.ni, src.nj, src.nk, dx, upper, src.sdf.p,
+                                     stream),
+              "exact band");
+    }
+    float jit[3];
+    RayInput rs = ray_input(local, src.tris, dx);
+    DBuf<RayTri> dray;
+    need(dray.alloc(nt));
+    check(hipMemcpyAsync(dray.p, rs.tris.data(), dray.bytes(), hipMemcpyHostToDevice, stream), "upload");
+    ray_jitter(jitter_seed, id, 0, dx, jit);
+    check(launch_ray_inside(dray.p, (int)nt, rs.ci, rs.cj, rs.ck, dx, jit, rs.go, src.ni, src.nj, src.nk, 1,
+                            nullptr, src.sdf.p, overflow.p, stream),
+          "sdf signs");
+    // Source cells: MeshObject::getCells on the domain's node grid.
+    RayInput rc = ray_input(src.verts, src.tris, dx);
+    DBuf<RayTri> dray2;
+    need(dray2.alloc(nt));
+    check(hipMemcpyAsync(dray2.p, rc.tris.data(), dray2.bytes(), hipMemcpyHostToDevice, stream), "upload");
+    const size_t nnodes = (size_t)(I + 1) * (J + 1) * (K + 1), nc = (size_t)I * J * K;
+    DBuf<uint8_t> nodes;
+    need(nodes.alloc(nnodes));
+    check(launch_fill_u8(nodes.p, nnodes, 0, stream), "fill");
+    ray_jitter(jitter_seed, id, 1, dx, jit);
+    check(launch_ray_inside(dray2.p, (int)nt, rc.ci, rc.cj, rc.ck, dx, jit, rc.go, I + 1, J + 1, K + 1, 0,
+                            nodes.p, nullptr, overflow.p, stream),
+          "source nodes");
+    need(src.cells.alloc(nc));
+    DBuf<int> flags, pos, cnt;
+    need(flags.alloc(nc)); need(pos.alloc(nc)); need(cnt.alloc(1));
+    check(launch_source_cell_scan(nodes.p, I, J, K, src.cells.p, flags.p, pos.p, scan_tmp.p, cnt.p, stream),
+          "source cells");
+    int host[2];
+    check(hipMemcpyAsync(&host[0], cnt.p, sizeof(int), hipMemcpyDeviceToHost, stream), "count");
+    check(hipMemcpyAsync(&host[1], overflow.p, sizeof(int), hipMemcpyDeviceToHost, stream), "overflow");
+    sync();
+    if (host[1] != 0)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "inflow mesh: a ray column crosses more than 64 triangles"};
+    src.ncells = host[0];
+    if (src.ncells > 0) {
+        need(src.list.alloc((size_t)src.ncells));
+        check(launch_source_cell_list(flags.p, pos.p, nc, src.list.p, stream), "cell list");
+        std::vector<int> cl((size_t)src.ncells);
+        check(hipMemcpyAsync(cl.data(), src.list.p, sizeof(int) * cl.size(), hipMemcpyDeviceToHost, stream),
+              "cell list");
+        sync();
+        for (int a = 0; a < 3; a++) { src.cmin[a] = INT32_MAX; src.cmax[a] = -1; }
+        for (int c : cl) {
+            int ijk[3] = {c % I, (c / I) % J, c / (I * J)};
+            for (int a = 0; a < 3; a++) {
+                src.cmin[a] = std::min(src.cmin[a], ijk[a]);
+                src.cmax[a] = std::max(src.cmax[a], ijk[a]);
+            }
+        }
+    }
+    src.built = true;
+}
+
+void Scene::ensure_inflow() {
+    bool fresh = false;
+    for (size_t q = 0; q < sources.size(); q++) {
+        if (sources[q]->built) continue;
+        build_inflow(*sources[q], (int)q);
+        fresh = true;
+    }
+    if (!fresh) return;
+    auto need = [&](hipError_t e) {
+        if (e != hipSuccess) throw StatusError{FLIPMI_ERR_OOM, "inflow allocation: out of device memory"};
+    };
+    const size_t nc = (size_t)g.I * g.J * g.K;
+    if (!inflow_mask.p) {
+        need(inflow_mask.alloc((nc + 3) / 4));
+        check(launch_fill_i32((int *)inflow_mask.p, inflow_mask.n, 0, stream), "fill");
+    }
+    int most = 0;
+    for (auto &s : sources) most = std::max(most, s->ncells);
+    const size_t m = (size_t)std::max(8 * most, 1);
+    if (emit_flags.n < m) {
+        need(emit_cand.alloc(3 * m));
+        need(emit_flags.alloc(m));
+        need(emit_pos.alloc(m));
+        need(emit_scan.alloc(scan_scratch_elems((int)m)));
+    }
+    if (!emit_nbase.p) need(emit_nbase.alloc(1));
+}
+
+InflowViews Scene::inflow_views(bool emitting) const {
+    InflowViews vs{};
+    for (size_t q = 0; q < sources.size() && q < (size_t)kMaxInflow; q++) {
+        const InflowSource &s = *sources[q];
+        InflowView &v = vs.v[vs.n++];
+        v.sdf = s.sdf.p;
+        v.ni = s.ni; v.nj = s.nj; v.nk = s.nk;
+        v.cells = s.cells.p;
+        v.list = s.list.p;
+        v.ncells = s.ncells;
+        for (int a = 0; a < 3; a++) {
+            v.off[a] = s.off[a];
+            v.cmin[a] = s.cmin[a];
+            v.cmax[a] = s.cmax[a];
+            v.vel[a] = s.vel[a];
+        }
+        // _updateInflowMeshFluidSource (fluidsimulation.cpp:4564-4606): a
+        // source with 0 substep emissions emits at the first substep only.
+        v.emit = emitting && s.built && s.enabled && s.ncells > 0 && (s.emissions > 0 || step_number == 0);
+        v.constrain = s.built && s.enabled && s.constrain && s.ncells > 0 ? 1 : 0;
+    }
+    return vs;
+}
+
+bool Scene::any_inflow_constraint() const {
+    for (auto &s : sources)
+        if (s->enabled && s->constrain) return true;
+    return false;
+}
+
+void Scene::grow(int64_t need_n) {
+    if (need_n <= cap) return;
+    const int64_t limit = INT32_MAX / 2;
+    if (need_n > limit) throw StatusError{FLIPMI_ERR_OOM, "particle count exceeds the supported maximum"};
+    const int64_t target = std::min<int64_t>(std::max<int64_t>(need_n, (int64_t)cap + cap / 2), limit);
+    const size_t pc = (size_t)target;
+    auto need = [&](hipError_t e) {
+        if (e != hipSuccess) throw StatusError{FLIPMI_ERR_OOM, "particle growth: out of device memory"};
+    };
+    for (int a = 0; a < 6; a++) {
+        DBuf<float> fresh;
+        need(fresh.alloc(pc));
+        if (n > 0)
+            check(hipMemcpyAsync(fresh.p, part[a].p, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, stream),
+                  "grow");
+        sync();
+        std::swap(part[a].p, fresh.p);
+        std::swap(part[a].n, fresh.n);
+        part_alt[a].release(); sorted[a].release();
+        need(part_alt[a].alloc(pc)); need(sorted[a].alloc(pc));
+    }
+    cell.release(); slot.release(); order.release(); keep.release(); keep_pos.release(); rank.release();
+    solid_removed.release(); scan_tmp.release();
+    need(cell.alloc(pc)); need(slot.alloc(pc)); need(order.alloc(pc));
+    need(keep.alloc(pc)); need(keep_pos.alloc(pc)); need(rank.alloc(pc));
+    need(solid_removed.alloc(pc));
+    const size_t nc = (size_t)g.I * g.J * g.K;
+    need(scan_tmp.alloc(std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc))));
+    cap = (int)target;
+}
+
+// FluidSimulation::_updateInflowMeshFluidSources (fluidsimulation.cpp:
+// 4675-4697) at the end of a substep.
+void Scene::emit_inflow() {
+    if (sources.empty()) return;
+    InflowViews vs = inflow_views(true);
+    int64_t extra = 0;
+    for (int q = 0; q < vs.n; q++)
+        if (vs.v[q].emit) extra += 8 * (int64_t)vs.v[q].ncells * std::max(sources[q]->emissions, 1);
+    if (extra == 0) return;
+    hipEvent_t ev = nullptr;
+    begin_stage(FLIPMI_STAGE_INFLOW, &ev);
+    grow((int64_t)n + extra);
+    check(launch_inflow_mask(g, part[0].p, part[1].p, part[2].p, n, vs, inflow_mask.p, stream), "inflow mask");
+    check(launch_fill_i32(emit_nbase.p, 1, n, stream), "fill");
+    EmitOut out{part[0].p, part[1].p, part[2].p, part[3].p, part[4].p, part[5].p};
+    EmitParams ep{};
+    ep.jitter = 0.25 * (jitter_factor - 1e-3) * g.dx;  // fluidsimulation.cpp:2623-2626
+    ep.jitter_surface = jitter_surface ? 1 : 0;
+    ep.seed = jitter_seed;
+    for (int q = 0; q < vs.n; q++) {
+        if (!vs.v[q].emit) continue;
+        const int passes = std::max(sources[q]->emissions, 1);
+        for (int e = 0; e < passes; e++) {
+            ep.stream = 0x1000000ull + emission_count++;
+            check(launch_inflow_emit(g, vs.v[q], solid.p, inflow_mask.p, ep, emit_cand.p, emit_flags.p,
+                                     emit_pos.p, emit_scan.p, emit_nbase.p, out, stream),
+                  "inflow emit");
+        }
+    }
+    int *nb = (int *)host_stage + 128;
+    check(hipMemcpyAsync(nb, emit_nbase.p, sizeof(int), hipMemcpyDeviceToHost, stream), "count");
+    sync();
+    n = *nb;
+    end_stage(FLIPMI_STAGE_INFLOW, ev);
+}
+
 void Scene::step_fluid(double dt) {
     ensure_solid();
+    if (!sources.empty()) ensure_inflow();
     hipEvent_t ev = nullptr;
     // Liquid SDF + post-process (fluidsimulation.cpp:3225-3247).
     begin_stage(FLIPMI_STAGE_SORT, &ev);
@@ -735,10 +1145,17 @@ void Scene::step_fluid(double dt) {
     float *macs[3] = {u.p, v.p, w.p};
     float *saved[3] = {su.p, sv.p, sw.p};
     size_t sizes[3] = {nU, nV, nW};
+    const bool held = any_inflow_constraint();
+    const InflowViews views = held ? inflow_views(false) : InflowViews{};
+    const uint8_t *valid[3] = {vu.p, vv.p, vw.p};
     for (int c = 0; c < 3; c++) {
         int do_add = std::fabs(bf[c]) > 1e-6f ? 1 : 0;
         float add = (float)((double)bf[c] * dt);
-        check(launch_save_and_body_force(sizes[c], macs[c], saved[c], add, do_add, stream), "save/force");
+        if (held)
+            check(launch_save_and_body_force_inflow(g, c, macs[c], saved[c], add, do_add, valid[c], views, stream),
+                  "save/force");
+        else
+            check(launch_save_and_body_force(sizes[c], macs[c], saved[c], add, do_add, stream), "save/force");
     }
     end_stage(FLIPMI_STAGE_GLUE, ev);
     // _pressureSolve (fluidsimulation.cpp:3732-3769).
@@ -755,6 +1172,8 @@ void Scene::step_fluid(double dt) {
     end_stage(FLIPMI_STAGE_GLUE, ev);
     // _updateMarkerParticleVelocities + _advanceMarkerParticles.
     if (n > 0) advance_particles(dt);
+    // _updateFluidObjects (fluidsimulation.cpp:4794-4806).
+    emit_inflow();
 }
 
 flipmi_frame_stats Scene::update(double dt) {

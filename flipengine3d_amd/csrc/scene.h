@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,28 @@ BoxMesh make_domain_box(int isize, int jsize, int ksize, double dx);
 // (fluidsimulation.cpp:4213-4215).
 void collision_box(int isize, int jsize, int ksize, double dx, float pos[3], double ext[3]);
 
+// One mesh inflow source: MeshFluidSource with a static mesh, inflow
+// (meshfluidsource.cpp:32-44, 74-76, 114-116), in grid coordinates.
+struct InflowSource {
+    std::vector<float> verts;  // xyz
+    std::vector<int> tris;     // three vertex indices per triangle
+    float vel[3] = {0, 0, 0};
+    bool enabled = true;       // enable() / disable()
+    bool constrain = true;     // _isConstrainedFluidVelocity (meshfluidsource.h:124)
+    int emissions = 1;         // _substepEmissions (meshfluidsource.h:119)
+    // Placement of the source SDF (MeshFluidSource::update, :178-222).
+    int go[3] = {0, 0, 0};     // sub-grid offset in cells
+    float off[3] = {0, 0, 0};  // its position
+    int ni = 0, nj = 0, nk = 0;  // sub-grid node dims
+    // Device state, built at the first substep that needs it.
+    bool built = false;
+    DBuf<float> sdf;
+    DBuf<uint8_t> cells;
+    DBuf<int> list;
+    int ncells = 0;
+    int cmin[3] = {0, 0, 0}, cmax[3] = {-1, -1, -1};
+};
+
 struct Scene {
     Ctx *ctx = nullptr;
     flipmi_scene_desc desc{};
@@ -97,6 +120,18 @@ struct Scene {
     // particle positions cross the boundary in domain coordinates.
     float domain_offset[3] = {0.0f, 0.0f, 0.0f};
     double domain_scale = 1.0;
+
+    // mesh inflow sources and the marker-particle jitter
+    // (setMarkerParticleJitterFactor, enable/disableJitterSurfaceMarkerParticles,
+    // fluidsimulation.cpp:185-210)
+    std::vector<std::unique_ptr<InflowSource>> sources;
+    double jitter_factor = 1.0;
+    bool jitter_surface = false;
+    uint64_t jitter_seed = 0;
+    uint64_t emission_count = 0;  // emission passes so far (keys the jitter)
+    DBuf<uint32_t> inflow_mask;   // ParticleMaskGrid bytes, four cells per word
+    DBuf<float> emit_cand;
+    DBuf<int> emit_flags, emit_pos, emit_scan, emit_nbase;
 
     // last pressure solve
     int last_iters = 0, last_ok = 1;
@@ -140,6 +175,17 @@ struct Scene {
     void advance_particles(double dt);
     void sort_particles();
     void extrapolate_all(int layers);
+    // Mesh inflow sources: add (checks the mesh), build the device state of
+    // every source, their device views, and the emission at the end of a
+    // substep (FluidSimulation::_updateFluidObjects).
+    int add_inflow(const float *verts, int nv, const int *tris, int nt, const double vel[3]);
+    void ensure_inflow();
+    void build_inflow(InflowSource &src, int id);
+    InflowViews inflow_views(bool emitting) const;
+    bool any_inflow_constraint() const;
+    void emit_inflow();
+    // Grows every particle-sized buffer to hold at least `need` particles.
+    void grow(int64_t need);
     void begin_stage(int stage, hipEvent_t *a);
     void end_stage(int stage, hipEvent_t a);
     // Books a pair of events already recorded (by a timed dispatch).

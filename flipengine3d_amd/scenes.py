@@ -165,3 +165,77 @@ def pool_drop(isize: int, jsize: int, ksize: int, dx: float, seed: int,
         return (y < depth * H) | ((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 < r * r)
 
     return seed_cells(isize, jsize, ksize, dx, inside, seed, jitter_factor, name="pool_drop")
+
+
+# ---------------------------------------------------------------------------
+# Closed triangle meshes for mesh inflow sources (MeshFluidSource with a
+# static mesh, src/engine/meshfluidsource.cpp).  Vertices are float32 xyz
+# triples and triangles int32 index triples, as the reference's
+# TriangleMesh holds them (trianglemesh.h:80-84).
+# ---------------------------------------------------------------------------
+
+def cylinder_mesh(x0: float, x1: float, centre_yz: Sequence[float], radius: float,
+                  segments: int = 32, phase: float = 0.0):
+    """Closed cylinder along +x from x0 to x1 around (y, z) = centre_yz: two
+    rings of ``segments`` vertices, quad sides split in two triangles and a
+    triangle fan per cap."""
+    n = int(segments)
+    th = phase + 2.0 * np.pi * np.arange(n) / n
+    yc, zc = float(centre_yz[0]), float(centre_yz[1])
+    ring = np.stack([yc + radius * np.cos(th), zc + radius * np.sin(th)], axis=1)
+    v = np.zeros((2 * n + 2, 3), np.float64)
+    v[:n, 0], v[:n, 1:] = x0, ring
+    v[n:2 * n, 0], v[n:2 * n, 1:] = x1, ring
+    v[2 * n] = (x0, yc, zc)
+    v[2 * n + 1] = (x1, yc, zc)
+    t = []
+    for i in range(n):
+        a, b = i, (i + 1) % n
+        t += [(a, b, n + b), (a, n + b, n + a), (2 * n, b, a), (2 * n + 1, n + a, n + b)]
+    return v.astype(np.float32), np.asarray(t, np.int32)
+
+
+def box_mesh(lo: Sequence[float], hi: Sequence[float], rotation=None, centre=None):
+    """Closed box [lo, hi] as 12 triangles, optionally rotated by a 3x3 matrix
+    about ``centre`` (default: the box centre)."""
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    c = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], np.float64)
+    v = lo + c * (hi - lo)
+    if rotation is not None:
+        ctr = (lo + hi) / 2 if centre is None else np.asarray(centre, np.float64)
+        v = (v - ctr) @ np.asarray(rotation, np.float64).T + ctr
+    # corner index = x + 2y + 4z
+    t = [(0, 2, 3), (0, 3, 1), (4, 5, 7), (4, 7, 6), (0, 1, 5), (0, 5, 4),
+         (2, 6, 7), (2, 7, 3), (0, 4, 6), (0, 6, 2), (1, 3, 7), (1, 7, 5)]
+    return v.astype(np.float32), np.asarray(t, np.int32)
+
+
+def rotation_matrix(seed: int) -> np.ndarray:
+    """A rotation drawn from a seeded generator (for test meshes)."""
+    rng = np.random.default_rng(seed)
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+@dataclasses.dataclass
+class Inflow:
+    """A mesh inflow source: closed mesh and fluid velocity."""
+    vertices: np.ndarray
+    triangles: np.ndarray
+    velocity: tuple
+
+
+def pool_jet(isize: int, jsize: int, ksize: int, dx: float, seed: int, depth: float = 0.25,
+             radius: float = 0.05, speed: float = 2.0, height: float = 0.6,
+             x_span=(0.05, 0.15), segments: int = 32, phase: float = 0.1):
+    """Pool filling the bottom ``depth`` of the domain plus an inflow jet: a
+    cylinder of ``radius`` (metres) along +x over ``x_span`` (fractions of
+    the domain width) at ``height`` (fraction of the domain height) and mid
+    depth, emitting fluid at ``speed`` m/s along +x."""
+    pool = pool_drop(isize, jsize, ksize, dx, seed, depth=depth, radius=0.0)
+    W, H, D = isize * dx, jsize * dx, ksize * dx
+    v, t = cylinder_mesh(x_span[0] * W, x_span[1] * W, (height * H, 0.5 * D), radius, segments, phase)
+    return pool, Inflow(v, t, (speed, 0.0, 0.0))

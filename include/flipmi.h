@@ -106,7 +106,8 @@ enum {
     FLIPMI_STAGE_K_FORWARD = 14,
     FLIPMI_STAGE_K_BACKWARD = 15,
     FLIPMI_STAGE_K_UPDATE = 16,
-    FLIPMI_STAGE_K_RESIDUAL = 17    /* mixed precision: FP64 true residual */
+    FLIPMI_STAGE_K_RESIDUAL = 17,   /* mixed precision: FP64 true residual */
+    FLIPMI_STAGE_INFLOW = 18        /* mesh inflow: particle mask and emission */
 };
 
 /* Grids readable with flipmi_scene_get_grid (sizes as in the header). */
@@ -169,6 +170,41 @@ int flipmi_scene_get_particles_range(flipmi_scene *scene, int64_t start, int64_t
  * for every read-back. */
 int flipmi_scene_set_domain(flipmi_scene *scene, double offset_x, double offset_y, double offset_z,
                             double scale);
+/* Mesh inflow source: MeshFluidSource(isize, jsize, ksize, dx) +
+ * updateMeshStatic(mesh) + setInflow() + setVelocity(v)
+ * (meshfluidsource.cpp:32-44, 74-76, 114-116), added with
+ * FluidSimulation::addMeshFluidSource (fluidsimulation.cpp:1953-1965).
+ * vertices_xyz: float triples in grid coordinates; triangles: int index
+ * triples.  Each substep ends with the source's emission
+ * (_updateInflowMeshFluidSources, fluidsimulation.cpp:4564-4697); its
+ * particles and faces keep the source velocity (:3372-3438, 4109-4185).
+ * The mesh must lie at least 4 cells inside the domain grid (the reference
+ * splits meshes that leave the grid, meshutils.cpp:392-428) and no ray
+ * column may cross it more than 64 times; otherwise FLIPMI_ERR_INVALID_ARG.
+ * At most 8 sources.  *source_id receives the source's index. */
+int flipmi_scene_add_inflow_source(flipmi_scene *scene, const float *vertices_xyz, int num_vertices,
+                                   const int *triangles, int num_triangles, const double velocity[3],
+                                   int *source_id);
+/* MeshFluidSource::getMeshLevelSet / getMeshLevelSetOffset / getCells
+ * (meshfluidsource.cpp:228-246) after initialize: dims receives the
+ * sub-grid offset (cells) and node dims (6 ints); sdf (node dims, i
+ * fastest) and cells (isize*jsize*ksize bytes, 1 = source cell) may be
+ * NULL. */
+int flipmi_scene_get_inflow_source(flipmi_scene *scene, int source_id, int dims[6], float *sdf,
+                                   uint8_t *cells);
+/* MeshFluidSource::enable()/disable(), enable/disableConstrainedFluidVelocity()
+ * and setSubstepEmissions(n) (meshfluidsource.cpp:51-72, 146-156): n > 0
+ * emits n times per substep, n = 0 only in the first substep of a frame;
+ * negative n counts as 0. */
+int flipmi_scene_set_inflow_source(flipmi_scene *scene, int source_id, int enabled,
+                                   int constrain_velocity, int substep_emissions);
+/* FluidSimulation::setMarkerParticleJitterFactor (fluidsimulation.cpp:185-196,
+ * factor in [0, 1]) and enable/disableJitterSurfaceMarkerParticles (:198-210)
+ * for the particles inflow sources add.  The reference draws the jitter from
+ * rand(); here it comes from a SplitMix64 stream keyed by (seed, emission
+ * pass, candidate), so a run is reproducible. */
+int flipmi_scene_set_marker_jitter(flipmi_scene *scene, double factor, int jitter_surface,
+                                   uint64_t seed);
 /* Copies an internal grid (FLIPMI_GRID_*) to host memory, e.g.
  * FluidSimulation::getVelocityField (fluidsimulation.cpp:2242). */
 int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out);

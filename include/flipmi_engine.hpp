@@ -613,6 +613,68 @@ struct FluidSimulationMarkerParticleData {
     char *velocities;
 };
 
+// trianglemesh.h:28-84: vertex list and index triples.
+struct Triangle {
+    int tri[3] = {0, 0, 0};
+};
+struct TriangleMesh {
+    std::vector<vmath::vec3> vertices;
+    std::vector<Triangle> triangles;
+};
+
+// meshfluidsource.h: a mesh inflow source with a static mesh
+// (updateMeshStatic).  Settings made after the source joins an initialized
+// simulation reach the device at once.
+class MeshFluidSource {
+public:
+    MeshFluidSource(int isize, int jsize, int ksize, double dx) { (void)isize; (void)jsize; (void)ksize; (void)dx; }
+    void updateMeshStatic(TriangleMesh mesh) {
+        if (_scene) throw std::runtime_error("flipmi: the source mesh is fixed once the source is on the device");
+        _mesh = mesh;
+    }
+    void setInflow() {}
+    bool isInflow() { return true; }
+    void setVelocity(vmath::vec3 v) {
+        if (_scene) throw std::runtime_error("flipmi: the source velocity is fixed once the source is on the device");
+        _velocity = v;
+    }
+    vmath::vec3 getVelocity() { return _velocity; }
+    void enable() { _enabled = true; _push(); }
+    void disable() { _enabled = false; _push(); }
+    bool isEnabled() { return _enabled; }
+    void enableConstrainedFluidVelocity() { _constrain = true; _push(); }
+    void disableConstrainedFluidVelocity() { _constrain = false; _push(); }
+    bool isConstrainedFluidVelocityEnabled() { return _constrain; }
+    void setSubstepEmissions(int n) { _emissions = n < 0 ? 0 : n; _push(); }  // meshfluidsource.cpp:63-68
+    int getSubstepEmissions() { return _emissions; }
+
+private:
+    friend class FluidSimulation;
+    void _attach(flipmi_scene *scene) {
+        std::vector<float> v;
+        std::vector<int> t;
+        for (const vmath::vec3 &p : _mesh.vertices) { v.push_back(p.x); v.push_back(p.y); v.push_back(p.z); }
+        for (const Triangle &q : _mesh.triangles) t.insert(t.end(), q.tri, q.tri + 3);
+        const double vel[3] = {_velocity.x, _velocity.y, _velocity.z};
+        detail::check(flipmi_scene_add_inflow_source(scene, v.data(), (int)_mesh.vertices.size(), t.data(),
+                                                     (int)_mesh.triangles.size(), vel, &_id),
+                      "FluidSimulation::addMeshFluidSource");
+        _scene = scene;
+        _push();
+    }
+    void _push() {
+        if (_scene)
+            detail::check(flipmi_scene_set_inflow_source(_scene, _id, _enabled, _constrain, _emissions),
+                          "MeshFluidSource");
+    }
+    TriangleMesh _mesh;
+    vmath::vec3 _velocity;
+    bool _enabled = true, _constrain = true;
+    int _emissions = 1;
+    flipmi_scene *_scene = nullptr;
+    int _id = -1;
+};
+
 class FluidSimulation {
 public:
     FluidSimulation(int isize, int jsize, int ksize, double dx) : _isize(isize), _jsize(jsize), _ksize(ksize), _dx(dx) {
@@ -643,9 +705,32 @@ public:
                           "FluidSimulation::loadMarkerParticleData");
         _loadPos.clear();
         _loadVel.clear();
+        detail::check(flipmi_scene_set_marker_jitter(_scene, _jitterFactor, _jitterSurface, _jitterSeed),
+                      "FluidSimulation::initialize");
+        for (MeshFluidSource *src : _sources) src->_attach(_scene);
         detail::check(flipmi_scene_initialize(_scene), "FluidSimulation::initialize");
         _initialized = true;
     }
+
+    // fluidsimulation.cpp:1953-1965; the source must outlive the simulation.
+    void addMeshFluidSource(MeshFluidSource *source) {
+        for (MeshFluidSource *s : _sources)
+            if (s == source) throw std::runtime_error("Error: Mesh fluid source has already been added.");
+        _sources.push_back(source);
+        if (_scene) source->_attach(_scene);
+    }
+    // fluidsimulation.cpp:185-210 for the particles inflow sources add; the
+    // seed keys this build's jitter draws (the reference calls rand()).
+    void setMarkerParticleJitterFactor(double jit) {
+        if (jit < 0.0) throw std::domain_error("Error: marker particle jitter must be greater than or equal to 0.");
+        _jitterFactor = jit;
+        _pushJitter();
+    }
+    double getMarkerParticleJitterFactor() { return _jitterFactor; }
+    void enableJitterSurfaceMarkerParticles() { _jitterSurface = 1; _pushJitter(); }
+    void disableJitterSurfaceMarkerParticles() { _jitterSurface = 0; _pushJitter(); }
+    bool isJitterSurfaceMarkerParticlesEnabled() { return _jitterSurface != 0; }
+    void setMarkerParticleJitterSeed(uint64_t seed) { _jitterSeed = seed; _pushJitter(); }
     bool isInitialized() { return _initialized; }
 
     // fluidsimulation.cpp:5755-5846
@@ -842,6 +927,11 @@ private:
     void _requireScene() {
         if (!_initialized) throw std::runtime_error("Error: FluidSimulation must be initialized first.");
     }
+    void _pushJitter() {
+        if (_scene)
+            detail::check(flipmi_scene_set_marker_jitter(_scene, _jitterFactor, _jitterSurface, _jitterSeed),
+                          "FluidSimulation::setMarkerParticleJitterFactor");
+    }
     void _range(int start, int end, float *pos, float *vel) {
         if (start < 0 || end > (int)getNumMarkerParticles() || start > end)
             throw std::domain_error("Error: invalid range.");
@@ -867,6 +957,10 @@ private:
     int _lastPcgIterations = 0;
     double _lastPcgResidual = 0.0;
     bool _lastPressureOk = true;
+    std::vector<MeshFluidSource *> _sources;
+    double _jitterFactor = 1.0;  // fluidsimulation.h:1553
+    int _jitterSurface = 0;
+    uint64_t _jitterSeed = 0;
 };
 
 }  // namespace flipmi

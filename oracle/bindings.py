@@ -72,6 +72,9 @@ class _Base:
         getattr(L, p + "advect_particles").argtypes = ([_c_int] * 3 + [_c_double] + [_vp] * 4
                                                       + [_c_int, _c_double, _c_int, _vp])
         getattr(L, p + "trilinear_velocity").argtypes = [_c_int] * 3 + [_c_double] + [_vp] * 4 + [_c_int, _vp]
+        getattr(L, p + "sim_add_inflow").argtypes = [_vp, _vp, _c_int, _vp, _c_int] + [_c_double] * 3
+        getattr(L, p + "sim_set_inflow").argtypes = [_vp] + [_c_int] * 4
+        getattr(L, p + "sim_inflow_sdf").argtypes = [_vp, _c_int, _vp, _vp]
 
     # -- whole-step driver -------------------------------------------------
     def sim_create(self, isize, jsize, ksize, dx):
@@ -221,6 +224,34 @@ class _SimHandle:
     def substeps(self):
         return self._f("sim_frame_substeps")(self.h)
 
+    # -- mesh inflow sources ---------------------------------------------
+    def add_inflow(self, verts, tris, velocity):
+        """MeshFluidSource with a static mesh, inflow, setVelocity; returns its index."""
+        verts = np.ascontiguousarray(verts, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        return self._f("sim_add_inflow")(self.h, _ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0],
+                                         float(velocity[0]), float(velocity[1]), float(velocity[2]))
+
+    def set_inflow(self, sid, enabled=True, constrain=True, emissions=1):
+        self._f("sim_set_inflow")(self.h, sid, int(enabled), int(constrain), int(emissions))
+
+    def set_jitter(self, factor, surface=False, seed=0):
+        """setMarkerParticleJitterFactor + jitter-surface switch (the oracle
+        also takes the seed of its jitter draws)."""
+        if self.L.prefix == "orc_":
+            self._f("sim_set_jitter")(self.h, float(factor), int(surface), int(seed))
+        else:
+            self._f("sim_set_jitter")(self.h, float(factor), int(surface))
+
+    def inflow_sdf(self, sid):
+        """(grid offset (i, j, k), nodal phi (nk, nj, ni)) of a source after its first update."""
+        dims = np.zeros(6, np.int32)
+        if self._f("sim_inflow_sdf")(self.h, sid, _ptr(dims), None) != 0:
+            return None
+        phi = np.zeros((dims[5], dims[4], dims[3]), np.float32)
+        self._f("sim_inflow_sdf")(self.h, sid, _ptr(dims), _ptr(phi))
+        return tuple(int(v) for v in dims[:3]), phi
+
     _FSHAPES = {
         0: lambda I, J, K: (K, J, I),
         1: lambda I, J, K: (K + 1, J + 1, I + 1),
@@ -257,6 +288,63 @@ class Oracle(_Base):
         L.orc_sim_pressure_info.argtypes = [_vp, _vp, _vp, _vp]
         L.orc_pressure_solve.argtypes = ([_c_int] * 3 + [_c_double, _c_double, _c_double, _c_int, _c_int]
                                          + [_vp] * 15 + [_vp] * 5)
+        L.orc_sim_set_jitter.argtypes = [_vp, _c_double, _c_int, ctypes.c_uint64]
+        L.orc_mesh_nodes_inside.argtypes = [_vp, _c_int, _vp, _c_int, _c_double] + [_c_int] * 3 + [_vp, _vp]
+        L.orc_mesh_sdf.argtypes = [_vp, _c_int, _vp, _c_int] + [_c_int] * 3 + [_c_double, _c_int, _vp, _vp]
+        L.orc_sim_inflow_cells.argtypes = [_vp, _c_int, _vp]
+        L.orc_ray_jitter.argtypes = [ctypes.c_uint64, _c_int, _c_int, _c_double, _vp]
+        L.orc_inflow_build.argtypes = ([_vp, _c_int, _vp, _c_int] + [_c_int] * 3
+                                       + [_c_double, ctypes.c_uint64, _c_int, _vp, _vp, _vp])
+
+    def _mesh_args(self, verts, tris):
+        verts = np.ascontiguousarray(verts, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        return verts, tris
+
+    def mesh_nodes_inside(self, verts, tris, dx, dims, jitter):
+        """Nodes (nk, nj, ni) of an ni x nj x nk node grid inside a closed mesh."""
+        verts, tris = self._mesh_args(verts, tris)
+        ni, nj, nk = dims
+        out = np.zeros((nk, nj, ni), np.uint8)
+        jit = np.ascontiguousarray(jitter, np.float32)
+        rc = self.lib.orc_mesh_nodes_inside(_ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0], dx,
+                                            ni, nj, nk, _ptr(jit), _ptr(out))
+        return None if rc != 0 else out
+
+    def mesh_sdf(self, verts, tris, isize, jsize, ksize, dx, band, jitter):
+        verts, tris = self._mesh_args(verts, tris)
+        phi = np.zeros((ksize + 1, jsize + 1, isize + 1), np.float32)
+        jit = np.ascontiguousarray(jitter, np.float32)
+        rc = self.lib.orc_mesh_sdf(_ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0], isize, jsize,
+                                   ksize, dx, band, _ptr(jit), _ptr(phi))
+        return None if rc != 0 else phi
+
+    def inflow_cells(self, sim, sid):
+        I, J, K = sim.dims
+        out = np.zeros((K, J, I), np.uint8)
+        if self.lib.orc_sim_inflow_cells(sim.h, sid, _ptr(out)) != 0:
+            return None
+        return out
+
+    def inflow_build(self, verts, tris, dims, dx, seed=0, sid=0):
+        """(sub-grid offset, nodal SDF (nk, nj, ni), cells (K, J, I)) of a source."""
+        verts, tris = self._mesh_args(verts, tris)
+        I, J, K = dims
+        d = np.zeros(6, np.int32)
+        L = self.lib
+        if L.orc_inflow_build(_ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0], I, J, K, dx, seed, sid,
+                              _ptr(d), None, None) != 0:
+            return None
+        phi = np.zeros((d[5], d[4], d[3]), np.float32)
+        cells = np.zeros((K, J, I), np.uint8)
+        L.orc_inflow_build(_ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0], I, J, K, dx, seed, sid,
+                           _ptr(d), _ptr(phi), _ptr(cells))
+        return tuple(int(v) for v in d[:3]), phi, cells
+
+    def ray_jitter(self, seed, sid, which, dx):
+        out = np.zeros(3, np.float32)
+        self.lib.orc_ray_jitter(seed, sid, which, dx, _ptr(out))
+        return out
 
     def pressure_solve(self, isize, jsize, ksize, dx, dt, liquid_phi, solid_phi, weights, mac, valid,
                        tol=1e-9, maxiter=1000, mode=0, solid_vel=None):
@@ -275,6 +363,12 @@ class Reference(_Base):
         L.ref_pressure_solve.argtypes = ([_c_int] * 3 + [_c_double, _c_double, _c_double, _c_int]
                                          + [_vp] * 15 + [_vp] * 5)
         L.ref_sim_pressure_status.argtypes = [_vp, ctypes.c_char_p, _c_int]
+        L.ref_sim_set_jitter.argtypes = [_vp, _c_double, _c_int]
+        L.ref_mesh_nodes_inside.argtypes = ([_vp, _c_int, _vp, _c_int, _c_double] + [_c_int] * 3
+                                            + [ctypes.c_uint, _vp, _vp])
+        L.ref_mesh_sdf.argtypes = ([_vp, _c_int, _vp, _c_int] + [_c_int] * 3
+                                   + [_c_double, _c_int, ctypes.c_uint, _vp, _vp])
+        L.ref_sim_inflow_cells.argtypes = [_vp, _c_int, ctypes.c_uint, _vp, _vp]
         self.threads = threads
 
     def sim_create(self, isize, jsize, ksize, dx):
@@ -283,6 +377,33 @@ class Reference(_Base):
     def set_threads(self, n):
         self.threads = n
         self.lib.ref_set_thread_count(n)
+
+    def mesh_nodes_inside(self, verts, tris, dx, dims, seed):
+        """(nodes, ray jitter) of getGridNodesInsideTriangleMesh with rand() seeded."""
+        verts = np.ascontiguousarray(verts, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        ni, nj, nk = dims
+        out = np.zeros((nk, nj, ni), np.uint8)
+        jit = np.zeros(3, np.float32)
+        self.lib.ref_mesh_nodes_inside(_ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0], dx, ni, nj, nk,
+                                       seed, _ptr(jit), _ptr(out))
+        return out, jit
+
+    def mesh_sdf(self, verts, tris, isize, jsize, ksize, dx, band, seed):
+        verts = np.ascontiguousarray(verts, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        phi = np.zeros((ksize + 1, jsize + 1, isize + 1), np.float32)
+        jit = np.zeros(3, np.float32)
+        self.lib.ref_mesh_sdf(_ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0], isize, jsize, ksize,
+                              dx, band, seed, _ptr(jit), _ptr(phi))
+        return phi, jit
+
+    def inflow_cells(self, sim, sid, seed):
+        I, J, K = sim.dims
+        out = np.zeros((K, J, I), np.uint8)
+        jit = np.zeros(3, np.float32)
+        self.lib.ref_sim_inflow_cells(sim.h, sid, seed, _ptr(jit), _ptr(out))
+        return out, jit
 
     def pressure_solve(self, isize, jsize, ksize, dx, dt, liquid_phi, solid_phi, weights, mac, valid,
                        tol=1e-9, maxiter=1000, mode=0, solid_vel=None):

@@ -203,6 +203,35 @@ struct Tri {
     Idx gmin, gmax;
 };
 
+// MeshLevelSet::_initializeTriangleData for one triangle (meshlevelset.cpp:
+// 878-915): the node range [floor(min/dx) - band, floor(max/dx) + band + 1]
+// of the triangle's AABB (aabb.cpp:84-112), clamped to the node grid; false
+// when the range misses the grid.
+static bool tri_band(Tri &T, int isize, int jsize, int ksize, double dx, int band) {
+    double mnx = T.v[0].x, mny = T.v[0].y, mnz = T.v[0].z;
+    double mxx = mnx, mxy = mny, mxz = mnz;
+    for (int c = 0; c < 3; c++) {
+        mnx = std::fmin((double)T.v[c].x, mnx); mny = std::fmin((double)T.v[c].y, mny);
+        mnz = std::fmin((double)T.v[c].z, mnz); mxx = std::fmax((double)T.v[c].x, mxx);
+        mxy = std::fmax((double)T.v[c].y, mxy); mxz = std::fmax((double)T.v[c].z, mxz);
+    }
+    Box tb;
+    tb.pos = V3((float)mnx, (float)mny, (float)mnz);
+    tb.w = mxx - mnx + 1e-9; tb.h = mxy - mny + 1e-9; tb.d = mxz - mnz + 1e-9;
+    V3 pmax = tb.maxp();
+    Idx gmin = pos_to_index(tb.pos, dx), gmax = pos_to_index(pmax, dx);
+    gmin.i -= band; gmin.j -= band; gmin.k -= band;
+    gmax.i += band + 1; gmax.j += band + 1; gmax.k += band + 1;
+    if (gmax.i < 0 || gmin.i >= isize + 1 || gmax.j < 0 || gmin.j >= jsize + 1 || gmax.k < 0 ||
+        gmin.k >= ksize + 1)
+        return false;
+    gmin.i = std::max(gmin.i, 0); gmin.j = std::max(gmin.j, 0); gmin.k = std::max(gmin.k, 0);
+    gmax.i = std::min(gmax.i, isize); gmax.j = std::min(gmax.j, jsize);
+    gmax.k = std::min(gmax.k, ksize);
+    T.gmin = gmin; T.gmax = gmax;
+    return true;
+}
+
 // Box mesh (FluidSimulation::_getTriangleMeshFromAABB, fluidsimulation.cpp:
 // 2805-2832) with per-triangle exact-band node ranges
 // (MeshLevelSet::_initializeTriangleData, meshlevelset.cpp:874-915).
@@ -225,29 +254,7 @@ static std::vector<Tri> box_triangles(const Box &bb, int isize, int jsize, int k
     for (int t = 0; t < 12; t++) {
         Tri T;
         for (int c = 0; c < 3; c++) T.v[c] = vert[tri[t][c]];
-        // AABB(Triangle, vertices) (aabb.cpp:84-112)
-        double mnx = T.v[0].x, mny = T.v[0].y, mnz = T.v[0].z;
-        double mxx = mnx, mxy = mny, mxz = mnz;
-        for (int c = 0; c < 3; c++) {
-            mnx = std::fmin((double)T.v[c].x, mnx); mny = std::fmin((double)T.v[c].y, mny);
-            mnz = std::fmin((double)T.v[c].z, mnz); mxx = std::fmax((double)T.v[c].x, mxx);
-            mxy = std::fmax((double)T.v[c].y, mxy); mxz = std::fmax((double)T.v[c].z, mxz);
-        }
-        Box tb;
-        tb.pos = V3((float)mnx, (float)mny, (float)mnz);
-        tb.w = mxx - mnx + 1e-9; tb.h = mxy - mny + 1e-9; tb.d = mxz - mnz + 1e-9;
-        V3 pmax = tb.maxp();
-        Idx gmin = pos_to_index(tb.pos, dx), gmax = pos_to_index(pmax, dx);
-        gmin.i -= band; gmin.j -= band; gmin.k -= band;
-        gmax.i += band + 1; gmax.j += band + 1; gmax.k += band + 1;
-        if (gmax.i < 0 || gmin.i >= isize + 1 || gmax.j < 0 || gmin.j >= jsize + 1 ||
-            gmax.k < 0 || gmin.k >= ksize + 1)
-            continue;
-        gmin.i = std::max(gmin.i, 0); gmin.j = std::max(gmin.j, 0); gmin.k = std::max(gmin.k, 0);
-        gmax.i = std::min(gmax.i, isize); gmax.j = std::min(gmax.j, jsize);
-        gmax.k = std::min(gmax.k, ksize);
-        T.gmin = gmin; T.gmax = gmax;
-        out.push_back(T);
+        if (tri_band(T, isize, jsize, ksize, dx, band)) out.push_back(T);
     }
     return out;
 }
@@ -1200,6 +1207,271 @@ static PressureResult pressure_solve(const PressureInputs &in) {
 // ---------------------------------------------------------------------------
 // Simulation driver (FluidSimulation, fluidsimulation.cpp)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Mesh inflow sources: MeshFluidSource with a static mesh
+// (meshfluidsource.cpp:41-44, 166-233; MeshObject::updateMeshStatic,
+// meshobject.cpp:50-59), inflow, without appended object velocity.
+// ---------------------------------------------------------------------------
+struct TriMesh {
+    std::vector<V3> v;
+    std::vector<int> t;  // three vertex indices per triangle
+    size_t ntri() const { return t.size() / 3; }
+};
+
+static inline V3 cross(V3 a, V3 b) {  // vmath.h:80-84
+    return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// AABB(std::vector<vec3>) (aabb.cpp:56-80): float corner, extents + 1e-9.
+static Box points_box(const std::vector<V3> &pts) {
+    double mnx = pts[0].x, mny = pts[0].y, mnz = pts[0].z;
+    double mxx = mnx, mxy = mny, mxz = mnz;
+    for (const V3 &p : pts) {
+        mnx = std::fmin((double)p.x, mnx); mny = std::fmin((double)p.y, mny);
+        mnz = std::fmin((double)p.z, mnz); mxx = std::fmax((double)p.x, mxx);
+        mxy = std::fmax((double)p.y, mxy); mxz = std::fmax((double)p.z, mxz);
+    }
+    Box b;
+    b.pos = V3((float)mnx, (float)mny, (float)mnz);
+    b.w = mxx - mnx + 1e-9; b.h = mxy - mny + 1e-9; b.d = mxz - mnz + 1e-9;
+    return b;
+}
+
+// Collision::lineIntersectsTriangle (collision.cpp:79-118) for the +z
+// direction; *z receives the crossing's z (p + (float)t * dir).
+static bool ray_hit_z(V3 p, V3 v0, V3 v1, V3 v2, float *z) {
+    const V3 dir(0.0f, 0.0f, 1.0f);
+    V3 e1 = v1 - v0, e2 = v2 - v0;
+    V3 h = cross(dir, e2);
+    float a = dot(e1, h);
+    double eps = 10e-9;
+    if (std::fabs(a) < eps) return false;
+    float f = 1 / a;
+    V3 s = p - v0;
+    float u = f * dot(s, h);
+    if (u < 0.0 || u > 1.0) return false;
+    V3 q = cross(s, e1);
+    float v = f * dot(dir, q);
+    if (v < 0.0 || u + v > 1.0) return false;
+    double t = f * dot(e2, q);
+    *z = p.z + (float)t * dir.z;
+    return true;
+}
+
+// MeshUtils::getCellsInsideTriangleMesh (meshutils.cpp:99-214) on an
+// ni x nj x nk cell grid.  Each (i, j) column casts a +z ray from its
+// jittered cell centre at k = -1; a cell is inside when its column has an
+// even number of crossings of which an odd number lie below the cell centre.
+// The reference first bins triangles into the columns their bounding boxes
+// cover; a crossing of a column's ray lies inside that column, so testing
+// every triangle finds the same crossings.
+static void cells_inside(const TriMesh &m, int ni, int nj, int nk, double dx, V3 jitter,
+                         GridB &inside) {
+    inside = GridB(ni, nj, nk, 0);
+    std::vector<float> zs;
+    for (int j = 0; j < nj; j++)
+        for (int i = 0; i < ni; i++) {
+            V3 gp = index_to_center(i, j, -1, dx) + jitter;
+            zs.clear();
+            for (size_t t = 0; t < m.ntri(); t++) {
+                float z;
+                if (ray_hit_z(gp, m.v[m.t[3 * t]], m.v[m.t[3 * t + 1]], m.v[m.t[3 * t + 2]], &z))
+                    zs.push_back(z);
+            }
+            if (zs.size() % 2 != 0) continue;
+            for (int k = 0; k < nk; k++) {
+                float zc = index_to_center(i, j, k, dx).z;
+                int below = 0;
+                for (float z : zs) below += z < zc;
+                if (below % 2 == 1) inside(i, j, k) = 1;
+            }
+        }
+}
+
+// MeshUtils::getGridNodesInsideTriangleMesh(mesh, dx, Array3d<bool>)
+// (meshutils.cpp:347-420) for meshes inside the node grid's box -- the only
+// case handled; false otherwise (the reference then splits the mesh).
+static bool nodes_inside(const TriMesh &m0, double dx, int nni, int nnj, int nnk, V3 jitter,
+                         GridB &nodes) {
+    nodes = GridB(nni, nnj, nnk, 0);
+    Box grid;
+    grid.w = (nni - 1) * dx; grid.h = (nnj - 1) * dx; grid.d = (nnk - 1) * dx;
+    for (const V3 &p : m0.v)
+        if (!grid.inside(p)) return false;
+    if (m0.v.empty()) return true;
+    // getGridNodesInsideTriangleMesh(mesh, dx, std::vector) (meshutils.cpp:347-372):
+    // nodes are the cell centres of the grid shifted by half a cell.
+    TriMesh m = m0;
+    float hw = (float)(0.5 * dx);
+    for (V3 &p : m.v) p = p + V3(hw, hw, hw);
+    Box bb = points_box(m.v);
+    Idx goff = pos_to_index(bb.pos, dx);
+    V3 off = index_to_pos(goff.i, goff.j, goff.k, dx);
+    for (V3 &p : m.v) p = p - off;
+    bb.pos = bb.pos - off;
+    Idx gmax = pos_to_index(bb.pos + V3((float)bb.w, (float)bb.h, (float)bb.d), dx);
+    GridB local;
+    cells_inside(m, gmax.i + 1, gmax.j + 1, gmax.k + 1, dx, jitter, local);
+    for (int k = 0; k < local.nk; k++)
+        for (int j = 0; j < local.nj; j++)
+            for (int i = 0; i < local.ni; i++)
+                if (local(i, j, k) && nodes.in(i + goff.i, j + goff.j, k + goff.k))
+                    nodes(i + goff.i, j + goff.j, k + goff.k) = 1;
+    return true;
+}
+
+// Exact-band unsigned distances of a triangle list on a node grid, each node
+// taking its minimum in the frame of the 10^3 block that owns it
+// (meshlevelset.cpp:781-876, 1115-1163); other nodes keep `upper`.
+static void exact_band(const std::vector<Tri> &tris, GridF &phi, double dx, float upper) {
+    const int bw = 10;  // meshlevelset.h:370 (_blockwidth)
+    phi.fill(upper);
+    for (int k = 0; k < phi.nk; k++)
+        for (int j = 0; j < phi.nj; j++)
+            for (int i = 0; i < phi.ni; i++) {
+                int bi = i / bw, bj = j / bw, bk = k / bw;
+                V3 boff = index_to_pos(bi, bj, bk, bw * dx);
+                V3 gpos = index_to_pos(i - bi * bw, j - bj * bw, k - bk * bw, dx);
+                float best = upper;
+                for (const Tri &t : tris) {
+                    if (i < t.gmin.i || j < t.gmin.j || k < t.gmin.k || i > t.gmax.i ||
+                        j > t.gmax.j || k > t.gmax.k)
+                        continue;
+                    V3 cp = closest_point_on_triangle(gpos, t.v[0] - boff, t.v[1] - boff,
+                                                      t.v[2] - boff);
+                    float dist = length(cp - gpos);
+                    if (dist < best) best = dist;
+                }
+                phi(i, j, k) = best;
+            }
+}
+
+// MeshLevelSet::fastCalculateSignedDistanceField (meshlevelset.cpp:577-601)
+// without velocity data: exact band, then negated inside the mesh
+// (_computeDistanceFieldSigns, meshlevelset.cpp:1298-1317).  phi must be
+// sized to the node grid already.  False if the mesh leaves the grid box.
+static bool mesh_sdf(const TriMesh &m, double dx, int band, V3 jitter, GridF &phi) {
+    int isize = phi.ni - 1, jsize = phi.nj - 1, ksize = phi.nk - 1;
+    float upper = (float)((phi.ni + phi.nj + phi.nk) * dx);  // meshlevelset.cpp:769-771
+    std::vector<Tri> tris;
+    for (size_t t = 0; t < m.ntri(); t++) {
+        Tri T;
+        for (int c = 0; c < 3; c++) T.v[c] = m.v[m.t[3 * t + c]];
+        if (tri_band(T, isize, jsize, ksize, dx, band)) tris.push_back(T);
+    }
+    exact_band(tris, phi, dx, upper);
+    GridB nodes;
+    if (!nodes_inside(m, dx, phi.ni, phi.nj, phi.nk, jitter, nodes)) return false;
+    for (size_t q = 0; q < phi.a.size(); q++)
+        if (nodes.a[q]) phi.a[q] = -phi.a[q];
+    return true;
+}
+
+struct InflowSource {
+    TriMesh mesh;
+    V3 velocity;              // setVelocity (meshfluidsource.cpp:114-116)
+    bool enabled = true;      // meshfluidsource.h
+    bool constrain = true;    // _isConstrainedFluidVelocity (meshfluidsource.h:124)
+    int emissions = 1;        // _substepEmissions (meshfluidsource.h:119)
+    bool ready = false;       // static mesh: the SDF and cells are built once
+    Idx gridOffset;           // _sourceSDFGridOffset
+    V3 sdfOffset;             // _sourceSDFOffset
+    GridF sdf;                // _sourceSDF, nodal on the source sub-grid
+    std::vector<Idx> cells;   // MeshObject::getCells
+    GridB isCell;
+};
+
+// Splitmix64, the generator behind the inflow jitter: the reference draws
+// rand() (fluidsimulation.h:1501-1503, meshutils.cpp:95-97), from several
+// threads at once; the value of a draw here is keyed by what it jitters.
+static inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+static inline double draw_unit(uint64_t seed, uint64_t stream, uint64_t item) {
+    uint64_t h = splitmix64(splitmix64(seed ^ (stream * 0xD1B54A32D192ED03ull)) + item);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+// The ray jitter of one inside test (meshutils.cpp:107-110):
+// _randomDouble(jit, -jit) per axis with jit = 0.001 dx.  Stream 0 keys the
+// source SDF's signs, stream 1 the source cells.
+static V3 ray_jitter(uint64_t seed, int source, int which, double dx) {
+    double jit = 0.001 * dx;
+    uint64_t st = ((uint64_t)(source + 1) << 8) | (uint64_t)which;
+    double r[3];
+    for (int a = 0; a < 3; a++) r[a] = jit + draw_unit(seed, st, a) * (-jit - jit);
+    return V3((float)r[0], (float)r[1], (float)r[2]);
+}
+
+// MeshFluidSource::update (meshfluidsource.cpp:178-222) with
+// _getGridBoundsFromTriangleMesh (:318-333, grid pad 4, meshfluidsource.h:
+// 133) and exact band 2 (meshfluidsource.h:116), plus the source cells of
+// MeshObject::getCells (meshobject.cpp:101-140).  The mesh must lie inside
+// the domain grid's box.
+static bool source_build(InflowSource &S, int I, int J, int K, double dx, uint64_t seed, int id) {
+    Box bb = points_box(S.mesh.v);
+    bb.expand(4.0 * dx);
+    Idx gmin = pos_to_index(bb.pos, dx), gmax = pos_to_index(bb.maxp(), dx);
+    gmin.i = (int)std::fmax(gmin.i, 0); gmin.j = (int)std::fmax(gmin.j, 0);
+    gmin.k = (int)std::fmax(gmin.k, 0);
+    gmax.i = (int)std::fmin(gmax.i, I - 1); gmax.j = (int)std::fmin(gmax.j, J - 1);
+    gmax.k = (int)std::fmin(gmax.k, K - 1);
+    S.gridOffset = gmin;
+    S.sdfOffset = index_to_pos(gmin.i, gmin.j, gmin.k, dx);
+    int isdf = std::max(gmax.i - gmin.i + 1, 1), jsdf = std::max(gmax.j - gmin.j + 1, 1),
+        ksdf = std::max(gmax.k - gmin.k + 1, 1);
+    TriMesh local = S.mesh;
+    V3 neg = V3(-S.sdfOffset.x, -S.sdfOffset.y, -S.sdfOffset.z);
+    for (V3 &p : local.v) p = p + neg;  // TriangleMesh::translate (trianglemesh.cpp:732-736)
+    S.sdf = GridF(isdf + 1, jsdf + 1, ksdf + 1, 0.0f);
+    if (!mesh_sdf(local, dx, 2, ray_jitter(seed, id, 0, dx), S.sdf)) return false;
+    GridB nodes;
+    if (!nodes_inside(S.mesh, dx, I + 1, J + 1, K + 1, ray_jitter(seed, id, 1, dx), nodes))
+        return false;
+    S.isCell = GridB(I, J, K, 0);
+    for (int k = 0; k < nodes.nk; k++)
+        for (int j = 0; j < nodes.nj; j++)
+            for (int i = 0; i < nodes.ni; i++) {
+                if (!nodes(i, j, k)) continue;
+                for (int c = 0; c < 8; c++) {  // Grid3d::getVertexGridIndexNeighbours
+                    int a = i - ((c & 1) ? 1 : 0), b = j - ((c & 2) ? 1 : 0), d = k - ((c & 4) ? 1 : 0);
+                    if (S.isCell.in(a, b, d)) S.isCell(a, b, d) = 1;
+                }
+            }
+    S.cells.clear();
+    for (int k = 0; k < K; k++)
+        for (int j = 0; j < J; j++)
+            for (int i = 0; i < I; i++)
+                if (S.isCell(i, j, k)) S.cells.push_back(Idx{i, j, k});
+    S.ready = true;
+    return true;
+}
+
+// ParticleMaskGrid (particlemaskgrid.cpp:54-105): one bit per half-cell
+// subcell, set from the particle's cell and its subcell's parities.
+struct MaskGrid {
+    GridB m;
+    double dx = 0;
+    static int bit(Idx s) {
+        int c = 0;
+        if (s.i % 2 == 1) c |= 1;
+        if (s.j % 2 == 1) c |= 2;
+        if (s.k % 2 == 1) c |= 4;
+        return 1 << c;
+    }
+    void add(V3 p) {
+        Idx s = pos_to_index(p, 0.5 * dx);
+        Idx g = pos_to_index(p, dx);
+        m(g.i, g.j, g.k) |= (uint8_t)bit(s);
+    }
+    bool is_set(V3 p) const {
+        Idx s = pos_to_index(p, 0.5 * dx);
+        return (m(s.i / 2, s.j / 2, s.k / 2) & bit(s)) != 0;
+    }
+};
+
 struct Sim {
     int I, J, K;
     double dx;
@@ -1228,6 +1500,14 @@ struct Sim {
     double nearSolidCellSize = 0;
     int lastIters = 0, lastOk = 1;
     double lastErr = 0;
+    // Mesh inflow sources (FluidSimulation::addMeshFluidSource,
+    // fluidsimulation.cpp:1953-1965) and the marker-particle jitter
+    // (fluidsimulation.h:1553-1555).
+    std::vector<InflowSource> sources;
+    double jitterFactor = 1.0;
+    bool jitterSurface = false;
+    uint64_t seed = 0;
+    uint64_t emissions = 0;  // inflow emission passes so far (keys the jitter)
 
     V3 body_force() const {  // fluidsimulation.cpp:3363-3370
         V3 bf;
@@ -1393,11 +1673,137 @@ static void constrain(MAC &m, const Sim &S) {
     con(m.W, S.w.W, S.solidW);
 }
 
+// FluidSimulation::_addNewFluidCells / _addNewFluidCellsThread
+// (fluidsimulation.cpp:4397-4431, 4523-4562) for one emission pass of one
+// source: the eight subcell centres of every source cell inside the source
+// SDF, jittered where deeper than a cell (or always with surface jitter),
+// kept where the solid SDF is positive and the subcell is still empty.
+static void emit_source(Sim &S, InflowSource &src, MaskGrid &mask) {
+    const double dx = S.dx;
+    const double q = 0.25 * dx;
+    const V3 offs[8] = {V3((float)-q, (float)-q, (float)-q), V3((float)q, (float)-q, (float)-q),
+                        V3((float)-q, (float)q, (float)-q),  V3((float)q, (float)q, (float)-q),
+                        V3((float)-q, (float)-q, (float)q),  V3((float)q, (float)-q, (float)q),
+                        V3((float)-q, (float)q, (float)q),   V3((float)q, (float)q, (float)q)};
+    const double jitter = 0.25 * (S.jitterFactor - 1e-3) * dx;  // fluidsimulation.cpp:2623-2626
+    const uint64_t stream = 0x1000000ull + S.emissions++;
+    std::vector<V3> cand;
+    for (size_t c = 0; c < src.cells.size(); c++) {
+        const Idx g = src.cells[c];
+        const V3 ctr = index_to_center(g.i, g.j, g.k, dx);
+        for (int o = 0; o < 8; o++) {
+            V3 p = ctr + offs[o];
+            double d = grid_trilinear(src.sdf, p - src.sdfOffset, dx);
+            if (d > 0) continue;
+            if (S.jitterSurface || d < -dx) {
+                // _jitterMarkerParticlePosition (fluidsimulation.cpp:2628-2635)
+                uint64_t item = 3 * (8 * (uint64_t)c + o);
+                p.x += -jitter + draw_unit(S.seed, stream, item) * (jitter - -jitter);
+                p.y += -jitter + draw_unit(S.seed, stream, item + 1) * (jitter - -jitter);
+                p.z += -jitter + draw_unit(S.seed, stream, item + 2) * (jitter - -jitter);
+            }
+            if (grid_trilinear(S.solid, p, dx) > 0) cand.push_back(p);
+        }
+    }
+    for (const V3 &p : cand) {
+        if (mask.is_set(p)) continue;
+        Idx g = pos_to_index(p, dx);  // _addMarkerParticle (fluidsimulation.cpp:2637-2642)
+        if (g.i >= 0 && g.j >= 0 && g.k >= 0 && g.i < S.I && g.j < S.J && g.k < S.K) {
+            S.pos.push_back(p);
+            S.vel.push_back(src.velocity);
+        }
+        mask.add(p);
+    }
+}
+
+static void ensure_sources(Sim &S) {
+    for (size_t i = 0; i < S.sources.size(); i++)
+        if (!S.sources[i].ready && !source_build(S.sources[i], S.I, S.J, S.K, S.dx, S.seed, (int)i))
+            throw std::runtime_error("inflow mesh must lie inside the domain grid");
+}
+
+// FluidSimulation::_updateInflowMeshFluidSources / _updateInflowMeshFluidSource
+// (fluidsimulation.cpp:4564-4606, 4675-4697), static meshes.
+static void update_inflow_sources(Sim &S) {
+    int active = 0;
+    for (const InflowSource &s : S.sources) active += s.enabled;
+    if (active == 0) return;
+    ensure_sources(S);
+    MaskGrid mask;
+    mask.dx = S.dx;
+    mask.m = GridB(S.I, S.J, S.K, 0);
+    for (const V3 &p : S.pos) mask.add(p);
+    for (InflowSource &src : S.sources) {
+        if (!src.enabled) continue;
+        int passes = src.emissions;
+        if (passes == 0) {
+            passes = 1;
+            if (S.stepNumber != 0) continue;
+        }
+        for (int e = 0; e < passes; e++) emit_source(S, src, mask);
+    }
+}
+
+// FluidSimulation::_constrainMarkerParticleVelocities (fluidsimulation.cpp:
+// 4109-4170): particles in a source cell where the source SDF, sampled at the
+// particle's domain position (the reference passes no sub-grid offset
+// here), is not positive take the source velocity.  Every emission pass of a
+// static source repeats the same assignment, so one pass is applied.
+static void constrain_inflow_particles(Sim &S) {
+    for (InflowSource &src : S.sources) {
+        if (!src.enabled || !src.constrain) continue;
+        for (size_t i = 0; i < S.pos.size(); i++) {
+            Idx g = pos_to_index(S.pos[i], S.dx);
+            if (!src.isCell(g.i, g.j, g.k)) continue;
+            if (grid_trilinear(src.sdf, S.pos[i], S.dx) > 0.0f) continue;
+            S.vel[i] = src.velocity;
+        }
+    }
+}
+
+// FluidSimulation::_getInflowConstrainedVelocityComponents (fluidsimulation.cpp:
+// 3372-3438): valid faces where a source SDF, sampled at the face's domain
+// position (again without the sub-grid offset), is negative take no body
+// force.
+static void inflow_face_mask(const Sim &S, Valid &ex) {
+    valid_init(ex, S.I, S.J, S.K);
+    for (const InflowSource &src : S.sources) {
+        if (!src.enabled || !src.constrain) continue;
+        const double dx = S.dx;
+        for (int k = 0; k < S.K; k++)
+            for (int j = 0; j < S.J; j++)
+                for (int i = 0; i < S.I + 1; i++) {
+                    if (!S.valid.U(i, j, k)) continue;
+                    V3 p((float)((float)i * dx), (float)(((float)j + 0.5) * dx),
+                         (float)(((float)k + 0.5) * dx));
+                    if (grid_trilinear(src.sdf, p, dx) < 0.0f) ex.U(i, j, k) = 1;
+                }
+        for (int k = 0; k < S.K; k++)
+            for (int j = 0; j < S.J + 1; j++)
+                for (int i = 0; i < S.I; i++) {
+                    if (!S.valid.V(i, j, k)) continue;
+                    V3 p((float)(((float)i + 0.5) * dx), (float)((float)j * dx),
+                         (float)(((float)k + 0.5) * dx));
+                    if (grid_trilinear(src.sdf, p, dx) < 0.0f) ex.V(i, j, k) = 1;
+                }
+        for (int k = 0; k < S.K + 1; k++)
+            for (int j = 0; j < S.J; j++)
+                for (int i = 0; i < S.I; i++) {
+                    if (!S.valid.W(i, j, k)) continue;
+                    V3 p((float)(((float)i + 0.5) * dx), (float)(((float)j + 0.5) * dx),
+                         (float)((float)k * dx));
+                    if (grid_trilinear(src.sdf, p, dx) < 0.0f) ex.W(i, j, k) = 1;
+                }
+    }
+}
+
 // FluidSimulation::_stepFluid (fluidsimulation.cpp:5471-5508) for scenes with
 // domain walls only and a constant body force.
 static void step_fluid(Sim &S, double dt) {
     // _updateObstacleObjects (fluidsimulation.cpp:3196-3210, 3028-3072).
-    if (!S.solidUpToDate && !S.pos.empty()) {
+    // The solid is built once particles or mesh fluid sources exist
+    // (fluidsimulation.cpp:3038-3043).
+    if (!S.solidUpToDate && (!S.pos.empty() || !S.sources.empty())) {
         domain_solid_phi(S.I, S.J, S.K, S.dx, S.solid);
         S.solidU = GridF(S.I + 1, S.J, S.K, 0.0f);
         S.solidV = GridF(S.I, S.J + 1, S.K, 0.0f);
@@ -1420,12 +1826,20 @@ static void step_fluid(Sim &S, double dt) {
         extrapolate(S.mac.W, S.valid.W, layers);
     }
     S.saved = S.mac;
-    // _applyConstantBodyForces (fluidsimulation.cpp:3450-3489); no inflow faces.
+    // _applyConstantBodyForces (fluidsimulation.cpp:3450-3489), skipping the
+    // faces the inflow sources constrain (:3372-3438, 3566-3586).
+    if (!S.sources.empty()) ensure_sources(S);
+    Valid ex;
+    inflow_face_mask(S, ex);
     V3 bf = S.body_force();
     float beps = 1e-6f;
-    if (std::fabs(bf.x) > beps) for (float &u : S.mac.U.a) u += (float)(bf.x * dt);
-    if (std::fabs(bf.y) > beps) for (float &v : S.mac.V.a) v += (float)(bf.y * dt);
-    if (std::fabs(bf.z) > beps) for (float &w : S.mac.W.a) w += (float)(bf.z * dt);
+    auto force = [&](GridF &g, const GridB &skip, float f) {
+        for (size_t q = 0; q < g.a.size(); q++)
+            if (!skip.a[q]) g.a[q] += (float)(f * dt);
+    };
+    if (std::fabs(bf.x) > beps) force(S.mac.U, ex.U, bf.x);
+    if (std::fabs(bf.y) > beps) force(S.mac.V, ex.V, bf.y);
+    if (std::fabs(bf.z) > beps) force(S.mac.W, ex.W, bf.z);
     // _pressureSolve (fluidsimulation.cpp:3732-3769).
     if (!S.weightsUpToDate) {
         compute_weights(S.solid, S.I, S.J, S.K, S.w);
@@ -1448,14 +1862,21 @@ static void step_fluid(Sim &S, double dt) {
         V3 vflip = S.vel[i] + vpic - mac_velocity(S.saved, p, S.dx, S.I, S.J, S.K);
         S.vel[i] = (float)S.picRatio * vpic + (float)(1 - S.picRatio) * vflip;
     }
+    constrain_inflow_particles(S);  // fluidsimulation.cpp:4172-4185
     advance_particles(S, dt);
+    // _updateFluidObjects (fluidsimulation.cpp:4794-4806).
+    update_inflow_sources(S);
 }
 
 // FluidSimulation::_calculateNextTimeStep (fluidsimulation.cpp:5593-5613).
 static double next_time_step(const Sim &S, double dt) {
     double maxu = 0.0;
     if (S.currentFrame == 0 && S.stepNumber == 0) {
-        maxu = 0.0 + (double)length(S.body_force()) * dt;  // :5524-5551
+        // _predictMaximumMarkerParticleSpeed (:5529-5551): the fastest
+        // enabled inflow source plus one step of body force.
+        for (const InflowSource &src : S.sources)
+            if (src.enabled) maxu = std::fmax((double)std::fmax((double)length(src.velocity), 0.0), maxu);
+        maxu += (double)length(S.body_force()) * dt;
     } else {
         double maxsq = 0.0;
         for (const V3 &v : S.vel) {
@@ -1661,6 +2082,101 @@ void orc_trilinear_velocity(int isize, int jsize, int ksize, double dx, const fl
             r = V3();
         out[3 * q] = r.x; out[3 * q + 1] = r.y; out[3 * q + 2] = r.z;
     }
+}
+
+static TriMesh make_mesh(const float *verts, int nv, const int *tris, int nt) {
+    TriMesh m;
+    for (int i = 0; i < nv; i++) m.v.push_back(V3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]));
+    m.t.assign(tris, tris + 3 * (size_t)nt);
+    return m;
+}
+
+// MeshUtils::getGridNodesInsideTriangleMesh on an nni x nnj x nnk node grid
+// with the given ray jitter; -1 if the mesh leaves the grid box.
+int orc_mesh_nodes_inside(const float *verts, int nv, const int *tris, int nt, double dx, int nni,
+                          int nnj, int nnk, const float *jitter, uint8_t *out) {
+    GridB nodes;
+    if (!nodes_inside(make_mesh(verts, nv, tris, nt), dx, nni, nnj, nnk,
+                      V3(jitter[0], jitter[1], jitter[2]), nodes))
+        return -1;
+    copy_out(nodes, out);
+    return 0;
+}
+
+// MeshLevelSet(isize, jsize, ksize, dx).fastCalculateSignedDistanceField(mesh,
+// band) with the given ray jitter; phi is (isize+1)(jsize+1)(ksize+1).
+int orc_mesh_sdf(const float *verts, int nv, const int *tris, int nt, int isize, int jsize,
+                 int ksize, double dx, int band, const float *jitter, float *phi) {
+    GridF g(isize + 1, jsize + 1, ksize + 1, 0.0f);
+    if (!mesh_sdf(make_mesh(verts, nv, tris, nt), dx, band, V3(jitter[0], jitter[1], jitter[2]), g))
+        return -1;
+    copy_out(g, phi);
+    return 0;
+}
+
+// MeshFluidSource::update + getCells of source `id` on an I x J x K grid,
+// stand-alone: dims = sub-grid offset and node dims; phi / cells optional.
+int orc_inflow_build(const float *verts, int nv, const int *tris, int nt, int I, int J, int K,
+                     double dx, uint64_t seed, int id, int *dims, float *phi, uint8_t *cells) {
+    InflowSource src;
+    src.mesh = make_mesh(verts, nv, tris, nt);
+    if (!source_build(src, I, J, K, dx, seed, id)) return -1;
+    dims[0] = src.gridOffset.i; dims[1] = src.gridOffset.j; dims[2] = src.gridOffset.k;
+    dims[3] = src.sdf.ni; dims[4] = src.sdf.nj; dims[5] = src.sdf.nk;
+    if (phi) copy_out(src.sdf, phi);
+    if (cells) copy_out(src.isCell, cells);
+    return 0;
+}
+
+// MeshFluidSource(isize, jsize, ksize, dx) + updateMeshStatic + setInflow +
+// setVelocity, added with FluidSimulation::addMeshFluidSource; returns its
+// index.
+int orc_sim_add_inflow(void *h, const float *verts, int nv, const int *tris, int nt, double vx,
+                       double vy, double vz) {
+    Sim *S = (Sim *)h;
+    InflowSource src;
+    src.mesh = make_mesh(verts, nv, tris, nt);
+    src.velocity = V3((float)vx, (float)vy, (float)vz);
+    S->sources.push_back(src);
+    return (int)S->sources.size() - 1;
+}
+// enable()/disable(), enable/disableConstrainedFluidVelocity() and
+// setSubstepEmissions() of one source (meshfluidsource.cpp:51-72, 146-156).
+void orc_sim_set_inflow(void *h, int id, int enabled, int constrain, int emissions) {
+    InflowSource &src = ((Sim *)h)->sources[id];
+    src.enabled = enabled != 0;
+    src.constrain = constrain != 0;
+    src.emissions = emissions;
+}
+// setMarkerParticleJitterFactor / enable/disableJitterSurfaceMarkerParticles
+// (fluidsimulation.cpp:185-210) and the seed of the jitter draws.
+void orc_sim_set_jitter(void *h, double factor, int surface, uint64_t seed) {
+    Sim *S = (Sim *)h;
+    S->jitterFactor = factor;
+    S->jitterSurface = surface != 0;
+    S->seed = seed;
+}
+// The source's SDF (nodal, sub-grid) and its placement after the first
+// update; dims/offset as ints (gi, gj, gk, ni, nj, nk).  -1 before it exists.
+int orc_sim_inflow_sdf(void *h, int id, int *dims, float *phi) {
+    InflowSource &src = ((Sim *)h)->sources[id];
+    if (!src.ready) return -1;
+    dims[0] = src.gridOffset.i; dims[1] = src.gridOffset.j; dims[2] = src.gridOffset.k;
+    dims[3] = src.sdf.ni; dims[4] = src.sdf.nj; dims[5] = src.sdf.nk;
+    if (phi) copy_out(src.sdf, phi);
+    return 0;
+}
+// The source's cells as a flat I*J*K byte grid.
+int orc_sim_inflow_cells(void *h, int id, uint8_t *cells) {
+    InflowSource &src = ((Sim *)h)->sources[id];
+    if (!src.ready) return -1;
+    copy_out(src.isCell, cells);
+    return 0;
+}
+// The jitter of one inside test of source id (which = 0: SDF signs, 1: cells).
+void orc_ray_jitter(uint64_t seed, int id, int which, double dx, float *out) {
+    V3 j = ray_jitter(seed, id, which, dx);
+    out[0] = j.x; out[1] = j.y; out[2] = j.z;
 }
 
 void *orc_sim_create(int isize, int jsize, int ksize, double dx) {

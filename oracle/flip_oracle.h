@@ -71,6 +71,22 @@ void orc_sim_get_float_grid(void *h, int which, float *out);
 void orc_sim_get_bool_grid(void *h, int which, uint8_t *out);
 int orc_sim_frame_substeps(void *h);
 void orc_sim_pressure_info(void *h, int *iters, double *err, int *ok);
+// Mesh inside tests, mesh level sets and inflow sources
+// (meshutils.cpp:99-428, meshlevelset.cpp:572-601, meshfluidsource.cpp,
+// fluidsimulation.cpp:3372-3438, 4109-4185, 4397-4806).
+int orc_mesh_nodes_inside(const float *verts, int nv, const int *tris, int nt, double dx, int nni,
+                          int nnj, int nnk, const float *jitter, uint8_t *out);
+int orc_mesh_sdf(const float *verts, int nv, const int *tris, int nt, int isize, int jsize,
+                 int ksize, double dx, int band, const float *jitter, float *phi);
+int orc_sim_add_inflow(void *h, const float *verts, int nv, const int *tris, int nt, double vx,
+                       double vy, double vz);
+void orc_sim_set_inflow(void *h, int id, int enabled, int constrain, int emissions);
+void orc_sim_set_jitter(void *h, double factor, int surface, uint64_t seed);
+int orc_sim_inflow_sdf(void *h, int id, int *dims, float *phi);
+int orc_sim_inflow_cells(void *h, int id, uint8_t *cells);
+void orc_ray_jitter(uint64_t seed, int id, int which, double dx, float *out);
+int orc_inflow_build(const float *verts, int nv, const int *tris, int nt, int I, int J, int K,
+                     double dx, uint64_t seed, int id, int *dims, float *phi, uint8_t *cells);
 
 #ifdef __cplusplus
 }

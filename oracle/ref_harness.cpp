@@ -53,6 +53,9 @@
 #include "particleadvector.h"
 #include "particlelevelset.h"
 #include "meshlevelset.h"
+#include "meshfluidsource.h"
+#include "meshutils.h"
+#include "trianglemesh.h"
 #include "macvelocityfield.h"
 #include "gridutils.h"
 #include "threadutils.h"
@@ -64,8 +67,32 @@ namespace {
 
 struct RefSim {
     FluidSimulation *sim = nullptr;
+    std::vector<MeshFluidSource *> sources;
     std::string last_error;
 };
+
+TriangleMesh make_tri_mesh(const float *verts, int nv, const int *tris, int nt) {
+    TriangleMesh m;
+    for (int i = 0; i < nv; i++)
+        m.vertices.push_back(vmath::vec3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]));
+    for (int i = 0; i < nt; i++) {
+        Triangle t;
+        t.tri[0] = tris[3 * i]; t.tri[1] = tris[3 * i + 1]; t.tri[2] = tris[3 * i + 2];
+        m.triangles.push_back(t);
+    }
+    return m;
+}
+
+// The ray jitter MeshUtils::_getCollisionGridZ draws right after srand(seed)
+// (meshutils.cpp:107-110), written as the same expression so the compiler
+// assigns the three draws to the axes in the same order.
+vmath::vec3 predicted_ray_jitter(unsigned seed, double dx) {
+    srand(seed);
+    double jit = 0.001 * dx;
+    vmath::vec3 jitter(MeshUtils::_randomDouble(jit, -jit), MeshUtils::_randomDouble(jit, -jit),
+                       MeshUtils::_randomDouble(jit, -jit));
+    return jitter;
+}
 
 template <class T>
 void copy_array(Array3d<T> &a, T *out) {
@@ -123,6 +150,7 @@ void ref_sim_destroy(void *hp) {
     RefSim *h = (RefSim *)hp;
     if (!h) return;
     delete h->sim;
+    for (MeshFluidSource *src : h->sources) delete src;
     delete h;
 }
 
@@ -261,6 +289,97 @@ void ref_sim_last_error(void *hp, char *buf, int buflen) {
 // ---------------------------------------------------------------------------
 // Stage-level entry points on caller-supplied inputs
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Mesh inside tests, mesh level sets and inflow sources
+// ---------------------------------------------------------------------------
+
+// MeshUtils::getGridNodesInsideTriangleMesh(mesh, dx, Array3d<bool>)
+// (meshutils.cpp:374-428) with rand() seeded by `seed`; the ray jitter that
+// seed produces is written to jitter_out.
+void ref_mesh_nodes_inside(const float *verts, int nv, const int *tris, int nt, double dx,
+                           int nni, int nnj, int nnk, unsigned seed, float *jitter_out,
+                           uint8_t *out) {
+    TriangleMesh m = make_tri_mesh(verts, nv, tris, nt);
+    vmath::vec3 j = predicted_ray_jitter(seed, dx);
+    jitter_out[0] = j.x; jitter_out[1] = j.y; jitter_out[2] = j.z;
+    Array3d<bool> nodes(nni, nnj, nnk, false);
+    srand(seed);
+    MeshUtils::getGridNodesInsideTriangleMesh(m, dx, nodes);
+    copy_bool_array(nodes, out);
+}
+
+// MeshLevelSet(isize, jsize, ksize, dx).fastCalculateSignedDistanceField(mesh,
+// band) (meshlevelset.cpp:572-601) without velocity data, rand() seeded by
+// `seed`.
+void ref_mesh_sdf(const float *verts, int nv, const int *tris, int nt, int isize, int jsize,
+                  int ksize, double dx, int band, unsigned seed, float *jitter_out, float *phi) {
+    TriangleMesh m = make_tri_mesh(verts, nv, tris, nt);
+    vmath::vec3 j = predicted_ray_jitter(seed, dx);
+    jitter_out[0] = j.x; jitter_out[1] = j.y; jitter_out[2] = j.z;
+    MeshLevelSet ls(isize, jsize, ksize, dx);
+    ls.disableVelocityData();
+    srand(seed);
+    ls.fastCalculateSignedDistanceField(m, band);
+    copy_array(ls._phi, phi);
+}
+
+// MeshFluidSource(isize, jsize, ksize, dx) + updateMeshStatic + setInflow +
+// setVelocity, then FluidSimulation::addMeshFluidSource
+// (meshfluidsource.cpp:32-44, 74-76, 114-116; fluidsimulation.cpp:1953-1965).
+int ref_sim_add_inflow(void *hp, const float *verts, int nv, const int *tris, int nt, double vx,
+                       double vy, double vz) {
+    RefSim *h = (RefSim *)hp;
+    FluidSimulation *sim = h->sim;
+    MeshFluidSource *src = new MeshFluidSource(sim->_isize, sim->_jsize, sim->_ksize, sim->_dx);
+    src->updateMeshStatic(make_tri_mesh(verts, nv, tris, nt));
+    src->setInflow();
+    src->setVelocity(vmath::vec3(vx, vy, vz));
+    sim->addMeshFluidSource(src);
+    h->sources.push_back(src);
+    return (int)h->sources.size() - 1;
+}
+
+void ref_sim_set_inflow(void *hp, int id, int enabled, int constrain, int emissions) {
+    MeshFluidSource *src = ((RefSim *)hp)->sources[id];
+    if (enabled) src->enable(); else src->disable();
+    if (constrain) src->enableConstrainedFluidVelocity(); else src->disableConstrainedFluidVelocity();
+    src->setSubstepEmissions(emissions);
+}
+
+void ref_sim_set_jitter(void *hp, double factor, int surface) {
+    FluidSimulation *sim = ((RefSim *)hp)->sim;
+    sim->setMarkerParticleJitterFactor(factor);
+    if (surface) sim->enableJitterSurfaceMarkerParticles();
+    else sim->disableJitterSurfaceMarkerParticles();
+}
+
+// The source's SDF after its first update: grid offset and node dims in
+// dims[0..5], phi nodal.
+int ref_sim_inflow_sdf(void *hp, int id, int *dims, float *phi) {
+    MeshFluidSource *src = ((RefSim *)hp)->sources[id];
+    MeshLevelSet *ls = src->getMeshLevelSet();
+    dims[0] = src->_sourceSDFGridOffset.i; dims[1] = src->_sourceSDFGridOffset.j;
+    dims[2] = src->_sourceSDFGridOffset.k;
+    dims[3] = ls->_phi.width; dims[4] = ls->_phi.height; dims[5] = ls->_phi.depth;
+    if (phi) copy_array(ls->_phi, phi);
+    return 0;
+}
+
+// MeshFluidSource::getCells (meshfluidsource.cpp:228-234) with rand() seeded
+// by `seed`, as a flat I*J*K byte grid.
+void ref_sim_inflow_cells(void *hp, int id, unsigned seed, float *jitter_out, uint8_t *cells) {
+    RefSim *h = (RefSim *)hp;
+    MeshFluidSource *src = h->sources[id];
+    vmath::vec3 j = predicted_ray_jitter(seed, h->sim->_dx);
+    jitter_out[0] = j.x; jitter_out[1] = j.y; jitter_out[2] = j.z;
+    std::vector<GridIndex> list;
+    srand(seed);
+    src->getCells(list);
+    int I = h->sim->_isize, J = h->sim->_jsize, K = h->sim->_ksize;
+    std::memset(cells, 0, (size_t)I * J * K);
+    for (const GridIndex &g : list) cells[(size_t)g.i + (size_t)I * ((size_t)g.j + (size_t)J * g.k)] = 1;
+}
 
 // Domain-box solid SDF (nodal) and the derived weight grid
 // (fluidsimulation.cpp:2918-2930, 3646-3730).

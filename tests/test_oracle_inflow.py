@@ -1,0 +1,157 @@
+"""Mesh inflow sources in the CPU oracle, pinned against the compiled
+reference (oracle/_ref) and closed forms.
+
+* Inside tests (MeshUtils::getGridNodesInsideTriangleMesh, meshutils.cpp:
+  347-428) and mesh SDFs (MeshLevelSet::fastCalculateSignedDistanceField,
+  meshlevelset.cpp:572-601): bit-exact against the reference on rotated boxes
+  and cylinders.  The reference jitters its rays with rand(); the harness
+  seeds rand() and reports the jitter it draws, which the oracle then uses.
+  Other jitters change the result on these meshes, so the check pins the
+  jitter as well.
+* A whole simulation with an inflow source (MeshFluidSource, emission,
+  particle and face velocity constraints; fluidsimulation.cpp:3372-3438,
+  4109-4185, 4397-4806): particles bit-exact against the reference for
+  several frames, with substep emissions 1, 2 and 0, with and without the
+  velocity constraint.  The reference jitters new particles with rand() from
+  several threads; with jitter factor 1e-3 the jitter is exactly 0, which
+  makes the reference deterministic.  The source is an axis-aligned box,
+  whose inside sets do not depend on the ray jitter (checked below).
+* Closed forms: nodes strictly inside an axis-aligned box; the jittered
+  inflow particles stay in their subcell, within +-j of its centre.
+"""
+
+import numpy as np
+import pytest
+
+from flipengine3d_amd import scenes
+from oracle.bindings import _SimHandle
+
+
+def _meshes():
+    out = []
+    for s in range(10):
+        if s % 2 == 0:
+            out.append(scenes.box_mesh((0.25, 0.3, 0.2), (0.7, 0.65, 0.75), scenes.rotation_matrix(s)))
+        else:
+            rng = np.random.default_rng(s)
+            out.append(scenes.cylinder_mesh(0.2 + 0.05 * rng.random(), 0.75, (0.5 + 0.05 * rng.random(), 0.45),
+                                            0.2 + 0.05 * rng.random(), segments=12 + s, phase=rng.random()))
+    return out
+
+
+MESHES = _meshes()
+
+
+@pytest.mark.parametrize("m", range(len(MESHES)))
+def test_nodes_inside_and_mesh_sdf_against_reference(oracle, reference, m):
+    v, t = MESHES[m]
+    dx = 1.0 / 24
+    nodes_ref, jit = reference.mesh_nodes_inside(v, t, dx, (25, 25, 25), 1000 + m)
+    nodes = oracle.mesh_nodes_inside(v, t, dx, (25, 25, 25), jit)
+    assert nodes is not None and int(nodes_ref.sum()) > 500
+    assert np.array_equal(nodes, nodes_ref)
+    phi_ref, jit2 = reference.mesh_sdf(v, t, 24, 24, 24, dx, 2, 2000 + m)
+    phi = oracle.mesh_sdf(v, t, 24, 24, 24, dx, 2, jit2)
+    assert phi.tobytes() == phi_ref.tobytes()
+
+
+def test_inside_test_depends_on_the_ray_jitter(oracle, reference):
+    """The reference's jitter is what the oracle must reproduce: a zero or
+    permuted jitter gives other inside sets on these meshes."""
+    dx = 1.0 / 24
+    wrong = 0
+    for m, (v, t) in enumerate(MESHES):
+        nodes_ref, jit = reference.mesh_nodes_inside(v, t, dx, (25, 25, 25), 1000 + m)
+        for j in (np.zeros(3, np.float32), jit[::-1].copy(), -jit):
+            wrong += int((oracle.mesh_nodes_inside(v, t, dx, (25, 25, 25), j) != nodes_ref).sum())
+    assert wrong > 0
+
+
+def test_nodes_inside_axis_box_closed_form(oracle):
+    """Nodes strictly inside [lo, hi] are inside, the others outside."""
+    dx = 1.0 / 20
+    lo, hi = np.array([0.213, 0.317, 0.151]), np.array([0.781, 0.643, 0.577])
+    v, t = scenes.box_mesh(lo, hi)
+    for j in ([0, 0, 0], [4e-5, -3e-5, 2e-5], [-4.9e-5, 4.9e-5, -4.9e-5]):
+        nodes = oracle.mesh_nodes_inside(v, t, dx, (21, 21, 21), np.float32(j))
+        k, jj, i = np.meshgrid(*(np.arange(21) * dx,) * 3, indexing="ij")
+        want = ((i > lo[0]) & (i < hi[0]) & (jj > lo[1]) & (jj < hi[1]) & (k > lo[2]) & (k < hi[2]))
+        assert np.array_equal(nodes.astype(bool), want)
+
+
+I = J = K = 24
+DX = 1.0 / 24
+BOX = scenes.box_mesh((0.31, 0.55, 0.36), (0.47, 0.71, 0.58))
+
+
+def test_box_source_is_jitter_free(oracle):
+    """The box source used below has the same SDF and cells for every ray
+    jitter the reference can draw (|j| <= 1e-3 dx per axis)."""
+    v, t = BOX
+    base = oracle.inflow_build(v, t, (I, J, K), DX, seed=0)
+    for seed in range(1, 6):
+        other = oracle.inflow_build(v, t, (I, J, K), DX, seed=seed)
+        assert other[0] == base[0]
+        assert other[1].tobytes() == base[1].tobytes() and np.array_equal(other[2], base[2])
+
+
+def _inflow_sim(L, emissions, constrain, substeps, pool=True):
+    s = _SimHandle(L, I, J, K, DX)
+    s.add_body_force(0.0, -9.81, 0.0)
+    s.set_substeps(*substeps)
+    if pool:
+        sc = scenes.pool_drop(I, J, K, DX, seed=7, depth=0.2, radius=0.0)
+        s.load_particles(sc.positions, sc.velocities)
+    sid = s.add_inflow(BOX[0], BOX[1], (1.5, 0.0, 0.5))
+    s.set_inflow(sid, True, constrain, emissions)
+    s.set_jitter(1e-3)
+    s.initialize()
+    return s
+
+
+@pytest.mark.parametrize("emissions,constrain,substeps,pool", [(1, True, (1, 6), True), (2, False, (1, 6), True),
+                                                               (0, True, (3, 6), True), (1, True, (1, 6), False)])
+def test_inflow_simulation_against_reference(oracle, reference, emissions, constrain, substeps, pool):
+    """pool=False starts from an empty domain: the source alone triggers the
+    solid SDF (fluidsimulation.cpp:3038-3043) and fills the domain."""
+    a = _inflow_sim(oracle, emissions, constrain, substeps, pool)
+    b = _inflow_sim(reference, emissions, constrain, substeps, pool)
+    n0 = a.num_particles
+    for frame in range(5):
+        a.update(1 / 30)
+        b.update(1 / 30)
+        assert a.substeps() == b.substeps()
+        pa, va = a.particles()
+        pb, vb = b.particles()
+        assert pa.shape == pb.shape, (frame, pa.shape, pb.shape)
+        assert pa.tobytes() == pb.tobytes() and va.tobytes() == vb.tobytes(), frame
+    assert a.num_particles > n0 + 500
+    sa, sb = a.inflow_sdf(0), b.inflow_sdf(0)
+    assert sa[0] == sb[0] and sa[1].tobytes() == sb[1].tobytes()
+    assert np.array_equal(oracle.inflow_cells(a, 0), reference.inflow_cells(b, 0, 5)[0])
+
+
+@pytest.mark.parametrize("surface", [False, True])
+def test_inflow_jitter_stays_in_subcell(oracle, surface):
+    """With jitter factor 1 every new particle lies within +-j of its subcell
+    centre, j = 0.25 (1 - 1e-3) dx; particles deeper than a cell, or all of
+    them with surface jitter, are jittered; one particle per subcell."""
+    s = _SimHandle(oracle, I, J, K, DX)
+    sid = s.add_inflow(BOX[0], BOX[1], (1.0, 0.0, 0.0))
+    s.set_inflow(sid, True, True, 1)
+    s.set_jitter(1.0, surface, 123)
+    s.set_substeps(1, 1)
+    s.initialize()
+    s.update(1e-4)  # no particles yet: the first substep only emits
+    p, v = s.particles()
+    assert p.shape[0] > 100
+    q = 0.5 * DX
+    centre = (np.floor(p.astype(np.float64) / q) + 0.5) * q
+    off = np.abs(p - centre)
+    j = 0.25 * (1.0 - 1e-3) * DX
+    assert (off <= j * (1 + 1e-6) + 1e-7).all()
+    moved = (off.max(axis=1) > 1e-3 * DX).mean()
+    assert moved > 0.95 if surface else 0.02 < moved < 0.5
+    assert np.all(v == np.float32([1.0, 0.0, 0.0]))
+    keys = np.floor(p.astype(np.float64) / q).astype(np.int64)
+    assert np.unique(keys, axis=0).shape[0] == p.shape[0]
