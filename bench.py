@@ -306,11 +306,11 @@ WORKLOADS = {
     # (0.5, 0.7, 0.5), 200 substeps of 1/120 s, FP64 PCG, seed 2.
     "pool_drop_128": ((128, 128, 128), 1 / 128, lambda sc: sc.pool_drop(128, 128, 128, 1 / 128, seed=2), "fp64",
                       1e-9, (1, 1), 1 / 120, 200),
-    # 512x256x256 pool, 25 % deep, mixed PCG at 1e-6 relative, 10 substeps of
-    # 1/480 s, seed 5 (the inflow jet of the full workload is not built).
-    "large_pool_512": ((512, 256, 256), 1 / 256,
-                       lambda sc: sc.pool_drop(512, 256, 256, 1 / 256, seed=5, depth=0.25, radius=0.0), "mixed",
-                       1e-6, (1, 1), 1 / 480, 10),
+    # 512x256x256 pool, 25 % deep, with an inflow jet (cylinder of radius
+    # 0.05 m along +x over x in [0.1, 0.3] m at height 0.6 m, mid depth,
+    # 2 m/s), mixed PCG at 1e-6 relative, 10 substeps of 1/480 s, seed 5.
+    "large_pool_512": ((512, 256, 256), 1 / 256, lambda sc: sc.pool_jet(512, 256, 256, 1 / 256, seed=5),
+                       "mixed", 1e-6, (1, 1), 1 / 480, 10),
 }
 
 
@@ -322,6 +322,9 @@ def run_workload(args):
     (I, J, K), dx, build, precision, tol, substeps, dt, updates = WORKLOADS[args.workload]
     t = time.perf_counter()
     scene = build(scenes)
+    inflow = None
+    if isinstance(scene, tuple):
+        scene, inflow = scene
     log(f"{args.workload}: {scene.num_particles} particles ({time.perf_counter() - t:.1f} s)", args)
     sim = fa.FluidSimulation(I, J, K, dx, precision=fa.PCG_MIXED if precision == "mixed" else fa.PCG_FP64)
     sim.addBodyForce(0.0, -9.81, 0.0)
@@ -329,6 +332,8 @@ def run_workload(args):
     sim.setMaxTimeStepsPerFrame(substeps[1])
     sim.setPressureSolverTolerance(tol)
     sim.loadMarkerParticleData(scene.positions, scene.velocities)
+    if inflow is not None:
+        sim.addMeshFluidSource(inflow.vertices, inflow.triangles, inflow.velocity)
     sim.initialize()
     lat, work, nsub, iters, failures = [], 0, 0, 0, 0
     n = sim.getNumMarkerParticles()
@@ -362,7 +367,10 @@ def run_workload(args):
         "data": "synthetic",
         "config": {"workload": args.workload, "grid": [I, J, K], "dx": dx, "particles": scene.num_particles,
                    "precision": precision, "tolerance": tol, "updates": updates, "substeps": nsub, "dt": dt,
-                   "pcg_iterations_last_substep_mean": iters / updates, "failed_pressure_solves": failures},
+                   "pcg_iterations_last_substep_mean": iters / updates, "failed_pressure_solves": failures,
+                   "inflow": None if inflow is None else {"velocity": list(inflow.velocity),
+                                                         "triangles": int(inflow.triangles.shape[0])},
+                   "particles_end": n},
     }
     if args.cpu_baseline and scene.num_particles <= 20_000_000:
         a = argparse.Namespace(**vars(args))

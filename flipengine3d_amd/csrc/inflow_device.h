@@ -11,7 +11,7 @@ namespace flipmi {
 // 4109-4170): a particle whose cell is a source cell and where the source
 // SDF is not positive takes the source velocity.  The reference samples the
 // sub-grid SDF at the particle's grid position without the sub-grid offset
-// (fluidsimulation.cpp:4142); that sampling is kept as is.
+// (fluidsimulation.cpp:4140); that sampling is kept as is.
 __device__ inline bool inflow_holds_particle(const InflowView &v, const GridGeom &g, F3 p) {
     I3 c = pos_to_index(p, g.invdx);
     if (c.i < v.cmin[0] || c.j < v.cmin[1] || c.k < v.cmin[2] || c.i > v.cmax[0] ||
