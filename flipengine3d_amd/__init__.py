@@ -108,7 +108,7 @@ EXPORTED = [
     "flipmi_scene_initialize", "flipmi_scene_update", "flipmi_scene_num_particles",
     "flipmi_scene_get_particles", "flipmi_scene_get_particles_range", "flipmi_scene_set_domain",
     "flipmi_scene_add_inflow_source", "flipmi_scene_set_inflow_source", "flipmi_scene_set_marker_jitter",
-    "flipmi_scene_get_inflow_source",
+    "flipmi_scene_get_inflow_source", "flipmi_scene_add_obstacle", "flipmi_scene_set_obstacle",
     "flipmi_scene_get_grid", "flipmi_scene_stage_times",
     "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
     "flipmi_liquid_sdf_postprocess", "flipmi_p2g",
@@ -148,6 +148,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_set_inflow_source.argtypes = [_vp, _c_int, _c_int, _c_int, _c_int]
     L.flipmi_scene_set_marker_jitter.argtypes = [_vp, _c_double, _c_int, ctypes.c_uint64]
     L.flipmi_scene_get_inflow_source.argtypes = [_vp, _c_int, _vp, _vp, _vp]
+    L.flipmi_scene_add_obstacle.argtypes = [_vp, _vp, _c_int, _vp, _c_int, P(_c_int)]
+    L.flipmi_scene_set_obstacle.argtypes = [_vp, _c_int, _c_int]
     L.flipmi_scene_get_grid.argtypes = [_vp, _c_int, _vp]
     L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
     L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
@@ -451,6 +453,24 @@ class FluidSimulation:
         self.ctx.check(self.L.flipmi_scene_get_particles_range(self.h, start, end, _ptr(pos) if m else None,
                                                                _ptr(vel) if m else None))
         return pos, vel
+
+    # ---- static mesh obstacles ----------------------------------------------
+    def addMeshObstacle(self, vertices: np.ndarray, triangles: np.ndarray) -> int:
+        """MeshObject with a static mesh (updateMeshStatic) added with
+        addMeshObstacle (meshobject.cpp:50-59, fluidsimulation.cpp:1994-2008);
+        grid coordinates.  Returns the obstacle's index."""
+        self._ensure()
+        v = np.ascontiguousarray(vertices, np.float32)
+        t = np.ascontiguousarray(triangles, np.int32)
+        oid = _c_int(-1)
+        self.ctx.check(self.L.flipmi_scene_add_obstacle(self.h, _ptr(v), v.shape[0], _ptr(t), t.shape[0],
+                                                        ctypes.byref(oid)))
+        return oid.value
+
+    def setMeshObstacle(self, obstacle_id: int, enabled: bool = True):
+        """MeshObject::enable()/disable(); the solid is rebuilt at the next substep."""
+        self._ensure()
+        self.ctx.check(self.L.flipmi_scene_set_obstacle(self.h, obstacle_id, int(enabled)))
 
     # ---- mesh inflow sources ------------------------------------------------
     def addMeshFluidSource(self, vertices: np.ndarray, triangles: np.ndarray, velocity) -> int:

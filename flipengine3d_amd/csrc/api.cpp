@@ -188,6 +188,28 @@ int flipmi_scene_set_domain(flipmi_scene *scene, double ox, double oy, double oz
     return FLIPMI_OK;
 }
 
+int flipmi_scene_add_obstacle(flipmi_scene *scene, const float *verts, int nv, const int *tris, int nt,
+                              int *obstacle_id) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    return guarded(s.ctx, [&] {
+        int id = s.add_obstacle(verts, nv, tris, nt);
+        if (obstacle_id) *obstacle_id = id;
+    });
+}
+
+int flipmi_scene_set_obstacle(flipmi_scene *scene, int id, int enabled) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    if (id < 0 || id >= (int)s.obstacles.size()) return fail(s.ctx, FLIPMI_ERR_INVALID_ARG, "unknown obstacle");
+    bool on = enabled != 0;
+    if (s.obstacles[(size_t)id]->enabled != on) {
+        s.obstacles[(size_t)id]->enabled = on;
+        s.solid_ready = false;
+    }
+    return FLIPMI_OK;
+}
+
 int flipmi_scene_add_inflow_source(flipmi_scene *scene, const float *verts, int nv, const int *tris, int nt,
                                    const double velocity[3], int *source_id) {
     if (!scene || !velocity) return FLIPMI_ERR_INVALID_ARG;

@@ -191,6 +191,30 @@ hipError_t launch_source_cell_list(const int *flags, const int *pos, size_t n, i
     return hipGetLastError();
 }
 
+// MeshLevelSet::calculateUnion (meshlevelset.cpp:603-638, 1758-1790) of a
+// sub-grid level set at node offset go into a nodal field: the smaller value
+// wins.
+__global__ void k_union_min(const float *__restrict__ phi, Dims S, int goi, int goj, int gok, Dims N,
+                            float *dst) {
+    size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= S.count()) return;
+    int i = (int)(tid % S.ni), j = (int)((tid / S.ni) % S.nj), k = (int)(tid / ((size_t)S.ni * S.nj));
+    int a = i + goi, b = j + goj, c = k + gok;
+    if (!N.in(a, b, c)) return;
+    size_t id = N.at(a, b, c);
+    float v = phi[tid];
+    if (v < dst[id]) dst[id] = v;
+}
+
+hipError_t launch_union_min(const float *phi, int ni, int nj, int nk, const int go[3], int Ni, int Nj,
+                            int Nk, float *dst, hipStream_t s) {
+    Dims S{ni, nj, nk};
+    size_t n = S.count();
+    if (n == 0) return hipSuccess;
+    k_union_min<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(phi, S, go[0], go[1], go[2], Dims{Ni, Nj, Nk}, dst);
+    return hipGetLastError();
+}
+
 // ---- per substep ----------------------------------------------------------
 
 // ParticleMaskGrid::addParticle: the bit of the particle's subcell parity in

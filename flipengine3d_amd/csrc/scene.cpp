@@ -559,6 +559,7 @@ void Scene::ensure_solid() {
     if (solid_ready || (n == 0 && sources.empty())) return;
     BoxMesh mesh = make_domain_box(desc.isize, desc.jsize, desc.ksize, desc.dx);
     check(launch_domain_solid(g, mesh, solid.p, stream), "domain solid");
+    union_obstacles();
     weights_ready = false;
     // _updateNearSolidGrid (fluidsimulation.cpp:3094-3126) depends only on
     // the static solid, so it is built once as well.
@@ -830,6 +831,181 @@ RayInput ray_input(const std::vector<float> &verts, const std::vector<int> &tris
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Static mesh obstacles
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kSolidBand = 3;  // _solidLevelSetExactBand (fluidsimulation.h:1580)
+
+// Level-set sub-grid of an obstacle: MeshObject::_getMeshIslandLevelSet for
+// the whole mesh (meshobject.cpp:519-553), the mesh box widened by the band
+// and clamped to the domain.  Empty when a side is not positive.
+struct ObstacleGrid {
+    int go[3] = {0, 0, 0};
+    int ni = 0, nj = 0, nk = 0;  // node dims
+    float off[3] = {0, 0, 0};
+    bool empty = true;
+};
+ObstacleGrid obstacle_grid(const std::vector<float> &verts, int I, int J, int K, double dx) {
+    ObstacleGrid og;
+    HostBox bb = points_box(verts.data(), verts.size() / 3);
+    const int top[3] = {I - 1, J - 1, K - 1};
+    int gmax[3];
+    for (int a = 0; a < 3; a++) {
+        og.go[a] = (int)std::fmax(floor_index(bb.pos[a], dx) - kSolidBand, 0);
+        gmax[a] = (int)std::fmin(floor_index(bb.pos[a] + (float)bb.ext[a], dx) + kSolidBand + 1, top[a]);
+        og.off[a] = index_position(og.go[a], dx);
+    }
+    const int w = gmax[0] - og.go[0], h = gmax[1] - og.go[1], d = gmax[2] - og.go[2];
+    if (w <= 0 || h <= 0 || d <= 0) return og;
+    og.empty = false;
+    og.ni = w + 1; og.nj = h + 1; og.nk = d + 1;
+    return og;
+}
+
+// Islands: components of vertices joined by triangle edges
+// (MeshUtils::_splitIntoMeshIslands, meshutils.cpp:455-563).
+int count_islands(size_t nv, const std::vector<int> &tris) {
+    std::vector<int> parent(nv);
+    for (size_t q = 0; q < nv; q++) parent[q] = (int)q;
+    auto find = [&](int a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+    };
+    for (size_t t = 0; t + 2 < tris.size(); t += 3) {
+        int a = find(tris[t]);
+        parent[find(tris[t + 1])] = a;
+        parent[find(tris[t + 2])] = find(a);
+    }
+    int n = 0;
+    for (size_t q = 0; q < nv; q++) n += find((int)q) == (int)q ? 1 : 0;
+    return n;
+}
+
+}  // namespace
+
+int Scene::add_obstacle(const float *verts, int nv, const int *tris, int nt) {
+    if (!verts || !tris || nv < 3 || nt < 1)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "obstacle mesh needs at least 3 vertices and 1 triangle"};
+    for (int q = 0; q < 3 * nt; q++)
+        if (tris[q] < 0 || tris[q] >= nv)
+            throw StatusError{FLIPMI_ERR_INVALID_ARG, "obstacle mesh triangle index out of range"};
+    auto ob = std::make_unique<ObstacleMesh>();
+    // TriangleMesh::removeExtraneousVertices: keep used vertices, in order.
+    std::vector<int> map((size_t)nv, -1);
+    for (int q = 0; q < 3 * nt; q++) map[(size_t)tris[q]] = 0;
+    for (int q = 0; q < nv; q++) {
+        if (map[(size_t)q] != 0) continue;
+        for (int a = 0; a < 3; a++) {
+            if (!std::isfinite(verts[3 * q + a]))
+                throw StatusError{FLIPMI_ERR_INVALID_ARG, "obstacle mesh vertex is not finite"};
+            ob->verts.push_back(verts[3 * q + a]);
+        }
+        map[(size_t)q] = (int)(ob->verts.size() / 3) - 1;
+    }
+    for (int q = 0; q < 3 * nt; q++) ob->tris.push_back(map[(size_t)tris[q]]);
+    // Meshes of 25 or more islands take the reference's per-island path
+    // (meshobject.cpp:254-262), which is not built.
+    if (count_islands(ob->verts.size() / 3, ob->tris) >= 25)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "obstacle meshes of 25 or more islands are not supported"};
+    // The inside test of the obstacle's level set needs the mesh inside the
+    // sub-grid's box (the reference splits other meshes, meshutils.cpp:392-428).
+    const double dx = g.dx;
+    ObstacleGrid og = obstacle_grid(ob->verts, g.I, g.J, g.K, dx);
+    if (!og.empty) {
+        const double box[3] = {(og.ni - 1) * dx, (og.nj - 1) * dx, (og.nk - 1) * dx};
+        for (size_t q = 0; q < ob->verts.size() / 3; q++) {
+            float p[3];
+            for (int a = 0; a < 3; a++) p[a] = ob->verts[3 * q + a] + -og.off[a];
+            if (!inside_origin_box(p, box))
+                throw StatusError{FLIPMI_ERR_INVALID_ARG,
+                                  "obstacle mesh must lie at least 4 cells inside the domain grid"};
+        }
+    }
+    obstacles.push_back(std::move(ob));
+    solid_ready = false;
+    return (int)obstacles.size() - 1;
+}
+
+// FluidSimulation::_addStaticObjectsToSDF (fluidsimulation.cpp:2932-2985):
+// each enabled obstacle's level set on its sub-grid (exact band 3, signs by
+// the ray test), unioned into the solid SDF in order.
+void Scene::union_obstacles() {
+    const double dx = g.dx;
+    auto need = [&](hipError_t e) {
+        if (e != hipSuccess) throw StatusError{FLIPMI_ERR_OOM, "obstacle allocation: out of device memory"};
+    };
+    for (size_t q = 0; q < obstacles.size(); q++) {
+        const ObstacleMesh &ob = *obstacles[q];
+        if (!ob.enabled) continue;
+        ObstacleGrid og = obstacle_grid(ob.verts, g.I, g.J, g.K, dx);
+        if (og.empty) continue;
+        const size_t nt = ob.tris.size() / 3;
+        // Triangle bands (meshlevelset.cpp:878-915): box from the untranslated
+        // vertices, then moved by the grid offset, like the vertices.
+        std::vector<BoxTri> band;
+        for (size_t t = 0; t < nt; t++) {
+            float v[3][3];
+            for (int c = 0; c < 3; c++)
+                for (int a = 0; a < 3; a++) v[c][a] = ob.verts[3 * (size_t)ob.tris[3 * t + c] + a];
+            HostBox tb = points_box(&v[0][0], 3);
+            BoxTri T;
+            bool keep = true;
+            const int size[3] = {og.ni - 1, og.nj - 1, og.nk - 1};
+            for (int a = 0; a < 3; a++) {
+                float lo = tb.pos[a] - og.off[a];
+                float hi = lo + (float)tb.ext[a];
+                T.gmin[a] = floor_index(lo, dx) - kSolidBand;
+                T.gmax[a] = floor_index(hi, dx) + kSolidBand + 1;
+                if (T.gmax[a] < 0 || T.gmin[a] >= size[a] + 1) keep = false;
+            }
+            if (!keep) continue;
+            for (int a = 0; a < 3; a++) {
+                T.gmin[a] = std::max(T.gmin[a], 0);
+                T.gmax[a] = std::min(T.gmax[a], size[a]);
+            }
+            for (int c = 0; c < 3; c++)
+                for (int a = 0; a < 3; a++) T.v[c][a] = v[c][a] - og.off[a];
+            band.push_back(T);
+        }
+        const size_t nn = (size_t)og.ni * og.nj * og.nk;
+        DBuf<float> phi;
+        need(phi.alloc(nn));
+        const float upper = (float)((double)(og.ni + og.nj + og.nk) * dx);  // meshlevelset.cpp:769-771
+        DBuf<BoxTri> dband;
+        if (band.empty()) {
+            check(launch_fill_f32(phi.p, nn, upper, stream), "fill");
+        } else {
+            need(dband.alloc(band.size()));
+            check(hipMemcpyAsync(dband.p, band.data(), dband.bytes(), hipMemcpyHostToDevice, stream), "upload");
+            check(launch_mesh_exact_band(dband.p, (int)band.size(), og.ni, og.nj, og.nk, dx, upper, phi.p, stream),
+                  "obstacle band");
+        }
+        std::vector<float> local(ob.verts.size());
+        for (size_t q2 = 0; q2 < local.size(); q2++) local[q2] = ob.verts[q2] + -og.off[q2 % 3];
+        RayInput r = ray_input(local, ob.tris, dx);
+        DBuf<RayTri> dray;
+        DBuf<int> overflow;
+        need(dray.alloc(nt));
+        need(overflow.alloc(1));
+        check(launch_fill_i32(overflow.p, 1, 0, stream), "fill");
+        check(hipMemcpyAsync(dray.p, r.tris.data(), dray.bytes(), hipMemcpyHostToDevice, stream), "upload");
+        float jit[3];
+        ray_jitter(jitter_seed, 100 + (int)q, 0, dx, jit);
+        check(launch_ray_inside(dray.p, (int)nt, r.ci, r.cj, r.ck, dx, jit, r.go, og.ni, og.nj, og.nk, 1, nullptr,
+                                phi.p, overflow.p, stream),
+              "obstacle signs");
+        check(launch_union_min(phi.p, og.ni, og.nj, og.nk, og.go, g.I + 1, g.J + 1, g.K + 1, solid.p, stream),
+              "obstacle union");
+        int ov = 0;
+        check(hipMemcpyAsync(&ov, overflow.p, sizeof(int), hipMemcpyDeviceToHost, stream), "overflow");
+        sync();
+        if (ov != 0)
+            throw StatusError{FLIPMI_ERR_INVALID_ARG, "obstacle mesh: a ray column crosses more than 64 triangles"};
+    }
+}
 
 int Scene::add_inflow(const float *verts, int nv, const int *tris, int nt, const double vel[3]) {
     if (!verts || !tris || nv < 3 || nt < 1)

@@ -79,6 +79,16 @@ struct InflowSource {
     int cmin[3] = {0, 0, 0}, cmax[3] = {-1, -1, -1};
 };
 
+// One static mesh obstacle: MeshObject with updateMeshStatic, added with
+// FluidSimulation::addMeshObstacle (meshobject.cpp:50-59,
+// fluidsimulation.cpp:1994-2008), in grid coordinates.  Only the vertices
+// some triangle uses are kept (TriangleMesh::removeExtraneousVertices).
+struct ObstacleMesh {
+    std::vector<float> verts;
+    std::vector<int> tris;
+    bool enabled = true;
+};
+
 struct Scene {
     Ctx *ctx = nullptr;
     flipmi_scene_desc desc{};
@@ -125,6 +135,7 @@ struct Scene {
     // (setMarkerParticleJitterFactor, enable/disableJitterSurfaceMarkerParticles,
     // fluidsimulation.cpp:185-210)
     std::vector<std::unique_ptr<InflowSource>> sources;
+    std::vector<std::unique_ptr<ObstacleMesh>> obstacles;
     double jitter_factor = 1.0;
     bool jitter_surface = false;
     uint64_t jitter_seed = 0;
@@ -184,6 +195,10 @@ struct Scene {
     InflowViews inflow_views(bool emitting) const;
     bool any_inflow_constraint() const;
     void emit_inflow();
+    // Static mesh obstacles: add (checks the mesh) and their union into the
+    // solid SDF (FluidSimulation::_addStaticObjectsToSDF).
+    int add_obstacle(const float *verts, int nv, const int *tris, int nt);
+    void union_obstacles();
     // Grows every particle-sized buffer to hold at least `need` particles.
     void grow(int64_t need);
     void begin_stage(int stage, hipEvent_t *a);

@@ -170,6 +170,21 @@ int flipmi_scene_get_particles_range(flipmi_scene *scene, int64_t start, int64_t
  * for every read-back. */
 int flipmi_scene_set_domain(flipmi_scene *scene, double offset_x, double offset_y, double offset_z,
                             double scale);
+/* Static mesh obstacle: MeshObject(isize, jsize, ksize, dx) +
+ * updateMeshStatic(mesh) added with FluidSimulation::addMeshObstacle
+ * (meshobject.cpp:50-59, fluidsimulation.cpp:1994-2008).  Its level set
+ * (exact band 3, meshobject.cpp:239-262, 519-553) is unioned into the solid
+ * SDF (fluidsimulation.cpp:2932-2985), which is rebuilt at the next substep.
+ * Grid coordinates.  The mesh must lie at least 4 cells inside the domain
+ * grid (the reference splits meshes that leave its level set's grid,
+ * meshutils.cpp:392-428), have fewer than 25 islands (the reference's
+ * per-island path, meshobject.cpp:254-262) and no ray column may cross it
+ * more than 64 times; otherwise FLIPMI_ERR_INVALID_ARG. */
+int flipmi_scene_add_obstacle(flipmi_scene *scene, const float *vertices_xyz, int num_vertices,
+                              const int *triangles, int num_triangles, int *obstacle_id);
+/* MeshObject::enable()/disable() of an obstacle: the solid SDF is rebuilt at
+ * the next substep (fluidsimulation.cpp:3028-3036). */
+int flipmi_scene_set_obstacle(flipmi_scene *scene, int obstacle_id, int enabled);
 /* Mesh inflow source: MeshFluidSource(isize, jsize, ksize, dx) +
  * updateMeshStatic(mesh) + setInflow() + setVelocity(v)
  * (meshfluidsource.cpp:32-44, 74-76, 114-116), added with

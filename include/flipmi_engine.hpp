@@ -622,6 +622,43 @@ struct TriangleMesh {
     std::vector<Triangle> triangles;
 };
 
+// meshobject.h: a static mesh obstacle (updateMeshStatic).  enable() and
+// disable() after it joins an initialized simulation rebuild the solid at the
+// next substep.
+class MeshObject {
+public:
+    MeshObject(int isize, int jsize, int ksize, double dx) { (void)isize; (void)jsize; (void)ksize; (void)dx; }
+    void updateMeshStatic(TriangleMesh mesh) {
+        if (_scene) throw std::runtime_error("flipmi: the obstacle mesh is fixed once it is on the device");
+        _mesh = mesh;
+    }
+    TriangleMesh getMesh() { return _mesh; }
+    void enable() { _enabled = true; _push(); }
+    void disable() { _enabled = false; _push(); }
+    bool isEnabled() { return _enabled; }
+
+private:
+    friend class FluidSimulation;
+    void _attach(flipmi_scene *scene) {
+        std::vector<float> v;
+        std::vector<int> t;
+        for (const vmath::vec3 &p : _mesh.vertices) { v.push_back(p.x); v.push_back(p.y); v.push_back(p.z); }
+        for (const Triangle &q : _mesh.triangles) t.insert(t.end(), q.tri, q.tri + 3);
+        detail::check(flipmi_scene_add_obstacle(scene, v.data(), (int)_mesh.vertices.size(), t.data(),
+                                                (int)_mesh.triangles.size(), &_id),
+                      "FluidSimulation::addMeshObstacle");
+        _scene = scene;
+        _push();
+    }
+    void _push() {
+        if (_scene) detail::check(flipmi_scene_set_obstacle(_scene, _id, _enabled), "MeshObject");
+    }
+    TriangleMesh _mesh;
+    bool _enabled = true;
+    flipmi_scene *_scene = nullptr;
+    int _id = -1;
+};
+
 // meshfluidsource.h: a mesh inflow source with a static mesh
 // (updateMeshStatic).  Settings made after the source joins an initialized
 // simulation reach the device at once.
@@ -707,11 +744,19 @@ public:
         _loadVel.clear();
         detail::check(flipmi_scene_set_marker_jitter(_scene, _jitterFactor, _jitterSurface, _jitterSeed),
                       "FluidSimulation::initialize");
+        for (MeshObject *ob : _obstacles) ob->_attach(_scene);
         for (MeshFluidSource *src : _sources) src->_attach(_scene);
         detail::check(flipmi_scene_initialize(_scene), "FluidSimulation::initialize");
         _initialized = true;
     }
 
+    // fluidsimulation.cpp:1994-2008; the obstacle must outlive the simulation.
+    void addMeshObstacle(MeshObject *obstacle) {
+        for (MeshObject *o : _obstacles)
+            if (o == obstacle) throw std::runtime_error("Error: mesh obstacle has already been added.");
+        _obstacles.push_back(obstacle);
+        if (_scene) obstacle->_attach(_scene);
+    }
     // fluidsimulation.cpp:1953-1965; the source must outlive the simulation.
     void addMeshFluidSource(MeshFluidSource *source) {
         for (MeshFluidSource *s : _sources)
@@ -958,6 +1003,7 @@ private:
     double _lastPcgResidual = 0.0;
     bool _lastPressureOk = true;
     std::vector<MeshFluidSource *> _sources;
+    std::vector<MeshObject *> _obstacles;
     double _jitterFactor = 1.0;  // fluidsimulation.h:1553
     int _jitterSurface = 0;
     uint64_t _jitterSeed = 0;

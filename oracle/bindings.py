@@ -235,6 +235,20 @@ class _SimHandle:
     def set_inflow(self, sid, enabled=True, constrain=True, emissions=1):
         self._f("sim_set_inflow")(self.h, sid, int(enabled), int(constrain), int(emissions))
 
+    def add_obstacle(self, verts, tris, jitter=None):
+        """MeshObject with a static mesh added with addMeshObstacle; the oracle
+        also takes the ray jitter of its inside test (None: keyed draw)."""
+        verts = np.ascontiguousarray(verts, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        if self.L.prefix == "orc_":
+            jit = None if jitter is None else np.ascontiguousarray(jitter, np.float32)
+            return self._f("sim_add_obstacle")(self.h, _ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0],
+                                               _ptr(jit))
+        return self._f("sim_add_obstacle")(self.h, _ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0])
+
+    def set_obstacle(self, oid, enabled):
+        self._f("sim_set_obstacle")(self.h, oid, int(enabled))
+
     def set_jitter(self, factor, surface=False, seed=0):
         """setMarkerParticleJitterFactor + jitter-surface switch (the oracle
         also takes the seed of its jitter draws)."""
@@ -293,6 +307,8 @@ class Oracle(_Base):
         L.orc_mesh_sdf.argtypes = [_vp, _c_int, _vp, _c_int] + [_c_int] * 3 + [_c_double, _c_int, _vp, _vp]
         L.orc_sim_inflow_cells.argtypes = [_vp, _c_int, _vp]
         L.orc_ray_jitter.argtypes = [ctypes.c_uint64, _c_int, _c_int, _c_double, _vp]
+        L.orc_sim_add_obstacle.argtypes = [_vp, _vp, _c_int, _vp, _c_int, _vp]
+        L.orc_sim_set_obstacle.argtypes = [_vp, _c_int, _c_int]
         L.orc_inflow_build.argtypes = ([_vp, _c_int, _vp, _c_int] + [_c_int] * 3
                                        + [_c_double, ctypes.c_uint64, _c_int, _vp, _vp, _vp])
 
@@ -369,6 +385,10 @@ class Reference(_Base):
         L.ref_mesh_sdf.argtypes = ([_vp, _c_int, _vp, _c_int] + [_c_int] * 3
                                    + [_c_double, _c_int, ctypes.c_uint, _vp, _vp])
         L.ref_sim_inflow_cells.argtypes = [_vp, _c_int, ctypes.c_uint, _vp, _vp]
+        L.ref_sim_add_obstacle.argtypes = [_vp, _vp, _c_int, _vp, _c_int]
+        L.ref_sim_set_obstacle.argtypes = [_vp, _c_int, _c_int]
+        L.ref_predict_ray_jitters.argtypes = [ctypes.c_uint, _c_double, _c_int, _vp]
+        L.ref_seed_rand.argtypes = [ctypes.c_uint]
         self.threads = threads
 
     def sim_create(self, isize, jsize, ksize, dx):
@@ -377,6 +397,14 @@ class Reference(_Base):
     def set_threads(self, n):
         self.threads = n
         self.lib.ref_set_thread_count(n)
+
+    def predict_ray_jitters(self, seed, dx, count):
+        """The ray jitters of the next `count` inside tests after srand(seed);
+        reseeds rand() with `seed` afterwards."""
+        out = np.zeros((count, 3), np.float32)
+        self.lib.ref_predict_ray_jitters(seed, dx, count, _ptr(out))
+        self.lib.ref_seed_rand(seed)
+        return out
 
     def mesh_nodes_inside(self, verts, tris, dx, dims, seed):
         """(nodes, ray jitter) of getGridNodesInsideTriangleMesh with rand() seeded."""

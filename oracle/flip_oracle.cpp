@@ -1472,6 +1472,127 @@ struct MaskGrid {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Static mesh obstacles: MeshObject::getMeshLevelSet (meshobject.cpp:239-262)
+// for meshes of fewer than 25 islands, where the whole mesh is one level set
+// on the sub-grid around its box (_getMeshIslandLevelSet, meshobject.cpp:
+// 519-553), unioned into the solid SDF (MeshLevelSet::calculateUnion,
+// meshlevelset.cpp:603-638, 1758-1790).
+// ---------------------------------------------------------------------------
+struct Obstacle {
+    TriMesh mesh;          // referenced vertices only (removeExtraneousVertices)
+    bool enabled = true;
+    bool hasJitter = false;
+    V3 jitter;             // ray jitter of its inside test, if given
+};
+
+// TriangleMesh::removeExtraneousVertices (trianglemesh.cpp): keeps the
+// vertices some triangle uses, in their order.
+static TriMesh referenced_only(const TriMesh &m) {
+    std::vector<int> map(m.v.size(), -1);
+    for (int idx : m.t) map[idx] = 0;
+    TriMesh out;
+    for (size_t q = 0; q < m.v.size(); q++)
+        if (map[q] == 0) { map[q] = (int)out.v.size(); out.v.push_back(m.v[q]); }
+    for (int idx : m.t) out.t.push_back(map[idx]);
+    return out;
+}
+
+// Islands of a mesh: components of vertices joined by triangle edges
+// (MeshUtils::_splitIntoMeshIslands, meshutils.cpp:455-563).
+static int island_count(const TriMesh &m) {
+    std::vector<int> parent(m.v.size());
+    for (size_t q = 0; q < parent.size(); q++) parent[q] = (int)q;
+    auto find = [&](int a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+    };
+    for (size_t t = 0; t < m.ntri(); t++) {
+        int a = find(m.t[3 * t]), b = find(m.t[3 * t + 1]), c = find(m.t[3 * t + 2]);
+        parent[b] = a;
+        parent[find(c)] = find(a);
+    }
+    int n = 0;
+    for (size_t q = 0; q < parent.size(); q++) n += find((int)q) == (int)q;
+    return n;
+}
+
+// MeshLevelSet::_initializeTriangleData with a grid offset (meshlevelset.cpp:
+// 878-915): the triangle's box is taken from its untranslated vertices and
+// then moved by the offset; the vertices are moved likewise.
+static bool tri_band_offset(const V3 orig[3], V3 off, int isize, int jsize, int ksize, double dx,
+                            int band, Tri &T) {
+    Tri tmp;
+    for (int c = 0; c < 3; c++) tmp.v[c] = orig[c];
+    double mnx = tmp.v[0].x, mny = tmp.v[0].y, mnz = tmp.v[0].z, mxx = mnx, mxy = mny, mxz = mnz;
+    for (int c = 0; c < 3; c++) {
+        mnx = std::fmin((double)tmp.v[c].x, mnx); mny = std::fmin((double)tmp.v[c].y, mny);
+        mnz = std::fmin((double)tmp.v[c].z, mnz); mxx = std::fmax((double)tmp.v[c].x, mxx);
+        mxy = std::fmax((double)tmp.v[c].y, mxy); mxz = std::fmax((double)tmp.v[c].z, mxz);
+    }
+    Box tb;
+    tb.pos = V3((float)mnx, (float)mny, (float)mnz) - off;
+    tb.w = mxx - mnx + 1e-9; tb.h = mxy - mny + 1e-9; tb.d = mxz - mnz + 1e-9;
+    V3 pmax = tb.maxp();
+    Idx gmin = pos_to_index(tb.pos, dx), gmax = pos_to_index(pmax, dx);
+    gmin.i -= band; gmin.j -= band; gmin.k -= band;
+    gmax.i += band + 1; gmax.j += band + 1; gmax.k += band + 1;
+    if (gmax.i < 0 || gmin.i >= isize + 1 || gmax.j < 0 || gmin.j >= jsize + 1 || gmax.k < 0 ||
+        gmin.k >= ksize + 1)
+        return false;
+    gmin.i = std::max(gmin.i, 0); gmin.j = std::max(gmin.j, 0); gmin.k = std::max(gmin.k, 0);
+    gmax.i = std::min(gmax.i, isize); gmax.j = std::min(gmax.j, jsize); gmax.k = std::min(gmax.k, ksize);
+    for (int c = 0; c < 3; c++) T.v[c] = orig[c] - off;
+    T.gmin = gmin; T.gmax = gmax;
+    return true;
+}
+
+// _getMeshIslandLevelSet for the whole mesh (meshobject.cpp:519-553): the
+// sub-grid is the mesh box widened by the band, clamped to the domain; gmin
+// receives its offset.  Returns 0 when the sub-grid is empty (nothing to
+// union), -1 when the mesh leaves the sub-grid's box, 1 otherwise.
+static int obstacle_sdf(const TriMesh &m, int I, int J, int K, double dx, int band, V3 jitter,
+                        Idx &gmin, GridF &phi) {
+    Box bb = points_box(m.v);
+    gmin = pos_to_index(bb.pos, dx);
+    Idx gmax = pos_to_index(bb.maxp(), dx);
+    gmin.i = (int)std::fmax(gmin.i - band, 0); gmin.j = (int)std::fmax(gmin.j - band, 0);
+    gmin.k = (int)std::fmax(gmin.k - band, 0);
+    gmax.i = (int)std::fmin(gmax.i + band + 1, I - 1); gmax.j = (int)std::fmin(gmax.j + band + 1, J - 1);
+    gmax.k = (int)std::fmin(gmax.k + band + 1, K - 1);
+    int gw = gmax.i - gmin.i, gh = gmax.j - gmin.j, gd = gmax.k - gmin.k;
+    if (gw <= 0 || gh <= 0 || gd <= 0) return 0;
+    V3 off = index_to_pos(gmin.i, gmin.j, gmin.k, dx);  // MeshLevelSet::setGridOffset
+    phi = GridF(gw + 1, gh + 1, gd + 1, 0.0f);
+    float upper = (float)((phi.ni + phi.nj + phi.nk) * dx);
+    std::vector<Tri> tris;
+    for (size_t t = 0; t < m.ntri(); t++) {
+        V3 v[3] = {m.v[m.t[3 * t]], m.v[m.t[3 * t + 1]], m.v[m.t[3 * t + 2]]};
+        Tri T;
+        if (tri_band_offset(v, off, gw, gh, gd, dx, band, T)) tris.push_back(T);
+    }
+    exact_band(tris, phi, dx, upper);
+    TriMesh local = m;
+    V3 neg(-off.x, -off.y, -off.z);
+    for (V3 &p : local.v) p = p + neg;
+    GridB nodes;
+    if (!nodes_inside(local, dx, phi.ni, phi.nj, phi.nk, jitter, nodes)) return -1;
+    for (size_t q = 0; q < phi.a.size(); q++)
+        if (nodes.a[q]) phi.a[q] = -phi.a[q];
+    return 1;
+}
+
+// Union into the solid SDF: the smaller value wins on the sub-grid's nodes.
+static void union_into(GridF &solid, const GridF &phi, Idx go) {
+    for (int k = 0; k < phi.nk; k++)
+        for (int j = 0; j < phi.nj; j++)
+            for (int i = 0; i < phi.ni; i++) {
+                int a = i + go.i, b = j + go.j, c = k + go.k;
+                if (!solid.in(a, b, c)) continue;
+                if (phi(i, j, k) < solid(a, b, c)) solid(a, b, c) = phi(i, j, k);
+            }
+}
+
 struct Sim {
     int I, J, K;
     double dx;
@@ -1508,6 +1629,7 @@ struct Sim {
     bool jitterSurface = false;
     uint64_t seed = 0;
     uint64_t emissions = 0;  // inflow emission passes so far (keys the jitter)
+    std::vector<Obstacle> obstacles;  // FluidSimulation::addMeshObstacle (fluidsimulation.cpp:1994-2008)
 
     V3 body_force() const {  // fluidsimulation.cpp:3363-3370
         V3 bf;
@@ -1805,6 +1927,18 @@ static void step_fluid(Sim &S, double dt) {
     // (fluidsimulation.cpp:3038-3043).
     if (!S.solidUpToDate && (!S.pos.empty() || !S.sources.empty())) {
         domain_solid_phi(S.I, S.J, S.K, S.dx, S.solid);
+        // _addStaticObjectsToSDF (fluidsimulation.cpp:2932-2985): enabled
+        // static obstacles unioned in order, exact band 3 (fluidsimulation.h:1580).
+        for (size_t q = 0; q < S.obstacles.size(); q++) {
+            const Obstacle &ob = S.obstacles[q];
+            if (!ob.enabled) continue;
+            V3 jit = ob.hasJitter ? ob.jitter : ray_jitter(S.seed, 100 + (int)q, 0, S.dx);
+            Idx go;
+            GridF phi;
+            int rc = obstacle_sdf(ob.mesh, S.I, S.J, S.K, S.dx, 3, jit, go, phi);
+            if (rc < 0) throw std::runtime_error("obstacle mesh must lie inside its level set's grid");
+            if (rc > 0) union_into(S.solid, phi, go);
+        }
         S.solidU = GridF(S.I + 1, S.J, S.K, 0.0f);
         S.solidV = GridF(S.I, S.J + 1, S.K, 0.0f);
         S.solidW = GridF(S.I, S.J, S.K + 1, 0.0f);
@@ -2131,6 +2265,29 @@ int orc_inflow_build(const float *verts, int nv, const int *tris, int nt, int I,
 // MeshFluidSource(isize, jsize, ksize, dx) + updateMeshStatic + setInflow +
 // setVelocity, added with FluidSimulation::addMeshFluidSource; returns its
 // index.
+// MeshObject(isize, jsize, ksize, dx) + updateMeshStatic(mesh), added with
+// FluidSimulation::addMeshObstacle (meshobject.cpp:50-59, fluidsimulation.cpp:
+// 1994-2008).  jitter (3 floats) fixes the ray jitter of its inside test;
+// NULL draws it from the keyed generator.  Returns the obstacle's index, or
+// -1 for a mesh of 25 or more islands.
+int orc_sim_add_obstacle(void *h, const float *verts, int nv, const int *tris, int nt,
+                         const float *jitter) {
+    Sim *S = (Sim *)h;
+    Obstacle ob;
+    ob.mesh = referenced_only(make_mesh(verts, nv, tris, nt));
+    if (island_count(ob.mesh) >= 25) return -1;
+    if (jitter) { ob.hasJitter = true; ob.jitter = V3(jitter[0], jitter[1], jitter[2]); }
+    S->obstacles.push_back(ob);
+    S->solidUpToDate = false;
+    return (int)S->obstacles.size() - 1;
+}
+// MeshObject::enable()/disable(): the solid is rebuilt at the next substep.
+void orc_sim_set_obstacle(void *h, int id, int enabled) {
+    Sim *S = (Sim *)h;
+    S->obstacles[id].enabled = enabled != 0;
+    S->solidUpToDate = false;
+}
+
 int orc_sim_add_inflow(void *h, const float *verts, int nv, const int *tris, int nt, double vx,
                        double vy, double vz) {
     Sim *S = (Sim *)h;

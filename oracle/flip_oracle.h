@@ -85,6 +85,9 @@ void orc_sim_set_jitter(void *h, double factor, int surface, uint64_t seed);
 int orc_sim_inflow_sdf(void *h, int id, int *dims, float *phi);
 int orc_sim_inflow_cells(void *h, int id, uint8_t *cells);
 void orc_ray_jitter(uint64_t seed, int id, int which, double dx, float *out);
+int orc_sim_add_obstacle(void *h, const float *verts, int nv, const int *tris, int nt,
+                         const float *jitter);
+void orc_sim_set_obstacle(void *h, int id, int enabled);
 int orc_inflow_build(const float *verts, int nv, const int *tris, int nt, int I, int J, int K,
                      double dx, uint64_t seed, int id, int *dims, float *phi, uint8_t *cells);
 

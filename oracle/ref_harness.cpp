@@ -68,6 +68,7 @@ namespace {
 struct RefSim {
     FluidSimulation *sim = nullptr;
     std::vector<MeshFluidSource *> sources;
+    std::vector<MeshObject *> obstacles;
     std::string last_error;
 };
 
@@ -151,6 +152,7 @@ void ref_sim_destroy(void *hp) {
     if (!h) return;
     delete h->sim;
     for (MeshFluidSource *src : h->sources) delete src;
+    for (MeshObject *ob : h->obstacles) delete ob;
     delete h;
 }
 
@@ -323,6 +325,39 @@ void ref_mesh_sdf(const float *verts, int nv, const int *tris, int nt, int isize
     ls.fastCalculateSignedDistanceField(m, band);
     copy_array(ls._phi, phi);
 }
+
+// MeshObject(isize, jsize, ksize, dx) + updateMeshStatic(mesh), added with
+// FluidSimulation::addMeshObstacle (meshobject.cpp:50-59, fluidsimulation.cpp:
+// 1994-2008).
+int ref_sim_add_obstacle(void *hp, const float *verts, int nv, const int *tris, int nt) {
+    RefSim *h = (RefSim *)hp;
+    FluidSimulation *sim = h->sim;
+    MeshObject *ob = new MeshObject(sim->_isize, sim->_jsize, sim->_ksize, sim->_dx);
+    ob->updateMeshStatic(make_tri_mesh(verts, nv, tris, nt));
+    sim->addMeshObstacle(ob);
+    h->obstacles.push_back(ob);
+    return (int)h->obstacles.size() - 1;
+}
+
+void ref_sim_set_obstacle(void *hp, int id, int enabled) {
+    MeshObject *ob = ((RefSim *)hp)->obstacles[id];
+    if (enabled) ob->enable(); else ob->disable();
+}
+
+// srand(seed), then the ray jitters of the next `count` inside tests, as
+// MeshUtils::_getCollisionGridZ draws them one after another.  Call
+// ref_seed_rand(seed) again before the run that uses them.
+void ref_predict_ray_jitters(unsigned seed, double dx, int count, float *out) {
+    srand(seed);
+    double jit = 0.001 * dx;
+    for (int q = 0; q < count; q++) {
+        vmath::vec3 jitter(MeshUtils::_randomDouble(jit, -jit), MeshUtils::_randomDouble(jit, -jit),
+                           MeshUtils::_randomDouble(jit, -jit));
+        out[3 * q] = jitter.x; out[3 * q + 1] = jitter.y; out[3 * q + 2] = jitter.z;
+    }
+}
+
+void ref_seed_rand(unsigned seed) { srand(seed); }
 
 // MeshFluidSource(isize, jsize, ksize, dx) + updateMeshStatic + setInflow +
 // setVelocity, then FluidSimulation::addMeshFluidSource
