@@ -61,6 +61,7 @@ __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
     __shared__ uint32_t phi_ord[kBW * kBW * kBW];
     __shared__ int run_begin[kSpan * kSpan];
     __shared__ int run_prefix[kSpan * kSpan + 1];
+    __shared__ float centre[kBW];  // Grid3d::GridIndexToCellCenter of local index c
 
     int nbi = (g.I + kBW - 1) / kBW, nbj = (g.J + kBW - 1) / kBW;
     int bid = blockIdx.x;
@@ -68,6 +69,7 @@ __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
     float maxd = (float)(3.0 * g.dx);  // particlelevelset.cpp:295-297
     uint32_t init = f2ord(maxd);
     for (int c = threadIdx.x; c < kBW * kBW * kBW; c += kSdfThreads) phi_ord[c] = init;
+    if (threadIdx.x < kBW) centre[threadIdx.x] = idx_to_center1(threadIdx.x, g.dx);
 
     // Candidate particles: cells [b0 - 2, b0 + 12) per axis, one contiguous
     // run of the cell-sorted arrays per (j, k) row.
@@ -110,13 +112,18 @@ __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
         gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
         gmax.i = min(gmax.i, kBW - 1); gmax.j = min(gmax.j, kBW - 1); gmax.k = min(gmax.k, kBW - 1);
         for (int k = gmin.k; k <= gmax.k; k++) {
-            float cz = idx_to_center1(k, g.dx);
+            float cz = centre[k];
             for (int j = gmin.j; j <= gmax.j; j++) {
-                float cy = idx_to_center1(j, g.dx);
+                float cy = centre[j];
                 for (int i = gmin.i; i <= gmax.i; i++) {
-                    F3 c = f3(idx_to_center1(i, g.dx), cy, cz);
+                    F3 c = f3(centre[i], cy, cz);
                     float dist = len3(c - p) - r;
-                    atomicMin(&phi_ord[i + kBW * (j + kBW * k)], f2ord(dist));
+                    // Values only fall, so a distance no smaller than the
+                    // value already there cannot change it: skip the atomic.
+                    uint32_t o = f2ord(dist);
+                    uint32_t *slot = &phi_ord[i + kBW * (j + kBW * k)];
+                    if (o < __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                        atomicMin(slot, o);
                 }
             }
         }

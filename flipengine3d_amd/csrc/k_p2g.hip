@@ -77,8 +77,16 @@ __global__ __launch_bounds__(256) void k_p2g(GridGeom g, const float *__restrict
             int b = offset[row + ci0], e = offset[row + ci1 + 1];
             for (int q = b; q < e; q++) {
                 F3 p = f3(sx[q], sy[q], sz[q]) - off;
-                if (!p2g_point_in_block(p, sr, blockdx, inv_blockdx, bi, bj, bk)) continue;
                 F3 pl = p - boff;
+                F3 v = gpos - pl;
+                float d2 = dot3(v, v);
+                // A pair adds to the face only if it is within the radius and
+                // the reference visits it (the particle is assigned to the
+                // face's block and the face lies in the particle's clamped
+                // local box).  The three tests are independent, so the cheap
+                // distance test goes first; the sums are unchanged.
+                if (!(d2 < rsq)) continue;
+                if (!p2g_point_in_block(p, sr, blockdx, inv_blockdx, bi, bj, bk)) continue;
                 I3 gmin = pos_to_index((double)(pl.x - sr), (double)(pl.y - sr), (double)(pl.z - sr), g.invdx);
                 I3 gmax = pos_to_index((double)(pl.x + sr), (double)(pl.y + sr), (double)(pl.z + sr), g.invdx);
                 gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
@@ -87,13 +95,9 @@ __global__ __launch_bounds__(256) void k_p2g(GridGeom g, const float *__restrict
                 if (li < gmin.i || li > gmax.i || lj < gmin.j || lj > gmax.j || lk < gmin.k ||
                     lk > gmax.k)
                     continue;
-                F3 v = gpos - pl;
-                float d2 = dot3(v, v);
-                if (d2 < rsq) {
-                    float wt = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
-                    sum += wt * svel[q];
-                    wsum += wt;
-                }
+                float wt = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
+                sum += wt * svel[q];
+                wsum += wt;
             }
         }
     }
