@@ -149,6 +149,13 @@ struct PressureBuffers {
     // the FP32 PCG vectors; x and the matrix stay FP64.
     float *invdiag32, *Lx32, *Ly32, *Lz32, *Lxm32, *Lym32, *Lzm32;
     float *r32, *z32, *s32;
+    // Mixed precision: the SpMV forms the new direction s = z + beta s from
+    // the old one, so s lives in two buffers: s32 is read, s32b written, and
+    // the host swaps them after each SpMV launch.  s_plain: the next SpMV
+    // reads s32 as it is (right after initialisation).
+    float *s32b;
+    int s_plain;
+    float *q32;  // mixed precision: q = A s, read by the axpy
     double *partials;                   // per-block partial sums
     int npartials;
     unsigned long long *edge;           // tagged values a tile hands to its j / k neighbours

@@ -1639,10 +1639,41 @@ __device__ __forceinline__ double spmv_slot_mixed(const PressureBuffers &pb, con
     return s;
 }
 
-// z32 = A32 s32 and alpha = rho / (s . z) with an FP64 dot.
-__global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, PressureBuffers pb) {
-    __shared__ double sh[kRedThreads / 64];
-    if (pb.sc->done) return;
+// The new direction s = z + beta s of the previous iteration
+// (pcgsolver.h:300-302), formed where the SpMV reads it.
+struct NextDirection {
+    const float *__restrict__ z, *__restrict__ s;
+    float beta;
+    __device__ __forceinline__ float operator[](size_t q) const { return z[q] + beta * s[q]; }
+};
+struct SameDirection {
+    const float *__restrict__ s;
+    __device__ __forceinline__ float operator[](size_t q) const { return s[q]; }
+};
+
+template <class Dir>
+__device__ __forceinline__ double spmv_slot_dir(const PressureBuffers &pb, const Dir &x, size_t p, int i,
+                                                int j, int k) {
+    const SlotMap &sm = pb.sm;
+    const size_t pzm = sm.slot(i, j, k - 1), pym = sm.slot(i, j - 1, k), pxm = sm.slot(i - 1, j, k);
+    const size_t pxp = sm.slot(i + 1, j, k), pyp = sm.slot(i, j + 1, k), pzp = sm.slot(i, j, k + 1);
+    double s = 0.0;
+    s += pb.offz[pzm] * (double)x[pzm];
+    s += pb.offy[pym] * (double)x[pym];
+    s += pb.offx[pxm] * (double)x[pxm];
+    s += pb.diag[p] * (double)x[p];
+    s += pb.offx[p] * (double)x[pxp];
+    s += pb.offy[p] * (double)x[pyp];
+    s += pb.offz[p] * (double)x[pzp];
+    return s;
+}
+
+// s32b = z32 + beta s32 (or s32 as it is), q32 = A s32b and alpha = rho /
+// (s . q) with an FP64 dot.  z32 and s32 are only read, so the stencil
+// reads race with no write.
+template <class Dir>
+__device__ __forceinline__ double spmv_dot32_body(const GridGeom &g, const PressureBuffers &pb,
+                                                  const Dir &dir) {
     double acc = 0.0;
     const ListRange lr = xcd_list(*pb.active_count);
     constexpr int U = 2;
@@ -1655,17 +1686,26 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, Pressure
         for (int u = 0; u < U; u++) {
             int i, j, k;
             in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
-            zc[u] = (float)spmv_slot_mixed(pb, pb.s32, p[u], i, j, k);
-            sp[u] = pb.s32[p[u]];
+            zc[u] = (float)spmv_slot_dir(pb, dir, p[u], i, j, k);
+            sp[u] = dir[p[u]];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (!ok[u]) continue;  // uniform
-            const float z = in[u] ? zc[u] : 0.0f;
-            pb.z32[p[u]] = z;
-            acc += (double)sp[u] * (double)z;
+            const float q = in[u] ? zc[u] : 0.0f;
+            pb.q32[p[u]] = q;
+            pb.s32b[p[u]] = sp[u];
+            acc += (double)sp[u] * (double)q;
         }
     }
+    return acc;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, PressureBuffers pb) {
+    __shared__ double sh[kRedThreads / 64];
+    if (pb.sc->done) return;
+    double acc = pb.s_plain ? spmv_dot32_body(g, pb, SameDirection{pb.s32})
+                            : spmv_dot32_body(g, pb, NextDirection{pb.z32, pb.s32, pb.sc->beta_f});
     double bs = block_reduce<0>(acc, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[blockIdx.x], bs);
     if (!last_block(&pb.sc->counter[1])) return;
@@ -1682,7 +1722,7 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, Pressure
     }
 }
 
-// x += alpha s32 (FP64), r32 -= alpha z32 (FP32), |r32|_inf; schedules a
+// x += alpha s32 (FP64), r32 -= alpha q32 (FP32), |r32|_inf; schedules a
 // true residual when it is due.
 __global__ __launch_bounds__(kRedThreads) void k_axpy_norm32(GridGeom g, PressureBuffers pb) {
     __shared__ double sh[kRedThreads / 64];
@@ -1701,7 +1741,7 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm32(GridGeom g, Pressur
 #pragma unroll
         for (int u = 0; u < U; u++) {
             xs[u] = pb.x[p[u]]; ss[u] = pb.s32[p[u]];
-            rs[u] = pb.r32[p[u]]; zs[u] = pb.z32[p[u]];
+            rs[u] = pb.r32[p[u]]; zs[u] = pb.q32[p[u]];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -1828,6 +1868,7 @@ hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, bool mix
         e = launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotInit, s);
         if (e != hipSuccess) return e;
         k_update_s32<<<kRedBlocks, kRedThreads, 0, s>>>(pb, 1);
+        pb.s_plain = 1;  // the first SpMV takes s = z as it is
         return hipGetLastError();
     }
     k_pcg_init_vectors<<<kRedBlocks, kRedThreads, 0, s>>>(g, pb);
@@ -1841,14 +1882,22 @@ hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, bool mix
     return hipGetLastError();
 }
 
-bool pcg_piece_used(int piece, bool mixed) { return piece != kPcgResidual || mixed; }
+// Mixed precision folds the direction update into the next SpMV; the FP64
+// path has no true-residual step.
+bool pcg_piece_used(int piece, bool mixed) { return mixed ? piece != kPcgUpdate : piece != kPcgResidual; }
 
 hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBuffers &pb,
                             hipStream_t s, const KernelTiming &kt) {
     const dim3 rg(kRedBlocks), rb(kRedThreads);
     if (mixed) {
         switch (piece) {
-            case kPcgSpmv: return launch_on(k_spmv_dot32, rg, rb, s, kt, g, pb);
+            case kPcgSpmv: {
+                hipError_t e = launch_on(k_spmv_dot32, rg, rb, s, kt, g, pb);
+                // The launch took s32 -> s32b by value; the new direction is s32 from here on.
+                std::swap(pb.s32, pb.s32b);
+                pb.s_plain = 0;
+                return e;
+            }
             case kPcgAxpy: return launch_on(k_axpy_norm32, rg, rb, s, kt, g, pb);
             case kPcgResidual: return launch_on(k_true_residual, rg, rb, s, kt, g, pb);
             case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kDotNone, s, kt);

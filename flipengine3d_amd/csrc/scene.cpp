@@ -150,7 +150,7 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
             (void)launch_fill_f32(arr32[q].p, ns, 0.0f, s);
         }
         float **dst[kSlotArrays32] = {&pb.invdiag32, &pb.Lx32, &pb.Ly32, &pb.Lz32, &pb.Lxm32, &pb.Lym32,
-                                      &pb.Lzm32, &pb.r32, &pb.z32, &pb.s32};
+                                      &pb.Lzm32, &pb.r32, &pb.z32, &pb.s32, &pb.s32b, &pb.q32};
         for (int q = 0; q < kSlotArrays32; q++) *dst[q] = arr32[q].p;
     }
     pb.npartials = npart;

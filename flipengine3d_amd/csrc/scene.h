@@ -237,7 +237,7 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
 // Lxm, Lym, Lzm), arr[16] the reduction partials.
 struct PressureStore {
     static constexpr int kSlotArrays = 16;
-    static constexpr int kSlotArrays32 = 10;  // FP32 factor and vectors (mixed precision)
+    static constexpr int kSlotArrays32 = 12;  // FP32 factor and vectors, s twice, q (mixed precision)
     DBuf<double> arr[kSlotArrays + 1];
     DBuf<float> arr32[kSlotArrays32];
     DBuf<unsigned long long> edge, mask;
