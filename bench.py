@@ -51,8 +51,8 @@ PCG_WORDS = {"k_spmv_dot": 6, "k_axpy_norm": 6, "k_forward": 6, "k_backward": 6,
 # profiles/traffic_pcg_kernels_double_dam_256x128x128_mixed.md.  Keyed by
 # (precision, grid); other workloads report null.
 MEASURED_TRAFFIC = {
-    ("mixed", (256, 128, 128)): {"k_backward": 64.24e6, "k_forward": 57.08e6, "k_spmv_dot": 43.31e6,
-                                 "k_axpy_norm": 53.39e6, "k_update_s": 20.04e6},
+    ("mixed", (256, 128, 128)): {"k_backward": 64.15e6, "k_forward": 56.98e6, "k_spmv_dot": 93.28e6,
+                                 "k_axpy_norm": 53.38e6, "k_true_residual": 96.42e6},
 }
 
 def parse_args(argv=None):

@@ -5,7 +5,7 @@
 // against the oracle.
 //
 //   wrapper_check <mode> <dir>
-//   modes: levelset  pressure  p2g  advect  sim
+//   modes: levelset  pressure  p2g  advect  sim  sources
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -258,6 +258,67 @@ int run_sim() {
 
 }  // namespace
 
+TriangleMesh read_mesh(const std::string &stem) {
+    std::vector<float> v = read_raw<float>(stem + "_verts.f32");
+    std::vector<int> t = read_raw<int>(stem + "_tris.i32");
+    TriangleMesh m;
+    for (size_t q = 0; q + 2 < v.size(); q += 3) m.vertices.push_back(vmath::vec3(v[q], v[q + 1], v[q + 2]));
+    for (size_t q = 0; q + 2 < t.size(); q += 3) {
+        Triangle tri;
+        tri.tri[0] = t[q]; tri.tri[1] = t[q + 1]; tri.tri[2] = t[q + 2];
+        m.triangles.push_back(tri);
+    }
+    return m;
+}
+
+// A pool with a static obstacle (MeshObject + addMeshObstacle) and an inflow
+// source (MeshFluidSource + addMeshFluidSource), stepped for d.extra frames.
+int run_sources() {
+    Dims d = read_dims();
+    std::vector<float> pos = read_raw<float>("positions.f32"), vel = read_raw<float>("velocities.f32");
+    std::vector<double> src_vel = read_raw<double>("src_vel.f64");
+    FluidSimulation sim(d.I, d.J, d.K, d.dx);
+    sim.addBodyForce(0.0, -9.81, 0.0);
+    sim.setMinTimeStepsPerFrame(1);
+    sim.setMaxTimeStepsPerFrame(6);
+    sim.setMarkerParticleJitterSeed(7);
+    FluidSimulationMarkerParticleData data;
+    data.size = (int)(pos.size() / 3);
+    data.positions = reinterpret_cast<char *>(pos.data());
+    data.velocities = reinterpret_cast<char *>(vel.data());
+    sim.loadMarkerParticleData(data);
+    MeshObject obstacle(d.I, d.J, d.K, d.dx);
+    obstacle.updateMeshStatic(read_mesh("obst"));
+    sim.addMeshObstacle(&obstacle);
+    MeshFluidSource source(d.I, d.J, d.K, d.dx);
+    source.updateMeshStatic(read_mesh("src"));
+    source.setInflow();
+    source.setVelocity(vmath::vec3(src_vel[0], src_vel[1], src_vel[2]));
+    sim.addMeshFluidSource(&source);
+    bool threw = false;
+    try {
+        sim.addMeshFluidSource(&source);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    if (!threw) throw std::runtime_error("adding a source twice did not throw");
+    sim.initialize();
+    std::ofstream st(g_dir + "/stats.txt");
+    for (int f = 0; f < (int)d.extra; f++) {
+        sim.update(d.dt);
+        FluidSimulationFrameStats fs = sim.getFrameStatsData();
+        st << fs.frame << " " << fs.substeps << " " << fs.fluidParticles << "\n";
+    }
+    std::vector<char> pblob(sim.getMarkerParticlePositionDataSize()), vblob(sim.getMarkerParticleVelocityDataSize());
+    if (!pblob.empty()) {
+        sim.getMarkerParticlePositionData(pblob.data());
+        sim.getMarkerParticleVelocityData(vblob.data());
+    }
+    write_raw("positions_out.f32", pblob.data(), pblob.size());
+    write_raw("velocities_out.f32", vblob.data(), vblob.size());
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc != 3) {
         std::cerr << "usage: wrapper_check <levelset|pressure|p2g|advect|sim> <dir>\n";
@@ -272,6 +333,7 @@ int main(int argc, char **argv) {
         if (mode == "p2g") return run_p2g();
         if (mode == "advect") return run_advect();
         if (mode == "sim") return run_sim();
+        if (mode == "sources") return run_sources();
     } catch (const std::exception &e) {
         std::cerr << "wrapper_check " << mode << ": " << e.what() << "\n";
         return 1;
