@@ -117,13 +117,20 @@ __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
                 float cy = centre[j];
                 for (int i = gmin.i; i <= gmax.i; i++) {
                     F3 c = f3(centre[i], cy, cz);
-                    float dist = len3(c - p) - r;
+                    F3 dv = c - p;
+                    float d2 = dot3(dv, dv);
                     // Values only fall, so a distance no smaller than the
-                    // value already there cannot change it: skip the atomic.
-                    uint32_t o = f2ord(dist);
+                    // value already there cannot change it.  A squared
+                    // distance clearly beyond (value + r)^2 (margins of 2e-5
+                    // relative cover every rounding of the exact test) skips
+                    // the square root; otherwise the exact distance decides.
                     uint32_t *slot = &phi_ord[i + kBW * (j + kBW * k)];
-                    if (o < __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-                        atomicMin(slot, o);
+                    uint32_t cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    float lim = ord2f(cur) + r * 1.00002f;
+                    if (lim > 0.0f && d2 > lim * lim * 1.00002f) continue;
+                    float dist = sqrtf(d2) - r;
+                    uint32_t o = f2ord(dist);
+                    if (o < cur) atomicMin(slot, o);
                 }
             }
         }
