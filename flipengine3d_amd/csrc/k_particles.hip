@@ -121,6 +121,12 @@ struct Box {
     }
 };
 
+// k1 is the field's velocity at p0, passed in when the caller has it.
+__device__ F3 rk3_from(const MacView &m, const GridGeom &g, F3 p0, F3 k1, double dt) {
+    F3 k2 = mac_velocity(m, g, p0 + (float)(0.5 * dt) * k1);
+    F3 k3 = mac_velocity(m, g, p0 + (float)(0.75 * dt) * k2);
+    return p0 + (float)(dt / 9.0f) * ((2.0f * k1 + 3.0f * k2) + 4.0f * k3);
+}
 __device__ F3 rk3(const MacView &m, const GridGeom &g, F3 p0, double dt) {
     F3 k1 = mac_velocity(m, g, p0);
     F3 k2 = mac_velocity(m, g, p0 + (float)(0.5 * dt) * k1);
@@ -213,7 +219,8 @@ __device__ __forceinline__ void particle_step_one(const GridGeom &g, const Parti
     }
     // Advection + collision (fluidsimulation.cpp:4200-4219).
     Box bnd{f3(a.bpos[0], a.bpos[1], a.bpos[2]), a.bext[0], a.bext[1], a.bext[2]};
-    F3 p1 = rk3(mac, g, p, a.dt);
+    // RK3's first stage samples the projected field at p: that is vpic.
+    F3 p1 = rk3_from(mac, g, p, vpic, a.dt);
     p1 = resolve_collision(a, g, bnd, p, p1);
     a.px[i] = p1.x; a.py[i] = p1.y; a.pz[i] = p1.z;
     a.vx[i] = vnew.x; a.vy[i] = vnew.y; a.vz[i] = vnew.z;
