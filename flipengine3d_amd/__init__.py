@@ -109,6 +109,7 @@ EXPORTED = [
     "flipmi_scene_get_particles", "flipmi_scene_get_particles_range", "flipmi_scene_set_domain",
     "flipmi_scene_add_inflow_source", "flipmi_scene_set_inflow_source", "flipmi_scene_set_marker_jitter",
     "flipmi_scene_get_inflow_source", "flipmi_scene_add_obstacle", "flipmi_scene_set_obstacle",
+    "flipmi_scene_set_outflow_source",
     "flipmi_scene_get_grid", "flipmi_scene_stage_times",
     "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
     "flipmi_liquid_sdf_postprocess", "flipmi_p2g",
@@ -150,6 +151,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_get_inflow_source.argtypes = [_vp, _c_int, _vp, _vp, _vp]
     L.flipmi_scene_add_obstacle.argtypes = [_vp, _vp, _c_int, _vp, _c_int, P(_c_int)]
     L.flipmi_scene_set_obstacle.argtypes = [_vp, _c_int, _c_int]
+    L.flipmi_scene_set_outflow_source.argtypes = [_vp, _c_int, _c_int, _c_int]
     L.flipmi_scene_get_grid.argtypes = [_vp, _c_int, _vp]
     L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
     L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
@@ -507,6 +509,12 @@ class FluidSimulation:
         cells = np.zeros((self.desc.ksize, self.desc.jsize, self.desc.isize), np.uint8)
         self.ctx.check(self.L.flipmi_scene_get_inflow_source(self.h, source_id, _ptr(d), _ptr(sdf), _ptr(cells)))
         return tuple(int(v) for v in d[:3]), sdf, cells
+
+    def setMeshFluidSourceOutflow(self, source_id: int, outflow: bool = True, inversed: bool = False):
+        """setOutflow() / setInflow() and outflowInverse() of a source
+        (meshfluidsource.cpp:74-84, 158-160)."""
+        self._ensure()
+        self.ctx.check(self.L.flipmi_scene_set_outflow_source(self.h, source_id, int(outflow), int(inversed)))
 
     def setMarkerParticleJitter(self, factor: float = 1.0, jitter_surface: bool = False, seed: int = 0):
         """setMarkerParticleJitterFactor + enable/disableJitterSurfaceMarkerParticles

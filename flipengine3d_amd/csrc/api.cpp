@@ -247,6 +247,15 @@ int flipmi_scene_set_inflow_source(flipmi_scene *scene, int id, int enabled, int
     return FLIPMI_OK;
 }
 
+int flipmi_scene_set_outflow_source(flipmi_scene *scene, int id, int outflow, int inversed) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    if (id < 0 || id >= (int)s.sources.size()) return fail(s.ctx, FLIPMI_ERR_INVALID_ARG, "unknown inflow source");
+    s.sources[(size_t)id]->outflow = outflow != 0;
+    s.sources[(size_t)id]->inversed = inversed != 0;
+    return FLIPMI_OK;
+}
+
 int flipmi_scene_set_marker_jitter(flipmi_scene *scene, double factor, int jitter_surface, uint64_t seed) {
     if (!scene) return FLIPMI_ERR_INVALID_ARG;
     Scene &s = scene->s;

@@ -241,6 +241,8 @@ struct InflowView {
     float vel[3];
     int emit;       // enabled: emits this substep
     int constrain;  // enabled with constrained fluid velocity
+    int outflow;    // enabled outflow source: removes particles
+    int inversed;   // the outflow acts outside the mesh
 };
 constexpr int kMaxInflow = 8;
 struct InflowViews {
@@ -273,6 +275,9 @@ hipError_t launch_inflow_emit(const GridGeom &g, const InflowView &v, const floa
                               uint32_t *mask, const EmitParams &ep, float *cand, int *flags,
                               int *pos, int *scan_tmp, int *nbase, const EmitOut &out,
                               hipStream_t s);
+// Keep flags of the outflow sources (1 = the particle stays).
+hipError_t launch_outflow_keep(const GridGeom &g, const float *px, const float *py, const float *pz, int n,
+                               const InflowViews &views, int *keep, hipStream_t s);
 // Body force on one MAC component that skips the valid faces an inflow
 // source constrains (fluidsimulation.cpp:3372-3438, 3450-3489).
 hipError_t launch_save_and_body_force_inflow(const GridGeom &g, int comp, const float *mac,

@@ -322,6 +322,39 @@ __global__ void k_emit_advance(int m, const int *__restrict__ flags, const int *
     *nbase += pos[m - 1] + flags[m - 1];
 }
 
+// FluidSimulation::_updateOutflowMeshFluidSource (fluidsimulation.cpp:
+// 4608-4673): a particle in an outflow cell (a source cell, or any other
+// cell when inversed) goes where the source SDF, sampled with the sub-grid
+// offset, is negative (not negative when inversed).
+__global__ __launch_bounds__(256) void k_outflow_keep(GridGeom g, const float *__restrict__ px,
+                                                      const float *__restrict__ py,
+                                                      const float *__restrict__ pz, int n,
+                                                      InflowViews views, int *keep) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    F3 p = f3(px[q], py[q], pz[q]);
+    I3 c = pos_to_index(p, g.invdx);
+    const size_t cell = (size_t)c.i + (size_t)g.I * ((size_t)c.j + (size_t)g.J * c.k);
+    int stay = 1;
+    for (int s = 0; s < views.n; s++) {
+        const InflowView &v = views.v[s];
+        if (!v.outflow) continue;
+        const bool in = v.cells[cell] != 0;
+        if (v.inversed ? in : !in) continue;
+        float d = (float)nodal_trilinear(v.sdf, Dims{v.ni, v.nj, v.nk}, g.dx, g.invdx,
+                                         p - f3(v.off[0], v.off[1], v.off[2]));
+        if (v.inversed ? d >= 0.0f : d < 0.0f) stay = 0;
+    }
+    keep[q] = stay;
+}
+
+hipError_t launch_outflow_keep(const GridGeom &g, const float *px, const float *py, const float *pz, int n,
+                               const InflowViews &views, int *keep, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_outflow_keep<<<(n + 255) / 256, 256, 0, s>>>(g, px, py, pz, n, views, keep);
+    return hipGetLastError();
+}
+
 hipError_t launch_inflow_mask(const GridGeom &g, const float *px, const float *py, const float *pz,
                               int n, const InflowViews &views, uint32_t *mask, hipStream_t s) {
     for (int q = 0; q < views.n; q++) {

@@ -681,7 +681,7 @@ double Scene::next_time_step(double dt) {
         // the fastest enabled inflow source plus one step of body force.
         maxu = 0.0;
         for (auto &src : sources) {
-            if (!src->enabled) continue;
+            if (!src->enabled || src->outflow) continue;
             const float *u3 = src->vel;
             float sp = std::sqrt(u3[0] * u3[0] + u3[1] * u3[1] + u3[2] * u3[2]);
             maxu = std::fmax(std::fmax((double)sp, 0.0), maxu);
@@ -1193,16 +1193,42 @@ InflowViews Scene::inflow_views(bool emitting) const {
         }
         // _updateInflowMeshFluidSource (fluidsimulation.cpp:4564-4606): a
         // source with 0 substep emissions emits at the first substep only.
-        v.emit = emitting && s.built && s.enabled && s.ncells > 0 && (s.emissions > 0 || step_number == 0);
-        v.constrain = s.built && s.enabled && s.constrain && s.ncells > 0 ? 1 : 0;
+        const bool on = s.built && s.enabled;
+        v.emit = emitting && on && !s.outflow && s.ncells > 0 && (s.emissions > 0 || step_number == 0);
+        v.constrain = on && !s.outflow && s.constrain && s.ncells > 0 ? 1 : 0;
+        v.outflow = on && s.outflow ? 1 : 0;
+        v.inversed = s.inversed ? 1 : 0;
     }
     return vs;
 }
 
 bool Scene::any_inflow_constraint() const {
     for (auto &s : sources)
-        if (s->enabled && s->constrain) return true;
+        if (s->enabled && !s->outflow && s->constrain) return true;
     return false;
+}
+
+void Scene::remove_outflow() {
+    bool any = false;
+    for (auto &s : sources) any = any || (s->enabled && s->outflow);
+    if (!any || n == 0) return;
+    hipEvent_t ev = nullptr;
+    begin_stage(FLIPMI_STAGE_INFLOW, &ev);
+    InflowViews vs = inflow_views(false);
+    check(launch_outflow_keep(g, part[0].p, part[1].p, part[2].p, n, vs, keep.p, stream), "outflow keep");
+    const float *src[6];
+    float *dst[6];
+    for (int a = 0; a < 6; a++) {
+        src[a] = part[a].p;
+        dst[a] = part_alt[a].p;
+    }
+    check(launch_compact_keep(keep.p, n, keep_pos.p, scan_tmp.p, src, dst, new_count.p, stream), "compact");
+    int *nc_host = (int *)host_stage + 192;
+    check(hipMemcpyAsync(nc_host, new_count.p, sizeof(int), hipMemcpyDeviceToHost, stream), "count");
+    sync();
+    n = *nc_host;
+    for (int a = 0; a < 6; a++) std::swap(part[a].p, part_alt[a].p);
+    end_stage(FLIPMI_STAGE_INFLOW, ev);
 }
 
 void Scene::grow(int64_t need_n) {
@@ -1350,6 +1376,7 @@ void Scene::step_fluid(double dt) {
     if (n > 0) advance_particles(dt);
     // _updateFluidObjects (fluidsimulation.cpp:4794-4806).
     emit_inflow();
+    remove_outflow();
 }
 
 flipmi_frame_stats Scene::update(double dt) {

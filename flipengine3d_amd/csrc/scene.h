@@ -66,6 +66,8 @@ struct InflowSource {
     bool enabled = true;       // enable() / disable()
     bool constrain = true;     // _isConstrainedFluidVelocity (meshfluidsource.h:124)
     int emissions = 1;         // _substepEmissions (meshfluidsource.h:119)
+    bool outflow = false;      // setOutflow() (meshfluidsource.cpp:82-84)
+    bool inversed = false;     // outflowInverse() (meshfluidsource.cpp:158-160)
     // Placement of the source SDF (MeshFluidSource::update, :178-222).
     int go[3] = {0, 0, 0};     // sub-grid offset in cells
     float off[3] = {0, 0, 0};  // its position
@@ -195,6 +197,8 @@ struct Scene {
     InflowViews inflow_views(bool emitting) const;
     bool any_inflow_constraint() const;
     void emit_inflow();
+    // FluidSimulation::_updateOutflowMeshFluidSources at the end of a substep.
+    void remove_outflow();
     // Static mesh obstacles: add (checks the mesh) and their union into the
     // solid SDF (FluidSimulation::_addStaticObjectsToSDF).
     int add_obstacle(const float *verts, int nv, const int *tris, int nt);

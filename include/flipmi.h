@@ -213,6 +213,13 @@ int flipmi_scene_get_inflow_source(flipmi_scene *scene, int source_id, int dims[
  * negative n counts as 0. */
 int flipmi_scene_set_inflow_source(flipmi_scene *scene, int source_id, int enabled,
                                    int constrain_velocity, int substep_emissions);
+/* MeshFluidSource::setOutflow() / setInflow() and outflowInverse()
+ * (meshfluidsource.cpp:74-84, 158-160): an outflow source adds nothing and
+ * removes, at the end of every substep, the particles in its cells where
+ * its SDF is negative (inversed: the particles outside its cells, or inside
+ * them where the SDF is not negative), keeping the others in order
+ * (fluidsimulation.cpp:4608-4673). */
+int flipmi_scene_set_outflow_source(flipmi_scene *scene, int source_id, int outflow, int inversed);
 /* FluidSimulation::setMarkerParticleJitterFactor (fluidsimulation.cpp:185-196,
  * factor in [0, 1]) and enable/disableJitterSurfaceMarkerParticles (:198-210)
  * for the particles inflow sources add.  The reference draws the jitter from

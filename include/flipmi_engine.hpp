@@ -669,8 +669,12 @@ public:
         if (_scene) throw std::runtime_error("flipmi: the source mesh is fixed once the source is on the device");
         _mesh = mesh;
     }
-    void setInflow() {}
-    bool isInflow() { return true; }
+    void setInflow() { _outflow = false; _push(); }
+    void setOutflow() { _outflow = true; _push(); }
+    bool isInflow() { return !_outflow; }
+    bool isOutflow() { return _outflow; }
+    void outflowInverse() { _inversed = !_inversed; _push(); }  // meshfluidsource.cpp:158-160
+    bool isOutflowInversed() { return _inversed; }
     void setVelocity(vmath::vec3 v) {
         if (_scene) throw std::runtime_error("flipmi: the source velocity is fixed once the source is on the device");
         _velocity = v;
@@ -700,14 +704,16 @@ private:
         _push();
     }
     void _push() {
-        if (_scene)
-            detail::check(flipmi_scene_set_inflow_source(_scene, _id, _enabled, _constrain, _emissions),
-                          "MeshFluidSource");
+        if (!_scene) return;
+        detail::check(flipmi_scene_set_inflow_source(_scene, _id, _enabled, _constrain, _emissions),
+                      "MeshFluidSource");
+        detail::check(flipmi_scene_set_outflow_source(_scene, _id, _outflow, _inversed), "MeshFluidSource");
     }
     TriangleMesh _mesh;
     vmath::vec3 _velocity;
     bool _enabled = true, _constrain = true;
     int _emissions = 1;
+    bool _outflow = false, _inversed = false;
     flipmi_scene *_scene = nullptr;
     int _id = -1;
 };

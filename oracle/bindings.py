@@ -74,6 +74,7 @@ class _Base:
         getattr(L, p + "trilinear_velocity").argtypes = [_c_int] * 3 + [_c_double] + [_vp] * 4 + [_c_int, _vp]
         getattr(L, p + "sim_add_inflow").argtypes = [_vp, _vp, _c_int, _vp, _c_int] + [_c_double] * 3
         getattr(L, p + "sim_set_inflow").argtypes = [_vp] + [_c_int] * 4
+        getattr(L, p + "sim_set_outflow").argtypes = [_vp] + [_c_int] * 3
         getattr(L, p + "sim_inflow_sdf").argtypes = [_vp, _c_int, _vp, _vp]
 
     # -- whole-step driver -------------------------------------------------
@@ -248,6 +249,10 @@ class _SimHandle:
 
     def set_obstacle(self, oid, enabled):
         self._f("sim_set_obstacle")(self.h, oid, int(enabled))
+
+    def set_outflow(self, sid, outflow=True, inversed=False):
+        """setOutflow() / outflowInverse() of a source."""
+        self._f("sim_set_outflow")(self.h, sid, int(outflow), int(inversed))
 
     def set_jitter(self, factor, surface=False, seed=0):
         """setMarkerParticleJitterFactor + jitter-surface switch (the oracle

@@ -1373,6 +1373,8 @@ struct InflowSource {
     bool enabled = true;      // meshfluidsource.h
     bool constrain = true;    // _isConstrainedFluidVelocity (meshfluidsource.h:124)
     int emissions = 1;        // _substepEmissions (meshfluidsource.h:119)
+    bool outflow = false;     // setOutflow() (meshfluidsource.cpp:82-84)
+    bool inversed = false;    // outflowInverse() (meshfluidsource.cpp:158-160)
     bool ready = false;       // static mesh: the SDF and cells are built once
     Idx gridOffset;           // _sourceSDFGridOffset
     V3 sdfOffset;             // _sourceSDFOffset
@@ -1848,7 +1850,7 @@ static void ensure_sources(Sim &S) {
 // (fluidsimulation.cpp:4564-4606, 4675-4697), static meshes.
 static void update_inflow_sources(Sim &S) {
     int active = 0;
-    for (const InflowSource &s : S.sources) active += s.enabled;
+    for (const InflowSource &s : S.sources) active += s.enabled && !s.outflow;
     if (active == 0) return;
     ensure_sources(S);
     MaskGrid mask;
@@ -1856,13 +1858,41 @@ static void update_inflow_sources(Sim &S) {
     mask.m = GridB(S.I, S.J, S.K, 0);
     for (const V3 &p : S.pos) mask.add(p);
     for (InflowSource &src : S.sources) {
-        if (!src.enabled) continue;
+        if (!src.enabled || src.outflow) continue;
         int passes = src.emissions;
         if (passes == 0) {
             passes = 1;
             if (S.stepNumber != 0) continue;
         }
         for (int e = 0; e < passes; e++) emit_source(S, src, mask);
+    }
+}
+
+// FluidSimulation::_updateOutflowMeshFluidSources / _updateOutflowMeshFluidSource
+// (fluidsimulation.cpp:4608-4673, 4699-4718), fluid outflow (no diffuse
+// particles exist): particles in the source's cells (all other cells when
+// inversed) are removed where the source SDF, sampled with its offset, is
+// negative (not negative when inversed); the rest keep their order.
+static void update_outflow_sources(Sim &S) {
+    bool any = false;
+    for (const InflowSource &s : S.sources) any = any || (s.enabled && s.outflow);
+    if (!any) return;
+    ensure_sources(S);
+    for (const InflowSource &src : S.sources) {
+        if (!src.enabled || !src.outflow) continue;
+        size_t cur = 0;
+        for (size_t i = 0; i < S.pos.size(); i++) {
+            Idx g = pos_to_index(S.pos[i], S.dx);
+            bool cell = src.isCell(g.i, g.j, g.k) != 0;
+            bool removed = false;
+            if (src.inversed ? !cell : cell) {
+                float d = grid_trilinear(src.sdf, S.pos[i] - src.sdfOffset, S.dx);
+                removed = src.inversed ? d >= 0.0f : d < 0.0f;
+            }
+            if (!removed) { S.pos[cur] = S.pos[i]; S.vel[cur] = S.vel[i]; cur++; }
+        }
+        S.pos.resize(cur);
+        S.vel.resize(cur);
     }
 }
 
@@ -1873,7 +1903,7 @@ static void update_inflow_sources(Sim &S) {
 // static source repeats the same assignment, so one pass is applied.
 static void constrain_inflow_particles(Sim &S) {
     for (InflowSource &src : S.sources) {
-        if (!src.enabled || !src.constrain) continue;
+        if (!src.enabled || src.outflow || !src.constrain) continue;
         for (size_t i = 0; i < S.pos.size(); i++) {
             Idx g = pos_to_index(S.pos[i], S.dx);
             if (!src.isCell(g.i, g.j, g.k)) continue;
@@ -1890,7 +1920,7 @@ static void constrain_inflow_particles(Sim &S) {
 static void inflow_face_mask(const Sim &S, Valid &ex) {
     valid_init(ex, S.I, S.J, S.K);
     for (const InflowSource &src : S.sources) {
-        if (!src.enabled || !src.constrain) continue;
+        if (!src.enabled || src.outflow || !src.constrain) continue;
         const double dx = S.dx;
         for (int k = 0; k < S.K; k++)
             for (int j = 0; j < S.J; j++)
@@ -2000,6 +2030,7 @@ static void step_fluid(Sim &S, double dt) {
     advance_particles(S, dt);
     // _updateFluidObjects (fluidsimulation.cpp:4794-4806).
     update_inflow_sources(S);
+    update_outflow_sources(S);
 }
 
 // FluidSimulation::_calculateNextTimeStep (fluidsimulation.cpp:5593-5613).
@@ -2009,7 +2040,8 @@ static double next_time_step(const Sim &S, double dt) {
         // _predictMaximumMarkerParticleSpeed (:5529-5551): the fastest
         // enabled inflow source plus one step of body force.
         for (const InflowSource &src : S.sources)
-            if (src.enabled) maxu = std::fmax((double)std::fmax((double)length(src.velocity), 0.0), maxu);
+            if (src.enabled && !src.outflow)
+                maxu = std::fmax((double)std::fmax((double)length(src.velocity), 0.0), maxu);
         maxu += (double)length(S.body_force()) * dt;
     } else {
         double maxsq = 0.0;
@@ -2296,6 +2328,13 @@ int orc_sim_add_inflow(void *h, const float *verts, int nv, const int *tris, int
     src.velocity = V3((float)vx, (float)vy, (float)vz);
     S->sources.push_back(src);
     return (int)S->sources.size() - 1;
+}
+// setOutflow() and outflowInverse() of one source (meshfluidsource.cpp:
+// 82-84, 158-160).
+void orc_sim_set_outflow(void *h, int id, int outflow, int inversed) {
+    InflowSource &src = ((Sim *)h)->sources[id];
+    src.outflow = outflow != 0;
+    src.inversed = inversed != 0;
 }
 // enable()/disable(), enable/disableConstrainedFluidVelocity() and
 // setSubstepEmissions() of one source (meshfluidsource.cpp:51-72, 146-156).

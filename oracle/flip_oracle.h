@@ -81,6 +81,7 @@ int orc_mesh_sdf(const float *verts, int nv, const int *tris, int nt, int isize,
 int orc_sim_add_inflow(void *h, const float *verts, int nv, const int *tris, int nt, double vx,
                        double vy, double vz);
 void orc_sim_set_inflow(void *h, int id, int enabled, int constrain, int emissions);
+void orc_sim_set_outflow(void *h, int id, int outflow, int inversed);
 void orc_sim_set_jitter(void *h, double factor, int surface, uint64_t seed);
 int orc_sim_inflow_sdf(void *h, int id, int *dims, float *phi);
 int orc_sim_inflow_cells(void *h, int id, uint8_t *cells);

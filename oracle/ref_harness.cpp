@@ -382,6 +382,12 @@ void ref_sim_set_inflow(void *hp, int id, int enabled, int constrain, int emissi
     src->setSubstepEmissions(emissions);
 }
 
+void ref_sim_set_outflow(void *hp, int id, int outflow, int inversed) {
+    MeshFluidSource *src = ((RefSim *)hp)->sources[id];
+    if (outflow) src->setOutflow(); else src->setInflow();
+    if (inversed != (src->isOutflowInversed() ? 1 : 0)) src->outflowInverse();
+}
+
 void ref_sim_set_jitter(void *hp, double factor, int surface) {
     FluidSimulation *sim = ((RefSim *)hp)->sim;
     sim->setMarkerParticleJitterFactor(factor);

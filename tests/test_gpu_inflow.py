@@ -112,3 +112,39 @@ def test_pool_and_jet_against_oracle(gpu, oracle):
             assert float(np.abs(vg - vo).max()) <= 1e-4 * max(1.0, float(np.abs(vo).max()))
     finally:
         sim.close()
+
+
+OUTFLOW_BOX = scenes.box_mesh((0.13, 0.07, 0.21), (0.36, 0.31, 0.79))
+
+
+@pytest.mark.parametrize("inversed", [False, True])
+def test_outflow_against_oracle(gpu, oracle, inversed):
+    """An outflow source (setOutflow, outflowInverse) in a dam break: the
+    particle counts after each of three frames equal the oracle's, positions
+    within 1e-4 dx and velocities within 1e-4 max(1, vmax)."""
+    dims, dx = (24, 24, 24), 1 / 24
+    sc = scenes.dam_break(*dims, dx, seed=3)
+    sim = fa.FluidSimulation(*dims, dx, ctx=gpu, precision=fa.PCG_FP64)
+    sim.addBodyForce(0.0, -9.81, 0.0)
+    sim.loadMarkerParticleData(sc.positions, sc.velocities)
+    sid = sim.addMeshFluidSource(OUTFLOW_BOX[0], OUTFLOW_BOX[1], (0.0, 0.0, 0.0))
+    sim.setMeshFluidSourceOutflow(sid, True, inversed)
+    sim.initialize()
+    h = oracle.sim(*dims, dx)
+    h.add_body_force(0.0, -9.81, 0.0)
+    h.load_particles(sc.positions, sc.velocities)
+    hid = h.add_inflow(OUTFLOW_BOX[0], OUTFLOW_BOX[1], (0.0, 0.0, 0.0))
+    h.set_outflow(hid, True, inversed)
+    h.initialize()
+    try:
+        for frame in range(3):
+            sim.update(1 / 30)
+            h.update(1 / 30)
+            pg, vg = sim.particles()
+            po, vo = h.particles()
+            assert pg.shape == po.shape, (frame, pg.shape, po.shape)
+            assert float(np.abs(pg - po).max()) <= 1e-4 * dx
+            assert float(np.abs(vg - vo).max()) <= 1e-4 * max(1.0, float(np.abs(vo).max()))
+        assert pg.shape[0] < sc.num_particles
+    finally:
+        sim.close()
