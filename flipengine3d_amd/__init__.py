@@ -109,7 +109,7 @@ EXPORTED = [
     "flipmi_scene_get_particles", "flipmi_scene_get_particles_range", "flipmi_scene_set_domain",
     "flipmi_scene_add_inflow_source", "flipmi_scene_set_inflow_source", "flipmi_scene_set_marker_jitter",
     "flipmi_scene_get_inflow_source", "flipmi_scene_add_obstacle", "flipmi_scene_set_obstacle",
-    "flipmi_scene_set_outflow_source",
+    "flipmi_scene_set_outflow_source", "flipmi_scene_set_obstacle_inversed",
     "flipmi_scene_get_grid", "flipmi_scene_stage_times",
     "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
     "flipmi_liquid_sdf_postprocess", "flipmi_p2g",
@@ -152,6 +152,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_add_obstacle.argtypes = [_vp, _vp, _c_int, _vp, _c_int, P(_c_int)]
     L.flipmi_scene_set_obstacle.argtypes = [_vp, _c_int, _c_int]
     L.flipmi_scene_set_outflow_source.argtypes = [_vp, _c_int, _c_int, _c_int]
+    L.flipmi_scene_set_obstacle_inversed.argtypes = [_vp, _c_int, _c_int]
     L.flipmi_scene_get_grid.argtypes = [_vp, _c_int, _vp]
     L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
     L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
@@ -473,6 +474,12 @@ class FluidSimulation:
         """MeshObject::enable()/disable(); the solid is rebuilt at the next substep."""
         self._ensure()
         self.ctx.check(self.L.flipmi_scene_set_obstacle(self.h, obstacle_id, int(enabled)))
+
+    def setMeshObstacleInversed(self, obstacle_id: int, inversed: bool = True):
+        """MeshObject::inverse() state: a container, solid outside the mesh.
+        Set it before the first update (it does not rebuild an existing solid)."""
+        self._ensure()
+        self.ctx.check(self.L.flipmi_scene_set_obstacle_inversed(self.h, obstacle_id, int(inversed)))
 
     # ---- mesh inflow sources ------------------------------------------------
     def addMeshFluidSource(self, vertices: np.ndarray, triangles: np.ndarray, velocity) -> int:

@@ -210,6 +210,14 @@ int flipmi_scene_set_obstacle(flipmi_scene *scene, int id, int enabled) {
     return FLIPMI_OK;
 }
 
+int flipmi_scene_set_obstacle_inversed(flipmi_scene *scene, int id, int inversed) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    if (id < 0 || id >= (int)s.obstacles.size()) return fail(s.ctx, FLIPMI_ERR_INVALID_ARG, "unknown obstacle");
+    s.obstacles[(size_t)id]->inversed = inversed != 0;
+    return FLIPMI_OK;
+}
+
 int flipmi_scene_add_inflow_source(flipmi_scene *scene, const float *verts, int nv, const int *tris, int nt,
                                    const double velocity[3], int *source_id) {
     if (!scene || !velocity) return FLIPMI_ERR_INVALID_ARG;

@@ -267,6 +267,7 @@ hipError_t launch_source_cell_scan(const uint8_t *nodes, int I, int J, int K, ui
                                   int *flags, int *pos, int *scan_tmp, int *count, hipStream_t s);
 hipError_t launch_source_cell_list(const int *flags, const int *pos, size_t n, int *list,
                                    hipStream_t s);
+hipError_t launch_negate(float *phi, size_t n, hipStream_t s);
 hipError_t launch_union_min(const float *phi, int ni, int nj, int nk, const int go[3], int Ni, int Nj,
                             int Nk, float *dst, hipStream_t s);
 hipError_t launch_inflow_mask(const GridGeom &g, const float *px, const float *py, const float *pz,

@@ -206,6 +206,17 @@ __global__ void k_union_min(const float *__restrict__ phi, Dims S, int goi, int 
     if (v < dst[id]) dst[id] = v;
 }
 
+__global__ void k_negate(float *phi, size_t n) {  // MeshLevelSet::negate (meshlevelset.cpp:702-708)
+    size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < n) phi[tid] = -phi[tid];
+}
+
+hipError_t launch_negate(float *phi, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_negate<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(phi, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_union_min(const float *phi, int ni, int nj, int nk, const int go[3], int Ni, int Nj,
                             int Nk, float *dst, hipStream_t s) {
     Dims S{ni, nj, nk};

@@ -937,11 +937,10 @@ void Scene::union_obstacles() {
     auto need = [&](hipError_t e) {
         if (e != hipSuccess) throw StatusError{FLIPMI_ERR_OOM, "obstacle allocation: out of device memory"};
     };
-    for (size_t q = 0; q < obstacles.size(); q++) {
-        const ObstacleMesh &ob = *obstacles[q];
-        if (!ob.enabled) continue;
+    // One obstacle's level set, unioned into the nodal field dst.
+    auto add_island = [&](const ObstacleMesh &ob, size_t q, float *dst) {
         ObstacleGrid og = obstacle_grid(ob.verts, g.I, g.J, g.K, dx);
-        if (og.empty) continue;
+        if (og.empty) return;
         const size_t nt = ob.tris.size() / 3;
         // Triangle bands (meshlevelset.cpp:878-915): box from the untranslated
         // vertices, then moved by the grid offset, like the vertices.
@@ -997,14 +996,34 @@ void Scene::union_obstacles() {
         check(launch_ray_inside(dray.p, (int)nt, r.ci, r.cj, r.ck, dx, jit, r.go, og.ni, og.nj, og.nk, 1, nullptr,
                                 phi.p, overflow.p, stream),
               "obstacle signs");
-        check(launch_union_min(phi.p, og.ni, og.nj, og.nk, og.go, g.I + 1, g.J + 1, g.K + 1, solid.p, stream),
+        check(launch_union_min(phi.p, og.ni, og.nj, og.nk, og.go, g.I + 1, g.J + 1, g.K + 1, dst, stream),
               "obstacle union");
         int ov = 0;
         check(hipMemcpyAsync(&ov, overflow.p, sizeof(int), hipMemcpyDeviceToHost, stream), "overflow");
         sync();
         if (ov != 0)
             throw StatusError{FLIPMI_ERR_INVALID_ARG, "obstacle mesh: a ray column crosses more than 64 triangles"};
+    };
+    bool inversed = false;
+    for (size_t q = 0; q < obstacles.size(); q++) {
+        if (!obstacles[q]->enabled) continue;
+        if (obstacles[q]->inversed) inversed = true;
+        else add_island(*obstacles[q], q, solid.p);
     }
+    if (!inversed) return;
+    // Inversed obstacles: their union, on a fresh full-grid level set, is
+    // negated and then unioned into the solid (fluidsimulation.cpp:2968-2984).
+    const int Ni = g.I + 1, Nj = g.J + 1, Nk = g.K + 1;
+    const size_t nn = (size_t)Ni * Nj * Nk;
+    DBuf<float> temp;
+    need(temp.alloc(nn));
+    check(launch_fill_f32(temp.p, nn, (float)((double)(Ni + Nj + Nk) * dx), stream), "fill");
+    for (size_t q = 0; q < obstacles.size(); q++)
+        if (obstacles[q]->enabled && obstacles[q]->inversed) add_island(*obstacles[q], q, temp.p);
+    check(launch_negate(temp.p, nn, stream), "negate");
+    const int origin[3] = {0, 0, 0};
+    check(launch_union_min(temp.p, Ni, Nj, Nk, origin, Ni, Nj, Nk, solid.p, stream), "inversed union");
+    sync();
 }
 
 int Scene::add_inflow(const float *verts, int nv, const int *tris, int nt, const double vel[3]) {

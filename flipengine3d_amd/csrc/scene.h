@@ -89,6 +89,7 @@ struct ObstacleMesh {
     std::vector<float> verts;
     std::vector<int> tris;
     bool enabled = true;
+    bool inversed = false;  // MeshObject::inverse() (meshobject.cpp:293-295): a container
 };
 
 struct Scene {

@@ -185,6 +185,11 @@ int flipmi_scene_add_obstacle(flipmi_scene *scene, const float *vertices_xyz, in
 /* MeshObject::enable()/disable() of an obstacle: the solid SDF is rebuilt at
  * the next substep (fluidsimulation.cpp:3028-3036). */
 int flipmi_scene_set_obstacle(flipmi_scene *scene, int obstacle_id, int enabled);
+/* MeshObject::inverse() state of an obstacle (meshobject.cpp:293-295): an
+ * inversed obstacle is a container, solid outside its mesh
+ * (fluidsimulation.cpp:2968-2984).  Like the reference, changing it does
+ * not rebuild a solid SDF that already exists. */
+int flipmi_scene_set_obstacle_inversed(flipmi_scene *scene, int obstacle_id, int inversed);
 /* Mesh inflow source: MeshFluidSource(isize, jsize, ksize, dx) +
  * updateMeshStatic(mesh) + setInflow() + setVelocity(v)
  * (meshfluidsource.cpp:32-44, 74-76, 114-116), added with

@@ -636,6 +636,8 @@ public:
     void enable() { _enabled = true; _push(); }
     void disable() { _enabled = false; _push(); }
     bool isEnabled() { return _enabled; }
+    void inverse() { _inversed = !_inversed; _push(); }  // meshobject.cpp:293-295
+    bool isInversed() { return _inversed; }
 
 private:
     friend class FluidSimulation;
@@ -651,10 +653,12 @@ private:
         _push();
     }
     void _push() {
-        if (_scene) detail::check(flipmi_scene_set_obstacle(_scene, _id, _enabled), "MeshObject");
+        if (!_scene) return;
+        detail::check(flipmi_scene_set_obstacle(_scene, _id, _enabled), "MeshObject");
+        detail::check(flipmi_scene_set_obstacle_inversed(_scene, _id, _inversed), "MeshObject");
     }
     TriangleMesh _mesh;
-    bool _enabled = true;
+    bool _enabled = true, _inversed = false;
     flipmi_scene *_scene = nullptr;
     int _id = -1;
 };

@@ -75,6 +75,7 @@ class _Base:
         getattr(L, p + "sim_add_inflow").argtypes = [_vp, _vp, _c_int, _vp, _c_int] + [_c_double] * 3
         getattr(L, p + "sim_set_inflow").argtypes = [_vp] + [_c_int] * 4
         getattr(L, p + "sim_set_outflow").argtypes = [_vp] + [_c_int] * 3
+        getattr(L, p + "sim_set_obstacle_inversed").argtypes = [_vp] + [_c_int] * 2
         getattr(L, p + "sim_inflow_sdf").argtypes = [_vp, _c_int, _vp, _vp]
 
     # -- whole-step driver -------------------------------------------------
@@ -249,6 +250,9 @@ class _SimHandle:
 
     def set_obstacle(self, oid, enabled):
         self._f("sim_set_obstacle")(self.h, oid, int(enabled))
+
+    def set_obstacle_inversed(self, oid, inversed=True):
+        self._f("sim_set_obstacle_inversed")(self.h, oid, int(inversed))
 
     def set_outflow(self, sid, outflow=True, inversed=False):
         """setOutflow() / outflowInverse() of a source."""

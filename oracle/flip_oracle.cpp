@@ -1484,6 +1484,7 @@ struct MaskGrid {
 struct Obstacle {
     TriMesh mesh;          // referenced vertices only (removeExtraneousVertices)
     bool enabled = true;
+    bool inversed = false; // MeshObject::inverse() (meshobject.cpp:293-295)
     bool hasJitter = false;
     V3 jitter;             // ray jitter of its inside test, if given
 };
@@ -1959,15 +1960,25 @@ static void step_fluid(Sim &S, double dt) {
         domain_solid_phi(S.I, S.J, S.K, S.dx, S.solid);
         // _addStaticObjectsToSDF (fluidsimulation.cpp:2932-2985): enabled
         // static obstacles unioned in order, exact band 3 (fluidsimulation.h:1580).
-        for (size_t q = 0; q < S.obstacles.size(); q++) {
-            const Obstacle &ob = S.obstacles[q];
-            if (!ob.enabled) continue;
-            V3 jit = ob.hasJitter ? ob.jitter : ray_jitter(S.seed, 100 + (int)q, 0, S.dx);
-            Idx go;
-            GridF phi;
-            int rc = obstacle_sdf(ob.mesh, S.I, S.J, S.K, S.dx, 3, jit, go, phi);
-            if (rc < 0) throw std::runtime_error("obstacle mesh must lie inside its level set's grid");
-            if (rc > 0) union_into(S.solid, phi, go);
+        // Normal obstacles first, then the inversed ones: their union on a
+        // fresh full-grid level set, negated (fluidsimulation.cpp:2955-2984).
+        GridF inv(S.I + 1, S.J + 1, S.K + 1, (float)(((S.I + 1) + (S.J + 1) + (S.K + 1)) * S.dx));
+        bool anyInversed = false;
+        for (int pass = 0; pass < 2; pass++)
+            for (size_t q = 0; q < S.obstacles.size(); q++) {
+                const Obstacle &ob = S.obstacles[q];
+                if (!ob.enabled || ob.inversed != (pass == 1)) continue;
+                V3 jit = ob.hasJitter ? ob.jitter : ray_jitter(S.seed, 100 + (int)q, 0, S.dx);
+                Idx go;
+                GridF phi;
+                int rc = obstacle_sdf(ob.mesh, S.I, S.J, S.K, S.dx, 3, jit, go, phi);
+                if (rc < 0) throw std::runtime_error("obstacle mesh must lie inside its level set's grid");
+                if (rc > 0) union_into(pass == 0 ? S.solid : inv, phi, go);
+                anyInversed = anyInversed || pass == 1;
+            }
+        if (anyInversed) {
+            for (float &v : inv.a) v = -v;
+            union_into(S.solid, inv, Idx{0, 0, 0});
         }
         S.solidU = GridF(S.I + 1, S.J, S.K, 0.0f);
         S.solidV = GridF(S.I, S.J + 1, S.K, 0.0f);
@@ -2312,6 +2323,10 @@ int orc_sim_add_obstacle(void *h, const float *verts, int nv, const int *tris, i
     S->obstacles.push_back(ob);
     S->solidUpToDate = false;
     return (int)S->obstacles.size() - 1;
+}
+// MeshObject::inverse() state (no rebuild, as in the reference).
+void orc_sim_set_obstacle_inversed(void *h, int id, int inversed) {
+    ((Sim *)h)->obstacles[id].inversed = inversed != 0;
 }
 // MeshObject::enable()/disable(): the solid is rebuilt at the next substep.
 void orc_sim_set_obstacle(void *h, int id, int enabled) {

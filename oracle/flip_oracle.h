@@ -89,6 +89,7 @@ void orc_ray_jitter(uint64_t seed, int id, int which, double dx, float *out);
 int orc_sim_add_obstacle(void *h, const float *verts, int nv, const int *tris, int nt,
                          const float *jitter);
 void orc_sim_set_obstacle(void *h, int id, int enabled);
+void orc_sim_set_obstacle_inversed(void *h, int id, int inversed);
 int orc_inflow_build(const float *verts, int nv, const int *tris, int nt, int I, int J, int K,
                      double dx, uint64_t seed, int id, int *dims, float *phi, uint8_t *cells);
 

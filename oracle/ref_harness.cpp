@@ -339,6 +339,11 @@ int ref_sim_add_obstacle(void *hp, const float *verts, int nv, const int *tris, 
     return (int)h->obstacles.size() - 1;
 }
 
+void ref_sim_set_obstacle_inversed(void *hp, int id, int inversed) {
+    MeshObject *ob = ((RefSim *)hp)->obstacles[id];
+    if ((inversed != 0) != ob->isInversed()) ob->inverse();
+}
+
 void ref_sim_set_obstacle(void *hp, int id, int enabled) {
     MeshObject *ob = ((RefSim *)hp)->obstacles[id];
     if (enabled) ob->enable(); else ob->disable();
