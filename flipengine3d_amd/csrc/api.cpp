@@ -476,12 +476,12 @@ int flipmi_p2g(flipmi_ctx *ctx, int I, int J, int K, double dx, const float *pos
         alloc_dev(du, (size_t)(I + 1) * J * K); alloc_dev(dv, (size_t)I * (J + 1) * K);
         alloc_dev(dw, (size_t)I * J * (K + 1));
         alloc_dev(dvu, du.n); alloc_dev(dvv, dv.n); alloc_dev(dvw, dw.n);
-        hipck(launch_p2g(g, 0, sp.sorted[0].p, sp.sorted[1].p, sp.sorted[2].p, sp.sorted[3].p, sp.offset.p,
-                         radius, du.p, dvu.p, s), "p2g");
-        hipck(launch_p2g(g, 1, sp.sorted[0].p, sp.sorted[1].p, sp.sorted[2].p, sp.sorted[4].p, sp.offset.p,
-                         radius, dv.p, dvv.p, s), "p2g");
-        hipck(launch_p2g(g, 2, sp.sorted[0].p, sp.sorted[1].p, sp.sorted[2].p, sp.sorted[5].p, sp.offset.p,
-                         radius, dw.p, dvw.p, s), "p2g");
+        const float *svel[3] = {sp.sorted[3].p, sp.sorted[4].p, sp.sorted[5].p};
+        float *outs[3] = {du.p, dv.p, dw.p};
+        uint8_t *valids[3] = {dvu.p, dvv.p, dvw.p};
+        hipck(launch_p2g3(g, sp.sorted[0].p, sp.sorted[1].p, sp.sorted[2].p, svel, sp.offset.p, radius, outs,
+                          valids, s),
+              "p2g");
         download(u, du, du.n, s); download(v, dv, dv.n, s); download(w, dw, dw.n, s);
         download(vu, dvu, dvu.n, s); download(vv, dvv, dvv.n, s); download(vw, dvw, dvw.n, s);
         hipck(hipStreamSynchronize(s), "sync");

@@ -72,9 +72,10 @@ hipError_t launch_liquid_postprocess(const GridGeom &g, const float *solid_phi_n
                                      hipStream_t s);
 
 // ---- P2G (k_p2g.hip) ----
-hipError_t launch_p2g(const GridGeom &g, int comp, const float *sx, const float *sy,
-                      const float *sz, const float *svel, const int *offset, double radius,
-                      float *out, uint8_t *valid, hipStream_t s);
+// P2G of the three MAC components in one launch.
+hipError_t launch_p2g3(const GridGeom &g, const float *sx, const float *sy, const float *sz,
+                       const float *const svel[3], const int *offset, double radius, float *const out[3],
+                       uint8_t *const valid[3], hipStream_t s);
 
 // ---- grid passes (k_grid.hip) ----
 hipError_t launch_extrapolate(int ni, int nj, int nk, float *grid, const uint8_t *valid,

@@ -9,11 +9,12 @@
 // 10^3 face block that owns f and assigns particles to blocks by their search
 // box; both are replayed here, so every per-pair weight is bit-identical.
 //
-// Instead of scattering with atomics, each thread owns one face and gathers
-// the particles of the 2x3x3 neighbouring cells from the cell-sorted arrays
-// in a fixed order (cells k, j, i ascending; particles ascending inside a
-// cell).  The sum is deterministic; it differs from the reference's
-// ascending-particle-index summation order only by float rounding.
+// Instead of scattering with atomics, each thread owns the U, V and W faces
+// of one index and gathers the particles of the neighbouring cells from the
+// cell-sorted arrays in a fixed order (cells k, j, i ascending; particles
+// ascending inside a cell).  The sum is deterministic; it differs from the
+// reference's ascending-particle-index summation order only by float
+// rounding.
 #include "device_math.h"
 #include "flipmi_kernels.h"
 
@@ -34,20 +35,30 @@ __device__ __forceinline__ bool p2g_point_in_block(F3 p, float sr, float blockdx
     return bi >= lo.i && bi <= hi.i && bj >= lo.j && bj <= hi.j && bk >= lo.k && bk <= hi.k;
 }
 
-template <int COMP>
-__global__ __launch_bounds__(256) void k_p2g(GridGeom g, const float *__restrict__ sx,
-                                             const float *__restrict__ sy,
-                                             const float *__restrict__ sz,
-                                             const float *__restrict__ svel,
-                                             const int *__restrict__ offset, float radius_f,
-                                             double radius, float *out, uint8_t *valid) {
-    const int ni = g.I + (COMP == 0), nj = g.J + (COMP == 1), nk = g.K + (COMP == 2);
+// All three components in one pass: thread (i, j, k) owns the U, V and W
+// faces of that index (those that exist) and walks the 3x3x3 cells around
+// it once.  A particle from a cell outside a component's 2x3x3 set lies at
+// least dx > r from that face, so it fails the distance test: every face
+// sums exactly the particles, in exactly the order, of the per-component
+// kernel.
+__global__ __launch_bounds__(256) void k_p2g3(GridGeom g, const float *__restrict__ sx,
+                                              const float *__restrict__ sy,
+                                              const float *__restrict__ sz,
+                                              const float *__restrict__ su,
+                                              const float *__restrict__ sv,
+                                              const float *__restrict__ sw,
+                                              const int *__restrict__ offset, float radius_f,
+                                              double radius, float *ou, float *ov, float *ow,
+                                              uint8_t *vu, uint8_t *vv, uint8_t *vw) {
+    const int ni = g.I + 1, nj = g.J + 1, nk = g.K + 1;
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (size_t)ni * nj * nk) return;
     int i = (int)(tid % ni), j = (int)((tid / ni) % nj), k = (int)(tid / ((size_t)ni * nj));
+    const bool has[3] = {j < g.J && k < g.K, i < g.I && k < g.K, i < g.I && j < g.J};
+    if (!has[0] && !has[1] && !has[2]) return;
 
     float hdx = (float)(0.5 * g.dx);  // velocityadvector.cpp:153-164
-    F3 off = COMP == 0 ? f3(0.0f, hdx, hdx) : (COMP == 1 ? f3(hdx, 0.0f, hdx) : f3(hdx, hdx, 0.0f));
+    const F3 off[3] = {f3(0.0f, hdx, hdx), f3(hdx, 0.0f, hdx), f3(hdx, hdx, 0.0f)};
     float eps = 1e-6f;
     float sr = (float)(radius + (double)eps);
     double chunkdx = g.dx * kChunk;
@@ -63,58 +74,63 @@ __global__ __launch_bounds__(256) void k_p2g(GridGeom g, const float *__restrict
     F3 boff = idx_to_pos(bi, bj, bk, kChunk * g.dx);
     F3 gpos = idx_to_pos(li, lj, lk, g.dx);
 
-    // Cells whose particles can reach this face.
-    int ci0 = COMP == 0 ? i - 1 : i - 1, ci1 = COMP == 0 ? i : i + 1;
-    int cj0 = j - 1, cj1 = COMP == 1 ? j : j + 1;
-    int ck0 = k - 1, ck1 = COMP == 2 ? k : k + 1;
-    ci0 = max(ci0, 0); cj0 = max(cj0, 0); ck0 = max(ck0, 0);
-    ci1 = min(ci1, g.I - 1); cj1 = min(cj1, g.J - 1); ck1 = min(ck1, g.K - 1);
+    int ci0 = max(i - 1, 0), ci1 = min(i + 1, g.I - 1);
+    int cj0 = max(j - 1, 0), cj1 = min(j + 1, g.J - 1);
+    int ck0 = max(k - 1, 0), ck1 = min(k + 1, g.K - 1);
+    const float *vel[3] = {su, sv, sw};
 
-    float sum = 0.0f, wsum = 0.0f;
+    float sum[3] = {0.0f, 0.0f, 0.0f}, wsum[3] = {0.0f, 0.0f, 0.0f};
     for (int ck = ck0; ck <= ck1; ck++) {
         for (int cj = cj0; cj <= cj1; cj++) {
             size_t row = (size_t)g.I * ((size_t)cj + (size_t)g.J * ck);
             int b = offset[row + ci0], e = offset[row + ci1 + 1];
             for (int q = b; q < e; q++) {
-                F3 p = f3(sx[q], sy[q], sz[q]) - off;
-                F3 pl = p - boff;
-                F3 v = gpos - pl;
-                float d2 = dot3(v, v);
-                // A pair adds to the face only if it is within the radius and
-                // the reference visits it (the particle is assigned to the
-                // face's block and the face lies in the particle's clamped
-                // local box).  The three tests are independent, so the cheap
-                // distance test goes first; the sums are unchanged.
-                if (!(d2 < rsq)) continue;
-                if (!p2g_point_in_block(p, sr, blockdx, inv_blockdx, bi, bj, bk)) continue;
-                I3 gmin = pos_to_index((double)(pl.x - sr), (double)(pl.y - sr), (double)(pl.z - sr), g.invdx);
-                I3 gmax = pos_to_index((double)(pl.x + sr), (double)(pl.y + sr), (double)(pl.z + sr), g.invdx);
-                gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
-                gmax.i = min(gmax.i, kChunk - 1); gmax.j = min(gmax.j, kChunk - 1);
-                gmax.k = min(gmax.k, kChunk - 1);
-                if (li < gmin.i || li > gmax.i || lj < gmin.j || lj > gmax.j || lk < gmin.k ||
-                    lk > gmax.k)
-                    continue;
-                float wt = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
-                sum += wt * svel[q];
-                wsum += wt;
+                const F3 p0 = f3(sx[q], sy[q], sz[q]);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    if (!has[c]) continue;
+                    F3 p = p0 - off[c];
+                    F3 pl = p - boff;
+                    F3 v = gpos - pl;
+                    float d2 = dot3(v, v);
+                    if (!(d2 < rsq)) continue;
+                    if (!p2g_point_in_block(p, sr, blockdx, inv_blockdx, bi, bj, bk)) continue;
+                    I3 gmin = pos_to_index((double)(pl.x - sr), (double)(pl.y - sr), (double)(pl.z - sr), g.invdx);
+                    I3 gmax = pos_to_index((double)(pl.x + sr), (double)(pl.y + sr), (double)(pl.z + sr), g.invdx);
+                    gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
+                    gmax.i = min(gmax.i, kChunk - 1); gmax.j = min(gmax.j, kChunk - 1);
+                    gmax.k = min(gmax.k, kChunk - 1);
+                    if (li < gmin.i || li > gmax.i || lj < gmin.j || lj > gmax.j || lk < gmin.k ||
+                        lk > gmax.k)
+                        continue;
+                    float wt = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
+                    sum[c] += wt * vel[c][q];
+                    wsum[c] += wt;
+                }
             }
         }
     }
-    float u = sum;
-    if (wsum > eps) u = sum / wsum;
-    out[tid] = u;
-    valid[tid] = wsum > eps ? 1 : 0;
+    float *outs[3] = {ou, ov, ow};
+    uint8_t *valids[3] = {vu, vv, vw};
+    const int cni[3] = {g.I + 1, g.I, g.I}, cnj[3] = {g.J, g.J + 1, g.J};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (!has[c]) continue;
+        size_t f = (size_t)i + (size_t)cni[c] * ((size_t)j + (size_t)cnj[c] * k);
+        float u = sum[c];
+        if (wsum[c] > eps) u = sum[c] / wsum[c];
+        outs[c][f] = u;
+        valids[c][f] = wsum[c] > eps ? 1 : 0;
+    }
 }
 
-hipError_t launch_p2g(const GridGeom &g, int comp, const float *sx, const float *sy,
-                      const float *sz, const float *svel, const int *offset, double radius,
-                      float *out, uint8_t *valid, hipStream_t s) {
-    size_t n = (size_t)(g.I + (comp == 0)) * (g.J + (comp == 1)) * (g.K + (comp == 2));
-    unsigned nb = (unsigned)((n + 255) / 256);
-    if (comp == 0) k_p2g<0><<<nb, 256, 0, s>>>(g, sx, sy, sz, svel, offset, (float)radius, radius, out, valid);
-    else if (comp == 1) k_p2g<1><<<nb, 256, 0, s>>>(g, sx, sy, sz, svel, offset, (float)radius, radius, out, valid);
-    else k_p2g<2><<<nb, 256, 0, s>>>(g, sx, sy, sz, svel, offset, (float)radius, radius, out, valid);
+hipError_t launch_p2g3(const GridGeom &g, const float *sx, const float *sy, const float *sz,
+                       const float *const svel[3], const int *offset, double radius, float *const out[3],
+                       uint8_t *const valid[3], hipStream_t s) {
+    size_t n = (size_t)(g.I + 1) * (g.J + 1) * (g.K + 1);
+    k_p2g3<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, sx, sy, sz, svel[0], svel[1], svel[2], offset,
+                                                       (float)radius, radius, out[0], out[1], out[2],
+                                                       valid[0], valid[1], valid[2]);
     return hipGetLastError();
 }
 

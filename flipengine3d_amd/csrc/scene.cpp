@@ -1343,12 +1343,11 @@ void Scene::step_fluid(double dt) {
     size_t nU = u.n, nV = v.n, nW = w.n;
     if (n > 0) {
         begin_stage(FLIPMI_STAGE_P2G, &ev);
-        check(launch_p2g(g, 0, sorted[0].p, sorted[1].p, sorted[2].p, sorted[3].p, offset.p, radius,
-                         u.p, vu.p, stream), "p2g u");
-        check(launch_p2g(g, 1, sorted[0].p, sorted[1].p, sorted[2].p, sorted[4].p, offset.p, radius,
-                         v.p, vv.p, stream), "p2g v");
-        check(launch_p2g(g, 2, sorted[0].p, sorted[1].p, sorted[2].p, sorted[5].p, offset.p, radius,
-                         w.p, vw.p, stream), "p2g w");
+        const float *svel[3] = {sorted[3].p, sorted[4].p, sorted[5].p};
+        float *outs[3] = {u.p, v.p, w.p};
+        uint8_t *valids[3] = {vu.p, vv.p, vw.p};
+        check(launch_p2g3(g, sorted[0].p, sorted[1].p, sorted[2].p, svel, offset.p, radius, outs, valids, stream),
+              "p2g");
         end_stage(FLIPMI_STAGE_P2G, ev);
         extrapolate_all(layers);
     } else {
