@@ -356,10 +356,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_view(const void *base, 
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
 }
 
-// sched_barrier mask: ALU (0x1-0x8) and LDS (0x80) instructions may be
-// scheduled across the barrier, vector memory instructions may not.
-constexpr int kAluAndLdsCross = 0x8F;
-
 // Tagged records.  A record carries a value and the 32-bit epoch of the
 // launch that wrote it, in 64-bit words written and read as single-copy-
 // atomic accesses, so a reader that sees the current epoch in every word has

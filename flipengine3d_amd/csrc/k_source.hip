@@ -15,9 +15,12 @@
 //   * new particles at the empty subcells (FluidSimulation::_addNewFluidCells,
 //     fluidsimulation.cpp:4397-4431, 4523-4562), appended in the reference's
 //     order (source cell list order, then subcell offset order) by a scan.
-// The velocity constraints of particles and faces live in the particle step
-// and body-force kernels (k_particles.hip, k_grid.hip) through
-// inflow_constrains().
+// Outflow sources remove particles through keep flags and the stable
+// compaction (k_outflow_keep).  The velocity constraints of particles and
+// faces live in the particle step and body-force kernels (k_particles.hip,
+// k_grid.hip) through inflow_holds_particle() and inflow_holds_face()
+// (inflow_device.h).  Obstacles reuse the set-up kernels for their level
+// sets and k_union_min / k_negate to join them to the solid SDF.
 #include "device_math.h"
 #include "flipmi_kernels.h"
 
