@@ -54,6 +54,10 @@ struct SortBuffers {
     int *order;      // [n] particle ids grouped by cell, ascending within a cell
     int *scan_tmp;   // scratch for the scan
     size_t scan_tmp_elems;
+    // When set: a flag raised if some cell holds more than 250 particles;
+    // without that, only the counts and offsets are formed (order is left
+    // as it was).
+    int *over = nullptr;
 };
 size_t scan_scratch_elems(int n);
 hipError_t launch_exclusive_scan(const int *in, int *out, int n, int *tmp, hipStream_t s);
@@ -311,7 +315,8 @@ hipError_t launch_particle_step(const GridGeom &g, const ParticleStepArgs &a, hi
 hipError_t launch_max_speed(const float *vx, const float *vy, const float *vz, int n,
                             unsigned int *out_bits, hipStream_t s);
 hipError_t launch_cell_rank_cells(const int *offset, int ncell, const int *order,
-                                  const uint8_t *solid_removed, int *rank, hipStream_t s);
+                                  const uint8_t *solid_removed, const int *over, int *rank,
+                                  hipStream_t s);
 hipError_t launch_keep_flags(const float *vx, const float *vy, const float *vz, int n,
                              const uint8_t *solid_removed, const int *rank, double maxspeedsq,
                              int *keep, hipStream_t s);

@@ -270,11 +270,13 @@ hipError_t launch_max_speed(const float *vx, const float *vy, const float *vz, i
 // Per-cell rank among particles that passed the solid test, in particle
 // order (the reference's running count grid, fluidsimulation.cpp:4330-4343).
 // Only cells holding more than the 250-particle cap need ranks; a thread
-// walks its cell's ascending id list.
+// walks its cell's ascending id list.  Nothing runs unless the sort raised
+// `over` (then the order exists).
 __global__ void k_cell_rank(const int *__restrict__ offset, int ncell, const int *__restrict__ order,
-                            const uint8_t *__restrict__ solid_removed, int *rank) {
+                            const uint8_t *__restrict__ solid_removed, const int *__restrict__ over,
+                            int *rank) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    if (c >= ncell || *over == 0) return;
     int b = offset[c], e = offset[c + 1];
     if (e - b <= 250) return;
     int r = 0;
@@ -286,8 +288,9 @@ __global__ void k_cell_rank(const int *__restrict__ offset, int ncell, const int
 }
 
 hipError_t launch_cell_rank_cells(const int *offset, int ncell, const int *order,
-                                  const uint8_t *solid_removed, int *rank, hipStream_t s) {
-    k_cell_rank<<<(ncell + 255) / 256, 256, 0, s>>>(offset, ncell, order, solid_removed, rank);
+                                  const uint8_t *solid_removed, const int *over, int *rank,
+                                  hipStream_t s) {
+    k_cell_rank<<<(ncell + 255) / 256, 256, 0, s>>>(offset, ncell, order, solid_removed, over, rank);
     return hipGetLastError();
 }
 

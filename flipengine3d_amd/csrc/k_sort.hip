@@ -97,7 +97,7 @@ hipError_t launch_exclusive_scan(const int *in, int *out, int n, int *tmp, hipSt
 // distinct; the within-cell order is fixed afterwards by particle id.
 __global__ __launch_bounds__(256) void k_cell_key(GridGeom g, const float *__restrict__ px,
                                                   const float *__restrict__ py, const float *__restrict__ pz,
-                                                  int n, int *cell, int *slot, int *count) {
+                                                  int n, int *cell, int *slot, int *count, int *over) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     int id = -1;
@@ -116,7 +116,10 @@ __global__ __launch_bounds__(256) void k_cell_key(GridGeom g, const float *__res
         const bool mine = pending && id == key;
         const unsigned long long peers = __ballot(mine);
         int base = 0;
-        if (lane == leader) base = atomicAdd(&count[key], __popcll(peers));
+        if (lane == leader) {
+            base = atomicAdd(&count[key], __popcll(peers));
+            if (over && base + __popcll(peers) > 250) atomicOr(over, 1);
+        }
         base = __shfl(base, leader, 64);
         if (mine) {
             sl = base + __popcll(peers & ((1ull << lane) - 1ull));
@@ -130,18 +133,18 @@ __global__ __launch_bounds__(256) void k_cell_key(GridGeom g, const float *__res
 }
 
 __global__ void k_scatter(const int *__restrict__ cell, const int *__restrict__ slot,
-                          const int *__restrict__ offset, int n, int *order) {
+                          const int *__restrict__ offset, int n, const int *over, int *order) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (over && *over == 0)) return;
     int c = cell[i];
     if (c < 0) return;
     order[offset[c] + slot[i]] = i;
 }
 
 // Ascending ids inside each cell (cells hold a handful of particles).
-__global__ void k_sort_within_cell(const int *__restrict__ offset, int ncell, int *order) {
+__global__ void k_sort_within_cell(const int *__restrict__ offset, int ncell, const int *over, int *order) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    if (c >= ncell || (over && *over == 0)) return;
     int b = offset[c], e = offset[c + 1];
     for (int p = b + 1; p < e; p++) {
         int v = order[p];
@@ -163,12 +166,14 @@ hipError_t launch_sort_by_cell(const GridGeom &g, const float *px, const float *
     int ncell = g.I * g.J * g.K;
     hipError_t e = launch_fill_i32(sb.count, (size_t)ncell, 0, s);
     if (e != hipSuccess) return e;
-    if (n > 0) k_cell_key<<<(n + 255) / 256, 256, 0, s>>>(g, px, py, pz, n, sb.cell, sb.slot, sb.count);
+    if (sb.over && (e = launch_fill_i32(sb.over, 1, 0, s)) != hipSuccess) return e;
+    if (n > 0)
+        k_cell_key<<<(n + 255) / 256, 256, 0, s>>>(g, px, py, pz, n, sb.cell, sb.slot, sb.count, sb.over);
     e = launch_exclusive_scan(sb.count, sb.offset, ncell, sb.scan_tmp, s);
     if (e != hipSuccess) return e;
     k_set_last<<<1, 1, 0, s>>>(sb.offset, ncell, sb.count);
-    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, s>>>(sb.cell, sb.slot, sb.offset, n, sb.order);
-    k_sort_within_cell<<<(ncell + 255) / 256, 256, 0, s>>>(sb.offset, ncell, sb.order);
+    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, s>>>(sb.cell, sb.slot, sb.offset, n, sb.over, sb.order);
+    k_sort_within_cell<<<(ncell + 255) / 256, 256, 0, s>>>(sb.offset, ncell, sb.over, sb.order);
     return hipGetLastError();
 }
 

@@ -363,7 +363,7 @@ void Scene::alloc_all() {
     need(count.alloc(nc)); need(offset.alloc(nc + 1));
     size_t st = std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc));
     need(scan_tmp.alloc(st));
-    need(new_count.alloc(1)); need(speed_hist.alloc(64)); need(max_speed_bits.alloc(1));
+    need(new_count.alloc(1)); need(over_cap.alloc(1)); need(speed_hist.alloc(64)); need(max_speed_bits.alloc(1));
     need(solid.alloc(nn));
     need(wU.alloc(nU)); need(wV.alloc(nV)); need(wW.alloc(nW)); need(wC.alloc(nc));
     need(sU.alloc(nU)); need(sV.alloc(nV)); need(sW.alloc(nW));
@@ -493,7 +493,7 @@ size_t Scene::device_bytes() const {
     size_t b = 0;
     for (int a = 0; a < 6; a++) b += part[a].bytes() + part_alt[a].bytes() + sorted[a].bytes();
     b += cell.bytes() + slot.bytes() + count.bytes() + offset.bytes() + order.bytes() + scan_tmp.bytes();
-    b += keep.bytes() + keep_pos.bytes() + rank.bytes() + new_count.bytes() + solid_removed.bytes();
+    b += keep.bytes() + keep_pos.bytes() + rank.bytes() + new_count.bytes() + over_cap.bytes() + solid_removed.bytes();
     b += speed_hist.bytes() + max_speed_bits.bytes();
     b += solid.bytes() + wU.bytes() + wV.bytes() + wW.bytes() + wC.bytes() + sU.bytes() + sV.bytes() + sW.bytes();
     b += near_a.bytes() + near_b.bytes() + liquid.bytes() + u.bytes() + v.bytes() + w.bytes();
@@ -569,8 +569,8 @@ void Scene::ensure_solid() {
     solid_ready = true;
 }
 
-void Scene::sort_particles() {
-    SortBuffers sb{cell.p, slot.p, count.p, offset.p, order.p, scan_tmp.p, scan_tmp.n};
+void Scene::sort_particles(int *over) {
+    SortBuffers sb{cell.p, slot.p, count.p, offset.p, order.p, scan_tmp.p, scan_tmp.n, over};
     check(launch_sort_by_cell(g, part[0].p, part[1].p, part[2].p, n, sb, stream), "sort");
 }
 
@@ -636,9 +636,11 @@ void Scene::advance_particles(double dt) {
     int *hist = (int *)host_stage;
     check(hipMemcpyAsync(hist, speed_hist.p, sizeof(int) * desc.max_substeps, hipMemcpyDeviceToHost, stream),
           "hist");
-    sort_particles();
+    // The per-cell order is built only when some cell is over the cap.
+    sort_particles(over_cap.p);
     check(launch_fill_i32(rank.p, (size_t)std::max(n, 1), -1, stream), "fill");
-    check(launch_cell_rank_cells(offset.p, g.I * g.J * g.K, order.p, solid_removed.p, rank.p, stream),
+    check(launch_cell_rank_cells(offset.p, g.I * g.J * g.K, order.p, solid_removed.p, over_cap.p, rank.p,
+                                 stream),
           "cell rank");
     sync();
     // _getMarkerParticleSpeedLimit (fluidsimulation.cpp:4298-4322).

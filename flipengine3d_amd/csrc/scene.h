@@ -103,7 +103,7 @@ struct Scene {
     int n = 0, cap = 0;
     DBuf<float> part[6], part_alt[6], sorted[6];
     DBuf<int> cell, slot, count, offset, order, scan_tmp;
-    DBuf<int> keep, keep_pos, rank, new_count;
+    DBuf<int> keep, keep_pos, rank, new_count, over_cap;
     DBuf<uint8_t> solid_removed;
     DBuf<int> speed_hist;
     DBuf<unsigned> max_speed_bits;
@@ -187,7 +187,7 @@ struct Scene {
     void ensure_solid();
     void pressure_solve(double dt);
     void advance_particles(double dt);
-    void sort_particles();
+    void sort_particles(int *over = nullptr);
     void extrapolate_all(int layers);
     // Mesh inflow sources: add (checks the mesh), build the device state of
     // every source, their device views, and the emission at the end of a
