@@ -112,7 +112,7 @@ EXPORTED = [
     "flipmi_scene_set_outflow_source", "flipmi_scene_set_obstacle_inversed",
     "flipmi_scene_get_grid", "flipmi_scene_stage_times",
     "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
-    "flipmi_liquid_sdf_postprocess", "flipmi_p2g",
+    "flipmi_liquid_sdf_postprocess", "flipmi_p2g", "flipmi_removal_counts",
     "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
     "flipmi_trilinear_velocity", "flipmi_server_desc_default", "flipmi_server_create", "flipmi_server_submit",
     "flipmi_server_wait", "flipmi_server_take", "flipmi_server_get_stats", "flipmi_server_destroy",
@@ -160,6 +160,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_domain_solid.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double] + [_vp] * 5
     L.flipmi_liquid_sdf.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_double, _vp, _vp]
     L.flipmi_liquid_sdf_postprocess.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp]
+    L.flipmi_removal_counts.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _vp, _vp]
     L.flipmi_p2g.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _c_i64, _c_double] + [_vp] * 6
     L.flipmi_extrapolate.argtypes = [_vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int]
     L.flipmi_pressure_solve.argtypes = [_vp, P(PressureArgs), P(PressureReport)]
@@ -247,6 +248,15 @@ class Context:
         self.check(self.L.flipmi_liquid_sdf(self.h, isize, jsize, ksize, dx, _ptr(pos), pos.shape[0], radius,
                                             _ptr(sp), _ptr(phi)))
         return phi
+
+    def removal_counts(self, isize, jsize, ksize, dx, pos, removed=None):
+        """Count grid of the removal pass's counting loop (int32 K x J x I)."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        rm = None if removed is None else np.ascontiguousarray(removed, np.uint8)
+        out = np.zeros((ksize, jsize, isize), np.int32)
+        self.check(self.L.flipmi_removal_counts(self.h, isize, jsize, ksize, dx, _ptr(pos), pos.shape[0],
+                                                _ptr(rm), _ptr(out)))
+        return out
 
     def liquid_sdf_postprocess(self, isize, jsize, ksize, dx, phi, solid_phi):
         out = np.ascontiguousarray(phi, np.float32).copy()
@@ -340,6 +350,7 @@ class FluidSimulation:
         9: lambda I, J, K: ((K, J, I + 1), np.uint8),
         10: lambda I, J, K: ((K, J + 1, I), np.uint8),
         11: lambda I, J, K: ((K + 1, J, I), np.uint8),
+        12: lambda I, J, K: ((K, J, I), np.int32),
     }
 
     def __init__(self, isize: int, jsize: int, ksize: int, dx: float, ctx: Optional[Context] = None,

@@ -339,6 +339,20 @@ int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out) {
             case FLIPMI_GRID_VALID_U: src = s.vu.p; bytes = s.vu.bytes(); break;
             case FLIPMI_GRID_VALID_V: src = s.vv.p; bytes = s.vv.bytes(); break;
             case FLIPMI_GRID_VALID_W: src = s.vw.p; bytes = s.vw.bytes(); break;
+            case FLIPMI_GRID_CELL_COUNT: {
+                const size_t nc = (size_t)s.g.I * s.g.J * s.g.K;
+                if (!s.removal_counts_ready) {
+                    std::memset(out, 0, sizeof(int) * nc);
+                    return;
+                }
+                DBuf<int> counts;
+                alloc_dev(counts, nc);
+                hipck(launch_removal_counts(s.g, s.count.p, s.removed_in_cell.p, 250, counts.p, s.stream),
+                      "removal counts");
+                download((int *)out, counts, nc, s.stream);
+                hipck(hipStreamSynchronize(s.stream), "sync");
+                return;
+            }
             default: throw StatusError{FLIPMI_ERR_INVALID_ARG, "unknown grid id"};
         }
         hipck(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, s.stream), "download");
@@ -441,6 +455,33 @@ int flipmi_liquid_sdf(flipmi_ctx *ctx, int I, int J, int K, double dx, const flo
                                 solid.p, phi.p, s),
               "liquid sdf");
         download(phi_out, phi, phi.n, s);
+        hipck(hipStreamSynchronize(s), "sync");
+    });
+}
+
+int flipmi_removal_counts(flipmi_ctx *ctx, int I, int J, int K, double dx, const float *pos, int64_t n,
+                          const uint8_t *removed, int32_t *counts_out) {
+    if (!ctx || !counts_out || (n > 0 && !pos) || n < 0) return FLIPMI_ERR_INVALID_ARG;
+    return guarded(&ctx->c, [&] {
+        GridGeom g = make_geom(I, J, K, dx);
+        hipStream_t s = ctx->c.stream;
+        SortedParticles sp;
+        sp.build(g, pos, nullptr, n, s);
+        const size_t nc = (size_t)I * J * K;
+        DBuf<uint8_t> rm;
+        DBuf<int> ric, out;
+        if (removed) {
+            upload(rm, removed, (size_t)n, s);
+        } else {
+            alloc_dev(rm, (size_t)std::max<int64_t>(n, 1));
+            hipck(launch_fill_u8(rm.p, rm.n, 0, s), "fill");
+        }
+        alloc_dev(ric, nc);
+        alloc_dev(out, nc);
+        hipck(launch_removed_in_cell(g, sp.raw[0].p, sp.raw[1].p, sp.raw[2].p, rm.p, (int)n, ric.p, s),
+              "removed in cell");
+        hipck(launch_removal_counts(g, sp.count.p, ric.p, 250, out.p, s), "removal counts");
+        download(counts_out, out, nc, s);
         hipck(hipStreamSynchronize(s), "sync");
     });
 }

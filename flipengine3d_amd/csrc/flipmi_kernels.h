@@ -64,6 +64,12 @@ hipError_t launch_exclusive_scan(const int *in, int *out, int n, int *tmp, hipSt
 hipError_t launch_sort_by_cell(const GridGeom &g, const float *px, const float *py,
                                const float *pz, int n, SortBuffers &sb, hipStream_t s);
 // Gathers the first *nsorted (= offset[ncell]) entries of the cell order.
+// Count grid of the removal pass: removed_in_cell[c] = solid-removed
+// particles in cell c (cleared first), then out = min(count - removed, cap).
+hipError_t launch_removed_in_cell(const GridGeom &g, const float *px, const float *py, const float *pz,
+                                  const uint8_t *removed, int n, int *removed_in_cell, hipStream_t s);
+hipError_t launch_removal_counts(const GridGeom &g, const int *count, const int *removed_in_cell, int cap,
+                                 int *out, hipStream_t s);
 hipError_t launch_gather_sorted(const int *order, const int *nsorted, int n,
                                 const float *const src[6], float *const dst[6], hipStream_t s);
 

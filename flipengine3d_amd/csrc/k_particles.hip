@@ -276,7 +276,7 @@ __global__ void k_cell_rank(const int *__restrict__ offset, int ncell, const int
                             const uint8_t *__restrict__ solid_removed, const int *__restrict__ over,
                             int *rank) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell || *over == 0) return;
+    if (c >= ncell || (over && *over == 0)) return;
     int b = offset[c], e = offset[c + 1];
     if (e - b <= 250) return;
     int r = 0;

@@ -177,6 +177,46 @@ hipError_t launch_sort_by_cell(const GridGeom &g, const float *px, const float *
     return hipGetLastError();
 }
 
+// The count grid of FluidSimulation::_removeMarkerParticles
+// (fluidsimulation.cpp:4325-4343): particles are counted in order, skipping
+// those removed by the solid test, and a cell stops counting at the cap
+// (later particles of that cell are removed before they are added).  So a
+// cell's entry is min(particles in the cell - solid-removed ones, cap),
+// whatever the order.  `count` is the cell histogram of all particles from
+// the sort; the solid-removed particles are subtracted per cell.
+__global__ void k_removed_in_cell(GridGeom g, const float *__restrict__ px, const float *__restrict__ py,
+                                  const float *__restrict__ pz, const uint8_t *__restrict__ removed, int n,
+                                  int *removed_in_cell) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !removed[i]) return;
+    I3 c = pos_to_index(f3(px[i], py[i], pz[i]), g.invdx);
+    if (c.i < 0 || c.j < 0 || c.k < 0 || c.i >= g.I || c.j >= g.J || c.k >= g.K) return;
+    atomicAdd(&removed_in_cell[c.i + g.I * (c.j + g.J * c.k)], 1);
+}
+
+__global__ void k_removal_counts(const int *__restrict__ count, const int *__restrict__ removed_in_cell,
+                                 size_t ncell, int cap, int *out) {
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int m = count[c] - removed_in_cell[c];
+    out[c] = m < cap ? m : cap;
+}
+
+hipError_t launch_removed_in_cell(const GridGeom &g, const float *px, const float *py, const float *pz,
+                                  const uint8_t *removed, int n, int *removed_in_cell, hipStream_t s) {
+    hipError_t e = launch_fill_i32(removed_in_cell, (size_t)g.I * g.J * g.K, 0, s);
+    if (e != hipSuccess || n <= 0) return e;
+    k_removed_in_cell<<<(n + 255) / 256, 256, 0, s>>>(g, px, py, pz, removed, n, removed_in_cell);
+    return hipGetLastError();
+}
+
+hipError_t launch_removal_counts(const GridGeom &g, const int *count, const int *removed_in_cell, int cap,
+                                 int *out, hipStream_t s) {
+    size_t nc = (size_t)g.I * g.J * g.K;
+    k_removal_counts<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(count, removed_in_cell, nc, cap, out);
+    return hipGetLastError();
+}
+
 __global__ void k_gather6(const int *__restrict__ order, const int *__restrict__ nsorted,
                           const float *__restrict__ a0,
                           const float *__restrict__ a1, const float *__restrict__ a2,

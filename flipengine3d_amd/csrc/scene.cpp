@@ -360,7 +360,7 @@ void Scene::alloc_all() {
     need(cell.alloc(pc)); need(slot.alloc(pc)); need(order.alloc(pc));
     need(keep.alloc(pc)); need(keep_pos.alloc(pc)); need(rank.alloc(pc));
     need(solid_removed.alloc(pc));
-    need(count.alloc(nc)); need(offset.alloc(nc + 1));
+    need(count.alloc(nc)); need(offset.alloc(nc + 1)); need(removed_in_cell.alloc(nc));
     size_t st = std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc));
     need(scan_tmp.alloc(st));
     need(new_count.alloc(1)); need(over_cap.alloc(1)); need(speed_hist.alloc(64)); need(max_speed_bits.alloc(1));
@@ -460,6 +460,7 @@ void Scene::initialize() {
 }
 
 bool Scene::rearm(const float *pos, const float *vel, int64_t count_in) {
+    removal_counts_ready = false;
     if (!initialized || !pending_events.empty()) return false;
     pending_pos.clear();
     pending_vel.clear();
@@ -493,6 +494,7 @@ size_t Scene::device_bytes() const {
     size_t b = 0;
     for (int a = 0; a < 6; a++) b += part[a].bytes() + part_alt[a].bytes() + sorted[a].bytes();
     b += cell.bytes() + slot.bytes() + count.bytes() + offset.bytes() + order.bytes() + scan_tmp.bytes();
+    b += removed_in_cell.bytes();
     b += keep.bytes() + keep_pos.bytes() + rank.bytes() + new_count.bytes() + over_cap.bytes() + solid_removed.bytes();
     b += speed_hist.bytes() + max_speed_bits.bytes();
     b += solid.bytes() + wU.bytes() + wV.bytes() + wW.bytes() + wC.bytes() + sU.bytes() + sV.bytes() + sW.bytes();
@@ -513,7 +515,7 @@ size_t estimate_scene_bytes(const flipmi_scene_desc &d, int64_t cap) {
     const size_t nf = std::max(nU, std::max(nV, nW));
     const size_t pc = (size_t)std::max<int64_t>(cap, 1);
     size_t b = pc * (18 * sizeof(float) + 6 * sizeof(int) + 1);
-    b += sizeof(int) * (nc + nc + 1);
+    b += sizeof(int) * (nc + nc + 1 + nc);  // count, offset, removed_in_cell
     b += sizeof(int) * std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc));
     b += sizeof(int) * (1 + 64) + sizeof(unsigned);
     b += sizeof(float) * nn;
@@ -638,6 +640,10 @@ void Scene::advance_particles(double dt) {
           "hist");
     // The per-cell order is built only when some cell is over the cap.
     sort_particles(over_cap.p);
+    check(launch_removed_in_cell(g, part[0].p, part[1].p, part[2].p, solid_removed.p, n, removed_in_cell.p,
+                                 stream),
+          "removed in cell");
+    removal_counts_ready = true;
     check(launch_fill_i32(rank.p, (size_t)std::max(n, 1), -1, stream), "fill");
     check(launch_cell_rank_cells(offset.p, g.I * g.J * g.K, order.p, solid_removed.p, over_cap.p, rank.p,
                                  stream),

@@ -105,6 +105,10 @@ struct Scene {
     DBuf<int> cell, slot, count, offset, order, scan_tmp;
     DBuf<int> keep, keep_pos, rank, new_count, over_cap;
     DBuf<uint8_t> solid_removed;
+    // Solid-removed particles per cell at the last removal pass; with `count`
+    // it gives the removal pass's count grid (FLIPMI_GRID_CELL_COUNT).
+    DBuf<int> removed_in_cell;
+    bool removal_counts_ready = false;
     DBuf<int> speed_hist;
     DBuf<unsigned> max_speed_bits;
 

@@ -123,7 +123,12 @@ enum {
     FLIPMI_GRID_WEIGHT_C = 8,
     FLIPMI_GRID_VALID_U = 9,   /* uint8 */
     FLIPMI_GRID_VALID_V = 10,  /* uint8 */
-    FLIPMI_GRID_VALID_W = 11   /* uint8 */
+    FLIPMI_GRID_VALID_W = 11,  /* uint8 */
+    /* int32, I x J x K: the count grid of the last substep's removal pass
+     * (FluidSimulation::_removeMarkerParticles, fluidsimulation.cpp:4325-4343):
+     * per cell, the advected particles not removed by the solid test, capped
+     * at 250.  All zero before the first substep. */
+    FLIPMI_GRID_CELL_COUNT = 12
 };
 
 /* ---- context ----------------------------------------------------------- */
@@ -259,6 +264,14 @@ int flipmi_liquid_sdf(flipmi_ctx *ctx, int isize, int jsize, int ksize, double d
                       const float *pos_xyz, int64_t n, double radius,
                       const float *solid_phi_nodal, float *phi_out);
 
+/* The counting loop of FluidSimulation::_removeMarkerParticles
+ * (fluidsimulation.cpp:4325-4343) on given particle positions: particles in
+ * order, those flagged in `removed` (the solid test, may be NULL) skipped,
+ * each cell counting up to 250 (fluidsimulation.h:1684).  counts_out is an
+ * int32 I x J x K grid; particles outside the grid count nowhere. */
+int flipmi_removal_counts(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx,
+                          const float *pos_xyz, int64_t n, const uint8_t *removed, int32_t *counts_out);
+
 /* ParticleLevelSet::postProcessSignedDistanceField(MeshLevelSet&)
  * (particlelevelset.cpp:172-196) on a field from flipmi_liquid_sdf computed
  * without a solid, in place. */
@@ -275,6 +288,7 @@ int flipmi_p2g(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx, cons
  * (gridutils.cpp:32-228) on one component grid, in place. */
 int flipmi_extrapolate(flipmi_ctx *ctx, int ni, int nj, int nk, float *grid,
                        const uint8_t *valid, int layers);
+
 
 /* PressureSolverParameters (pressuresolver.h:63-79). */
 typedef struct {

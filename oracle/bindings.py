@@ -299,6 +299,15 @@ class _SimHandle:
         self._f("sim_get_bool_grid")(self.h, which, _ptr(out))
         return out
 
+    def count_grid(self):
+        """(count grid of the last removal pass, particles that pass removed
+        for their speed, speed removals since creation).  Oracle only."""
+        I, J, K = self.dims
+        out = np.zeros((K, J, I), np.int32)
+        last, total = ctypes.c_int(0), ctypes.c_int(0)
+        self._f("sim_get_count_grid")(self.h, _ptr(out), ctypes.byref(last), ctypes.byref(total))
+        return out, last.value, total.value
+
 
 class Oracle(_Base):
     prefix = "orc_"
@@ -320,6 +329,17 @@ class Oracle(_Base):
         L.orc_sim_set_obstacle.argtypes = [_vp, _c_int, _c_int]
         L.orc_inflow_build.argtypes = ([_vp, _c_int, _vp, _c_int] + [_c_int] * 3
                                        + [_c_double, ctypes.c_uint64, _c_int, _vp, _vp, _vp])
+        L.orc_sim_get_count_grid.argtypes = [_vp, _vp, _vp, _vp]
+        L.orc_removal_counts.argtypes = [_c_int] * 3 + [_c_double, _vp, _c_int, _vp, _vp]
+
+    def removal_counts(self, isize, jsize, ksize, dx, pos, removed=None):
+        """Count grid of the removal pass's counting loop on given positions
+        (removed: optional per-particle flags of the solid test)."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        rm = None if removed is None else np.ascontiguousarray(removed, np.uint8)
+        out = np.zeros((ksize, jsize, isize), np.int32)
+        self.lib.orc_removal_counts(isize, jsize, ksize, dx, _ptr(pos), pos.shape[0], _ptr(rm), _ptr(out))
+        return out
 
     def _mesh_args(self, verts, tris):
         verts = np.ascontiguousarray(verts, np.float32)

@@ -1046,10 +1046,77 @@ static double absmax(const std::vector<double> &a) {
     return m;
 }
 
+// Dot product with Neumaier's compensated summation (a sum whose error does
+// not grow with n), for the converged FP64 error oracle.
+static double dot_compensated(const std::vector<double> &a, const std::vector<double> &b) {
+    double s = 0, c = 0;
+    for (size_t i = 0; i < a.size(); i++) {
+        double p = a[i] * b[i];
+        double t = s + p;
+        if (std::fabs(s) >= std::fabs(p)) c += (s - t) + p;
+        else c += (p - t) + s;
+        s = t;
+    }
+    return s + c;
+}
+
+// The converged FP64 error oracle (mode ORC_PCG_FP64_TRUE): the same MIC(0)
+// preconditioned CG with alpha/beta in double and compensated dots; when the
+// recursive residual reaches the tolerance the true residual b - Ax is
+// formed, and the solve ends only if that one is below the tolerance too
+// (otherwise CG restarts from it).  This is the reference's iteration made
+// exact enough to measure other solvers' error against, not the reference's
+// arithmetic.
+static bool pcg_true_residual(const Rows &R, const std::vector<double> &b, std::vector<double> &x, double tolf,
+                              int maxit, double &resid, int &iters) {
+    size_t n = R.cell.size();
+    x.assign(n, 0.0);
+    std::vector<double> r = b, z, s, q, ax;
+    resid = absmax(r);
+    if (resid == 0) { iters = 0; return true; }
+    if (tolf < 1e-30) tolf = 1e-30;
+    const double tol = tolf * resid;
+    MIC M;
+    mic0_factor(R, M);
+    int it = 0;
+    while (it < maxit) {
+        mic0_apply(R, M, r, z);
+        double rho = dot_compensated(z, r);
+        if (rho == 0 || rho != rho) break;
+        s = z;
+        bool restart = false;
+        for (; it < maxit && !restart; it++) {
+            spmv(R, s, q);
+            double alpha = rho / dot_compensated(s, q);
+            for (size_t i = 0; i < n; i++) x[i] += alpha * s[i];
+            for (size_t i = 0; i < n; i++) r[i] -= alpha * q[i];
+            if (absmax(r) <= tol) {
+                spmv(R, x, ax);
+                for (size_t i = 0; i < n; i++) r[i] = b[i] - ax[i];
+                resid = absmax(r);
+                if (resid <= tol) { iters = it + 1; return true; }
+                restart = true;
+                continue;
+            }
+            mic0_apply(R, M, r, z);
+            double rhoNew = dot_compensated(z, r);
+            double beta = rhoNew / rho;
+            for (size_t i = 0; i < n; i++) s[i] = z[i] + beta * s[i];
+            rho = rhoNew;
+        }
+    }
+    spmv(R, x, ax);
+    for (size_t i = 0; i < n; i++) r[i] = b[i] - ax[i];
+    resid = absmax(r);
+    iters = it;
+    return resid <= tol;
+}
+
 // PCGSolver<double>::solve (pcgsolver.h:248-302).  mode 0 rounds alpha/beta
 // to float as BLAS::addScaled's float parameter does (blaswrapper.h:135-158).
 static bool pcg(const Rows &R, const std::vector<double> &b, std::vector<double> &x, double tolf,
                 int maxit, int mode, double &resid, int &iters) {
+    if (mode == ORC_PCG_FP64_TRUE) return pcg_true_residual(R, b, x, tolf, maxit, resid, iters);
     size_t n = R.cell.size();
     x.assign(n, 0.0);
     std::vector<double> r = b, z, s, q;
@@ -1624,6 +1691,10 @@ struct Sim {
     double nearSolidCellSize = 0;
     int lastIters = 0, lastOk = 1;
     double lastErr = 0;
+    // Count grid of the last removal pass (fluidsimulation.cpp:4325-4343) and
+    // how many particles that pass removed for their speed.
+    Grid<int> removalCount;
+    int removedBySpeed = 0, removedBySpeedTotal = 0;
     // Mesh inflow sources (FluidSimulation::addMeshFluidSource,
     // fluidsimulation.cpp:1953-1965) and the marker-particle jitter
     // (fluidsimulation.h:1553-1555).
@@ -1764,16 +1835,23 @@ static void advance_particles(Sim &S, double dt) {
     // Removal (fluidsimulation.cpp:4324-4353) uses the frame step.
     float maxspeed = speed_limit(S, S.frameDt);
     double maxsq = maxspeed * maxspeed;
-    Grid<int> count(S.I, S.J, S.K, 0);
+    Grid<int> &count = S.removalCount;
+    count = Grid<int>(S.I, S.J, S.K, 0);
     std::vector<uint8_t> removed(S.pos.size(), 0);
     for (size_t i = 0; i < S.pos.size(); i++)
         removed[i] = grid_trilinear(S.solid, S.pos[i], S.dx) < 0.0f;
+    S.removedBySpeed = 0;
     for (size_t i = 0; i < S.pos.size(); i++) {
         if (removed[i]) continue;
         Idx g = pos_to_index(S.pos[i], S.dx);
         if (count(g.i, g.j, g.k) >= 250) { removed[i] = 1; continue; }
         count(g.i, g.j, g.k) += 1;
-        if (dot(S.vel[i], S.vel[i]) > maxsq) { removed[i] = 1; continue; }
+        if (dot(S.vel[i], S.vel[i]) > maxsq) {
+            removed[i] = 1;
+            S.removedBySpeed++;
+            S.removedBySpeedTotal++;
+            continue;
+        }
     }
     size_t cur = 0;
     for (size_t i = 0; i < S.pos.size(); i++) {
@@ -2499,6 +2577,34 @@ void orc_sim_get_bool_grid(void *h, int which, uint8_t *out) {
         default: break;
     }
 }
+// Count grid of the last removal pass (zeros before the first substep), the
+// particles that pass removed for their speed, and all speed removals since
+// the simulation was created.
+void orc_sim_get_count_grid(void *h, int32_t *out, int *speed_last, int *speed_total) {
+    Sim *S = (Sim *)h;
+    const size_t nc = (size_t)S->I * S->J * S->K;
+    if (S->removalCount.a.size() == nc) copy_out(S->removalCount, out);
+    else std::memset(out, 0, sizeof(int32_t) * nc);
+    if (speed_last) *speed_last = S->removedBySpeed;
+    if (speed_total) *speed_total = S->removedBySpeedTotal;
+}
+
+// The counting loop of FluidSimulation::_removeMarkerParticles
+// (fluidsimulation.cpp:4325-4343) alone: particles in order, skipping the
+// removed ones, each cell counting up to 250.
+void orc_removal_counts(int I, int J, int K, double dx, const float *pos, int n, const uint8_t *removed,
+                        int32_t *out) {
+    Grid<int> count(I, J, K, 0);
+    for (int i = 0; i < n; i++) {
+        if (removed && removed[i]) continue;
+        Idx g = pos_to_index(V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), dx);
+        if (!count.in(g.i, g.j, g.k)) continue;  // outside the grid: no cell
+        if (count(g.i, g.j, g.k) >= 250) continue;
+        count(g.i, g.j, g.k) += 1;
+    }
+    copy_out(count, out);
+}
+
 int orc_sim_frame_substeps(void *h) { return ((Sim *)h)->lastSubsteps; }
 void orc_sim_pressure_info(void *h, int *iters, double *err, int *ok) {
     Sim *S = (Sim *)h;

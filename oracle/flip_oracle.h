@@ -21,9 +21,12 @@ extern "C" {
 // Pressure-solver modes for orc_pressure_solve / orc_sim_set_pcg_mode.
 //   0: reference-exact -- alpha/beta rounded to float before each axpy and
 //      sequential dot products (pcgsolver.h:248-302, blaswrapper.h:35-158).
-//   1: pure FP64 -- alpha/beta kept in double; used as the tightly converged
-//      error oracle.
-enum { ORC_PCG_REF_EXACT = 0, ORC_PCG_FP64 = 1 };
+//   1: pure FP64 -- alpha/beta kept in double, sequential dots, stop on the
+//      recursive residual.
+//   2: converged FP64 error oracle -- alpha/beta in double, compensated dot
+//      products, and the solve ends only when the TRUE residual b - Ax is
+//      below the tolerance (run at 1e-12 to measure other solvers' error).
+enum { ORC_PCG_REF_EXACT = 0, ORC_PCG_FP64 = 1, ORC_PCG_FP64_TRUE = 2 };
 
 void orc_domain_solid(int isize, int jsize, int ksize, double dx, float *phi_nodal, float *wU,
                       float *wV, float *wW, float *wC);
@@ -70,6 +73,9 @@ void orc_sim_get_particles(void *h, float *pos, float *vel);
 void orc_sim_get_float_grid(void *h, int which, float *out);
 void orc_sim_get_bool_grid(void *h, int which, uint8_t *out);
 int orc_sim_frame_substeps(void *h);
+void orc_sim_get_count_grid(void *h, int32_t *out, int *speed_last, int *speed_total);
+void orc_removal_counts(int I, int J, int K, double dx, const float *pos, int n, const uint8_t *removed,
+                        int32_t *out);
 void orc_sim_pressure_info(void *h, int *iters, double *err, int *ok);
 // Mesh inside tests, mesh level sets and inflow sources
 // (meshutils.cpp:99-428, meshlevelset.cpp:572-601, meshfluidsource.cpp,

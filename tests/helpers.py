@@ -67,3 +67,21 @@ def pocket_inputs(oracle, I, J, K, seed=6):
     rng = np.random.default_rng(seed)
     solid_vel = tuple((rng.standard_normal(g.shape) * 0.2).astype(np.float32) for g in mac)
     return dx, solid, (wu, wv, ww, wc), liquid, mac, valid, solid_vel, cavity, (sk, sj, si)
+
+
+def fast_particle_scene(I, J, K, nfast, speed, seed=7):
+    """A dam break in which a few particles start far faster than
+    CFL * dx / dt, so _getMarkerParticleSpeedLimit (fluidsimulation.cpp:
+    4298-4322) sets a limit that removes them."""
+    s = dam(I, J, K, seed=seed)
+    vel = s.velocities.copy()
+    ids = np.linspace(17, s.num_particles - 17, nfast).astype(np.int64)
+    vel[ids] = np.array([speed, 0.3 * speed, -0.2 * speed], np.float32)
+    return s, vel, ids
+
+
+def cell_histogram(pos, I, J, K, dx):
+    g = np.floor(pos.astype(np.float64) * (1.0 / dx)).astype(np.int64)
+    h = np.zeros((K, J, I), np.int64)
+    np.add.at(h, (g[:, 2], g[:, 1], g[:, 0]), 1)
+    return h

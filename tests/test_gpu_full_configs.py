@@ -88,3 +88,31 @@ def test_double_dam_256_mixed_substep_against_fp64(gpu):
     assert a[3].shape == b[3].shape
     assert np.abs(a[3] - b[3]).max() <= 1e-4 * s.dx
     assert np.abs(a[4] - b[4]).max() <= 1e-5 * max(1.0, float(np.abs(a[4]).max()))
+
+
+def test_pool_drop_128_counts_bitexact(gpu, oracle, pool_drop):
+    """Per-cell counts at the full 128^3 size.  The counting loop of the
+    removal pass (flipmi_removal_counts, fluidsimulation.cpp:4325-4343) is
+    bit-exact against the oracle on the scene's 4.8 M particles, with a third
+    of them flagged as solid-removed and one cell crowded past the 250 cap;
+    and after a substep the scene's own count grid is exactly the histogram
+    of its surviving particles (no speed removals in this scene)."""
+    from tests.helpers import cell_histogram
+    s = pool_drop
+    I, J, K, dx = s.isize, s.jsize, s.ksize, s.dx
+    rng = np.random.default_rng(2)
+    removed = (rng.random(s.num_particles) < 0.33).astype(np.uint8)
+    crowd = np.repeat(s.positions[1000:1001], 400, axis=0)
+    pos = np.concatenate([s.positions, crowd]).astype(np.float32)
+    rm = np.concatenate([removed, np.zeros(400, np.uint8)])
+    got = gpu.removal_counts(I, J, K, dx, pos, rm)
+    want = oracle.removal_counts(I, J, K, dx, pos, rm)
+    assert bits_equal(got, want), f"ndiff={ndiff(got, want)}"
+    assert int(got.max()) == 250
+    sim = _sim(gpu, s, fa.PCG_FP64)
+    st = sim.update(1 / 120)
+    assert st.pressure_ok == 1
+    counts = sim.grid(12)
+    p, _ = sim.particles()
+    sim.close()
+    assert np.array_equal(counts, cell_histogram(p, I, J, K, dx))

@@ -12,7 +12,7 @@ reference sources are present).
 import numpy as np
 import pytest
 
-from tests.helpers import bits_equal, dam, ndiff, radius, random_velocities
+from tests.helpers import bits_equal, cell_histogram, dam, fast_particle_scene, ndiff, radius, random_velocities
 
 SIZES = [(16, 16, 16), (23, 17, 20)]
 
@@ -207,3 +207,71 @@ def test_particle_io_domain_coordinates(oracle, reference, offset, scale):
         assert bits_equal(ra[0], rb[0]) and bits_equal(ra[1], rb[1]), f"range {rng}"
     for bad in ((-1, 3), (0, n + 1), (5, 4)):
         assert a.particle_range(*bad) is None and b.particle_range(*bad) is None
+
+
+def test_speed_removal_and_count_grid(oracle, reference):
+    """Extreme-velocity removal (fluidsimulation.cpp:4298-4353): the oracle
+    removes the same particles as the reference, frame by frame, bit for bit.
+    Its count grid of the removal pass (fluidsimulation.cpp:4325-4343) is
+    pinned by the reference's surviving particles: every counted particle
+    survives except the ones the speed test removed in that pass."""
+    I, J, K = 20, 20, 20
+    s, vel, ids = fast_particle_scene(I, J, K, nfast=3, speed=40.0)
+    # maxRemoval = min(int(0.0005 N), 35) must admit the fast particles.
+    assert min(int(s.num_particles * 0.0005), 35) >= 3
+    reference.set_threads(1)
+    sims = []
+    for lib in (oracle, reference):
+        h = lib.sim(I, J, K, s.dx)
+        h.add_body_force(0.0, -9.81, 0.0)
+        h.load_particles(s.positions, vel)
+        h.initialize()
+        sims.append(h)
+    a, b = sims
+    n0 = s.num_particles
+    for f in range(3):
+        for h in sims:
+            h.update(1 / 30)
+        assert a.num_particles == b.num_particles, f"frame {f}"
+        pa, va = a.particles()
+        pb, vb = b.particles()
+        assert bits_equal(pa, pb) and bits_equal(va, vb), f"frame {f}"
+        counts, last, total = a.count_grid()
+        hist = cell_histogram(pb, I, J, K, s.dx)
+        assert counts.max() <= 250
+        assert np.all(counts >= hist), f"frame {f}"
+        assert int(counts.sum() - hist.sum()) == last, f"frame {f}"
+        if last == 0:
+            assert np.array_equal(counts, hist), f"frame {f}"
+    # The speed test did run: the three fast particles are gone and were the
+    # only speed removals; nothing else left (no solid or cap removals here).
+    assert total == 3
+    assert a.num_particles == n0 - 3
+    fast = s.positions[ids]
+    assert not np.any(np.all(np.isin(pb, fast), axis=1))
+
+
+def test_removal_count_stage(oracle, reference):
+    """orc_removal_counts (the counting loop alone) against the reference's
+    survivors: with no speed removal, the count grid of a removal pass is the
+    cell histogram of the particles that survive it."""
+    I, J, K = 18, 14, 16
+    s = dam(I, J, K, seed=8)
+    reference.set_threads(1)
+    h = reference.sim(I, J, K, s.dx)
+    h.add_body_force(0.0, -9.81, 0.0)
+    h.load_particles(s.positions, s.velocities)
+    h.initialize()
+    h.update(1 / 30)
+    pb, _ = h.particles()
+    got = oracle.removal_counts(I, J, K, s.dx, pb)
+    assert np.array_equal(got, cell_histogram(pb, I, J, K, s.dx))
+    # Flagged particles are skipped and a cell stops at 250.
+    rm = np.zeros(pb.shape[0], np.uint8)
+    rm[::3] = 1
+    assert np.array_equal(oracle.removal_counts(I, J, K, s.dx, pb, rm),
+                          cell_histogram(pb[rm == 0], I, J, K, s.dx))
+    crowd = np.repeat(pb[:1], 300, axis=0)
+    c = oracle.removal_counts(I, J, K, s.dx, np.concatenate([crowd, pb[1:]]))
+    g = np.floor(pb[0].astype(np.float64) / s.dx).astype(int)
+    assert c[g[2], g[1], g[0]] == 250
