@@ -116,6 +116,7 @@ EXPORTED = [
     "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
     "flipmi_trilinear_velocity", "flipmi_server_desc_default", "flipmi_server_create", "flipmi_server_submit",
     "flipmi_server_wait", "flipmi_server_take", "flipmi_server_get_stats", "flipmi_server_destroy",
+    "flipmi_extrapolate_noalloc",
 ]
 
 _lib = None
@@ -163,6 +164,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_removal_counts.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _vp, _vp]
     L.flipmi_p2g.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _c_i64, _c_double] + [_vp] * 6
     L.flipmi_extrapolate.argtypes = [_vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int]
+    L.flipmi_extrapolate_noalloc.argtypes = [_vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp]
     L.flipmi_pressure_solve.argtypes = [_vp, P(PressureArgs), P(PressureReport)]
     L.flipmi_mic0_apply.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _c_double] + [_vp] * 8
     L.flipmi_advect_particles.argtypes = ([_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _vp, _vp, _c_i64,

@@ -98,6 +98,12 @@ int flipmi_ctx_create(int device, flipmi_ctx **out) {
         delete c;
         return FLIPMI_ERR_HIP;
     }
+    // Device address of this device's flipmi_extrapolate_noalloc scratch.
+    if (noalloc_scratch_init(device) != hipSuccess) {
+        (void)hipStreamDestroy(c->c.stream);
+        delete c;
+        return FLIPMI_ERR_HIP;
+    }
     *out = c;
     return FLIPMI_OK;
 }

@@ -90,6 +90,9 @@ hipError_t launch_p2g3(const GridGeom &g, const float *sx, const float *sy, cons
 // ---- grid passes (k_grid.hip) ----
 hipError_t launch_extrapolate(int ni, int nj, int nk, float *grid, const uint8_t *valid,
                               uint8_t *status, int layers, hipStream_t s);
+// Looks up the device address of the no-allocation scratch array on
+// `device` (current) once; noalloc.hip.
+hipError_t noalloc_scratch_init(int device);
 hipError_t launch_save_and_body_force(size_t n, const float *mac, float *saved, float add,
                                       int do_add, hipStream_t s);
 hipError_t launch_constrain(size_t n, float *mac, const float *weight, const float *solid_vel,

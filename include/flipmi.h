@@ -289,6 +289,23 @@ int flipmi_p2g(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx, cons
 int flipmi_extrapolate(flipmi_ctx *ctx, int ni, int nj, int nk, float *grid,
                        const uint8_t *valid, int layers);
 
+/* Size of the per-device scratch array behind flipmi_extrapolate_noalloc
+ * (one status byte per face): 64 MiB, held by every device the library
+ * runs on until the process ends. */
+#define FLIPMI_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+
+/* No counterpart in the reference.  The same result as flipmi_extrapolate
+ * (GridUtils::extrapolateGrid, gridutils.cpp:32-228), for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops.
+ * It calls no allocator.  grid_dev and valid_dev are DEVICE pointers owned by
+ * the caller (ni*nj*nk floats / bytes); the BFS status grid lives in a fixed
+ * per-device array of FLIPMI_NOALLOC_SCRATCH_BYTES, so ni*nj*nk above that
+ * is rejected with FLIPMI_ERR_INVALID_ARG.  The work is enqueued on `stream`
+ * (a hipStream_t; NULL = the context's stream) and not waited for.  All calls
+ * on one device share that device's array: calls that overlap on a device
+ * race. */
+int flipmi_extrapolate_noalloc(flipmi_ctx *ctx, int ni, int nj, int nk, float *grid_dev,
+                               const uint8_t *valid_dev, int layers, void *stream);
 
 /* PressureSolverParameters (pressuresolver.h:63-79). */
 typedef struct {
