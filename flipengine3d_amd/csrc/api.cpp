@@ -420,27 +420,31 @@ int flipmi_domain_solid(flipmi_ctx *ctx, int I, int J, int K, double dx, float *
 }
 
 namespace {
-// Sorts host particles by cell on the device and returns sorted SoA arrays.
+// Sorts host particles on the device -- by cell, or into half-cell bins as
+// the scene's step does -- and returns sorted SoA arrays.
 struct SortedParticles {
     DBuf<float> raw[6], sorted[6];
     DBuf<int> cell, slot, count, offset, order, scan_tmp;
-    void build(const GridGeom &g, const float *pos, const float *vel, int64_t n, hipStream_t s) {
+    void build(const GridGeom &g, const float *pos, const float *vel, int64_t n, hipStream_t s,
+               bool half_cells = false) {
         if (n > INT32_MAX / 2) throw StatusError{FLIPMI_ERR_INVALID_ARG, "too many particles"};
         upload_soa(raw, pos, n, s);
         if (vel) upload_soa(raw + 3, vel, n, s);
         else for (int a = 3; a < 6; a++) { alloc_dev(raw[a], (size_t)std::max<int64_t>(n, 1)); hipck(launch_fill_f32(raw[a].p, raw[a].n, 0.0f, s), "fill"); }
-        size_t nc = (size_t)g.I * g.J * g.K;
+        const size_t nbins = half_cells ? subcell_bins(g) : (size_t)g.I * g.J * g.K;
+        if (nbins >= (size_t)INT32_MAX) throw StatusError{FLIPMI_ERR_INVALID_ARG, "grid too large"};
         size_t pn = (size_t)std::max<int64_t>(n, 1);
         for (int a = 0; a < 6; a++) alloc_dev(sorted[a], pn);
         alloc_dev(cell, pn); alloc_dev(slot, pn); alloc_dev(order, pn);
-        alloc_dev(count, nc); alloc_dev(offset, nc + 1);
-        alloc_dev(scan_tmp, std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pn)));
+        alloc_dev(count, nbins); alloc_dev(offset, nbins + 1);
+        alloc_dev(scan_tmp, std::max(scan_scratch_elems((int)nbins), scan_scratch_elems((int)pn)));
         SortBuffers sb{cell.p, slot.p, count.p, offset.p, order.p, scan_tmp.p, scan_tmp.n};
-        hipck(launch_sort_by_cell(g, raw[0].p, raw[1].p, raw[2].p, (int)n, sb, s), "sort");
+        if (half_cells) hipck(launch_sort_by_subcell(g, raw[0].p, raw[1].p, raw[2].p, (int)n, sb, s), "sort");
+        else hipck(launch_sort_by_cell(g, raw[0].p, raw[1].p, raw[2].p, (int)n, sb, s), "sort");
         const float *src[6];
         float *dst[6];
         for (int a = 0; a < 6; a++) { src[a] = raw[a].p; dst[a] = sorted[a].p; }
-        hipck(launch_gather_sorted(order.p, offset.p + nc, (int)n, src, dst, s), "gather");
+        hipck(launch_gather_sorted(order.p, offset.p + nbins, (int)n, src, dst, s), "gather");
     }
 };
 }  // namespace
@@ -452,7 +456,7 @@ int flipmi_liquid_sdf(flipmi_ctx *ctx, int I, int J, int K, double dx, const flo
         GridGeom g = make_geom(I, J, K, dx);
         hipStream_t s = ctx->c.stream;
         SortedParticles sp;
-        sp.build(g, pos, nullptr, n, s);
+        sp.build(g, pos, nullptr, n, s, true);
         size_t nn = (size_t)(I + 1) * (J + 1) * (K + 1);
         DBuf<float> solid, phi;
         if (solid_phi_nodal) upload(solid, solid_phi_nodal, nn, s);
@@ -517,17 +521,19 @@ int flipmi_p2g(flipmi_ctx *ctx, int I, int J, int K, double dx, const float *pos
         GridGeom g = make_geom(I, J, K, dx);
         hipStream_t s = ctx->c.stream;
         SortedParticles sp;
-        sp.build(g, pos, vel, n, s);
+        sp.build(g, pos, vel, n, s, true);
         DBuf<float> du, dv, dw;
         DBuf<uint8_t> dvu, dvv, dvw;
+        DBuf<int> ovf, ovf_count;
         alloc_dev(du, (size_t)(I + 1) * J * K); alloc_dev(dv, (size_t)I * (J + 1) * K);
         alloc_dev(dw, (size_t)I * J * (K + 1));
         alloc_dev(dvu, du.n); alloc_dev(dvv, dv.n); alloc_dev(dvw, dw.n);
+        alloc_dev(ovf, p2g_face_count(g)); alloc_dev(ovf_count, 1);
         const float *svel[3] = {sp.sorted[3].p, sp.sorted[4].p, sp.sorted[5].p};
         float *outs[3] = {du.p, dv.p, dw.p};
         uint8_t *valids[3] = {dvu.p, dvv.p, dvw.p};
-        hipck(launch_p2g3(g, sp.sorted[0].p, sp.sorted[1].p, sp.sorted[2].p, svel, sp.offset.p, radius, outs,
-                          valids, s),
+        hipck(launch_p2g(g, sp.sorted[0].p, sp.sorted[1].p, sp.sorted[2].p, svel, sp.order.p, sp.offset.p, (int)n,
+                         radius, outs, valids, ovf_count.p, ovf.p, s),
               "p2g");
         download(u, du, du.n, s); download(v, dv, dv.n, s); download(w, dw, dw.n, s);
         download(vu, dvu, dvu.n, s); download(vv, dvv, dvv.n, s); download(vw, dvw, dvw.n, s);

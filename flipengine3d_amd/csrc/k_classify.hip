@@ -6,7 +6,8 @@
 // The reference bins particles into 10^3 cell blocks and takes, per block,
 //     phi(c) = min(3 dx, min_p |c_local - p_local| - r)
 // in block-local float coordinates.  Here one workgroup owns one 10^3 block,
-// gathers the particles of the cells around it from the cell-sorted arrays,
+// gathers the particles of the cells around it from the arrays sorted into
+// half-cell bins,
 // replays the reference's particle-to-block assignment and local arithmetic,
 // and folds distances into LDS with an order-independent atomic minimum.
 // The result is bit-identical to the reference.
@@ -54,13 +55,17 @@ __device__ __forceinline__ float postprocess_phi(const GridGeom &g, const float 
     return v;
 }
 
+constexpr int kRows = 4 * kSpan * kSpan;           // half-cell rows of the candidate box
+constexpr int kRowsPerThread = (kRows + kSdfThreads - 1) / kSdfThreads;
+
 __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
     GridGeom g, const float *__restrict__ sx, const float *__restrict__ sy,
-    const float *__restrict__ sz, const int *__restrict__ offset, float radius,
+    const float *__restrict__ sz, const int *__restrict__ sub_offset, float radius,
     const float *__restrict__ solid, float *phi_out) {
     __shared__ uint32_t phi_ord[kBW * kBW * kBW];
-    __shared__ int run_begin[kSpan * kSpan];
-    __shared__ int run_prefix[kSpan * kSpan + 1];
+    __shared__ int run_begin[kRows];
+    __shared__ int run_prefix[kRows + 1];
+    __shared__ int wave_total[kSdfThreads / 64];
     __shared__ float centre[kBW];  // Grid3d::GridIndexToCellCenter of local index c
 
     int nbi = (g.I + kBW - 1) / kBW, nbj = (g.J + kBW - 1) / kBW;
@@ -71,24 +76,50 @@ __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
     for (int c = threadIdx.x; c < kBW * kBW * kBW; c += kSdfThreads) phi_ord[c] = init;
     if (threadIdx.x < kBW) centre[threadIdx.x] = idx_to_center1(threadIdx.x, g.dx);
 
-    // Candidate particles: cells [b0 - 2, b0 + 12) per axis, one contiguous
-    // run of the cell-sorted arrays per (j, k) row.
+    // Candidate particles: cells [b0 - 2, b0 + 12) per axis.  In the
+    // half-cell sort each (sy, sz) row of half-cells over those cells is one
+    // contiguous run: 4 rows per (j, k) row of cells.
     int i0 = max(bi * kBW - kHalo, 0), i1 = min(bi * kBW + kBW + kHalo, g.I);
     int j0 = max(bj * kBW - kHalo, 0), j1 = min(bj * kBW + kBW + kHalo, g.J);
     int k0 = max(bk * kBW - kHalo, 0), k1 = min(bk * kBW + kBW + kHalo, g.K);
-    int nj = j1 - j0, nk = k1 - k0, nrows = nj * nk;
-    for (int r = threadIdx.x; r < nrows; r += kSdfThreads) {
-        int j = j0 + r % nj, k = k0 + r / nj;
-        size_t c0 = (size_t)i0 + (size_t)g.I * ((size_t)j + (size_t)g.J * k);
-        size_t c1 = c0 + (size_t)(i1 - i0);
-        run_begin[r] = offset[c0];
-        run_prefix[r + 1] = offset[c1] - offset[c0];
+    int nsy = 2 * (j1 - j0), nsz = 2 * (k1 - k0), nrows = nsy * nsz;
+    const size_t SI = 2 * (size_t)g.I, SJ = 2 * (size_t)g.J;
+    // Each thread takes kRowsPerThread consecutive rows; a block-wide
+    // exclusive scan of the row lengths gives every row's start.
+    int len[kRowsPerThread];
+    int mine = 0;
+#pragma unroll
+    for (int q = 0; q < kRowsPerThread; q++) {
+        int r = threadIdx.x * kRowsPerThread + q;
+        len[q] = 0;
+        if (r < nrows) {
+            size_t sy = 2 * (size_t)j0 + (size_t)(r % nsy), sz = 2 * (size_t)k0 + (size_t)(r / nsy);
+            size_t b0 = 2 * (size_t)i0 + SI * (sy + SJ * sz);
+            size_t b1 = b0 + 2 * (size_t)(i1 - i0);
+            int s0 = sub_offset[b0];
+            run_begin[r] = s0;
+            len[q] = sub_offset[b1] - s0;
+        }
+        mine += len[q];
     }
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) wave_total[wave] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        run_prefix[0] = 0;
-        for (int r = 0; r < nrows; r++) run_prefix[r + 1] += run_prefix[r];
+    int run = inc - mine;
+    for (int w = 0; w < wave; w++) run += wave_total[w];
+#pragma unroll
+    for (int q = 0; q < kRowsPerThread; q++) {
+        int r = threadIdx.x * kRowsPerThread + q;
+        if (r < nrows) run_prefix[r] = run;
+        run += len[q];
     }
+    if (threadIdx.x == kSdfThreads - 1) run_prefix[nrows] = run;
     __syncthreads();
     int total = run_prefix[nrows];
 
@@ -168,10 +199,10 @@ hipError_t launch_liquid_postprocess(const GridGeom &g, const float *solid_phi_n
 }
 
 hipError_t launch_liquid_sdf(const GridGeom &g, const float *sx, const float *sy, const float *sz,
-                             const int *offset, double radius, const float *solid_phi_nodal,
+                             const int *sub_offset, double radius, const float *solid_phi_nodal,
                              float *phi, hipStream_t s) {
     int nbi = (g.I + kBW - 1) / kBW, nbj = (g.J + kBW - 1) / kBW, nbk = (g.K + kBW - 1) / kBW;
-    k_liquid_sdf<<<nbi * nbj * nbk, kSdfThreads, 0, s>>>(g, sx, sy, sz, offset, (float)radius,
+    k_liquid_sdf<<<nbi * nbj * nbk, kSdfThreads, 0, s>>>(g, sx, sy, sz, sub_offset, (float)radius,
                                                          solid_phi_nodal, phi);
     return hipGetLastError();
 }

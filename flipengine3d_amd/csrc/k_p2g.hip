@@ -1,4 +1,5 @@
-// Particle-to-grid transfer onto one staggered MAC component.
+// Particle-to-grid transfer onto the staggered MAC grid, in the reference's
+// summation order.
 //
 // Reference: VelocityAdvector::advect / _advectGrid / _advectionProducerThread
 // (velocityadvector.cpp:38-459).  For every face f of component U, V or W
@@ -8,19 +9,47 @@
 // sum exceeds 1e-6.  The reference evaluates distances in the frame of the
 // 10^3 face block that owns f and assigns particles to blocks by their search
 // box; both are replayed here, so every per-pair weight is bit-identical.
+// Its per-block particle lists are filled in ascending particle index
+// (velocityadvector.cpp:331-388) and each face accumulates in that list's
+// order (:413-446), so a face's sums run over its contributing particles in
+// ascending particle index.  That order is what makes the float sums, and
+// the valid flags, bit-identical here.
 //
-// Instead of scattering with atomics, each thread owns the U, V and W faces
-// of one index and gathers the particles of the neighbouring cells from the
-// cell-sorted arrays in a fixed order (cells k, j, i ascending; particles
-// ascending inside a cell).  The sum is deterministic; it differs from the
-// reference's ascending-particle-index summation order only by float
-// rounding.
+// One thread owns one face.  Its candidates are the particles of the 4 x 4
+// x 4 half-cells around the face (the particles sorted into half-cell bins,
+// sx fastest, so 16 contiguous runs): every particle within r < 0.87 dx of
+// the face lies there, with a margin of 0.13 dx.  Each hit's (w v, w) goes
+// to LDS with a key (particle id, slot); the keys are sorted in registers by
+// a bitonic network sized to the wave's largest hit count, and the thread
+// sums in key order.  Faces with more than kCap hits are listed and summed
+// by k_p2g_overflow, one wavefront per face, with the same order.
+#include <stdint.h>
+
 #include "device_math.h"
 #include "flipmi_kernels.h"
 
 namespace flipmi {
 
-constexpr int kChunk = 10;  // velocityadvector.h:163 (_chunkWidth)
+constexpr int kChunk = 10;       // velocityadvector.h:163 (_chunkWidth)
+constexpr int kThreads = 64;     // one wavefront per workgroup
+constexpr int kCap = 32;         // hits per face sorted in registers
+constexpr int kOvfCap = 1024;    // hits per face sorted in LDS by the overflow kernel
+constexpr int kOvfBlocks = 2048;
+constexpr int kCompShift = 29;   // overflow list entry: comp << 29 | face
+
+struct P2GArgs {
+    GridGeom g;
+    const float *sx, *sy, *sz;  // positions, half-cell order
+    const float *sv[3];         // velocity components, half-cell order
+    const int *ids;             // particle id of each sorted slot
+    const int *sub_offset;      // [8 I J K + 1]
+    float radius_f;
+    double radius;
+    float *out[3];
+    uint8_t *valid[3];
+    int *ovf_count;
+    int *ovf_list;
+};
 
 __device__ __forceinline__ bool p2g_point_in_block(F3 p, float sr, float blockdx,
                                                    double inv_blockdx, int bi, int bj, int bk) {
@@ -35,102 +64,419 @@ __device__ __forceinline__ bool p2g_point_in_block(F3 p, float sr, float blockdx
     return bi >= lo.i && bi <= hi.i && bj >= lo.j && bj <= hi.j && bk >= lo.k && bk <= hi.k;
 }
 
-// All three components in one pass: thread (i, j, k) owns the U, V and W
-// faces of that index (those that exist) and walks the 3x3x3 cells around
-// it once.  A particle from a cell outside a component's 2x3x3 set lies at
-// least dx > r from that face, so it fails the distance test: every face
-// sums exactly the particles, in exactly the order, of the per-component
-// kernel.
-__global__ __launch_bounds__(256) void k_p2g3(GridGeom g, const float *__restrict__ sx,
-                                              const float *__restrict__ sy,
-                                              const float *__restrict__ sz,
-                                              const float *__restrict__ su,
-                                              const float *__restrict__ sv,
-                                              const float *__restrict__ sw,
-                                              const int *__restrict__ offset, float radius_f,
-                                              double radius, float *ou, float *ov, float *ow,
-                                              uint8_t *vu, uint8_t *vv, uint8_t *vw) {
-    const int ni = g.I + 1, nj = g.J + 1, nk = g.K + 1;
-    size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (size_t)ni * nj * nk) return;
-    int i = (int)(tid % ni), j = (int)((tid / ni) % nj), k = (int)(tid / ((size_t)ni * nj));
-    const bool has[3] = {j < g.J && k < g.K, i < g.I && k < g.K, i < g.I && j < g.J};
-    if (!has[0] && !has[1] && !has[2]) return;
+// Everything one face needs to test a candidate particle exactly as
+// _advectionProducerThread does (velocityadvector.cpp:392-446).
+struct FaceCtx {
+    int comp, i, j, k;
+    size_t f;           // flat index in the component's grid
+    F3 off;             // _getDirectionOffset
+    float sr, blockdx, rsq, coef1, coef2, coef3;
+    double inv_blockdx;
+    int bi, bj, bk, li, lj, lk;
+    F3 boff, gpos;
+    int lo[3], hi[3];   // candidate half-cells, inclusive
+};
 
+__device__ __forceinline__ void face_dims(const GridGeom &g, int comp, int &ni, int &nj, int &nk) {
+    ni = g.I + (comp == 0);
+    nj = g.J + (comp == 1);
+    nk = g.K + (comp == 2);
+}
+
+__device__ __forceinline__ void face_setup(const P2GArgs &a, int comp, size_t f, FaceCtx &c) {
+    const GridGeom &g = a.g;
+    int ni, nj, nk;
+    face_dims(g, comp, ni, nj, nk);
+    c.comp = comp;
+    c.f = f;
+    c.i = (int)(f % (size_t)ni);
+    c.j = (int)((f / (size_t)ni) % (size_t)nj);
+    c.k = (int)(f / ((size_t)ni * nj));
     float hdx = (float)(0.5 * g.dx);  // velocityadvector.cpp:153-164
-    const F3 off[3] = {f3(0.0f, hdx, hdx), f3(hdx, 0.0f, hdx), f3(hdx, hdx, 0.0f)};
+    c.off = comp == 0 ? f3(0.0f, hdx, hdx) : comp == 1 ? f3(hdx, 0.0f, hdx) : f3(hdx, hdx, 0.0f);
     float eps = 1e-6f;
-    float sr = (float)(radius + (double)eps);
+    c.sr = (float)(a.radius + (double)eps);
     double chunkdx = g.dx * kChunk;
-    float blockdx = (float)chunkdx;
-    double inv_blockdx = 1.0 / (double)blockdx;
-    float r = radius_f, rsq = r * r;
-    float coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
-    float coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
-    float coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
-
-    int bi = i / kChunk, bj = j / kChunk, bk = k / kChunk;
-    int li = i - bi * kChunk, lj = j - bj * kChunk, lk = k - bk * kChunk;
-    F3 boff = idx_to_pos(bi, bj, bk, kChunk * g.dx);
-    F3 gpos = idx_to_pos(li, lj, lk, g.dx);
-
-    int ci0 = max(i - 1, 0), ci1 = min(i + 1, g.I - 1);
-    int cj0 = max(j - 1, 0), cj1 = min(j + 1, g.J - 1);
-    int ck0 = max(k - 1, 0), ck1 = min(k + 1, g.K - 1);
-    const float *vel[3] = {su, sv, sw};
-
-    float sum[3] = {0.0f, 0.0f, 0.0f}, wsum[3] = {0.0f, 0.0f, 0.0f};
-    for (int ck = ck0; ck <= ck1; ck++) {
-        for (int cj = cj0; cj <= cj1; cj++) {
-            size_t row = (size_t)g.I * ((size_t)cj + (size_t)g.J * ck);
-            int b = offset[row + ci0], e = offset[row + ci1 + 1];
-            for (int q = b; q < e; q++) {
-                const F3 p0 = f3(sx[q], sy[q], sz[q]);
+    c.blockdx = (float)chunkdx;
+    c.inv_blockdx = 1.0 / (double)c.blockdx;
+    float r = a.radius_f;
+    c.rsq = r * r;
+    c.coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
+    c.coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
+    c.coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
+    c.bi = c.i / kChunk; c.bj = c.j / kChunk; c.bk = c.k / kChunk;
+    c.li = c.i - c.bi * kChunk; c.lj = c.j - c.bj * kChunk; c.lk = c.k - c.bk * kChunk;
+    c.boff = idx_to_pos(c.bi, c.bj, c.bk, kChunk * g.dx);
+    c.gpos = idx_to_pos(c.li, c.lj, c.lk, g.dx);
+    // The face in half-cell units; particles within r lie in half-cells
+    // [F - 2, F + 1] along each axis.
+    const int F[3] = {2 * c.i + (comp != 0), 2 * c.j + (comp != 1), 2 * c.k + (comp != 2)};
+    const int N[3] = {2 * g.I, 2 * g.J, 2 * g.K};
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    if (!has[c]) continue;
-                    F3 p = p0 - off[c];
-                    F3 pl = p - boff;
-                    F3 v = gpos - pl;
-                    float d2 = dot3(v, v);
-                    if (!(d2 < rsq)) continue;
-                    if (!p2g_point_in_block(p, sr, blockdx, inv_blockdx, bi, bj, bk)) continue;
-                    I3 gmin = pos_to_index((double)(pl.x - sr), (double)(pl.y - sr), (double)(pl.z - sr), g.invdx);
-                    I3 gmax = pos_to_index((double)(pl.x + sr), (double)(pl.y + sr), (double)(pl.z + sr), g.invdx);
-                    gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
-                    gmax.i = min(gmax.i, kChunk - 1); gmax.j = min(gmax.j, kChunk - 1);
-                    gmax.k = min(gmax.k, kChunk - 1);
-                    if (li < gmin.i || li > gmax.i || lj < gmin.j || lj > gmax.j || lk < gmin.k ||
-                        lk > gmax.k)
-                        continue;
-                    float wt = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
-                    sum[c] += wt * vel[c][q];
-                    wsum[c] += wt;
+    for (int d = 0; d < 3; d++) {
+        c.lo[d] = max(F[d] - 2, 0);
+        c.hi[d] = min(F[d] + 1, N[d] - 1);
+    }
+}
+
+// The reference's per-pair test and weight for a particle at p0.
+__device__ __forceinline__ bool face_hit_at(const P2GArgs &a, const FaceCtx &c, F3 p0, float &wt) {
+    const F3 p = p0 - c.off;
+    const F3 pl = p - c.boff;
+    const F3 v = c.gpos - pl;
+    const float d2 = dot3(v, v);
+    if (!(d2 < c.rsq)) return false;
+    if (!p2g_point_in_block(p, c.sr, c.blockdx, c.inv_blockdx, c.bi, c.bj, c.bk)) return false;
+    I3 gmin = pos_to_index((double)(pl.x - c.sr), (double)(pl.y - c.sr), (double)(pl.z - c.sr), a.g.invdx);
+    I3 gmax = pos_to_index((double)(pl.x + c.sr), (double)(pl.y + c.sr), (double)(pl.z + c.sr), a.g.invdx);
+    gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
+    gmax.i = min(gmax.i, kChunk - 1); gmax.j = min(gmax.j, kChunk - 1); gmax.k = min(gmax.k, kChunk - 1);
+    if (c.li < gmin.i || c.li > gmax.i || c.lj < gmin.j || c.lj > gmax.j || c.lk < gmin.k || c.lk > gmax.k)
+        return false;
+    wt = 1.0f - c.coef1 * d2 * d2 * d2 + c.coef2 * d2 * d2 - c.coef3 * d2;
+    return true;
+}
+
+// The same for the particle in sorted slot q, with its w v
+// (`scalar += weight * velocity`, velocityadvector.cpp:438).
+__device__ __forceinline__ bool face_hit(const P2GArgs &a, const FaceCtx &c, int q, float &prod, float &wt) {
+    if (!face_hit_at(a, c, f3(a.sx[q], a.sy[q], a.sz[q]), wt)) return false;
+    prod = wt * a.sv[c.comp][q];
+    return true;
+}
+
+// Calls fn(q) for every candidate slot q of the face, lanes of a wavefront
+// striding each run when `stride` is 64 (the overflow kernel) or one thread
+// walking it when `stride` is 1.
+template <class Fn>
+__device__ __forceinline__ void for_candidates(const P2GArgs &a, const FaceCtx &c, int first, int stride, Fn &&fn) {
+    const size_t SI = 2 * (size_t)a.g.I, SJ = 2 * (size_t)a.g.J;
+    for (int sz = c.lo[2]; sz <= c.hi[2]; sz++) {
+        for (int sy = c.lo[1]; sy <= c.hi[1]; sy++) {
+            const size_t row = SI * ((size_t)sy + SJ * (size_t)sz);
+            const int b = a.sub_offset[row + (size_t)c.lo[0]];
+            const int e = a.sub_offset[row + (size_t)c.hi[0] + 1];
+            for (int q = b + first; q < e; q += stride) fn(q);
+        }
+    }
+}
+
+__device__ __forceinline__ void write_face(const P2GArgs &a, const FaceCtx &c, float sum, float wsum) {
+    const float eps = 1e-6f;  // velocityadvector.cpp:440-446
+    float u = sum;
+    if (wsum > eps) u = sum / wsum;
+    a.out[c.comp][c.f] = u;
+    a.valid[c.comp][c.f] = wsum > eps ? 1 : 0;
+}
+
+// Sort key of a hit: particle id above the LDS slot, so keys order by id.
+template <class Key>
+struct KeyTraits;
+template <>
+struct KeyTraits<uint32_t> {
+    static constexpr int kSlotBits = 5;  // kCap = 32 slots; ids below 2^27
+    __device__ static uint32_t make(int id, int slot) { return ((uint32_t)id << kSlotBits) | (uint32_t)slot; }
+    __device__ static int slot(uint32_t k) { return (int)(k & ((1u << kSlotBits) - 1u)); }
+    static constexpr uint32_t kMax = 0xffffffffu;
+};
+template <>
+struct KeyTraits<uint64_t> {
+    __device__ static uint64_t make(int id, int slot) { return ((uint64_t)(uint32_t)id << 32) | (uint32_t)slot; }
+    __device__ static int slot(uint64_t k) { return (int)(uint32_t)k; }
+    static constexpr uint64_t kMax = ~0ull;
+};
+
+// Ascending bitonic sort of k[0..N); every index is a compile-time constant
+// after unrolling, so the network lives in registers.
+template <int N, class Key>
+__device__ __forceinline__ void bitonic_sort(Key (&k)[N]) {
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const Key x = k[i], y = k[j];
+                    const Key lo = x < y ? x : y, hi = x < y ? y : x;
+                    if ((i & size) == 0) { k[i] = lo; k[j] = hi; }
+                    else { k[i] = hi; k[j] = lo; }
                 }
             }
         }
     }
-    float *outs[3] = {ou, ov, ow};
-    uint8_t *valids[3] = {vu, vv, vw};
-    const int cni[3] = {g.I + 1, g.I, g.I}, cnj[3] = {g.J, g.J + 1, g.J};
+}
+
+// The m hits of this thread (sorted slot q and weight w in LDS): the ids
+// and velocities of all hits are loaded together, w v (the reference's float
+// product) replaces q in LDS, the keys (particle id, hit slot) are sorted in
+// registers, and the sums run in key order.
+template <int N, class Key>
+__device__ __forceinline__ void sorted_sum(const P2GArgs &a, int comp, int (&s_q)[kCap][kThreads],
+                                           const float (&s_w)[kCap][kThreads], int m, float &sum, float &wsum) {
+    const int t = threadIdx.x;
+    const float *vel = a.sv[comp];
+    int id[N];
+    float v[N];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        if (!has[c]) continue;
-        size_t f = (size_t)i + (size_t)cni[c] * ((size_t)j + (size_t)cnj[c] * k);
-        float u = sum[c];
-        if (wsum[c] > eps) u = sum[c] / wsum[c];
-        outs[c][f] = u;
-        valids[c][f] = wsum[c] > eps ? 1 : 0;
+    for (int s = 0; s < N; s++) {
+        id[s] = 0;
+        v[s] = 0.0f;
+        if (s < m) {
+            const int q = s_q[s][t];
+            id[s] = a.ids[q];
+            v[s] = vel[q];
+        }
+    }
+    Key k[N];
+#pragma unroll
+    for (int s = 0; s < N; s++) {
+        k[s] = KeyTraits<Key>::kMax;
+        if (s < m) {
+            k[s] = KeyTraits<Key>::make(id[s], s);
+            s_q[s][t] = __float_as_int(s_w[s][t] * v[s]);
+        }
+    }
+    bitonic_sort<N>(k);
+#pragma unroll
+    for (int s = 0; s < N; s++) {
+        if (s < m) {
+            const int slot = KeyTraits<Key>::slot(k[s]);
+            sum += __int_as_float(s_q[slot][t]);
+            wsum += s_w[slot][t];
+        }
     }
 }
 
-hipError_t launch_p2g3(const GridGeom &g, const float *sx, const float *sy, const float *sz,
-                       const float *const svel[3], const int *offset, double radius, float *const out[3],
-                       uint8_t *const valid[3], hipStream_t s) {
-    size_t n = (size_t)(g.I + 1) * (g.J + 1) * (g.K + 1);
-    k_p2g3<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, sx, sy, sz, svel[0], svel[1], svel[2], offset,
-                                                       (float)radius, radius, out[0], out[1], out[2],
-                                                       valid[0], valid[1], valid[2]);
+// Work tiles: 32 faces along i by 2 along j, at one k, of one component.
+// The U, V and W tiles of the same place are adjacent in tile order, and the
+// blocks of each XCD (workgroups go to the 8 XCDs round-robin) take one
+// contiguous eighth of the tiles, so neighbouring faces -- which read the
+// same particles -- share an L2.
+struct TileMap {
+    int ntc[3];  // i-tiles per component
+    int per_group, njp, ngroups;
+    unsigned ntiles, nblocks;
+};
+
+inline TileMap make_tile_map(const GridGeom &g) {
+    TileMap m;
+    m.ntc[0] = (g.I + 1 + 31) / 32;
+    m.ntc[1] = (g.I + 31) / 32;
+    m.ntc[2] = m.ntc[1];
+    m.per_group = m.ntc[0] + m.ntc[1] + m.ntc[2];
+    m.njp = (g.J + 2) / 2;  // pairs of j covering J + 1 rows (V)
+    m.ngroups = m.njp * (g.K + 1);
+    m.ntiles = (unsigned)m.ngroups * (unsigned)m.per_group;
+    m.nblocks = (m.ntiles + 7u) / 8u * 8u;
+    return m;
+}
+
+// This block's face, or false when its lane has none.
+__device__ __forceinline__ bool tile_face(const GridGeom &g, const TileMap &m, int &comp, size_t &f) {
+    const unsigned b = blockIdx.x;
+    const unsigned tile = (b & 7u) * (m.nblocks >> 3) + (b >> 3);
+    if (tile >= m.ntiles) return false;
+    const int group = (int)(tile / (unsigned)m.per_group);
+    int local = (int)(tile % (unsigned)m.per_group);
+    comp = local < m.ntc[0] ? 0 : (local < m.ntc[0] + m.ntc[1] ? 1 : 2);
+    local -= comp == 0 ? 0 : (comp == 1 ? m.ntc[0] : m.ntc[0] + m.ntc[1]);
+    const int jp = group % m.njp, k = group / m.njp;
+    const int i = local * 32 + (int)(threadIdx.x & 31), j = 2 * jp + (int)(threadIdx.x >> 5);
+    int ni, nj, nk;
+    face_dims(g, comp, ni, nj, nk);
+    if (i >= ni || j >= nj || k >= nk) return false;
+    f = (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
+    return true;
+}
+
+template <class Key>
+__global__ __launch_bounds__(kThreads) void k_p2g(P2GArgs a, TileMap tm) {
+    __shared__ int s_q[kCap][kThreads];
+    __shared__ float s_w[kCap][kThreads];
+    int comp = 0;
+    size_t f = 0;
+    const bool active = tile_face(a.g, tm, comp, f);
+    const int t = threadIdx.x;
+    FaceCtx c;
+    int nh = 0;
+    if (active) {
+        face_setup(a, comp, f, c);
+        // The 16 rows' candidate ranges, all loaded before any is used.
+        const size_t SI = 2 * (size_t)a.g.I, SJ = 2 * (size_t)a.g.J;
+        int rb[16], re[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int sy = c.lo[1] + (r & 3), sz = c.lo[2] + (r >> 2);
+            rb[r] = re[r] = 0;
+            if (sy <= c.hi[1] && sz <= c.hi[2]) {
+                const size_t row = SI * ((size_t)sy + SJ * (size_t)sz);
+                rb[r] = a.sub_offset[row + (size_t)c.lo[0]];
+                re[r] = a.sub_offset[row + (size_t)c.hi[0] + 1];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            for (int q0 = rb[r]; q0 < re[r]; q0 += 4) {
+                // Four candidates' positions in flight at once.
+                float px[4], py[4], pz[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    px[u] = py[u] = pz[u] = 0.0f;
+                    if (q0 + u < re[r]) {
+                        px[u] = a.sx[q0 + u];
+                        py[u] = a.sy[q0 + u];
+                        pz[u] = a.sz[q0 + u];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (q0 + u >= re[r]) break;
+                    float wt;
+                    if (!face_hit_at(a, c, f3(px[u], py[u], pz[u]), wt)) continue;
+                    if (nh < kCap) {
+                        s_q[nh][t] = q0 + u;
+                        s_w[nh][t] = wt;
+                    }
+                    nh++;
+                }
+            }
+        }
+        if (nh > kCap) {
+            const int e = atomicAdd(a.ovf_count, 1);
+            a.ovf_list[e] = (comp << kCompShift) | (int)f;
+        }
+    }
+    const int m = nh <= kCap ? nh : 0;
+    int wm = m;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
+    float sum = 0.0f, wsum = 0.0f;
+    if (wm > 16) sorted_sum<32, Key>(a, comp, s_q, s_w, m, sum, wsum);
+    else if (wm > 8) sorted_sum<16, Key>(a, comp, s_q, s_w, m, sum, wsum);
+    else if (wm > 4) sorted_sum<8, Key>(a, comp, s_q, s_w, m, sum, wsum);
+    else if (wm > 0) sorted_sum<4, Key>(a, comp, s_q, s_w, m, sum, wsum);
+    if (active && nh <= kCap) write_face(a, c, sum, wsum);
+}
+
+// One wavefront per listed face: the hits go to LDS keyed (id, slot), are
+// sorted there, and lane 0 sums them in order.  Beyond kOvfCap hits the
+// wavefront instead selects the next-larger id one at a time, rescanning the
+// candidates (only reachable with hundreds of particles around one face).
+__global__ __launch_bounds__(64) void k_p2g_overflow(P2GArgs a) {
+    __shared__ uint64_t s_k[kOvfCap];
+    __shared__ float2 s_v[kOvfCap];
+    __shared__ int s_n;
+    const int lane = threadIdx.x;
+    const int count = *a.ovf_count;
+    for (int e = blockIdx.x; e < count; e += gridDim.x) {
+        const int code = a.ovf_list[e];
+        FaceCtx c;
+        face_setup(a, code >> kCompShift, (size_t)(code & ((1 << kCompShift) - 1)), c);
+        if (lane == 0) s_n = 0;
+        __syncthreads();
+        for_candidates(a, c, lane, 64, [&](int q) {
+            float prod, wt;
+            if (!face_hit(a, c, q, prod, wt)) return;
+            const int pos = atomicAdd(&s_n, 1);
+            if (pos < kOvfCap) {
+                s_k[pos] = ((uint64_t)(uint32_t)a.ids[q] << 32) | (uint32_t)pos;
+                s_v[pos] = make_float2(prod, wt);
+            }
+        });
+        __syncthreads();
+        const int n = s_n;
+        float sum = 0.0f, wsum = 0.0f;
+        if (n <= kOvfCap) {
+            int P = 1;
+            while (P < n) P <<= 1;
+            for (int x = n + lane; x < P; x += 64) s_k[x] = ~0ull;
+            __syncthreads();
+            for (int size = 2; size <= P; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int x = lane; x < P; x += 64) {
+                        const int y = x ^ stride;
+                        if (y > x) {
+                            const uint64_t u = s_k[x], w = s_k[y];
+                            const bool up = (x & size) == 0;
+                            if ((u > w) == up) { s_k[x] = w; s_k[y] = u; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            if (lane == 0) {
+                for (int x = 0; x < n; x++) {
+                    const float2 v = s_v[(uint32_t)s_k[x]];
+                    sum += v.x;
+                    wsum += v.y;
+                }
+            }
+        } else {
+            long long last = -1;
+            for (;;) {
+                long long best = 0x7fffffffffffffffll;
+                float bp = 0.0f, bw = 0.0f;
+                for_candidates(a, c, lane, 64, [&](int q) {
+                    const long long id = a.ids[q];
+                    if (id <= last || id >= best) return;
+                    float prod, wt;
+                    if (!face_hit(a, c, q, prod, wt)) return;
+                    best = id; bp = prod; bw = wt;
+                });
+                long long wbest = best;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    const long long o = __shfl_xor(wbest, off, 64);
+                    wbest = o < wbest ? o : wbest;
+                }
+                if (wbest == 0x7fffffffffffffffll) break;
+                const unsigned long long who = __ballot(best == wbest);
+                const int src = __ffsll((long long)who) - 1;
+                const float p = __shfl(bp, src, 64), w = __shfl(bw, src, 64);
+                sum += p;  // every lane keeps the same running sums
+                wsum += w;
+                last = wbest;
+            }
+        }
+        if (lane == 0) write_face(a, c, sum, wsum);
+        __syncthreads();
+    }
+}
+
+hipError_t launch_p2g(const GridGeom &g, const float *sx, const float *sy, const float *sz,
+                      const float *const svel[3], const int *ids, const int *sub_offset, int n, double radius,
+                      float *const out[3], uint8_t *const valid[3], int *ovf_count, int *ovf_list,
+                      hipStream_t s) {
+    size_t nmax = 0;
+    for (int comp = 0; comp < 3; comp++) {
+        const size_t nf = (size_t)(g.I + (comp == 0)) * (g.J + (comp == 1)) * (g.K + (comp == 2));
+        nmax = nf > nmax ? nf : nmax;
+    }
+    if (nmax >= ((size_t)1 << kCompShift)) return hipErrorInvalidValue;  // overflow list encoding
+    const TileMap tm = make_tile_map(g);
+    hipError_t e = launch_fill_i32(ovf_count, 1, 0, s);
+    if (e != hipSuccess) return e;
+    P2GArgs a{};
+    a.g = g;
+    a.sx = sx; a.sy = sy; a.sz = sz;
+    for (int c = 0; c < 3; c++) {
+        a.sv[c] = svel[c];
+        a.out[c] = out[c];
+        a.valid[c] = valid[c];
+    }
+    a.ids = ids;
+    a.sub_offset = sub_offset;
+    a.radius_f = (float)radius;
+    a.radius = radius;
+    a.ovf_count = ovf_count;
+    a.ovf_list = ovf_list;
+    if (n < (1 << (32 - KeyTraits<uint32_t>::kSlotBits)))
+        k_p2g<uint32_t><<<tm.nblocks, kThreads, 0, s>>>(a, tm);
+    else
+        k_p2g<uint64_t><<<tm.nblocks, kThreads, 0, s>>>(a, tm);
+    k_p2g_overflow<<<kOvfBlocks, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -177,6 +177,53 @@ hipError_t launch_sort_by_cell(const GridGeom &g, const float *px, const float *
     return hipGetLastError();
 }
 
+// Half-cell ("sub-cell") bin of a position: bins are 2I x 2J x 2K, sx
+// fastest.  The cell part is Grid3d::positionToGridIndex in the same double
+// value the cell sort uses, and the half along each axis is whether that
+// value's fraction is at least 1/2, so a bin's cell is exactly (sx>>1,
+// sy>>1, sz>>1).  -1 outside the grid.
+__device__ __forceinline__ int subcell_key(const GridGeom &g, float x, float y, float z) {
+    const double tx = (double)x * g.invdx, ty = (double)y * g.invdx, tz = (double)z * g.invdx;
+    const double fx = floor(tx), fy = floor(ty), fz = floor(tz);
+    const int cx = (int)fx, cy = (int)fy, cz = (int)fz;
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= g.I || cy >= g.J || cz >= g.K) return -1;
+    const int sx = 2 * cx + (tx - fx >= 0.5 ? 1 : 0);
+    const int sy = 2 * cy + (ty - fy >= 0.5 ? 1 : 0);
+    const int sz = 2 * cz + (tz - fz >= 0.5 ? 1 : 0);
+    return sx + 2 * g.I * (sy + 2 * g.J * sz);
+}
+
+// Bin and arrival slot per particle.  Neighbouring particles rarely share a
+// half-cell, so each lane takes its slot with its own atomic.
+__global__ __launch_bounds__(256) void k_subcell_key(GridGeom g, const float *__restrict__ px,
+                                                     const float *__restrict__ py,
+                                                     const float *__restrict__ pz, int n, int *key, int *slot,
+                                                     int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = subcell_key(g, px[i], py[i], pz[i]);
+    key[i] = b;
+    slot[i] = b >= 0 ? atomicAdd(&count[b], 1) : 0;
+}
+
+// Counting sort of particles into half-cell bins (order within a bin is the
+// arrival order of the atomics; every consumer of this sort -- the liquid
+// SDF's minimum and P2G's per-face sort by particle id -- is independent of
+// it).  sb.count / sb.offset hold 8 I J K (+1) entries.
+hipError_t launch_sort_by_subcell(const GridGeom &g, const float *px, const float *py, const float *pz, int n,
+                                  SortBuffers &sb, hipStream_t s) {
+    const size_t nbins = subcell_bins(g);
+    if (nbins >= (size_t)INT32_MAX) return hipErrorInvalidValue;
+    hipError_t e = launch_fill_i32(sb.count, nbins, 0, s);
+    if (e != hipSuccess) return e;
+    if (n > 0) k_subcell_key<<<(n + 255) / 256, 256, 0, s>>>(g, px, py, pz, n, sb.cell, sb.slot, sb.count);
+    e = launch_exclusive_scan(sb.count, sb.offset, (int)nbins, sb.scan_tmp, s);
+    if (e != hipSuccess) return e;
+    k_set_last<<<1, 1, 0, s>>>(sb.offset, (int)nbins, sb.count);
+    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, s>>>(sb.cell, sb.slot, sb.offset, n, nullptr, sb.order);
+    return hipGetLastError();
+}
+
 // The count grid of FluidSimulation::_removeMarkerParticles
 // (fluidsimulation.cpp:4325-4343): particles are counted in order, skipping
 // those removed by the solid test, and a cell stops counting at the cap

@@ -342,6 +342,11 @@ void Scene::collect_events() {
     pending_events.clear();
 }
 
+// Scratch of the prefix scans: the half-cell bins or the particles.
+static size_t scan_elems(const GridGeom &g, size_t particles) {
+    return std::max(scan_scratch_elems((int)subcell_bins(g)), scan_scratch_elems((int)particles));
+}
+
 void Scene::alloc_all() {
     int I = desc.isize, J = desc.jsize, K = desc.ksize;
     size_t nc = (size_t)I * J * K;
@@ -361,8 +366,11 @@ void Scene::alloc_all() {
     need(keep.alloc(pc)); need(keep_pos.alloc(pc)); need(rank.alloc(pc));
     need(solid_removed.alloc(pc));
     need(count.alloc(nc)); need(offset.alloc(nc + 1)); need(removed_in_cell.alloc(nc));
-    size_t st = std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc));
-    need(scan_tmp.alloc(st));
+    const size_t nsub = subcell_bins(g);
+    if (nsub >= (size_t)INT32_MAX) throw StatusError{FLIPMI_ERR_INVALID_ARG, "grid too large for the half-cell sort"};
+    need(sub_count.alloc(nsub)); need(sub_offset.alloc(nsub + 1));
+    need(p2g_ovf.alloc(p2g_face_count(g))); need(p2g_ovf_count.alloc(1));
+    need(scan_tmp.alloc(scan_elems(g, pc)));
     need(new_count.alloc(1)); need(over_cap.alloc(1)); need(speed_hist.alloc(64)); need(max_speed_bits.alloc(1));
     need(solid.alloc(nn));
     need(wU.alloc(nU)); need(wV.alloc(nV)); need(wW.alloc(nW)); need(wC.alloc(nc));
@@ -494,7 +502,7 @@ size_t Scene::device_bytes() const {
     size_t b = 0;
     for (int a = 0; a < 6; a++) b += part[a].bytes() + part_alt[a].bytes() + sorted[a].bytes();
     b += cell.bytes() + slot.bytes() + count.bytes() + offset.bytes() + order.bytes() + scan_tmp.bytes();
-    b += removed_in_cell.bytes();
+    b += removed_in_cell.bytes() + sub_count.bytes() + sub_offset.bytes() + p2g_ovf.bytes() + p2g_ovf_count.bytes();
     b += keep.bytes() + keep_pos.bytes() + rank.bytes() + new_count.bytes() + over_cap.bytes() + solid_removed.bytes();
     b += speed_hist.bytes() + max_speed_bits.bytes();
     b += solid.bytes() + wU.bytes() + wV.bytes() + wW.bytes() + wC.bytes() + sU.bytes() + sV.bytes() + sW.bytes();
@@ -516,7 +524,9 @@ size_t estimate_scene_bytes(const flipmi_scene_desc &d, int64_t cap) {
     const size_t pc = (size_t)std::max<int64_t>(cap, 1);
     size_t b = pc * (18 * sizeof(float) + 6 * sizeof(int) + 1);
     b += sizeof(int) * (nc + nc + 1 + nc);  // count, offset, removed_in_cell
-    b += sizeof(int) * std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc));
+    const GridGeom g = make_geom(d.isize, d.jsize, d.ksize, d.dx);
+    b += sizeof(int) * (2 * subcell_bins(g) + 1 + p2g_face_count(g) + 1);  // half-cell sort, P2G overflow
+    b += sizeof(int) * scan_elems(g, pc);
     b += sizeof(int) * (1 + 64) + sizeof(unsigned);
     b += sizeof(float) * nn;
     b += sizeof(float) * 2 * (nU + nV + nW) + sizeof(float) * nc;  // weights (faces, centre), solid face velocities
@@ -574,6 +584,11 @@ void Scene::ensure_solid() {
 void Scene::sort_particles(int *over) {
     SortBuffers sb{cell.p, slot.p, count.p, offset.p, order.p, scan_tmp.p, scan_tmp.n, over};
     check(launch_sort_by_cell(g, part[0].p, part[1].p, part[2].p, n, sb, stream), "sort");
+}
+
+void Scene::sort_subcells() {
+    SortBuffers sb{cell.p, slot.p, sub_count.p, sub_offset.p, order.p, scan_tmp.p, scan_tmp.n};
+    check(launch_sort_by_subcell(g, part[0].p, part[1].p, part[2].p, n, sb, stream), "half-cell sort");
 }
 
 void Scene::extrapolate_all(int layers) {
@@ -1284,8 +1299,7 @@ void Scene::grow(int64_t need_n) {
     need(cell.alloc(pc)); need(slot.alloc(pc)); need(order.alloc(pc));
     need(keep.alloc(pc)); need(keep_pos.alloc(pc)); need(rank.alloc(pc));
     need(solid_removed.alloc(pc));
-    const size_t nc = (size_t)g.I * g.J * g.K;
-    need(scan_tmp.alloc(std::max(scan_scratch_elems((int)nc), scan_scratch_elems((int)pc))));
+    need(scan_tmp.alloc(scan_elems(g, pc)));
     cap = (int)target;
 }
 
@@ -1331,18 +1345,17 @@ void Scene::step_fluid(double dt) {
     hipEvent_t ev = nullptr;
     // Liquid SDF + post-process (fluidsimulation.cpp:3225-3247).
     begin_stage(FLIPMI_STAGE_SORT, &ev);
-    sort_particles();
+    sort_subcells();
     const float *src[6];
     float *dst[6];
     for (int a = 0; a < 6; a++) {
         src[a] = part[a].p;
         dst[a] = sorted[a].p;
     }
-    check(launch_gather_sorted(order.p, offset.p + (size_t)g.I * g.J * g.K, n, src, dst, stream),
-          "gather");
+    check(launch_gather_sorted(order.p, sub_offset.p + subcell_bins(g), n, src, dst, stream), "gather");
     end_stage(FLIPMI_STAGE_SORT, ev);
     begin_stage(FLIPMI_STAGE_CLASSIFY, &ev);
-    check(launch_liquid_sdf(g, sorted[0].p, sorted[1].p, sorted[2].p, offset.p, radius, solid.p,
+    check(launch_liquid_sdf(g, sorted[0].p, sorted[1].p, sorted[2].p, sub_offset.p, radius, solid.p,
                             liquid.p, stream),
           "liquid sdf");
     end_stage(FLIPMI_STAGE_CLASSIFY, ev);
@@ -1354,7 +1367,8 @@ void Scene::step_fluid(double dt) {
         const float *svel[3] = {sorted[3].p, sorted[4].p, sorted[5].p};
         float *outs[3] = {u.p, v.p, w.p};
         uint8_t *valids[3] = {vu.p, vv.p, vw.p};
-        check(launch_p2g3(g, sorted[0].p, sorted[1].p, sorted[2].p, svel, offset.p, radius, outs, valids, stream),
+        check(launch_p2g(g, sorted[0].p, sorted[1].p, sorted[2].p, svel, order.p, sub_offset.p, n, radius, outs,
+                         valids, p2g_ovf_count.p, p2g_ovf.p, stream),
               "p2g");
         end_stage(FLIPMI_STAGE_P2G, ev);
         extrapolate_all(layers);

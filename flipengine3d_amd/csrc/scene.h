@@ -103,6 +103,9 @@ struct Scene {
     int n = 0, cap = 0;
     DBuf<float> part[6], part_alt[6], sorted[6];
     DBuf<int> cell, slot, count, offset, order, scan_tmp;
+    // Half-cell bins of the step's particle sort (8 I J K, +1), read by the
+    // liquid SDF and P2G; P2G's list of faces with more than 32 contributions.
+    DBuf<int> sub_count, sub_offset, p2g_ovf, p2g_ovf_count;
     DBuf<int> keep, keep_pos, rank, new_count, over_cap;
     DBuf<uint8_t> solid_removed;
     // Solid-removed particles per cell at the last removal pass; with `count`
@@ -192,6 +195,7 @@ struct Scene {
     void pressure_solve(double dt);
     void advance_particles(double dt);
     void sort_particles(int *over = nullptr);
+    void sort_subcells();
     void extrapolate_all(int layers);
     // Mesh inflow sources: add (checks the mesh), build the device state of
     // every source, their device views, and the emission at the end of a

@@ -39,13 +39,20 @@ def test_liquid_sdf(oracle, reference, dims):
         assert bits_equal(a, b), f"ndiff={ndiff(a, b)}"
 
 
+@pytest.mark.parametrize("shuffled", [False, True])
 @pytest.mark.parametrize("dims", SIZES)
-def test_p2g(oracle, reference, dims):
+def test_p2g(oracle, reference, dims, shuffled):
+    # Shuffled: particle index order unrelated to position, so every face's
+    # sum depends on summing in the reference's ascending-index order.
     I, J, K = dims
     s = dam(I, J, K, seed=5)
+    pos = s.positions
     vel = random_velocities(s.num_particles, seed=1)
-    (mo, vo) = oracle.p2g(I, J, K, s.dx, s.positions, vel, radius(s.dx))
-    (mr, vr) = reference.p2g(I, J, K, s.dx, s.positions, vel, radius(s.dx))
+    if shuffled:
+        perm = np.random.default_rng(11).permutation(s.num_particles)
+        pos, vel = pos[perm], vel[perm]
+    (mo, vo) = oracle.p2g(I, J, K, s.dx, pos, vel, radius(s.dx))
+    (mr, vr) = reference.p2g(I, J, K, s.dx, pos, vel, radius(s.dx))
     for a, b in zip(mo + vo, mr + vr):
         assert bits_equal(a, b), f"ndiff={ndiff(a, b)}"
 
