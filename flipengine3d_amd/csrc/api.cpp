@@ -423,7 +423,8 @@ namespace {
 // Sorts host particles on the device -- by cell, or into half-cell bins as
 // the scene's step does -- and returns sorted SoA arrays.
 struct SortedParticles {
-    DBuf<float> raw[6], sorted[6];
+    DBuf<float> raw[6], svel[3];
+    DBuf<float4> spos;
     DBuf<int> cell, slot, count, offset, order, scan_tmp;
     void build(const GridGeom &g, const float *pos, const float *vel, int64_t n, hipStream_t s,
                bool half_cells = false) {
@@ -434,7 +435,8 @@ struct SortedParticles {
         const size_t nbins = half_cells ? subcell_bins(g) : (size_t)g.I * g.J * g.K;
         if (nbins >= (size_t)INT32_MAX) throw StatusError{FLIPMI_ERR_INVALID_ARG, "grid too large"};
         size_t pn = (size_t)std::max<int64_t>(n, 1);
-        for (int a = 0; a < 6; a++) alloc_dev(sorted[a], pn);
+        alloc_dev(spos, pn);
+        for (int a = 0; a < 3; a++) alloc_dev(svel[a], pn);
         alloc_dev(cell, pn); alloc_dev(slot, pn); alloc_dev(order, pn);
         alloc_dev(count, nbins); alloc_dev(offset, nbins + 1);
         alloc_dev(scan_tmp, std::max(scan_scratch_elems((int)nbins), scan_scratch_elems((int)pn)));
@@ -442,9 +444,9 @@ struct SortedParticles {
         if (half_cells) hipck(launch_sort_by_subcell(g, raw[0].p, raw[1].p, raw[2].p, (int)n, sb, s), "sort");
         else hipck(launch_sort_by_cell(g, raw[0].p, raw[1].p, raw[2].p, (int)n, sb, s), "sort");
         const float *src[6];
-        float *dst[6];
-        for (int a = 0; a < 6; a++) { src[a] = raw[a].p; dst[a] = sorted[a].p; }
-        hipck(launch_gather_sorted(order.p, offset.p + nbins, (int)n, src, dst, s), "gather");
+        for (int a = 0; a < 6; a++) src[a] = raw[a].p;
+        float *sv[3] = {svel[0].p, svel[1].p, svel[2].p};
+        hipck(launch_gather_sorted(order.p, offset.p + nbins, (int)n, src, spos.p, sv, s), "gather");
     }
 };
 }  // namespace
@@ -461,9 +463,7 @@ int flipmi_liquid_sdf(flipmi_ctx *ctx, int I, int J, int K, double dx, const flo
         DBuf<float> solid, phi;
         if (solid_phi_nodal) upload(solid, solid_phi_nodal, nn, s);
         alloc_dev(phi, (size_t)I * J * K);
-        hipck(launch_liquid_sdf(g, sp.sorted[0].p, sp.sorted[1].p, sp.sorted[2].p, sp.offset.p, radius,
-                                solid.p, phi.p, s),
-              "liquid sdf");
+        hipck(launch_liquid_sdf(g, sp.spos.p, sp.offset.p, radius, solid.p, phi.p, s), "liquid sdf");
         download(phi_out, phi, phi.n, s);
         hipck(hipStreamSynchronize(s), "sync");
     });
@@ -529,11 +529,10 @@ int flipmi_p2g(flipmi_ctx *ctx, int I, int J, int K, double dx, const float *pos
         alloc_dev(dw, (size_t)I * J * (K + 1));
         alloc_dev(dvu, du.n); alloc_dev(dvv, dv.n); alloc_dev(dvw, dw.n);
         alloc_dev(ovf, p2g_face_count(g)); alloc_dev(ovf_count, 1);
-        const float *svel[3] = {sp.sorted[3].p, sp.sorted[4].p, sp.sorted[5].p};
+        const float *svel[3] = {sp.svel[0].p, sp.svel[1].p, sp.svel[2].p};
         float *outs[3] = {du.p, dv.p, dw.p};
         uint8_t *valids[3] = {dvu.p, dvv.p, dvw.p};
-        hipck(launch_p2g(g, sp.sorted[0].p, sp.sorted[1].p, sp.sorted[2].p, svel, sp.order.p, sp.offset.p, (int)n,
-                         radius, outs, valids, ovf_count.p, ovf.p, s),
+        hipck(launch_p2g(g, sp.spos.p, svel, sp.offset.p, (int)n, radius, outs, valids, ovf_count.p, ovf.p, s),
               "p2g");
         download(u, du, du.n, s); download(v, dv, dv.n, s); download(w, dw, dw.n, s);
         download(vu, dvu, dvu.n, s); download(vv, dvv, dvv.n, s); download(vw, dvw, dvw.n, s);

@@ -76,29 +76,30 @@ hipError_t launch_removed_in_cell(const GridGeom &g, const float *px, const floa
                                   const uint8_t *removed, int n, int *removed_in_cell, hipStream_t s);
 hipError_t launch_removal_counts(const GridGeom &g, const int *count, const int *removed_in_cell, int cap,
                                  int *out, hipStream_t s);
-// Gathers the first *nsorted (= the sort's last offset) entries of the order.
-hipError_t launch_gather_sorted(const int *order, const int *nsorted, int n,
-                                const float *const src[6], float *const dst[6], hipStream_t s);
+// Gathers the first *nsorted (= the sort's last offset) entries of the
+// order from the SoA particles src (x, y, z, u, v, w): spos = (x, y, z,
+// particle id as int bits), svel = the velocity components.
+hipError_t launch_gather_sorted(const int *order, const int *nsorted, int n, const float *const src[6],
+                                float4 *spos, float *const svel[3], hipStream_t s);
 
 // ---- classification (k_classify.hip) ----
-// From particles sorted into half-cell bins (sub_offset: subcell_bins(g) + 1).
-hipError_t launch_liquid_sdf(const GridGeom &g, const float *sx, const float *sy, const float *sz,
-                             const int *sub_offset, double radius, const float *solid_phi_nodal,
-                             float *phi, hipStream_t s);
+// From particles sorted into half-cell bins (spos from launch_gather_sorted,
+// sub_offset: subcell_bins(g) + 1).
+hipError_t launch_liquid_sdf(const GridGeom &g, const float4 *spos, const int *sub_offset, double radius,
+                             const float *solid_phi_nodal, float *phi, hipStream_t s);
 // ParticleLevelSet::postProcessSignedDistanceField on an existing field.
 hipError_t launch_liquid_postprocess(const GridGeom &g, const float *solid_phi_nodal, float *phi,
                                      hipStream_t s);
 
 // ---- P2G (k_p2g.hip) ----
 // P2G of the three MAC components from particles sorted into half-cell bins
-// (positions sx/sy/sz, velocities svel, particle ids `ids` in sorted order),
-// each face summed in ascending particle id.  ovf_count (1 int) and
-// ovf_list (faces of U, V and W together) are scratch for faces with more
-// than 32 contributions.  n = particle count (ids below n).
-hipError_t launch_p2g(const GridGeom &g, const float *sx, const float *sy, const float *sz,
-                      const float *const svel[3], const int *ids, const int *sub_offset, int n, double radius,
-                      float *const out[3], uint8_t *const valid[3], int *ovf_count, int *ovf_list,
-                      hipStream_t s);
+// (spos / svel from launch_gather_sorted), each face summed in ascending
+// particle id.  ovf_count (1 int) and ovf_list (faces of U, V and W
+// together) are scratch for faces with more than 32 contributions.
+// n = particle count (ids below n).
+hipError_t launch_p2g(const GridGeom &g, const float4 *spos, const float *const svel[3], const int *sub_offset,
+                      int n, double radius, float *const out[3], uint8_t *const valid[3], int *ovf_count,
+                      int *ovf_list, hipStream_t s);
 // Faces of U, V and W together (the size of launch_p2g's overflow list).
 inline size_t p2g_face_count(const GridGeom &g) {
     return (size_t)(g.I + 1) * g.J * g.K + (size_t)g.I * (g.J + 1) * g.K + (size_t)g.I * g.J * (g.K + 1);

@@ -59,8 +59,7 @@ constexpr int kRows = 4 * kSpan * kSpan;           // half-cell rows of the cand
 constexpr int kRowsPerThread = (kRows + kSdfThreads - 1) / kSdfThreads;
 
 __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
-    GridGeom g, const float *__restrict__ sx, const float *__restrict__ sy,
-    const float *__restrict__ sz, const int *__restrict__ sub_offset, float radius,
+    GridGeom g, const float4 *__restrict__ spos, const int *__restrict__ sub_offset, float radius,
     const float *__restrict__ solid, float *phi_out) {
     __shared__ uint32_t phi_ord[kBW * kBW * kBW];
     __shared__ int run_begin[kRows];
@@ -135,7 +134,8 @@ __global__ __launch_bounds__(kSdfThreads) void k_liquid_sdf(
             if (run_prefix[mid] <= t) lo = mid; else hi = mid - 1;
         }
         int q = run_begin[lo] + (t - run_prefix[lo]);
-        F3 p0 = f3(sx[q], sy[q], sz[q]);
+        const float4 p4 = spos[q];
+        F3 p0 = f3(p4.x, p4.y, p4.z);
         if (!point_in_block(p0, sr, blockdx, inv_blockdx, bi, bj, bk)) continue;
         F3 p = p0 - boff;
         I3 gmin = pos_to_index((double)(p.x - sr), (double)(p.y - sr), (double)(p.z - sr), g.invdx);
@@ -198,12 +198,11 @@ hipError_t launch_liquid_postprocess(const GridGeom &g, const float *solid_phi_n
     return hipGetLastError();
 }
 
-hipError_t launch_liquid_sdf(const GridGeom &g, const float *sx, const float *sy, const float *sz,
-                             const int *sub_offset, double radius, const float *solid_phi_nodal,
-                             float *phi, hipStream_t s) {
+hipError_t launch_liquid_sdf(const GridGeom &g, const float4 *spos, const int *sub_offset, double radius,
+                             const float *solid_phi_nodal, float *phi, hipStream_t s) {
     int nbi = (g.I + kBW - 1) / kBW, nbj = (g.J + kBW - 1) / kBW, nbk = (g.K + kBW - 1) / kBW;
-    k_liquid_sdf<<<nbi * nbj * nbk, kSdfThreads, 0, s>>>(g, sx, sy, sz, sub_offset, (float)radius,
-                                                         solid_phi_nodal, phi);
+    k_liquid_sdf<<<nbi * nbj * nbk, kSdfThreads, 0, s>>>(g, spos, sub_offset, (float)radius, solid_phi_nodal,
+                                                         phi);
     return hipGetLastError();
 }
 

@@ -39,9 +39,8 @@ constexpr int kCompShift = 29;   // overflow list entry: comp << 29 | face
 
 struct P2GArgs {
     GridGeom g;
-    const float *sx, *sy, *sz;  // positions, half-cell order
+    const float4 *sp;           // (x, y, z, particle id bits), half-cell order
     const float *sv[3];         // velocity components, half-cell order
-    const int *ids;             // particle id of each sorted slot
     const int *sub_offset;      // [8 I J K + 1]
     float radius_f;
     double radius;
@@ -70,7 +69,7 @@ struct FaceCtx {
     int comp, i, j, k;
     size_t f;           // flat index in the component's grid
     F3 off;             // _getDirectionOffset
-    float sr, blockdx, rsq, coef1, coef2, coef3;
+    float sr, blockdx, rsq, rsq_fast, coef1, coef2, coef3;
     double inv_blockdx;
     int bi, bj, bk, li, lj, lk;
     F3 boff, gpos;
@@ -101,6 +100,9 @@ __device__ __forceinline__ void face_setup(const P2GArgs &a, int comp, size_t f,
     c.inv_blockdx = 1.0 / (double)c.blockdx;
     float r = a.radius_f;
     c.rsq = r * r;
+    // (0.99 r)^2, below which the assignment tests are skipped (face_hit_at).
+    const bool small_grid = g.I <= 16384 && g.J <= 16384 && g.K <= 16384;
+    c.rsq_fast = small_grid ? 0.98f * c.rsq : 0.0f;
     c.coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
     c.coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
     c.coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
@@ -119,6 +121,24 @@ __device__ __forceinline__ void face_setup(const P2GArgs &a, int comp, size_t f,
     }
 }
 
+// velocityadvector.cpp:430 (Wyvill kernel), as evaluated there.
+__device__ __forceinline__ float face_weight(const FaceCtx &c, float d2) {
+    return 1.0f - c.coef1 * d2 * d2 * d2 + c.coef2 * d2 * d2 - c.coef3 * d2;
+}
+
+// The reference's block assignment of the (offset) particle p and the
+// face's local index range over its search box (velocityadvector.cpp:
+// 280-320, 397-408), for a pair already within r.
+__device__ __forceinline__ bool assignment_tests(const P2GArgs &a, const FaceCtx &c, F3 p, F3 pl) {
+    if (!p2g_point_in_block(p, c.sr, c.blockdx, c.inv_blockdx, c.bi, c.bj, c.bk)) return false;
+    I3 gmin = pos_to_index((double)(pl.x - c.sr), (double)(pl.y - c.sr), (double)(pl.z - c.sr), a.g.invdx);
+    I3 gmax = pos_to_index((double)(pl.x + c.sr), (double)(pl.y + c.sr), (double)(pl.z + c.sr), a.g.invdx);
+    gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
+    gmax.i = min(gmax.i, kChunk - 1); gmax.j = min(gmax.j, kChunk - 1); gmax.k = min(gmax.k, kChunk - 1);
+    return !(c.li < gmin.i || c.li > gmax.i || c.lj < gmin.j || c.lj > gmax.j || c.lk < gmin.k ||
+             c.lk > gmax.k);
+}
+
 // The reference's per-pair test and weight for a particle at p0.
 __device__ __forceinline__ bool face_hit_at(const P2GArgs &a, const FaceCtx &c, F3 p0, float &wt) {
     const F3 p = p0 - c.off;
@@ -126,23 +146,43 @@ __device__ __forceinline__ bool face_hit_at(const P2GArgs &a, const FaceCtx &c, 
     const F3 v = c.gpos - pl;
     const float d2 = dot3(v, v);
     if (!(d2 < c.rsq)) return false;
-    if (!p2g_point_in_block(p, c.sr, c.blockdx, c.inv_blockdx, c.bi, c.bj, c.bk)) return false;
-    I3 gmin = pos_to_index((double)(pl.x - c.sr), (double)(pl.y - c.sr), (double)(pl.z - c.sr), a.g.invdx);
-    I3 gmax = pos_to_index((double)(pl.x + c.sr), (double)(pl.y + c.sr), (double)(pl.z + c.sr), a.g.invdx);
-    gmin.i = max(gmin.i, 0); gmin.j = max(gmin.j, 0); gmin.k = max(gmin.k, 0);
-    gmax.i = min(gmax.i, kChunk - 1); gmax.j = min(gmax.j, kChunk - 1); gmax.k = min(gmax.k, kChunk - 1);
-    if (c.li < gmin.i || c.li > gmax.i || c.lj < gmin.j || c.lj > gmax.j || c.lk < gmin.k || c.lk > gmax.k)
-        return false;
-    wt = 1.0f - c.coef1 * d2 * d2 * d2 + c.coef2 * d2 * d2 - c.coef3 * d2;
+    // The block-assignment and index-range tests below pass for every pair
+    // within r in exact arithmetic: the particle's search box reaches
+    // r + 1e-6 > |p - f| on each axis, so it contains the face, hence covers
+    // the face's block (overlapping case) or lies inside it (simple case),
+    // and the face's local index lies in [floor((p - sr)/dx),
+    // floor((p + sr)/dx)].  They can only fail through float rounding at a
+    // distance within rounding of r.  Inside 0.99 r the margin is at least
+    // 0.0087 dx, over a hundred times the rounding of any coordinate of a
+    // grid up to 16384 cells per axis, so such pairs skip the tests.
+    if (!(d2 < c.rsq_fast) && !assignment_tests(a, c, p, pl)) return false;
+    wt = face_weight(c, d2);
     return true;
+}
+
+// The distance part alone (the main kernel's inner loop): 0 = no hit,
+// 1 = hit, 2 = within r but near it, needs the assignment tests (settled
+// later by face_hit_at on the same particle).  d2 is the squared distance.
+__device__ __forceinline__ int face_probe(const FaceCtx &c, F3 p0, float &d2) {
+    const F3 p = p0 - c.off;
+    const F3 pl = p - c.boff;
+    const F3 v = c.gpos - pl;
+    d2 = dot3(v, v);
+    return !(d2 < c.rsq) ? 0 : (d2 < c.rsq_fast ? 1 : 2);
 }
 
 // The same for the particle in sorted slot q, with its w v
 // (`scalar += weight * velocity`, velocityadvector.cpp:438).
 __device__ __forceinline__ bool face_hit(const P2GArgs &a, const FaceCtx &c, int q, float &prod, float &wt) {
-    if (!face_hit_at(a, c, f3(a.sx[q], a.sy[q], a.sz[q]), wt)) return false;
+    const float4 p4 = a.sp[q];
+    if (!face_hit_at(a, c, f3(p4.x, p4.y, p4.z), wt)) return false;
     prod = wt * a.sv[c.comp][q];
     return true;
+}
+
+// The particle id stored in the fourth word of sorted slot q.
+__device__ __forceinline__ int sorted_id(const P2GArgs &a, int q) {
+    return reinterpret_cast<const int *>(a.sp)[4 * (size_t)q + 3];
 }
 
 // Calls fn(q) for every candidate slot q of the face, lanes of a wavefront
@@ -213,35 +253,43 @@ __device__ __forceinline__ void bitonic_sort(Key (&k)[N]) {
 // product) replaces q in LDS, the keys (particle id, hit slot) are sorted in
 // registers, and the sums run in key order.
 template <int N, class Key>
-__device__ __forceinline__ void sorted_sum(const P2GArgs &a, int comp, int (&s_q)[kCap][kThreads],
-                                           const float (&s_w)[kCap][kThreads], int m, float &sum, float &wsum) {
+__device__ __forceinline__ void sorted_sum(const P2GArgs &a, const FaceCtx &c, int comp,
+                                           int (&s_q)[kCap + 1][kThreads], float (&s_w)[kCap + 1][kThreads], int m,
+                                           int dropped, float &sum, float &wsum) {
+    // s_w holds each hit's squared distance on entry.  Slots with q = -1
+    // (pairs that failed the assignment tests) sort last and are not summed.
     const int t = threadIdx.x;
     const float *vel = a.sv[comp];
     int id[N];
     float v[N];
 #pragma unroll
     for (int s = 0; s < N; s++) {
-        id[s] = 0;
+        id[s] = -1;
         v[s] = 0.0f;
         if (s < m) {
             const int q = s_q[s][t];
-            id[s] = a.ids[q];
-            v[s] = vel[q];
+            if (q >= 0) {
+                id[s] = sorted_id(a, q);
+                v[s] = vel[q];
+            }
         }
     }
     Key k[N];
 #pragma unroll
     for (int s = 0; s < N; s++) {
         k[s] = KeyTraits<Key>::kMax;
-        if (s < m) {
+        if (s < m && id[s] >= 0) {
             k[s] = KeyTraits<Key>::make(id[s], s);
-            s_q[s][t] = __float_as_int(s_w[s][t] * v[s]);
+            const float wt = face_weight(c, s_w[s][t]);
+            s_q[s][t] = __float_as_int(wt * v[s]);
+            s_w[s][t] = wt;
         }
     }
     bitonic_sort<N>(k);
+    const int kept = m - dropped;
 #pragma unroll
     for (int s = 0; s < N; s++) {
-        if (s < m) {
+        if (s < kept) {
             const int slot = KeyTraits<Key>::slot(k[s]);
             sum += __int_as_float(s_q[slot][t]);
             wsum += s_w[slot][t];
@@ -293,59 +341,102 @@ __device__ __forceinline__ bool tile_face(const GridGeom &g, const TileMap &m, i
 
 template <class Key>
 __global__ __launch_bounds__(kThreads) void k_p2g(P2GArgs a, TileMap tm) {
-    __shared__ int s_q[kCap][kThreads];
-    __shared__ float s_w[kCap][kThreads];
+    // Per thread: the hits' sorted slots and squared distances; slot kCap
+    // takes the stores of non-hits and of hits beyond the cap.
+    __shared__ int s_q[kCap + 1][kThreads];
+    __shared__ float s_w[kCap + 1][kThreads];
     int comp = 0;
     size_t f = 0;
     const bool active = tile_face(a.g, tm, comp, f);
     const int t = threadIdx.x;
     FaceCtx c;
-    int nh = 0;
+    int nh = 0, dropped = 0;
     if (active) {
         face_setup(a, comp, f, c);
-        // The 16 rows' candidate ranges, all loaded before any is used.
+        // Candidate ranges of the 4 rows (sy) at half-cell plane sz.
         const size_t SI = 2 * (size_t)a.g.I, SJ = 2 * (size_t)a.g.J;
-        int rb[16], re[16];
+        auto plane_ranges = [&](int sz, int (&b)[4], int (&e)[4]) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int sy = c.lo[1] + (r & 3), sz = c.lo[2] + (r >> 2);
-            rb[r] = re[r] = 0;
-            if (sy <= c.hi[1] && sz <= c.hi[2]) {
-                const size_t row = SI * ((size_t)sy + SJ * (size_t)sz);
-                rb[r] = a.sub_offset[row + (size_t)c.lo[0]];
-                re[r] = a.sub_offset[row + (size_t)c.hi[0] + 1];
+            for (int rr = 0; rr < 4; rr++) {
+                const int sy = c.lo[1] + rr;
+                b[rr] = e[rr] = 0;
+                if (sy <= c.hi[1] && sz <= c.hi[2]) {
+                    const size_t row = SI * ((size_t)sy + SJ * (size_t)sz);
+                    b[rr] = a.sub_offset[row + (size_t)c.lo[0]];
+                    e[rr] = a.sub_offset[row + (size_t)c.hi[0] + 1];
+                }
             }
-        }
+        };
+        // Every candidate stores (q, d2) at slot min(nh, kCap) and nh counts
+        // only hits, so no branch depends on the outcome.  Pairs near r go in
+        // as -(q + 2) and are settled after the loop, by one copy of the
+        // assignment tests.
+        bool pending = false;
+        auto consider = [&](float4 p4, int q, bool valid) {
+            float d2;
+            int h = face_probe(c, f3(p4.x, p4.y, p4.z), d2);
+            h = valid ? h : 0;
+            const int slot = nh < kCap ? nh : kCap;
+            s_q[slot][t] = h == 1 ? q : -(q + 2);
+            s_w[slot][t] = d2;
+            pending |= h == 2;
+            nh += h != 0;
+        };
+        // One half-cell plane (4 rows) per iteration: the first four
+        // candidates of each row are loaded together (16 loads in flight)
+        // while the next plane's ranges are fetched, then tested; rows with
+        // more candidates (denser than one particle per half-cell) finish one
+        // by one.  A plane no lane of the wavefront has candidates in is
+        // skipped.
+        int cb[4], ce[4], nb[4], ne[4];
+        plane_ranges(c.lo[2], cb, ce);
+#pragma unroll 1
+        for (int pl = 0; pl < 4; pl++) {
+            const bool mine = cb[0] < ce[0] || cb[1] < ce[1] || cb[2] < ce[2] || cb[3] < ce[3];
+            plane_ranges(c.lo[2] + pl + 1, nb, ne);
+            if (__ballot(mine) != 0ull) {
+                float4 p[4][4];
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            for (int q0 = rb[r]; q0 < re[r]; q0 += 4) {
-                // Four candidates' positions in flight at once.
-                float px[4], py[4], pz[4];
+                for (int rr = 0; rr < 4; rr++) {
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    px[u] = py[u] = pz[u] = 0.0f;
-                    if (q0 + u < re[r]) {
-                        px[u] = a.sx[q0 + u];
-                        py[u] = a.sy[q0 + u];
-                        pz[u] = a.sz[q0 + u];
+                    for (int u = 0; u < 4; u++) {
+                        p[rr][u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        if (cb[rr] + u < ce[rr]) p[rr][u] = a.sp[cb[rr] + u];
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    if (q0 + u >= re[r]) break;
-                    float wt;
-                    if (!face_hit_at(a, c, f3(px[u], py[u], pz[u]), wt)) continue;
-                    if (nh < kCap) {
-                        s_q[nh][t] = q0 + u;
-                        s_w[nh][t] = wt;
-                    }
-                    nh++;
+                for (int rr = 0; rr < 4; rr++) {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) consider(p[rr][u], cb[rr] + u, cb[rr] + u < ce[rr]);
                 }
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++)
+                    for (int q = cb[rr] + 4; q < ce[rr]; q++) consider(a.sp[q], q, true);
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                cb[rr] = nb[rr];
+                ce[rr] = ne[rr];
             }
         }
         if (nh > kCap) {
             const int e = atomicAdd(a.ovf_count, 1);
             a.ovf_list[e] = (comp << kCompShift) | (int)f;
+        } else if (pending) {
+            // Settle the pairs near r; the ones that fail become -1 and are
+            // left out of the sums.
+            for (int s = 0; s < nh; s++) {
+                const int qe = s_q[s][t];
+                if (qe >= -1) continue;
+                const int q = -qe - 2;
+                float prod, wt;
+                if (face_hit(a, c, q, prod, wt)) {
+                    s_q[s][t] = q;  // its squared distance stays in s_w
+                } else {
+                    s_q[s][t] = -1;
+                    dropped++;
+                }
+            }
         }
     }
     const int m = nh <= kCap ? nh : 0;
@@ -353,10 +444,10 @@ __global__ __launch_bounds__(kThreads) void k_p2g(P2GArgs a, TileMap tm) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
     float sum = 0.0f, wsum = 0.0f;
-    if (wm > 16) sorted_sum<32, Key>(a, comp, s_q, s_w, m, sum, wsum);
-    else if (wm > 8) sorted_sum<16, Key>(a, comp, s_q, s_w, m, sum, wsum);
-    else if (wm > 4) sorted_sum<8, Key>(a, comp, s_q, s_w, m, sum, wsum);
-    else if (wm > 0) sorted_sum<4, Key>(a, comp, s_q, s_w, m, sum, wsum);
+    if (wm > 16) sorted_sum<32, Key>(a, c, comp, s_q, s_w, m, dropped, sum, wsum);
+    else if (wm > 8) sorted_sum<16, Key>(a, c, comp, s_q, s_w, m, dropped, sum, wsum);
+    else if (wm > 4) sorted_sum<8, Key>(a, c, comp, s_q, s_w, m, dropped, sum, wsum);
+    else if (wm > 0) sorted_sum<4, Key>(a, c, comp, s_q, s_w, m, dropped, sum, wsum);
     if (active && nh <= kCap) write_face(a, c, sum, wsum);
 }
 
@@ -381,7 +472,7 @@ __global__ __launch_bounds__(64) void k_p2g_overflow(P2GArgs a) {
             if (!face_hit(a, c, q, prod, wt)) return;
             const int pos = atomicAdd(&s_n, 1);
             if (pos < kOvfCap) {
-                s_k[pos] = ((uint64_t)(uint32_t)a.ids[q] << 32) | (uint32_t)pos;
+                s_k[pos] = ((uint64_t)(uint32_t)sorted_id(a, q) << 32) | (uint32_t)pos;
                 s_v[pos] = make_float2(prod, wt);
             }
         });
@@ -419,7 +510,7 @@ __global__ __launch_bounds__(64) void k_p2g_overflow(P2GArgs a) {
                 long long best = 0x7fffffffffffffffll;
                 float bp = 0.0f, bw = 0.0f;
                 for_candidates(a, c, lane, 64, [&](int q) {
-                    const long long id = a.ids[q];
+                    const long long id = sorted_id(a, q);
                     if (id <= last || id >= best) return;
                     float prod, wt;
                     if (!face_hit(a, c, q, prod, wt)) return;
@@ -445,10 +536,9 @@ __global__ __launch_bounds__(64) void k_p2g_overflow(P2GArgs a) {
     }
 }
 
-hipError_t launch_p2g(const GridGeom &g, const float *sx, const float *sy, const float *sz,
-                      const float *const svel[3], const int *ids, const int *sub_offset, int n, double radius,
-                      float *const out[3], uint8_t *const valid[3], int *ovf_count, int *ovf_list,
-                      hipStream_t s) {
+hipError_t launch_p2g(const GridGeom &g, const float4 *spos, const float *const svel[3], const int *sub_offset,
+                      int n, double radius, float *const out[3], uint8_t *const valid[3], int *ovf_count,
+                      int *ovf_list, hipStream_t s) {
     size_t nmax = 0;
     for (int comp = 0; comp < 3; comp++) {
         const size_t nf = (size_t)(g.I + (comp == 0)) * (g.J + (comp == 1)) * (g.K + (comp == 2));
@@ -460,13 +550,12 @@ hipError_t launch_p2g(const GridGeom &g, const float *sx, const float *sy, const
     if (e != hipSuccess) return e;
     P2GArgs a{};
     a.g = g;
-    a.sx = sx; a.sy = sy; a.sz = sz;
+    a.sp = spos;
     for (int c = 0; c < 3; c++) {
         a.sv[c] = svel[c];
         a.out[c] = out[c];
         a.valid[c] = valid[c];
     }
-    a.ids = ids;
     a.sub_offset = sub_offset;
     a.radius_f = (float)radius;
     a.radius = radius;

@@ -264,24 +264,25 @@ hipError_t launch_removal_counts(const GridGeom &g, const int *count, const int 
     return hipGetLastError();
 }
 
-__global__ void k_gather6(const int *__restrict__ order, const int *__restrict__ nsorted,
-                          const float *__restrict__ a0,
-                          const float *__restrict__ a1, const float *__restrict__ a2,
-                          const float *__restrict__ a3, const float *__restrict__ a4,
-                          const float *__restrict__ a5, float *b0, float *b1, float *b2, float *b3,
-                          float *b4, float *b5) {
+// Sorted copies: the position with the particle's id in the fourth word
+// (one 16-byte load per candidate for the gathers), and the velocity.
+__global__ void k_gather_sorted(const int *__restrict__ order, const int *__restrict__ nsorted,
+                                const float *__restrict__ px, const float *__restrict__ py,
+                                const float *__restrict__ pz, const float *__restrict__ vx,
+                                const float *__restrict__ vy, const float *__restrict__ vz, float4 *spos,
+                                float *su, float *sv, float *sw) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *nsorted) return;  // particles outside the grid are not in the order
     int o = order[i];
-    b0[i] = a0[o]; b1[i] = a1[o]; b2[i] = a2[o];
-    b3[i] = a3[o]; b4[i] = a4[o]; b5[i] = a5[o];
+    spos[i] = make_float4(px[o], py[o], pz[o], __int_as_float(o));
+    su[i] = vx[o]; sv[i] = vy[o]; sw[i] = vz[o];
 }
 
-hipError_t launch_gather_sorted(const int *order, const int *nsorted, int n,
-                                const float *const src[6], float *const dst[6], hipStream_t s) {
+hipError_t launch_gather_sorted(const int *order, const int *nsorted, int n, const float *const src[6],
+                                float4 *spos, float *const svel[3], hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    k_gather6<<<(n + 255) / 256, 256, 0, s>>>(order, nsorted, src[0], src[1], src[2], src[3], src[4],
-                                              src[5], dst[0], dst[1], dst[2], dst[3], dst[4], dst[5]);
+    k_gather_sorted<<<(n + 255) / 256, 256, 0, s>>>(order, nsorted, src[0], src[1], src[2], src[3], src[4],
+                                                    src[5], spos, svel[0], svel[1], svel[2]);
     return hipGetLastError();
 }
 

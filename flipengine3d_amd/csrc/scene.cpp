@@ -360,8 +360,9 @@ void Scene::alloc_all() {
     for (int a = 0; a < 6; a++) {
         need(part[a].alloc(pc));
         need(part_alt[a].alloc(pc));
-        need(sorted[a].alloc(pc));
     }
+    need(spos.alloc(pc));
+    for (int a = 0; a < 3; a++) need(svel[a].alloc(pc));
     need(cell.alloc(pc)); need(slot.alloc(pc)); need(order.alloc(pc));
     need(keep.alloc(pc)); need(keep_pos.alloc(pc)); need(rank.alloc(pc));
     need(solid_removed.alloc(pc));
@@ -500,7 +501,8 @@ bool Scene::rearm(const float *pos, const float *vel, int64_t count_in) {
 
 size_t Scene::device_bytes() const {
     size_t b = 0;
-    for (int a = 0; a < 6; a++) b += part[a].bytes() + part_alt[a].bytes() + sorted[a].bytes();
+    for (int a = 0; a < 6; a++) b += part[a].bytes() + part_alt[a].bytes();
+    b += spos.bytes() + svel[0].bytes() + svel[1].bytes() + svel[2].bytes();
     b += cell.bytes() + slot.bytes() + count.bytes() + offset.bytes() + order.bytes() + scan_tmp.bytes();
     b += removed_in_cell.bytes() + sub_count.bytes() + sub_offset.bytes() + p2g_ovf.bytes() + p2g_ovf_count.bytes();
     b += keep.bytes() + keep_pos.bytes() + rank.bytes() + new_count.bytes() + over_cap.bytes() + solid_removed.bytes();
@@ -522,7 +524,7 @@ size_t estimate_scene_bytes(const flipmi_scene_desc &d, int64_t cap) {
     const size_t nU = (I + 1) * J * K, nV = I * (J + 1) * K, nW = I * J * (K + 1);
     const size_t nf = std::max(nU, std::max(nV, nW));
     const size_t pc = (size_t)std::max<int64_t>(cap, 1);
-    size_t b = pc * (18 * sizeof(float) + 6 * sizeof(int) + 1);
+    size_t b = pc * (19 * sizeof(float) + 6 * sizeof(int) + 1);  // part, part_alt, spos, svel
     b += sizeof(int) * (nc + nc + 1 + nc);  // count, offset, removed_in_cell
     const GridGeom g = make_geom(d.isize, d.jsize, d.ksize, d.dx);
     b += sizeof(int) * (2 * subcell_bins(g) + 1 + p2g_face_count(g) + 1);  // half-cell sort, P2G overflow
@@ -1291,8 +1293,14 @@ void Scene::grow(int64_t need_n) {
         sync();
         std::swap(part[a].p, fresh.p);
         std::swap(part[a].n, fresh.n);
-        part_alt[a].release(); sorted[a].release();
-        need(part_alt[a].alloc(pc)); need(sorted[a].alloc(pc));
+        part_alt[a].release();
+        need(part_alt[a].alloc(pc));
+    }
+    spos.release();
+    need(spos.alloc(pc));
+    for (int a = 0; a < 3; a++) {
+        svel[a].release();
+        need(svel[a].alloc(pc));
     }
     cell.release(); slot.release(); order.release(); keep.release(); keep_pos.release(); rank.release();
     solid_removed.release(); scan_tmp.release();
@@ -1347,28 +1355,24 @@ void Scene::step_fluid(double dt) {
     begin_stage(FLIPMI_STAGE_SORT, &ev);
     sort_subcells();
     const float *src[6];
-    float *dst[6];
-    for (int a = 0; a < 6; a++) {
-        src[a] = part[a].p;
-        dst[a] = sorted[a].p;
-    }
-    check(launch_gather_sorted(order.p, sub_offset.p + subcell_bins(g), n, src, dst, stream), "gather");
+    for (int a = 0; a < 6; a++) src[a] = part[a].p;
+    float *sorted_vel[3] = {svel[0].p, svel[1].p, svel[2].p};
+    check(launch_gather_sorted(order.p, sub_offset.p + subcell_bins(g), n, src, spos.p, sorted_vel, stream),
+          "gather");
     end_stage(FLIPMI_STAGE_SORT, ev);
     begin_stage(FLIPMI_STAGE_CLASSIFY, &ev);
-    check(launch_liquid_sdf(g, sorted[0].p, sorted[1].p, sorted[2].p, sub_offset.p, radius, solid.p,
-                            liquid.p, stream),
-          "liquid sdf");
+    check(launch_liquid_sdf(g, spos.p, sub_offset.p, radius, solid.p, liquid.p, stream), "liquid sdf");
     end_stage(FLIPMI_STAGE_CLASSIFY, ev);
     // _advectVelocityField (fluidsimulation.cpp:3253-3277).
     int layers = (int)std::ceil(desc.cfl) + 2;
     size_t nU = u.n, nV = v.n, nW = w.n;
     if (n > 0) {
         begin_stage(FLIPMI_STAGE_P2G, &ev);
-        const float *svel[3] = {sorted[3].p, sorted[4].p, sorted[5].p};
+        const float *sv3[3] = {svel[0].p, svel[1].p, svel[2].p};
         float *outs[3] = {u.p, v.p, w.p};
         uint8_t *valids[3] = {vu.p, vv.p, vw.p};
-        check(launch_p2g(g, sorted[0].p, sorted[1].p, sorted[2].p, svel, order.p, sub_offset.p, n, radius, outs,
-                         valids, p2g_ovf_count.p, p2g_ovf.p, stream),
+        check(launch_p2g(g, spos.p, sv3, sub_offset.p, n, radius, outs, valids, p2g_ovf_count.p, p2g_ovf.p,
+                         stream),
               "p2g");
         end_stage(FLIPMI_STAGE_P2G, ev);
         extrapolate_all(layers);

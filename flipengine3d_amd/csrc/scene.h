@@ -101,7 +101,9 @@ struct Scene {
 
     // particles (canonical order) and work buffers
     int n = 0, cap = 0;
-    DBuf<float> part[6], part_alt[6], sorted[6];
+    DBuf<float> part[6], part_alt[6];
+    DBuf<float4> spos;     // half-cell order: (x, y, z, particle id bits)
+    DBuf<float> svel[3];   // half-cell order: velocity
     DBuf<int> cell, slot, count, offset, order, scan_tmp;
     // Half-cell bins of the step's particle sort (8 I J K, +1), read by the
     // liquid SDF and P2G; P2G's list of faces with more than 32 contributions.
