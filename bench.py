@@ -35,15 +35,27 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per matrix row (fluid cell) of each PCG kernel: every
-# logically required vector / coefficient read or written once.  Vectors and
-# coefficients are 8 bytes on the FP64 path and 4 bytes on the mixed path.
-#   spmv_dot:   s, diag, 3 upper couplings (the lower three are the
-#               neighbours' upper ones), z               -> 6 words
-#   axpy_norm:  x r/w, s, r r/w, z                       -> 6 words
-#   forward:    r, invdiag, 3 lower factor entries, z    -> 6 words
-#   backward:   z, invdiag, 3 upper factor entries, z    -> 6 words
-#   update_s:   z, s r/w                                 -> 3 words
-PCG_WORDS = {"k_spmv_dot": 6, "k_axpy_norm": 6, "k_forward": 6, "k_backward": 6, "k_update_s": 3}
+# array the kernel must read or write, once.  The matrix (diag and the three
+# upper couplings; the lower three are the neighbours' upper ones) is FP64 on
+# both paths (32 bytes).  FP64 path, 8-byte vectors:
+#   spmv_dot:   s, matrix, z (= A s)                       -> 8 + 32 + 8
+#   axpy_norm:  x r/w, s, r r/w, z                          -> 6 x 8
+#   forward:    r, invdiag, 3 lower factor entries, z       -> 6 x 8
+#   backward:   z r/w, invdiag, 3 upper factor entries, and
+#               r for the fused z.r                         -> 7 x 8
+#   update_s:   z, s r/w                                    -> 3 x 8
+# Mixed path: FP32 vectors and factor, FP64 x and matrix:
+#   spmv_dot:   z, old s, matrix, new s, q (= A s)          -> 4 + 4 + 32 + 4 + 4
+#   axpy_norm:  x r/w (FP64), s, r r/w, q                   -> 16 + 4 + 8 + 4
+#   forward:    6 x 4;  backward: 7 x 4
+#   the direction update is folded into spmv_dot.
+# A true residual (mixed path only) reads x, b and the matrix (FP64) and
+# writes r (FP32): 52 bytes.
+PCG_BYTES = {
+    "fp64": {"k_spmv_dot": 48, "k_axpy_norm": 48, "k_forward": 48, "k_backward": 56, "k_update_s": 24},
+    "mixed": {"k_spmv_dot": 48, "k_axpy_norm": 32, "k_forward": 24, "k_backward": 28, "k_update_s": 0},
+}
+TRUE_RESIDUAL_BYTES = 52
 
 # HBM bytes moved per launch, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes scaled by a plain 256 MiB copy (x2.0 and x1.0), on the default
@@ -207,7 +219,7 @@ def run_gpu(fa, scene, args, precision, steps, warmup, profiling):
         # The warmup timed every PCG kernel; the timed region times only the
         # dominant one (each timed launch costs an event pair).
         wt = sim.stage_times()
-        kern = max(PCG_WORDS, key=lambda k: wt[k][0])
+        kern = max(PCG_BYTES[precision], key=lambda k: wt[k][0])
         sim.set_kernel_timing([kern])
     sim.stage_times(reset=True)
 
@@ -234,18 +246,17 @@ def run_gpu(fa, scene, args, precision, steps, warmup, profiling):
     times = sim.stage_times()
     sim.close()
 
-    word = 4 if precision == "mixed" else 8
+    per_row = PCG_BYTES[precision]
     rows_per_launch = rows_iters / max(iters, 1)
-    # PCG bandwidth: algorithmic bytes of every iteration (27 words per row,
-    # see PCG_WORDS) plus 40 bytes per row for each FP64 true residual, over
-    # the PCG stage time.
-    pcg_bytes = rows_iters * sum(PCG_WORDS.values()) * word + refinements * rows_per_launch * 40
+    # PCG bandwidth: algorithmic bytes of every iteration (PCG_BYTES per
+    # row) plus those of each true residual, over the PCG stage time.
+    pcg_bytes = rows_iters * sum(per_row.values()) + refinements * rows_per_launch * TRUE_RESIDUAL_BYTES
     pcg_ms = times["pcg"][0]
     roof = {"kernel": None, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": None, "traffic": None}
     if kern is not None:
         kms, kcalls = times[kern]
-        bytes_per_launch = PCG_WORDS[kern] * word * rows_per_launch
+        bytes_per_launch = per_row[kern] * rows_per_launch
         # Iterations are enqueued in batches, so a few launches after
         # convergence return at once; the average is over the launches that
         # iterated.

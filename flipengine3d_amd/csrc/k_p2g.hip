@@ -101,7 +101,7 @@ __device__ __forceinline__ void face_setup(const P2GArgs &a, int comp, size_t f,
     float r = a.radius_f;
     c.rsq = r * r;
     // (0.99 r)^2, below which the assignment tests are skipped (face_hit_at).
-    const bool small_grid = g.I <= 16384 && g.J <= 16384 && g.K <= 16384;
+    const bool small_grid = g.I <= 4096 && g.J <= 4096 && g.K <= 4096;
     c.rsq_fast = small_grid ? 0.98f * c.rsq : 0.0f;
     c.coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
     c.coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
@@ -152,9 +152,13 @@ __device__ __forceinline__ bool face_hit_at(const P2GArgs &a, const FaceCtx &c, 
     // the face's block (overlapping case) or lies inside it (simple case),
     // and the face's local index lies in [floor((p - sr)/dx),
     // floor((p + sr)/dx)].  They can only fail through float rounding at a
-    // distance within rounding of r.  Inside 0.99 r the margin is at least
-    // 0.0087 dx, over a hundred times the rounding of any coordinate of a
-    // grid up to 16384 cells per axis, so such pairs skip the tests.
+    // distance within rounding of r.  Inside 0.99 r every axis keeps a
+    // margin of 0.01 r = 0.0087 dx.  On a grid of at most 4096 cells per
+    // axis a coordinate is below 2^12 dx, so one float rounding moves it by
+    // at most 2^-12 dx; the tests round a coordinate at most three times
+    // (p0 - off, then - sr or - boff, and the block corner), 0.00073 dx in
+    // all, a twelfth of the margin.  Such pairs skip the tests; on larger
+    // grids every pair within r runs them.
     if (!(d2 < c.rsq_fast) && !assignment_tests(a, c, p, pl)) return false;
     wt = face_weight(c, d2);
     return true;

@@ -19,9 +19,14 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 def test_pcg_byte_model_matches_the_kernels():
     sys.path.insert(0, ROOT)
     import bench
-    # 6 words per row for the SpMV, the two axpys, each triangular sweep and
-    # 3 for the direction update: 27 words = 216 B (FP64) per row per iteration.
-    assert sum(bench.PCG_WORDS.values()) == 27
+    # FP64 path: SpMV 48 B per row (s, z, FP64 diag + 3 upper couplings),
+    # axpy 48, forward sweep 48, backward sweep 56 (it also reads r for the
+    # fused z.r), direction update 24: 224 B per row per iteration.  Mixed:
+    # FP32 vectors and factor, FP64 x and matrix, the direction update in
+    # the SpMV: 48 + 32 + 24 + 28 = 132 B.
+    assert sum(bench.PCG_BYTES["fp64"].values()) == 224
+    assert sum(bench.PCG_BYTES["mixed"].values()) == 132
+    assert bench.TRUE_RESIDUAL_BYTES == 8 + 8 + 32 + 4
     args = bench.parse_args([])
     assert args.gpus == 1 and args.size == [256, 128, 128]
     assert json.dumps({"x": None})  # the line is plain JSON
