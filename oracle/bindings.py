@@ -77,6 +77,9 @@ class _Base:
         getattr(L, p + "sim_set_outflow").argtypes = [_vp] + [_c_int] * 3
         getattr(L, p + "sim_set_obstacle_inversed").argtypes = [_vp] + [_c_int] * 2
         getattr(L, p + "sim_inflow_sdf").argtypes = [_vp, _c_int, _vp, _vp]
+        getattr(L, p + "sim_isomesh").argtypes = [_vp] * 5
+        surface = [_vp, _c_int, _c_int, _c_double, _c_int, _c_double, _c_double, _c_int, _c_int, _c_int]
+        getattr(L, p + "sim_set_surface").argtypes = surface + ([_c_double] if p == "orc_" else [])
 
     # -- whole-step driver -------------------------------------------------
     def sim_create(self, isize, jsize, ksize, dx):
@@ -299,6 +302,27 @@ class _SimHandle:
         self._f("sim_get_bool_grid")(self.h, which, _ptr(out))
         return out
 
+    def set_surface(self, enabled=True, subdivisions=1, smooth_value=0.5, smooth_iterations=2,
+                    particle_scale=3.0, obstacle_offset=0.0, remove_near_domain=False, remove_distance=0,
+                    invert_contact=False):
+        """Surface reconstruction settings (the engine's setters; meshing
+        after the first substep of every frame)."""
+        args = [self.h, int(enabled), subdivisions, smooth_value, smooth_iterations, particle_scale,
+                obstacle_offset, int(remove_near_domain), remove_distance, int(invert_contact)]
+        if self.L.prefix == "orc_":
+            args.append(0.08)
+        self._f("sim_set_surface")(*args)
+
+    def isomesh(self):
+        """(vertices (n, 3) float32, triangles (m, 3) int32) of getIsomesh()."""
+        nv, nt = ctypes.c_int(0), ctypes.c_int(0)
+        f = self._f("sim_isomesh")
+        f(self.h, ctypes.byref(nv), ctypes.byref(nt), None, None)
+        verts = np.zeros((nv.value, 3), np.float32)
+        tris = np.zeros((nt.value, 3), np.int32)
+        f(self.h, ctypes.byref(nv), ctypes.byref(nt), _ptr(verts), _ptr(tris))
+        return verts, tris
+
     def count_grid(self):
         """(count grid of the last removal pass, particles that pass removed
         for their speed, speed removals since creation).  Oracle only."""
@@ -331,6 +355,35 @@ class Oracle(_Base):
                                        + [_c_double, ctypes.c_uint64, _c_int, _vp, _vp, _vp])
         L.orc_sim_get_count_grid.argtypes = [_vp, _vp, _vp, _vp]
         L.orc_removal_counts.argtypes = [_c_int] * 3 + [_c_double, _vp, _c_int, _vp, _vp]
+
+    def surface_mesh(self, isize, jsize, ksize, dx, pos, solid_phi, subdivisions=1, smooth_value=0.5,
+                     smooth_iterations=2, particle_scale=3.0, obstacle_offset=0.0, remove_near_domain=False,
+                     remove_distance=0, invert_contact=False, contact_threshold=0.08, domain_offset=(0, 0, 0),
+                     domain_scale=1.0):
+        """(vertices, triangles) of the surface of `pos` over the nodal solid
+        SDF, as getIsomesh() returns it."""
+        L = self.lib
+        L.orc_surface_mesh.restype = _vp
+        L.orc_surface_mesh.argtypes = ([_c_int] * 3 + [_c_double, _vp, _c_int, _vp, _c_int, _c_double, _c_int,
+                                       _c_double, _c_double, _c_int, _c_int, _c_int, _c_double, _vp, _c_double])
+        L.orc_mesh_get.argtypes = [_vp] * 5
+        L.orc_mesh_free.argtypes = [_vp]
+        pos = np.ascontiguousarray(pos, np.float32)
+        sp = np.ascontiguousarray(solid_phi, np.float32)
+        off = np.asarray(domain_offset, np.float32)
+        h = L.orc_surface_mesh(isize, jsize, ksize, dx, _ptr(pos), pos.shape[0], _ptr(sp), subdivisions,
+                               smooth_value, smooth_iterations, particle_scale, obstacle_offset,
+                               int(remove_near_domain), remove_distance, int(invert_contact), contact_threshold,
+                               _ptr(off), domain_scale)
+        try:
+            nv, nt = ctypes.c_int(0), ctypes.c_int(0)
+            L.orc_mesh_get(h, ctypes.byref(nv), ctypes.byref(nt), None, None)
+            verts = np.zeros((nv.value, 3), np.float32)
+            tris = np.zeros((nt.value, 3), np.int32)
+            L.orc_mesh_get(h, ctypes.byref(nv), ctypes.byref(nt), _ptr(verts), _ptr(tris))
+        finally:
+            L.orc_mesh_free(h)
+        return verts, tris
 
     def removal_counts(self, isize, jsize, ksize, dx, pos, removed=None):
         """Count grid of the removal pass's counting loop on given positions

@@ -99,6 +99,21 @@ void orc_sim_set_obstacle_inversed(void *h, int id, int inversed);
 int orc_inflow_build(const float *verts, int nv, const int *tris, int nt, int I, int J, int K,
                      double dx, uint64_t seed, int id, int *dims, float *phi, uint8_t *cells);
 
+// Surface reconstruction (FluidSimulation::_outputSurfaceMeshThread,
+// fluidsimulation.cpp:5134-5243; ParticleMesher, Polygonizer3d,
+// TriangleMesh::smooth).  One polygonizer slice, no meshing volume, no
+// preview or motion blur, minimum polyhedron size 0.
+void orc_sim_set_surface(void *h, int enabled, int subdivisions, double smooth_value, int smooth_iterations,
+                         double particle_scale, double obstacle_offset, int remove_near_domain,
+                         int remove_distance, int invert_contact, double contact_threshold);
+void orc_sim_isomesh(void *h, int *nverts, int *ntris, float *verts, int *tris);
+void *orc_surface_mesh(int I, int J, int K, double dx, const float *pos, int n, const float *solid_phi,
+                       int subdivisions, double smooth_value, int smooth_iterations, double particle_scale,
+                       double obstacle_offset, int remove_near_domain, int remove_distance, int invert_contact,
+                       double contact_threshold, const float *domain_offset, double domain_scale);
+void orc_mesh_get(void *h, int *nverts, int *ntris, float *verts, int *tris);
+void orc_mesh_free(void *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -158,6 +158,40 @@ void ref_sim_destroy(void *hp) {
 
 void ref_set_thread_count(int n) { ThreadUtils::setMaxThreadCount(n > 0 ? n : 1); }
 
+// Surface reconstruction settings through the engine's own setters
+// (fluidsimulation.cpp:168-540), meshing synchronously.
+void ref_sim_set_surface(void *hp, int enabled, int subdivisions, double smooth_value, int smooth_iterations,
+                         double particle_scale, double obstacle_offset, int remove_near_domain,
+                         int remove_distance, int invert_contact) {
+    FluidSimulation *s = ((RefSim *)hp)->sim;
+    if (enabled) s->enableSurfaceReconstruction(); else s->disableSurfaceReconstruction();
+    s->disableAsynchronousMeshing();
+    s->setSurfaceSubdivisionLevel(subdivisions);
+    s->setSurfaceSmoothingValue(smooth_value);
+    s->setSurfaceSmoothingIterations(smooth_iterations);
+    s->setMarkerParticleScale(particle_scale);
+    s->setObstacleMeshingOffset(obstacle_offset);
+    if (remove_near_domain) s->enableRemoveSurfaceNearDomain(); else s->disableRemoveSurfaceNearDomain();
+    s->setRemoveSurfaceNearDomainDistance(remove_distance);
+    if (invert_contact) s->enableInvertedContactNormals(); else s->disableInvertedContactNormals();
+}
+
+// FluidSimulation::getIsomesh (fluidsimulation.cpp:2543-2546).
+void ref_sim_isomesh(void *hp, int *nverts, int *ntris, float *verts, int *tris) {
+    TriangleMesh &m = ((RefSim *)hp)->sim->getIsomesh();
+    *nverts = (int)m.vertices.size();
+    *ntris = (int)m.triangles.size();
+    if (verts)
+        for (size_t v = 0; v < m.vertices.size(); v++) {
+            verts[3 * v] = m.vertices[v].x;
+            verts[3 * v + 1] = m.vertices[v].y;
+            verts[3 * v + 2] = m.vertices[v].z;
+        }
+    if (tris)
+        for (size_t t = 0; t < m.triangles.size(); t++)
+            for (int c = 0; c < 3; c++) tris[3 * t + c] = m.triangles[t].tri[c];
+}
+
 void ref_sim_add_body_force(void *hp, double fx, double fy, double fz) {
     ((RefSim *)hp)->sim->addBodyForce(fx, fy, fz);
 }
