@@ -97,6 +97,14 @@ class ServerStats(ctypes.Structure):
                 ("scenes_evicted", _c_i64)]
 
 
+class SurfaceDesc(ctypes.Structure):
+    _fields_ = [("enabled", _c_int), ("subdivisions", _c_int), ("polygonizer_slices", _c_int),
+                ("smoothing_value", _c_double), ("smoothing_iterations", _c_int),
+                ("min_polyhedron_triangles", _c_int), ("particle_scale", _c_double),
+                ("obstacle_meshing_offset", _c_double), ("remove_near_domain", _c_int),
+                ("remove_near_domain_distance", _c_int), ("invert_contact_normals", _c_int)]
+
+
 class PressureReport(ctypes.Structure):
     _fields_ = [("iterations", _c_int), ("residual", _c_double), ("rows", _c_int),
                 ("success", _c_int), ("refinements", _c_int)]
@@ -116,7 +124,9 @@ EXPORTED = [
     "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
     "flipmi_trilinear_velocity", "flipmi_server_desc_default", "flipmi_server_create", "flipmi_server_submit",
     "flipmi_server_wait", "flipmi_server_take", "flipmi_server_get_stats", "flipmi_server_destroy",
-    "flipmi_extrapolate_noalloc",
+    "flipmi_extrapolate_noalloc", "flipmi_surface_desc_default", "flipmi_scene_set_surface",
+    "flipmi_scene_get_isomesh", "flipmi_mesh_particles", "flipmi_mesh_counts", "flipmi_mesh_copy",
+    "flipmi_mesh_destroy",
 ]
 
 _lib = None
@@ -171,6 +181,15 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                           _c_double, _c_int, _vp])
     L.flipmi_trilinear_velocity.argtypes = ([_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp, _vp, _vp,
                                             _c_i64, _vp])
+    L.flipmi_surface_desc_default.argtypes = [P(SurfaceDesc)]
+    L.flipmi_surface_desc_default.restype = None
+    L.flipmi_scene_set_surface.argtypes = [_vp, P(SurfaceDesc)]
+    L.flipmi_scene_get_isomesh.argtypes = [_vp, P(_c_i64), P(_c_i64), _vp, _vp]
+    L.flipmi_mesh_particles.argtypes = ([_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _vp, P(SurfaceDesc),
+                                         _c_double, _vp, _c_double, P(_vp)])
+    L.flipmi_mesh_counts.argtypes = [_vp, P(_c_i64), P(_c_i64)]
+    L.flipmi_mesh_copy.argtypes = [_vp, _vp, _vp]
+    L.flipmi_mesh_destroy.argtypes = [_vp]
     L.flipmi_server_desc_default.argtypes = [P(ServerDesc)]
     L.flipmi_server_desc_default.restype = None
     L.flipmi_server_create.argtypes = [_vp, P(ServerDesc), P(_vp)]
@@ -259,6 +278,37 @@ class Context:
         self.check(self.L.flipmi_removal_counts(self.h, isize, jsize, ksize, dx, _ptr(pos), pos.shape[0],
                                                 _ptr(rm), _ptr(out)))
         return out
+
+    def surface_desc(self, **settings) -> SurfaceDesc:
+        """The reference's surface settings (enabled), with overrides."""
+        d = SurfaceDesc()
+        self.L.flipmi_surface_desc_default(ctypes.byref(d))
+        for k, v in settings.items():
+            setattr(d, k, v)
+        return d
+
+    def mesh_particles(self, isize, jsize, ksize, dx, pos, solid_phi, kernel_radius=0.0, domain_offset=None,
+                       domain_scale=1.0, **settings):
+        """(vertices (n, 3) float32, triangles (m, 3) int32): the surface of
+        `pos` (grid coordinates) over the nodal solid SDF, as the simulation
+        builds it; settings are SurfaceDesc fields."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        sp = np.ascontiguousarray(solid_phi, np.float32)
+        d = self.surface_desc(**settings)
+        off = None if domain_offset is None else np.ascontiguousarray(domain_offset, np.float64)
+        m = _vp()
+        self.check(self.L.flipmi_mesh_particles(self.h, isize, jsize, ksize, dx, _ptr(pos), pos.shape[0], _ptr(sp),
+                                                ctypes.byref(d), kernel_radius, _ptr(off), domain_scale,
+                                                ctypes.byref(m)))
+        try:
+            nv, nt = _c_i64(0), _c_i64(0)
+            self.check(self.L.flipmi_mesh_counts(m, ctypes.byref(nv), ctypes.byref(nt)))
+            verts = np.zeros((nv.value, 3), np.float32)
+            tris = np.zeros((nt.value, 3), np.int32)
+            self.check(self.L.flipmi_mesh_copy(m, _ptr(verts), _ptr(tris)))
+        finally:
+            self.L.flipmi_mesh_destroy(m)
+        return verts, tris
 
     def liquid_sdf_postprocess(self, isize, jsize, ksize, dx, phi, solid_phi):
         out = np.ascontiguousarray(phi, np.float32).copy()
@@ -367,6 +417,9 @@ class FluidSimulation:
         d.pcg_precision = precision
         d.max_particles = max_particles
         self.desc = d
+        self.surface = SurfaceDesc()
+        self.L.flipmi_surface_desc_default(ctypes.byref(self.surface))
+        self.surface.enabled = 0  # a scene starts with reconstruction off
         self.h = None
         self.last_stats = FrameStats()
 
@@ -376,6 +429,7 @@ class FluidSimulation:
             self.ctx.check(self.L.flipmi_scene_create(self.ctx.h, ctypes.byref(self.desc), ctypes.byref(h)))
             self.h = h
             self._push_domain()
+            self._push_surface()
 
     def addBodyForce(self, fx: float, fy: float, fz: float):
         if self.h is not None:
@@ -559,6 +613,80 @@ class FluidSimulation:
 
     def getDomainScale(self) -> float:
         return getattr(self, "_domain_scale", 1.0)
+
+    # -- surface reconstruction (fluidsimulation.cpp:168-540) ------------------
+    def _push_surface(self):
+        if self.h is not None:
+            self.ctx.check(self.L.flipmi_scene_set_surface(self.h, ctypes.byref(self.surface)))
+
+    def _set_surface(self, **kv):
+        old = {k: getattr(self.surface, k) for k in kv}
+        for k, v in kv.items():
+            setattr(self.surface, k, v)
+        try:
+            self._push_surface()
+        except (FlipError, ValueError):
+            for k, v in old.items():
+                setattr(self.surface, k, v)
+            raise
+
+    def enableSurfaceReconstruction(self):
+        self._set_surface(enabled=1)
+
+    def disableSurfaceReconstruction(self):
+        self._set_surface(enabled=0)
+
+    def isSurfaceReconstructionEnabled(self) -> bool:
+        return bool(self.surface.enabled)
+
+    def setSurfaceSubdivisionLevel(self, n: int):
+        self._set_surface(subdivisions=n)
+
+    def setNumPolygonizerSlices(self, n: int):
+        self._set_surface(polygonizer_slices=n)
+
+    def setSurfaceSmoothingValue(self, s: float):
+        self._set_surface(smoothing_value=s)
+
+    def setSurfaceSmoothingIterations(self, n: int):
+        self._set_surface(smoothing_iterations=n)
+
+    def setMinPolyhedronTriangleCount(self, n: int):
+        self._set_surface(min_polyhedron_triangles=n)
+
+    def setMarkerParticleScale(self, s: float):
+        self._set_surface(particle_scale=s)
+
+    def setObstacleMeshingOffset(self, s: float):
+        self._set_surface(obstacle_meshing_offset=s)
+
+    def enableRemoveSurfaceNearDomain(self):
+        self._set_surface(remove_near_domain=1)
+
+    def disableRemoveSurfaceNearDomain(self):
+        self._set_surface(remove_near_domain=0)
+
+    def setRemoveSurfaceNearDomainDistance(self, n: int):
+        self._set_surface(remove_near_domain_distance=n)
+
+    def enableInvertedContactNormals(self):
+        self._set_surface(invert_contact_normals=1)
+
+    def disableInvertedContactNormals(self):
+        self._set_surface(invert_contact_normals=0)
+
+    def getIsomesh(self):
+        """(vertices (n, 3) float32 in domain coordinates, triangles (m, 3)
+        int32) of the last surface mesh (fluidsimulation.cpp:2543-2546)."""
+        nv, nt = _c_i64(0), _c_i64(0)
+        if self.h is None:
+            return np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32)
+        self.ctx.check(self.L.flipmi_scene_get_isomesh(self.h, ctypes.byref(nv), ctypes.byref(nt), None, None))
+        verts = np.zeros((nv.value, 3), np.float32)
+        tris = np.zeros((nt.value, 3), np.int32)
+        self.ctx.check(self.L.flipmi_scene_get_isomesh(self.h, ctypes.byref(nv), ctypes.byref(nt), _ptr(verts),
+                                                        _ptr(tris)))
+        return verts, tris
 
     def _push_domain(self):
         if self.h is None:

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "scene.h"
+#include "surface.h"
 
 using namespace flipmi;
 
@@ -325,6 +326,43 @@ int flipmi_scene_get_particles(flipmi_scene *scene, float *pos, float *vel) {
     });
 }
 
+void flipmi_surface_desc_default(flipmi_surface_desc *d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->enabled = 1;                    // fluidsimulation.h:1586
+    d->subdivisions = 1;               // :1605
+    d->polygonizer_slices = 1;         // :1606
+    d->smoothing_value = 0.5;          // :1607
+    d->smoothing_iterations = 2;       // :1608
+    d->min_polyhedron_triangles = 0;   // :1609
+    d->particle_scale = 3.0;           // :1611
+    d->obstacle_meshing_offset = 0.0;  // :1593
+    d->remove_near_domain = 0;         // :1594
+    d->remove_near_domain_distance = 0;
+    d->invert_contact_normals = 0;     // :1588
+}
+
+int flipmi_scene_set_surface(flipmi_scene *scene, const flipmi_surface_desc *desc) {
+    if (!scene || !desc) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    return guarded(s.ctx, [&] {
+        validate_surface_desc(*desc, s.g);
+        s.surface = *desc;
+    });
+}
+
+int flipmi_scene_get_isomesh(flipmi_scene *scene, int64_t *num_vertices, int64_t *num_triangles,
+                             float *vertices_xyz, int32_t *triangles) {
+    if (!scene || !num_vertices || !num_triangles) return FLIPMI_ERR_INVALID_ARG;
+    Scene &s = scene->s;
+    return guarded(s.ctx, [&] {
+        const bool have = s.mesh && s.mesh->valid;
+        *num_vertices = have ? s.mesh->nverts : 0;
+        *num_triangles = have ? s.mesh->ntris : 0;
+        if (have) copy_surface(*s.mesh, vertices_xyz, triangles, s.stream);
+    });
+}
+
 int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out) {
     if (!scene || !out) return FLIPMI_ERR_INVALID_ARG;
     Scene &s = scene->s;
@@ -494,6 +532,74 @@ int flipmi_removal_counts(flipmi_ctx *ctx, int I, int J, int K, double dx, const
         download(counts_out, out, nc, s);
         hipck(hipStreamSynchronize(s), "sync");
     });
+}
+
+struct flipmi_mesh {
+    flipmi_ctx *ctx = nullptr;
+    SurfaceStore st;
+};
+
+int flipmi_mesh_particles(flipmi_ctx *ctx, int I, int J, int K, double dx, const float *pos, int64_t n,
+                          const float *solid_phi_nodal, const flipmi_surface_desc *desc, double kernel_radius,
+                          const double *domain_offset, double domain_scale, flipmi_mesh **out) {
+    if (!ctx || !out || !desc || !solid_phi_nodal || (n > 0 && !pos) || n < 0 || kernel_radius < 0.0)
+        return FLIPMI_ERR_INVALID_ARG;
+    *out = nullptr;
+    flipmi_mesh *m = new flipmi_mesh();
+    m->ctx = ctx;
+    int rc = guarded(&ctx->c, [&] {
+        GridGeom g = make_geom(I, J, K, dx);
+        validate_surface_desc(*desc, g);
+        // The mesher bins every particle into the grid's blocks
+        // (particlemesher.cpp:113-118): positions must lie in the grid.
+        for (int64_t q = 0; q < n; q++)
+            for (int a = 0; a < 3; a++) {
+                const double x = (double)pos[3 * q + a] * g.invdx;
+                const int hi = a == 0 ? I : (a == 1 ? J : K);
+                if (!(x >= 0.0 && x < (double)hi))
+                    throw StatusError{FLIPMI_ERR_INVALID_ARG, "particle positions must lie inside the grid"};
+            }
+        hipStream_t s = ctx->c.stream;
+        SortedParticles sp;
+        sp.build(g, pos, nullptr, n, s, true);
+        DBuf<float> solid;
+        upload(solid, solid_phi_nodal, (size_t)(I + 1) * (J + 1) * (K + 1), s);
+        SurfaceInputs in;
+        in.g = g;
+        in.spos = sp.spos.p;
+        in.sub_offset = sp.offset.p;
+        in.n = (int)n;
+        in.solid = solid.p;
+        in.desc = *desc;
+        in.kernel_radius = kernel_radius > 0.0 ? kernel_radius : marker_particle_radius(dx) * desc->particle_scale;
+        for (int a = 0; a < 3; a++) in.domain_offset[a] = domain_offset ? (float)domain_offset[a] : 0.0f;
+        in.domain_scale = domain_scale;
+        build_surface(in, m->st, s);
+        hipck(hipStreamSynchronize(s), "sync");
+    });
+    if (rc != FLIPMI_OK) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return FLIPMI_OK;
+}
+
+int flipmi_mesh_counts(flipmi_mesh *mesh, int64_t *num_vertices, int64_t *num_triangles) {
+    if (!mesh || !num_vertices || !num_triangles) return FLIPMI_ERR_INVALID_ARG;
+    *num_vertices = mesh->st.nverts;
+    *num_triangles = mesh->st.ntris;
+    return FLIPMI_OK;
+}
+
+int flipmi_mesh_copy(flipmi_mesh *mesh, float *vertices_xyz, int32_t *triangles) {
+    if (!mesh) return FLIPMI_ERR_INVALID_ARG;
+    return guarded(&mesh->ctx->c, [&] { copy_surface(mesh->st, vertices_xyz, triangles, mesh->ctx->c.stream); });
+}
+
+int flipmi_mesh_destroy(flipmi_mesh *mesh) {
+    delete mesh;
+    return FLIPMI_OK;
 }
 
 int flipmi_liquid_sdf_postprocess(flipmi_ctx *ctx, int I, int J, int K, double dx,

@@ -7,6 +7,7 @@
 // host reads back a handful of scalars per substep (maximum speed, |rhs|,
 // PCG progress, the speed histogram and the surviving particle count).
 #include "scene.h"
+#include "surface.h"
 
 #include <algorithm>
 #include <cmath>
@@ -292,6 +293,7 @@ Scene::~Scene() {
     for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
     if (host_stage) (void)hipHostFree(host_stage);
     delete press;
+    delete mesh;
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -470,6 +472,10 @@ void Scene::initialize() {
 
 bool Scene::rearm(const float *pos, const float *vel, int64_t count_in) {
     removal_counts_ready = false;
+    if (mesh) {
+        mesh->valid = false;
+        mesh->nverts = mesh->ntris = 0;
+    }
     if (!initialized || !pending_events.empty()) return false;
     pending_pos.clear();
     pending_vel.clear();
@@ -511,6 +517,7 @@ size_t Scene::device_bytes() const {
     b += near_a.bytes() + near_b.bytes() + liquid.bytes() + u.bytes() + v.bytes() + w.bytes();
     b += su.bytes() + sv.bytes() + sw.bytes() + vu.bytes() + vv.bytes() + vw.bytes() + status.bytes();
     if (press) b += press->device_bytes();
+    if (mesh) b += mesh->device_bytes();
     for (auto &src : sources) b += src->sdf.bytes() + src->cells.bytes() + src->list.bytes();
     b += inflow_mask.bytes() + emit_cand.bytes() + emit_flags.bytes() + emit_pos.bytes() + emit_scan.bytes() +
          emit_nbase.bytes();
@@ -591,6 +598,32 @@ void Scene::sort_particles(int *over) {
 void Scene::sort_subcells() {
     SortBuffers sb{cell.p, slot.p, sub_count.p, sub_offset.p, order.p, scan_tmp.p, scan_tmp.n};
     check(launch_sort_by_subcell(g, part[0].p, part[1].p, part[2].p, n, sb, stream), "half-cell sort");
+}
+
+void Scene::build_mesh() {
+    hipEvent_t ev = nullptr;
+    begin_stage(FLIPMI_STAGE_SURFACE, &ev);
+    if (!mesh) mesh = new SurfaceStore();
+    if (n > 0) {
+        sort_subcells();
+        const float *src[6];
+        for (int a = 0; a < 6; a++) src[a] = part[a].p;
+        float *sorted_vel[3] = {svel[0].p, svel[1].p, svel[2].p};
+        check(launch_gather_sorted(order.p, sub_offset.p + subcell_bins(g), n, src, spos.p, sorted_vel, stream),
+              "gather");
+    }
+    SurfaceInputs in;
+    in.g = g;
+    in.spos = spos.p;
+    in.sub_offset = sub_offset.p;
+    in.n = n;
+    in.solid = solid.p;
+    in.desc = surface;
+    in.kernel_radius = marker_particle_radius(g.dx) * surface.particle_scale;
+    for (int a = 0; a < 3; a++) in.domain_offset[a] = domain_offset[a];
+    in.domain_scale = domain_scale;
+    build_surface(in, *mesh, stream);
+    end_stage(FLIPMI_STAGE_SURFACE, ev);
 }
 
 void Scene::extrapolate_all(int layers) {
@@ -1445,7 +1478,12 @@ flipmi_frame_stats Scene::update(double dt) {
         if (done + cur_step > limit) cur_step = std::fmin(substepTime, remaining);
         if (step_number == desc.max_substeps - 1) cur_step = remaining;
         remaining -= cur_step;
-        if (!skipped) step_fluid(cur_step);
+        if (!skipped) {
+            step_fluid(cur_step);
+            // _outputSimulationData (fluidsimulation.cpp:5445-5463): the
+            // surface mesh at the end of the frame's first substep.
+            if (step_number == 0 && surface.enabled) build_mesh();
+        }
         step_number++;
     } while (remaining > eps);
     check(hipEventRecord(f1, stream), "event");

@@ -92,6 +92,8 @@ struct ObstacleMesh {
     bool inversed = false;  // MeshObject::inverse() (meshobject.cpp:293-295): a container
 };
 
+struct SurfaceStore;
+
 struct Scene {
     Ctx *ctx = nullptr;
     flipmi_scene_desc desc{};
@@ -156,6 +158,10 @@ struct Scene {
     DBuf<float> emit_cand;
     DBuf<int> emit_flags, emit_pos, emit_scan, emit_nbase;
 
+    // Surface reconstruction (fluidsimulation.h:1586-1611) and the last mesh.
+    flipmi_surface_desc surface{};
+    SurfaceStore *mesh = nullptr;  // owned
+
     // last pressure solve
     int last_iters = 0, last_ok = 1;
     double last_resid = 0;
@@ -198,6 +204,8 @@ struct Scene {
     void advance_particles(double dt);
     void sort_particles(int *over = nullptr);
     void sort_subcells();
+    // The isomesh of the current particles (_outputSurfaceMeshThread).
+    void build_mesh();
     void extrapolate_all(int layers);
     // Mesh inflow sources: add (checks the mesh), build the device state of
     // every source, their device views, and the emission at the end of a

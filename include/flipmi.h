@@ -107,7 +107,8 @@ enum {
     FLIPMI_STAGE_K_BACKWARD = 15,
     FLIPMI_STAGE_K_UPDATE = 16,
     FLIPMI_STAGE_K_RESIDUAL = 17,   /* mixed precision: FP64 true residual */
-    FLIPMI_STAGE_INFLOW = 18        /* mesh inflow: particle mask and emission */
+    FLIPMI_STAGE_INFLOW = 18,       /* mesh inflow: particle mask and emission */
+    FLIPMI_STAGE_SURFACE = 19       /* surface reconstruction (sort, field, marching cubes, smoothing) */
 };
 
 /* Grids readable with flipmi_scene_get_grid (sizes as in the header). */
@@ -240,6 +241,59 @@ int flipmi_scene_set_marker_jitter(flipmi_scene *scene, double factor, int jitte
 /* Copies an internal grid (FLIPMI_GRID_*) to host memory, e.g.
  * FluidSimulation::getVelocityField (fluidsimulation.cpp:2242). */
 int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out);
+
+/* Surface reconstruction settings: FluidSimulation's surface setters
+ * (fluidsimulation.cpp:168-540, fields of fluidsimulation.h:1586-1611).
+ * flipmi_surface_desc_default gives the reference's defaults, enabled.
+ * Values the reference rejects (subdivisions < 1, slices < 1, iterations
+ * < 0, particle scale < 0, polyhedron count < 0, distance < 0) are
+ * FLIPMI_ERR_INVALID_ARG, as it throws
+ * domain_error.  Not built here, also FLIPMI_ERR_INVALID_ARG: more than one
+ * polygonizer slice, a minimum polyhedron triangle count above 0, and a
+ * kernel search box wider than 40 half-cells per 10^3 node block (particle
+ * scale above about 9 at subdivision level 1). */
+typedef struct {
+    int enabled;                      /* enable/disableSurfaceReconstruction, default 1 */
+    int subdivisions;                 /* setSurfaceSubdivisionLevel, default 1 */
+    int polygonizer_slices;           /* setNumPolygonizerSlices, default 1 */
+    double smoothing_value;           /* setSurfaceSmoothingValue, default 0.5 */
+    int smoothing_iterations;         /* setSurfaceSmoothingIterations, default 2 */
+    int min_polyhedron_triangles;     /* setMinPolyhedronTriangleCount, default 0 */
+    double particle_scale;            /* setMarkerParticleScale, default 3.0 */
+    double obstacle_meshing_offset;   /* setObstacleMeshingOffset (cells), default 0 */
+    int remove_near_domain;           /* enable/disableRemoveSurfaceNearDomain, default 0 */
+    int remove_near_domain_distance;  /* setRemoveSurfaceNearDomainDistance, default 0 */
+    int invert_contact_normals;       /* enable/disableInvertedContactNormals, default 0 */
+} flipmi_surface_desc;
+void flipmi_surface_desc_default(flipmi_surface_desc *desc);
+/* A scene starts with surface reconstruction off.  When on, the mesh of the
+ * particles is built at the end of the first substep of every frame, as the
+ * reference's _outputSimulationData does (fluidsimulation.cpp:5445-5463). */
+int flipmi_scene_set_surface(flipmi_scene *scene, const flipmi_surface_desc *desc);
+/* FluidSimulation::getIsomesh (fluidsimulation.cpp:2543-2546): the last
+ * mesh, vertices in domain coordinates (xyz floats) and triangles (three
+ * int32 vertex indices each).  Either array pointer may be NULL to query
+ * the counts only; an empty mesh before the first one is built. */
+int flipmi_scene_get_isomesh(flipmi_scene *scene, int64_t *num_vertices, int64_t *num_triangles,
+                             float *vertices_xyz, int32_t *triangles);
+
+/* The surface of a particle set over a nodal solid SDF, as the simulation
+ * builds it with `desc` (enabled is ignored): ParticleMesher::meshParticles
+ * (particlemesher.cpp:36-55) followed by the post-processing of
+ * _outputSurfaceMeshThread.  With smoothing_iterations = 0, no removal and
+ * no inversion, and an identity domain transform it is meshParticles'
+ * mesh.  kernel_radius: the particle kernel radius (ParticleMesherParameters
+ * ::radius); 0 = the simulation's marker particle radius times
+ * desc->particle_scale.  Positions in grid coordinates; domain_offset may be
+ * NULL.  The result stays on the device until copied and destroyed. */
+typedef struct flipmi_mesh flipmi_mesh;
+int flipmi_mesh_particles(flipmi_ctx *ctx, int isize, int jsize, int ksize, double dx, const float *pos_xyz,
+                          int64_t n, const float *solid_phi_nodal, const flipmi_surface_desc *desc,
+                          double kernel_radius, const double *domain_offset, double domain_scale,
+                          flipmi_mesh **out);
+int flipmi_mesh_counts(flipmi_mesh *mesh, int64_t *num_vertices, int64_t *num_triangles);
+int flipmi_mesh_copy(flipmi_mesh *mesh, float *vertices_xyz, int32_t *triangles);
+int flipmi_mesh_destroy(flipmi_mesh *mesh);
 int flipmi_scene_stage_times(flipmi_scene *scene, flipmi_stage_times *out, int reset);
 /* hipEvent timing on the scene's stream: level 0 off, 1 one event pair per
  * stage, 2 additionally one pair around every PCG kernel launch. */

@@ -52,7 +52,8 @@ def test_struct_layouts_match_header():
             'sizeof(flipmi_scene_desc),sizeof(flipmi_frame_stats),sizeof(flipmi_stage_times),'
             'sizeof(flipmi_pressure_report));printf("%zu\\n",sizeof(flipmi_pressure_args));'
             'printf("%zu %zu %zu %zu\\n",sizeof(flipmi_server_desc),sizeof(flipmi_request),'
-            'sizeof(flipmi_request_report),sizeof(flipmi_server_stats));return 0;}\n')
+            'sizeof(flipmi_request_report),sizeof(flipmi_server_stats));'
+            'printf("%zu\\n",sizeof(flipmi_surface_desc));return 0;}\n')
     import subprocess
     import tempfile
     with tempfile.TemporaryDirectory() as d:
@@ -65,7 +66,7 @@ def test_struct_layouts_match_header():
     assert sizes == [ctypes.sizeof(fa.SceneDesc), ctypes.sizeof(fa.FrameStats),
                      ctypes.sizeof(fa.StageTimes), ctypes.sizeof(fa.PressureReport),
                      ctypes.sizeof(fa.PressureArgs), ctypes.sizeof(fa.ServerDesc), ctypes.sizeof(fa.Request),
-                     ctypes.sizeof(fa.RequestReport), ctypes.sizeof(fa.ServerStats)]
+                     ctypes.sizeof(fa.RequestReport), ctypes.sizeof(fa.ServerStats), ctypes.sizeof(fa.SurfaceDesc)]
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
