@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -92,6 +93,28 @@ struct vec3 {
     float operator[](int q) const { return q == 0 ? x : (q == 1 ? y : z); }
     float &operator[](int q) { return q == 0 ? x : (q == 1 ? y : z); }
 };
+// vmath.cpp:33-123: componentwise float arithmetic; division multiplies by
+// a float reciprocal formed in double.
+inline vec3 operator+(const vec3 &a, const vec3 &b) { return vec3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline vec3 operator-(const vec3 &a, const vec3 &b) { return vec3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline vec3 operator-(const vec3 &a) { return vec3(-a.x, -a.y, -a.z); }
+inline vec3 operator*(float s, const vec3 &v) { return vec3(v.x * s, v.y * s, v.z * s); }
+inline vec3 operator*(const vec3 &v, float s) { return vec3(v.x * s, v.y * s, v.z * s); }
+inline vec3 operator/(const vec3 &v, float s) {
+    float inv = (float)(1.0 / s);
+    return vec3(v.x * inv, v.y * inv, v.z * inv);
+}
+inline vec3 &operator+=(vec3 &a, const vec3 &b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+inline vec3 &operator-=(vec3 &a, const vec3 &b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; return a; }
+inline vec3 &operator*=(vec3 &v, float s) { v.x *= s; v.y *= s; v.z *= s; return v; }
+inline vec3 &operator/=(vec3 &v, float s) { float inv = (float)(1.0 / s); v.x *= inv; v.y *= inv; v.z *= inv; return v; }
+inline float dot(const vec3 &a, const vec3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline vec3 cross(const vec3 &a, const vec3 &b) {
+    return vec3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline float lengthsq(const vec3 &v) { return dot(v, v); }
+inline float length(const vec3 &v) { return (float)std::sqrt((double)lengthsq(v)); }
+inline vec3 normalize(const vec3 &v) { return v / length(v); }
 }  // namespace vmath
 static_assert(sizeof(vmath::vec3) == 12, "vec3 must be 3 packed floats");
 
@@ -616,10 +639,89 @@ struct FluidSimulationMarkerParticleData {
 // trianglemesh.h:28-84: vertex list and index triples.
 struct Triangle {
     int tri[3] = {0, 0, 0};
+    Triangle() {}
+    Triangle(int p1, int p2, int p3) { tri[0] = p1; tri[1] = p2; tri[2] = p3; }
 };
 struct TriangleMesh {
     std::vector<vmath::vec3> vertices;
     std::vector<Triangle> triangles;
+    int numVertices() { return (int)vertices.size(); }
+    int numTriangles() { return (int)triangles.size(); }
+    void clear() { vertices.clear(); triangles.clear(); }
+};
+
+// aabb.h:31-80: float corner, double extents.
+struct AABB {
+    vmath::vec3 position;
+    double width = 0.0, height = 0.0, depth = 0.0;
+    AABB() {}
+    AABB(double x, double y, double z, double w, double h, double d)
+        : position(x, y, z), width(w), height(h), depth(d) {}
+    AABB(vmath::vec3 p, double w, double h, double d) : position(p), width(w), height(h), depth(d) {}
+    vmath::vec3 getMinPoint() { return position; }
+    vmath::vec3 getMaxPoint() { return position + vmath::vec3(width, height, depth); }
+};
+
+// particlemesher.h:36-48
+struct ParticleMesherParameters {
+    int isize = 0;
+    int jsize = 0;
+    int ksize = 0;
+    double dx = 0.0;
+    int subdivisions = 1;
+    int computechunks = 1;
+    double radius = 0.0;
+    bool isPreviewMesherEnabled = false;
+    double previewdx = 0.0;
+    std::vector<vmath::vec3> *particles = nullptr;
+    class MeshLevelSet *solidSDF = nullptr;
+};
+
+namespace detail {
+inline void copy_mesh(flipmi_mesh *m, TriangleMesh &out) {
+    int64_t nv = 0, nt = 0;
+    detail::check(flipmi_mesh_counts(m, &nv, &nt), "mesh");
+    out.vertices.assign((size_t)nv, vmath::vec3());
+    out.triangles.assign((size_t)nt, Triangle());
+    detail::check(flipmi_mesh_copy(m, reinterpret_cast<float *>(out.vertices.data()),
+                                   reinterpret_cast<int32_t *>(out.triangles.data())),
+                  "mesh");
+}
+}  // namespace detail
+static_assert(sizeof(Triangle) == 12, "Triangle must be 3 packed ints");
+
+// particlemesher.h:50-60, particlemesher.cpp:36-55: the surface of the
+// particles over the solid SDF, before the simulation's post-processing
+// (one compute chunk; no preview mesh).  Positions in grid coordinates,
+// inside the grid.
+class ParticleMesher {
+public:
+    ParticleMesher() {}
+    TriangleMesh meshParticles(ParticleMesherParameters params) {
+        if (params.computechunks != 1) throw std::runtime_error("flipmi: one polygonizer slice only");
+        if (params.isPreviewMesherEnabled) throw std::runtime_error("flipmi: the preview mesh is not built");
+        if (!params.particles || !params.solidSDF) throw std::runtime_error("flipmi: particles and a solid SDF are required");
+        flipmi_surface_desc d;
+        flipmi_surface_desc_default(&d);
+        d.subdivisions = params.subdivisions;
+        d.smoothing_iterations = 0;
+        flipmi_mesh *m = nullptr;
+        detail::check(flipmi_mesh_particles(detail::ctx(), params.isize, params.jsize, params.ksize, params.dx,
+                                            reinterpret_cast<const float *>(params.particles->data()),
+                                            (int64_t)params.particles->size(),
+                                            params.solidSDF->getPhiArray3d()->getRawArray(), &d, params.radius,
+                                            nullptr, 1.0, &m),
+                      "ParticleMesher::meshParticles");
+        TriangleMesh out;
+        try {
+            detail::copy_mesh(m, out);
+        } catch (...) {
+            flipmi_mesh_destroy(m);
+            throw;
+        }
+        flipmi_mesh_destroy(m);
+        return out;
+    }
 };
 
 // meshobject.h: a static mesh obstacle (updateMeshStatic).  enable() and
@@ -730,6 +832,7 @@ public:
         _desc.jsize = jsize;
         _desc.ksize = ksize;
         _desc.dx = dx;
+        flipmi_surface_desc_default(&_surface);
     }
     FluidSimulation(const FluidSimulation &) = delete;
     FluidSimulation &operator=(const FluidSimulation &) = delete;
@@ -754,6 +857,7 @@ public:
         _loadVel.clear();
         detail::check(flipmi_scene_set_marker_jitter(_scene, _jitterFactor, _jitterSurface, _jitterSeed),
                       "FluidSimulation::initialize");
+        detail::check(flipmi_scene_set_surface(_scene, &_surface), "FluidSimulation::initialize");
         for (MeshObject *ob : _obstacles) ob->_attach(_scene);
         for (MeshFluidSource *src : _sources) src->_attach(_scene);
         detail::check(flipmi_scene_initialize(_scene), "FluidSimulation::initialize");
@@ -795,6 +899,7 @@ public:
         auto t0 = std::chrono::steady_clock::now();
         flipmi_frame_stats st;
         detail::check(flipmi_scene_update(_scene, dt, &st), "FluidSimulation::update");
+        _isomeshStale = true;
         auto t1 = std::chrono::steady_clock::now();
         _stats = FluidSimulationFrameStats();
         _stats.frame = _currentFrame;
@@ -881,11 +986,78 @@ public:
         _desc.pic_ratio = r;
     }
     double getPICFLIPRatio() { return _desc.pic_ratio; }
-    // No surface mesh is produced and nothing is printed on this path.
-    void disableSurfaceReconstruction() {}
-    bool isSurfaceReconstructionEnabled() { return false; }
+    // Nothing is printed on this path.
     void disableConsoleOutput() {}
     void enableConsoleOutput() {}
+
+    // Surface reconstruction (fluidsimulation.cpp:168-540): on by default, as
+    // in the reference; the mesh of the particles after the first substep of
+    // each frame, read with getIsomesh().  One polygonizer slice and a
+    // minimum polyhedron size of 0 are built; other values throw at the
+    // setter (or at initialize()).
+    void enableSurfaceReconstruction() { _surface.enabled = 1; _pushSurface(); }
+    void disableSurfaceReconstruction() { _surface.enabled = 0; _pushSurface(); }
+    bool isSurfaceReconstructionEnabled() { return _surface.enabled != 0; }
+    int getSurfaceSubdivisionLevel() { return _surface.subdivisions; }
+    void setSurfaceSubdivisionLevel(int n) {
+        if (n < 1) throw std::domain_error("Error: subdivision level must be greater than or equal to 1.");
+        _surface.subdivisions = n;
+        _pushSurface();
+    }
+    int getNumPolygonizerSlices() { return _surface.polygonizer_slices; }
+    void setNumPolygonizerSlices(int n) {
+        if (n < 1) throw std::domain_error("Error: number of polygonizer slices must be greater than or equal to 1.");
+        _surface.polygonizer_slices = n;
+        _pushSurface();
+    }
+    double getSurfaceSmoothingValue() { return _surface.smoothing_value; }
+    void setSurfaceSmoothingValue(double s) { _surface.smoothing_value = s; _pushSurface(); }
+    int getSurfaceSmoothingIterations() { return _surface.smoothing_iterations; }
+    void setSurfaceSmoothingIterations(int n) {
+        if (n < 0) throw std::domain_error("Error: smoothing iterations must be greater than or equal to 0.");
+        _surface.smoothing_iterations = n;
+        _pushSurface();
+    }
+    int getMinPolyhedronTriangleCount() { return _surface.min_polyhedron_triangles; }
+    void setMinPolyhedronTriangleCount(int n) {
+        if (n < 0) throw std::domain_error("Error: min triangle count must be greater than or equal to 0.");
+        _surface.min_polyhedron_triangles = n;
+        _pushSurface();
+    }
+    double getMarkerParticleScale() { return _surface.particle_scale; }
+    void setMarkerParticleScale(double s) {
+        if (s < 0.0) throw std::domain_error("Error: marker particle scale must be greater than or equal to 0.");
+        _surface.particle_scale = s;
+        _pushSurface();
+    }
+    double getObstacleMeshingOffset() { return _surface.obstacle_meshing_offset; }
+    void setObstacleMeshingOffset(double s) { _surface.obstacle_meshing_offset = s; _pushSurface(); }
+    void enableRemoveSurfaceNearDomain() { _surface.remove_near_domain = 1; _pushSurface(); }
+    void disableRemoveSurfaceNearDomain() { _surface.remove_near_domain = 0; _pushSurface(); }
+    bool isRemoveSurfaceNearDomainEnabled() { return _surface.remove_near_domain != 0; }
+    int getRemoveSurfaceNearDomainDistance() { return _surface.remove_near_domain_distance; }
+    void setRemoveSurfaceNearDomainDistance(int n) {
+        if (n < 0) throw std::domain_error("Error: distance must be greater than or equal to 0.");
+        _surface.remove_near_domain_distance = n;
+        _pushSurface();
+    }
+    void enableInvertedContactNormals() { _surface.invert_contact_normals = 1; _pushSurface(); }
+    void disableInvertedContactNormals() { _surface.invert_contact_normals = 0; _pushSurface(); }
+    bool isInvertedContactNormalsEnabled() { return _surface.invert_contact_normals != 0; }
+    // fluidsimulation.cpp:2543-2546: the last surface mesh, domain coordinates.
+    TriangleMesh &getIsomesh() {
+        if (_scene && _isomeshStale) {
+            int64_t nv = 0, nt = 0;
+            detail::check(flipmi_scene_get_isomesh(_scene, &nv, &nt, nullptr, nullptr), "FluidSimulation::getIsomesh");
+            _isomesh.vertices.assign((size_t)nv, vmath::vec3());
+            _isomesh.triangles.assign((size_t)nt, Triangle());
+            detail::check(flipmi_scene_get_isomesh(_scene, &nv, &nt, reinterpret_cast<float *>(_isomesh.vertices.data()),
+                                                   reinterpret_cast<int32_t *>(_isomesh.triangles.data())),
+                          "FluidSimulation::getIsomesh");
+            _isomeshStale = false;
+        }
+        return _isomesh;
+    }
 
     // Not in the reference: PCG arithmetic (FLIPMI_PCG_FP64 = the reference's).
     void setPressureSolverPrecision(int precision) {
@@ -982,6 +1154,9 @@ private:
     void _requireScene() {
         if (!_initialized) throw std::runtime_error("Error: FluidSimulation must be initialized first.");
     }
+    void _pushSurface() {
+        if (_scene) detail::check(flipmi_scene_set_surface(_scene, &_surface), "FluidSimulation surface setting");
+    }
     void _pushJitter() {
         if (_scene)
             detail::check(flipmi_scene_set_marker_jitter(_scene, _jitterFactor, _jitterSurface, _jitterSeed),
@@ -1014,6 +1189,9 @@ private:
     bool _lastPressureOk = true;
     std::vector<MeshFluidSource *> _sources;
     std::vector<MeshObject *> _obstacles;
+    flipmi_surface_desc _surface;
+    TriangleMesh _isomesh;
+    bool _isomeshStale = true;
     double _jitterFactor = 1.0;  // fluidsimulation.h:1553
     int _jitterSurface = 0;
     uint64_t _jitterSeed = 0;

@@ -5,7 +5,8 @@
 // against the oracle.
 //
 //   wrapper_check <mode> <dir>
-//   modes: levelset  pressure  p2g  advect  sim  sources
+//   modes: levelset  pressure  p2g  advect  sim  sources  surface
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -319,6 +320,72 @@ int run_sources() {
     return 0;
 }
 
+// FluidSimulation with the viewer's surface setting (subdivision level 2,
+// FluidManager.cpp:52) for d.extra frames of one substep each; the isomesh
+// is read and turned into vertex normals the way FluidManager::iGetVertices
+// and iGetIndices do (FluidManager.cpp:104-170).  Then ParticleMesher::
+// meshParticles on the loaded particles over the domain's solid SDF.
+int run_surface() {
+    Dims d = read_dims();
+    std::vector<float> pos = read_raw<float>("positions.f32"), vel = read_raw<float>("velocities.f32");
+    FluidSimulation sim(d.I, d.J, d.K, d.dx);
+    if (!sim.isSurfaceReconstructionEnabled()) throw std::runtime_error("surface reconstruction is off by default");
+    sim.setSurfaceSubdivisionLevel(2);
+    sim.addBodyForce(0.0, -9.81, 0.0);
+    sim.setMinTimeStepsPerFrame(1);
+    sim.setMaxTimeStepsPerFrame(1);
+    FluidSimulationMarkerParticleData data;
+    data.size = (int)(pos.size() / 3);
+    data.positions = reinterpret_cast<char *>(pos.data());
+    data.velocities = reinterpret_cast<char *>(vel.data());
+    sim.loadMarkerParticleData(data);
+    sim.initialize();
+    if (!sim.getIsomesh().vertices.empty()) throw std::runtime_error("a mesh before the first update");
+    for (int f = 0; f < (int)d.extra; f++) sim.update(d.dt);
+    using namespace vmath;
+    TriangleMesh isomesh = sim.getIsomesh();
+    std::vector<vec3> normal(isomesh.vertices.size(), vec3(0.0f, 0.0f, 0.0f));
+    std::vector<unsigned int> indices;
+    for (size_t i = 0; i < isomesh.triangles.size(); i++) {
+        int i0 = isomesh.triangles[i].tri[0], i1 = isomesh.triangles[i].tri[1], i2 = isomesh.triangles[i].tri[2];
+        vec3 e0 = isomesh.vertices[i1] - isomesh.vertices[i0];
+        vec3 e1 = isomesh.vertices[i2] - isomesh.vertices[i0];
+        vec3 faceN = cross(e0, e1);
+        normal[i0] += faceN;
+        normal[i1] += faceN;
+        normal[i2] += faceN;
+        for (int j = 0; j < 3; j++) indices.push_back(static_cast<unsigned int>(isomesh.triangles[i].tri[j]));
+    }
+    for (vec3 &n : normal) n = normalize(n);
+    write_raw("iso_verts.f32", reinterpret_cast<const float *>(isomesh.vertices.data()), 3 * isomesh.vertices.size());
+    write_raw("iso_tris.i32", indices.data(), indices.size());
+    write_raw("iso_normals.f32", reinterpret_cast<const float *>(normal.data()), 3 * normal.size());
+    std::vector<char> pblob(sim.getMarkerParticlePositionDataSize());
+    if (!pblob.empty()) sim.getMarkerParticlePositionData(pblob.data());
+    write_raw("positions_out.f32", pblob.data(), pblob.size());
+
+    MeshLevelSet solid(d.I, d.J, d.K, d.dx);
+    solid.calculateDomainBoundary();
+    std::vector<vmath::vec3> particles;
+    for (size_t q = 0; q + 2 < pos.size(); q += 3) particles.push_back(vmath::vec3(pos[q], pos[q + 1], pos[q + 2]));
+    ParticleMesherParameters params;
+    params.isize = d.I;
+    params.jsize = d.J;
+    params.ksize = d.K;
+    params.dx = d.dx;
+    params.subdivisions = 1;
+    params.computechunks = 1;
+    const double volume = d.dx * d.dx * d.dx / 8.0, pi = 3.141592653;
+    params.radius = std::pow(3 * volume / (4 * pi), 1.0 / 3.0) * 3.0;
+    params.particles = &particles;
+    params.solidSDF = &solid;
+    ParticleMesher mesher;
+    TriangleMesh m = mesher.meshParticles(params);
+    write_raw("mesher_verts.f32", reinterpret_cast<const float *>(m.vertices.data()), 3 * m.vertices.size());
+    write_raw("mesher_tris.i32", reinterpret_cast<const int *>(m.triangles.data()), 3 * m.triangles.size());
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc != 3) {
         std::cerr << "usage: wrapper_check <levelset|pressure|p2g|advect|sim> <dir>\n";
@@ -334,6 +401,7 @@ int main(int argc, char **argv) {
         if (mode == "advect") return run_advect();
         if (mode == "sim") return run_sim();
         if (mode == "sources") return run_sources();
+        if (mode == "surface") return run_surface();
     } catch (const std::exception &e) {
         std::cerr << "wrapper_check " << mode << ": " << e.what() << "\n";
         return 1;
