@@ -98,7 +98,8 @@ def parse_args(argv=None):
                    help="serving: hardware queues per process (GPU_MAX_HW_QUEUES, at most 32; 0 = as set)")
     p.add_argument("--plain-requests", type=int, default=24,
                    help="serving: requests also run one at a time on the plain FP64 path")
-    p.add_argument("--workload", choices=["dam_break_64", "pool_drop_128", "large_pool_512"], default=None,
+    p.add_argument("--workload", choices=["dam_break_64", "pool_drop_128", "large_pool_512", "double_dam_surface_256"],
+                   default=None,
                    help="one of the other workloads instead of the headline (see run_workload)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 256 MiB device to device 3 times (known bytes, to scale HBM counters)")
@@ -154,14 +155,14 @@ class _StdoutToStderr:
         return False
 
 
-def cpu_baseline(scene, args, substeps=(1, 1)):
+def cpu_baseline(scene, args, substeps=(1, 1), surface=False):
     """Times the reference engine (or, where it was not built, the CPU
     oracle) on the same scene for whole updates until --cpu-seconds pass."""
     with _StdoutToStderr():
-        return _cpu_baseline(scene, args, substeps)
+        return _cpu_baseline(scene, args, substeps, surface)
 
 
-def _cpu_baseline(scene, args, substeps=(1, 1)):
+def _cpu_baseline(scene, args, substeps=(1, 1), surface=False):
     from oracle import bindings
     I, J, K = args.size
     if bindings.ref_available():
@@ -176,6 +177,8 @@ def _cpu_baseline(scene, args, substeps=(1, 1)):
     h.add_body_force(0.0, -9.81, 0.0)
     h.set_substeps(*substeps)
     h.load_particles(scene.positions, scene.velocities)
+    if surface:
+        h.set_surface(True)
     h.initialize()
     done = 0
     work = 0
@@ -197,7 +200,8 @@ def _cpu_baseline(scene, args, substeps=(1, 1)):
         "cores": threads,
         "kind": kind,
         "sample": (f"{done} update(s) of dt={args.dt:.6g} s of the same {I}x{J}x{K} {scene.name.replace('_', ' ')} "
-                   f"({scene.num_particles} particles), FP64 PCG, {threads} thread(s), {elapsed:.1f} s"),
+                   f"({scene.num_particles} particles), FP64 PCG, {threads} thread(s), {elapsed:.1f} s"
+                   + (", surface reconstruction on" if surface else "")),
     }
 
 
@@ -322,6 +326,12 @@ WORKLOADS = {
     # 2 m/s), mixed PCG at 1e-6 relative, 10 substeps of 1/480 s, seed 5.
     "large_pool_512": ((512, 256, 256), 1 / 256, lambda sc: sc.pool_jet(512, 256, 256, 1 / 256, seed=5),
                        "mixed", 1e-6, (1, 1), 1 / 480, 10),
+    # The headline's 256x128x128 double dam break (seed 3, mixed PCG, one
+    # substep of 1/240 s per update) with surface reconstruction on, at the
+    # reference's default settings: getIsomesh's mesh is built every frame.
+    "double_dam_surface_256": ((256, 128, 128), 1 / 128,
+                               lambda sc: sc.double_dam_break(256, 128, 128, 1 / 128, seed=3), "mixed", 1e-9,
+                               (1, 1), 1 / 240, 20, {"surface": True}),
 }
 
 
@@ -330,7 +340,8 @@ def run_workload(args):
     iterations) with the reference engine timed on a bounded sample of it."""
     import flipengine3d_amd as fa
     from flipengine3d_amd import scenes
-    (I, J, K), dx, build, precision, tol, substeps, dt, updates = WORKLOADS[args.workload]
+    (I, J, K), dx, build, precision, tol, substeps, dt, updates, *extra = WORKLOADS[args.workload]
+    surface = bool(extra and extra[0].get("surface"))
     t = time.perf_counter()
     scene = build(scenes)
     inflow = None
@@ -345,7 +356,11 @@ def run_workload(args):
     sim.loadMarkerParticleData(scene.positions, scene.velocities)
     if inflow is not None:
         sim.addMeshFluidSource(inflow.vertices, inflow.triangles, inflow.velocity)
+    if surface:
+        sim.enableSurfaceReconstruction()
     sim.initialize()
+    if surface:
+        sim.set_profiling(1)  # stage events: the surface build's time per frame
     lat, work, nsub, iters, failures = [], 0, 0, 0, 0
     n = sim.getNumMarkerParticles()
     sim.ctx.synchronize()
@@ -362,6 +377,12 @@ def run_workload(args):
         if (q + 1) % 20 == 0:
             log(f"{args.workload}: {q + 1} updates, {time.perf_counter() - t0:.1f} s", args)
     elapsed = time.perf_counter() - t0
+    mesh = None
+    if surface:
+        v, t = sim.getIsomesh()
+        ms, calls = sim.stage_times()["surface"]
+        mesh = {"surface_ms_per_frame": ms / max(calls, 1), "frames_meshed": calls, "vertices": int(v.shape[0]),
+                "triangles": int(t.shape[0])}
     sim.close()
     lat.sort()
     out = {
@@ -381,12 +402,12 @@ def run_workload(args):
                    "pcg_iterations_last_substep_mean": iters / updates, "failed_pressure_solves": failures,
                    "inflow": None if inflow is None else {"velocity": list(inflow.velocity),
                                                          "triangles": int(inflow.triangles.shape[0])},
-                   "particles_end": n},
+                   "particles_end": n, "surface": mesh},
     }
     if args.cpu_baseline and scene.num_particles <= 20_000_000:
         a = argparse.Namespace(**vars(args))
         a.size, a.dt = [I, J, K], dt
-        out["cpu_baseline"] = cpu_baseline(scene, a, substeps=substeps)
+        out["cpu_baseline"] = cpu_baseline(scene, a, substeps=substeps, surface=surface)
         out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     else:
         out["cpu_baseline"] = None

@@ -36,7 +36,7 @@ PCG_MIXED = 1
 
 STAGES = ["sort", "classify", "p2g", "extrapolate", "assemble", "mic0", "pcg", "apply",
           "g2p_advect", "remove", "glue", "pcg_iters", "k_spmv_dot", "k_axpy_norm", "k_forward",
-          "k_backward", "k_update_s", "k_true_residual", "inflow"]
+          "k_backward", "k_update_s", "k_true_residual", "inflow", "surface"]
 NUM_STAGES = 24
 
 _c_int = ctypes.c_int
