@@ -96,6 +96,8 @@ def parse_args(argv=None):
     p.add_argument("--budget-gib", type=float, default=200.0, help="serving: device memory budget")
     p.add_argument("--hw-queues", type=int, default=16,
                    help="serving: hardware queues per process (GPU_MAX_HW_QUEUES, at most 32; 0 = as set)")
+    p.add_argument("--launch-graphs", type=int, default=1, choices=[0, 1],
+                   help="serving: PCG iteration batches as captured HIP graphs")
     p.add_argument("--plain-requests", type=int, default=24,
                    help="serving: requests also run one at a time on the plain FP64 path")
     p.add_argument("--workload", choices=["dam_break_64", "pool_drop_128", "large_pool_512", "double_dam_surface_256"],
@@ -438,28 +440,18 @@ def serving_requests(args):
     return out
 
 
-def run_serving(args):
-    """Serving: requests arrive on their schedule and are admitted by the
-    server (continuous batching, memory budget); value = particle-substeps of
-    all requests over the makespan.  The plain path (one scene at a time,
-    FP64 PCG, fresh FluidSimulation each) runs a prefix of the same requests
-    back to back for comparison."""
+def _serve(fa, ctx, reqs, args, arrivals=True):
+    """Runs the requests through one server; with arrivals=False every
+    request is submitted at once (the server's capacity).  Returns (reports,
+    makespan s, server stats)."""
     import threading
-
-    if args.hw_queues:
-        # Streams beyond the hardware queues share one; read at device init.
-        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, args.hw_queues)))
-    import flipengine3d_amd as fa
-    t = time.perf_counter()
-    reqs = serving_requests(args)
-    log(f"serving: {len(reqs)} requests built in {time.perf_counter() - t:.1f} s", args)
-    ctx = fa.Context(0)
-    server = fa.Server(ctx, max_active=args.max_active, memory_budget=int(args.budget_gib * (1 << 30)))
+    server = fa.Server(ctx, max_active=args.max_active, memory_budget=int(args.budget_gib * (1 << 30)),
+                       launch_graphs=bool(args.launch_graphs))
     ids = [None] * len(reqs)
 
     def submitter(t0):
         for q, (ta, dims, dx, sc) in enumerate(reqs):
-            delay = t0 + ta - time.perf_counter()
+            delay = t0 + ta - time.perf_counter() if arrivals else 0.0
             if delay > 0:
                 time.sleep(delay)
             desc = fa.scene_desc(*dims, dx, precision=fa.PCG_MIXED, substeps=(1, 1))
@@ -481,32 +473,62 @@ def run_serving(args):
     makespan = time.perf_counter() - t0
     st = server.stats()
     server.close()
-    work = sum(r["particle_substeps"] for r in reps)
-    failed = sum(1 for r in reps if r["status"] != 0)
+    return reps, makespan, st
 
-    # Plain path: one scene at a time, FP64 PCG, fresh scene per request.
-    plain_work = 0
-    plain_time = 0.0
-    plain_steps = []
-    for ta, dims, dx, sc in reqs[:args.plain_requests]:
-        sim = fa.FluidSimulation(*dims, dx, ctx=ctx, precision=fa.PCG_FP64)
+
+def _plain(fa, ctx, reqs, args, precision):
+    """The requests one at a time, a fresh FluidSimulation each; the clock
+    covers scene creation, loading, initialize and every update, as the
+    server's makespan does."""
+    work, elapsed, steps = 0, 0.0, []
+    for ta, dims, dx, sc in reqs:
+        ts = time.perf_counter()
+        sim = fa.FluidSimulation(*dims, dx, ctx=ctx, precision=precision)
         sim.addBodyForce(0.0, -9.81, 0.0)
         sim.setMinTimeStepsPerFrame(1)
         sim.setMaxTimeStepsPerFrame(1)
-        ts = time.perf_counter()
         sim.loadMarkerParticleData(sc.positions, sc.velocities)
         sim.initialize()
         for _ in range(args.request_steps):
             n = sim.getNumMarkerParticles()
             t1 = time.perf_counter()
             stp = sim.update(args.dt)
-            plain_steps.append((time.perf_counter() - t1) * 1e3)
-            plain_work += n * stp.substeps
-        plain_time += time.perf_counter() - ts
+            steps.append((time.perf_counter() - t1) * 1e3)
+            work += n * stp.substeps
         sim.close()
-    plain_value = plain_work / plain_time if plain_time > 0 else None
+        elapsed += time.perf_counter() - ts
+    steps.sort()
+    return {"value": work / elapsed if elapsed > 0 else None, "requests": len(reqs),
+            "p50_ms": steps[len(steps) // 2] if steps else None,
+            "p99_ms": steps[min(len(steps) - 1, int(round(0.99 * (len(steps) - 1))))] if steps else None}
+
+
+def run_serving(args):
+    """Serving: requests arrive on their schedule and are admitted by the
+    server (continuous batching, memory budget); value = particle-substeps of
+    all requests over the makespan.  The same requests submitted all at once
+    give the server's capacity.  The plain path (one scene at a time, a fresh
+    FluidSimulation each, creation inside the clock as in the server's
+    makespan) runs a prefix of the same requests in the same mixed precision,
+    and in FP64."""
+    if args.hw_queues:
+        # Streams beyond the hardware queues share one; read at device init.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, args.hw_queues)))
+    import flipengine3d_amd as fa
+    t = time.perf_counter()
+    reqs = serving_requests(args)
+    log(f"serving: {len(reqs)} requests built in {time.perf_counter() - t:.1f} s", args)
+    ctx = fa.Context(0)
+    reps, makespan, st = _serve(fa, ctx, reqs, args, arrivals=True)
+    creps, cmakespan, cst = _serve(fa, ctx, reqs, args, arrivals=False)
+    work = sum(r["particle_substeps"] for r in reps)
+    cwork = sum(r["particle_substeps"] for r in creps)
+    failed = sum(1 for r in reps if r["status"] != 0) + sum(1 for r in creps if r["status"] != 0)
+    prefix = reqs[:args.plain_requests]
+    plain_mixed = _plain(fa, ctx, prefix, args, fa.PCG_MIXED)
+    plain_fp64 = _plain(fa, ctx, prefix, args, fa.PCG_FP64)
     value = work / makespan
-    plain_steps.sort()
+    capacity = cwork / cmakespan
     nps = [sc.num_particles for _, _, _, sc in reqs]
     return {
         "metric": "serving particle-substeps/s (mixed-size scenes, Poisson arrivals); p50/p99 step latency",
@@ -524,6 +546,7 @@ def run_serving(args):
                          f"{args.rate:g}/s, seed 4"),
             "particles_mean": sum(nps) / len(nps),
             "max_active": args.max_active,
+            "launch_graphs": bool(args.launch_graphs),
             "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
             "budget_gib": args.budget_gib,
             "makespan_s": makespan,
@@ -536,14 +559,12 @@ def run_serving(args):
             "peak_resident_gib": st["peak_resident_bytes"] / (1 << 30),
             "queue_ms_p50": statistics.median(r["queue_ms"] for r in reps),
         },
-        "plain_fp64_path": {
-            "value": plain_value,
-            "requests": min(args.plain_requests, len(reqs)),
-            "p50_ms": plain_steps[len(plain_steps) // 2] if plain_steps else None,
-            "p99_ms": plain_steps[min(len(plain_steps) - 1, int(round(0.99 * (len(plain_steps) - 1))))]
-            if plain_steps else None,
-        },
-        "vs_plain": value / plain_value if plain_value else None,
+        "capacity": {"value": capacity, "makespan_s": cmakespan, "p50_ms": cst["step_ms_p50"],
+                     "p99_ms": cst["step_ms_p99"], "scenes_created": cst["scenes_created"],
+                     "scenes_reused": cst["scenes_reused"]},
+        "plain_mixed_path": plain_mixed,
+        "plain_fp64_path": plain_fp64,
+        "vs_plain": capacity / plain_mixed["value"] if plain_mixed["value"] else None,
     }
 
 

@@ -74,7 +74,8 @@ class PressureArgs(ctypes.Structure):
 
 
 class ServerDesc(ctypes.Structure):
-    _fields_ = [("max_active", _c_int), ("memory_budget", _c_i64), ("particle_capacity_per_cell", _c_double)]
+    _fields_ = [("max_active", _c_int), ("memory_budget", _c_i64), ("particle_capacity_per_cell", _c_double),
+                ("launch_graphs", _c_int)]
 
 
 class Request(ctypes.Structure):
@@ -126,7 +127,7 @@ EXPORTED = [
     "flipmi_server_wait", "flipmi_server_take", "flipmi_server_get_stats", "flipmi_server_destroy",
     "flipmi_extrapolate_noalloc", "flipmi_surface_desc_default", "flipmi_scene_set_surface",
     "flipmi_scene_get_isomesh", "flipmi_mesh_particles", "flipmi_mesh_counts", "flipmi_mesh_copy",
-    "flipmi_mesh_destroy",
+    "flipmi_mesh_destroy", "flipmi_scene_set_launch_graphs",
 ]
 
 _lib = None
@@ -168,6 +169,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
     L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
     L.flipmi_scene_set_kernel_timing.argtypes = [_vp, ctypes.c_uint]
+    L.flipmi_scene_set_launch_graphs.argtypes = [_vp, _c_int]
     L.flipmi_domain_solid.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double] + [_vp] * 5
     L.flipmi_liquid_sdf.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_double, _vp, _vp]
     L.flipmi_liquid_sdf_postprocess.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _vp]
@@ -483,6 +485,11 @@ class FluidSimulation:
         lvl = int(level) if not isinstance(level, bool) else (1 if level else 0)
         self.ctx.check(self.L.flipmi_scene_set_profiling(self.h, lvl))
 
+    def set_launch_graphs(self, enabled=True):
+        """PCG iteration batches as captured HIP graphs (bit-identical results)."""
+        self._ensure()
+        self.ctx.check(self.L.flipmi_scene_set_launch_graphs(self.h, int(bool(enabled))))
+
     def set_kernel_timing(self, kernels=None):
         """PCG kernels timed at profiling level 2 (names from STAGES, e.g.
         "k_backward"); None = all."""
@@ -733,12 +740,13 @@ class Server:
     memory budget and stepped concurrently, one HIP stream per slot."""
 
     def __init__(self, ctx: Optional[Context] = None, max_active: int = 8, memory_budget: int = 200 << 30,
-                 particle_capacity_per_cell: float = 3.0):
+                 particle_capacity_per_cell: float = 3.0, launch_graphs: bool = True):
         self.ctx = ctx or Context()
         self.L = self.ctx.L
         d = ServerDesc()
         self.L.flipmi_server_desc_default(ctypes.byref(d))
         d.max_active, d.memory_budget, d.particle_capacity_per_cell = max_active, memory_budget, particle_capacity_per_cell
+        d.launch_graphs = int(bool(launch_graphs))
         h = _vp()
         self.ctx.check(self.L.flipmi_server_create(self.ctx.h, ctypes.byref(d), ctypes.byref(h)))
         self.h = h

@@ -363,6 +363,12 @@ int flipmi_scene_get_isomesh(flipmi_scene *scene, int64_t *num_vertices, int64_t
     });
 }
 
+int flipmi_scene_set_launch_graphs(flipmi_scene *scene, int enabled) {
+    if (!scene) return FLIPMI_ERR_INVALID_ARG;
+    scene->s.launch_graphs = enabled != 0;
+    return FLIPMI_OK;
+}
+
 int flipmi_scene_get_grid(flipmi_scene *scene, int which, void *out) {
     if (!scene || !out) return FLIPMI_ERR_INVALID_ARG;
     Scene &s = scene->s;

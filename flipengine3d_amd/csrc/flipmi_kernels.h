@@ -133,6 +133,10 @@ struct PcgScalars {
     double delta;                       // replace when |r32| < delta * rmax
     int refresh;                        // a true residual is due
     int refinements;                    // true residuals computed
+    // Epoch of the next sweep launch: read by every workgroup of a sweep,
+    // advanced by its last tile, so launches replayed from a captured graph
+    // still tag their records afresh.  Starts at 1 (records start at 0).
+    unsigned int sweep_epoch;
 };
 
 // Sweep-ordered ("slot") storage of the pressure system.  Cells are grouped
@@ -199,7 +203,7 @@ struct PressureBuffers {
     // solve) and their number; the per-group PCG kernels walk this list.
     int *active_list;
     int *active_count;
-    unsigned int epoch;                 // tag of the current sweep launch (host counter)
+    unsigned int epoch;                 // sweep launches so far (host count, an upper bound of sweep_epoch)
     unsigned int *progress;             // factorisation: tile progress counters
     int ntiles_j, ntiles_k;
     unsigned int *ticket;

@@ -483,7 +483,7 @@ __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const Gri
 // The last wave of a sweep launch resets the shared counters for the next
 // launch and, for the backward sweep, reduces the dot product.
 __device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, int tile,
-                             bool reset_progress) {
+                             bool reset_progress, unsigned next_epoch = 0) {
     const int lane = threadIdx.x;
     const int ntiles = pb.sm.ntj * pb.sm.ntk;
     if (dotmode != kDotNone) {
@@ -500,6 +500,7 @@ __device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, i
     if (lane == 0) {
         *pb.ticket = 0;
         pb.sc->counter[0] = 0;
+        if (next_epoch) pb.sc->sweep_epoch = next_epoch;
     }
     if (dotmode != kDotNone) {
         // Fixed-order sum of the per-tile partials.
@@ -730,6 +731,7 @@ struct SweepMailbox {
     int consumed;  // chunks read by the compute wave
     int tile;      // the workgroup's ticket
     int skip;      // the solve has finished: nothing to do
+    unsigned epoch;  // this launch's record tag
 };
 
 __device__ __forceinline__ int lds_acquire(int *p) {
@@ -875,7 +877,7 @@ __device__ void sweep_poller(PressureBuffers &pb, SweepMailbox<T> &mb, int tile,
                              int nchunks, int ubj, int uej, int ubk, int uek, int lane) {
     constexpr int C = kChunk, RW = Record<T>::W;
     const int S = pb.sm.S;
-    const unsigned epoch = pb.epoch;
+    const unsigned epoch = mb.epoch;
     const int rs = lane >> 3, re = lane & 7;  // step within the chunk, edge lane
     const int aj = FWD ? 7 : 0, bk = FWD ? 7 : 0;
     const int rslot_j = re, rslot_k = re == aj ? bk : 8 + re;
@@ -946,6 +948,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
         const int skip = ld_dev(&pb.sc->done) != 0;
         mb.skip = skip;
         mb.tile = skip ? 0 : (int)atomicAdd(pb.ticket, 1u);
+        mb.epoch = ld_dev(&pb.sc->sweep_epoch);
         mb.ready = 0;
         mb.consumed = 0;
     }
@@ -979,7 +982,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
     const int aj = FWD ? 7 : 0, bk = FWD ? 7 : 0;
     const bool pub = a == aj || b == bk;
     const int pub_slot = a == aj ? b : 8 + a;
-    const unsigned epoch = pb.epoch;
+    const unsigned epoch = mb.epoch;
     const size_t grp0 = (size_t)tile * (size_t)(S >> 2);
     const bool dot = !FWD && dotmode != kDotNone;
     // Without a dot product the backward sweep still streams a vector (its
@@ -1165,7 +1168,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
             }
         }
     }
-    finish_sweep(pb, dot ? dotmode : kDotNone, dot_acc, tile, false);
+    finish_sweep(pb, dot ? dotmode : kDotNone, dot_acc, tile, false, epoch + 1);
 }
 
 // Launches a kernel on s.  With a start event the dispatch itself records
@@ -1194,10 +1197,15 @@ struct SweepDepth<float> {
 template <class T>
 static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, SweepOps<T> op,
                                int dotmode, hipStream_t s, const KernelTiming &kt = KernelTiming{}) {
-    // A fresh epoch per launch; records of older launches never match it.
+    // A fresh epoch per launch (the device's sweep_epoch, advanced by each
+    // launch's last tile); records of older launches never match it.  The
+    // host counts launches, an upper bound of the device epoch, and clears
+    // the records before the tag could wrap.
     if (++pb.epoch >= 0xfffffff0u) {
         size_t n = edge_record_count(pb.sm);
         hipError_t e = launch_fill_u8((uint8_t *)pb.edge, n * 8, 0, s);
+        if (e != hipSuccess) return e;
+        e = launch_fill_i32((int *)&pb.sc->sweep_epoch, 1, 1, s);
         if (e != hipSuccess) return e;
         pb.epoch = 1;
     }

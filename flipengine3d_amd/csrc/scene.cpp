@@ -118,6 +118,7 @@ size_t PressureStore::device_bytes() const {
 }
 
 void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
+    graphs.clear();  // captured launches point into the buffers replaced here
     SlotMap sm = make_slot_map(g.I, g.J, g.K);
     // Slot arrays carry the zero and sink groups after the last tile.
     size_t ns = sm.alloc_count();
@@ -135,6 +136,7 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     (void)launch_fill_i32((int *)progress.p, progress.n, 0, s);
     (void)launch_fill_i32((int *)ticket.p, 1, 0, s);
     (void)launch_fill_u8((uint8_t *)scal.p, sizeof(PcgScalars), 0, s);
+    (void)launch_fill_i32((int *)&scal.p->sweep_epoch, 1, 1, s);
     // Slots outside the grid are never written by the kernels: start at zero.
     for (int q = 0; q < kSlotArrays; q++) (void)launch_fill_f64(arr[q].p, ns, 0.0, s);
     (void)launch_fill_u8((uint8_t *)edge.p, nrec * sizeof(unsigned long long), 0, s);
@@ -183,6 +185,36 @@ static void hip_check(hipError_t e, const char *what) {
         throw StatusError{FLIPMI_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
 }
 
+// One batch of PCG iterations as a graph: captured the first time this
+// (batch size, starting s buffer) occurs, replayed afterwards.  The kernels
+// read their scalars (alpha, beta, the sweep epoch, done) from device memory,
+// so a replay computes exactly what launching the kernels would; only the
+// host's bookkeeping of the launches (the s buffer swap of each SpMV, the
+// count of sweep launches) is advanced by hand on a replay.
+static void launch_pcg_batch_graph(PcgGraphs &G, const GridGeom &g, bool mixed, PressureBuffers &pb, int batch,
+                                   hipStream_t s) {
+    const auto key = std::make_pair(batch, (const void *)pb.s32);
+    auto it = G.exec.find(key);
+    if (it == G.exec.end()) {
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "pcg graph capture");
+        hipError_t le = hipSuccess;
+        for (int q = 0; q < batch && le == hipSuccess; q++) le = launch_pcg_iteration(g, mixed, pb, s);
+        hipGraph_t graph = nullptr;
+        hipError_t ce = hipStreamEndCapture(s, &graph);
+        hip_check(le, "pcg iteration (capture)");
+        hip_check(ce, "pcg graph capture");
+        hipGraphExec_t exec = nullptr;
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hip_check(ie, "pcg graph instantiate");
+        it = G.exec.emplace(key, exec).first;
+    } else {
+        if (mixed && (batch & 1)) std::swap(pb.s32, pb.s32b);
+        pb.epoch += 2u * (unsigned)batch;
+    }
+    hip_check(hipGraphLaunch(it->second, s), "pcg graph launch");
+}
+
 PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScalars *hs,
                              hipStream_t s, Scene *prof) {
     PressureOutcome out;
@@ -224,6 +256,11 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
     double prev_resid = bnorm;
     const double target = w.tol * bnorm;
     for (;;) {
+        // A batch runs from a captured graph once the first SpMV (which reads
+        // s as initialised) is behind, unless kernels are being timed or the
+        // sweep tag is near its wrap-around (which clears the records).
+        const bool use_graph = w.graphs && pb.s_plain == 0 && !(prof && prof->profiling >= 2) &&
+                               pb.epoch + 2u * (unsigned)batch + 8u < 0xfffffff0u;
         for (int q = 0; q < batch; q++) {
             if (prof && prof->profiling >= 2 && prof->kernel_mask != 0) {
                 // Time the selected kernels of the iteration on the launch stream.
@@ -239,6 +276,9 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
                     hip_check(launch_pcg_piece(piece, w.g, mixed, pb, s, kt), "pcg kernel");
                     if (kt.start) prof->add_timed(stage, kt.start, kt.stop);
                 }
+            } else if (use_graph) {
+                launch_pcg_batch_graph(*w.graphs, w.g, mixed, pb, batch, s);
+                break;  // the graph holds the whole batch
             } else {
                 hip_check(launch_pcg_iteration(w.g, mixed, pb, s), "pcg iteration");
             }
@@ -646,6 +686,7 @@ void Scene::pressure_solve(double dt) {
     pw.liquid = liquid.p; pw.wU = wU.p; pw.wV = wV.p; pw.wW = wW.p; pw.wC = wC.p;
     pw.sU = sU.p; pw.sV = sV.p; pw.sW = sW.p;
     pw.u = u.p; pw.v = v.p; pw.w = w.p; pw.vu = vu.p; pw.vv = vv.p; pw.vw = vw.p;
+    pw.graphs = launch_graphs ? &press->graphs : nullptr;
     PressureOutcome o =
         run_pressure(pw, press->pb, (PcgScalars *)host_stage, stream, profiling ? this : nullptr);
     last_iters = o.iterations;

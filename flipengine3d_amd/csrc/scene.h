@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -162,6 +163,9 @@ struct Scene {
     flipmi_surface_desc surface{};
     SurfaceStore *mesh = nullptr;  // owned
 
+    // PCG iteration batches launched as captured HIP graphs.
+    bool launch_graphs = false;
+
     // last pressure solve
     int last_iters = 0, last_ok = 1;
     double last_resid = 0;
@@ -235,6 +239,20 @@ struct Scene {
 
 // Pressure solve over caller grids already on the device (used by the
 // PressureSolver drop-in and by Scene::pressure_solve).
+// PCG iteration batches captured as HIP graphs, one per (batch size, s32
+// buffer the batch starts from); replayed instead of launching each kernel.
+struct PcgGraphs {
+    std::map<std::pair<int, const void *>, hipGraphExec_t> exec;
+    PcgGraphs() {}
+    PcgGraphs(const PcgGraphs &) = delete;
+    PcgGraphs &operator=(const PcgGraphs &) = delete;
+    ~PcgGraphs() { clear(); }
+    void clear() {
+        for (auto &e : exec) (void)hipGraphExecDestroy(e.second);
+        exec.clear();
+    }
+};
+
 struct PressureWork {
     GridGeom g;
     double dt, tol, acceptable;
@@ -242,6 +260,7 @@ struct PressureWork {
     const float *liquid, *wU, *wV, *wW, *wC, *sU, *sV, *sW;
     float *u, *v, *w;
     uint8_t *vu, *vv, *vw;
+    PcgGraphs *graphs = nullptr;  // non-null: iteration batches run as captured graphs
 };
 struct PressureOutcome {
     int iterations = 0;
@@ -269,6 +288,7 @@ struct PressureStore {
     DBuf<int> alist;  // active group list (groups) and its length (last element)
     DBuf<PcgScalars> scal;
     PressureBuffers pb{};
+    PcgGraphs graphs;             // captured iteration batches (pointers into these buffers)
     void alloc(const GridGeom &g, hipStream_t s, bool mixed);
     size_t device_bytes() const;
 };

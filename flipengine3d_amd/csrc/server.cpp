@@ -210,6 +210,7 @@ void flipmi_server::worker(int slot) {
                 s->g = make_geom(r->desc.isize, r->desc.jsize, r->desc.ksize, r->desc.dx);
                 s->stream = stream;
                 s->own_stream = false;
+                s->launch_graphs = this->desc.launch_graphs != 0;
                 s->load(r->pos.data(), vel, n);
                 p->scene.reset();
                 p->scene = std::move(s);
@@ -304,6 +305,7 @@ void flipmi_server_desc_default(flipmi_server_desc *d) {
     d->max_active = 8;
     d->memory_budget = (int64_t)200 << 30;
     d->particle_capacity_per_cell = 3.0;
+    d->launch_graphs = 1;
 }
 
 int flipmi_server_create(flipmi_ctx *ctx, const flipmi_server_desc *desc, flipmi_server **out) {

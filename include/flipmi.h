@@ -302,6 +302,12 @@ int flipmi_scene_set_profiling(flipmi_scene *scene, int level);
  * stage minus FLIPMI_STAGE_K_SPMV (bit 5: the true residual); default all.
  * Each timed launch adds an event pair, which delays the launches around it. */
 int flipmi_scene_set_kernel_timing(flipmi_scene *scene, unsigned mask);
+/* Not in the reference: run the PCG's iteration batches as HIP graphs,
+ * each (batch size, starting buffer) captured on first use and replayed
+ * afterwards, so a batch of k iterations is one launch instead of 5 k.
+ * Results are bit-identical either way.  Off by default; graphs are not
+ * used while PCG kernels are timed (profiling level 2). */
+int flipmi_scene_set_launch_graphs(flipmi_scene *scene, int enabled);
 /* Blocks until all work queued by the context's scenes has finished. */
 int flipmi_ctx_synchronize(flipmi_ctx *ctx);
 
@@ -431,6 +437,8 @@ typedef struct {
     int64_t memory_budget;             /* device bytes for resident scenes (default 200 GiB) */
     double particle_capacity_per_cell; /* particle capacity of a new scene per cell, so later
                                           requests of the same kind fit (default 3) */
+    int launch_graphs;                 /* the scenes' PCG iteration batches run as captured HIP
+                                          graphs (flipmi_scene_set_launch_graphs; default 1) */
 } flipmi_server_desc;
 
 typedef struct {
