@@ -35,27 +35,28 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per matrix row (fluid cell) of each PCG kernel: every
-# array the kernel must read or write, once.  The matrix (diag and the three
-# upper couplings; the lower three are the neighbours' upper ones) is FP64 on
-# both paths (32 bytes).  FP64 path, 8-byte vectors:
-#   spmv_dot:   s, matrix, z (= A s)                       -> 8 + 32 + 8
+# array the kernel must read or write, once.  The matrix is the FP64 diagonal
+# and the three upper couplings (the lower three are the neighbours' upper
+# ones), stored as the FP32 face weights the FP64 entries are rebuilt from
+# exactly: 8 + 3 x 4 = 20 bytes on both paths.  FP64 path, 8-byte vectors:
+#   spmv_dot:   s, matrix, z (= A s)                       -> 8 + 20 + 8
 #   axpy_norm:  x r/w, s, r r/w, z                          -> 6 x 8
 #   forward:    r, invdiag, 3 lower factor entries, z       -> 6 x 8
 #   backward:   z r/w, invdiag, 3 upper factor entries, and
 #               r for the fused z.r                         -> 7 x 8
 #   update_s:   z, s r/w                                    -> 3 x 8
-# Mixed path: FP32 vectors and factor, FP64 x and matrix:
-#   spmv_dot:   z, old s, matrix, new s, q (= A s)          -> 4 + 4 + 32 + 4 + 4
+# Mixed path: FP32 vectors and factor, FP64 x and matrix entries:
+#   spmv_dot:   z, old s, matrix, new s, q (= A s)          -> 4 + 4 + 20 + 4 + 4
 #   axpy_norm:  x r/w (FP64), s, r r/w, q                   -> 16 + 4 + 8 + 4
 #   forward:    6 x 4;  backward: 7 x 4
 #   the direction update is folded into spmv_dot.
-# A true residual (mixed path only) reads x, b and the matrix (FP64) and
-# writes r (FP32): 52 bytes.
+# A true residual (mixed path only) reads x and b (FP64) and the matrix and
+# writes r (FP32): 8 + 8 + 20 + 4 = 40 bytes.
 PCG_BYTES = {
-    "fp64": {"k_spmv_dot": 48, "k_axpy_norm": 48, "k_forward": 48, "k_backward": 56, "k_update_s": 24},
-    "mixed": {"k_spmv_dot": 48, "k_axpy_norm": 32, "k_forward": 24, "k_backward": 28, "k_update_s": 0},
+    "fp64": {"k_spmv_dot": 36, "k_axpy_norm": 48, "k_forward": 48, "k_backward": 56, "k_update_s": 24},
+    "mixed": {"k_spmv_dot": 36, "k_axpy_norm": 32, "k_forward": 24, "k_backward": 28, "k_update_s": 0},
 }
-TRUE_RESIDUAL_BYTES = 52
+TRUE_RESIDUAL_BYTES = 40
 
 # HBM bytes moved per launch, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes scaled by a plain 256 MiB copy (x2.0 and x1.0), on the default

@@ -740,7 +740,8 @@ int flipmi_pressure_solve(flipmi_ctx *ctx, const flipmi_pressure_args *a, flipmi
         const SlotMap &sm = ps.pb.sm;
         if (a->pressure_rows) {
             x.resize(sm.count());
-            download(x.data(), ps.arr[5], sm.count(), s);
+            hipck(hipMemcpyAsync(x.data(), ps.pb.x, sm.count() * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "download");
         }
         download(hl.data(), liq, nc, s);
         hipck(hipStreamSynchronize(s), "sync");
@@ -814,7 +815,7 @@ int flipmi_mic0_apply(flipmi_ctx *ctx, int I, int J, int K, double dx, double dt
         hipck(launch_precond_apply(g, ps.pb, ps.pb.r, ps.pb.z, s), "precond");
         std::vector<double> zd(ns), inv(ns);
         hipck(hipMemcpyAsync(zd.data(), ps.pb.z, ns * sizeof(double), hipMemcpyDeviceToHost, s), "download");
-        download(inv.data(), ps.arr[9], ns, s);
+        hipck(hipMemcpyAsync(inv.data(), ps.pb.invdiag, ns * sizeof(double), hipMemcpyDeviceToHost, s), "download");
         hipck(hipStreamSynchronize(s), "sync");
         PcgScalars hs;
         hipck(hipMemcpy(&hs, ps.pb.sc, sizeof(hs), hipMemcpyDeviceToHost), "scalars");

@@ -128,6 +128,7 @@ struct PcgScalars {
     unsigned int counter[4];
     unsigned int rows;                  // rows of the system (fluid interior cells)
     unsigned long long absmax_bits;
+    double mf;                          // dt / dx^2 of the assembled system (coupling scale)
     // mixed precision: true-residual replacement ("reliable update")
     double rmax;                        // |r| at the last true residual
     double delta;                       // replace when |r32| < delta * rmax
@@ -165,6 +166,14 @@ struct SlotMap {
         size_t grp = tile * (size_t)(S >> 2) + (size_t)(t >> 2);
         return ((grp * 64 + (size_t)(a + 8 * b)) << 2) + (size_t)(t & 3);
     }
+    // slot() in 32-bit arithmetic, for kernels (the store holds at most 2^32
+    // slots: PressureStore::alloc checks).
+    __host__ __device__ unsigned slot32(int i, int j, int k) const {
+        const unsigned a = (unsigned)(j & 7), b = (unsigned)(k & 7), t = (unsigned)i + a + b;
+        const unsigned tile = (unsigned)(j >> 3) + (unsigned)ntj * (unsigned)(k >> 3);
+        const unsigned grp = tile * ((unsigned)S >> 2) + (t >> 2);
+        return ((grp * 64u + a + 8u * b) << 2) + (t & 3u);
+    }
 };
 inline SlotMap make_slot_map(int I, int J, int K) {
     SlotMap m;
@@ -176,7 +185,11 @@ inline SlotMap make_slot_map(int I, int J, int K) {
 
 struct PressureBuffers {
     SlotMap sm;
-    double *diag, *offx, *offy, *offz;  // matrix: diagonal and couplings to the +x/+y/+z rows
+    double *diag;                       // matrix: diagonal
+    // Couplings to the +x/+y/+z rows, stored as the FP32 face weight w each
+    // is formed from: entry = -(double)w * mf (PcgScalars::mf), bit for bit
+    // the FP64 entry; -0.0f where a slot has no such coupling (entry +0.0).
+    float *wx, *wy, *wz;
     double *rhs;
     double *x, *r, *z, *s;              // PCG vectors (FP64)
     double *invdiag, *Lx, *Ly, *Lz;     // MIC(0) factor: column entries (rows +x/+y/+z)

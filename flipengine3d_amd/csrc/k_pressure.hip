@@ -45,18 +45,25 @@ size_t edge_record_count(const SlotMap &sm) {
     return (size_t)sm.ntj * sm.ntk * sm.S * kRecSlots * kRecWords;
 }
 
-// Cell of a slot (may lie outside the grid: skew corners and padding).
+// Cell of a slot (may lie outside the grid: skew corners and padding), in
+// 32-bit arithmetic: two 32-bit divisions instead of 64-bit ones, which cost
+// several times the instructions (slot indices stay below 2^32).
 __device__ __forceinline__ void decode_slot(const SlotMap &sm, size_t p, int &i, int &j, int &k) {
-    int lane = (int)((p >> 2) & 63);
-    size_t grp = p >> 8;
-    const size_t S4 = (size_t)(sm.S >> 2);
-    int t = (int)(grp % S4) * 4 + (int)(p & 3);
-    size_t tile = grp / S4;
-    int a = lane & 7, b = lane >> 3;
-    j = (int)(tile % (size_t)sm.ntj) * kTile + a;
-    k = (int)(tile / (size_t)sm.ntj) * kTile + b;
+    const unsigned q = (unsigned)p;
+    const unsigned grp = q >> 8, S4 = (unsigned)sm.S >> 2, ntj = (unsigned)sm.ntj;
+    const unsigned tile = grp / S4, kt = tile / ntj;
+    const int t = (int)((grp - tile * S4) * 4u + (q & 3u));
+    const int lane = (int)((q >> 2) & 63u), a = lane & 7, b = lane >> 3;
+    j = (int)(tile - kt * ntj) * kTile + a;
+    k = (int)kt * kTile + b;
     i = t - a - b;
 }
+
+// A coupling entry of the matrix from the face weight it is formed from:
+// -weight * dt / dx^2, the product the assembly takes (pressuresolver.cpp:
+// 725, 753-798), so the FP64 entry comes out bit for bit.  A slot without
+// the coupling holds -0.0f, which gives the +0.0 it stands for.
+__device__ __forceinline__ double coupling(float w, double mf) { return -(double)w * mf; }
 
 __device__ __forceinline__ bool interior(const GridGeom &g, int i, int j, int k) {
     return i >= 1 && j >= 1 && k >= 1 && i <= g.I - 2 && j <= g.J - 2 && k <= g.K - 2;
@@ -155,7 +162,12 @@ __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liqu
     if (p >= pb.sm.count()) return;
     int i, j, k;
     decode_slot(pb.sm, p, i, j, k);
-    double rhs = 0, diag = 0, ox = 0, oy = 0, oz = 0;
+    double rhs = 0, diag = 0;
+    // Couplings to the +x/+y/+z rows as the face weight they are formed from
+    // (entry = -weight * mf); -0.0f stands for "no coupling" (entry +0.0).
+    float wx = -0.0f, wy = -0.0f, wz = -0.0f;
+    const double mf = dt / (g.dx * g.dx);
+    if (p == 0) pb.sc->mf = mf;
     if (is_row_cell(g, liquid, i, j, k)) {
         size_t c = (size_t)i + (size_t)g.I * ((size_t)j + (size_t)g.J * k);
         Dims DU{g.I + 1, g.J, g.K}, DV{g.I, g.J + 1, g.K}, DW{g.I, g.J, g.K + 1};
@@ -179,25 +191,25 @@ __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liqu
         div += -factor * (volBack - volCenter) * sW[DW.at(i, j, k)];
         rhs = div;
 
-        double mf = dt / (g.dx * g.dx);
         double eps = 1e-9;
         double pc = liquid[c];
         diag = (volRight + volLeft + volTop + volBottom + volFront + volBack) * mf;
-        auto side = [&](double vol, int a, int b, int d, double *off) {
+        auto side = [&](double vol, int a, int b, int d, float *off) {
             double pn = liquid[(size_t)a + (size_t)g.I * ((size_t)b + (size_t)g.J * d)];
             if (pn < 0.0) {
-                if (off && is_row_cell(g, liquid, a, b, d)) *off = -vol * mf;
+                // vol is a float weight widened: (float)vol is exact.
+                if (off && is_row_cell(g, liquid, a, b, d)) *off = (float)vol;
             } else {
                 double theta = pn / (pc + eps);
                 theta = theta < -kMaxTheta ? -kMaxTheta : (theta > kMaxTheta ? kMaxTheta : theta);
                 diag -= vol * mf * theta;
             }
         };
-        side(volRight, i + 1, j, k, &ox);
+        side(volRight, i + 1, j, k, &wx);
         side(volLeft, i - 1, j, k, nullptr);
-        side(volTop, i, j + 1, k, &oy);
+        side(volTop, i, j + 1, k, &wy);
         side(volBottom, i, j - 1, k, nullptr);
-        side(volFront, i, j, k + 1, &oz);
+        side(volFront, i, j, k + 1, &wz);
         side(volBack, i, j, k - 1, nullptr);
         diag = diag > 0.0 ? diag : 0.0;
         // max|rhs| as an order-independent integer maximum over IEEE bits.
@@ -207,9 +219,9 @@ __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liqu
     }
     pb.rhs[p] = rhs;
     pb.diag[p] = diag;
-    pb.offx[p] = ox;
-    pb.offy[p] = oy;
-    pb.offz[p] = oz;
+    pb.wx[p] = wx;
+    pb.wy[p] = wy;
+    pb.wz[p] = wz;
 }
 
 hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *liquid,
@@ -230,8 +242,8 @@ __global__ __launch_bounds__(256) void k_group_mask(PressureBuffers pb) {
     const size_t S4 = (size_t)(pb.sm.S >> 2);
     for (size_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         const size_t p = (grp << 8) + threadIdx.x;
-        const int nz = (pb.diag[p] != 0.0) | (pb.offx[p] != 0.0) | (pb.offy[p] != 0.0) |
-                       (pb.offz[p] != 0.0) | (pb.rhs[p] != 0.0);
+        const int nz = (pb.diag[p] != 0.0) | (pb.wx[p] != 0.0f) | (pb.wy[p] != 0.0f) |
+                       (pb.wz[p] != 0.0f) | (pb.rhs[p] != 0.0);
         if (__syncthreads_or(nz) && threadIdx.x == 0) {
             const size_t tile = grp / S4;
             const int gi = (int)(grp % S4);
@@ -484,7 +496,7 @@ __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const Gri
 // launch and, for the backward sweep, reduces the dot product.
 __device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, int tile,
                              bool reset_progress, unsigned next_epoch = 0) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int ntiles = pb.sm.ntj * pb.sm.ntk;
     if (dotmode != kDotNone) {
         double part = wave_sum_fixed(dot_acc);
@@ -561,6 +573,7 @@ constexpr int kFactorChunk = 8;
 __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
     constexpr int C = kFactorChunk;
     const int lane = threadIdx.x;
+    const double mf = pb.sc->mf;
     const SlotMap sm = pb.sm;
     const int ntj = sm.ntj;
     int jt, kt;
@@ -599,9 +612,9 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
                 bool live = t0 + u < t1 && live_at(t0 + u);
                 size_t p = slot_at(t0 + u);
                 v[0][u] = live ? pb.diag[p] : 0.0;
-                v[1][u] = live ? pb.offx[p] : 0.0;
-                v[2][u] = live ? pb.offy[p] : 0.0;
-                v[3][u] = live ? pb.offz[p] : 0.0;
+                v[1][u] = live ? coupling(pb.wx[p], mf) : 0.0;
+                v[2][u] = live ? coupling(pb.wy[p], mf) : 0.0;
+                v[3][u] = live ? coupling(pb.wz[p], mf) : 0.0;
             }
             if (up_j >= 0 || up_k >= 0) ok = wait_progress2(pb, up_j, up_k, need);
             if (!ok) break;
@@ -614,12 +627,12 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
 #pragma unroll
                 for (int q = 0; q < 3; q++) { ey[q][u] = 0.0; ez[q][u] = 0.0; }
                 if (live && edge_j) {
-                    size_t pn = sm.slot(i, j - 1, k);
+                    size_t pn = sm.slot32(i, j - 1, k);
                     ey[0][u] = ld_dev(&pb.Lx[pn]); ey[1][u] = ld_dev(&pb.Ly[pn]);
                     ey[2][u] = ld_dev(&pb.Lz[pn]);
                 }
                 if (live && edge_k) {
-                    size_t pn = sm.slot(i, j, k - 1);
+                    size_t pn = sm.slot32(i, j, k - 1);
                     ez[0][u] = ld_dev(&pb.Lx[pn]); ez[1][u] = ld_dev(&pb.Ly[pn]);
                     ez[2][u] = ld_dev(&pb.Lz[pn]);
                 }
@@ -687,9 +700,9 @@ __global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
         decode_slot(pb.sm, p, i, j, k);
         double lx = 0.0, ly = 0.0, lz = 0.0;
         if (interior(g, i, j, k)) {
-            lx = pb.Lx[pb.sm.slot(i - 1, j, k)];
-            ly = pb.Ly[pb.sm.slot(i, j - 1, k)];
-            lz = pb.Lz[pb.sm.slot(i, j, k - 1)];
+            lx = pb.Lx[pb.sm.slot32(i - 1, j, k)];
+            ly = pb.Ly[pb.sm.slot32(i, j - 1, k)];
+            lz = pb.Lz[pb.sm.slot32(i, j, k - 1)];
         }
         pb.Lxm[p] = lx;
         pb.Lym[p] = ly;
@@ -1285,19 +1298,19 @@ __device__ bool last_block(unsigned *counter) {
 
 // y = A x at slot p of cell (i, j, k) (sparsematrix.h:187-216: row entries
 // in increasing column order).
-__device__ __forceinline__ double spmv_slot(const PressureBuffers &pb, const double *__restrict__ x,
+__device__ __forceinline__ double spmv_slot(const PressureBuffers &pb, double mf, const double *__restrict__ x,
                                             size_t p, int i, int j, int k) {
     const SlotMap &sm = pb.sm;
-    const size_t pzm = sm.slot(i, j, k - 1), pym = sm.slot(i, j - 1, k), pxm = sm.slot(i - 1, j, k);
-    const size_t pxp = sm.slot(i + 1, j, k), pyp = sm.slot(i, j + 1, k), pzp = sm.slot(i, j, k + 1);
+    const unsigned pzm = sm.slot32(i, j, k - 1), pym = sm.slot32(i, j - 1, k), pxm = sm.slot32(i - 1, j, k);
+    const unsigned pxp = sm.slot32(i + 1, j, k), pyp = sm.slot32(i, j + 1, k), pzp = sm.slot32(i, j, k + 1);
     double s = 0;
-    s += pb.offz[pzm] * x[pzm];
-    s += pb.offy[pym] * x[pym];
-    s += pb.offx[pxm] * x[pxm];
+    s += coupling(pb.wz[pzm], mf) * x[pzm];
+    s += coupling(pb.wy[pym], mf) * x[pym];
+    s += coupling(pb.wx[pxm], mf) * x[pxm];
     s += pb.diag[p] * x[p];
-    s += pb.offx[p] * x[pxp];
-    s += pb.offy[p] * x[pyp];
-    s += pb.offz[p] * x[pzp];
+    s += coupling(pb.wx[p], mf) * x[pxp];
+    s += coupling(pb.wy[p], mf) * x[pyp];
+    s += coupling(pb.wz[p], mf) * x[pzp];
     return s;
 }
 
@@ -1308,6 +1321,7 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBu
     if (pb.sc->done) return;
     double acc = 0.0;
     const ListRange lr = xcd_list(*pb.active_count);
+    const double mf = pb.sc->mf;  // coupling scale of the assembled system
     constexpr int U = 2;
     for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
         size_t p[U];
@@ -1318,7 +1332,7 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBu
         for (int u = 0; u < U; u++) {
             int i, j, k;
             in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
-            zc[u] = spmv_slot(pb, pb.s, p[u], i, j, k);
+            zc[u] = spmv_slot(pb, mf, pb.s, p[u], i, j, k);
             sp[u] = pb.s[p[u]];
         }
 #pragma unroll
@@ -1624,25 +1638,10 @@ __global__ void k_pcg_init_vectors32(PressureBuffers pb) {
     }
 }
 
-// q = A s for the FP32 iteration: the FP64 matrix (the FP32 copy's rounding
-// costs about 9 % more iterations on the workloads here: 126 against 115),
-// FP32 s, FP64 sums, rounded once.
-__device__ __forceinline__ double spmv_slot_mixed(const PressureBuffers &pb, const float *__restrict__ x,
-                                                  size_t p, int i, int j, int k) {
-    const SlotMap &sm = pb.sm;
-    const size_t pzm = sm.slot(i, j, k - 1), pym = sm.slot(i, j - 1, k), pxm = sm.slot(i - 1, j, k);
-    const size_t pxp = sm.slot(i + 1, j, k), pyp = sm.slot(i, j + 1, k), pzp = sm.slot(i, j, k + 1);
-    double s = 0.0;
-    s += pb.offz[pzm] * (double)x[pzm];
-    s += pb.offy[pym] * (double)x[pym];
-    s += pb.offx[pxm] * (double)x[pxm];
-    s += pb.diag[p] * (double)x[p];
-    s += pb.offx[p] * (double)x[pxp];
-    s += pb.offy[p] * (double)x[pyp];
-    s += pb.offz[p] * (double)x[pzp];
-    return s;
-}
-
+// q = A s for the FP32 iteration: the FP64 matrix entries (an FP32 copy's
+// rounding costs about 9 % more iterations on the workloads here: 126
+// against 115), rebuilt from the FP32 face weights, FP32 s, FP64 sums,
+// rounded once.
 // The new direction s = z + beta s of the previous iteration
 // (pcgsolver.h:300-302), formed where the SpMV reads it.
 struct NextDirection {
@@ -1656,19 +1655,19 @@ struct SameDirection {
 };
 
 template <class Dir>
-__device__ __forceinline__ double spmv_slot_dir(const PressureBuffers &pb, const Dir &x, size_t p, int i,
-                                                int j, int k) {
+__device__ __forceinline__ double spmv_slot_dir(const PressureBuffers &pb, double mf, const Dir &x, size_t p,
+                                                int i, int j, int k) {
     const SlotMap &sm = pb.sm;
-    const size_t pzm = sm.slot(i, j, k - 1), pym = sm.slot(i, j - 1, k), pxm = sm.slot(i - 1, j, k);
-    const size_t pxp = sm.slot(i + 1, j, k), pyp = sm.slot(i, j + 1, k), pzp = sm.slot(i, j, k + 1);
+    const unsigned pzm = sm.slot32(i, j, k - 1), pym = sm.slot32(i, j - 1, k), pxm = sm.slot32(i - 1, j, k);
+    const unsigned pxp = sm.slot32(i + 1, j, k), pyp = sm.slot32(i, j + 1, k), pzp = sm.slot32(i, j, k + 1);
     double s = 0.0;
-    s += pb.offz[pzm] * (double)x[pzm];
-    s += pb.offy[pym] * (double)x[pym];
-    s += pb.offx[pxm] * (double)x[pxm];
+    s += coupling(pb.wz[pzm], mf) * (double)x[pzm];
+    s += coupling(pb.wy[pym], mf) * (double)x[pym];
+    s += coupling(pb.wx[pxm], mf) * (double)x[pxm];
     s += pb.diag[p] * (double)x[p];
-    s += pb.offx[p] * (double)x[pxp];
-    s += pb.offy[p] * (double)x[pyp];
-    s += pb.offz[p] * (double)x[pzp];
+    s += coupling(pb.wx[p], mf) * (double)x[pxp];
+    s += coupling(pb.wy[p], mf) * (double)x[pyp];
+    s += coupling(pb.wz[p], mf) * (double)x[pzp];
     return s;
 }
 
@@ -1680,6 +1679,7 @@ __device__ __forceinline__ double spmv_dot32_body(const GridGeom &g, const Press
                                                   const Dir &dir) {
     double acc = 0.0;
     const ListRange lr = xcd_list(*pb.active_count);
+    const double mf = pb.sc->mf;  // coupling scale of the assembled system
     constexpr int U = 2;
     for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
         size_t p[U];
@@ -1690,7 +1690,7 @@ __device__ __forceinline__ double spmv_dot32_body(const GridGeom &g, const Press
         for (int u = 0; u < U; u++) {
             int i, j, k;
             in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
-            zc[u] = (float)spmv_slot_dir(pb, dir, p[u], i, j, k);
+            zc[u] = (float)spmv_slot_dir(pb, mf, dir, p[u], i, j, k);
             sp[u] = dir[p[u]];
         }
 #pragma unroll
@@ -1782,6 +1782,7 @@ __global__ __launch_bounds__(kRedThreads) void k_true_residual(GridGeom g, Press
     if (pb.sc->done || !pb.sc->refresh) return;
     double mx = 0.0;
     const ListRange lr = xcd_list(*pb.active_count);
+    const double mf = pb.sc->mf;  // coupling scale of the assembled system
     constexpr int U = 2;
     for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
         size_t p[U];
@@ -1792,7 +1793,7 @@ __global__ __launch_bounds__(kRedThreads) void k_true_residual(GridGeom g, Press
         for (int u = 0; u < U; u++) {
             int i, j, k;
             in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
-            ax[u] = spmv_slot(pb, pb.x, p[u], i, j, k);
+            ax[u] = spmv_slot(pb, mf, pb.x, p[u], i, j, k);
             bs[u] = pb.rhs[p[u]];
         }
 #pragma unroll
@@ -1940,11 +1941,12 @@ int pcg_piece_at(int q) { return kPieceOrder[q]; }
 __global__ void k_spmv_plain(GridGeom g, PressureBuffers pb, const double *__restrict__ x,
                              double *y) {
     size_t n = pb.sm.count();
+    const double mf = pb.sc->mf;  // coupling scale of the assembled system
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (size_t)gridDim.x * blockDim.x) {
         int i, j, k;
         decode_slot(pb.sm, p, i, j, k);
-        y[p] = interior(g, i, j, k) ? spmv_slot(pb, x, p, i, j, k) : 0.0;
+        y[p] = interior(g, i, j, k) ? spmv_slot(pb, mf, x, p, i, j, k) : 0.0;
     }
 }
 
@@ -1989,8 +1991,8 @@ __global__ void k_apply(GridGeom g, SlotMap sm, double dt, const float *__restri
         float factor = (float)(dt / g.dx);
         float p1, p2;
         if (fa && fb) {
-            p1 = (float)p[sm.slot(ai, aj, ak)];
-            p2 = (float)p[sm.slot(i, j, k)];
+            p1 = (float)p[sm.slot32(ai, aj, ak)];
+            p2 = (float)p[sm.slot32(i, j, k)];
         } else {
             float phi1 = liquid[cell(ai, aj, ak)], phi2 = liquid[cell(i, j, k)];
             const double tension = 0.0;
@@ -2001,7 +2003,7 @@ __global__ void k_apply(GridGeom g, SlotMap sm, double dt, const float *__restri
                 float tt = (phi1 - phi2) / (phi1 + eps);
                 double ttd = (double)tt;
                 tt = (float)(ttd < -kMaxTheta ? -kMaxTheta : (ttd > kMaxTheta ? kMaxTheta : ttd));
-                p1 = (float)p[sm.slot(ai, aj, ak)];
+                p1 = (float)p[sm.slot32(ai, aj, ak)];
                 p2 = (float)(tt * tension + tp * p1);
             } else {
                 float tp = phi1 / (phi2 + eps);
@@ -2010,7 +2012,7 @@ __global__ void k_apply(GridGeom g, SlotMap sm, double dt, const float *__restri
                 float tt = (phi2 - phi1) / (phi2 + eps);
                 double ttd = (double)tt;
                 tt = (float)(ttd < -kMaxTheta ? -kMaxTheta : (ttd > kMaxTheta ? kMaxTheta : ttd));
-                p2 = (float)p[sm.slot(i, j, k)];
+                p2 = (float)p[sm.slot32(i, j, k)];
                 p1 = (float)(tt * tension + tp * p2);
             }
         }

@@ -113,6 +113,7 @@ size_t PressureStore::device_bytes() const {
     size_t b = edge.bytes() + mask.bytes() + progress.bytes() + ticket.bytes() + irange.bytes() + scal.bytes() +
                alist.bytes();
     for (int q = 0; q <= kSlotArrays; q++) b += arr[q].bytes();
+    for (int q = 0; q < kWeightArrays; q++) b += wgt[q].bytes();
     for (int q = 0; q < kSlotArrays32; q++) b += arr32[q].bytes();
     return b;
 }
@@ -122,10 +123,16 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     SlotMap sm = make_slot_map(g.I, g.J, g.K);
     // Slot arrays carry the zero and sink groups after the last tile.
     size_t ns = sm.alloc_count();
+    // The kernels index slots in 32-bit arithmetic.
+    if (ns > 0xffffffffull)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "pressure grid too large: more than 2^32 slots"};
     int ntj = sm.ntj, ntk = sm.ntk;
     int npart = 2 * 1024 + ntj * ntk;
     for (int q = 0; q < kSlotArrays; q++)
         if (arr[q].alloc(ns) != hipSuccess)
+            throw StatusError{FLIPMI_ERR_OOM, "pressure arrays: out of device memory"};
+    for (int q = 0; q < kWeightArrays; q++)
+        if (wgt[q].alloc(ns) != hipSuccess)
             throw StatusError{FLIPMI_ERR_OOM, "pressure arrays: out of device memory"};
     size_t nrec = edge_record_count(sm);
     if (arr[kSlotArrays].alloc((size_t)npart) != hipSuccess || edge.alloc(nrec) != hipSuccess ||
@@ -139,12 +146,14 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     (void)launch_fill_i32((int *)&scal.p->sweep_epoch, 1, 1, s);
     // Slots outside the grid are never written by the kernels: start at zero.
     for (int q = 0; q < kSlotArrays; q++) (void)launch_fill_f64(arr[q].p, ns, 0.0, s);
+    for (int q = 0; q < kWeightArrays; q++) (void)launch_fill_f32(wgt[q].p, ns, -0.0f, s);  // no coupling
     (void)launch_fill_u8((uint8_t *)edge.p, nrec * sizeof(unsigned long long), 0, s);
     pb.sm = sm;
-    pb.diag = arr[0].p; pb.offx = arr[1].p; pb.offy = arr[2].p; pb.offz = arr[3].p;
-    pb.rhs = arr[4].p; pb.x = arr[5].p; pb.r = arr[6].p; pb.z = arr[7].p; pb.s = arr[8].p;
-    pb.invdiag = arr[9].p; pb.Lx = arr[10].p; pb.Ly = arr[11].p; pb.Lz = arr[12].p;
-    pb.Lxm = arr[13].p; pb.Lym = arr[14].p; pb.Lzm = arr[15].p;
+    pb.diag = arr[0].p;
+    pb.rhs = arr[1].p; pb.x = arr[2].p; pb.r = arr[3].p; pb.z = arr[4].p; pb.s = arr[5].p;
+    pb.invdiag = arr[6].p; pb.Lx = arr[7].p; pb.Ly = arr[8].p; pb.Lz = arr[9].p;
+    pb.Lxm = arr[10].p; pb.Lym = arr[11].p; pb.Lzm = arr[12].p;
+    pb.wx = wgt[0].p; pb.wy = wgt[1].p; pb.wz = wgt[2].p;
     pb.partials = arr[kSlotArrays].p;
     if (mixed) {
         for (int q = 0; q < kSlotArrays32; q++) {
@@ -587,6 +596,7 @@ size_t estimate_scene_bytes(const flipmi_scene_desc &d, int64_t cap) {
     SlotMap sm = make_slot_map(d.isize, d.jsize, d.ksize);
     const size_t ns = sm.alloc_count(), ntiles = (size_t)sm.ntj * sm.ntk;
     b += sizeof(double) * (PressureStore::kSlotArrays * ns + 2 * 1024 + ntiles);
+    b += sizeof(float) * PressureStore::kWeightArrays * ns;
     if (d.pcg_precision == FLIPMI_PCG_MIXED) b += sizeof(float) * PressureStore::kSlotArrays32 * ns;
     b += sizeof(unsigned long long) * edge_record_count(sm);
     b += sizeof(unsigned long long) * ntiles * (size_t)((sm.S / 4 + 63) / 64);

@@ -273,14 +273,16 @@ struct PressureOutcome {
 PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScalars *host_sc,
                              hipStream_t s, Scene *prof);
 
-// Device storage of the pressure system for one grid size.
-// Device storage of one pressure system in slot order: arr[0..15] are the
-// slot arrays (diag, offx, offy, offz, rhs, x, r, z, s, invdiag, Lx, Ly, Lz,
-// Lxm, Lym, Lzm), arr[16] the reduction partials.
+// Device storage of one pressure system in slot order: arr[0..12] are the
+// FP64 slot arrays (diag, rhs, x, r, z, s, invdiag, Lx, Ly, Lz, Lxm, Lym,
+// Lzm), arr[13] the reduction partials, wgt[0..2] the coupling weights
+// (wx, wy, wz).
 struct PressureStore {
-    static constexpr int kSlotArrays = 16;
+    static constexpr int kSlotArrays = 13;
+    static constexpr int kWeightArrays = 3;
     static constexpr int kSlotArrays32 = 12;  // FP32 factor and vectors, s twice, q (mixed precision)
     DBuf<double> arr[kSlotArrays + 1];
+    DBuf<float> wgt[kWeightArrays];
     DBuf<float> arr32[kSlotArrays32];
     DBuf<unsigned long long> edge, mask;
     DBuf<unsigned> progress, ticket;

@@ -19,14 +19,14 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 def test_pcg_byte_model_matches_the_kernels():
     sys.path.insert(0, ROOT)
     import bench
-    # FP64 path: SpMV 48 B per row (s, z, FP64 diag + 3 upper couplings),
-    # axpy 48, forward sweep 48, backward sweep 56 (it also reads r for the
-    # fused z.r), direction update 24: 224 B per row per iteration.  Mixed:
-    # FP32 vectors and factor, FP64 x and matrix, the direction update in
-    # the SpMV: 48 + 32 + 24 + 28 = 132 B.
-    assert sum(bench.PCG_BYTES["fp64"].values()) == 224
-    assert sum(bench.PCG_BYTES["mixed"].values()) == 132
-    assert bench.TRUE_RESIDUAL_BYTES == 8 + 8 + 32 + 4
+    # FP64 path: SpMV 36 B per row (s, z, FP64 diag + 3 upper couplings
+    # stored as FP32 face weights), axpy 48, forward sweep 48, backward sweep
+    # 56 (it also reads r for the fused z.r), direction update 24: 212 B per
+    # row per iteration.  Mixed: FP32 vectors and factor, FP64 x, the same
+    # matrix, the direction update in the SpMV: 36 + 32 + 24 + 28 = 120 B.
+    assert sum(bench.PCG_BYTES["fp64"].values()) == 212
+    assert sum(bench.PCG_BYTES["mixed"].values()) == 120
+    assert bench.TRUE_RESIDUAL_BYTES == 8 + 8 + 8 + 3 * 4 + 4
     args = bench.parse_args([])
     assert args.gpus == 1 and args.size == [256, 128, 128]
     assert json.dumps({"x": None})  # the line is plain JSON
