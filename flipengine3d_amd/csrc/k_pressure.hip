@@ -606,15 +606,18 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
             const int t1 = t0 + C < te ? t0 + C : te;
             unsigned need = (unsigned)t1 + 7;
             if (need > total) need = total;
-            double v[4][C];
+            // The chunk's operands, loaded before the wait; the couplings
+            // stay face weights until used, so nothing here waits on them.
+            double v0[C];
+            float vw[3][C];
 #pragma unroll
             for (int u = 0; u < C; u++) {
                 bool live = t0 + u < t1 && live_at(t0 + u);
                 size_t p = slot_at(t0 + u);
-                v[0][u] = live ? pb.diag[p] : 0.0;
-                v[1][u] = live ? coupling(pb.wx[p], mf) : 0.0;
-                v[2][u] = live ? coupling(pb.wy[p], mf) : 0.0;
-                v[3][u] = live ? coupling(pb.wz[p], mf) : 0.0;
+                v0[u] = live ? pb.diag[p] : 0.0;
+                vw[0][u] = live ? pb.wx[p] : -0.0f;
+                vw[1][u] = live ? pb.wy[p] : -0.0f;
+                vw[2][u] = live ? pb.wz[p] : -0.0f;
             }
             if (up_j >= 0 || up_k >= 0) ok = wait_progress2(pb, up_j, up_k, need);
             if (!ok) break;
@@ -653,7 +656,7 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
                 double zLx = edge_k ? ez[0][u] : nk_Lx, zLy = edge_k ? ez[1][u] : nk_Ly;
                 double zLz = edge_k ? ez[2][u] : nk_Lz;
                 const double tau = 0.97, sigma = 0.25;  // pcgsolver.h:72-73
-                double A = v[0][u];
+                double A = v0[u];
                 double d = A;
                 double L, missing;
                 // Columns in increasing index order: -z, -y, -x neighbour.
@@ -664,7 +667,7 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
                 L = curLx; missing = 0; missing += curLy; missing += curLz;
                 d -= L * L; d -= tau * L * missing;
                 double inv;
-                double fx = v[1][u], fy = v[2][u], fz = v[3][u];
+                double fx = coupling(vw[0][u], mf), fy = coupling(vw[1][u], mf), fz = coupling(vw[2][u], mf);
                 if (A == 0) {
                     inv = d;
                 } else {
