@@ -889,6 +889,46 @@ __device__ __forceinline__ void store_group(__amdgpu_buffer_rsrc_t r, unsigned v
     __builtin_amdgcn_raw_buffer_store_b128(v1, r, vo + 16u, so, 0);
 }
 
+// Active-group bits of a tile (up to 256 groups; longer tiles treat every
+// group as active).  Four scalar words, selected with uniform conditionals
+// (an array here would be indexed through scratch memory).
+struct GroupBits {
+    unsigned long long w0, w1, w2, w3;
+    // Without a usable mask all four words are all-ones, so the bit test
+    // alone decides (no second condition for the optimiser to thread).
+    __device__ __forceinline__ bool test(int gi) const {
+        const int w = gi >> 6;
+        const unsigned long long lo = (w & 1) ? w1 : w0, hi = (w & 1) ? w3 : w2;
+        const unsigned long long word = (w & 2) ? hi : lo;
+        return (word >> (gi & 63)) & 1ull;
+    }
+};
+__device__ __forceinline__ GroupBits tile_group_bits(const PressureBuffers &pb, int tile) {
+    GroupBits gb{~0ull, ~0ull, ~0ull, ~0ull};
+    if (pb.group_mask && pb.mask_words <= 4) {
+        const unsigned long long *mw = pb.group_mask + (size_t)tile * pb.mask_words;
+        auto rd = [&](int w) -> unsigned long long {
+            if (w >= pb.mask_words) return ~0ull;
+            const unsigned long long v = mw[w];
+            // readfirstlane yields an int: widen through unsigned (no sign extension).
+            const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffull));
+            const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+            return (unsigned long long)lo | ((unsigned long long)hi << 32);
+        };
+        gb.w0 = rd(0); gb.w1 = rd(1); gb.w2 = rd(2); gb.w3 = rd(3);
+    }
+    return gb;
+}
+// Whether the chunk at march step u0 (u0 % 8 == 0) holds a row of the tile:
+// its two slot groups (forward: steps u0 .. u0 + 7; backward: t = S - 1 - u).
+template <bool FWD>
+__device__ __forceinline__ int chunk_group(int S, int u0) { return FWD ? (u0 >> 2) : ((S - 8 - u0) >> 2); }
+template <bool FWD>
+__device__ __forceinline__ bool chunk_has_rows(const GroupBits &gb, int S, int u0) {
+    const int gl = chunk_group<FWD>(S, u0);
+    return gb.test(gl) || gb.test(gl + 1);
+}
+
 // The poller wave of a sweep workgroup: for each chunk of the tile's march,
 // polls the upstream records until every needed one carries this launch's
 // epoch, then delivers the values to the mailbox.  Records outside an
@@ -896,7 +936,7 @@ __device__ __forceinline__ void store_group(__amdgpu_buffer_rsrc_t r, unsigned v
 // timeout it raises the abort flag and releases the compute wave.
 template <class T, bool FWD>
 __device__ void sweep_poller(PressureBuffers &pb, SweepMailbox<T> &mb, int tile, int up_j, int up_k, int tb,
-                             int nchunks, int ubj, int uej, int ubk, int uek, int lane) {
+                             int nchunks, int ubj, int uej, int ubk, int uek, int lane, const GroupBits &gb) {
     constexpr int C = kChunk, RW = Record<T>::W;
     const int S = pb.sm.S;
     const unsigned epoch = mb.epoch;
@@ -928,8 +968,11 @@ __device__ void sweep_poller(PressureBuffers &pb, SweepMailbox<T> &mb, int tile,
             __builtin_amdgcn_s_sleep(1);
         }
         const int uu = tb + ci * C + 7 + rs;
-        const bool nj = (up_j >= 0) & (uu >= ubj) & (uu < uej);
-        const bool nk = (up_k >= 0) & (uu >= ubk) & (uu < uek);
+        // A chunk without rows computes zeros whatever its upstream values:
+        // it is delivered at once (its compute wave does not read it).
+        const bool rows = chunk_has_rows<FWD>(gb, S, tb + ci * C);
+        const bool nj = rows & (up_j >= 0) & (uu >= ubj) & (uu < uej);
+        const bool nk = rows & (up_k >= 0) & (uu >= ubk) & (uu < uek);
         T vj = T(0), vk = T(0);
         if (__ballot(nj | nk) != 0ull) {
             // The records of the chunk in one round trip per poll.  Every
@@ -989,7 +1032,9 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
     if (up_j >= 0) tile_urange(pb, g, up_j, !FWD, ubj, uej);
     if (up_k >= 0) tile_urange(pb, g, up_k, !FWD, ubk, uek);
     const int nchunks = te > tb ? (te - tb) / C : 0;
-    if (wave == 1) return sweep_poller<T, FWD>(pb, mb, tile, up_j, up_k, tb, nchunks, ubj, uej, ubk, uek, lane);
+    const GroupBits gb = tile_group_bits(pb, tile);
+    if (wave == 1)
+        return sweep_poller<T, FWD>(pb, mb, tile, up_j, up_k, tb, nchunks, ubj, uej, ubk, uek, lane, gb);
 
     const int src_j = (FWD ? lane - 1 : lane + 1) & 63;  // neighbour lane along j
     const int src_k = (FWD ? lane - 8 : lane + 8) & 63;  // neighbour lane along k
@@ -1012,31 +1057,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
     const T *dotsrc = dot ? op.dotvec : op.out;
 
 
-    // Active-group bits of this tile (up to 256 groups; longer tiles treat
-    // every group as active).  Four scalar words, selected with uniform
-    // conditionals (an array here would be indexed through scratch memory).
-    unsigned long long gm0 = ~0ull, gm1 = ~0ull, gm2 = ~0ull, gm3 = ~0ull;
-    const bool use_mask = pb.group_mask && pb.mask_words <= 4;
-    if (use_mask) {
-        const unsigned long long *mw = pb.group_mask + (size_t)tile * pb.mask_words;
-        auto rd = [&](int w) -> unsigned long long {
-            if (w >= pb.mask_words) return ~0ull;
-            const unsigned long long v = mw[w];
-            // readfirstlane yields an int: widen through unsigned (no sign extension).
-            const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffull));
-            const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-            return (unsigned long long)lo | ((unsigned long long)hi << 32);
-        };
-        gm0 = rd(0); gm1 = rd(1); gm2 = rd(2); gm3 = rd(3);
-    }
-    // Without a usable mask all four words stay all-ones, so the bit test
-    // alone decides (no second condition for the optimiser to thread).
-    auto active = [&](int gi) -> bool {
-        const int w = gi >> 6;
-        const unsigned long long lo = (w & 1) ? gm1 : gm0, hi = (w & 1) ? gm3 : gm2;
-        const unsigned long long word = (w & 2) ? hi : lo;
-        return (word >> (gi & 63)) & 1ull;
-    };
+    auto active = [&](int gi) -> bool { return gb.test(gi); };
     // Buffer views of the tile's slots in each array (group gi of the tile
     // at byte gi * 256 * sizeof(T), this lane's four slots at lane * 4 *
     // sizeof(T) within it).  A group without rows is read and written
@@ -1061,7 +1082,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
     double dot_acc = 0.0;
     if (te > tb) {
         // Lower slot group of the chunk at march step u0 (u0 % 8 == 0).
-        auto chunk_group = [&](int u0) -> int { return FWD ? (u0 >> 2) : ((S - 8 - u0) >> 2); };
+        auto chunk_group = [&](int u0) -> int { return flipmi::chunk_group<FWD>(S, u0); };
         // Step s of a chunk is element m of group h: forward s = 4h + m,
         // backward s = 7 - (4h + m).
         // act[h] keeps each group's mask bit for the store of the same chunk.
@@ -1131,59 +1152,71 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                 // ring (an exit here costs the compiler its load accounting).
                 const int uc = u0 + d * C;
                 const int ci = (uc - tb) / C;
-                const T *const mj = mb.val[ci % kMailChunks][0][b];
-                const T *const mk = mb.val[ci % kMailChunks][1][a];
-                // The chunk's upstream values: an LDS poll while the poller
-                // has not delivered them.
-                wait_mail(ci + 1);
-                T evj[C], evk[C];
-                read_mail(mj, 0, evj);
-                read_mail(mk, 0, evk);
-                read_mail(mj, 1, evj);
-                read_mail(mk, 1, evk);
-                // The release waits for the reads above, so the poller may
-                // then reuse the ring slot.
-                if (lane == 0) lds_release(&mb.consumed, ci + 1);
-                const T(&v)[NV][C] = buf[d];
                 unsigned long long *const recp =
                     pb.edge + ((((size_t)tile * S + (size_t)uc) * kRecSlots + rec_slot) * RW);
-                T qv[C];
+                if (!chunk_has_rows<FWD>(gb, S, uc)) {
+                    // No rows in the chunk: each of its values is zero whatever
+                    // the upstream values (every slot's invdiag is zero), so it
+                    // publishes zero records without waiting for its mail
+                    // (which its poller does not fetch) or running its steps.
+                    // Its outputs are zero already.
+                    if (lane == 0) lds_release(&mb.consumed, ci + 1);
 #pragma unroll
-                for (int s = 0; s < C; s++) {
-                    // Values produced one step earlier by the neighbouring lanes.
-                    // FP32: DPP / permlane exchange; FP64: the LDS crossbar
-                    // (two dwords per value make the permlane route longer).
-                    T nj_cur, nk_cur;
-                    if constexpr (sizeof(T) == 4) {
-                        nj_cur = lane_j<FWD>(cur);
-                        nk_cur = lane_k<FWD>(cur, low_half, row_sel);
-                    } else {
-                        nj_cur = shfl_t(cur, src_j);
-                        nk_cur = shfl_t(cur, src_k);
+                    for (int s = 0; s < C; s++) Record<T>::put(recp + (size_t)s * kRecSlots * RW, T(0), epoch);
+                    cur = T(0);
+                } else {
+                    const T *const mj = mb.val[ci % kMailChunks][0][b];
+                    const T *const mk = mb.val[ci % kMailChunks][1][a];
+                    // The chunk's upstream values: an LDS poll while the poller
+                    // has not delivered them.
+                    wait_mail(ci + 1);
+                    T evj[C], evk[C];
+                    read_mail(mj, 0, evj);
+                    read_mail(mk, 0, evk);
+                    read_mail(mj, 1, evj);
+                    read_mail(mk, 1, evk);
+                    // The release waits for the reads above, so the poller may
+                    // then reuse the ring slot.
+                    if (lane == 0) lds_release(&mb.consumed, ci + 1);
+                    const T(&v)[NV][C] = buf[d];
+                    T qv[C];
+    #pragma unroll
+                    for (int s = 0; s < C; s++) {
+                        // Values produced one step earlier by the neighbouring lanes.
+                        // FP32: DPP / permlane exchange; FP64: the LDS crossbar
+                        // (two dwords per value make the permlane route longer).
+                        T nj_cur, nk_cur;
+                        if constexpr (sizeof(T) == 4) {
+                            nj_cur = lane_j<FWD>(cur);
+                            nk_cur = lane_k<FWD>(cur, low_half, row_sel);
+                        } else {
+                            nj_cur = shfl_t(cur, src_j);
+                            nk_cur = shfl_t(cur, src_k);
+                        }
+                        const T qy = edge_j ? evj[s] : nj_cur;
+                        const T qz = edge_k ? evk[s] : nk_cur;
+                        // c1..c3 multiply the -z, -y, -x neighbours (forward) or
+                        // the +x, +y, +z ones (backward).
+                        T q;
+                        if constexpr (sizeof(T) == 8) {
+                            q = FWD ? ref_step(v[0][s], v[1][s], qz, v[2][s], qy, v[3][s], cur, v[4][s])
+                                    : ref_step(v[0][s], v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
+                        } else {
+                            q = FWD ? fused_step(v[0][s], v[3][s], cur, v[2][s], qy, v[1][s], qz, v[4][s])
+                                    : fused_step(v[0][s], v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
+                        }
+                        if (!FWD) dot_acc = __builtin_fma((double)q, (double)v[NV - 1][s], dot_acc);
+                        qv[s] = q;
+                        Record<T>::put(recp + (size_t)s * kRecSlots * RW, q, epoch);
+                        // Keep each step's record where it is computed: no
+                        // memory operation is scheduled across this point (ALU
+                        // and LDS instructions may be).
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        __builtin_amdgcn_sched_barrier(0);
+                        cur = q;
                     }
-                    const T qy = edge_j ? evj[s] : nj_cur;
-                    const T qz = edge_k ? evk[s] : nk_cur;
-                    // c1..c3 multiply the -z, -y, -x neighbours (forward) or
-                    // the +x, +y, +z ones (backward).
-                    T q;
-                    if constexpr (sizeof(T) == 8) {
-                        q = FWD ? ref_step(v[0][s], v[1][s], qz, v[2][s], qy, v[3][s], cur, v[4][s])
-                                : ref_step(v[0][s], v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
-                    } else {
-                        q = FWD ? fused_step(v[0][s], v[3][s], cur, v[2][s], qy, v[1][s], qz, v[4][s])
-                                : fused_step(v[0][s], v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
-                    }
-                    if (!FWD) dot_acc = __builtin_fma((double)q, (double)v[NV - 1][s], dot_acc);
-                    qv[s] = q;
-                    Record<T>::put(recp + (size_t)s * kRecSlots * RW, q, epoch);
-                    // Keep each step's record where it is computed: no
-                    // memory operation is scheduled across this point (ALU
-                    // and LDS instructions may be).
-                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                    __builtin_amdgcn_sched_barrier(0);
-                    cur = q;
+                    store_chunk(uc, qv, actr[d]);
                 }
-                store_chunk(uc, qv, actr[d]);
                 const int uf = uc + D * C;
                 load_chunk(uf < last ? uf : last, buf[d], actr[d]);
                 __builtin_amdgcn_sched_barrier(0);
