@@ -656,6 +656,7 @@ int flipmi_extrapolate(flipmi_ctx *ctx, int ni, int nj, int nk, float *grid, con
                        int layers) {
     if (!ctx || !grid || !valid || ni < 1 || nj < 1 || nk < 1 || layers < 0 || layers > 250)
         return FLIPMI_ERR_INVALID_ARG;
+    if ((double)ni * nj * nk >= 4294967296.0) return FLIPMI_ERR_INVALID_ARG;  // 32-bit kernel indexing
     return guarded(&ctx->c, [&] {
         hipStream_t s = ctx->c.stream;
         size_t n = (size_t)ni * nj * nk;

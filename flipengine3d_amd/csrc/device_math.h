@@ -35,6 +35,18 @@ struct I3 {
     int i, j, k;
 };
 
+// (i, j, k) of flat index q of an ni x nj x nk array, i fastest, in 32-bit
+// arithmetic: two 32-bit divisions instead of three 64-bit ones, which cost
+// several times the instructions.  Grids hold fewer than 2^32 elements
+// (make_geom checks).
+__device__ __forceinline__ void unflatten3(size_t q, int ni, int nj, int &i, int &j, int &k) {
+    const unsigned u = (unsigned)q, a = (unsigned)ni, b = (unsigned)nj;
+    const unsigned r = u / a, kk = r / b;
+    i = (int)(u - r * a);
+    j = (int)(r - kk * b);
+    k = (int)kk;
+}
+
 // Grid3d::positionToGridIndex: floor(x * (1/dx)) in double.
 __device__ __forceinline__ I3 pos_to_index(double x, double y, double z, double invdx) {
     return I3{(int)floor(x * invdx), (int)floor(y * invdx), (int)floor(z * invdx)};

@@ -37,7 +37,8 @@ __global__ void k_extrap_init(int ni, int nj, int nk, const uint8_t *__restrict_
                               uint8_t *status) {
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (size_t)ni * nj * nk) return;
-    int i = (int)(tid % ni), j = (int)((tid / ni) % nj), k = (int)(tid / ((size_t)ni * nj));
+    int i, j, k;
+    unflatten3(tid, ni, nj, i, j, k);
     uint8_t st;
     if (i == 0 || j == 0 || k == 0 || i == ni - 1 || j == nj - 1 || k == nk - 1) st = kBorder;
     else st = valid[tid] ? 0 : kUnknown;
@@ -115,7 +116,8 @@ __global__ void k_save_force_inflow(GridGeom g, int comp, size_t n, float *mac, 
     if (!do_add) return;
     if (valid[f]) {
         const int ni = g.I + (comp == 0), nj = g.J + (comp == 1);
-        int i = (int)(f % ni), j = (int)((f / ni) % nj), k = (int)(f / ((size_t)ni * nj));
+        int i, j, k;
+        unflatten3(f, ni, nj, i, j, k);
         F3 p = f3(comp == 0 ? (float)((double)(float)i * g.dx) : (float)(((double)(float)i + 0.5) * g.dx),
                   comp == 1 ? (float)((double)(float)j * g.dx) : (float)(((double)(float)j + 0.5) * g.dx),
                   comp == 2 ? (float)((double)(float)k * g.dx) : (float)(((double)(float)k + 0.5) * g.dx));

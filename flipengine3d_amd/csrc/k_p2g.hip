@@ -88,9 +88,7 @@ __device__ __forceinline__ void face_setup(const P2GArgs &a, int comp, size_t f,
     face_dims(g, comp, ni, nj, nk);
     c.comp = comp;
     c.f = f;
-    c.i = (int)(f % (size_t)ni);
-    c.j = (int)((f / (size_t)ni) % (size_t)nj);
-    c.k = (int)(f / ((size_t)ni * nj));
+    unflatten3(f, ni, nj, c.i, c.j, c.k);
     float hdx = (float)(0.5 * g.dx);  // velocityadvector.cpp:153-164
     c.off = comp == 0 ? f3(0.0f, hdx, hdx) : comp == 1 ? f3(hdx, 0.0f, hdx) : f3(hdx, hdx, 0.0f);
     float eps = 1e-6f;

@@ -1567,8 +1567,8 @@ __global__ void k_pocket_union(GridGeom g, const float *__restrict__ liquid, con
     Dims DU{g.I + 1, g.J, g.K}, DV{g.I, g.J + 1, g.K}, DW{g.I, g.J, g.K + 1};
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
         if (!(liquid[c] < 0.0f)) continue;
-        const int i = (int)(c % (size_t)g.I), j = (int)((c / (size_t)g.I) % (size_t)g.J);
-        const int k = (int)(c / ((size_t)g.I * g.J));
+        int i, j, k;
+        unflatten3(c, g.I, g.J, i, j, k);
         // Each face once, from the cell on its low side.
         if (i + 1 < g.I && liquid[c + 1] < 0.0f && wU[DU.at(i + 1, j, k)] >= kOpenFace)
             uf_unite(parent, (int)c, (int)(c + 1));
@@ -1588,8 +1588,8 @@ __global__ void k_pocket_flags(GridGeom g, const float *__restrict__ liquid, con
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
         if (ld_dev(parent + c) < 0) continue;
         const int r = uf_find(parent, (int)c);
-        const int i = (int)(c % (size_t)g.I), j = (int)((c / (size_t)g.I) % (size_t)g.J);
-        const int k = (int)(c / ((size_t)g.I * g.J));
+        int i, j, k;
+        unflatten3(c, g.I, g.J, i, j, k);
         unsigned f = (r != (int)c) ? kPocketMulti : 0u;
         if (interior(g, i, j, k)) {
             f |= kPocketInterior;
@@ -1615,8 +1615,8 @@ __global__ void k_pocket_apply(GridGeom g, int *parent, const unsigned *__restri
         if (ld_dev(parent + c) < 0) continue;
         const unsigned f = flags[uf_find(parent, (int)c)];
         if (!(f & kPocketInterior) || !(f & kPocketMulti) || (f & kPocketAir)) continue;
-        const int i = (int)(c % (size_t)g.I), j = (int)((c / (size_t)g.I) % (size_t)g.J);
-        const int k = (int)(c / ((size_t)g.I * g.J));
+        int i, j, k;
+        unflatten3(c, g.I, g.J, i, j, k);
         sU[DU.at(i, j, k)] = 0.0f;
         sU[DU.at(i + 1, j, k)] = 0.0f;
         sV[DV.at(i, j, k)] = 0.0f;
@@ -2008,7 +2008,8 @@ __global__ void k_apply(GridGeom g, SlotMap sm, double dt, const float *__restri
     const int ni = g.I + (COMP == 0), nj = g.J + (COMP == 1), nk = g.K + (COMP == 2);
     size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= (size_t)ni * nj * nk) return;
-    int i = (int)(f % ni), j = (int)((f / ni) % nj), k = (int)(f / ((size_t)ni * nj));
+    int i, j, k;
+    unflatten3(f, ni, nj, i, j, k);
     int idx = COMP == 0 ? i : (COMP == 1 ? j : k);
     int lim = COMP == 0 ? g.I : (COMP == 1 ? g.J : g.K);
     uint8_t vflag = 0;

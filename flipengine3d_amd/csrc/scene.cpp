@@ -609,6 +609,9 @@ GridGeom make_geom(int I, int J, int K, double dx) {
     if (I < 1 || J < 1 || K < 1 || !(dx > 0.0))
         throw StatusError{FLIPMI_ERR_INVALID_ARG, "dimensions and cell size must be greater than 0"};
     if ((double)I * J * K > 1.5e9) throw StatusError{FLIPMI_ERR_INVALID_ARG, "grid too large"};
+    // Kernels index grids (nodes included) in 32-bit arithmetic.
+    if ((double)(I + 1) * (J + 1) * (K + 1) >= 4294967296.0)
+        throw StatusError{FLIPMI_ERR_INVALID_ARG, "grid too large"};
     GridGeom g;
     g.I = I; g.J = J; g.K = K; g.dx = dx; g.invdx = 1.0 / dx;
     return g;
