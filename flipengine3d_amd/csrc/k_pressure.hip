@@ -568,6 +568,46 @@ __global__ __launch_bounds__(64) void k_tile_irange(GridGeom g, const float *__r
     }
 }
 
+// Active-group bits of a tile (up to 256 groups; longer tiles treat every
+// group as active).  Four scalar words, selected with uniform conditionals
+// (an array here would be indexed through scratch memory).
+struct GroupBits {
+    unsigned long long w0, w1, w2, w3;
+    // Without a usable mask all four words are all-ones, so the bit test
+    // alone decides (no second condition for the optimiser to thread).
+    __device__ __forceinline__ bool test(int gi) const {
+        const int w = gi >> 6;
+        const unsigned long long lo = (w & 1) ? w1 : w0, hi = (w & 1) ? w3 : w2;
+        const unsigned long long word = (w & 2) ? hi : lo;
+        return (word >> (gi & 63)) & 1ull;
+    }
+};
+__device__ __forceinline__ GroupBits tile_group_bits(const PressureBuffers &pb, int tile) {
+    GroupBits gb{~0ull, ~0ull, ~0ull, ~0ull};
+    if (pb.group_mask && pb.mask_words <= 4) {
+        const unsigned long long *mw = pb.group_mask + (size_t)tile * pb.mask_words;
+        auto rd = [&](int w) -> unsigned long long {
+            if (w >= pb.mask_words) return ~0ull;
+            const unsigned long long v = mw[w];
+            // readfirstlane yields an int: widen through unsigned (no sign extension).
+            const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffull));
+            const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+            return (unsigned long long)lo | ((unsigned long long)hi << 32);
+        };
+        gb.w0 = rd(0); gb.w1 = rd(1); gb.w2 = rd(2); gb.w3 = rd(3);
+    }
+    return gb;
+}
+// Whether the chunk at march step u0 (u0 % 8 == 0) holds a row of the tile:
+// its two slot groups (forward: steps u0 .. u0 + 7; backward: t = S - 1 - u).
+template <bool FWD>
+__device__ __forceinline__ int chunk_group(int S, int u0) { return FWD ? (u0 >> 2) : ((S - 8 - u0) >> 2); }
+template <bool FWD>
+__device__ __forceinline__ bool chunk_has_rows(const GroupBits &gb, int S, int u0) {
+    const int gl = chunk_group<FWD>(S, u0);
+    return gb.test(gl) || gb.test(gl + 1);
+}
+
 // ---------------------------------------------------------------------------
 // MIC(0) factorisation (factorModifiedIncompleteColesky0, pcgsolver.h:69-185)
 // ---------------------------------------------------------------------------
@@ -596,6 +636,7 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
     const size_t grp0 = (size_t)tile * (sm.S >> 2);
     int tb, te;
     tile_urange(pb, g, tile, false, tb, te);
+    const GroupBits gb = tile_group_bits(pb, tile);
 
     bool ok = true;
     if (te > tb) {
@@ -610,6 +651,15 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
         double curLx = 0.0, curLy = 0.0, curLz = 0.0;
         for (int t0 = tb; t0 < te; t0 += C) {
             const int t1 = t0 + C < te ? t0 + C : te;
+            if (C == 8 && !chunk_has_rows<true>(gb, sm.S, t0)) {
+                // No rows in the chunk: its entries are the zeros the arrays
+                // hold on entry, whatever its upstream neighbours, so it
+                // neither waits nor computes.  (The previous chunk's stores
+                // were drained before its progress was published.)
+                curLx = curLy = curLz = 0.0;
+                if (lane == 0) st_dev(&pb.progress[tile], (unsigned)t1);
+                continue;
+            }
             unsigned need = (unsigned)t1 + 7;
             if (need > total) need = total;
             // The chunk's operands, loaded before the wait; the couplings
@@ -887,46 +937,6 @@ __device__ __forceinline__ void store_group(__amdgpu_buffer_rsrc_t r, unsigned v
     v1.z = (unsigned)__double2loint(e[3]); v1.w = (unsigned)__double2hiint(e[3]);
     __builtin_amdgcn_raw_buffer_store_b128(v0, r, vo, so, 0);
     __builtin_amdgcn_raw_buffer_store_b128(v1, r, vo + 16u, so, 0);
-}
-
-// Active-group bits of a tile (up to 256 groups; longer tiles treat every
-// group as active).  Four scalar words, selected with uniform conditionals
-// (an array here would be indexed through scratch memory).
-struct GroupBits {
-    unsigned long long w0, w1, w2, w3;
-    // Without a usable mask all four words are all-ones, so the bit test
-    // alone decides (no second condition for the optimiser to thread).
-    __device__ __forceinline__ bool test(int gi) const {
-        const int w = gi >> 6;
-        const unsigned long long lo = (w & 1) ? w1 : w0, hi = (w & 1) ? w3 : w2;
-        const unsigned long long word = (w & 2) ? hi : lo;
-        return (word >> (gi & 63)) & 1ull;
-    }
-};
-__device__ __forceinline__ GroupBits tile_group_bits(const PressureBuffers &pb, int tile) {
-    GroupBits gb{~0ull, ~0ull, ~0ull, ~0ull};
-    if (pb.group_mask && pb.mask_words <= 4) {
-        const unsigned long long *mw = pb.group_mask + (size_t)tile * pb.mask_words;
-        auto rd = [&](int w) -> unsigned long long {
-            if (w >= pb.mask_words) return ~0ull;
-            const unsigned long long v = mw[w];
-            // readfirstlane yields an int: widen through unsigned (no sign extension).
-            const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v & 0xffffffffull));
-            const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-            return (unsigned long long)lo | ((unsigned long long)hi << 32);
-        };
-        gb.w0 = rd(0); gb.w1 = rd(1); gb.w2 = rd(2); gb.w3 = rd(3);
-    }
-    return gb;
-}
-// Whether the chunk at march step u0 (u0 % 8 == 0) holds a row of the tile:
-// its two slot groups (forward: steps u0 .. u0 + 7; backward: t = S - 1 - u).
-template <bool FWD>
-__device__ __forceinline__ int chunk_group(int S, int u0) { return FWD ? (u0 >> 2) : ((S - 8 - u0) >> 2); }
-template <bool FWD>
-__device__ __forceinline__ bool chunk_has_rows(const GroupBits &gb, int S, int u0) {
-    const int gl = chunk_group<FWD>(S, u0);
-    return gb.test(gl) || gb.test(gl + 1);
 }
 
 // The poller wave of a sweep workgroup: for each chunk of the tile's march,
