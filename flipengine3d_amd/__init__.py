@@ -120,7 +120,8 @@ EXPORTED = [
     "flipmi_scene_get_inflow_source", "flipmi_scene_add_obstacle", "flipmi_scene_set_obstacle",
     "flipmi_scene_set_outflow_source", "flipmi_scene_set_obstacle_inversed",
     "flipmi_scene_get_grid", "flipmi_scene_stage_times",
-    "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_domain_solid", "flipmi_liquid_sdf",
+    "flipmi_scene_set_profiling", "flipmi_scene_set_kernel_timing", "flipmi_scene_set_kernel_sampling",
+    "flipmi_domain_solid", "flipmi_liquid_sdf",
     "flipmi_liquid_sdf_postprocess", "flipmi_p2g", "flipmi_removal_counts",
     "flipmi_extrapolate", "flipmi_pressure_solve", "flipmi_mic0_apply", "flipmi_advect_particles",
     "flipmi_trilinear_velocity", "flipmi_server_desc_default", "flipmi_server_create", "flipmi_server_submit",
@@ -169,6 +170,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.flipmi_scene_stage_times.argtypes = [_vp, P(StageTimes), _c_int]
     L.flipmi_scene_set_profiling.argtypes = [_vp, _c_int]
     L.flipmi_scene_set_kernel_timing.argtypes = [_vp, ctypes.c_uint]
+    L.flipmi_scene_set_kernel_sampling.argtypes = [_vp, _c_int]
     L.flipmi_scene_set_launch_graphs.argtypes = [_vp, _c_int]
     L.flipmi_domain_solid.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double] + [_vp] * 5
     L.flipmi_liquid_sdf.argtypes = [_vp, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_double, _vp, _vp]
@@ -490,9 +492,10 @@ class FluidSimulation:
         self._ensure()
         self.ctx.check(self.L.flipmi_scene_set_launch_graphs(self.h, int(bool(enabled))))
 
-    def set_kernel_timing(self, kernels=None):
+    def set_kernel_timing(self, kernels=None, every: int = 1):
         """PCG kernels timed at profiling level 2 (names from STAGES, e.g.
-        "k_backward"); None = all."""
+        "k_backward"); None = all.  every: time them on every n-th PCG
+        iteration only."""
         self._ensure()
         mask = 0xFFFFFFFF
         if kernels is not None:
@@ -500,6 +503,7 @@ class FluidSimulation:
             for k in kernels:
                 mask |= 1 << (STAGES.index(k) - STAGES.index("k_spmv_dot"))
         self.ctx.check(self.L.flipmi_scene_set_kernel_timing(self.h, mask))
+        self.ctx.check(self.L.flipmi_scene_set_kernel_sampling(self.h, int(every)))
 
     def stage_times(self, reset: bool = False) -> dict:
         t = StageTimes()

@@ -436,6 +436,12 @@ int flipmi_scene_set_kernel_timing(flipmi_scene *scene, unsigned mask) {
     return FLIPMI_OK;
 }
 
+int flipmi_scene_set_kernel_sampling(flipmi_scene *scene, int every) {
+    if (!scene || every < 1) return FLIPMI_ERR_INVALID_ARG;
+    scene->s.kernel_every = every;
+    return FLIPMI_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Stage drop-ins
 // ---------------------------------------------------------------------------

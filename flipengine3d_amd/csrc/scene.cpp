@@ -271,7 +271,9 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
         const bool use_graph = w.graphs && pb.s_plain == 0 && !(prof && prof->profiling >= 2) &&
                                pb.epoch + 2u * (unsigned)batch + 8u < 0xfffffff0u;
         for (int q = 0; q < batch; q++) {
-            if (prof && prof->profiling >= 2 && prof->kernel_mask != 0) {
+            const bool timed = prof && prof->profiling >= 2 && prof->kernel_mask != 0 &&
+                               prof->pcg_iteration_count++ % prof->kernel_every == 0;
+            if (timed) {
                 // Time the selected kernels of the iteration on the launch stream.
                 for (int q = 0; q < kPcgPieces; q++) {
                     int piece = pcg_piece_at(q);
@@ -286,6 +288,8 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
                     if (kt.start) prof->add_timed(stage, kt.start, kt.stop);
                 }
             } else if (use_graph) {
+                // (Never at profiling level 2: there every iteration runs on
+                // its own, timed or not.)
                 launch_pcg_batch_graph(*w.graphs, w.g, mixed, pb, batch, s);
                 break;  // the graph holds the whole batch
             } else {

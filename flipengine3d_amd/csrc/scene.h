@@ -175,6 +175,8 @@ struct Scene {
     // profiling
     int profiling = 0;  // 0 off, 1 stage events, 2 also each selected PCG kernel
     unsigned kernel_mask = ~0u;  // PCG kernels timed at level 2 (bit = kPcg* piece)
+    int kernel_every = 1;         // level 2: time them on every n-th PCG iteration
+    int64_t pcg_iteration_count = 0;  // PCG iterations launched, for kernel_every
     double stage_ms[FLIPMI_NUM_STAGES] = {0};
     int64_t stage_calls[FLIPMI_NUM_STAGES] = {0};
     struct PendingEvent {

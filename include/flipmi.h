@@ -302,6 +302,10 @@ int flipmi_scene_set_profiling(flipmi_scene *scene, int level);
  * stage minus FLIPMI_STAGE_K_SPMV (bit 5: the true residual); default all.
  * Each timed launch adds an event pair, which delays the launches around it. */
 int flipmi_scene_set_kernel_timing(flipmi_scene *scene, unsigned mask);
+/* Times the selected kernels on every `every`-th PCG iteration only (default
+ * 1), so a long run is timed at a fraction of the event pairs' delay.  No
+ * counterpart in the reference: measurement only. */
+int flipmi_scene_set_kernel_sampling(flipmi_scene *scene, int every);
 /* Not in the reference: run the PCG's iteration batches as HIP graphs,
  * each (batch size, starting buffer) captured on first use and replayed
  * afterwards, so a batch of k iterations is one launch instead of 5 k.
