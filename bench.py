@@ -50,16 +50,18 @@ KERNEL_SAMPLE = 8
 #   update_s:   z, s r/w                                    -> 3 x 8
 # Mixed path: FP32 vectors and factor, FP64 x and matrix entries:
 #   spmv_dot:   z, old s, matrix, new s, q (= A s)          -> 4 + 4 + 20 + 4 + 4
-#   axpy_norm:  x r/w (FP64), s, r r/w, q                   -> 16 + 4 + 8 + 4
-#   forward:    6 x 4;  backward: 7 x 4
+#   forward with the axpy folded in: r r/w, q, invdiag, 3 lower factor
+#               entries, z, and x r/w (FP64), s             -> 8 + 4 + 4 + 12 + 4 + 16 + 4
+#   backward:   7 x 4
 #   the direction update is folded into spmv_dot.
 # A true residual (mixed path only) reads x and b (FP64) and the matrix and
-# writes r (FP32): 8 + 8 + 20 + 4 = 40 bytes.
+# writes r (FP32): 8 + 8 + 20 + 4 = 40 bytes; the forward sweep over it
+# (the same launch) adds r, invdiag, 3 lower factor entries, z: 6 x 4.
 PCG_BYTES = {
     "fp64": {"k_spmv_dot": 36, "k_axpy_norm": 48, "k_forward": 48, "k_backward": 56, "k_update_s": 24},
-    "mixed": {"k_spmv_dot": 36, "k_axpy_norm": 32, "k_forward": 24, "k_backward": 28, "k_update_s": 0},
+    "mixed": {"k_spmv_dot": 36, "k_axpy_norm": 0, "k_forward": 52, "k_backward": 28, "k_update_s": 0},
 }
-TRUE_RESIDUAL_BYTES = 40
+TRUE_RESIDUAL_BYTES = 40 + 24
 
 # HBM bytes moved per launch, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes scaled by a plain 256 MiB copy (x2.0 and x1.0), on the default

@@ -246,13 +246,19 @@ hipError_t launch_mic0_factor(const GridGeom &g, const float *liquid, PressureBu
 hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, bool mixed,
                            PressureBuffers &pb, hipStream_t s);
 hipError_t launch_pcg_iteration(const GridGeom &g, bool mixed, PressureBuffers &pb, hipStream_t s);
-// The kernels of one PCG iteration, in order; kPcgResidual runs only on the
-// mixed path (it exits at once unless a true residual is due).
+// The kernels of one PCG iteration.  The FP64 path runs spmv, axpy,
+// forward, backward, update.  The mixed path folds the axpy into the forward
+// sweep and the direction update into the SpMV, and runs kPcgResidual (a
+// true residual and a forward sweep over it) between the forward and the
+// backward sweep; it exits at once unless the forward sweep's residual norm
+// called for a true residual.
 enum {
     kPcgSpmv = 0, kPcgAxpy = 1, kPcgForward = 2, kPcgBackward = 3, kPcgUpdate = 4,
     kPcgResidual = 5, kPcgPieces = 6
 };
 bool pcg_piece_used(int piece, bool mixed);
+// Sweep launches per iteration (each takes a record epoch).
+inline unsigned pcg_sweeps_per_iteration(bool mixed) { return mixed ? 3u : 2u; }
 int pcg_piece_at(int q);  // piece executed q-th within an iteration
 // With kt.start set, the kernel's dispatch records its start and end in the
 // two events.

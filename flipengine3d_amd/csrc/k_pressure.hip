@@ -324,7 +324,11 @@ hipError_t launch_group_mask(PressureBuffers &pb, hipStream_t s) {
 // ---------------------------------------------------------------------------
 
 enum { kFactor = 0, kForward = 1, kBackward = 2 };
-enum { kDotNone = 0, kDotInit = 1, kDotIter = 2 };
+// What a sweep launch reduces or decides at its end: nothing, the z.r dot
+// of the first or a later iteration (backward), the residual norm of the
+// axpy the forward sweep did, or (forward, only when a true residual is
+// due) the norm of the true residual it computes before sweeping.
+enum { kDotNone = 0, kDotInit = 1, kDotIter = 2, kAxpyNorm = 3, kRedoForward = 4 };
 
 // Values another workgroup reads within the same kernel are stored and
 // loaded as relaxed device-scope atomics, which are coherent across XCDs by
@@ -437,6 +441,15 @@ __device__ __forceinline__ double wave_sum_fixed(double v) {
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+// Maximum over the wave; the operands are never NaN.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
 
 // Tile order of the dispatch: an atomic ticket hands tiles out in
 // dependency order, so every tile's producers were dispatched before it.
@@ -499,13 +512,16 @@ __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const Gri
 }
 
 // The last wave of a sweep launch resets the shared counters for the next
-// launch and, for the backward sweep, reduces the dot product.
+// launch and finishes the launch's reduction: the backward sweep's dot
+// product, or the residual norm of a forward sweep that did the axpy.
 __device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, int tile,
                              bool reset_progress, unsigned next_epoch = 0) {
     const int lane = threadIdx.x & 63;
     const int ntiles = pb.sm.ntj * pb.sm.ntk;
-    if (dotmode != kDotNone) {
-        double part = wave_sum_fixed(dot_acc);
+    const bool dot = dotmode == kDotInit || dotmode == kDotIter;
+    const bool norm = dotmode == kAxpyNorm || dotmode == kRedoForward;
+    if (dot || norm) {
+        double part = dot ? wave_sum_fixed(dot_acc) : wave_max(dot_acc);
         if (lane == 0) st_dev(&pb.partials[tile], part);
     }
     drain_memory();
@@ -520,7 +536,38 @@ __device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, i
         pb.sc->counter[0] = 0;
         if (next_epoch) pb.sc->sweep_epoch = next_epoch;
     }
-    if (dotmode != kDotNone) {
+    if (norm) {
+        double m = 0.0;
+        for (int q = lane; q < ntiles; q += 64) {
+            const double v = ld_dev(&pb.partials[q]);
+            m = v > m ? v : m;
+        }
+        m = wave_max(m);
+        if (lane == 0) {
+            PcgScalars *sc = pb.sc;
+            if (dotmode == kAxpyNorm) {
+                // |r32|_inf after the axpy: a true residual is due when it
+                // meets the tolerance, has fallen by delta since the last
+                // one, or the iteration cap is reached.
+                sc->iters += 1;
+                if (m <= sc->tol || m < sc->delta * sc->rmax || sc->iters >= sc->maxiter) sc->refresh = 1;
+                else sc->resid = m;
+            } else {
+                // |b - A x|_inf in FP64 decides convergence.
+                sc->resid = m;
+                sc->refinements += 1;
+                sc->refresh = 0;
+                sc->rmax = m;
+                if (m <= sc->tol) {
+                    sc->converged = 1;
+                    sc->done = 1;
+                } else if (sc->iters >= sc->maxiter) {
+                    sc->done = 1;
+                }
+            }
+        }
+    }
+    if (dot) {
         // Fixed-order sum of the per-tile partials.
         double acc = 0.0;
         for (int q = lane; q < ntiles; q += 64) acc += ld_dev(&pb.partials[q]);
@@ -1011,16 +1058,134 @@ __device__ void sweep_poller(PressureBuffers &pb, SweepMailbox<T> &mb, int tile,
     }
 }
 
-template <class T, int DIR, int D>
-__global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, SweepOps<T> op,
-                                               int dotmode) {
+// y = A x at slot p of cell (i, j, k) (sparsematrix.h:187-216: row entries
+// in increasing column order).
+__device__ __forceinline__ double spmv_slot(const PressureBuffers &pb, double mf, const double *__restrict__ x,
+                                            size_t p, int i, int j, int k) {
+    const SlotMap &sm = pb.sm;
+    const unsigned pzm = sm.slot32(i, j, k - 1), pym = sm.slot32(i, j - 1, k), pxm = sm.slot32(i - 1, j, k);
+    const unsigned pxp = sm.slot32(i + 1, j, k), pyp = sm.slot32(i, j + 1, k), pzp = sm.slot32(i, j, k + 1);
+    double s = 0;
+    s += coupling(pb.wz[pzm], mf) * x[pzm];
+    s += coupling(pb.wy[pym], mf) * x[pym];
+    s += coupling(pb.wx[pxm], mf) * x[pxm];
+    s += pb.diag[p] * x[p];
+    s += coupling(pb.wx[p], mf) * x[pxp];
+    s += coupling(pb.wy[p], mf) * x[pyp];
+    s += coupling(pb.wz[p], mf) * x[pzp];
+    return s;
+}
+
+// The true residual of one tile for the forward sweep that follows it
+// (kRedoForward): r = b - A x in FP64 at the slots of the tile's active
+// groups, r32 = (float) r, as a separate true-residual pass over the active
+// slot list computes them; returns max |r| over the tile to every thread.
+// The stores are visible to the workgroup's loads after it returns.
+template <int NT>
+__device__ double tile_true_residual(const GridGeom &g, const PressureBuffers &pb, int tile, int tb, int te,
+                                     const GroupBits &gb) {
+    __shared__ double wm[NT / 64];
+    const double mf = pb.sc->mf;  // coupling scale of the assembled system
+    const size_t tile0 = (size_t)tile * (size_t)(pb.sm.S >> 2) * 256;
+    const int g0 = tb >> 2, total = ((te >> 2) - g0) * 256;
+    double mx = 0.0;
+    constexpr int U = 4;
+    for (int e0 = (int)threadIdx.x; e0 < total; e0 += U * NT) {
+        size_t p[U];
+        bool ok[U], in[U];
+        double ax[U], bs[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = e0 + u * NT, gi = g0 + (e >> 8);
+            ok[u] = e < total && gb.test(gi);
+            p[u] = ok[u] ? tile0 + (size_t)gi * 256 + (size_t)(e & 255) : tile0;
+            int i, j, k;
+            in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
+            ax[u] = spmv_slot(pb, mf, pb.x, p[u], i, j, k);
+            bs[u] = pb.rhs[p[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) continue;
+            const double r = in[u] ? bs[u] - ax[u] : 0.0;
+            pb.r32[p[u]] = (float)r;
+            const double a = fabs(r);
+            mx = a > mx ? a : mx;
+        }
+    }
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    double m = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) m = wm[w] > m ? wm[w] : m;
+    return m;
+}
+
+// The x half of the axpy that an AX forward sweep does (pcgsolver.h:290),
+// by the workgroup's third wave: x += alpha s32 (FP64) over the tile's
+// active slot groups, four chunks (eight groups) at a time once the compute
+// wave has reached them, so this traffic follows the march instead of
+// meeting the first tiles' polls all at once.  All active groups lie in the
+// march range [tb, te); slots of an active group that are not rows hold
+// s32 = 0 and x = 0, which stays 0.
+template <class T>
+__device__ void sweep_x_wave(const PressureBuffers &pb, SweepMailbox<T> &mb, int tile, int tb, int te, int lane,
+                             const GroupBits &gb) {
+    constexpr int U = 8;
+    const double alpha = pb.sc->alpha;
+    const int S = pb.sm.S;
+    const size_t tile0 = (size_t)tile * (size_t)(S >> 2) * 256;
+    const auto r_x = buffer_view(pb.x + tile0, (unsigned)S * 64u * 8u);
+    const auto r_s = buffer_view(pb.s32 + tile0, (unsigned)S * 64u * 4u);
+    const unsigned vx = (unsigned)lane * 32u, vs = (unsigned)lane * 16u;
+    const int gfirst = tb >> 2, gend = te >> 2;
+    int seen = 0;  // chunks the compute wave is known to have reached
+    for (int g0 = gfirst; g0 < gend; g0 += U) {
+        const int need = (g0 - gfirst) / 2 + 1;  // chunk holding group g0, + 1
+        unsigned spins = 0;
+        while (seen < need) {
+            seen = lds_acquire(&mb.consumed);
+            if (seen >= need) break;
+            if (++spins > kSpinLimit) break;  // the compute wave gave up: finish anyway
+            __builtin_amdgcn_s_sleep(2);
+        }
+        double xv[U][4];
+        float sv[U][4];
+        bool act[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int gi = g0 + u;
+            act[u] = gi < gend && gb.test(gi);
+            load_group(r_x, act[u] ? vx : kOutside, (unsigned)gi * 2048u, xv[u]);
+            load_group(r_s, act[u] ? vs : kOutside, (unsigned)gi * 1024u, sv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) xv[u][m] = xv[u][m] + alpha * (double)sv[u][m];
+            store_group(r_x, act[u] ? vx : kOutside, (unsigned)(g0 + u) * 2048u, xv[u]);
+        }
+    }
+}
+
+// AX (FP32 forward sweep of the mixed path only): the sweep also does the
+// iteration's axpy (pcgsolver.h:289-290).  The compute wave reads
+// r32 + (-alpha) q32 in place of r32 (off the step's dependency chain),
+// stores it back to r32 and takes its max norm; the third wave updates x
+// (sweep_x_wave).  Each value is rounded as a separate axpy pass would
+// round it, and the max is order-independent.
+template <class T, int DIR, int D, bool AX = false>
+__global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, SweepOps<T> op,
+                                                          int dotmode) {
     constexpr int C = kChunk;
     constexpr bool FWD = DIR == kForward;
-    constexpr int NV = FWD ? 5 : 6;
+    static_assert(!AX || (FWD && sizeof(T) == 4), "the axpy is folded into the FP32 forward sweep only");
+    constexpr int NV = FWD ? (AX ? 6 : 5) : 6;
     constexpr int RW = Record<T>::W;
     __shared__ SweepMailbox<T> mb;
     if (threadIdx.x == 0) {
-        const int skip = ld_dev(&pb.sc->done) != 0;
+        const int skip = ld_dev(&pb.sc->done) != 0 || (dotmode == kRedoForward && ld_dev(&pb.sc->refresh) == 0);
         mb.skip = skip;
         mb.tile = skip ? 0 : (int)atomicAdd(pb.ticket, 1u);
         mb.epoch = ld_dev(&pb.sc->sweep_epoch);
@@ -1043,8 +1208,18 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
     if (up_k >= 0) tile_urange(pb, g, up_k, !FWD, ubk, uek);
     const int nchunks = te > tb ? (te - tb) / C : 0;
     const GroupBits gb = tile_group_bits(pb, tile);
+    // kRedoForward: the tile's true residual first (every wave), then the sweep over it.
+    double tres_max = 0.0;
+    if constexpr (FWD && !AX && sizeof(T) == 4) {
+        if (dotmode == kRedoForward) {
+            tres_max = tile_true_residual<256>(g, pb, tile, tb, te, gb);
+            if (wave >= 2) return;  // those waves only help with the residual
+        }
+    }
     if (wave == 1)
         return sweep_poller<T, FWD>(pb, mb, tile, up_j, up_k, tb, nchunks, ubj, uej, ubk, uek, lane, gb);
+    if constexpr (AX)
+        if (wave == 2) return sweep_x_wave(pb, mb, tile, tb, te, lane, gb);
 
     const int src_j = (FWD ? lane - 1 : lane + 1) & 63;  // neighbour lane along j
     const int src_k = (FWD ? lane - 8 : lane + 8) & 63;  // neighbour lane along k
@@ -1083,6 +1258,11 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
     const auto r_inv = buffer_view(op.inv + tile0, tile_bytes);
     const auto r_dot = buffer_view(dotsrc + tile0, tile_bytes);
     const auto r_out = buffer_view(op.out + tile0, tile_bytes);
+    // AX: the SpMV result q32 read beside r32, and r32 written back.
+    const auto r_q = buffer_view((AX ? (const T *)pb.q32 : op.c1) + tile0, tile_bytes);
+    const auto r_rw = buffer_view((AX ? (const T *)pb.r32 : op.out) + tile0, tile_bytes);
+    const T naf = AX ? T(-pb.sc->alpha_f) : T(0);
+    T rmax = T(0);  // AX: max |r| over the tile's updated residual
     const unsigned vlane = (unsigned)lane * 4u * (unsigned)sizeof(T);
     // Every lane stores a record each step (no branch around the store):
     // lanes that publish nothing write slot 8 + aj, which the corner lane
@@ -1110,6 +1290,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                 load_group(r_c3, vo, so, e[3]);
                 load_group(r_inv, vo, so, e[4]);
                 if (!FWD) load_group(r_dot, vo, so, e[NV - 1]);
+                if (AX) load_group(r_q, vo, so, e[NV - 1]);
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
                     const int s = FWD ? 4 * h + m : 7 - (4 * h + m);
@@ -1118,7 +1299,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                 }
             }
         };
-        auto store_chunk = [&](int u0, const T (&qv)[C], const bool (&act)[2]) {
+        auto store_chunk = [&](__amdgpu_buffer_rsrc_t dst, int u0, const T (&qv)[C], const bool (&act)[2]) {
             const int gl = chunk_group(u0);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -1127,7 +1308,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                 for (int m = 0; m < 4; m++) e[m] = qv[FWD ? 4 * h + m : 7 - (4 * h + m)];
                 // The output of a group without rows stays zero: its values
                 // (all zero) are dropped.
-                store_group(r_out, act[h] ? vlane : kOutside, (unsigned)(gl + h) * 256u * (unsigned)sizeof(T), e);
+                store_group(dst, act[h] ? vlane : kOutside, (unsigned)(gl + h) * 256u * (unsigned)sizeof(T), e);
             }
         };
         // The operands of the first D chunks.
@@ -1189,7 +1370,7 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                     // then reuse the ring slot.
                     if (lane == 0) lds_release(&mb.consumed, ci + 1);
                     const T(&v)[NV][C] = buf[d];
-                    T qv[C];
+                    T qv[C], bv[C];
     #pragma unroll
                     for (int s = 0; s < C; s++) {
                         // Values produced one step earlier by the neighbouring lanes.
@@ -1205,6 +1386,15 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                         }
                         const T qy = edge_j ? evj[s] : nj_cur;
                         const T qz = edge_k ? evk[s] : nk_cur;
+                        // AX: the updated residual r + (-alpha) q, off the
+                        // step's dependency chain; fmaxf(m, |r|) is the
+                        // compare-and-keep maximum here (m is never NaN).
+                        T bs = v[0][s];
+                        if constexpr (AX) {
+                            bs = v[0][s] + naf * v[NV - 1][s];
+                            bv[s] = bs;
+                            rmax = fmaxf(rmax, fabsf(bs));
+                        }
                         // c1..c3 multiply the -z, -y, -x neighbours (forward) or
                         // the +x, +y, +z ones (backward).
                         T q;
@@ -1212,8 +1402,8 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                             q = FWD ? ref_step(v[0][s], v[1][s], qz, v[2][s], qy, v[3][s], cur, v[4][s])
                                     : ref_step(v[0][s], v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
                         } else {
-                            q = FWD ? fused_step(v[0][s], v[3][s], cur, v[2][s], qy, v[1][s], qz, v[4][s])
-                                    : fused_step(v[0][s], v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
+                            q = FWD ? fused_step(bs, v[3][s], cur, v[2][s], qy, v[1][s], qz, v[4][s])
+                                    : fused_step(bs, v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
                         }
                         if (!FWD) dot_acc = __builtin_fma((double)q, (double)v[NV - 1][s], dot_acc);
                         qv[s] = q;
@@ -1225,7 +1415,8 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
                         __builtin_amdgcn_sched_barrier(0);
                         cur = q;
                     }
-                    store_chunk(uc, qv, actr[d]);
+                    store_chunk(r_out, uc, qv, actr[d]);
+                    if constexpr (AX) store_chunk(r_rw, uc, bv, actr[d]);
                 }
                 const int uf = uc + D * C;
                 load_chunk(uf < last ? uf : last, buf[d], actr[d]);
@@ -1233,7 +1424,9 @@ __global__ __launch_bounds__(128) void k_sweep(GridGeom g, PressureBuffers pb, S
             }
         }
     }
-    finish_sweep(pb, dot ? dotmode : kDotNone, dot_acc, tile, false, epoch + 1);
+    const int fin = FWD ? (AX ? kAxpyNorm : (dotmode == kRedoForward ? kRedoForward : kDotNone))
+                        : (dot ? dotmode : kDotNone);
+    finish_sweep(pb, fin, AX ? (double)rmax : (FWD ? tres_max : dot_acc), tile, false, epoch + 1);
 }
 
 // Launches a kernel on s.  With a start event the dispatch itself records
@@ -1276,7 +1469,17 @@ static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, 
     }
     int ntiles = pb.sm.ntj * pb.sm.ntk;
     constexpr int D = SweepDepth<T>::D;
-    if (dir == kForward) return launch_on(k_sweep<T, kForward, D>, ntiles, 128, s, kt, g, pb, op, dotmode);
+    if (dotmode == kAxpyNorm) {
+        if constexpr (sizeof(T) == 4) {
+            if (dir == kForward) return launch_on(k_sweep<T, kForward, D, true>, ntiles, 192, s, kt, g, pb, op, dotmode);
+        }
+        return hipErrorInvalidValue;
+    }
+    // The forward sweep over a fresh true residual has four waves for the
+    // residual; two of them then leave.
+    if (dir == kForward)
+        return launch_on(k_sweep<T, kForward, D>, ntiles, dotmode == kRedoForward ? 256 : 128, s, kt, g, pb, op,
+                         dotmode);
     return launch_on(k_sweep<T, kBackward, D>, ntiles, 128, s, kt, g, pb, op, dotmode);
 }
 
@@ -1346,24 +1549,6 @@ __device__ bool last_block(unsigned *counter) {
     if (threadIdx.x == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
     __syncthreads();
     return is_last;
-}
-
-// y = A x at slot p of cell (i, j, k) (sparsematrix.h:187-216: row entries
-// in increasing column order).
-__device__ __forceinline__ double spmv_slot(const PressureBuffers &pb, double mf, const double *__restrict__ x,
-                                            size_t p, int i, int j, int k) {
-    const SlotMap &sm = pb.sm;
-    const unsigned pzm = sm.slot32(i, j, k - 1), pym = sm.slot32(i, j - 1, k), pxm = sm.slot32(i - 1, j, k);
-    const unsigned pxp = sm.slot32(i + 1, j, k), pyp = sm.slot32(i, j + 1, k), pzp = sm.slot32(i, j, k + 1);
-    double s = 0;
-    s += coupling(pb.wz[pzm], mf) * x[pzm];
-    s += coupling(pb.wy[pym], mf) * x[pym];
-    s += coupling(pb.wx[pxm], mf) * x[pxm];
-    s += pb.diag[p] * x[p];
-    s += coupling(pb.wx[p], mf) * x[pxp];
-    s += coupling(pb.wy[p], mf) * x[pyp];
-    s += coupling(pb.wz[p], mf) * x[pzp];
-    return s;
 }
 
 // z = A s and alpha = rho / (s . z), alpha rounded to float as the
@@ -1778,110 +1963,6 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, Pressure
     }
 }
 
-// x += alpha s32 (FP64), r32 -= alpha q32 (FP32), |r32|_inf; schedules a
-// true residual when it is due.
-__global__ __launch_bounds__(kRedThreads) void k_axpy_norm32(GridGeom g, PressureBuffers pb) {
-    __shared__ double sh[kRedThreads / 64];
-    if (pb.sc->done) return;
-    double alpha = pb.sc->alpha;
-    float naf = -pb.sc->alpha_f;
-    double mx = 0.0;
-    const ListRange lr = xcd_list(*pb.active_count);
-    constexpr int U = 4;
-    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
-        size_t p[U];
-        bool ok[U];
-        double xs[U];
-        float ss[U], rs[U], zs[U];
-        list_slots<U>(pb, lr, e0, p, ok);
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            xs[u] = pb.x[p[u]]; ss[u] = pb.s32[p[u]];
-            rs[u] = pb.r32[p[u]]; zs[u] = pb.q32[p[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            if (!ok[u]) continue;  // uniform
-            pb.x[p[u]] = xs[u] + alpha * (double)ss[u];
-            const float r = rs[u] + naf * zs[u];
-            pb.r32[p[u]] = r;
-            const double a = fabs((double)r);
-            mx = a > mx ? a : mx;
-        }
-    }
-    double bm = block_reduce<1>(mx, sh);
-    if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
-    if (!last_block(&pb.sc->counter[2])) return;
-    double m = 0.0;
-    for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) {
-        double v = ld_dev(&pb.partials[kRedBlocks + q]);
-        m = v > m ? v : m;
-    }
-    m = block_reduce<1>(m, sh);
-    if (threadIdx.x == 0) {
-        PcgScalars *sc = pb.sc;
-        sc->iters += 1;
-        if (m <= sc->tol || m < sc->delta * sc->rmax || sc->iters >= sc->maxiter) sc->refresh = 1;
-        else sc->resid = m;
-        sc->counter[2] = 0;
-    }
-}
-
-// r = b - A x in FP64 (A in FP64), r32 = (float) r, |r|_inf; decides
-// convergence.  Exits at once unless a true residual is due.
-__global__ __launch_bounds__(kRedThreads) void k_true_residual(GridGeom g, PressureBuffers pb) {
-    __shared__ double sh[kRedThreads / 64];
-    if (pb.sc->done || !pb.sc->refresh) return;
-    double mx = 0.0;
-    const ListRange lr = xcd_list(*pb.active_count);
-    const double mf = pb.sc->mf;  // coupling scale of the assembled system
-    constexpr int U = 2;
-    for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
-        size_t p[U];
-        bool ok[U], in[U];
-        double ax[U], bs[U];
-        list_slots<U>(pb, lr, e0, p, ok);
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            int i, j, k;
-            in[u] = stencil_cell(g, pb.sm, p[u], i, j, k);
-            ax[u] = spmv_slot(pb, mf, pb.x, p[u], i, j, k);
-            bs[u] = pb.rhs[p[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            if (!ok[u]) continue;  // uniform
-            const double r = in[u] ? bs[u] - ax[u] : 0.0;
-            pb.r32[p[u]] = (float)r;
-            const double a = fabs(r);
-            mx = a > mx ? a : mx;
-        }
-    }
-    double bm = block_reduce<1>(mx, sh);
-    if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
-    if (!last_block(&pb.sc->counter[3])) return;
-    double m = 0.0;
-    for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) {
-        double v = ld_dev(&pb.partials[kRedBlocks + q]);
-        m = v > m ? v : m;
-    }
-    m = block_reduce<1>(m, sh);
-    if (threadIdx.x == 0) {
-        PcgScalars *sc = pb.sc;
-        sc->resid = m;
-        sc->refinements += 1;
-        sc->refresh = 0;
-        sc->rmax = m;
-        if (m <= sc->tol) {
-            sc->converged = 1;
-            sc->done = 1;
-        } else if (sc->iters >= sc->maxiter) {
-            sc->done = 1;
-        }
-        sc->counter[3] = 0;
-    }
-}
-
 __global__ __launch_bounds__(kRedThreads) void k_update_s32(PressureBuffers pb, int first) {
     if (pb.sc->done) return;
     float bf = first ? 0.0f : pb.sc->beta_f;
@@ -1939,9 +2020,12 @@ hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, bool mix
     return hipGetLastError();
 }
 
-// Mixed precision folds the direction update into the next SpMV; the FP64
-// path has no true-residual step.
-bool pcg_piece_used(int piece, bool mixed) { return mixed ? piece != kPcgUpdate : piece != kPcgResidual; }
+// Mixed precision folds the axpy into the forward sweep and the direction
+// update into the next SpMV; the FP64 path has no true-residual step.
+bool pcg_piece_used(int piece, bool mixed) {
+    if (mixed) return piece != kPcgUpdate && piece != kPcgAxpy;
+    return piece != kPcgResidual;
+}
 
 hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBuffers &pb,
                             hipStream_t s, const KernelTiming &kt) {
@@ -1955,9 +2039,10 @@ hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBu
                 pb.s_plain = 0;
                 return e;
             }
-            case kPcgAxpy: return launch_on(k_axpy_norm32, dim3(kAxpyBlocks), rb, s, kt, g, pb);
-            case kPcgResidual: return launch_on(k_true_residual, rg, rb, s, kt, g, pb);
-            case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kDotNone, s, kt);
+            // The forward sweep does the axpy; when its residual norm calls
+            // for a true residual, the sweep is redone over that residual.
+            case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kAxpyNorm, s, kt);
+            case kPcgResidual: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kRedoForward, s, kt);
             case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotIter, s, kt);
             case kPcgUpdate: return launch_on(k_update_s32, rg, rb, s, kt, pb, 0);
             default: return hipErrorInvalidValue;
@@ -1975,8 +2060,8 @@ hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBu
 }
 
 // Execution order of the pieces within one iteration.
-static const int kPieceOrder[kPcgPieces] = {kPcgSpmv, kPcgAxpy, kPcgResidual,
-                                            kPcgForward, kPcgBackward, kPcgUpdate};
+static const int kPieceOrder[kPcgPieces] = {kPcgSpmv,     kPcgAxpy,     kPcgForward,
+                                            kPcgResidual, kPcgBackward, kPcgUpdate};
 
 hipError_t launch_pcg_iteration(const GridGeom &g, bool mixed, PressureBuffers &pb, hipStream_t s) {
     for (int q = 0; q < kPcgPieces; q++) {

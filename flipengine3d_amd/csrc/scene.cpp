@@ -219,7 +219,7 @@ static void launch_pcg_batch_graph(PcgGraphs &G, const GridGeom &g, bool mixed, 
         it = G.exec.emplace(key, exec).first;
     } else {
         if (mixed && (batch & 1)) std::swap(pb.s32, pb.s32b);
-        pb.epoch += 2u * (unsigned)batch;
+        pb.epoch += pcg_sweeps_per_iteration(mixed) * (unsigned)batch;
     }
     hip_check(hipGraphLaunch(it->second, s), "pcg graph launch");
 }
@@ -269,7 +269,7 @@ PressureOutcome run_pressure(const PressureWork &w, PressureBuffers &pb, PcgScal
         // s as initialised) is behind, unless kernels are being timed or the
         // sweep tag is near its wrap-around (which clears the records).
         const bool use_graph = w.graphs && pb.s_plain == 0 && !(prof && prof->profiling >= 2) &&
-                               pb.epoch + 2u * (unsigned)batch + 8u < 0xfffffff0u;
+                               pb.epoch + pcg_sweeps_per_iteration(mixed) * (unsigned)batch + 8u < 0xfffffff0u;
         for (int q = 0; q < batch; q++) {
             const bool timed = prof && prof->profiling >= 2 && prof->kernel_mask != 0 &&
                                prof->pcg_iteration_count++ % prof->kernel_every == 0;

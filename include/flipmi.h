@@ -106,7 +106,7 @@ enum {
     FLIPMI_STAGE_K_FORWARD = 14,
     FLIPMI_STAGE_K_BACKWARD = 15,
     FLIPMI_STAGE_K_UPDATE = 16,
-    FLIPMI_STAGE_K_RESIDUAL = 17,   /* mixed precision: FP64 true residual */
+    FLIPMI_STAGE_K_RESIDUAL = 17,   /* mixed precision: FP64 true residual + forward sweep over it */
     FLIPMI_STAGE_INFLOW = 18,       /* mesh inflow: particle mask and emission */
     FLIPMI_STAGE_SURFACE = 19       /* surface reconstruction (sort, field, marching cubes, smoothing) */
 };
@@ -299,7 +299,9 @@ int flipmi_scene_stage_times(flipmi_scene *scene, flipmi_stage_times *out, int r
  * stage, 2 additionally one pair around every PCG kernel launch. */
 int flipmi_scene_set_profiling(flipmi_scene *scene, int level);
 /* PCG kernels timed at profiling level 2, one bit per FLIPMI_STAGE_K_*
- * stage minus FLIPMI_STAGE_K_SPMV (bit 5: the true residual); default all.
+ * stage minus FLIPMI_STAGE_K_SPMV (bit 5: the true residual and, on the
+ * mixed path, the forward sweep over it; the mixed path's forward sweep also
+ * does its axpy, so its FLIPMI_STAGE_K_AXPY stays empty); default all.
  * Each timed launch adds an event pair, which delays the launches around it. */
 int flipmi_scene_set_kernel_timing(flipmi_scene *scene, unsigned mask);
 /* Times the selected kernels on every `every`-th PCG iteration only (default

@@ -23,10 +23,13 @@ def test_pcg_byte_model_matches_the_kernels():
     # stored as FP32 face weights), axpy 48, forward sweep 48, backward sweep
     # 56 (it also reads r for the fused z.r), direction update 24: 212 B per
     # row per iteration.  Mixed: FP32 vectors and factor, FP64 x, the same
-    # matrix, the direction update in the SpMV: 36 + 32 + 24 + 28 = 120 B.
+    # matrix, the direction update in the SpMV, the axpy in the forward
+    # sweep (which reads r there anyway): 36 + (24 + 28) + 28 = 116 B.
     assert sum(bench.PCG_BYTES["fp64"].values()) == 212
-    assert sum(bench.PCG_BYTES["mixed"].values()) == 120
-    assert bench.TRUE_RESIDUAL_BYTES == 8 + 8 + 8 + 3 * 4 + 4
+    assert sum(bench.PCG_BYTES["mixed"].values()) == 116
+    assert bench.PCG_BYTES["mixed"]["k_axpy_norm"] == 0
+    # A true residual (x, b, diag, 3 weights, r) and the forward sweep over it.
+    assert bench.TRUE_RESIDUAL_BYTES == (8 + 8 + 8 + 3 * 4 + 4) + 6 * 4
     args = bench.parse_args([])
     assert args.gpus == 1 and args.size == [256, 128, 128]
     assert json.dumps({"x": None})  # the line is plain JSON
