@@ -851,6 +851,7 @@ struct SweepMailbox {
     int tile;      // the workgroup's ticket
     int skip;      // the solve has finished: nothing to do
     unsigned epoch;  // this launch's record tag
+    int stored;    // chunks whose outputs the compute wave has stored (the dot wave follows it)
 };
 
 __device__ __forceinline__ int lds_acquire(int *p) {
@@ -1169,19 +1170,72 @@ __device__ void sweep_x_wave(const PressureBuffers &pb, SweepMailbox<T> &mb, int
     }
 }
 
+// The fused z.r of a backward sweep of the mixed path (the dot of
+// pcgsolver.h:296), by the workgroup's third wave: it follows the compute
+// wave, four chunks at a time once their z values are stored, reading z and
+// r for them, and accumulates fma((double) z, (double) r, acc) per lane in
+// march and step order, as the compute wave would.  Chunks without rows
+// hold z = 0 and add nothing.  It finishes the launch (finish_sweep).
+__device__ void sweep_dot_wave(PressureBuffers &pb, SweepMailbox<float> &mb, const SweepOps<float> &op, int tile,
+                               int tb, int nchunks, int lane, const GroupBits &gb, int dotmode) {
+    constexpr int C = kChunk, K = 4;
+    const int S = pb.sm.S;
+    const size_t tile0 = (size_t)tile * (size_t)(S >> 2) * 256;
+    const unsigned fbytes = (unsigned)S * 64u * 4u;
+    const auto r_z = buffer_view(op.out + tile0, fbytes);
+    const auto r_r = buffer_view(op.dotvec + tile0, fbytes);
+    const unsigned vf = (unsigned)lane * 16u;
+    double acc = 0.0;
+    int seen = 0;
+    for (int c0 = 0; c0 < nchunks; c0 += K) {
+        const int c1 = min(c0 + K, nchunks);
+        unsigned spins = 0;
+        while (seen < c1) {
+            seen = lds_acquire(&mb.stored);
+            if (seen >= c1) break;
+            if (++spins > kSpinLimit) break;  // the compute wave gave up: finish anyway
+            __builtin_amdgcn_s_sleep(1);
+        }
+        float zv[K][2][4], rv[K][2][4];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int gl = chunk_group<false>(S, tb + (c0 + k) * C);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const bool act = c0 + k < c1 && gb.test(gl + h);
+                const unsigned so = (unsigned)(gl + h) * 1024u;
+                load_group(r_z, act ? vf : kOutside, so, zv[k][h]);
+                load_group(r_r, act ? vf : kOutside, so, rv[k][h]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int s = 0; s < C; s++) {
+                // Step s of a backward chunk is element (7 - s) % 4 of group (7 - s) / 4.
+                const int h = (7 - s) >> 2, m = (7 - s) & 3;
+                acc = __builtin_fma((double)zv[k][h][m], (double)rv[k][h][m], acc);
+            }
+    }
+    finish_sweep(pb, dotmode, acc, tile, false, mb.epoch + 1);
+}
+
 // AX (FP32 forward sweep of the mixed path only): the sweep also does the
 // iteration's axpy (pcgsolver.h:289-290).  The compute wave reads
 // r32 + (-alpha) q32 in place of r32 (off the step's dependency chain),
 // stores it back to r32 and takes its max norm; the third wave updates x
 // (sweep_x_wave).  Each value is rounded as a separate axpy pass would
 // round it, and the max is order-independent.
-template <class T, int DIR, int D, bool AX = false>
+// AUX (FP32 sweeps of the mixed path): a third wave helps, the axpy's x
+// update on the forward sweep (AX), the z.r dot on the backward one (DW).
+template <class T, int DIR, int D, bool AUX = false>
 __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, SweepOps<T> op,
                                                           int dotmode) {
     constexpr int C = kChunk;
     constexpr bool FWD = DIR == kForward;
-    static_assert(!AX || (FWD && sizeof(T) == 4), "the axpy is folded into the FP32 forward sweep only");
-    constexpr int NV = FWD ? (AX ? 6 : 5) : 6;
+    static_assert(!AUX || sizeof(T) == 4, "helper waves serve the FP32 sweeps only");
+    constexpr bool AX = AUX && FWD, DW = AUX && !FWD;
+    constexpr int NV = FWD ? (AX ? 6 : 5) : (DW ? 5 : 6);
     constexpr int RW = Record<T>::W;
     __shared__ SweepMailbox<T> mb;
     if (threadIdx.x == 0) {
@@ -1191,6 +1245,7 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
         mb.epoch = ld_dev(&pb.sc->sweep_epoch);
         mb.ready = 0;
         mb.consumed = 0;
+        mb.stored = 0;
     }
     __syncthreads();
     if (mb.skip) return;
@@ -1220,6 +1275,8 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
         return sweep_poller<T, FWD>(pb, mb, tile, up_j, up_k, tb, nchunks, ubj, uej, ubk, uek, lane, gb);
     if constexpr (AX)
         if (wave == 2) return sweep_x_wave(pb, mb, tile, tb, te, lane, gb);
+    if constexpr (DW)
+        if (wave == 2) return sweep_dot_wave(pb, mb, op, tile, tb, nchunks, lane, gb, dotmode);
 
     const int src_j = (FWD ? lane - 1 : lane + 1) & 63;  // neighbour lane along j
     const int src_k = (FWD ? lane - 8 : lane + 8) & 63;  // neighbour lane along k
@@ -1236,7 +1293,7 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
     const int pub_slot = a == aj ? b : 8 + a;
     const unsigned epoch = mb.epoch;
     const size_t grp0 = (size_t)tile * (size_t)(S >> 2);
-    const bool dot = !FWD && dotmode != kDotNone;
+    const bool dot = !FWD && !DW && dotmode != kDotNone;
     // Without a dot product the backward sweep still streams a vector (its
     // own output) so the hot loop has no data-dependent branches.
     const T *dotsrc = dot ? op.dotvec : op.out;
@@ -1289,7 +1346,7 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
                 load_group(r_c2, vo, so, e[2]);
                 load_group(r_c3, vo, so, e[3]);
                 load_group(r_inv, vo, so, e[4]);
-                if (!FWD) load_group(r_dot, vo, so, e[NV - 1]);
+                if (!FWD && !DW) load_group(r_dot, vo, so, e[NV - 1]);
                 if (AX) load_group(r_q, vo, so, e[NV - 1]);
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
@@ -1405,7 +1462,7 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
                             q = FWD ? fused_step(bs, v[3][s], cur, v[2][s], qy, v[1][s], qz, v[4][s])
                                     : fused_step(bs, v[1][s], cur, v[2][s], qy, v[3][s], qz, v[4][s]);
                         }
-                        if (!FWD) dot_acc = __builtin_fma((double)q, (double)v[NV - 1][s], dot_acc);
+                        if (!FWD && !DW) dot_acc = __builtin_fma((double)q, (double)v[NV - 1][s], dot_acc);
                         qv[s] = q;
                         Record<T>::put(recp + (size_t)s * kRecSlots * RW, q, epoch);
                         // Keep each step's record where it is computed: no
@@ -1418,12 +1475,16 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
                     store_chunk(r_out, uc, qv, actr[d]);
                     if constexpr (AX) store_chunk(r_rw, uc, bv, actr[d]);
                 }
+                // The dot wave may read the chunk's z: the release orders the
+                // stores above before the count (one CU, in issue order).
+                if (DW && lane == 0) lds_release(&mb.stored, ci + 1);
                 const int uf = uc + D * C;
                 load_chunk(uf < last ? uf : last, buf[d], actr[d]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
+    if constexpr (DW) return;  // the dot wave finishes the launch
     const int fin = FWD ? (AX ? kAxpyNorm : (dotmode == kRedoForward ? kRedoForward : kDotNone))
                         : (dot ? dotmode : kDotNone);
     finish_sweep(pb, fin, AX ? (double)rmax : (FWD ? tres_max : dot_acc), tile, false, epoch + 1);
@@ -1480,6 +1541,10 @@ static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, 
     if (dir == kForward)
         return launch_on(k_sweep<T, kForward, D>, ntiles, dotmode == kRedoForward ? 256 : 128, s, kt, g, pb, op,
                          dotmode);
+    // The FP32 backward sweep's z.r is taken by a third wave.
+    if constexpr (sizeof(T) == 4)
+        if (dotmode == kDotInit || dotmode == kDotIter)
+            return launch_on(k_sweep<T, kBackward, D, true>, ntiles, 192, s, kt, g, pb, op, dotmode);
     return launch_on(k_sweep<T, kBackward, D>, ntiles, 128, s, kt, g, pb, op, dotmode);
 }
 
