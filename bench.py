@@ -69,8 +69,8 @@ TRUE_RESIDUAL_BYTES = 40 + 24
 # profiles/traffic_pcg_kernels_double_dam_256x128x128_mixed.md.  Keyed by
 # (precision, grid); other workloads report null.
 MEASURED_TRAFFIC = {
-    ("mixed", (256, 128, 128)): {"k_backward": 64.13e6, "k_forward": 57.12e6, "k_spmv_dot": 69.31e6,
-                                 "k_axpy_norm": 53.39e6, "k_true_residual": 73.51e6},
+    ("mixed", (256, 128, 128)): {"k_backward": 59.63e6, "k_forward": 99.55e6, "k_spmv_dot": 69.48e6,
+                                 "k_true_residual": 116.03e6},
 }
 
 def parse_args(argv=None):
