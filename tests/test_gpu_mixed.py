@@ -23,8 +23,8 @@ pytestmark = pytest.mark.gpu
 PCG_MIXED = 1
 
 
-def _projection_inputs(oracle, I, J, K, seed=4):
-    s = dam(I, J, K, seed=seed)
+def _projection_inputs(oracle, I, J, K, seed=4, box=(0.0, 0.0, 0.0, 0.4, 0.5, 1.0)):
+    s = dam(I, J, K, seed=seed, box=box)
     dx = s.dx
     solid, weights = oracle.domain_solid(I, J, K, dx)
     liquid = oracle.liquid_sdf(I, J, K, dx, s.positions, radius(dx), solid)
@@ -57,6 +57,31 @@ def test_mixed_pressure_matches_fp64_oracle(gpu, oracle, dims):
     vmax = max(float(np.abs(g).max()) for g in po["mac"])
     for a, b in zip(pm["mac"], po["mac"]):
         assert np.abs(a.astype(np.float64) - b).max() <= 1e-5 * vmax
+
+
+@pytest.mark.parametrize("dims,box", [((1100, 16, 16), (0.0, 0.0, 0.0, 0.9, 0.6, 1.0)),
+                                      ((128, 24, 16), (0.45, 0.3, 0.0, 0.6, 0.8, 1.0))])
+def test_mixed_pressure_long_tiles_and_rows_far_along_i(gpu, oracle, dims, box):
+    # The mixed iteration's forward sweep also does the axpy (a third wave
+    # updates x) and the backward sweep's z.r is taken by a third wave; both
+    # walk a tile's active slot groups.  Tiles of more than 256 groups
+    # (I > 1010) have no group mask and take every group; rows only at large
+    # i move each tile's march range away from step 0.  Same bounds as above.
+    I, J, K = dims
+    dx, solid, weights, liquid, mac, valid = _projection_inputs(oracle, I, J, K, seed=6, box=box)
+    dt = 1 / 60
+    pm = gpu.pressure_solve(I, J, K, dx, dt, liquid, weights, mac, valid, precision=PCG_MIXED)
+    po = oracle.pressure_solve(I, J, K, dx, dt, liquid, solid, weights, mac, valid, mode=1)
+    assert pm["ok"] == po["ok"] == 1
+    assert pm["rows"] == len(po["soln"])
+    bnorm = np.abs(po["rhs"]).max()
+    assert pm["err"] <= 1e-9 * bnorm
+    rel = np.abs(pm["soln"] - po["soln"]).max() / np.abs(po["soln"]).max()
+    print(f"{dims}: iterations mixed {pm['iters']} fp64 {po['iters']}, refinements {pm['refinements']}, "
+          f"relative pressure error {rel:.3e}", flush=True)
+    assert rel <= 1e-6
+    for a, b in zip(pm["valid"], po["valid"]):
+        assert bits_equal(a, b)
 
 
 def test_mixed_scene_tracks_fp64_scene(gpu):
