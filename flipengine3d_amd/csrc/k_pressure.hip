@@ -324,11 +324,12 @@ hipError_t launch_group_mask(PressureBuffers &pb, hipStream_t s) {
 // ---------------------------------------------------------------------------
 
 enum { kFactor = 0, kForward = 1, kBackward = 2 };
-// What a sweep launch reduces or decides at its end: nothing, the z.r dot
-// of the first or a later iteration (backward), the residual norm of the
-// axpy the forward sweep did, or (forward, only when a true residual is
-// due) the norm of the true residual it computes before sweeping.
-enum { kDotNone = 0, kDotInit = 1, kDotIter = 2, kAxpyNorm = 3, kRedoForward = 4 };
+// A sweep launch's mode: what it reduces or decides at its end.  Nothing;
+// the z.r dot of the first or a later iteration (backward); the residual
+// norm of the axpy the forward sweep did; or (forward, a launch that runs
+// only when a true residual is due) the norm of the true residual it
+// computes before sweeping.
+enum { kDotNone = 0, kDotInit = 1, kDotIter = 2, kAxpyNorm = 3, kTrueResidual = 4 };
 
 // Values another workgroup reads within the same kernel are stored and
 // loaded as relaxed device-scope atomics, which are coherent across XCDs by
@@ -512,16 +513,17 @@ __device__ __forceinline__ void tile_urange(const PressureBuffers &pb, const Gri
 }
 
 // The last wave of a sweep launch resets the shared counters for the next
-// launch and finishes the launch's reduction: the backward sweep's dot
-// product, or the residual norm of a forward sweep that did the axpy.
-__device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, int tile,
+// launch and finishes the launch's reduction (mode): the backward sweep's
+// dot product, or the max norm of the residual a forward sweep updated or
+// recomputed.  tile_acc is the calling wave's per-lane share of it.
+__device__ void finish_sweep(PressureBuffers &pb, int mode, double tile_acc, int tile,
                              bool reset_progress, unsigned next_epoch = 0) {
     const int lane = threadIdx.x & 63;
     const int ntiles = pb.sm.ntj * pb.sm.ntk;
-    const bool dot = dotmode == kDotInit || dotmode == kDotIter;
-    const bool norm = dotmode == kAxpyNorm || dotmode == kRedoForward;
+    const bool dot = mode == kDotInit || mode == kDotIter;
+    const bool norm = mode == kAxpyNorm || mode == kTrueResidual;
     if (dot || norm) {
-        double part = dot ? wave_sum_fixed(dot_acc) : wave_max(dot_acc);
+        double part = dot ? wave_sum_fixed(tile_acc) : wave_max(tile_acc);
         if (lane == 0) st_dev(&pb.partials[tile], part);
     }
     drain_memory();
@@ -545,7 +547,7 @@ __device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, i
         m = wave_max(m);
         if (lane == 0) {
             PcgScalars *sc = pb.sc;
-            if (dotmode == kAxpyNorm) {
+            if (mode == kAxpyNorm) {
                 // |r32|_inf after the axpy: a true residual is due when it
                 // meets the tolerance, has fallen by delta since the last
                 // one, or the iteration cap is reached.
@@ -574,7 +576,7 @@ __device__ void finish_sweep(PressureBuffers &pb, int dotmode, double dot_acc, i
         double rho = wave_sum_fixed(acc);
         if (lane == 0) {
             PcgScalars *sc = pb.sc;
-            if (dotmode == kDotInit) {
+            if (mode == kDotInit) {
                 sc->rho = rho;
                 if (rho == 0 || rho != rho) { sc->failed = 1; sc->done = 1; }
             } else {
@@ -1078,7 +1080,7 @@ __device__ __forceinline__ double spmv_slot(const PressureBuffers &pb, double mf
 }
 
 // The true residual of one tile for the forward sweep that follows it
-// (kRedoForward): r = b - A x in FP64 at the slots of the tile's active
+// (kTrueResidual): r = b - A x in FP64 at the slots of the tile's active
 // groups, r32 = (float) r, as a separate true-residual pass over the active
 // slot list computes them; returns max |r| over the tile to every thread.
 // The stores are visible to the workgroup's loads after it returns.
@@ -1177,7 +1179,7 @@ __device__ void sweep_x_wave(const PressureBuffers &pb, SweepMailbox<T> &mb, int
 // march and step order, as the compute wave would.  Chunks without rows
 // hold z = 0 and add nothing.  It finishes the launch (finish_sweep).
 __device__ void sweep_dot_wave(PressureBuffers &pb, SweepMailbox<float> &mb, const SweepOps<float> &op, int tile,
-                               int tb, int nchunks, int lane, const GroupBits &gb, int dotmode) {
+                               int tb, int nchunks, int lane, const GroupBits &gb, int mode) {
     constexpr int C = kChunk, K = 4;
     const int S = pb.sm.S;
     const size_t tile0 = (size_t)tile * (size_t)(S >> 2) * 256;
@@ -1217,7 +1219,7 @@ __device__ void sweep_dot_wave(PressureBuffers &pb, SweepMailbox<float> &mb, con
                 acc = __builtin_fma((double)zv[k][h][m], (double)rv[k][h][m], acc);
             }
     }
-    finish_sweep(pb, dotmode, acc, tile, false, mb.epoch + 1);
+    finish_sweep(pb, mode, acc, tile, false, mb.epoch + 1);
 }
 
 // AX (FP32 forward sweep of the mixed path only): the sweep also does the
@@ -1230,7 +1232,7 @@ __device__ void sweep_dot_wave(PressureBuffers &pb, SweepMailbox<float> &mb, con
 // update on the forward sweep (AX), the z.r dot on the backward one (DW).
 template <class T, int DIR, int D, bool AUX = false>
 __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, SweepOps<T> op,
-                                                          int dotmode) {
+                                                          int mode) {
     constexpr int C = kChunk;
     constexpr bool FWD = DIR == kForward;
     static_assert(!AUX || sizeof(T) == 4, "helper waves serve the FP32 sweeps only");
@@ -1239,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
     constexpr int RW = Record<T>::W;
     __shared__ SweepMailbox<T> mb;
     if (threadIdx.x == 0) {
-        const int skip = ld_dev(&pb.sc->done) != 0 || (dotmode == kRedoForward && ld_dev(&pb.sc->refresh) == 0);
+        const int skip = ld_dev(&pb.sc->done) != 0 || (mode == kTrueResidual && ld_dev(&pb.sc->refresh) == 0);
         mb.skip = skip;
         mb.tile = skip ? 0 : (int)atomicAdd(pb.ticket, 1u);
         mb.epoch = ld_dev(&pb.sc->sweep_epoch);
@@ -1263,10 +1265,10 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
     if (up_k >= 0) tile_urange(pb, g, up_k, !FWD, ubk, uek);
     const int nchunks = te > tb ? (te - tb) / C : 0;
     const GroupBits gb = tile_group_bits(pb, tile);
-    // kRedoForward: the tile's true residual first (every wave), then the sweep over it.
+    // kTrueResidual: the tile's true residual first (every wave), then the sweep over it.
     double tres_max = 0.0;
     if constexpr (FWD && !AX && sizeof(T) == 4) {
-        if (dotmode == kRedoForward) {
+        if (mode == kTrueResidual) {
             tres_max = tile_true_residual<256>(g, pb, tile, tb, te, gb);
             if (wave >= 2) return;  // those waves only help with the residual
         }
@@ -1276,7 +1278,7 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
     if constexpr (AX)
         if (wave == 2) return sweep_x_wave(pb, mb, tile, tb, te, lane, gb);
     if constexpr (DW)
-        if (wave == 2) return sweep_dot_wave(pb, mb, op, tile, tb, nchunks, lane, gb, dotmode);
+        if (wave == 2) return sweep_dot_wave(pb, mb, op, tile, tb, nchunks, lane, gb, mode);
 
     const int src_j = (FWD ? lane - 1 : lane + 1) & 63;  // neighbour lane along j
     const int src_k = (FWD ? lane - 8 : lane + 8) & 63;  // neighbour lane along k
@@ -1293,7 +1295,7 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
     const int pub_slot = a == aj ? b : 8 + a;
     const unsigned epoch = mb.epoch;
     const size_t grp0 = (size_t)tile * (size_t)(S >> 2);
-    const bool dot = !FWD && !DW && dotmode != kDotNone;
+    const bool dot = !FWD && !DW && mode != kDotNone;
     // Without a dot product the backward sweep still streams a vector (its
     // own output) so the hot loop has no data-dependent branches.
     const T *dotsrc = dot ? op.dotvec : op.out;
@@ -1485,8 +1487,8 @@ __global__ __launch_bounds__(256) void k_sweep(GridGeom g, PressureBuffers pb, S
         }
     }
     if constexpr (DW) return;  // the dot wave finishes the launch
-    const int fin = FWD ? (AX ? kAxpyNorm : (dotmode == kRedoForward ? kRedoForward : kDotNone))
-                        : (dot ? dotmode : kDotNone);
+    const int fin = FWD ? (AX ? kAxpyNorm : (mode == kTrueResidual ? kTrueResidual : kDotNone))
+                        : (dot ? mode : kDotNone);
     finish_sweep(pb, fin, AX ? (double)rmax : (FWD ? tres_max : dot_acc), tile, false, epoch + 1);
 }
 
@@ -1515,7 +1517,7 @@ struct SweepDepth<float> {
 
 template <class T>
 static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, SweepOps<T> op,
-                               int dotmode, hipStream_t s, const KernelTiming &kt = KernelTiming{}) {
+                               int mode, hipStream_t s, const KernelTiming &kt = KernelTiming{}) {
     // A fresh epoch per launch (the device's sweep_epoch, advanced by each
     // launch's last tile); records of older launches never match it.  The
     // host counts launches, an upper bound of the device epoch, and clears
@@ -1530,22 +1532,22 @@ static hipError_t launch_sweep(int dir, const GridGeom &g, PressureBuffers &pb, 
     }
     int ntiles = pb.sm.ntj * pb.sm.ntk;
     constexpr int D = SweepDepth<T>::D;
-    if (dotmode == kAxpyNorm) {
+    if (mode == kAxpyNorm) {
         if constexpr (sizeof(T) == 4) {
-            if (dir == kForward) return launch_on(k_sweep<T, kForward, D, true>, ntiles, 192, s, kt, g, pb, op, dotmode);
+            if (dir == kForward) return launch_on(k_sweep<T, kForward, D, true>, ntiles, 192, s, kt, g, pb, op, mode);
         }
         return hipErrorInvalidValue;
     }
     // The forward sweep over a fresh true residual has four waves for the
     // residual; two of them then leave.
     if (dir == kForward)
-        return launch_on(k_sweep<T, kForward, D>, ntiles, dotmode == kRedoForward ? 256 : 128, s, kt, g, pb, op,
-                         dotmode);
+        return launch_on(k_sweep<T, kForward, D>, ntiles, mode == kTrueResidual ? 256 : 128, s, kt, g, pb, op,
+                         mode);
     // The FP32 backward sweep's z.r is taken by a third wave.
     if constexpr (sizeof(T) == 4)
-        if (dotmode == kDotInit || dotmode == kDotIter)
-            return launch_on(k_sweep<T, kBackward, D, true>, ntiles, 192, s, kt, g, pb, op, dotmode);
-    return launch_on(k_sweep<T, kBackward, D>, ntiles, 128, s, kt, g, pb, op, dotmode);
+        if (mode == kDotInit || mode == kDotIter)
+            return launch_on(k_sweep<T, kBackward, D, true>, ntiles, 192, s, kt, g, pb, op, mode);
+    return launch_on(k_sweep<T, kBackward, D>, ntiles, 128, s, kt, g, pb, op, mode);
 }
 
 static SweepOps<double> fwd_ops64(PressureBuffers &pb, const double *r, double *z) {
@@ -2107,7 +2109,7 @@ hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBu
             // The forward sweep does the axpy; when its residual norm calls
             // for a true residual, the sweep is redone over that residual.
             case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kAxpyNorm, s, kt);
-            case kPcgResidual: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kRedoForward, s, kt);
+            case kPcgResidual: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kTrueResidual, s, kt);
             case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotIter, s, kt);
             case kPcgUpdate: return launch_on(k_update_s32, rg, rb, s, kt, pb, 0);
             default: return hipErrorInvalidValue;
