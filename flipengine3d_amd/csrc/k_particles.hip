@@ -239,21 +239,50 @@ hipError_t launch_particle_step(const GridGeom &g, const ParticleStepArgs &a, hi
 
 // Maximum squared speed as float bits (non-negative floats order as
 // unsigned integers), FluidSimulation::_getMaximumMarkerParticleSpeed
-// (fluidsimulation.cpp:5553-5565).
-__global__ void k_max_speed(const float *__restrict__ vx, const float *__restrict__ vy,
-                            const float *__restrict__ vz, int n, unsigned int *out) {
+// (fluidsimulation.cpp:5553-5565).  Each thread keeps four loads in flight;
+// the block reduces its waves' maxima in LDS and takes one atomic, so few
+// atomics meet on the one address.  (The maximum is order-independent; NaN
+// speeds are skipped.)
+constexpr int kMaxSpeedBlocks = 512;
+__global__ __launch_bounds__(256) void k_max_speed(const float *__restrict__ vx, const float *__restrict__ vy,
+                                                   const float *__restrict__ vz, int n, unsigned int *out) {
+    __shared__ unsigned wave_best[4];
     unsigned best = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    auto take = [&](size_t i) {
         float d = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
         unsigned b = __float_as_uint(d);
         if (d == d && b > best) best = b;
+    };
+    const size_t stride = (size_t)gridDim.x * blockDim.x, end = (size_t)n;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < end; i += 4 * stride) {
+        float x[4], y[4], z[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            x[u] = vx[i + u * stride];
+            y[u] = vy[i + u * stride];
+            z[u] = vz[i + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            float d = x[u] * x[u] + y[u] * y[u] + z[u] * z[u];
+            unsigned b = __float_as_uint(d);
+            if (d == d && b > best) best = b;
+        }
     }
+    for (; i < end; i += stride) take(i);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         unsigned o = __shfl_xor(best, off, 64);
         best = o > best ? o : best;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+    if ((threadIdx.x & 63) == 0) wave_best[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned m = wave_best[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = wave_best[w] > m ? wave_best[w] : m;
+        atomicMax(out, m);
+    }
 }
 
 hipError_t launch_max_speed(const float *vx, const float *vy, const float *vz, int n,
@@ -262,7 +291,7 @@ hipError_t launch_max_speed(const float *vx, const float *vy, const float *vz, i
     if (e != hipSuccess) return e;
     if (n <= 0) return hipSuccess;
     int blocks = (n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
+    if (blocks > kMaxSpeedBlocks) blocks = kMaxSpeedBlocks;
     k_max_speed<<<blocks, 256, 0, s>>>(vx, vy, vz, n, out_bits);
     return hipGetLastError();
 }
