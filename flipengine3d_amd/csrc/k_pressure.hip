@@ -154,18 +154,22 @@ __device__ __forceinline__ bool is_row_cell(const GridGeom &g, const float *liqu
 }
 
 // Right-hand side (pressuresolver.cpp:580-669) and matrix coefficients
-// (pressuresolver.cpp:723-808) of one row; couplings to the +x/+y/+z rows
-// are stored on this row, couplings to -x/-y/-z on the neighbour.  One
-// thread per slot, so every slot (rows, other cells, padding) is written.
-__global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liquid,
-                           const float *__restrict__ wU, const float *__restrict__ wV,
-                           const float *__restrict__ wW, const float *__restrict__ wC,
-                           const float *__restrict__ u, const float *__restrict__ v,
-                           const float *__restrict__ w, const float *__restrict__ sU,
-                           const float *__restrict__ sV, const float *__restrict__ sW,
-                           PressureBuffers pb) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pb.sm.count()) return;
+// (pressuresolver.cpp:723-808) of the row at slot p; couplings to the
+// +x/+y/+z rows are stored on this row, couplings to -x/-y/-z on the
+// neighbour.  Every slot (rows, other cells, padding) is written.  A row
+// raises the thread's max |rhs| (IEEE bits of a non-negative double order as
+// integers) and its row count.
+struct AssembleInputs {
+    const float *liquid, *wU, *wV, *wW, *wC, *u, *v, *w, *sU, *sV, *sW;
+};
+__device__ __forceinline__ void assemble_slot(const GridGeom &g, double dt, const AssembleInputs &in,
+                                              PressureBuffers &pb, size_t p, unsigned long long &bmax,
+                                              unsigned &rows) {
+    const float *__restrict__ liquid = in.liquid;
+    const float *__restrict__ wU = in.wU, *__restrict__ wV = in.wV, *__restrict__ wW = in.wW;
+    const float *__restrict__ wC = in.wC;
+    const float *__restrict__ u = in.u, *__restrict__ v = in.v, *__restrict__ w = in.w;
+    const float *__restrict__ sU = in.sU, *__restrict__ sV = in.sV, *__restrict__ sW = in.sW;
     int i, j, k;
     decode_slot(pb.sm, p, i, j, k);
     double rhs = 0, diag = 0;
@@ -218,10 +222,9 @@ __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liqu
         side(volFront, i, j, k + 1, &wz);
         side(volBack, i, j, k - 1, nullptr);
         diag = diag > 0.0 ? diag : 0.0;
-        // max|rhs| as an order-independent integer maximum over IEEE bits.
-        unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(rhs));
-        atomicMax(&pb.sc->absmax_bits, bits);
-        atomicAdd(&pb.sc->rows, 1u);
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(rhs));
+        bmax = bits > bmax ? bits : bmax;
+        rows++;
     }
     pb.rhs[p] = rhs;
     pb.diag[p] = diag;
@@ -230,14 +233,85 @@ __global__ void k_assemble(GridGeom g, double dt, const float *__restrict__ liqu
     pb.wz[p] = wz;
 }
 
+// The assembly on a fixed grid of blocks (grid-stride over the slots).
+// max|rhs| and the row count are order-independent; each block leaves its
+// share in the partials (bits of the maximum in the first kAssembleBlocks
+// words, the count after them) and k_assemble_total combines them, so no
+// two blocks contend for one address (an atomic per row-holding wave cost
+// 460 us per assembly of 1.2 M rows).
+constexpr int kAssembleBlocks = 2048;
+__global__ __launch_bounds__(256) void k_assemble(GridGeom g, double dt, AssembleInputs in, PressureBuffers pb) {
+    __shared__ unsigned long long wbmax[4];
+    __shared__ unsigned wrows[4];
+    unsigned long long bmax = 0;
+    unsigned rows = 0;
+    const size_t n = pb.sm.count(), stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
+        assemble_slot(g, dt, in, pb, p, bmax, rows);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bmax, off, 64);
+        bmax = o > bmax ? o : bmax;
+        rows += __shfl_xor(rows, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wbmax[threadIdx.x >> 6] = bmax;
+        wrows[threadIdx.x >> 6] = rows;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; q++) {
+            bmax = wbmax[q] > bmax ? wbmax[q] : bmax;
+            rows += wrows[q];
+        }
+        reinterpret_cast<unsigned long long *>(pb.partials)[blockIdx.x] = bmax;
+        pb.partials[kAssembleBlocks + blockIdx.x] = (double)rows;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_assemble_total(PressureBuffers pb, int nblocks) {
+    __shared__ unsigned long long wbmax[16];
+    __shared__ unsigned wrows[16];
+    unsigned long long bmax = 0;
+    unsigned rows = 0;
+    for (int q = (int)threadIdx.x; q < nblocks; q += 1024) {
+        const unsigned long long b = reinterpret_cast<const unsigned long long *>(pb.partials)[q];
+        bmax = b > bmax ? b : bmax;
+        rows += (unsigned)pb.partials[kAssembleBlocks + q];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bmax, off, 64);
+        bmax = o > bmax ? o : bmax;
+        rows += __shfl_xor(rows, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wbmax[threadIdx.x >> 6] = bmax;
+        wrows[threadIdx.x >> 6] = rows;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 16; q++) {
+            bmax = wbmax[q] > bmax ? wbmax[q] : bmax;
+            rows += wrows[q];
+        }
+        pb.sc->absmax_bits = bmax;
+        pb.sc->rows = rows;
+    }
+}
+
 hipError_t launch_pressure_assemble(const GridGeom &g, double dt, const float *liquid,
                                     const float *wU, const float *wV, const float *wW,
                                     const float *wC, const float *u, const float *v,
                                     const float *w, const float *sU, const float *sV,
                                     const float *sW, PressureBuffers &pb, hipStream_t s) {
-    size_t n = pb.sm.count();
-    k_assemble<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, dt, liquid, wU, wV, wW, wC, u, v, w,
-                                                           sU, sV, sW, pb);
+    // The partials hold kAssembleBlocks maxima and as many counts.
+    if (pb.npartials < 2 * kAssembleBlocks) return hipErrorInvalidValue;
+    const size_t n = pb.sm.count();
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, (size_t)kAssembleBlocks);
+    const AssembleInputs in{liquid, wU, wV, wW, wC, u, v, w, sU, sV, sW};
+    k_assemble<<<blocks, 256, 0, s>>>(g, dt, in, pb);
+    k_assemble_total<<<1, 1024, 0, s>>>(pb, blocks);
     return hipGetLastError();
 }
 

@@ -127,7 +127,7 @@ void PressureStore::alloc(const GridGeom &g, hipStream_t s, bool mixed) {
     if (ns > 0xffffffffull)
         throw StatusError{FLIPMI_ERR_INVALID_ARG, "pressure grid too large: more than 2^32 slots"};
     int ntj = sm.ntj, ntk = sm.ntk;
-    int npart = 2 * 1024 + ntj * ntk;
+    int npart = 2 * 2048 + ntj * ntk;  // the assembly's 2 x 2048 block shares; the PCG's per-block and per-tile partials
     for (int q = 0; q < kSlotArrays; q++)
         if (arr[q].alloc(ns) != hipSuccess)
             throw StatusError{FLIPMI_ERR_OOM, "pressure arrays: out of device memory"};
@@ -599,7 +599,7 @@ size_t estimate_scene_bytes(const flipmi_scene_desc &d, int64_t cap) {
     b += sizeof(float) * nc + sizeof(float) * 2 * (nU + nV + nW) + (nU + nV + nW) + nf;
     SlotMap sm = make_slot_map(d.isize, d.jsize, d.ksize);
     const size_t ns = sm.alloc_count(), ntiles = (size_t)sm.ntj * sm.ntk;
-    b += sizeof(double) * (PressureStore::kSlotArrays * ns + 2 * 1024 + ntiles);
+    b += sizeof(double) * (PressureStore::kSlotArrays * ns + 2 * 2048 + ntiles);
     b += sizeof(float) * PressureStore::kWeightArrays * ns;
     if (d.pcg_precision == FLIPMI_PCG_MIXED) b += sizeof(float) * PressureStore::kSlotArrays32 * ns;
     b += sizeof(unsigned long long) * edge_record_count(sm);
