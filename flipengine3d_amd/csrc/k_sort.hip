@@ -93,8 +93,11 @@ hipError_t launch_exclusive_scan(const int *in, int *out, int n, int *tmp, hipSt
 // are neighbours in memory mostly share cells, so the lanes of a wave that
 // hold the same cell take their slots with one atomic (the leader adds the
 // group's size, each lane its rank in the group): a cell's count is then
-// bumped once per wave instead of once per particle.  Slots only need to be
-// distinct; the within-cell order is fixed afterwards by particle id.
+// bumped once per wave instead of once per particle.  The groups are found
+// first (ballots only), then every leader takes its atomic in the same
+// instruction, so a wave waits for one atomic round trip, not one per cell.
+// Slots only need to be distinct; the within-cell order is fixed afterwards
+// by particle id.
 __global__ __launch_bounds__(256) void k_cell_key(GridGeom g, const float *__restrict__ px,
                                                   const float *__restrict__ py, const float *__restrict__ pz,
                                                   int n, int *cell, int *slot, int *count, int *over) {
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(256) void k_cell_key(GridGeom g, const float *__res
         if (c.i >= 0 && c.j >= 0 && c.k >= 0 && c.i < g.I && c.j < g.J && c.k < g.K)
             id = c.i + g.I * (c.j + g.J * c.k);
     }
-    int sl = 0;
+    int my_leader = lane, rank = 0, size = 0;  // size: the group's, on its leader
     bool pending = id >= 0;
     for (;;) {
         const unsigned long long m = __ballot(pending);
@@ -115,17 +118,19 @@ __global__ __launch_bounds__(256) void k_cell_key(GridGeom g, const float *__res
         const int key = __shfl(id, leader, 64);
         const bool mine = pending && id == key;
         const unsigned long long peers = __ballot(mine);
-        int base = 0;
-        if (lane == leader) {
-            base = atomicAdd(&count[key], __popcll(peers));
-            if (over && base + __popcll(peers) > 250) atomicOr(over, 1);
-        }
-        base = __shfl(base, leader, 64);
         if (mine) {
-            sl = base + __popcll(peers & ((1ull << lane) - 1ull));
+            my_leader = leader;
+            rank = __popcll(peers & ((1ull << lane) - 1ull));
+            if (lane == leader) size = __popcll(peers);
             pending = false;
         }
     }
+    int base = 0;
+    if (size > 0) {
+        base = atomicAdd(&count[id], size);
+        if (over && base + size > 250) atomicOr(over, 1);
+    }
+    const int sl = __shfl(base, my_leader, 64) + rank;
     if (i < n) {
         cell[i] = id;
         slot[i] = sl;
