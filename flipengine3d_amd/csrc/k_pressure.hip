@@ -874,10 +874,24 @@ __global__ __launch_bounds__(64) void k_factor(GridGeom g, PressureBuffers pb) {
 
 // The factor's entries seen from each row: Lxm(c) = Lx(c - x) etc., so the
 // forward solve reads all its operands from the row's own slot.
-__global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
-    size_t n = pb.sm.count();
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * blockDim.x) {
+// The slots a sweep reads from the factor arrays: those of the active groups
+// while each tile's group mask is in use (at most 256 groups per tile), every
+// slot otherwise (the sweeps then read every group of the march).  One
+// thread per slot of a 256-slot group: blockDim.x is 256.
+template <class F>
+__device__ __forceinline__ void for_swept_slots(const PressureBuffers &pb, F f) {
+    if (pb.group_mask && pb.active_list && pb.mask_words <= 4) {
+        const ListRange lr = xcd_list(*pb.active_count);
+        for (int e = lr.begin; e < lr.end; e += lr.step) f(((size_t)pb.active_list[e] << 8) + threadIdx.x);
+    } else {
+        const size_t n = pb.sm.count();
+        for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
+            f(p);
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_lower_couplings(GridGeom g, PressureBuffers pb) {
+    for_swept_slots(pb, [&](size_t p) {
         int i, j, k;
         decode_slot(pb.sm, p, i, j, k);
         double lx = 0.0, ly = 0.0, lz = 0.0;
@@ -889,7 +903,7 @@ __global__ void k_lower_couplings(GridGeom g, PressureBuffers pb) {
         pb.Lxm[p] = lx;
         pb.Lym[p] = ly;
         pb.Lzm[p] = lz;
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -1991,10 +2005,9 @@ hipError_t launch_pressure_reset(PressureBuffers &pb, hipStream_t s) {
 // ---------------------------------------------------------------------------
 
 // FP32 copy of the (FP64-computed) factor.
-__global__ void k_to_f32(PressureBuffers pb) {
-    size_t n = pb.sm.count();
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * blockDim.x) {
+// The FP32 factor of the slots the sweeps read.
+__global__ __launch_bounds__(kRedThreads) void k_to_f32(PressureBuffers pb) {
+    for_swept_slots(pb, [&](size_t p) {
         pb.invdiag32[p] = (float)pb.invdiag[p];
         pb.Lx32[p] = (float)pb.Lx[p];
         pb.Ly32[p] = (float)pb.Ly[p];
@@ -2002,7 +2015,7 @@ __global__ void k_to_f32(PressureBuffers pb) {
         pb.Lxm32[p] = (float)pb.Lxm[p];
         pb.Lym32[p] = (float)pb.Lym[p];
         pb.Lzm32[p] = (float)pb.Lzm[p];
-    }
+    });
 }
 
 __global__ void k_pcg_init_vectors32(PressureBuffers pb) {
