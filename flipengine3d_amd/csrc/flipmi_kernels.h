@@ -126,6 +126,9 @@ struct PcgScalars {
     float alpha_f, beta_f;
     int iters, done, converged, failed, abort, maxiter, precision;
     unsigned int counter[4];
+    // Per-XCD arrival counts for the last-block tickets of counter[1] and
+    // counter[2], one 128-byte line each (atomics on one line serialise).
+    alignas(128) unsigned int group_counter[2][8][32];
     unsigned int rows;                  // rows of the system (fluid interior cells)
     unsigned long long absmax_bits;
     double mf;                          // dt / dx^2 of the assembled system (coupling scale)

@@ -1696,12 +1696,28 @@ __device__ double block_reduce(double v, double *sh) {
 
 // Returns true in the one block that finished last.  Each block stores its
 // partial with a coherent store and drains it before taking a ticket, so the
-// last block reads every partial with coherent loads.
-__device__ bool last_block(unsigned *counter) {
+// last block reads every partial with coherent loads.  Tickets are taken in
+// two levels so that no cache line takes an atomic from every block (such
+// atomics serialise): each block counts in at its XCD's group counter
+// (blocks go to XCDs round-robin; one 128-byte line per counter), the last
+// block of a group resets that counter and counts the group in at
+// `counter`, and the last group's last block is the last block.  The
+// caller's last block resets `counter`.
+__device__ bool last_block(unsigned *counter, unsigned (*groups)[32]) {
     __shared__ bool is_last;
     drain_memory();
     __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) {
+        const unsigned nb = gridDim.x, grp = blockIdx.x % kXcds;
+        const unsigned gsize = nb / kXcds + (grp < nb % kXcds ? 1u : 0u);
+        const unsigned ngroups = nb < kXcds ? nb : kXcds;
+        bool last = false;
+        if (atomicAdd(&groups[grp][0], 1u) == gsize - 1) {
+            st_dev(&groups[grp][0], 0u);  // every block of the group has counted in
+            last = atomicAdd(counter, 1u) == ngroups - 1;
+        }
+        is_last = last;
+    }
     __syncthreads();
     return is_last;
 }
@@ -1737,7 +1753,7 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot(GridGeom g, PressureBu
     }
     double bs = block_reduce<0>(acc, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[blockIdx.x], bs);
-    if (!last_block(&pb.sc->counter[1])) return;
+    if (!last_block(&pb.sc->counter[1], pb.sc->group_counter[0])) return;
     double tot = 0.0;
     for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) tot += ld_dev(&pb.partials[q]);
     tot = block_reduce<0>(tot, sh);
@@ -1783,7 +1799,7 @@ __global__ __launch_bounds__(kRedThreads) void k_axpy_norm(GridGeom g, PressureB
     }
     double bm = block_reduce<1>(mx, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[kRedBlocks + blockIdx.x], bm);
-    if (!last_block(&pb.sc->counter[2])) return;
+    if (!last_block(&pb.sc->counter[2], pb.sc->group_counter[1])) return;
     double m = 0.0;
     for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) {
         double v = ld_dev(&pb.partials[kRedBlocks + q]);
@@ -2103,7 +2119,7 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, Pressure
                             : spmv_dot32_body(g, pb, NextDirection{pb.z32, pb.s32, pb.sc->beta_f});
     double bs = block_reduce<0>(acc, sh);
     if (threadIdx.x == 0) st_dev(&pb.partials[blockIdx.x], bs);
-    if (!last_block(&pb.sc->counter[1])) return;
+    if (!last_block(&pb.sc->counter[1], pb.sc->group_counter[0])) return;
     double tot = 0.0;
     for (int q = threadIdx.x; q < (int)gridDim.x; q += kRedThreads) tot += ld_dev(&pb.partials[q]);
     tot = block_reduce<0>(tot, sh);
