@@ -2133,25 +2133,23 @@ __global__ __launch_bounds__(kRedThreads) void k_spmv_dot32(GridGeom g, Pressure
     }
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_update_s32(PressureBuffers pb, int first) {
+// s32 = z32: a solve's first direction (pcgsolver.h:279).  Later
+// directions are formed by the SpMV that reads them.
+__global__ __launch_bounds__(kRedThreads) void k_first_direction32(PressureBuffers pb) {
     if (pb.sc->done) return;
-    float bf = first ? 0.0f : pb.sc->beta_f;
     const ListRange lr = xcd_list(*pb.active_count);
     constexpr int U = 4;
     for (int e0 = lr.begin; e0 < lr.end; e0 += U * lr.step) {
         size_t p[U];
         bool ok[U];
-        float zs[U], ss[U];
+        float zs[U];
         list_slots<U>(pb, lr, e0, p, ok);
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            zs[u] = pb.z32[p[u]];
-            ss[u] = pb.s32[p[u]];
-        }
+        for (int u = 0; u < U; u++) zs[u] = pb.z32[p[u]];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (!ok[u]) continue;  // uniform
-            pb.s32[p[u]] = first ? zs[u] : zs[u] + bf * ss[u];
+            pb.s32[p[u]] = zs[u];
         }
     }
 }
@@ -2175,7 +2173,7 @@ hipError_t launch_pcg_init(const GridGeom &g, double tolf, int maxiter, bool mix
         if (e != hipSuccess) return e;
         e = launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotInit, s);
         if (e != hipSuccess) return e;
-        k_update_s32<<<kRedBlocks, kRedThreads, 0, s>>>(pb, 1);
+        k_first_direction32<<<kRedBlocks, kRedThreads, 0, s>>>(pb);
         pb.s_plain = 1;  // the first SpMV takes s = z as it is
         return hipGetLastError();
     }
@@ -2214,7 +2212,8 @@ hipError_t launch_pcg_piece(int piece, const GridGeom &g, bool mixed, PressureBu
             case kPcgForward: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kAxpyNorm, s, kt);
             case kPcgResidual: return launch_sweep(kForward, g, pb, fwd_ops32(pb), kTrueResidual, s, kt);
             case kPcgBackward: return launch_sweep(kBackward, g, pb, bwd_ops32(pb), kDotIter, s, kt);
-            case kPcgUpdate: return launch_on(k_update_s32, rg, rb, s, kt, pb, 0);
+            // kPcgAxpy and kPcgUpdate are folded into the forward sweep and
+            // the SpMV.
             default: return hipErrorInvalidValue;
         }
     }
