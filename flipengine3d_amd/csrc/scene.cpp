@@ -442,6 +442,7 @@ void Scene::alloc_all() {
     need(status.alloc(nf));
     press = new PressureStore();
     press->alloc(g, stream, desc.pcg_precision == FLIPMI_PCG_MIXED);
+    static_assert(sizeof(PcgScalars) <= 4096, "the PCG scalars are read back through the staging buffer");
     check(hipHostMalloc(&host_stage, 4096, hipHostMallocDefault), "pinned staging");
     // Fresh MeshLevelSet: phi = distance upper bound everywhere
     // (meshlevelset.cpp:35-42, 760-762); zero solid face velocities.
